@@ -1,0 +1,284 @@
+/*
+ * zombie_amd.h -- C-ABI of the MI355X (gfx950) Walk-on-Spheres / Walk-on-Stars engine.
+ *
+ * The reference solves at the granularity of
+ *   WalkOnSpheres<T,DIM>::solve(pde, walkSettings, nWalks, samplePts, statistics)
+ *   (include/zombie/point_estimation/walk_on_spheres.h:38-44, walk_on_stars.h:40-46)
+ * with std::function callback tables for the geometry (core/geometric_queries.h:58-124)
+ * and the PDE data (core/pde.h:19-45).  std::function closures cannot run on the GPU,
+ * so this boundary sits at solve() granularity: the caller builds a device scene once
+ * (the counterpart of FcpwDirichletBoundaryHandler::buildAccelerationStructure +
+ * populateGeometricQueriesFor*Boundary, utils/fcpw_geometric_queries.h:569-586,1062-1304),
+ * describes the PDE data with device-representable function descriptors, and calls one
+ * solve per batch of points.
+ *
+ * Conventions
+ *  - Every entry point returns zg_status (0 = ok) and never exits the process; the text of
+ *    the last error on the calling thread is available through zg_last_error().
+ *  - Array arguments of the query and solve entry points are DEVICE pointers (HBM) unless
+ *    the parameter name ends in _host.  `stream` is a hipStream_t passed as void* (NULL =
+ *    the default stream).  Solve and query calls are stream-ordered; entry points that fill
+ *    a host-side report synchronise the stream before returning.
+ *  - Statistics are caller-owned, FP64, structure-of-arrays, and accumulate across calls
+ *    with Chan's parallel merge (point_estimation/common.h:268-310): k calls of W walks
+ *    estimate the same quantity as one call of k*W walks.
+ *  - Random numbers: PCG32 (O'Neill, PCG-XSH-RR 64/32).  Walk w of point p in a call uses
+ *    the generator pcg32(initstate = seed, initseq = (p << 32) | (stream_offset + w)),
+ *    so results are reproducible and independent of the launch shape.  The reference seeds
+ *    one generator per point from the clock (core/sampling.h:36-41); parity is statistical.
+ */
+#ifndef ZOMBIE_AMD_H
+#define ZOMBIE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t zg_status;
+#define ZG_OK 0
+#define ZG_ERR_INVALID_ARGUMENT 1
+#define ZG_ERR_UNSUPPORTED 2
+#define ZG_ERR_DEVICE 3
+#define ZG_ERR_OUT_OF_MEMORY 4
+#define ZG_ERR_NO_DEVICE 5
+
+/* ------------------------------------------------------------------------------------ */
+/* Settings: POD mirror of WalkSettings (point_estimation/common.h:19-91), same order.   */
+/* ------------------------------------------------------------------------------------ */
+typedef struct zg_walk_settings {
+    float epsilon_shell_for_absorbing_boundary;
+    float epsilon_shell_for_reflecting_boundary;
+    float silhouette_precision;
+    float russian_roulette_threshold;
+    float splitting_threshold;
+    int32_t max_walk_length;
+    int32_t steps_before_applying_tikhonov;
+    int32_t steps_before_using_maximal_spheres;
+    uint8_t solve_double_sided;
+    uint8_t use_gradient_control_variates;
+    uint8_t use_gradient_antithetic_variates;
+    uint8_t use_cosine_sampling_for_derivatives;
+    uint8_t ignore_absorbing_boundary_contribution;
+    uint8_t ignore_reflecting_boundary_contribution;
+    uint8_t ignore_source_contribution;
+    uint8_t print_logs;
+} zg_walk_settings;
+
+/* The 4-argument constructor of WalkSettings (common.h:21-40): silhouette precision 1e-3,
+ * Russian roulette off (0), splitting off (FLT_MAX), Tikhonov at step 0, maximal spheres
+ * after max_walk_length steps, gradient control/antithetic variates on. */
+void zg_walk_settings_default(zg_walk_settings* ws, float epsilon_shell_absorbing,
+                              float epsilon_shell_reflecting, int32_t max_walk_length,
+                              int32_t solve_double_sided);
+
+/* ------------------------------------------------------------------------------------ */
+/* PDE data: device-representable replacement of the PDE<T,DIM> callbacks (pde.h:19-45). */
+/* ------------------------------------------------------------------------------------ */
+/* function kinds; params[] meaning per kind */
+#define ZG_FN_ZERO 0            /* 0                                                        */
+#define ZG_FN_CONSTANT 1        /* p0                                                       */
+#define ZG_FN_EXP_COS 2         /* p0 * exp(p1*x) * cos(p1*y)           (harmonic)          */
+#define ZG_FN_SIN_PRODUCT 3     /* p0 * prod_{i<DIM} sin(p1*x_i)        (-Lap u = DIM p1^2 u) */
+#define ZG_FN_CUBIC_HARMONIC 4  /* p0 * (x^3 - 3 x y^2)                 (harmonic)          */
+#define ZG_FN_EXP_LINEAR 5      /* p0 * exp(p1*x + p2*y + p3*z)         (Lap u = |a|^2 u)   */
+
+/* which quantity of the manufactured field u a descriptor evaluates */
+#define ZG_EVAL_VALUE 0              /* u(x)                                                 */
+#define ZG_EVAL_NORMAL_DERIVATIVE 1  /* s * grad u(x) . n, s = -1 on the normal-aligned side  */
+#define ZG_EVAL_SCREENED_SOURCE 2    /* -Lap u(x) + absorption * u(x)                        */
+
+typedef struct zg_function {
+    int32_t kind;      /* ZG_FN_*   */
+    int32_t eval;      /* ZG_EVAL_* */
+    float params[6];
+} zg_function;
+
+typedef struct zg_pde {
+    float absorption_coeff;                  /* pde.h:25, >= 0                        */
+    uint8_t is_source_constant;              /* pde.h:26                              */
+    uint8_t are_robin_conditions_pure_neumann; /* pde.h:27                            */
+    uint8_t are_robin_coeffs_nonnegative;    /* pde.h:28                              */
+    uint8_t reserved;
+    zg_function source;                      /* pde.h:31  source(x)                   */
+    zg_function dirichlet;                   /* pde.h:34  dirichlet(x, normalAligned)  */
+    zg_function robin;                       /* pde.h:37  robin(x, n, normalAligned)   */
+    zg_function robin_coeff;                 /* pde.h:38  robinCoeff(x, n, aligned)    */
+} zg_pde;
+
+/* ------------------------------------------------------------------------------------ */
+/* Scene: boundary meshes + wide BVHs in HBM.                                            */
+/* ------------------------------------------------------------------------------------ */
+typedef struct zg_scene zg_scene;
+
+typedef struct zg_build_options {
+    int32_t bvh_width;  /* 4 or 8 children per node                                    */
+    int32_t leaf_size;  /* max primitives per leaf, 1..8                                */
+    int32_t sah_bins;   /* binned-SAH buckets per axis, 4..64                           */
+    int32_t reserved;
+} zg_build_options;
+
+typedef struct zg_scene_info {
+    int32_t dim;
+    int32_t bvh_width;
+    int64_t n_absorbing_prims, n_reflecting_prims;
+    int64_t n_absorbing_nodes, n_reflecting_nodes;
+    int32_t absorbing_depth, reflecting_depth;
+    int64_t node_bytes;          /* bytes of one packed node                              */
+    int64_t prim_bytes;          /* bytes of one packed primitive                         */
+    int64_t device_bytes;        /* total HBM held by the scene                           */
+    int64_t n_silhouette_vertices;
+} zg_scene_info;
+
+/* Builds the device scene from a host mesh (positions_host [n_vertices*dim], indices_host
+ * [n_prims*dim]: segments in 2D, triangles in 3D).  is_reflecting_host (nullable, per
+ * primitive) partitions the mesh into the absorbing (Dirichlet) part and the reflecting
+ * (Neumann) part, the role of partitionBoundaryMesh (fcpw_geometric_queries.h:460-510).
+ * domain_min/max_host give the bounding domain of outsideBoundingDomain
+ * (geometric_queries.h:376-379).  Replaces FcpwDirichletBoundaryHandler /
+ * FcpwNeumannBoundaryHandler::buildAccelerationStructure (fcpw_geometric_queries.h:537-679)
+ * and populateGeometricQueriesFor{Dirichlet,Neumann}Boundary (:1062-1304); for the
+ * reflecting part (2D) the silhouette candidates of computeSilhouettes (:631) are built with
+ * getIgnoreCandidateSilhouetteCallback(solve_double_sided, silhouette_precision) (:1029-1038). */
+zg_status zg_scene_create(int32_t dim, const float* positions_host, int64_t n_vertices,
+                          const int32_t* indices_host, int64_t n_prims,
+                          const uint8_t* is_reflecting_host,
+                          const float* domain_min_host, const float* domain_max_host,
+                          int32_t solve_double_sided, float silhouette_precision,
+                          const zg_build_options* opts, zg_scene** out);
+zg_status zg_scene_destroy(zg_scene* scene);
+zg_status zg_scene_get_info(const zg_scene* scene, zg_scene_info* info_host);
+
+/* ------------------------------------------------------------------------------------ */
+/* Batched geometric queries (device arrays).                                            */
+/* ------------------------------------------------------------------------------------ */
+/* boundary: 0 = absorbing, 1 = reflecting */
+/* computeDistToAbsorbingBoundary / projectToAbsorbingBoundary (geometric_queries.h:73,78;
+ * fcpw_geometric_queries.h:1069-1096): unsigned distance, closest point, unit normal of the
+ * closest primitive (segment: (s.y,-s.x)/|s|; triangle: (b-a)x(c-a)/|.|, geometry_helpers.h
+ * :15-21,77-81) and the index of that primitive in the caller's mesh.  Outputs nullable
+ * except dist.  dist = FLT_MAX when the boundary is empty. */
+zg_status zg_closest_point(const zg_scene* scene, int32_t boundary, int64_t n,
+                           const float* pts, float* dist, float* closest, float* normal,
+                           int32_t* prim, void* stream);
+
+/* insideDomain (geometric_queries.h:357-371): a point is inside when, for every axis, the
+ * ray along +axis hits the whole boundary (absorbing + reflecting) an odd number of times. */
+zg_status zg_inside_domain(const zg_scene* scene, int64_t n, const float* pts,
+                           uint8_t* inside, void* stream);
+
+/* intersectReflectingBoundary / intersectAbsorbingBoundary first hit
+ * (fcpw_geometric_queries.h:1097-1115,1178-1196): ray origin + t dir, 0 <= t <= t_max.
+ * hit[i] = 1 when a primitive is hit; t, point, normal of the first hit. */
+zg_status zg_intersect_ray(const zg_scene* scene, int32_t boundary, int64_t n,
+                           const float* origins, const float* dirs, const float* t_max,
+                           uint8_t* hit, float* t, float* point, float* normal, void* stream);
+
+/* computeStarRadiusForReflectingBoundary for a Neumann boundary (fcpw_geometric_queries.h
+ * :1254-1268): the distance to the closest silhouette point of the reflecting boundary
+ * within max_radius, clamped below by min_radius.  2D only. */
+zg_status zg_star_radius(const zg_scene* scene, int64_t n, const float* pts,
+                         const float* min_radius, const float* max_radius,
+                         float silhouette_precision, const uint8_t* flip_normal_orientation,
+                         float* radius, void* stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* Solves.                                                                               */
+/* ------------------------------------------------------------------------------------ */
+/* SampleType (common.h:139-143) */
+#define ZG_SAMPLE_IN_DOMAIN 0
+#define ZG_SAMPLE_ON_ABSORBING_BOUNDARY 1
+#define ZG_SAMPLE_ON_REFLECTING_BOUNDARY 2
+
+/* SamplePoint (common.h:151-195), structure of arrays, device pointers. */
+typedef struct zg_sample_points {
+    int64_t n;
+    const float* pt;                         /* [n*dim]                                   */
+    const float* normal;                     /* [n*dim] nullable (zero)                   */
+    const uint8_t* type;                     /* [n] nullable (all ZG_SAMPLE_IN_DOMAIN)     */
+    const float* dist_to_absorbing_boundary; /* [n] required (see zg_closest_point)       */
+    const float* dist_to_reflecting_boundary;/* [n] nullable                              */
+    const uint8_t* estimate_boundary_normal_aligned; /* [n] nullable                     */
+    float* first_sphere_radius;              /* [n] nullable; written when a point has no  */
+                                             /*     previous estimates                     */
+    const int32_t* n_walks;                  /* [n] walks per point                        */
+    int32_t n_walks_max;                     /* host-side upper bound of n_walks[]         */
+    int32_t reserved;
+} zg_sample_points;
+
+/* SampleStatistics (common.h:200-395) in FP64, structure of arrays, device pointers. */
+typedef struct zg_statistics {
+    int64_t* n_solution_estimates;
+    double* solution_mean;
+    double* solution_m2;
+    int64_t* total_walk_length;
+    int64_t* total_splits;
+} zg_statistics;
+
+/* WalkCompletionCode (common.h:132-137) */
+#define ZG_REACHED_ABSORBING_BOUNDARY 0
+#define ZG_TERMINATED_WITH_RUSSIAN_ROULETTE 1
+#define ZG_EXCEEDED_MAX_WALK_LENGTH 2
+#define ZG_ESCAPED_DOMAIN 3
+
+typedef struct zg_solve_options {
+    uint64_t seed;            /* PCG32 initstate of every walk of the call              */
+    uint32_t stream_offset;   /* added to the walk index in the PCG32 stream id          */
+    int32_t walks_per_lane;   /* walks of one point a lane claims at a time (1..64)       */
+    int32_t regroup_interval; /* steps between wavefront ballots that refill idle lanes  */
+    int32_t waves_per_cu;     /* persistent grid size = CUs * waves_per_cu (0 = auto)    */
+    int32_t count_visits;     /* 1: instrumented kernel counting BVH node/primitive visits */
+    int32_t reserved;
+} zg_solve_options;
+
+void zg_solve_options_default(zg_solve_options* opts);
+
+typedef struct zg_solve_report {
+    int64_t completion_codes[4]; /* walks per WalkCompletionCode                         */
+    int64_t total_steps;         /* sum of walkLength over completed walks               */
+    int64_t total_walks;         /* walks started                                        */
+    int64_t node_visits;         /* count_visits only: BVH nodes visited (all queries)   */
+    int64_t prim_visits;         /* count_visits only: primitives tested                 */
+    int64_t queries;             /* count_visits only: closest-point / traversal queries */
+    float kernel_ms;             /* walk-kernel time, HIP events on `stream`             */
+    float reserved;
+} zg_solve_report;
+
+/* WalkOnSpheres<float,DIM>::solve (walk_on_spheres.h:131-168 -> estimateSolution :320-414,
+ * walk :228-291, getTerminalContribution :293-318) for every point in one persistent launch.
+ * report_host nullable; when given, the call synchronises `stream` and fills it. */
+zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
+                       const zg_walk_settings* settings_host, const zg_sample_points* points_host,
+                       zg_statistics* stats, const zg_solve_options* opts_host,
+                       zg_solve_report* report_host, void* stream);
+
+/* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
+ * walk :345-490); 2D scenes with a Neumann (reflecting) boundary. */
+zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
+                        const zg_walk_settings* settings_host, const zg_sample_points* points_host,
+                        zg_statistics* stats, const zg_solve_options* opts_host,
+                        zg_solve_report* report_host, void* stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* Device memory helpers for callers without their own HIP runtime (ctypes, cgo, ...).   */
+/* ------------------------------------------------------------------------------------ */
+zg_status zg_device_count(int32_t* count);
+zg_status zg_malloc(void** ptr, size_t bytes);
+zg_status zg_free(void* ptr);
+zg_status zg_memcpy_htod(void* dst, const void* src_host, size_t bytes);
+zg_status zg_memcpy_dtoh(void* dst_host, const void* src, size_t bytes);
+zg_status zg_memset(void* dst, int32_t value, size_t bytes);
+zg_status zg_synchronize(void);
+
+/* Text of the last error on the calling thread (empty when none); returns its length. */
+int32_t zg_last_error(char* buf, int32_t buf_len);
+/* Library version "major.minor.patch". */
+const char* zg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZOMBIE_AMD_H */

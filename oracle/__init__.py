@@ -1,0 +1,234 @@
+"""CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of oracle/_build/libzombie_oracle.so, the C++ restatement of the reference
+estimator (see zombie_oracle.cpp for the file:line map).  Only tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg may import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from zombie_amd import _abi as A
+
+LIB = Path(__file__).resolve().parent / "_build" / "libzombie_oracle.so"
+_lib = None
+
+RNG_PER_POINT = 0   # the reference's scheme: one generator per point, consumed across walks
+RNG_PER_WALK = 1    # the GPU's scheme: one generator per walk
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB))
+        P = C.c_void_p
+        L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, P]
+        L.orc_pcg32_f32.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, P]
+        L.orc_pcg32_default_u32.argtypes = [C.c_int64, P]
+        L.orc_sample_unit_sphere.argtypes = [C.c_int, C.c_uint64, C.c_int64, P]
+        L.orc_sample_ball_radius.argtypes = [C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int64, P, P]
+        L.orc_greens_ball.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, P]
+        L.orc_offset_point.argtypes = [C.c_int, P, P, P]
+        L.orc_eval_function.argtypes = [C.c_int, C.POINTER(A.zg_function), P, P, C.c_int, C.c_float]
+        L.orc_eval_function.restype = C.c_float
+        L.orc_scene_create.argtypes = [C.c_int, P, C.c_int64, P, C.c_int64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_float]
+        L.orc_scene_create.restype = P
+        L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_stats.argtypes = [P, P]
+        L.orc_closest_point.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, C.c_int, P, C.c_int]
+        L.orc_intersect_ray.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, P, C.c_int]
+        L.orc_inside_domain.argtypes = [P, C.c_int64, P, P, C.c_int]
+        L.orc_star_radius.argtypes = [P, C.c_int64, P, P, P, C.c_float, P]
+        L.orc_silhouette_candidates.argtypes = [P, P, C.c_float, P, P]
+        L.orc_silhouette_candidates.restype = C.c_int64
+        L.orc_sample_reflecting.argtypes = [P, P, C.c_float, P, P, P, P]
+        L.orc_sample_reflecting.restype = C.c_int
+        L.orc_line_of_sight.argtypes = [P, P, P]
+        L.orc_line_of_sight.restype = C.c_int
+        L.orc_solve.argtypes = [C.c_int, P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings), C.c_int64, P, P, P, P,
+                                P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, P, P, P, P, P, P, P]
+        L.orc_solve.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def pcg32_u32(initstate, initseq, n):
+    out = np.empty(n, np.uint32)
+    lib().orc_pcg32_u32(initstate, initseq, n, _p(out))
+    return out
+
+
+def pcg32_f32(initstate, initseq, n):
+    out = np.empty(n, np.float32)
+    lib().orc_pcg32_f32(initstate, initseq, n, _p(out))
+    return out
+
+
+def pcg32_default_u32(n):
+    out = np.empty(n, np.uint32)
+    lib().orc_pcg32_default_u32(n, _p(out))
+    return out
+
+
+def sample_unit_sphere(dim, seed, n):
+    out = np.empty((n, dim), np.float32)
+    lib().orc_sample_unit_sphere(dim, seed, n, _p(out))
+    return out
+
+
+def sample_ball_radius(dim, R, lam, seed, n):
+    r = np.empty(n, np.float32)
+    pdf = np.empty(n, np.float32)
+    lib().orc_sample_ball_radius(dim, R, lam, seed, n, _p(r), _p(pdf))
+    return r, pdf
+
+
+def greens_ball(dim, R, lam, r):
+    out = np.empty(4, np.float32)
+    lib().orc_greens_ball(dim, R, lam, r, _p(out))
+    return dict(norm=float(out[0]), poisson_kernel=float(out[1]), G=float(out[2]), dampening=float(out[3]))
+
+
+def offset_point(p, n):
+    p = np.ascontiguousarray(p, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    out = np.empty_like(p)
+    lib().orc_offset_point(len(p), _p(p), _p(n), _p(out))
+    return out
+
+
+def eval_function(dim, f, x, n=None, aligned=False, absorption=0.0):
+    x = np.ascontiguousarray(x, np.float32)
+    nn = None if n is None else np.ascontiguousarray(n, np.float32)
+    return lib().orc_eval_function(dim, C.byref(f), _p(x), _p(nn), 1 if aligned else 0, absorption)
+
+
+def default_threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+class Scene:
+    def __init__(self, positions, indices, is_reflecting=None, domain_min=None, domain_max=None,
+                 leaf_size=4, buckets=8, solve_double_sided=False, silhouette_precision=1e-3):
+        self.V = np.ascontiguousarray(positions, np.float32)
+        self.I = np.ascontiguousarray(indices, np.int32)
+        self.dim = self.V.shape[1]
+        self.refl = None if is_reflecting is None else np.ascontiguousarray(is_reflecting, np.uint8)
+        self.dmin = np.ascontiguousarray(domain_min if domain_min is not None else self.V.min(0) - 1, np.float32)
+        self.dmax = np.ascontiguousarray(domain_max if domain_max is not None else self.V.max(0) + 1, np.float32)
+        self.h = lib().orc_scene_create(self.dim, _p(self.V), len(self.V), _p(self.I), len(self.I), _p(self.refl),
+                                        _p(self.dmin), _p(self.dmax), leaf_size, buckets,
+                                        1 if solve_double_sided else 0, silhouette_precision)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def stats(self):
+        out = np.zeros(5, np.int64)
+        lib().orc_scene_stats(self.h, _p(out))
+        return dict(abs_nodes=int(out[0]), abs_depth=int(out[1]), refl_nodes=int(out[2]), refl_depth=int(out[3]),
+                    silhouette_candidates=int(out[4]))
+
+    def closest_point(self, pts, boundary=0, brute=False, threads=None):
+        pts = np.ascontiguousarray(pts, np.float32)
+        n = len(pts)
+        d = np.empty(n, np.float32)
+        cp = np.empty((n, self.dim), np.float32)
+        nr = np.empty((n, self.dim), np.float32)
+        pr = np.empty(n, np.int32)
+        visits = np.zeros(2, np.int64)
+        lib().orc_closest_point(self.h, boundary, n, _p(pts), _p(d), _p(cp), _p(nr), _p(pr), 1 if brute else 0,
+                                _p(visits), threads or default_threads())
+        return d, cp, nr, pr, visits
+
+    def intersect_ray(self, o, d, tmax, boundary=0, brute=False):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        tmax = np.ascontiguousarray(tmax, np.float32)
+        n = len(o)
+        hit = np.empty(n, np.uint8)
+        t = np.empty(n, np.float32)
+        pr = np.empty(n, np.int32)
+        lib().orc_intersect_ray(self.h, boundary, n, _p(o), _p(d), _p(tmax), _p(hit), _p(t), _p(pr), 1 if brute else 0)
+        return hit, t, pr
+
+    def inside_domain(self, pts, threads=None):
+        pts = np.ascontiguousarray(pts, np.float32)
+        out = np.empty(len(pts), np.uint8)
+        lib().orc_inside_domain(self.h, len(pts), _p(pts), _p(out), threads or default_threads())
+        return out.astype(bool)
+
+    def star_radius(self, pts, min_r, max_r, precision=1e-3):
+        pts = np.ascontiguousarray(pts, np.float32)
+        n = len(pts)
+        mn = np.ascontiguousarray(np.broadcast_to(min_r, n), np.float32)
+        mx = np.ascontiguousarray(np.broadcast_to(max_r, n), np.float32)
+        out = np.empty(n, np.float32)
+        lib().orc_star_radius(self.h, n, _p(pts), _p(mn), _p(mx), precision, _p(out))
+        return out
+
+    def silhouette_candidates(self, x, precision=1e-3):
+        x = np.ascontiguousarray(x, np.float32)
+        n = lib().orc_silhouette_candidates(self.h, _p(x), precision, None, None)
+        v = np.empty((n, 2), np.float32)
+        s = np.empty(n, np.uint8)
+        lib().orc_silhouette_candidates(self.h, _p(x), precision, _p(v), _p(s))
+        return v, s.astype(bool)
+
+    def sample_reflecting(self, x, radius, u):
+        x = np.ascontiguousarray(x, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        pt = np.empty(2, np.float32)
+        nr = np.empty(2, np.float32)
+        pdf = np.empty(1, np.float32)
+        ok = lib().orc_sample_reflecting(self.h, _p(x), radius, _p(u), _p(pt), _p(nr), _p(pdf))
+        return bool(ok), pt, nr, float(pdf[0])
+
+    def line_of_sight(self, p1, p2):
+        return bool(lib().orc_line_of_sight(self.h, _p(np.ascontiguousarray(p1, np.float32)),
+                                            _p(np.ascontiguousarray(p2, np.float32))))
+
+
+class Stats:
+    def __init__(self, n):
+        self.n = np.zeros(n, np.int64)
+        self.mean = np.zeros(n, np.float64)
+        self.m2 = np.zeros(n, np.float64)
+        self.walk_length = np.zeros(n, np.int64)
+        self.splits = np.zeros(n, np.int64)
+
+    def variance(self):
+        return self.m2 / np.maximum(self.n - 1, 1)
+
+
+def solve(scene: Scene, pde, ws, pts, dist, n_walks, stats: Stats, seed=0x5EED, stream_offset=0,
+          rng_mode=RNG_PER_WALK, threads=None, wost=False, types=None, normals=None, aligned=None,
+          first_sphere_radius=None):
+    """Runs the oracle estimator; returns the report dict."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    n = len(pts)
+    dist = np.ascontiguousarray(dist, np.float32)
+    nw = np.ascontiguousarray(np.broadcast_to(np.asarray(n_walks, np.int32), n), np.int32)
+    rep = np.zeros(8, np.int64)
+    types = None if types is None else np.ascontiguousarray(types, np.uint8)
+    normals = None if normals is None else np.ascontiguousarray(normals, np.float32)
+    aligned = None if aligned is None else np.ascontiguousarray(aligned, np.uint8)
+    st = lib().orc_solve(1 if wost else 0, scene.h, C.byref(pde), C.byref(ws), n, _p(pts), _p(normals), _p(types),
+                         _p(aligned), _p(dist), _p(nw), seed, stream_offset, rng_mode, threads or default_threads(),
+                         _p(stats.n), _p(stats.mean), _p(stats.m2), _p(stats.walk_length), _p(stats.splits),
+                         _p(first_sphere_radius), _p(rep))
+    if st != 0:
+        raise RuntimeError(f"oracle solve failed with status {st}")
+    return dict(codes=rep[:4].copy(), total_steps=int(rep[4]), node_visits=int(rep[5]), prim_visits=int(rep[6]))

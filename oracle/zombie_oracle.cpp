@@ -1,0 +1,1367 @@
+// zombie_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference estimator (rohan-sawhney/zombie) used as the parity
+// oracle for the GPU engine.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg load this library; the product (zombie_amd/) never links or calls it.
+//
+// What it restates (reference file:line):
+//   pcg32 (deps/pcg32, absent: restated from O'Neill's PCG-XSH-RR 64/32 and the
+//     mantissa nextFloat used by core/sampling.h:108-119,171-183)
+//   SphereSampler<2,3>::sampleUnitSphereUniform         core/sampling.h:104-111,164-174
+//   HarmonicGreensFnBall<2>, <3>                         core/distributions.h:453-665
+//   YukawaGreensFnBall<3> (+ rejection sampler)          core/distributions.h:419-440,835-1005
+//   closest point on segment / triangle (Ericson)        core/geometry_helpers.h:15-161
+//   offsetPointAlongDirection (RTG ch. 6)                core/geometric_queries.h:153-207
+//   outsideBoundingDomain, insideDomain                  core/geometric_queries.h:357-379
+//   FCPW Bvh_SurfaceArea (absent dep): binary binned-SAH BVH, leaf size 4, 8 buckets,
+//     nearest-child-first closest-point traversal, first-hit ray, all-hits parity, as seen
+//     through utils/reflectance_boundary_bvh/bvh.h:38-42,369-506 and the call sites
+//     utils/fcpw_geometric_queries.h:1069-1242
+//   silhouette candidates + closest silhouette point     fcpw_geometric_queries.h:1029-1038,1254-1268
+//   branchTraversalWeight                                 fcpw_geometric_queries.h:1391-1398
+//   SampleStatistics (FP64 here; float in the reference) point_estimation/common.h:200-395
+//   WalkOnSpheres::estimateSolution / walk / weight window / terminal
+//                                                        point_estimation/walk_on_spheres.h:170-414
+//   WalkOnStars::estimateSolution / walk / reflecting + source contributions / throughput
+//                                                        point_estimation/walk_on_stars.h:189-662
+//   tbb::parallel_for over points -> std::thread pool    walk_on_spheres.h:153-166
+//
+// Parity pins: see tests/test_oracle_*.py (PCG32 known answers, closed forms of the ball
+// Green's functions, brute force against the BVH, manufactured solutions).  The stochastic
+// sphere-sampling pdf of the reflecting boundary and FCPW's exact silhouette precision test
+// cannot be pinned without the (absent) FCPW source: parity unpinned for those two, checked
+// only through unbiasedness identities.
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "../include/zombie_amd.h"
+
+namespace orc {
+
+// ------------------------------------------------------------------------------------
+// PCG32 (PCG-XSH-RR 64/32), M. E. O'Neill, "PCG: A Family of Simple Fast Space-Efficient
+// Statistically Good Algorithms for Random Number Generation" (2014).
+// ------------------------------------------------------------------------------------
+struct Pcg32 {
+    uint64_t state = 0x853c49e6748fea9bULL;
+    uint64_t inc = 0xda3e39cb94b95bdbULL;
+    Pcg32() {}
+    Pcg32(uint64_t initstate, uint64_t initseq) { seed(initstate, initseq); }
+    void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        nextUInt();
+        state += initstate;
+        nextUInt();
+    }
+    uint32_t nextUInt() {
+        uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xorshifted >> rot) | (xorshifted << ((~rot + 1u) & 31u));
+    }
+    // 23-bit mantissa construction: a float in [1,2) minus 1
+    float nextFloat() {
+        uint32_t bits = (nextUInt() >> 9) | 0x3f800000u;
+        float f;
+        std::memcpy(&f, &bits, 4);
+        return f - 1.0f;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// small fixed-size vectors
+// ------------------------------------------------------------------------------------
+template <int D>
+struct V {
+    float v[D];
+    float& operator[](int i) { return v[i]; }
+    float operator[](int i) const { return v[i]; }
+    static V zero() { V r; for (int i = 0; i < D; i++) r.v[i] = 0.0f; return r; }
+};
+template <int D> inline V<D> operator+(V<D> a, const V<D>& b) { for (int i = 0; i < D; i++) a.v[i] += b.v[i]; return a; }
+template <int D> inline V<D> operator-(V<D> a, const V<D>& b) { for (int i = 0; i < D; i++) a.v[i] -= b.v[i]; return a; }
+template <int D> inline V<D> operator*(V<D> a, float s) { for (int i = 0; i < D; i++) a.v[i] *= s; return a; }
+template <int D> inline V<D> operator*(float s, V<D> a) { return a * s; }
+template <int D> inline float dot(const V<D>& a, const V<D>& b) { float s = 0.0f; for (int i = 0; i < D; i++) s += a.v[i] * b.v[i]; return s; }
+template <int D> inline float norm2(const V<D>& a) { return dot(a, a); }
+template <int D> inline float norm(const V<D>& a) { return std::sqrt(norm2(a)); }
+inline float cross2(const V<2>& a, const V<2>& b) { return a[0] * b[1] - a[1] * b[0]; }
+inline V<3> cross3(const V<3>& a, const V<3>& b) {
+    V<3> r; r[0] = a[1] * b[2] - a[2] * b[1]; r[1] = a[2] * b[0] - a[0] * b[2]; r[2] = a[0] * b[1] - a[1] * b[0]; return r;
+}
+
+template <int D>
+V<D> load(const float* p) { V<D> r; for (int i = 0; i < D; i++) r.v[i] = p[i]; return r; }
+template <int D>
+void store(float* p, const V<D>& a) { for (int i = 0; i < D; i++) p[i] = a.v[i]; }
+
+static const float PI_F = 3.14159265358979323846f;
+
+// ------------------------------------------------------------------------------------
+// SphereSampler (core/sampling.h:104-111, 164-174)
+// ------------------------------------------------------------------------------------
+inline V<2> sampleUnitSphere2(Pcg32& rng) {
+    float u0 = rng.nextFloat();
+    float phi = (float)(2.0 * M_PI * u0);
+    V<2> d; d[0] = std::cos(phi); d[1] = std::sin(phi);
+    return d;
+}
+inline V<3> sampleUnitSphere3(Pcg32& rng) {
+    float u0 = rng.nextFloat();
+    float u1 = rng.nextFloat();
+    float z = 1.0f - 2.0f * u0;
+    float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+    float phi = (float)(2.0 * M_PI * u1);
+    V<3> d; d[0] = r * std::cos(phi); d[1] = r * std::sin(phi); d[2] = z;
+    return d;
+}
+template <int D> V<D> sampleUnitSphere(Pcg32& rng);
+template <> V<2> sampleUnitSphere<2>(Pcg32& rng) { return sampleUnitSphere2(rng); }
+template <> V<3> sampleUnitSphere<3>(Pcg32& rng) { return sampleUnitSphere3(rng); }
+
+// ------------------------------------------------------------------------------------
+// Green's functions of the ball (core/distributions.h:323-665, 835-1005)
+// ------------------------------------------------------------------------------------
+template <int D>
+struct GreensBall {
+    bool yukawa = false;
+    float sqrtLambda = 0.0f;
+    V<D> c;
+    float R = 0.0f;
+    float rClamp = 1e-4f;
+    // Yukawa cached terms (distributions.h:843-852)
+    float muR = 0, expmuR = 0, sinhmuR = 0, K32muR = 0, I32muR = 0;
+
+    void makeHarmonic() { yukawa = false; }
+    void makeYukawa(float lambda) { yukawa = true; sqrtLambda = std::sqrt(lambda); }
+
+    void updateBall(const V<D>& c_, float R_) {
+        c = c_; R = R_; rClamp = 1e-4f;
+        if (yukawa) {
+            muR = R * sqrtLambda;
+            expmuR = std::exp(-muR);
+            float exp2muR = expmuR * expmuR;
+            float coshmuR = (1.0f + exp2muR) / (2.0f * expmuR);
+            sinhmuR = (1.0f - exp2muR) / (2.0f * expmuR);
+            K32muR = expmuR * (1.0f + 1.0f / muR);
+            I32muR = coshmuR - sinhmuR / muR;
+        }
+    }
+
+    float evaluate(float r) const {
+        if (!yukawa) {
+            if (D == 2) return std::log(R / r) / (2.0f * PI_F);
+            return (1.0f / r - 1.0f / R) / (4.0f * PI_F);
+        }
+        float mur = r * sqrtLambda;
+        float expmur = std::exp(-mur);
+        float sinhmur = (1.0f - expmur * expmur) / (2.0f * expmur);
+        return (expmur - expmuR * sinhmur / sinhmuR) / (4.0f * PI_F * r);
+    }
+
+    float poissonKernel() const {  // centred, |y-c| = R
+        if (!yukawa) return D == 2 ? 1.0f / (2.0f * PI_F) : 1.0f / (4.0f * PI_F);
+        return muR / (4.0f * PI_F * sinhmuR);
+    }
+
+    float norm() const {
+        if (!yukawa) return D == 2 ? R * R / 4.0f : R * R / 6.0f;
+        float lambda = sqrtLambda * sqrtLambda;
+        return (1.0f - 4.0f * PI_F * poissonKernel()) / lambda;
+    }
+
+    float poissonKernelDampeningFactor(float r) const {
+        float mur = r * sqrtLambda;
+        float expmur = std::exp(-mur);
+        float exp2mur = expmur * expmur;
+        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
+        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
+        float K32mur = expmur * (1.0f + 1.0f / mur);
+        float I32mur = coshmur - sinhmur / mur;
+        float Q = K32mur + I32mur * expmuR / sinhmuR;
+        return mur * Q;
+    }
+
+    float directionSampledPoissonKernel(const V<D>& y) const {
+        if (!yukawa) return 1.0f;
+        float r = std::max(rClamp, orc::norm(y - c));
+        return poissonKernelDampeningFactor(r);
+    }
+
+    // reflectance for pure Neumann conditions (robinCoeff = 0): 1 (harmonic), Q (Yukawa)
+    float reflectance(float r) const {
+        if (!yukawa) return 1.0f;
+        return poissonKernelDampeningFactor(r);
+    }
+
+    float pdfSampleSphereUniform(float r) const {
+        return D == 2 ? 1.0f / (2.0f * PI_F * r) : 1.0f / (4.0f * PI_F * r * r);
+    }
+
+    // sampleVolume(dir, rng, r, pdf)
+    V<D> sampleVolumeDir(const V<D>& dir, Pcg32& rng, float& r, float& pdf) {
+        if (!yukawa) {
+            if (D == 2) {
+                float u1 = rng.nextFloat();
+                float u2 = rng.nextFloat();
+                r = R * std::sqrt(u1 * u2);
+            } else {
+                float u1 = rng.nextFloat();
+                float u2 = rng.nextFloat();
+                float u3 = rng.nextFloat();
+                float uMedian = std::max(std::min(u1, u2), std::min(std::max(u1, u2), u3));
+                r = R * uMedian;
+            }
+            r = std::max(rClamp, r);
+            if (r > R) r = R / 2.0f;
+            pdf = evaluate(r) / norm();
+            return c + r * dir;
+        }
+        // rejection sampling (distributions.h:855-864 -> 419-440)
+        float lambda = sqrtLambda * sqrtLambda;
+        float bound = R <= lambda ?
+            std::max(std::max(2.0f / R, 2.0f / lambda), std::max(0.5f * std::sqrt(R), 0.5f * sqrtLambda)) :
+            std::max(std::min(2.0f / R, 2.0f / lambda), std::min(0.5f * std::sqrt(R), 0.5f * sqrtLambda));
+        int iter = 0;
+        float fnNorm = norm();
+        do {
+            float u = rng.nextFloat();
+            r = rng.nextFloat() * R;
+            pdf = evaluate(r) / fnNorm;
+            float pdfRadius = pdf / pdfSampleSphereUniform(r);
+            iter++;
+            if (u < pdfRadius / bound) break;
+        } while (iter < 1000);
+        r = std::max(rClamp, r);
+        if (r > R) r = R / 2.0f;
+        return c + r * dir;
+    }
+
+    V<D> sampleVolume(Pcg32& rng, float& r, float& pdf) {
+        V<D> dir = sampleUnitSphere<D>(rng);
+        return sampleVolumeDir(dir, rng, r, pdf);
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// manufactured PDE data (zg_function descriptors), evaluated in float like the
+// reference's T=float callbacks
+// ------------------------------------------------------------------------------------
+template <int D>
+float evalField(const zg_function& f, const V<D>& x, const V<D>& n, bool normalAligned, float absorption) {
+    const float* p = f.params;
+    float z = D == 3 ? x[2] : 0.0f;
+    float u = 0.0f;
+    V<3> g; g[0] = g[1] = g[2] = 0.0f;  // gradient
+    float negLap = 0.0f;
+    switch (f.kind) {
+    case ZG_FN_ZERO: break;
+    case ZG_FN_CONSTANT: u = p[0]; break;
+    case ZG_FN_EXP_COS: {
+        float e = std::exp(p[1] * x[0]);
+        float cs = std::cos(p[1] * x[1]);
+        float sn = std::sin(p[1] * x[1]);
+        u = p[0] * e * cs;
+        g[0] = p[0] * p[1] * e * cs;
+        g[1] = -p[0] * p[1] * e * sn;
+        negLap = 0.0f;
+        break;
+    }
+    case ZG_FN_SIN_PRODUCT: {
+        float s[3] = {1.0f, 1.0f, 1.0f}, c[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < D; i++) { s[i] = std::sin(p[1] * x[i]); c[i] = std::cos(p[1] * x[i]); }
+        u = p[0] * s[0] * s[1] * s[2];
+        g[0] = p[0] * p[1] * c[0] * s[1] * s[2];
+        g[1] = p[0] * p[1] * s[0] * c[1] * s[2];
+        if (D == 3) g[2] = p[0] * p[1] * s[0] * s[1] * c[2];
+        negLap = (float)D * p[1] * p[1] * u;
+        break;
+    }
+    case ZG_FN_CUBIC_HARMONIC: {
+        float xx = x[0], yy = x[1];
+        u = p[0] * (xx * xx * xx - 3.0f * xx * yy * yy);
+        g[0] = p[0] * (3.0f * xx * xx - 3.0f * yy * yy);
+        g[1] = -p[0] * 6.0f * xx * yy;
+        negLap = 0.0f;
+        break;
+    }
+    case ZG_FN_EXP_LINEAR: {
+        float a = p[1] * x[0] + p[2] * x[1] + (D == 3 ? p[3] * z : 0.0f);
+        u = p[0] * std::exp(a);
+        g[0] = p[1] * u; g[1] = p[2] * u; g[2] = D == 3 ? p[3] * u : 0.0f;
+        float a2 = p[1] * p[1] + p[2] * p[2] + (D == 3 ? p[3] * p[3] : 0.0f);
+        negLap = -a2 * u;
+        break;
+    }
+    default: break;
+    }
+    if (f.eval == ZG_EVAL_VALUE) return u;
+    if (f.eval == ZG_EVAL_NORMAL_DERIVATIVE) {
+        float dn = 0.0f;
+        for (int i = 0; i < D; i++) dn += g[i] * n[i];
+        return normalAligned ? -dn : dn;
+    }
+    return negLap + absorption * u;
+}
+
+// ------------------------------------------------------------------------------------
+// primitives (core/geometry_helpers.h)
+// ------------------------------------------------------------------------------------
+inline float closestPointSegment(const V<2>& pa, const V<2>& pb, const V<2>& x, V<2>& pt) {
+    V<2> u = pb - pa, v = x - pa;
+    float c1 = dot(u, v);
+    if (c1 <= 0.0f) { pt = pa; return norm(x - pa); }
+    float c2 = dot(u, u);
+    if (c2 <= c1) { pt = pb; return norm(x - pb); }
+    float t = c1 / c2;
+    pt = pa + u * t;
+    return norm(x - pt);
+}
+
+inline float closestPointTriangle(const V<3>& pa, const V<3>& pb, const V<3>& pc, const V<3>& x, V<3>& pt) {
+    V<3> ab = pb - pa, ac = pc - pa, ax = x - pa;
+    float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    if (d1 <= 0.0f && d2 <= 0.0f) { pt = pa; return norm(x - pa); }
+    V<3> bx = x - pb;
+    float d3 = dot(ab, bx), d4 = dot(ac, bx);
+    if (d3 >= 0.0f && d4 <= d3) { pt = pb; return norm(x - pb); }
+    V<3> cx = x - pc;
+    float d5 = dot(ab, cx), d6 = dot(ac, cx);
+    if (d6 >= 0.0f && d5 <= d6) { pt = pc; return norm(x - pc); }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) { float v = d1 / (d1 - d3); pt = pa + ab * v; return norm(x - pt); }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) { float w = d2 / (d2 - d6); pt = pa + ac * w; return norm(x - pt); }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        pt = pb + (pc - pb) * w;
+        return norm(x - pt);
+    }
+    float denom = 1.0f / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    pt = pa + ab * v + ac * w;
+    return norm(x - pt);
+}
+
+inline V<2> segmentNormal(const V<2>& pa, const V<2>& pb) {
+    V<2> s = pb - pa; V<2> n; n[0] = s[1]; n[1] = -s[0];
+    float l = norm(n); return n * (1.0f / l);
+}
+inline V<3> triangleNormal(const V<3>& pa, const V<3>& pb, const V<3>& pc) {
+    V<3> n = cross3(pb - pa, pc - pa); float l = norm(n); return n * (1.0f / l);
+}
+
+// ray vs segment: origin o, direction d, t in [0, tMax]
+inline bool raySegment(const V<2>& o, const V<2>& d, float tMax, const V<2>& pa, const V<2>& pb, float& t, float& s) {
+    V<2> u = pa - o, v = pb - pa;
+    float dv = cross2(d, v);
+    if (std::fabs(dv) <= 0.0f) return false;
+    float ud = cross2(u, d);
+    s = ud / dv;
+    if (s < 0.0f || s > 1.0f) return false;
+    float uv = cross2(u, v);
+    t = uv / dv;
+    return t >= 0.0f && t <= tMax;
+}
+
+// Moller-Trumbore
+inline bool rayTriangle(const V<3>& o, const V<3>& d, float tMax, const V<3>& pa, const V<3>& pb, const V<3>& pc, float& t) {
+    V<3> e1 = pb - pa, e2 = pc - pa;
+    V<3> p = cross3(d, e2);
+    float det = dot(e1, p);
+    if (std::fabs(det) <= 0.0f) return false;
+    float inv = 1.0f / det;
+    V<3> s = o - pa;
+    float u = dot(s, p) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    V<3> q = cross3(s, e1);
+    float v = dot(d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, q) * inv;
+    return t >= 0.0f && t <= tMax;
+}
+
+// offsetPointAlongDirection (core/geometric_queries.h:175-207)
+inline int floatAsInt(float a) { int b; std::memcpy(&b, &a, 4); return b; }
+inline float intAsFloat(int a) { float b; std::memcpy(&b, &a, 4); return b; }
+template <int D>
+V<D> offsetPoint(const V<D>& p, const V<D>& n) {
+    const float origin = 1.0f / 32.0f, floatScale = 1.0f / 65536.0f, intScale = 256.0f;
+    V<D> r;
+    for (int i = 0; i < D; i++) {
+        int nOff = (int)(n[i] * intScale);
+        float pOff = intAsFloat(floatAsInt(p[i]) + (p[i] < 0 ? -nOff : nOff));
+        r[i] = std::fabs(p[i]) < origin ? p[i] + floatScale * n[i] : pOff;
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------------------------
+// binary binned-SAH BVH (FCPW Bvh_SurfaceArea defaults: leaf size 4, 8 buckets)
+// ------------------------------------------------------------------------------------
+template <int D>
+struct Box {
+    V<D> lo, hi;
+    Box() { for (int i = 0; i < D; i++) { lo[i] = FLT_MAX; hi[i] = -FLT_MAX; } }
+    void expand(const V<D>& p) { for (int i = 0; i < D; i++) { lo[i] = std::min(lo[i], p[i]); hi[i] = std::max(hi[i], p[i]); } }
+    void expand(const Box& b) { for (int i = 0; i < D; i++) { lo[i] = std::min(lo[i], b.lo[i]); hi[i] = std::max(hi[i], b.hi[i]); } }
+    float area() const {
+        V<D> e = hi - lo;
+        for (int i = 0; i < D; i++) if (e[i] < 0) return 0.0f;
+        if (D == 2) return 2.0f * (e[0] + e[1]);
+        return 2.0f * (e[0] * e[1] + e[1] * e[2] + e[0] * e[2]);
+    }
+    float dist2(const V<D>& x) const {
+        float s = 0.0f;
+        for (int i = 0; i < D; i++) {
+            float d = std::max(std::max(lo[i] - x[i], x[i] - hi[i]), 0.0f);
+            s += d * d;
+        }
+        return s;
+    }
+    V<D> centroid() const { return (lo + hi) * 0.5f; }
+    // slab test
+    bool rayOverlap(const V<D>& o, const V<D>& invD, float tMax) const {
+        float t0 = 0.0f, t1 = tMax;
+        for (int i = 0; i < D; i++) {
+            float a = (lo[i] - o[i]) * invD[i], b = (hi[i] - o[i]) * invD[i];
+            if (a > b) std::swap(a, b);
+            t0 = std::max(t0, a); t1 = std::min(t1, b);
+            if (std::isnan(a) || std::isnan(b)) { t0 = 0.0f; }
+        }
+        return t0 <= t1 * 1.00001f + 1e-7f;
+    }
+};
+
+template <int D>
+struct Bvh {
+    struct Node { Box<D> box; int left = -1, right = -1, start = 0, count = 0; };
+    std::vector<Node> nodes;
+    std::vector<int> order;  // primitive indices in leaf order
+    int depth = 0;
+
+    void build(const std::vector<Box<D>>& boxes, int leafSize, int nBuckets) {
+        int n = (int)boxes.size();
+        order.resize(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+        nodes.clear();
+        if (n == 0) return;
+        std::vector<V<D>> cent(n);
+        for (int i = 0; i < n; i++) cent[i] = boxes[i].centroid();
+        nodes.reserve(2 * n);
+        depth = 0;
+        buildRec(boxes, cent, 0, n, leafSize, nBuckets, 1);
+    }
+
+    int buildRec(const std::vector<Box<D>>& boxes, const std::vector<V<D>>& cent,
+                 int start, int end, int leafSize, int nBuckets, int level) {
+        depth = std::max(depth, level);
+        int id = (int)nodes.size();
+        nodes.push_back(Node());
+        Box<D> bb, cb;
+        for (int i = start; i < end; i++) { bb.expand(boxes[order[i]]); cb.expand(cent[order[i]]); }
+        nodes[id].box = bb;
+        int n = end - start;
+        if (n <= leafSize) { nodes[id].start = start; nodes[id].count = n; return id; }
+        // binned SAH over all axes
+        float bestCost = FLT_MAX; int bestAxis = -1, bestSplit = -1;
+        for (int a = 0; a < D; a++) {
+            float ext = cb.hi[a] - cb.lo[a];
+            if (ext <= 0.0f) continue;
+            std::vector<Box<D>> bbox(nBuckets);
+            std::vector<int> cnt(nBuckets, 0);
+            for (int i = start; i < end; i++) {
+                int b = (int)(nBuckets * (cent[order[i]][a] - cb.lo[a]) / ext);
+                b = std::min(std::max(b, 0), nBuckets - 1);
+                cnt[b]++; bbox[b].expand(boxes[order[i]]);
+            }
+            for (int s = 1; s < nBuckets; s++) {
+                Box<D> l, r; int nl = 0, nr = 0;
+                for (int b = 0; b < s; b++) { if (cnt[b]) { l.expand(bbox[b]); nl += cnt[b]; } }
+                for (int b = s; b < nBuckets; b++) { if (cnt[b]) { r.expand(bbox[b]); nr += cnt[b]; } }
+                if (nl == 0 || nr == 0) continue;
+                float cost = l.area() * nl + r.area() * nr;
+                if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = s; }
+            }
+        }
+        int mid;
+        if (bestAxis < 0) {
+            mid = start + n / 2;
+        } else {
+            float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
+            auto it = std::partition(order.begin() + start, order.begin() + end, [&](int p) {
+                int b = (int)(nBuckets * (cent[p][bestAxis] - cb.lo[bestAxis]) / ext);
+                b = std::min(std::max(b, 0), nBuckets - 1);
+                return b < bestSplit;
+            });
+            mid = (int)(it - order.begin());
+            if (mid == start || mid == end) mid = start + n / 2;
+        }
+        int l = buildRec(boxes, cent, start, mid, leafSize, nBuckets, level + 1);
+        int r = buildRec(boxes, cent, mid, end, leafSize, nBuckets, level + 1);
+        nodes[id].left = l; nodes[id].right = r;
+        return id;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// boundary meshes (absorbing / reflecting) with queries
+// ------------------------------------------------------------------------------------
+template <int D>
+struct Mesh {
+    std::vector<V<D>> verts;
+    std::vector<int> idx;          // D indices per primitive
+    std::vector<int> origId;       // primitive id in the caller's mesh
+    std::vector<Box<D>> boxes;
+    Bvh<D> bvh;
+    // 2D silhouette candidates: vertex -> adjacent primitive(s)
+    struct SilVertex { int v; int p0; int p1; };  // p1 = -1 when the vertex has one neighbour
+    std::vector<SilVertex> sil;
+
+    int nPrims() const { return (int)origId.size(); }
+    V<D> vtx(int p, int k) const { return verts[idx[p * D + k]]; }
+
+    V<D> primNormal(int p) const {
+        if constexpr (D == 2) return segmentNormal(vtx(p, 0), vtx(p, 1));
+        else return triangleNormal(vtx(p, 0), vtx(p, 1), vtx(p, 2));
+    }
+    float primArea(int p) const {
+        if constexpr (D == 2) return norm(vtx(p, 1) - vtx(p, 0));
+        else return 0.5f * norm(cross3(vtx(p, 1) - vtx(p, 0), vtx(p, 2) - vtx(p, 0)));
+    }
+    float primClosest(int p, const V<D>& x, V<D>& pt) const {
+        if constexpr (D == 2) return closestPointSegment(vtx(p, 0), vtx(p, 1), x, pt);
+        else return closestPointTriangle(vtx(p, 0), vtx(p, 1), vtx(p, 2), x, pt);
+    }
+    bool primRay(int p, const V<D>& o, const V<D>& d, float tMax, float& t) const {
+        if constexpr (D == 2) { float s; return raySegment(o, d, tMax, vtx(p, 0), vtx(p, 1), t, s); }
+        else return rayTriangle(o, d, tMax, vtx(p, 0), vtx(p, 1), vtx(p, 2), t);
+    }
+
+    void build(int leafSize, int nBuckets) {
+        int n = nPrims();
+        boxes.resize(n);
+        for (int p = 0; p < n; p++) { Box<D> b; for (int k = 0; k < D; k++) b.expand(vtx(p, k)); boxes[p] = b; }
+        bvh.build(boxes, leafSize, nBuckets);
+    }
+
+    // silhouette candidates (2D): each vertex with its adjacent segments; with one-sided
+    // problems, convex vertices are ignored (getIgnoreCandidateSilhouetteCallback,
+    // fcpw_geometric_queries.h:1029-1038)
+    void buildSilhouettes(bool solveDoubleSided, float precision) {
+        sil.clear();
+        if constexpr (D == 2) {
+            int nv = (int)verts.size();
+            std::vector<int> first(nv, -1), second(nv, -1), asEnd(nv, -1), asStart(nv, -1);
+            for (int p = 0; p < nPrims(); p++) {
+                for (int k = 0; k < 2; k++) {
+                    int v = idx[p * 2 + k];
+                    if (first[v] < 0) first[v] = p; else if (second[v] < 0) second[v] = p;
+                    if (k == 0) asStart[v] = p; else asEnd[v] = p;
+                }
+            }
+            for (int v = 0; v < nv; v++) {
+                if (first[v] < 0) continue;
+                if (second[v] < 0) { sil.push_back({v, first[v], -1}); continue; }
+                // dihedral angle at v, positive at reflex vertices of a boundary with outward
+                // normals; convex (and flat) candidates are ignored for one-sided problems
+                if (!solveDoubleSided && asEnd[v] >= 0 && asStart[v] >= 0) {
+                    V<2> n0 = primNormal(asEnd[v]), n1 = primNormal(asStart[v]);
+                    float ang = std::atan2(cross2(n1, n0), dot(n0, n1));
+                    if (ang < precision) continue;
+                }
+                sil.push_back({v, first[v], second[v]});
+            }
+        }
+    }
+
+    bool isSilhouette(const SilVertex& s, const V<2>& x, float precision) const {
+        if (s.p1 < 0) return true;
+        V<2> v = verts[s.v];
+        V<2> view = x - v;
+        float l = norm(view);
+        if (l <= 0.0f) return true;
+        view = view * (1.0f / l);
+        float f0 = dot(primNormal(s.p0), view), f1 = dot(primNormal(s.p1), view);
+        bool bothFront = f0 > precision && f1 > precision;
+        bool bothBack = f0 < -precision && f1 < -precision;
+        return !(bothFront || bothBack);
+    }
+};
+
+template <int D>
+struct Scene {
+    Mesh<D> abs, refl;
+    V<D> dmin, dmax;
+    bool hasAbs() const { return abs.nPrims() > 0; }
+    bool hasRefl() const { return refl.nPrims() > 0; }
+    bool outsideBoundingDomain(const V<D>& x) const {
+        for (int i = 0; i < D; i++) if (x[i] < dmin[i] || x[i] > dmax[i]) return true;
+        return false;
+    }
+};
+
+struct QueryCount { int64_t nodes = 0, prims = 0; };
+
+// closest point: nearest-child-first traversal with pruning (fcpw findClosestPoint with
+// sphere r2 = maxFloat, fcpw_geometric_queries.h:1069-1096)
+template <int D>
+bool closestPoint(const Mesh<D>& m, const V<D>& x, float& d, V<D>& pt, int& prim, QueryCount* qc) {
+    prim = -1; d = FLT_MAX;
+    if (m.nPrims() == 0) return false;
+    float best2 = FLT_MAX;
+    struct E { int n; float d2; };
+    E stack[128];
+    int sp = 0;
+    const auto& nodes = m.bvh.nodes;
+    stack[sp++] = {0, nodes[0].box.dist2(x)};
+    while (sp > 0) {
+        E e = stack[--sp];
+        if (e.d2 > best2) continue;
+        const auto& nd = nodes[e.n];
+        if (qc) qc->nodes++;
+        if (nd.left < 0) {
+            for (int i = nd.start; i < nd.start + nd.count; i++) {
+                int p = m.bvh.order[i];
+                V<D> q; float dd = m.primClosest(p, x, q);
+                if (qc) qc->prims++;
+                if (dd * dd < best2) { best2 = dd * dd; d = dd; pt = q; prim = p; }
+            }
+            continue;
+        }
+        float dl = nodes[nd.left].box.dist2(x), dr = nodes[nd.right].box.dist2(x);
+        int nearN = nd.left, farN = nd.right; float dn = dl, df = dr;
+        if (dr < dl) { nearN = nd.right; farN = nd.left; dn = dr; df = dl; }
+        if (df <= best2) stack[sp++] = {farN, df};
+        if (dn <= best2) stack[sp++] = {nearN, dn};
+    }
+    return prim >= 0;
+}
+
+template <int D>
+bool closestPointBrute(const Mesh<D>& m, const V<D>& x, float& d, V<D>& pt, int& prim) {
+    prim = -1; d = FLT_MAX;
+    for (int p = 0; p < m.nPrims(); p++) {
+        V<D> q; float dd = m.primClosest(p, x, q);
+        if (dd < d) { d = dd; pt = q; prim = p; }
+    }
+    return prim >= 0;
+}
+
+// first hit along a ray
+template <int D>
+bool intersectRay(const Mesh<D>& m, const V<D>& o, const V<D>& dir, float tMax, float& tHit, int& prim, bool brute) {
+    prim = -1; tHit = tMax;
+    if (m.nPrims() == 0) return false;
+    if (brute) {
+        for (int p = 0; p < m.nPrims(); p++) { float t; if (m.primRay(p, o, dir, tHit, t)) { if (prim < 0 || t < tHit) { tHit = t; prim = p; } } }
+        return prim >= 0;
+    }
+    V<D> inv; for (int i = 0; i < D; i++) inv[i] = 1.0f / dir[i];
+    int stack[128]; int sp = 0; stack[sp++] = 0;
+    const auto& nodes = m.bvh.nodes;
+    while (sp > 0) {
+        int ni = stack[--sp];
+        const auto& nd = nodes[ni];
+        if (!nd.box.rayOverlap(o, inv, tHit)) continue;
+        if (nd.left < 0) {
+            for (int i = nd.start; i < nd.start + nd.count; i++) {
+                int p = m.bvh.order[i]; float t;
+                if (m.primRay(p, o, dir, tHit, t)) { if (prim < 0 || t < tHit) { tHit = t; prim = p; } }
+            }
+            continue;
+        }
+        stack[sp++] = nd.left; stack[sp++] = nd.right;
+    }
+    return prim >= 0;
+}
+
+// number of hits along a ray (all hits), for insideDomain
+template <int D>
+int countHits(const Mesh<D>& m, const V<D>& o, const V<D>& dir) {
+    int hits = 0;
+    for (int p = 0; p < m.nPrims(); p++) { float t; if (m.primRay(p, o, dir, FLT_MAX, t)) hits++; }
+    return hits;
+}
+
+// closest silhouette point (2D), fcpw_geometric_queries.h:1254-1268
+inline float starRadius2(const Mesh<2>& m, const V<2>& x, float minRadius, float maxRadius, float precision) {
+    if (minRadius > maxRadius) return maxRadius;
+    float best = FLT_MAX; bool found = false;
+    for (const auto& s : m.sil) {
+        float d = norm(x - m.verts[s.v]);
+        if (d > maxRadius) continue;
+        if (d >= best) continue;
+        if (m.isSilhouette(s, x, precision)) { best = d; found = true; }
+    }
+    return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
+}
+
+// line of sight (2D): no reflecting segment crosses the open segment p1 -> p2
+inline bool hasLineOfSight(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) {
+    V<2> d = p2 - p1; float L = norm(d);
+    if (L <= 0.0f) return true;
+    V<2> dir = d * (1.0f / L);
+    for (int p = 0; p < m.nPrims(); p++) {
+        float t, s;
+        if (raySegment(p1, dir, L, m.vtx(p, 0), m.vtx(p, 1), t, s)) {
+            if (t > 0.0f && t < L) return false;
+        }
+    }
+    return true;
+}
+
+// sampleReflectingBoundary (2D): stochastic selection of a reflecting segment whose box
+// overlaps the ball, with probability proportional to branchTraversalWeight(d^2 to the
+// segment's box centre) * segment length, then a uniform point on it.  The returned pdf is
+// the exact density of the sample (the estimator alpha*G*h/pdf stays unbiased for any pdf
+// that is positive on the boundary inside the ball).
+inline float branchWeight(float r2) {
+    float r = std::max(std::sqrt(r2), 1e-2f);
+    return std::fabs(1.0f / (4.0f * PI_F * r));  // HarmonicGreensFnFreeSpace<3>::evaluate
+}
+inline bool sampleReflecting2(const Mesh<2>& m, const V<2>& x, float radius, const float* u,
+                              V<2>& pt, V<2>& nrm, float& pdf) {
+    float R2 = radius * radius;
+    double total = 0.0;
+    std::vector<std::pair<int, double>> cand;
+    for (int p = 0; p < m.nPrims(); p++) {
+        if (m.boxes[p].dist2(x) > R2) continue;
+        double w = (double)branchWeight(norm2(m.boxes[p].centroid() - x)) * m.primArea(p);
+        if (w <= 0.0) continue;
+        cand.push_back({p, w}); total += w;
+    }
+    if (cand.empty()) return false;
+    double target = u[0] * total, acc = 0.0; int chosen = cand.back().first; double wch = cand.back().second;
+    for (auto& c : cand) { acc += c.second; if (target < acc) { chosen = c.first; wch = c.second; break; } }
+    V<2> pa = m.vtx(chosen, 0), pb = m.vtx(chosen, 1);
+    pt = pa + (pb - pa) * u[1];
+    nrm = m.primNormal(chosen);
+    pdf = (float)(wch / total) / m.primArea(chosen);
+    return true;
+}
+
+// ------------------------------------------------------------------------------------
+// statistics accumulation (FP64 Welford + Chan merge, common.h:225-310,381-386)
+// ------------------------------------------------------------------------------------
+struct Stats {
+    int64_t n = 0; double mean = 0.0, m2 = 0.0; int64_t len = 0, splits = 0;
+    void add(double x) { n++; double d = x - mean; mean += d / (double)n; double d2 = x - mean; m2 += d * d2; }
+};
+inline void mergeInto(int64_t& n, double& mean, double& m2, const Stats& o) {
+    if (o.n <= 0) return;
+    int64_t N1 = n, N2 = o.n, N = N1 + N2;
+    if (N1 == 0) { mean = o.mean; m2 = o.m2; }
+    else {
+        double d = o.mean - mean;
+        mean = mean + d * ((double)N2 / (double)N);
+        m2 = m2 + o.m2 + d * d * ((double)N1 * (double)N2 / (double)N);
+    }
+    n = N;
+}
+
+// ------------------------------------------------------------------------------------
+// walk state (common.h:93-130)
+// ------------------------------------------------------------------------------------
+template <int D>
+struct WalkState {
+    float totalReflectingBoundaryContribution = 0.0f;
+    float totalSourceContribution = 0.0f;
+    V<D> currentPt, currentNormal, prevDirection;
+    float prevDistance = 0.0f, throughput = 1.0f;
+    int walkLength = 0;
+    bool onReflectingBoundary = false;
+};
+
+enum Code { Reached = 0, RR = 1, MaxLen = 2, Escaped = 3 };
+
+struct SolveCtx {
+    const zg_pde* pde;
+    const zg_walk_settings* ws;
+    uint64_t seed; uint32_t streamOffset; int rngMode;
+};
+
+// applyWeightWindow (walk_on_spheres.h:192-226, walk_on_stars.h:309-343)
+template <int D>
+bool applyWeightWindow(const zg_walk_settings& ws, Pcg32& rng, WalkState<D>& st, std::deque<WalkState<D>>& q) {
+    if (st.throughput > ws.splitting_threshold) {
+        float left = st.throughput - ws.splitting_threshold;
+        st.throughput = ws.splitting_threshold;
+        WalkState<D> s2; s2.currentPt = st.currentPt; s2.currentNormal = st.currentNormal;
+        s2.prevDirection = st.prevDirection; s2.prevDistance = st.prevDistance;
+        s2.throughput = st.throughput; s2.walkLength = 0; s2.onReflectingBoundary = st.onReflectingBoundary;
+        while (left > ws.splitting_threshold) { left -= ws.splitting_threshold; q.push_back(s2); }
+        if (rng.nextFloat() < left / ws.splitting_threshold) q.push_back(s2);
+    } else if (st.throughput < ws.russian_roulette_threshold) {
+        float survival = st.throughput / ws.russian_roulette_threshold;
+        if (survival < rng.nextFloat()) { st.throughput = 0.0f; return true; }
+        st.throughput = ws.russian_roulette_threshold;
+    }
+    return false;
+}
+
+// ----- Walk on Spheres: walk (walk_on_spheres.h:228-291) -----
+template <int D>
+Code wosWalk(const Scene<D>& sc, const SolveCtx& cx, float dist, GreensBall<D>& gf, Pcg32& rng,
+             WalkState<D>& st, std::deque<WalkState<D>>& q, QueryCount* qc) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    while (dist > ws.epsilon_shell_for_absorbing_boundary) {
+        gf.updateBall(st.currentPt, dist);
+        if (!ws.ignore_source_contribution) {  // computeSourceContribution :170-190
+            float sc_;
+            V<D> nz = V<D>::zero();
+            if (pde.is_source_constant) {
+                sc_ = gf.norm() * evalField<D>(pde.source, st.currentPt, nz, false, pde.absorption_coeff);
+            } else {
+                float r, pdf;
+                V<D> y = gf.sampleVolume(rng, r, pdf);
+                sc_ = gf.norm() * evalField<D>(pde.source, y, nz, false, pde.absorption_coeff);
+            }
+            st.totalSourceContribution += st.throughput * sc_;
+        }
+        V<D> dir = sampleUnitSphere<D>(rng);
+        st.currentPt = st.currentPt + dist * dir;
+        st.throughput *= gf.directionSampledPoissonKernel(st.currentPt);
+        if (sc.outsideBoundingDomain(st.currentPt)) return Escaped;
+        if (applyWeightWindow(ws, rng, st, q)) return RR;
+        st.walkLength++;
+        if (st.walkLength >= ws.max_walk_length) return MaxLen;
+        if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == st.walkLength)
+            gf.makeYukawa(pde.absorption_coeff);
+        V<D> cp; int prim;
+        closestPoint(sc.abs, st.currentPt, dist, cp, prim, qc);
+    }
+    return Reached;
+}
+
+// getTerminalContribution (walk_on_spheres.h:293-318)
+template <int D>
+float wosTerminal(const Scene<D>& sc, const SolveCtx& cx, Code code, WalkState<D>& st, QueryCount* qc) {
+    const zg_walk_settings& ws = *cx.ws;
+    if ((code == Reached || code == MaxLen) && !ws.ignore_absorbing_boundary_contribution) {
+        float d; V<D> cp; int prim;
+        bool found = closestPoint(sc.abs, st.currentPt, d, cp, prim, qc);
+        bool aligned = false;
+        if (found) {
+            V<D> n = sc.abs.primNormal(prim);
+            float sd = dot(st.currentPt - cp, n);
+            st.currentPt = cp; st.currentNormal = n;
+            aligned = ws.solve_double_sided && sd > 0.0f;
+        }
+        return evalField<D>(cx.pde->dirichlet, st.currentPt, st.currentNormal, aligned, cx.pde->absorption_coeff);
+    }
+    return 0.0f;
+}
+
+inline Pcg32 makeRng(const SolveCtx& cx, int64_t p, int w) {
+    uint64_t seq = ((uint64_t)p << 32) | (uint64_t)(uint32_t)(cx.streamOffset + (cx.rngMode == 1 ? (uint32_t)w : 0u));
+    return Pcg32(cx.seed, seq);
+}
+
+// estimateSolution (walk_on_spheres.h:320-414)
+template <int D>
+void wosEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>& pt, uint8_t type,
+                 float distA, int nWalks, Stats& out, int64_t codes[4], QueryCount* qc,
+                 int64_t prevN, float* firstSphereRadius) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    bool hasPrev = prevN > 0;
+    if (type == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
+        if (!hasPrev) {
+            float tc = 0.0f;
+            if (!ws.ignore_absorbing_boundary_contribution)
+                tc = evalField<D>(pde.dirichlet, pt, V<D>::zero(), false, pde.absorption_coeff);
+            out.add(tc);
+            if (firstSphereRadius) *firstSphereRadius = 0.0f;
+        }
+        return;
+    } else if (distA <= ws.epsilon_shell_for_absorbing_boundary) {
+        nWalks = std::min(nWalks, 1);
+    }
+    if (!hasPrev && firstSphereRadius) *firstSphereRadius = distA;
+    Pcg32 pointRng = makeRng(cx, p, 0);
+    std::deque<WalkState<D>> q;
+    for (int w = 0; w < nWalks; w++) {
+        Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
+        Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
+        WalkState<D> st0; st0.currentPt = pt; st0.currentNormal = V<D>::zero(); st0.prevDirection = V<D>::zero();
+        q.push_back(st0);
+        int splits = -1; float total = 0.0f; bool success = false;
+        while (!q.empty()) {
+            WalkState<D> st = q.front(); q.pop_front();
+            splits++;
+            GreensBall<D> gf;
+            if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
+            float dist = distA;
+            if (splits > 0) { V<D> cp; int prim; closestPoint(sc.abs, st.currentPt, dist, cp, prim, qc); }
+            Code code = wosWalk<D>(sc, cx, dist, gf, rng, st, q, qc);
+            codes[code]++;
+            if (code != Escaped) {
+                float term = wosTerminal<D>(sc, cx, code, st, qc);
+                total += st.throughput * term + st.totalSourceContribution;
+                out.len += st.walkLength;
+                success = true;
+            }
+        }
+        if (success) { out.add((double)total); out.splits += splits; }
+    }
+}
+
+// ----- Walk on Stars (walk_on_stars.h:189-662), 2D -----
+template <int D>
+void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBall<D>& gf, float starRadius,
+                                bool flip, Pcg32& rng, WalkState<D>& st) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    if (!(sc.hasRefl() && !ws.ignore_reflecting_boundary_contribution)) return;
+    float u[D]; for (int i = 0; i < D; i++) u[i] = rng.nextFloat();
+    V<D> bpt, bn; float bpdf;
+    bool ok = false;
+    if constexpr (D == 2) ok = sampleReflecting2(sc.refl, st.currentPt, starRadius, u, bpt, bn, bpdf);
+    if (!ok) return;
+    V<D> toS = bpt - st.currentPt;
+    float distS = norm(toS);
+    float alpha = st.onReflectingBoundary ? 2.0f : 1.0f;
+    bool aligned = false;
+    if (ws.solve_double_sided) {
+        toS = toS * (1.0f / distS);
+        if (flip) aligned = true;
+        else if (dot(toS, bn) < -ws.silhouette_precision) {
+            bool flipS = true;
+            if (alpha > 1.0f) flipS = dot(toS, st.currentNormal) < -ws.silhouette_precision;
+            if (flipS) aligned = true;
+        }
+    }
+    V<D> bnS = bn * (aligned ? -1.0f : 1.0f);
+    if (bpdf > 0.0f && distS < starRadius) {
+        V<D> p1 = st.onReflectingBoundary ? offsetPoint<D>(st.currentPt, st.currentNormal * -1.0f) : st.currentPt;
+        V<D> p2 = offsetPoint<D>(bpt, bnS * -1.0f);
+        bool blocked = false;
+        if constexpr (D == 2) blocked = !hasLineOfSight(sc.refl, p1, p2);
+        if (!blocked) {
+            float r = std::max(gf.rClamp, distS);
+            float G = gf.evaluate(r);
+            bool retAligned = ws.solve_double_sided && aligned;
+            float h = evalField<D>(pde.robin, bpt, bn, retAligned, pde.absorption_coeff);
+            st.totalReflectingBoundaryContribution += st.throughput * alpha * G * h / bpdf;
+        }
+    }
+}
+
+template <int D>
+float wostStarRadius(const Scene<D>& sc, const SolveCtx& cx, const V<D>& x, float distA, bool flip) {
+    const zg_walk_settings& ws = *cx.ws;
+    (void)flip;
+    float r = distA;
+    if constexpr (D == 2) {
+        if (sc.hasRefl()) r = starRadius2(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision);
+    }
+    if (sc.hasRefl() && ws.epsilon_shell_for_reflecting_boundary <= distA)
+        r = std::max(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
+    return r;
+}
+
+template <int D>
+Code wostWalk(const Scene<D>& sc, const SolveCtx& cx, float distA, float firstSphereRadius, bool flip,
+              GreensBall<D>& gf, Pcg32& rng, WalkState<D>& st, std::deque<WalkState<D>>& q, QueryCount* qc) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    bool firstStep = true;
+    while (distA > ws.epsilon_shell_for_absorbing_boundary) {
+        float starRadius;
+        if (firstStep && firstSphereRadius > 0.0f) {
+            starRadius = firstSphereRadius;
+        } else {
+            flip = false;
+            if (ws.solve_double_sided && st.onReflectingBoundary) {
+                if (st.prevDistance > 0.0f && dot(st.prevDirection, st.currentNormal) < 0.0f) {
+                    st.currentNormal = st.currentNormal * -1.0f; flip = true;
+                }
+            }
+            if (ws.steps_before_using_maximal_spheres <= st.walkLength) starRadius = distA;
+            else starRadius = wostStarRadius<D>(sc, cx, st.currentPt, distA, flip);
+        }
+        gf.updateBall(st.currentPt, starRadius);
+        V<D> dir = sampleUnitSphere<D>(rng);
+        if (st.onReflectingBoundary && dot(st.currentNormal, dir) > 0.0f) dir = dir * -1.0f;
+        // intersectReflectingBoundary (fcpw_geometric_queries.h:1178-1196)
+        V<D> o = st.onReflectingBoundary ? offsetPoint<D>(st.currentPt, st.currentNormal * -1.0f) : st.currentPt;
+        float tHit; int prim = -1;
+        bool hit = sc.hasRefl() && intersectRay(sc.refl, o, dir, starRadius, tHit, prim, false);
+        V<D> ipt, inrm = V<D>::zero(); float idist;  // IntersectionPoint() default normal is zero
+        if (hit) { ipt = o + tHit * dir; inrm = sc.refl.primNormal(prim); idist = tHit;
+            // fcpw reports the hit point on the primitive
+            if constexpr (D == 2) {
+                V<2> pa = sc.refl.vtx(prim, 0), pb = sc.refl.vtx(prim, 1); float t2, s2;
+                if (raySegment(o, dir, FLT_MAX, pa, pb, t2, s2)) ipt = pa + (pb - pa) * s2;
+            }
+        } else { ipt = o + starRadius * dir; idist = starRadius; }
+        wostReflectingContribution<D>(sc, cx, gf, starRadius, flip, rng, st);
+        if (!ws.ignore_source_contribution) {  // computeSourceContribution :252-275
+            float r, pdf;
+            V<D> y = gf.sampleVolumeDir(dir, rng, r, pdf);
+            if (r <= idist) {
+                float s_ = gf.norm() * evalField<D>(pde.source, y, V<D>::zero(), false, pde.absorption_coeff);
+                st.totalSourceContribution += st.throughput * s_;
+            }
+        }
+        st.prevDistance = idist;
+        st.prevDirection = dir;
+        st.currentPt = ipt;
+        st.currentNormal = inrm;
+        st.onReflectingBoundary = hit;
+        // computeWalkStepThroughput (:277-307), pure Neumann
+        float thr;
+        if (st.onReflectingBoundary && st.prevDistance > FLT_EPSILON) {
+            thr = std::min(std::max(gf.reflectance(st.prevDistance), 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
+        } else {
+            thr = gf.directionSampledPoissonKernel(st.currentPt);
+        }
+        st.throughput *= thr;
+        if (!st.onReflectingBoundary && sc.outsideBoundingDomain(st.currentPt)) return Escaped;
+        if (applyWeightWindow(ws, rng, st, q)) return RR;
+        st.walkLength++;
+        if (st.walkLength >= ws.max_walk_length) return MaxLen;
+        if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == st.walkLength)
+            gf.makeYukawa(pde.absorption_coeff);
+        V<D> cp; int pr;
+        closestPoint(sc.abs, st.currentPt, distA, cp, pr, qc);
+        firstStep = false;
+    }
+    return Reached;
+}
+
+template <int D>
+void wostEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>& pt, const V<D>& normal,
+                  uint8_t type, bool estAligned, float distA, int nWalks, Stats& out, int64_t codes[4],
+                  QueryCount* qc, int64_t prevN, float* firstSphereRadiusIO) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    bool hasPrev = prevN > 0;
+    if (type == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
+        if (!hasPrev) {
+            float tc = 0.0f;
+            if (!ws.ignore_absorbing_boundary_contribution)
+                tc = evalField<D>(pde.dirichlet, pt, V<D>::zero(), ws.solve_double_sided && estAligned, pde.absorption_coeff);
+            out.add(tc);
+            if (firstSphereRadiusIO) *firstSphereRadiusIO = 0.0f;
+        }
+        return;
+    } else if (distA <= ws.epsilon_shell_for_absorbing_boundary) {
+        nWalks = std::min(nWalks, 1);
+    }
+    V<D> curN = normal, prevDir = normal; float prevDist = FLT_MAX; bool flip = false;
+    if (ws.solve_double_sided && type == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && estAligned) {
+        curN = curN * -1.0f; prevDir = prevDir * -1.0f; flip = true;
+    }
+    float fsr = firstSphereRadiusIO ? *firstSphereRadiusIO : 0.0f;
+    if (!hasPrev) {
+        if (distA <= ws.epsilon_shell_for_absorbing_boundary || ws.steps_before_using_maximal_spheres == 0) fsr = distA;
+        else fsr = wostStarRadius<D>(sc, cx, pt, distA, flip);
+        if (firstSphereRadiusIO) *firstSphereRadiusIO = fsr;
+    }
+    Pcg32 pointRng = makeRng(cx, p, 0);
+    std::deque<WalkState<D>> q;
+    for (int w = 0; w < nWalks; w++) {
+        Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
+        Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
+        WalkState<D> st0; st0.currentPt = pt; st0.currentNormal = curN; st0.prevDirection = prevDir;
+        st0.prevDistance = prevDist; st0.onReflectingBoundary = type == ZG_SAMPLE_ON_REFLECTING_BOUNDARY;
+        q.push_back(st0);
+        int splits = -1; float total = 0.0f; bool success = false;
+        while (!q.empty()) {
+            WalkState<D> st = q.front(); q.pop_front();
+            splits++;
+            GreensBall<D> gf;
+            if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
+            float dA = distA, fs = fsr;
+            if (splits > 0) { V<D> cp; int pr; closestPoint(sc.abs, st.currentPt, dA, cp, pr, qc); fs = 0.0f; }
+            Code code = wostWalk<D>(sc, cx, dA, fs, flip, gf, rng, st, q, qc);
+            codes[code]++;
+            if (code != Escaped) {
+                float term = 0.0f;
+                if (code == Reached && !ws.ignore_absorbing_boundary_contribution) {
+                    float d; V<D> cp; int prim;
+                    bool aligned = false;
+                    if (closestPoint(sc.abs, st.currentPt, d, cp, prim, qc)) {
+                        V<D> n = sc.abs.primNormal(prim);
+                        float sd = dot(st.currentPt - cp, n);
+                        st.currentPt = cp; st.currentNormal = n;
+                        aligned = ws.solve_double_sided && sd > 0.0f;
+                    }
+                    term = evalField<D>(pde.dirichlet, st.currentPt, st.currentNormal, aligned, pde.absorption_coeff);
+                }
+                total += st.throughput * term + st.totalReflectingBoundaryContribution + st.totalSourceContribution;
+                out.len += st.walkLength;
+                success = true;
+            }
+        }
+        if (success) { out.add((double)total); out.splits += splits; }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// thread pool over points (tbb::parallel_for replacement)
+// ------------------------------------------------------------------------------------
+template <class F>
+void parallelFor(int64_t n, int nThreads, F f) {
+    if (nThreads <= 1 || n < 64) { for (int64_t i = 0; i < n; i++) f(i, 0); return; }
+    std::atomic<int64_t> next{0};
+    const int64_t chunk = 8;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nThreads; t++) {
+        th.emplace_back([&, t]() {
+            for (;;) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                int64_t e = std::min(n, b + chunk);
+                for (int64_t i = b; i < e; i++) f(i, t);
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+}
+
+struct SceneHandle {
+    int dim;
+    std::unique_ptr<Scene<2>> s2;
+    std::unique_ptr<Scene<3>> s3;
+};
+
+template <int D>
+void buildScene(Scene<D>& sc, const float* V_, int64_t nV, const int32_t* I, int64_t nP, const uint8_t* refl,
+                const float* dmin, const float* dmax, int leafSize, int nBuckets, bool dbl, float prec) {
+    for (Mesh<D>* m : {&sc.abs, &sc.refl}) {
+        m->verts.resize(nV);
+        for (int64_t i = 0; i < nV; i++) m->verts[i] = load<D>(V_ + i * D);
+    }
+    for (int64_t p = 0; p < nP; p++) {
+        Mesh<D>& m = (refl && refl[p]) ? sc.refl : sc.abs;
+        for (int k = 0; k < D; k++) m.idx.push_back(I[p * D + k]);
+        m.origId.push_back((int)p);
+    }
+    sc.abs.build(leafSize, nBuckets);
+    sc.refl.build(leafSize, nBuckets);
+    sc.refl.buildSilhouettes(dbl, prec);
+    sc.dmin = load<D>(dmin); sc.dmax = load<D>(dmax);
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+template <int D>
+static void closestImpl(Scene<D>& s, int boundary, int64_t n, const float* pts, float* dist, float* cp, float* nrm,
+                        int32_t* prim, int brute, int64_t* visits, int nThreads) {
+    Mesh<D>& m = boundary ? s.refl : s.abs;
+    std::atomic<int64_t> nodes{0}, prims{0};
+    parallelFor(n, nThreads, [&](int64_t i, int) {
+        V<D> x = load<D>(pts + i * D), q; float d; int p; QueryCount qc;
+        bool ok = brute ? closestPointBrute(m, x, d, q, p) : closestPoint(m, x, d, q, p, &qc);
+        nodes += qc.nodes; prims += qc.prims;
+        dist[i] = ok ? d : FLT_MAX;
+        if (cp) store<D>(cp + i * D, ok ? q : V<D>::zero());
+        if (nrm) store<D>(nrm + i * D, ok ? m.primNormal(p) : V<D>::zero());
+        if (prim) prim[i] = ok ? m.origId[p] : -1;
+    });
+    if (visits) { visits[0] = nodes.load(); visits[1] = prims.load(); }
+}
+
+template <int D>
+static void rayImpl(Scene<D>& s, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
+                    uint8_t* hit, float* t, int32_t* prim, int brute) {
+    Mesh<D>& m = boundary ? s.refl : s.abs;
+    for (int64_t i = 0; i < n; i++) {
+        float th; int p;
+        bool h_ = intersectRay(m, load<D>(o + i * D), load<D>(d + i * D), tmax[i], th, p, brute != 0);
+        hit[i] = h_; t[i] = h_ ? th : tmax[i]; if (prim) prim[i] = h_ ? m.origId[p] : -1;
+    }
+}
+
+template <int D>
+static void insideImpl(Scene<D>& s, int64_t n, const float* pts, uint8_t* inside, int nThreads) {
+    parallelFor(n, nThreads, [&](int64_t i, int) {
+        V<D> x = load<D>(pts + i * D);
+        bool in = true;
+        for (int a = 0; a < D; a++) {
+            V<D> dir = V<D>::zero(); dir[a] = 1.0f;
+            int hits = countHits(s.abs, x, dir) + countHits(s.refl, x, dir);
+            in = in && (hits % 2 == 1);
+        }
+        inside[i] = in;
+    });
+}
+
+template <int D>
+static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_settings* ws, int64_t n,
+                     const float* pts, const float* normals, const uint8_t* types, const uint8_t* aligned,
+                     const float* distA, const int32_t* nWalks, uint64_t seed, uint32_t streamOffset, int rngMode,
+                     int nThreads, int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits,
+                     float* firstSphereRadius, int64_t* report) {
+    if (pde->absorption_coeff > 0.0f && D == 2) return ZG_ERR_UNSUPPORTED;
+    SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
+    int nt = std::max(1, nThreads);
+    std::vector<std::array<int64_t, 8>> per(nt);
+    for (auto& a : per) a.fill(0);
+    parallelFor(n, nt, [&](int64_t i, int t) {
+        V<D> x = load<D>(pts + i * D);
+        V<D> nr = normals ? load<D>(normals + i * D) : V<D>::zero();
+        uint8_t ty = types ? types[i] : 0;
+        Stats st; int64_t codes[4] = {0, 0, 0, 0}; QueryCount qc;
+        if (wost) wostEstimate<D>(sc, cx, i, x, nr, ty, aligned ? aligned[i] != 0 : false, distA[i], nWalks[i], st, codes, &qc, sN[i], firstSphereRadius ? firstSphereRadius + i : nullptr);
+        else wosEstimate<D>(sc, cx, i, x, ty, distA[i], nWalks[i], st, codes, &qc, sN[i], firstSphereRadius ? firstSphereRadius + i : nullptr);
+        mergeInto(sN[i], sMean[i], sM2[i], st);
+        sLen[i] += st.len; sSplits[i] += st.splits;
+        for (int k = 0; k < 4; k++) per[t][k] += codes[k];
+        per[t][4] += st.len; per[t][5] += qc.nodes; per[t][6] += qc.prims;
+    });
+    if (report) { for (int k = 0; k < 7; k++) { report[k] = 0; for (auto& a : per) report[k] += a[k]; } }
+    return ZG_OK;
+}
+
+
+extern "C" {
+
+void orc_pcg32_u32(uint64_t initstate, uint64_t initseq, int64_t n, uint32_t* out) {
+    Pcg32 r(initstate, initseq);
+    for (int64_t i = 0; i < n; i++) out[i] = r.nextUInt();
+}
+void orc_pcg32_f32(uint64_t initstate, uint64_t initseq, int64_t n, float* out) {
+    Pcg32 r(initstate, initseq);
+    for (int64_t i = 0; i < n; i++) out[i] = r.nextFloat();
+}
+void orc_pcg32_default_u32(int64_t n, uint32_t* out) {
+    Pcg32 r;
+    for (int64_t i = 0; i < n; i++) out[i] = r.nextUInt();
+}
+
+// sphere / ball samplers for moment and closed-form tests
+void orc_sample_unit_sphere(int dim, uint64_t seed, int64_t n, float* out) {
+    Pcg32 r(seed, 7);
+    for (int64_t i = 0; i < n; i++) {
+        if (dim == 2) store<2>(out + 2 * i, sampleUnitSphere2(r));
+        else store<3>(out + 3 * i, sampleUnitSphere3(r));
+    }
+}
+// radius samples of the harmonic / Yukawa ball source sampler (sampleVolume); lambda 0 = harmonic
+void orc_sample_ball_radius(int dim, float R, float lambda, uint64_t seed, int64_t n, float* r_out, float* pdf_out) {
+    Pcg32 rng(seed, 11);
+    for (int64_t i = 0; i < n; i++) {
+        float r = 0, pdf = 0;
+        if (dim == 2) { GreensBall<2> g; if (lambda > 0) g.makeYukawa(lambda); g.updateBall(V<2>::zero(), R); g.sampleVolume(rng, r, pdf); }
+        else { GreensBall<3> g; if (lambda > 0) g.makeYukawa(lambda); g.updateBall(V<3>::zero(), R); g.sampleVolume(rng, r, pdf); }
+        r_out[i] = r; if (pdf_out) pdf_out[i] = pdf;
+    }
+}
+// Green's function quantities of a ball of radius R: out = [norm, poissonKernel, G(r), dampening(R)]
+void orc_greens_ball(int dim, float R, float lambda, float r, float* out) {
+    if (dim == 2) {
+        GreensBall<2> g; if (lambda > 0) g.makeYukawa(lambda); g.updateBall(V<2>::zero(), R);
+        out[0] = g.norm(); out[1] = g.poissonKernel(); out[2] = g.evaluate(r); out[3] = lambda > 0 ? g.poissonKernelDampeningFactor(R) : 1.0f;
+    } else {
+        GreensBall<3> g; if (lambda > 0) g.makeYukawa(lambda); g.updateBall(V<3>::zero(), R);
+        out[0] = g.norm(); out[1] = g.poissonKernel(); out[2] = g.evaluate(r); out[3] = lambda > 0 ? g.poissonKernelDampeningFactor(R) : 1.0f;
+    }
+}
+void orc_offset_point(int dim, const float* p, const float* n, float* out) {
+    if (dim == 2) store<2>(out, offsetPoint<2>(load<2>(p), load<2>(n)));
+    else store<3>(out, offsetPoint<3>(load<3>(p), load<3>(n)));
+}
+float orc_eval_function(int dim, const zg_function* f, const float* x, const float* n, int aligned, float absorption) {
+    if (dim == 2) return evalField<2>(*f, load<2>(x), n ? load<2>(n) : V<2>::zero(), aligned != 0, absorption);
+    return evalField<3>(*f, load<3>(x), n ? load<3>(n) : V<3>::zero(), aligned != 0, absorption);
+}
+
+void* orc_scene_create(int dim, const float* V_, int64_t nV, const int32_t* I, int64_t nP, const uint8_t* refl,
+                       const float* dmin, const float* dmax, int leafSize, int nBuckets, int dbl, float prec) {
+    SceneHandle* h = new SceneHandle();
+    h->dim = dim;
+    if (dim == 2) { h->s2.reset(new Scene<2>()); buildScene<2>(*h->s2, V_, nV, I, nP, refl, dmin, dmax, leafSize, nBuckets, dbl, prec); }
+    else { h->s3.reset(new Scene<3>()); buildScene<3>(*h->s3, V_, nV, I, nP, refl, dmin, dmax, leafSize, nBuckets, dbl, prec); }
+    return h;
+}
+void orc_scene_destroy(void* h) { delete (SceneHandle*)h; }
+// out: [absNodes, absDepth, reflNodes, reflDepth, nSilhouetteCandidates]
+void orc_scene_stats(void* hv, int64_t* out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) { auto& s = *h->s2; out[0] = s.abs.bvh.nodes.size(); out[1] = s.abs.bvh.depth; out[2] = s.refl.bvh.nodes.size(); out[3] = s.refl.bvh.depth; out[4] = s.refl.sil.size(); }
+    else { auto& s = *h->s3; out[0] = s.abs.bvh.nodes.size(); out[1] = s.abs.bvh.depth; out[2] = s.refl.bvh.nodes.size(); out[3] = s.refl.bvh.depth; out[4] = 0; }
+}
+
+void orc_closest_point(void* hv, int boundary, int64_t n, const float* pts, float* dist, float* cp, float* nrm,
+                       int32_t* prim, int brute, int64_t* visits, int nThreads) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) closestImpl<2>(*h->s2, boundary, n, pts, dist, cp, nrm, prim, brute, visits, nThreads);
+    else closestImpl<3>(*h->s3, boundary, n, pts, dist, cp, nrm, prim, brute, visits, nThreads);
+}
+
+void orc_intersect_ray(void* hv, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
+                       uint8_t* hit, float* t, int32_t* prim, int brute) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) rayImpl<2>(*h->s2, boundary, n, o, d, tmax, hit, t, prim, brute);
+    else rayImpl<3>(*h->s3, boundary, n, o, d, tmax, hit, t, prim, brute);
+}
+
+void orc_inside_domain(void* hv, int64_t n, const float* pts, uint8_t* inside, int nThreads) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) insideImpl<2>(*h->s2, n, pts, inside, nThreads);
+    else insideImpl<3>(*h->s3, n, pts, inside, nThreads);
+}
+
+void orc_star_radius(void* hv, int64_t n, const float* pts, const float* minR, const float* maxR, float prec, float* out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) { for (int64_t i = 0; i < n; i++) out[i] = maxR[i]; return; }
+    for (int64_t i = 0; i < n; i++) out[i] = h->s2->hasRefl() ? starRadius2(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec) : maxR[i];
+}
+
+// silhouette test of every candidate vertex seen from x; out[k] = 1 when candidate k is a silhouette
+int64_t orc_silhouette_candidates(void* hv, const float* x, float prec, float* verts_out, uint8_t* is_sil_out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) return 0;
+    const auto& m = h->s2->refl;
+    for (size_t k = 0; k < m.sil.size(); k++) {
+        if (verts_out) store<2>(verts_out + 2 * k, m.verts[m.sil[k].v]);
+        if (is_sil_out) is_sil_out[k] = m.isSilhouette(m.sil[k], load<2>(x), prec);
+    }
+    return (int64_t)m.sil.size();
+}
+
+int orc_sample_reflecting(void* hv, const float* x, float radius, const float* u, float* pt, float* nrm, float* pdf) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) return 0;
+    V<2> p, nn; float pd;
+    bool ok = sampleReflecting2(h->s2->refl, load<2>(x), radius, u, p, nn, pd);
+    if (ok) { store<2>(pt, p); store<2>(nrm, nn); *pdf = pd; }
+    return ok;
+}
+
+int orc_line_of_sight(void* hv, const float* p1, const float* p2) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) return 1;
+    return hasLineOfSight(h->s2->refl, load<2>(p1), load<2>(p2));
+}
+
+// Solves.  rng_mode 0: one generator per point consumed across its walks (the reference's
+// scheme, seeded deterministically); 1: one generator per walk (the GPU's scheme).
+// Stats arrays are accumulated with Chan's merge.  report: [codes0..3, totalSteps, nodeVisits, primVisits]
+
+int orc_solve(int wost, void* hv, const zg_pde* pde, const zg_walk_settings* ws, int64_t n, const float* pts,
+              const float* normals, const uint8_t* types, const uint8_t* aligned, const float* distA,
+              const int32_t* nWalks, uint64_t seed, uint32_t streamOffset, int rngMode, int nThreads,
+              int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits, float* firstSphereRadius,
+              int64_t* report) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) return solveImpl<2>(wost != 0, *h->s2, pde, ws, n, pts, normals, types, aligned, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report);
+    return solveImpl<3>(wost != 0, *h->s3, pde, ws, n, pts, normals, types, aligned, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report);
+}
+
+}  // extern "C"

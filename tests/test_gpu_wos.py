@@ -1,0 +1,188 @@
+"""Statistical parity of the GPU Walk-on-Spheres solve with the CPU oracle, plus
+manufactured-solution accuracy and completion-code bookkeeping."""
+import numpy as np
+import pytest
+
+from zombie_amd import _abi as A
+from zombie_amd import engine as E
+from zombie_amd import problems as PB
+
+from helpers import check_four_se, rmse, z_scores
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_points(torch, pts, dist, n_walks, types=None):
+    n = len(pts)
+    return E.DevicePoints(pt=torch.from_numpy(np.ascontiguousarray(pts, np.float32)).cuda(),
+                          dist_to_absorbing=torch.from_numpy(np.ascontiguousarray(dist, np.float32)).cuda(),
+                          n_walks=torch.full((n,), n_walks, dtype=torch.int32, device="cuda"),
+                          n_walks_max=n_walks,
+                          type=None if types is None else torch.from_numpy(types.astype(np.uint8)).cuda())
+
+
+def _gpu_solve(torch, prob, pts, dist, n_walks, seed=1, stream_offset=0, stats=None, ws=None, opts=None, types=None,
+               scene=None):
+    sc = scene or E.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max)
+    dp = _device_points(torch, pts, dist, n_walks, types)
+    st = stats or E.DeviceStats.zeros(len(pts))
+    o = opts or E.SolveOptions(seed=seed, stream_offset=stream_offset)
+    rep = E.wos_solve(sc, prob.pde, ws or prob.settings, dp, st, o)
+    return st, rep
+
+
+def _host(st):
+    return dict(n=st.n.cpu().numpy(), mean=st.mean.cpu().numpy(), var=st.variance().cpu().numpy(),
+                len=st.walk_length.cpu().numpy())
+
+
+@pytest.fixture(scope="module")
+def star(oracle_lib):
+    p = PB.star2d()
+    pts = p.meta["points"][::12]
+    sc = oracle_lib.Scene(p.positions, p.indices, None, p.domain_min, p.domain_max)
+    d, *_ = sc.closest_point(pts)
+    return p, pts, d, sc
+
+
+@pytest.fixture(scope="module")
+def ball3(oracle_lib):
+    V, F = PB.icosphere(4)
+    V = PB.sh_displace(V, 11)
+    p = PB.sphere3d()
+    p = PB.Problem("ball3", 3, V.astype(np.float32), F.astype(np.int32), None,
+                   (V.min(0) - 1e-3).astype(np.float32), (V.max(0) + 1e-3).astype(np.float32),
+                   p.pde, p.settings, 64, p.exact)
+    sc = oracle_lib.Scene(p.positions, p.indices, None, p.domain_min, p.domain_max)
+    cand = PB.sample_box(p.domain_min, p.domain_max, 6000, 5)
+    pts = cand[sc.inside_domain(cand)][:1500]
+    d, *_ = sc.closest_point(pts)
+    return p, pts, d, sc
+
+
+def test_wos2d_parity_with_oracle(gpu, oracle_lib, star):
+    torch = gpu
+    O = oracle_lib
+    p, pts, d, osc = star
+    st, rep = _gpu_solve(torch, p, pts, d, 256, seed=101)
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    orep = O.solve(osc, p.pde, p.settings, pts, d, 256, ost, seed=202, rng_mode=O.RNG_PER_POINT)
+    assert (g["n"] == ost.n).all()
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, "wos2d gpu vs oracle")
+    u = p.exact(pts)
+    se = np.sqrt(np.mean(g["var"] / g["n"]))
+    assert rmse(g["mean"], u) < 1.5 * se + 2e-3
+    assert rep.completion_codes[0] == rep.total_walks
+    # mean walk length agrees with the oracle's (same estimator)
+    ml_g = g["len"].sum() / g["n"].sum()
+    ml_o = ost.walk_length.sum() / ost.n.sum()
+    assert abs(ml_g - ml_o) / ml_o < 0.02
+
+
+def test_wos2d_same_streams_track_oracle(gpu, oracle_lib, star):
+    """With the same per-walk PCG32 streams the GPU and the oracle draw the same numbers, so
+    walks agree except where FP32 rounding moves a walk across the epsilon shell."""
+    torch = gpu
+    O = oracle_lib
+    p, pts, d, osc = star
+    st, _ = _gpu_solve(torch, p, pts, d, 64, seed=77)
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    O.solve(osc, p.pde, p.settings, pts, d, 64, ost, seed=77, rng_mode=O.RNG_PER_WALK)
+    same_len = (g["len"] == ost.walk_length).mean()
+    assert same_len > 0.5, f"walk-length agreement {same_len:.3f}"
+    se = np.sqrt(np.maximum(g["var"], 1e-12) / 64)
+    assert np.median(np.abs(g["mean"] - ost.mean) / se) < 0.2
+
+
+def test_wos3d_poisson_parity_with_oracle(gpu, oracle_lib, ball3):
+    torch = gpu
+    O = oracle_lib
+    p, pts, d, osc = ball3
+    st, rep = _gpu_solve(torch, p, pts, d, 128, seed=5)
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    O.solve(osc, p.pde, p.settings, pts, d, 128, ost, seed=6, rng_mode=O.RNG_PER_POINT)
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, "wos3d gpu vs oracle")
+    u = p.exact(pts)
+    se = np.sqrt(np.mean(g["var"] / g["n"]))
+    assert rmse(g["mean"], u) < 1.5 * se + 2e-3
+
+
+def test_progressive_calls_equal_one_long_call(gpu, star):
+    torch = gpu
+    p, pts, d, _ = star
+    sc = E.Scene(p.positions, p.indices, None, p.domain_min, p.domain_max)
+    st2 = E.DeviceStats.zeros(len(pts))
+    _gpu_solve(torch, p, pts, d, 32, seed=9, stream_offset=0, stats=st2, scene=sc)
+    _gpu_solve(torch, p, pts, d, 32, seed=9, stream_offset=32, stats=st2, scene=sc)
+    st1, _ = _gpu_solve(torch, p, pts, d, 64, seed=9, stream_offset=0, scene=sc)
+    a, b = _host(st2), _host(st1)
+    # points inside the epsilon shell run one walk per call (walk_on_spheres.h:348-351)
+    shell = d <= p.settings.epsilon_shell_for_absorbing_boundary
+    assert (a["n"][shell] == 2).all() and (b["n"][shell] == 1).all()
+    m = ~shell
+    assert (a["n"][m] == b["n"][m]).all()
+    assert (a["len"][m] == b["len"][m]).all()
+    np.testing.assert_allclose(a["mean"][m], b["mean"][m], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(a["var"][m], b["var"][m], rtol=1e-7, atol=1e-12)
+
+
+def test_completion_codes_forced_from_host(gpu, oracle_lib, star):
+    torch = gpu
+    O = oracle_lib
+    p, pts, d, osc = star
+    n = len(pts)
+    # max walk length 1: every walk that starts outside the shell exceeds it
+    ws = E.walk_settings_host(1e-3, 1e-3, 1)
+    st, rep = _gpu_solve(torch, p, pts, d, 8, ws=ws)
+    ost = O.Stats(n)
+    orep = O.solve(osc, p.pde, ws, pts, d, 8, ost)
+    assert tuple(rep.completion_codes) == tuple(orep["codes"])
+    # escape: a bounding domain smaller than the geometry
+    p2 = PB.Problem(**{**p.__dict__, "domain_min": np.float32([-0.2, -0.2]), "domain_max": np.float32([0.2, 0.2])})
+    st, rep = _gpu_solve(torch, p2, pts, d, 8)
+    osc2 = O.Scene(p.positions, p.indices, None, p2.domain_min, p2.domain_max)
+    ost = O.Stats(n)
+    orep = O.solve(osc2, p.pde, p.settings, pts, d, 8, ost, seed=1)
+    assert rep.completion_codes[3] > 0
+    assert abs(rep.completion_codes[3] - orep["codes"][3]) < 0.05 * orep["codes"][3] + 20
+    # escaped walks are dropped: those points get fewer estimates
+    assert (st.n.cpu().numpy() <= 8).all()
+    # sample points on the absorbing boundary record g once and run no walk
+    types = np.zeros(n, np.uint8)
+    types[::3] = A.ZG_SAMPLE_ON_ABSORBING_BOUNDARY
+    st, rep = _gpu_solve(torch, p, pts, d, 8, types=types)
+    g = _host(st)
+    assert (g["n"][::3] == 1).all()
+    np.testing.assert_allclose(g["mean"][::3], p.exact(pts[::3]), rtol=1e-5, atol=1e-5)
+    assert rep.total_walks <= 8 * (n - len(types[::3]))
+
+
+def test_russian_roulette_and_screened_poisson(gpu, oracle_lib, ball3):
+    torch = gpu
+    O = oracle_lib
+    p0, pts, d, osc = ball3
+    sigma = 4.0
+    a = np.sqrt(sigma) * np.array([0.6, 0.0, 0.8])
+    u = E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a))
+    pde = E.make_pde(source=E.zero(), dirichlet=u, absorption=sigma, is_source_constant=True)
+    for rr in (0.0, 0.5):
+        ws = E.walk_settings_host(1e-4, 1e-4, 1024, russian_roulette_threshold=rr)
+        prob = PB.Problem(**{**p0.__dict__, "pde": pde, "settings": ws})
+        st, rep = _gpu_solve(torch, prob, pts, d, 128, seed=3)
+        g = _host(st)
+        ost = O.Stats(len(pts))
+        orep = O.solve(osc, pde, ws, pts, d, 128, ost, seed=4, rng_mode=O.RNG_PER_POINT)
+        z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+        check_four_se(z, ok, f"screened rr={rr}")
+        ue = PB.exp_linear(pts, a)
+        se = np.sqrt(np.mean(g["var"] / g["n"]))
+        assert rmse(g["mean"], ue) < 1.5 * se + 3e-3
+        if rr > 0:
+            assert rep.completion_codes[1] > 0 and orep["codes"][1] > 0
+            ratio = rep.completion_codes[1] / orep["codes"][1]
+            assert 0.9 < ratio < 1.1

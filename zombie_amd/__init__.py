@@ -1,0 +1,20 @@
+"""zombie_amd: an MI355X (gfx950) Walk-on-Spheres / Walk-on-Stars engine.
+
+The GPU work happens in libzombie_amd.so (hand-written HIP for gfx950) behind the C-ABI of
+include/zombie_amd.h.  `engine` is the array-level host API, `api` mirrors the reference's
+Python surface (zombie.Core / zombie.Solvers / zombie.Utils), `problems` builds the
+manufactured benchmark problems.
+"""
+from . import _abi
+from ._abi import ZombieError, ZombieLibraryError
+
+__version__ = "0.1.0"
+
+
+def library_path():
+    return _abi.LIB_PATH
+
+
+def load_library():
+    """Loads libzombie_amd.so; raises ZombieLibraryError when it is missing."""
+    return _abi.lib()

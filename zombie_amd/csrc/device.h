@@ -1,0 +1,538 @@
+// device.h -- gfx950 device building blocks: PCG32, samplers, ball Green's functions,
+// manufactured PDE data, and wide-BVH traversals with an LDS short stack per lane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/zombie_amd.h"
+#include "layout.h"
+
+namespace zg {
+
+#define ZG_DEV __device__ __forceinline__
+
+// ------------------------------------------------------------------------------------
+// small vectors
+// ------------------------------------------------------------------------------------
+template <int D>
+struct Vec {
+    float v[D];
+    ZG_DEV float& operator[](int i) { return v[i]; }
+    ZG_DEV float operator[](int i) const { return v[i]; }
+};
+template <int D> ZG_DEV Vec<D> vzero() { Vec<D> r; for (int i = 0; i < D; i++) r.v[i] = 0.0f; return r; }
+template <int D> ZG_DEV Vec<D> operator+(Vec<D> a, const Vec<D>& b) { for (int i = 0; i < D; i++) a.v[i] += b.v[i]; return a; }
+template <int D> ZG_DEV Vec<D> operator-(Vec<D> a, const Vec<D>& b) { for (int i = 0; i < D; i++) a.v[i] -= b.v[i]; return a; }
+template <int D> ZG_DEV Vec<D> operator*(Vec<D> a, float s) { for (int i = 0; i < D; i++) a.v[i] *= s; return a; }
+template <int D> ZG_DEV float dot(const Vec<D>& a, const Vec<D>& b) { float s = 0.0f; for (int i = 0; i < D; i++) s += a.v[i] * b.v[i]; return s; }
+template <int D> ZG_DEV float norm2(const Vec<D>& a) { return dot(a, a); }
+template <int D> ZG_DEV Vec<D> vload(const float* p) { Vec<D> r; for (int i = 0; i < D; i++) r.v[i] = p[i]; return r; }
+template <int D> ZG_DEV void vstore(float* p, const Vec<D>& a) { for (int i = 0; i < D; i++) p[i] = a.v[i]; }
+ZG_DEV Vec<3> cross3(const Vec<3>& a, const Vec<3>& b) {
+    Vec<3> r; r[0] = a[1] * b[2] - a[2] * b[1]; r[1] = a[2] * b[0] - a[0] * b[2]; r[2] = a[0] * b[1] - a[1] * b[0]; return r;
+}
+ZG_DEV float cross2(const Vec<2>& a, const Vec<2>& b) { return a[0] * b[1] - a[1] * b[0]; }
+
+// ------------------------------------------------------------------------------------
+// PCG32 (PCG-XSH-RR 64/32, O'Neill 2014); same stream semantics as the pcg32 dependency
+// of core/sampling.h:19 (seed(initstate, initseq), nextUInt, mantissa nextFloat)
+// ------------------------------------------------------------------------------------
+struct Pcg32 {
+    uint64_t state, inc;
+    ZG_DEV uint32_t next() {
+        uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+    }
+    ZG_DEV void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+    ZG_DEV float nextFloat() { return __uint_as_float((next() >> 9) | 0x3f800000u) - 1.0f; }
+};
+
+// ------------------------------------------------------------------------------------
+// SphereSampler<DIM>::sampleUnitSphereUniform (core/sampling.h:104-111,164-174).
+// v_sin_f32 / v_cos_f32 take their argument in revolutions: sin(2*pi*u) = v_sin(u).
+// ------------------------------------------------------------------------------------
+template <int D> ZG_DEV Vec<D> sampleUnitSphere(Pcg32& rng);
+template <> ZG_DEV Vec<2> sampleUnitSphere<2>(Pcg32& rng) {
+    float u = rng.nextFloat();
+    Vec<2> d; d[0] = __builtin_amdgcn_cosf(u); d[1] = __builtin_amdgcn_sinf(u);
+    return d;
+}
+template <> ZG_DEV Vec<3> sampleUnitSphere<3>(Pcg32& rng) {
+    float u0 = rng.nextFloat();
+    float u1 = rng.nextFloat();
+    float z = 1.0f - 2.0f * u0;
+    float r = __builtin_sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    Vec<3> d; d[0] = r * __builtin_amdgcn_cosf(u1); d[1] = r * __builtin_amdgcn_sinf(u1); d[2] = z;
+    return d;
+}
+
+// ------------------------------------------------------------------------------------
+// Green's function of the ball (core/distributions.h:453-665 harmonic, 835-1005 Yukawa 3D)
+// ------------------------------------------------------------------------------------
+constexpr float kPi = 3.14159265358979323846f;
+
+template <int D>
+struct Ball {
+    bool yukawa;
+    float sqrtLambda;
+    float R;
+    float muR, expmuR, sinhmuR;  // Yukawa terms of updateBall (distributions.h:843-852)
+
+    ZG_DEV void update(float R_) {
+        R = R_;
+        if (yukawa) {
+            muR = R * sqrtLambda;
+            expmuR = __expf(-muR);
+            sinhmuR = (1.0f - expmuR * expmuR) / (2.0f * expmuR);
+        }
+    }
+    ZG_DEV float evaluate(float r) const {
+        if (!yukawa) {
+            if (D == 2) return __logf(R / r) / (2.0f * kPi);
+            return (1.0f / r - 1.0f / R) / (4.0f * kPi);
+        }
+        float mur = r * sqrtLambda;
+        float expmur = __expf(-mur);
+        float sinhmur = (1.0f - expmur * expmur) / (2.0f * expmur);
+        return (expmur - expmuR * sinhmur / sinhmuR) / (4.0f * kPi * r);
+    }
+    ZG_DEV float norm() const {
+        if (!yukawa) return D == 2 ? R * R * 0.25f : R * R * (1.0f / 6.0f);
+        float lambda = sqrtLambda * sqrtLambda;
+        return (1.0f - muR / sinhmuR) / lambda;  // (1 - 4 pi P)/lambda, P = muR/(4 pi sinh muR)
+    }
+    // poissonKernelDampeningFactor(r) = mur (K32(mur) + I32(mur) e^{-muR}/sinh muR)
+    ZG_DEV float dampening(float r) const {
+        float mur = r * sqrtLambda;
+        float expmur = __expf(-mur);
+        float exp2mur = expmur * expmur;
+        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
+        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
+        float K32 = expmur * (1.0f + 1.0f / mur);
+        float I32 = coshmur - sinhmur / mur;
+        return mur * (K32 + I32 * expmuR / sinhmuR);
+    }
+    ZG_DEV float pdfSphere(float r) const { return D == 2 ? 1.0f / (2.0f * kPi * r) : 1.0f / (4.0f * kPi * r * r); }
+
+    // sampleVolume(dir, rng, r, pdf) radius part; returns r (distributions.h:460-470,565-577,855-864,419-440)
+    ZG_DEV float sampleRadius(Pcg32& rng) const {
+        float r;
+        if (!yukawa) {
+            if (D == 2) {
+                float u1 = rng.nextFloat();
+                float u2 = rng.nextFloat();
+                r = R * __builtin_sqrtf(u1 * u2);
+            } else {
+                float u1 = rng.nextFloat();
+                float u2 = rng.nextFloat();
+                float u3 = rng.nextFloat();
+                float med = fmaxf(fminf(u1, u2), fminf(fmaxf(u1, u2), u3));
+                r = R * med;
+            }
+        } else {
+            float lambda = sqrtLambda * sqrtLambda;
+            float bound = R <= lambda ?
+                fmaxf(fmaxf(2.0f / R, 2.0f / lambda), fmaxf(0.5f * __builtin_sqrtf(R), 0.5f * sqrtLambda)) :
+                fmaxf(fminf(2.0f / R, 2.0f / lambda), fminf(0.5f * __builtin_sqrtf(R), 0.5f * sqrtLambda));
+            float fnNorm = norm();
+            int iter = 0;
+            do {
+                float u = rng.nextFloat();
+                r = rng.nextFloat() * R;
+                float pdf = evaluate(r) / fnNorm;
+                float pdfRadius = pdf / pdfSphere(r);
+                iter++;
+                if (u < pdfRadius / bound) break;
+            } while (iter < 1000);
+        }
+        r = fmaxf(1e-4f, r);
+        if (r > R) r = R * 0.5f;
+        return r;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// manufactured PDE data: zg_function descriptors (stand-ins for the PDE<T,DIM> callbacks)
+// ------------------------------------------------------------------------------------
+template <int D>
+ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, bool aligned, float absorption) {
+    const float* p = f.params;
+    float u = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, negLap = 0.0f;
+    switch (f.kind) {
+    case ZG_FN_CONSTANT: u = p[0]; break;
+    case ZG_FN_EXP_COS: {
+        float e = expf(p[1] * x[0]);
+        float s, c;
+        sincosf(p[1] * x[1], &s, &c);
+        u = p[0] * e * c;
+        g0 = p[0] * p[1] * e * c;
+        g1 = -p[0] * p[1] * e * s;
+        break;
+    }
+    case ZG_FN_SIN_PRODUCT: {
+        float s[3] = {1.0f, 1.0f, 1.0f}, c[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < D; i++) sincosf(p[1] * x[i], &s[i], &c[i]);
+        u = p[0] * s[0] * s[1] * s[2];
+        g0 = p[0] * p[1] * c[0] * s[1] * s[2];
+        g1 = p[0] * p[1] * s[0] * c[1] * s[2];
+        g2 = D == 3 ? p[0] * p[1] * s[0] * s[1] * c[2] : 0.0f;
+        negLap = (float)D * p[1] * p[1] * u;
+        break;
+    }
+    case ZG_FN_CUBIC_HARMONIC: {
+        float xx = x[0], yy = x[1];
+        u = p[0] * (xx * xx * xx - 3.0f * xx * yy * yy);
+        g0 = p[0] * (3.0f * xx * xx - 3.0f * yy * yy);
+        g1 = -p[0] * 6.0f * xx * yy;
+        break;
+    }
+    case ZG_FN_EXP_LINEAR: {
+        float a = p[1] * x[0] + p[2] * x[1] + (D == 3 ? p[3] * x[D == 3 ? 2 : 0] : 0.0f);
+        u = p[0] * expf(a);
+        g0 = p[1] * u; g1 = p[2] * u; g2 = D == 3 ? p[3] * u : 0.0f;
+        float a2 = p[1] * p[1] + p[2] * p[2] + (D == 3 ? p[3] * p[3] : 0.0f);
+        negLap = -a2 * u;
+        break;
+    }
+    default: break;
+    }
+    if (f.eval == ZG_EVAL_VALUE) return u;
+    if (f.eval == ZG_EVAL_NORMAL_DERIVATIVE) {
+        float dn = g0 * n[0] + g1 * n[1] + (D == 3 ? g2 * n[D == 3 ? 2 : 0] : 0.0f);
+        return aligned ? -dn : dn;
+    }
+    return negLap + absorption * u;
+}
+
+// ------------------------------------------------------------------------------------
+// primitives (Ericson, "Real-Time Collision Detection" 5.1.2 / 5.1.5; the same closed forms
+// as core/geometry_helpers.h:28-49,94-161)
+// ------------------------------------------------------------------------------------
+ZG_DEV float closestOnSegment2(const Vec<2>& pa, const Vec<2>& pb, const Vec<2>& x, Vec<2>& pt) {
+    Vec<2> u = pb - pa, v = x - pa;
+    float c1 = dot(u, v);
+    float c2 = dot(u, u);
+    float t = c1 <= 0.0f ? 0.0f : (c2 <= c1 ? 1.0f : c1 / c2);
+    pt = c1 <= 0.0f ? pa : (c2 <= c1 ? pb : pa + u * t);
+    return norm2(x - pt);
+}
+
+ZG_DEV float closestOnTriangle2(const Vec<3>& pa, const Vec<3>& pb, const Vec<3>& pc, const Vec<3>& x, Vec<3>& pt) {
+    Vec<3> ab = pb - pa, ac = pc - pa, ax = x - pa;
+    float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    Vec<3> bx = x - pb;
+    float d3 = dot(ab, bx), d4 = dot(ac, bx);
+    Vec<3> cx = x - pc;
+    float d5 = dot(ab, cx), d6 = dot(ac, cx);
+    float vc = d1 * d4 - d3 * d2;
+    float vb = d5 * d2 - d1 * d6;
+    float va = d3 * d6 - d5 * d4;
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        pt = pa;
+    } else if (d3 >= 0.0f && d4 <= d3) {
+        pt = pb;
+    } else if (d6 >= 0.0f && d5 <= d6) {
+        pt = pc;
+    } else if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        pt = pa + ab * (d1 / (d1 - d3));
+    } else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        pt = pa + ac * (d2 / (d2 - d6));
+    } else if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        pt = pb + (pc - pb) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)));
+    } else {
+        float denom = 1.0f / (va + vb + vc);
+        pt = pa + ab * (vb * denom) + ac * (vc * denom);
+    }
+    return norm2(x - pt);
+}
+
+ZG_DEV Vec<2> segmentNormal(const Vec<2>& pa, const Vec<2>& pb) {
+    Vec<2> s = pb - pa; Vec<2> n; n[0] = s[1]; n[1] = -s[0];
+    return n * (1.0f / __builtin_sqrtf(norm2(n)));
+}
+ZG_DEV Vec<3> triangleNormal(const Vec<3>& pa, const Vec<3>& pb, const Vec<3>& pc) {
+    Vec<3> n = cross3(pb - pa, pc - pa);
+    return n * (1.0f / __builtin_sqrtf(norm2(n)));
+}
+
+template <int D> struct Prim;
+template <> struct Prim<2> {
+    Vec<2> a, b;
+    ZG_DEV void load(const float* prims, int i) {
+        float4 q = reinterpret_cast<const float4*>(prims)[i];
+        a[0] = q.x; a[1] = q.y; b[0] = q.z; b[1] = q.w;
+    }
+    ZG_DEV float closest2(const Vec<2>& x, Vec<2>& pt) const { return closestOnSegment2(a, b, x, pt); }
+    ZG_DEV Vec<2> normal() const { return segmentNormal(a, b); }
+    // ray o + t d, t in [0, tMax]
+    ZG_DEV bool ray(const Vec<2>& o, const Vec<2>& d, float tMax, float& t) const {
+        Vec<2> u = a - o, v = b - a;
+        float dv = cross2(d, v);
+        if (dv == 0.0f) return false;
+        float s = cross2(u, d) / dv;
+        if (s < 0.0f || s > 1.0f) return false;
+        t = cross2(u, v) / dv;
+        return t >= 0.0f && t <= tMax;
+    }
+};
+template <> struct Prim<3> {
+    Vec<3> a, b, c;
+    ZG_DEV void load(const float* prims, int i) {
+        const float4* q = reinterpret_cast<const float4*>(prims) + 3 * i;
+        float4 q0 = q[0], q1 = q[1], q2 = q[2];
+        a[0] = q0.x; a[1] = q0.y; a[2] = q0.z; b[0] = q0.w;
+        b[1] = q1.x; b[2] = q1.y; c[0] = q1.z; c[1] = q1.w; c[2] = q2.x;
+    }
+    ZG_DEV float closest2(const Vec<3>& x, Vec<3>& pt) const { return closestOnTriangle2(a, b, c, x, pt); }
+    ZG_DEV Vec<3> normal() const { return triangleNormal(a, b, c); }
+    // Moller-Trumbore
+    ZG_DEV bool ray(const Vec<3>& o, const Vec<3>& d, float tMax, float& t) const {
+        Vec<3> e1 = b - a, e2 = c - a;
+        Vec<3> p = cross3(d, e2);
+        float det = dot(e1, p);
+        if (det == 0.0f) return false;
+        float inv = 1.0f / det;
+        Vec<3> s = o - a;
+        float u = dot(s, p) * inv;
+        if (u < 0.0f || u > 1.0f) return false;
+        Vec<3> q = cross3(s, e1);
+        float v = dot(d, q) * inv;
+        if (v < 0.0f || u + v > 1.0f) return false;
+        t = dot(e2, q) * inv;
+        return t >= 0.0f && t <= tMax;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// wide node access
+// ------------------------------------------------------------------------------------
+template <int D, int W>
+struct WideNode {
+    float lo[D][W], hi[D][W];
+    int ch[W];
+    ZG_DEV void load(const float* nodes, int idx) {
+        using L = NodeLayout<D, W>;
+        const float4* q = reinterpret_cast<const float4*>(nodes + (size_t)idx * L::kFloats);
+        constexpr int kBoxVec = (2 * D * W) / 4;
+        float tmp[2 * D * W];
+#pragma unroll
+        for (int k = 0; k < kBoxVec; k++) {
+            float4 v = q[k];
+            tmp[4 * k] = v.x; tmp[4 * k + 1] = v.y; tmp[4 * k + 2] = v.z; tmp[4 * k + 3] = v.w;
+        }
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int k = 0; k < W; k++) { lo[a][k] = tmp[a * W + k]; hi[a][k] = tmp[(D + a) * W + k]; }
+        const int4* c = reinterpret_cast<const int4*>(q + kBoxVec);
+#pragma unroll
+        for (int k = 0; k < W / 4; k++) {
+            int4 v = c[k];
+            ch[4 * k] = v.x; ch[4 * k + 1] = v.y; ch[4 * k + 2] = v.z; ch[4 * k + 3] = v.w;
+        }
+    }
+    ZG_DEV float dist2(int k, const Vec<D>& x) const {
+        float s = 0.0f;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            float d = fmaxf(fmaxf(lo[a][k] - x[a], x[a] - hi[a][k]), 0.0f);
+            s += d * d;
+        }
+        return s;
+    }
+};
+
+ZG_DEV bool isLeaf(int c) { return c < 0; }
+ZG_DEV int leafFirstD(int c) { return (~c) >> 4; }
+ZG_DEV int leafCountD(int c) { return ((~c) & 15) + 1; }
+
+// Traversal stack: the top kLds entries of a lane live in LDS (column tid of a
+// [kLds][blockDim] array, so lanes at the same depth hit consecutive banks); deeper
+// entries spill to a per-lane private array (scratch), used only by rare deep traversals.
+constexpr int kLdsStack = 16;
+constexpr int kSpillStack = 96;
+
+struct Stack {
+    uint2* lds;       // &smem[0][threadIdx.x]
+    int stride;       // blockDim.x
+    int sp;           // entries in LDS
+    int ovf;          // entries in the spill array
+    uint2 spill[kSpillStack];
+    bool overflow;
+    ZG_DEV void init(uint2* base, int s) { lds = base; stride = s; sp = 0; ovf = 0; overflow = false; }
+    ZG_DEV void push(int node, float d2) {
+        uint2 e = make_uint2((unsigned)node, __float_as_uint(d2));
+        if (sp < kLdsStack) { lds[sp * stride] = e; sp++; }
+        else if (ovf < kSpillStack) { spill[ovf++] = e; }
+        else overflow = true;
+    }
+    ZG_DEV bool pop(int& node, float& d2) {
+        uint2 e;
+        if (ovf > 0) e = spill[--ovf];
+        else if (sp > 0) { sp--; e = lds[sp * stride]; }
+        else return false;
+        node = (int)e.x; d2 = __uint_as_float(e.y);
+        return true;
+    }
+};
+
+struct VisitCount { uint32_t nodes, prims; };
+
+// Closest point on one boundary part: exact nearest primitive under the squared-distance
+// bound best2 (pass FLT_MAX for an unbounded search).  Leaves met in a node are tested at
+// once (tightening best2 early), the nearest inner child is entered next and the other
+// inner children still within best2 are pushed.  Returns false when nothing lies within
+// the bound.  (Role of fcpw Aggregate::findClosestPoint, fcpw_geometric_queries.h:1069-1096.)
+template <int D, int W, bool COUNT>
+ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, Vec<D>& cp, int& prim,
+                         Stack& st, VisitCount& vc) {
+    bool found = false;
+    if (b.nNodes == 0) return false;
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<D, W> nd;
+        nd.load(b.nodes, node);
+        if (COUNT) vc.nodes++;
+        float d2[W];
+#pragma unroll
+        for (int k = 0; k < W; k++) d2[k] = nd.dist2(k, x);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (isLeaf(nd.ch[k]) && d2[k] <= best2) {
+                int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
+                for (int i = first; i < first + cnt; i++) {
+                    Prim<D> pr; pr.load(b.prims, i);
+                    Vec<D> q;
+                    float dd = pr.closest2(x, q);
+                    if (COUNT) vc.prims++;
+                    if (dd < best2) { best2 = dd; cp = q; prim = i; found = true; }
+                }
+            }
+        }
+        int next = -1; float nextD = 0.0f; int nk = -1;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (!isLeaf(nd.ch[k]) && d2[k] <= best2 && (next < 0 || d2[k] < nextD)) { next = nd.ch[k]; nextD = d2[k]; nk = k; }
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (!isLeaf(nd.ch[k]) && d2[k] <= best2 && k != nk) st.push(nd.ch[k], d2[k]);
+        }
+        if (next < 0) {
+            int n; float dn;
+            bool got = false;
+            while (st.pop(n, dn)) { if (dn <= best2) { got = true; break; } }
+            if (!got) break;
+            next = n;
+        }
+        node = next;
+    }
+    return found;
+}
+
+// First hit of a ray with one boundary part, 0 <= t <= tMax.
+template <int D, int W>
+ZG_DEV bool intersectRay(const BoundaryView& b, const Vec<D>& o, const Vec<D>& dir, float tMax, float& tHit,
+                         int& prim, Stack& st) {
+    bool found = false;
+    tHit = tMax;
+    if (b.nNodes == 0) return false;
+    Vec<D> inv;
+#pragma unroll
+    for (int a = 0; a < D; a++) inv[a] = 1.0f / dir[a];
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<D, W> nd;
+        nd.load(b.nodes, node);
+        float tn[W];
+        bool hitk[W];
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float t0 = 0.0f, t1 = tHit;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                float ta = (nd.lo[a][k] - o[a]) * inv[a], tb = (nd.hi[a][k] - o[a]) * inv[a];
+                t0 = fmaxf(t0, fminf(ta, tb));
+                t1 = fminf(t1, fmaxf(ta, tb));
+            }
+            hitk[k] = t0 <= t1;
+            tn[k] = t0;
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (hitk[k] && isLeaf(nd.ch[k]) && tn[k] <= tHit) {
+                int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
+                for (int i = first; i < first + cnt; i++) {
+                    Prim<D> pr; pr.load(b.prims, i);
+                    float t;
+                    if (pr.ray(o, dir, tHit, t) && (!found || t < tHit)) { tHit = t; prim = i; found = true; }
+                }
+            }
+        }
+        int next = -1; float nextT = 0.0f; int nk = -1;
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            if (hitk[k] && !isLeaf(nd.ch[k]) && tn[k] <= tHit && (next < 0 || tn[k] < nextT)) { next = nd.ch[k]; nextT = tn[k]; nk = k; }
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            if (hitk[k] && !isLeaf(nd.ch[k]) && tn[k] <= tHit && k != nk) st.push(nd.ch[k], tn[k]);
+        if (next < 0) {
+            int n; float tnn;
+            bool got = false;
+            while (st.pop(n, tnn)) { if (tnn <= tHit) { got = true; break; } }
+            if (!got) break;
+            next = n;
+        }
+        node = next;
+    }
+    return found;
+}
+
+// Number of primitives crossed by the ray from x along +axis (all hits, t >= 0).
+template <int D, int W>
+ZG_DEV int countAxisHits(const BoundaryView& b, const Vec<D>& x, int axis, Stack& st) {
+    int hits = 0;
+    if (b.nNodes == 0) return 0;
+    Vec<D> dir = vzero<D>(); dir[axis] = 1.0f;
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<D, W> nd;
+        nd.load(b.nodes, node);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            bool ov = nd.hi[axis][k] >= x[axis] && nd.lo[axis][k] < 1e29f;
+#pragma unroll
+            for (int a = 0; a < D; a++)
+                if (a != axis) ov = ov && nd.lo[a][k] <= x[a] && x[a] <= nd.hi[a][k];
+            if (!ov) continue;
+            if (isLeaf(nd.ch[k])) {
+                int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
+                for (int i = first; i < first + cnt; i++) {
+                    Prim<D> pr; pr.load(b.prims, i);
+                    float t;
+                    if (pr.ray(x, dir, 3.0e38f, t)) hits++;
+                }
+            } else {
+                st.push(nd.ch[k], 0.0f);
+            }
+        }
+        int n; float dn;
+        if (!st.pop(n, dn)) break;
+        node = n;
+    }
+    return hits;
+}
+
+}  // namespace zg

@@ -1,0 +1,76 @@
+// kernels.h -- kernel parameter blocks and host launchers (implemented in *.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/zombie_amd.h"
+#include "layout.h"
+
+namespace zg {
+
+// counters[] slots of a solve
+enum CounterSlot {
+    kCntReached = 0, kCntRR = 1, kCntMaxLen = 2, kCntEscaped = 3,
+    kCntSteps = 4, kCntWalks = 5, kCntNodes = 6, kCntPrims = 7, kCntQueries = 8,
+    kCntClaims = 9, kCntError = 10, kCntSlots = 16
+};
+
+struct WosParams {
+    SceneView scene;
+    zg_pde pde;
+    zg_walk_settings ws;
+    const float* pt;
+    const float* normal;
+    const uint8_t* type;
+    const uint8_t* aligned;
+    const float* dist0;
+    const int32_t* nWalks;
+    int64_t nPoints;
+    int32_t chunksPerPoint;
+    int32_t wpl;
+    int32_t regroup;
+    int32_t pad;
+    unsigned long long nClaims;
+    uint64_t seed;
+    uint32_t streamOffset;
+    uint32_t pad2;
+    double* accSum;
+    double* accSq;
+    unsigned long long* accN;
+    unsigned long long* accLen;
+    unsigned long long* claimCounter;
+    unsigned long long* counters;
+};
+
+struct MergeParams {
+    int64_t n;
+    const float* pt;
+    const uint8_t* type;
+    const uint8_t* aligned;
+    const float* dist0;
+    const double* accSum;
+    const double* accSq;
+    const unsigned long long* accN;
+    const unsigned long long* accLen;
+    int64_t* sN;
+    double* sMean;
+    double* sM2;
+    int64_t* sLen;
+    float* firstSphereRadius;
+    zg_function dirichlet;
+    float absorption;
+    int32_t ignoreAbsorbing;
+    int32_t solveDoubleSided;
+    int32_t writeFirstSphereRadius;
+};
+
+hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s);
+hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, const float* pts, float* dist,
+                              float* closest, float* normal, int32_t* prim, hipStream_t s);
+hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);
+hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
+                     uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
+hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
+
+}  // namespace zg

@@ -1,0 +1,422 @@
+// wos_kernel.hip -- persistent Walk-on-Spheres kernel for gfx950, plus the batched query
+// kernels and the statistics merge.
+//
+// Work decomposition.  The walks of a solve are cut into claims: claim c covers walks
+// [k*wpl, (k+1)*wpl) of point p = c / chunksPerPoint (k = c % chunksPerPoint).  A grid of
+// persistent waves runs one walk per lane.  Every `regroup` steps (or as soon as every lane
+// of the wave is idle) the wave ballots its idle lanes and hands them the next claims from a
+// global counter with one atomic per wave (mbcnt gives each idle lane its slot), so the
+// lanes of a wave stay busy although walk lengths are heavy-tailed.  A lane accumulates the
+// FP64 sum and sum of squares of its claim's walk estimates in registers and flushes them
+// with FP64 atomics when the claim is done; a merge kernel then folds the per-point batch
+// into the caller's statistics with Chan's formula.
+//
+// Per walk the lane restates WalkOnSpheres::walk (point_estimation/walk_on_spheres.h:228-291)
+// with estimateSolution's per-point rules (:320-414) and getTerminalContribution (:293-318):
+//   ball update, source term (norm * f(y), y drawn from the ball Green's function),
+//   uniform direction, move, throughput (Yukawa dampening), escape test, Russian roulette,
+//   walk length / max length, Tikhonov switch, closest-point query of the next radius.
+// The closest point of the last query is kept: it bounds the next query (the new closest
+// distance is at most |x_new - y_prev|) and it is the projection used for the terminal
+// Dirichlet value when the walk enters the epsilon shell.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace zg {
+
+// ------------------------------------------------------------------------------------
+// batched queries
+// ------------------------------------------------------------------------------------
+template <int D, int W>
+__global__ void __launch_bounds__(256) closestPointKernel(BoundaryView b, int64_t n, const float* __restrict__ pts,
+                                                          float* __restrict__ dist, float* __restrict__ closest,
+                                                          float* __restrict__ normal, int32_t* __restrict__ prim) {
+    __shared__ uint2 stk[kLdsStack][256];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    VisitCount vc{0, 0};
+    Vec<D> x = vload<D>(pts + i * D);
+    float best2 = FLT_MAX;
+    Vec<D> cp = vzero<D>(); int pi = -1;
+    bool found = closestPoint<D, W, false>(b, x, best2, cp, pi, st, vc);
+    dist[i] = found ? __builtin_sqrtf(best2) : FLT_MAX;
+    if (closest) vstore<D>(closest + i * D, found ? cp : vzero<D>());
+    if (normal || prim) {
+        Vec<D> nn = vzero<D>(); int id = -1;
+        if (found) {
+            Prim<D> pr; pr.load(b.prims, pi);
+            nn = pr.normal();
+            if (D == 3) id = __float_as_int(b.prims[(size_t)pi * PrimLayout<D>::kFloats + 9]);
+            else id = b.primIds[pi];
+        }
+        if (normal) vstore<D>(normal + i * D, nn);
+        if (prim) prim[i] = id;
+    }
+}
+
+template <int D, int W>
+__global__ void __launch_bounds__(256) insideDomainKernel(SceneView s, int64_t n, const float* __restrict__ pts,
+                                                          uint8_t* __restrict__ inside) {
+    __shared__ uint2 stk[kLdsStack][256];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    Vec<D> x = vload<D>(pts + i * D);
+    bool in = true;
+    for (int a = 0; a < D; a++) {
+        int h = countAxisHits<D, W>(s.abs, x, a, st) + countAxisHits<D, W>(s.refl, x, a, st);
+        in = in && (h & 1);
+    }
+    inside[i] = in ? 1 : 0;
+}
+
+template <int D, int W>
+__global__ void __launch_bounds__(256) rayKernel(BoundaryView b, int64_t n, const float* __restrict__ o,
+                                                 const float* __restrict__ d, const float* __restrict__ tmax,
+                                                 uint8_t* __restrict__ hit, float* __restrict__ t, float* __restrict__ point,
+                                                 float* __restrict__ normal) {
+    __shared__ uint2 stk[kLdsStack][256];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    Vec<D> oo = vload<D>(o + i * D), dd = vload<D>(d + i * D);
+    float th; int pi = -1;
+    bool h = intersectRay<D, W>(b, oo, dd, tmax[i], th, pi, st);
+    hit[i] = h ? 1 : 0;
+    if (t) t[i] = h ? th : tmax[i];
+    if (point) vstore<D>(point + i * D, h ? oo + dd * th : vzero<D>());
+    if (normal) {
+        Vec<D> nn = vzero<D>();
+        if (h) { Prim<D> pr; pr.load(b.prims, pi); nn = pr.normal(); }
+        vstore<D>(normal + i * D, nn);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// walk-on-spheres
+// ------------------------------------------------------------------------------------
+ZG_DEV uint32_t mbcnt64(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int D>
+ZG_DEV bool outsideDomain(const SceneView& s, const Vec<D>& x) {
+    bool o = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) o = o || x[a] < s.dmin[a] || x[a] > s.dmax[a];
+    return o;
+}
+
+template <int D, int W, bool COUNT>
+__global__ void __launch_bounds__(256) wosKernel(WosParams P) {
+    __shared__ uint2 stk[kLdsStack][256];
+    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    VisitCount vc{0, 0};
+    const zg_walk_settings& ws = P.ws;
+    const float eps = ws.epsilon_shell_for_absorbing_boundary;
+    const float sigma = P.pde.absorption_coeff;
+    const int lane = __lane_id();
+
+    // claim
+    int mode = 0;          // 0 idle, 1 stepping, 2 waiting for the projection query
+    int pendingCode = 0;
+    int64_t p = 0;
+    int w = 0, wEnd = 0;
+    // walk
+    Vec<D> x = vzero<D>(), yc = vzero<D>();
+    int ycPrim = -1;
+    bool ycValid = false;
+    float dist = 0.0f, T = 1.0f, S = 0.0f;
+    int len = 0;
+    Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = __builtin_sqrtf(sigma);
+    Pcg32 rng; rng.state = 0; rng.inc = 1;
+    // per-claim FP64 accumulators
+    double aSum = 0.0, aSq = 0.0;
+    uint32_t aN = 0, aLen = 0;
+    // per-lane counters
+    uint64_t cnt[4] = {0, 0, 0, 0};
+    uint64_t steps = 0, walks = 0, queries = 0;
+
+    auto startWalk = [&]() {
+        x = vload<D>(P.pt + p * D);
+        dist = P.dist0[p];
+        T = 1.0f; S = 0.0f; len = 0;
+        ball.yukawa = sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
+        ycValid = false;
+        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(P.streamOffset + (uint32_t)w));
+        walks++;
+        if (dist > eps) { mode = 1; }
+        else { mode = 2; pendingCode = ZG_REACHED_ABSORBING_BOUNDARY; }
+    };
+
+    auto flushClaim = [&]() {
+        if (aN > 0) {
+            atomicAdd(&P.accSum[p], aSum);
+            atomicAdd(&P.accSq[p], aSq);
+            atomicAdd(&P.accN[p], (unsigned long long)aN);
+            atomicAdd(&P.accLen[p], (unsigned long long)aLen);
+        }
+        aSum = 0.0; aSq = 0.0; aN = 0; aLen = 0;
+    };
+
+    auto finishWalk = [&](int code) {
+        cnt[code]++;
+        if (code != ZG_ESCAPED_DOMAIN) {
+            float g = 0.0f;
+            if ((code == ZG_REACHED_ABSORBING_BOUNDARY || code == ZG_EXCEEDED_MAX_WALK_LENGTH) &&
+                !ws.ignore_absorbing_boundary_contribution) {
+                Vec<D> nrm = vzero<D>();
+                bool aligned = false;
+                if (ws.solve_double_sided && ycPrim >= 0) {
+                    Prim<D> pr; pr.load(P.scene.abs.prims, ycPrim);
+                    nrm = pr.normal();
+                    aligned = dot(x - yc, nrm) > 0.0f;
+                }
+                g = evalField<D>(P.pde.dirichlet, yc, nrm, aligned, sigma);
+            }
+            float c = T * g + S;
+            aSum += (double)c;
+            aSq += (double)c * (double)c;
+            aN++;
+            aLen += (uint32_t)len;
+            steps += (uint64_t)len;
+        }
+        w++;
+        if (w < wEnd) startWalk();
+        else { flushClaim(); mode = 0; }
+    };
+
+    bool workLeft = true;
+    int iter = 0;
+    for (;;) {
+        // -------- regroup: refill idle lanes with new claims --------
+        if (iter % P.regroup == 0 || !__any(mode != 0)) {
+            for (int round = 0; round < 4; round++) {
+                uint64_t idle = __ballot(mode == 0);
+                if (!workLeft || idle == 0) break;
+                uint32_t nIdle = (uint32_t)__popcll(idle);
+                int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(P.claimCounter, (unsigned long long)nIdle);
+                base = __shfl(base, leader);
+                if (base + nIdle >= P.nClaims) workLeft = false;
+                if (mode == 0) {
+                    unsigned long long c = base + mbcnt64(idle);
+                    if (c < P.nClaims) {
+                        p = (int64_t)(c / (unsigned long long)P.chunksPerPoint);
+                        int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
+                        int nw = P.nWalks[p];
+                        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+                        if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
+                        else if (P.dist0[p] <= eps) nw = min(nw, 1);
+                        w = k * P.wpl;
+                        wEnd = min(w + P.wpl, nw);
+                        if (w < wEnd) startWalk();
+                    }
+                }
+            }
+            if (!workLeft && !__any(mode != 0)) break;
+        }
+        iter++;
+
+        // -------- walk step (walk_on_spheres.h:237-285) --------
+        bool query = false;
+        if (mode == 1 && dist <= eps) finishWalk(ZG_REACHED_ABSORBING_BOUNDARY);
+        if (mode == 1) {
+            float R = dist;
+            Vec<D> c = x;
+            ball.update(R);
+            if (!ws.ignore_source_contribution) {
+                float f;
+                if (P.pde.is_source_constant) {
+                    f = evalField<D>(P.pde.source, x, vzero<D>(), false, sigma);
+                } else {
+                    Vec<D> sdir = sampleUnitSphere<D>(rng);
+                    float r = ball.sampleRadius(rng);
+                    f = evalField<D>(P.pde.source, c + sdir * r, vzero<D>(), false, sigma);
+                }
+                S += T * ball.norm() * f;
+            }
+            Vec<D> dir = sampleUnitSphere<D>(rng);
+            x = c + dir * R;
+            if (ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
+            int code = -1;
+            if (outsideDomain<D>(P.scene, x)) {
+                code = ZG_ESCAPED_DOMAIN;
+            } else {
+                if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
+                    float survival = T / ws.russian_roulette_threshold;
+                    if (survival < rng.nextFloat()) { T = 0.0f; code = ZG_TERMINATED_WITH_RUSSIAN_ROULETTE; }
+                    else T = ws.russian_roulette_threshold;
+                }
+                if (code < 0) {
+                    len++;
+                    if (len >= ws.max_walk_length) {
+                        code = ZG_EXCEEDED_MAX_WALK_LENGTH;
+                    } else if (sigma > 0.0f && ws.steps_before_applying_tikhonov == len) {
+                        ball.yukawa = true;
+                    }
+                }
+            }
+            if (code == ZG_EXCEEDED_MAX_WALK_LENGTH && !ws.ignore_absorbing_boundary_contribution) {
+                mode = 2; pendingCode = code;      // terminal needs the projection of x
+            } else if (code >= 0) {
+                finishWalk(code);
+            } else {
+                query = true;
+            }
+        }
+        // -------- closest-point query: next radius, or the terminal projection --------
+        if (query || mode == 2) {
+            float best2 = FLT_MAX;
+            if (ycValid) {
+                best2 = norm2(x - yc) * 1.0001f + 1e-30f;
+            } else if (mode == 1) {
+                float r2 = 2.0f * dist;
+                best2 = r2 * r2 * 1.0001f + 1e-30f;
+            }
+            Vec<D> q = yc; int qp = -1;
+            bool found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc);
+            if (!found) {
+                best2 = FLT_MAX;
+                found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc);
+            }
+            queries++;
+            if (found) { yc = q; ycPrim = qp; ycValid = true; dist = __builtin_sqrtf(best2); }
+            else { dist = FLT_MAX; ycValid = false; }
+            if (mode == 2) finishWalk(pendingCode);
+        }
+    }
+
+    // -------- per-lane counters --------
+    for (int k = 0; k < 4; k++) if (cnt[k]) atomicAdd(&P.counters[k], (unsigned long long)cnt[k]);
+    if (steps) atomicAdd(&P.counters[4], (unsigned long long)steps);
+    if (walks) atomicAdd(&P.counters[5], (unsigned long long)walks);
+    if (COUNT) {
+        atomicAdd(&P.counters[6], (unsigned long long)vc.nodes);
+        atomicAdd(&P.counters[7], (unsigned long long)vc.prims);
+        atomicAdd(&P.counters[8], (unsigned long long)queries);
+    }
+    if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[10]), 1u);
+}
+
+// per-point batch -> caller statistics (Chan merge, common.h:268-310); sample points on the
+// absorbing boundary record the boundary value once (walk_on_spheres.h:329-346)
+template <int D>
+__global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.n) return;
+    int64_t N1 = M.sN[i];
+    bool hasPrev = N1 > 0;
+    uint8_t ty = M.type ? M.type[i] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+    double sum = M.accSum[i], sq = M.accSq[i];
+    int64_t nb = (int64_t)M.accN[i];
+    if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
+        if (!hasPrev) {
+            float g = 0.0f;
+            if (!M.ignoreAbsorbing) {
+                Vec<D> x = vload<D>(M.pt + i * D);
+                g = evalField<D>(M.dirichlet, x, vzero<D>(), M.solveDoubleSided && M.aligned && M.aligned[i], M.absorption);
+            }
+            sum = g; sq = (double)g * g; nb = 1;
+        }
+        if (M.firstSphereRadius && !hasPrev) M.firstSphereRadius[i] = 0.0f;
+    } else if (M.firstSphereRadius && !hasPrev && M.writeFirstSphereRadius) {
+        M.firstSphereRadius[i] = M.dist0[i];
+    }
+    if (nb > 0) {
+        double meanB = sum / (double)nb;
+        double m2B = fmax(0.0, sq - sum * meanB);
+        if (N1 == 0) {
+            M.sMean[i] = meanB; M.sM2[i] = m2B;
+        } else {
+            double N = (double)(N1 + nb);
+            double delta = meanB - M.sMean[i];
+            M.sMean[i] = M.sMean[i] + delta * ((double)nb / N);
+            M.sM2[i] = M.sM2[i] + m2B + delta * delta * ((double)N1 * (double)nb / N);
+        }
+        M.sN[i] = N1 + nb;
+    }
+    if (M.sLen) M.sLen[i] += (int64_t)M.accLen[i];
+}
+
+__global__ void fillZeroKernel(uint64_t* __restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+
+// ------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------
+static inline unsigned blocksFor(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s) {
+    if (nWords <= 0) return hipSuccess;
+    unsigned g = (unsigned)std::min<int64_t>(4096, (nWords + 255) / 256);
+    hipLaunchKernelGGL(fillZeroKernel, dim3(g), dim3(256), 0, s, p, nWords);
+    return hipGetLastError();
+}
+
+hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, const float* pts, float* dist,
+                              float* closest, float* normal, int32_t* prim, hipStream_t s) {
+    const BoundaryView& b = boundary ? sv.refl : sv.abs;
+    if (n <= 0) return hipSuccess;
+    dim3 g(blocksFor(n)), bl(256);
+    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((closestPointKernel<2, 4>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
+    else if (sv.dim == 2) hipLaunchKernelGGL((closestPointKernel<2, 8>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
+    else if (sv.width == 4) hipLaunchKernelGGL((closestPointKernel<3, 4>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
+    else hipLaunchKernelGGL((closestPointKernel<3, 8>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
+    return hipGetLastError();
+}
+
+hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, uint8_t* inside, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 g(blocksFor(n)), bl(256);
+    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((insideDomainKernel<2, 4>), g, bl, 0, s, sv, n, pts, inside);
+    else if (sv.dim == 2) hipLaunchKernelGGL((insideDomainKernel<2, 8>), g, bl, 0, s, sv, n, pts, inside);
+    else if (sv.width == 4) hipLaunchKernelGGL((insideDomainKernel<3, 4>), g, bl, 0, s, sv, n, pts, inside);
+    else hipLaunchKernelGGL((insideDomainKernel<3, 8>), g, bl, 0, s, sv, n, pts, inside);
+    return hipGetLastError();
+}
+
+hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
+                     uint8_t* hit, float* t, float* point, float* normal, hipStream_t s) {
+    const BoundaryView& b = boundary ? sv.refl : sv.abs;
+    if (n <= 0) return hipSuccess;
+    dim3 g(blocksFor(n)), bl(256);
+    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((rayKernel<2, 4>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
+    else if (sv.dim == 2) hipLaunchKernelGGL((rayKernel<2, 8>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
+    else if (sv.width == 4) hipLaunchKernelGGL((rayKernel<3, 4>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
+    else hipLaunchKernelGGL((rayKernel<3, 8>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
+    return hipGetLastError();
+}
+
+hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s) {
+    dim3 g(gridBlocks), bl(256);
+#define ZG_WOS(D, Wd, C) hipLaunchKernelGGL((wosKernel<D, Wd, C>), g, bl, 0, s, P)
+    if (dim == 2) {
+        if (width == 4) { if (count) ZG_WOS(2, 4, true); else ZG_WOS(2, 4, false); }
+        else { if (count) ZG_WOS(2, 8, true); else ZG_WOS(2, 8, false); }
+    } else {
+        if (width == 4) { if (count) ZG_WOS(3, 4, true); else ZG_WOS(3, 4, false); }
+        else { if (count) ZG_WOS(3, 8, true); else ZG_WOS(3, 8, false); }
+    }
+#undef ZG_WOS
+    return hipGetLastError();
+}
+
+hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s) {
+    if (M.n <= 0) return hipSuccess;
+    dim3 g(blocksFor(M.n)), bl(256);
+    if (dim == 2) hipLaunchKernelGGL(mergeStatsKernel<2>, g, bl, 0, s, M);
+    else hipLaunchKernelGGL(mergeStatsKernel<3>, g, bl, 0, s, M);
+    return hipGetLastError();
+}
+
+}  // namespace zg

@@ -1,0 +1,294 @@
+"""Array-level host API over the C-ABI (device arrays are torch tensors on the GPU).
+
+PyTorch is used only for device memory and streams; every computation runs in
+libzombie_amd.so.  This layer is what the reference-shaped API in zombie_amd.api and the
+benchmark build on.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+
+def _torch():
+    import torch  # deferred: importing torch is slow and the CPU tests do not need it
+    return torch
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _stream_ptr(stream=None) -> int | None:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+# ------------------------------------------------------------------------------------
+# descriptors
+# ------------------------------------------------------------------------------------
+def function(kind: int, eval_: int = A.ZG_EVAL_VALUE, params=()) -> A.zg_function:
+    f = A.zg_function()
+    f.kind = kind
+    f.eval = eval_
+    for i, v in enumerate(params):
+        f.params[i] = float(v)
+    return f
+
+
+def constant(value: float) -> A.zg_function:
+    return function(A.ZG_FN_CONSTANT, A.ZG_EVAL_VALUE, (value,))
+
+
+def zero() -> A.zg_function:
+    return function(A.ZG_FN_ZERO)
+
+
+def make_pde(source=None, dirichlet=None, robin=None, absorption: float = 0.0,
+             is_source_constant: bool = False) -> A.zg_pde:
+    p = A.zg_pde()
+    p.absorption_coeff = float(absorption)
+    p.is_source_constant = 1 if is_source_constant else 0
+    p.are_robin_conditions_pure_neumann = 1
+    p.are_robin_coeffs_nonnegative = 1
+    p.source = source if source is not None else zero()
+    p.dirichlet = dirichlet if dirichlet is not None else zero()
+    p.robin = robin if robin is not None else zero()
+    p.robin_coeff = zero()
+    return p
+
+
+def walk_settings(eps_absorbing: float, eps_reflecting: float, max_walk_length: int,
+                  solve_double_sided: bool = False, **overrides) -> A.zg_walk_settings:
+    ws = A.zg_walk_settings()
+    A.lib().zg_walk_settings_default(C.byref(ws), eps_absorbing, eps_reflecting, max_walk_length,
+                                     1 if solve_double_sided else 0)
+    for k, v in overrides.items():
+        setattr(ws, k, v)
+    return ws
+
+
+def walk_settings_host(eps_absorbing: float, eps_reflecting: float, max_walk_length: int,
+                       solve_double_sided: bool = False, **overrides) -> A.zg_walk_settings:
+    """Same defaults as zg_walk_settings_default, without loading the library."""
+    ws = A.zg_walk_settings()
+    ws.epsilon_shell_for_absorbing_boundary = eps_absorbing
+    ws.epsilon_shell_for_reflecting_boundary = eps_reflecting
+    ws.silhouette_precision = 1e-3
+    ws.russian_roulette_threshold = 0.0
+    ws.splitting_threshold = A.FLT_MAX
+    ws.max_walk_length = max_walk_length
+    ws.steps_before_applying_tikhonov = 0
+    ws.steps_before_using_maximal_spheres = max_walk_length
+    ws.solve_double_sided = 1 if solve_double_sided else 0
+    ws.use_gradient_control_variates = 1
+    ws.use_gradient_antithetic_variates = 1
+    for k, v in overrides.items():
+        setattr(ws, k, v)
+    return ws
+
+
+@dataclass
+class SolveOptions:
+    seed: int = 0x5EED
+    stream_offset: int = 0
+    walks_per_lane: int = 8
+    regroup_interval: int = 16
+    waves_per_cu: int = 0
+    count_visits: bool = False
+
+    def to_c(self) -> A.zg_solve_options:
+        o = A.zg_solve_options()
+        o.seed = self.seed
+        o.stream_offset = self.stream_offset
+        o.walks_per_lane = self.walks_per_lane
+        o.regroup_interval = self.regroup_interval
+        o.waves_per_cu = self.waves_per_cu
+        o.count_visits = 1 if self.count_visits else 0
+        return o
+
+
+@dataclass
+class SolveReport:
+    completion_codes: tuple
+    total_steps: int
+    total_walks: int
+    node_visits: int
+    prim_visits: int
+    queries: int
+    kernel_ms: float
+
+    @staticmethod
+    def from_c(r: A.zg_solve_report) -> "SolveReport":
+        return SolveReport(tuple(r.completion_codes), r.total_steps, r.total_walks, r.node_visits,
+                           r.prim_visits, r.queries, r.kernel_ms)
+
+
+# ------------------------------------------------------------------------------------
+# scene
+# ------------------------------------------------------------------------------------
+class Scene:
+    """Device scene: packed wide BVHs of the absorbing and reflecting boundary parts."""
+
+    def __init__(self, positions, indices, is_reflecting=None, domain_min=None, domain_max=None,
+                 bvh_width: int = 8, leaf_size: int = 4, sah_bins: int = 16,
+                 solve_double_sided: bool = False, silhouette_precision: float = 1e-3):
+        V = np.ascontiguousarray(positions, dtype=np.float32)
+        I = np.ascontiguousarray(indices, dtype=np.int32)
+        self.dim = V.shape[1]
+        if I.ndim != 2 or I.shape[1] != self.dim:
+            raise ValueError("indices must be [n_prims, dim]")
+        refl = None if is_reflecting is None else np.ascontiguousarray(is_reflecting, dtype=np.uint8)
+        dmin = np.ascontiguousarray(domain_min if domain_min is not None else V.min(0) - 1.0, dtype=np.float32)
+        dmax = np.ascontiguousarray(domain_max if domain_max is not None else V.max(0) + 1.0, dtype=np.float32)
+        opts = A.zg_build_options(bvh_width, leaf_size, sah_bins, 0)
+        h = C.c_void_p()
+        A.check(A.lib().zg_scene_create(
+            self.dim, V.ctypes.data, V.shape[0], I.ctypes.data, I.shape[0],
+            None if refl is None else refl.ctypes.data, dmin.ctypes.data, dmax.ctypes.data,
+            1 if solve_double_sided else 0, silhouette_precision, C.byref(opts), C.byref(h)))
+        self._h = h
+        self.domain_min, self.domain_max = dmin, dmax
+        info = A.zg_scene_info()
+        A.check(A.lib().zg_scene_get_info(self._h, C.byref(info)))
+        self.info = info
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            A.lib().zg_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # batched queries (device tensors in, device tensors out)
+    def closest_point(self, pts, boundary: int = 0, want_point=True, want_normal=False, want_prim=False):
+        torch = _torch()
+        n = pts.shape[0]
+        dev = pts.device
+        dist = torch.empty(n, dtype=torch.float32, device=dev)
+        cp = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if want_point else None
+        nr = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if want_normal else None
+        pr = torch.empty(n, dtype=torch.int32, device=dev) if want_prim else None
+        A.check(A.lib().zg_closest_point(self._h, boundary, n, _ptr(pts), _ptr(dist), _ptr(cp), _ptr(nr), _ptr(pr),
+                                         _stream_ptr()))
+        return dist, cp, nr, pr
+
+    def inside_domain(self, pts):
+        torch = _torch()
+        out = torch.empty(pts.shape[0], dtype=torch.uint8, device=pts.device)
+        A.check(A.lib().zg_inside_domain(self._h, pts.shape[0], _ptr(pts), _ptr(out), _stream_ptr()))
+        return out
+
+    def intersect_ray(self, origins, dirs, t_max, boundary: int = 0):
+        torch = _torch()
+        n = origins.shape[0]
+        dev = origins.device
+        hit = torch.empty(n, dtype=torch.uint8, device=dev)
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        p = torch.empty((n, self.dim), dtype=torch.float32, device=dev)
+        nr = torch.empty((n, self.dim), dtype=torch.float32, device=dev)
+        A.check(A.lib().zg_intersect_ray(self._h, boundary, n, _ptr(origins), _ptr(dirs), _ptr(t_max), _ptr(hit),
+                                         _ptr(t), _ptr(p), _ptr(nr), _stream_ptr()))
+        return hit, t, p, nr
+
+
+# ------------------------------------------------------------------------------------
+# points + statistics on the device
+# ------------------------------------------------------------------------------------
+@dataclass
+class DevicePoints:
+    pt: object
+    dist_to_absorbing: object
+    n_walks: object
+    n_walks_max: int
+    type: object = None
+    normal: object = None
+    aligned: object = None
+    dist_to_reflecting: object = None
+    first_sphere_radius: object = None
+
+    @property
+    def n(self) -> int:
+        return int(self.pt.shape[0])
+
+    def to_c(self) -> A.zg_sample_points:
+        s = A.zg_sample_points()
+        s.n = self.n
+        s.pt = _ptr(self.pt)
+        s.normal = _ptr(self.normal)
+        s.type = _ptr(self.type)
+        s.dist_to_absorbing_boundary = _ptr(self.dist_to_absorbing)
+        s.dist_to_reflecting_boundary = _ptr(self.dist_to_reflecting)
+        s.estimate_boundary_normal_aligned = _ptr(self.aligned)
+        s.first_sphere_radius = _ptr(self.first_sphere_radius)
+        s.n_walks = _ptr(self.n_walks)
+        s.n_walks_max = int(self.n_walks_max)
+        return s
+
+
+@dataclass
+class DeviceStats:
+    n: object
+    mean: object
+    m2: object
+    walk_length: object
+    splits: object
+
+    @staticmethod
+    def zeros(count: int, device="cuda") -> "DeviceStats":
+        torch = _torch()
+        return DeviceStats(torch.zeros(count, dtype=torch.int64, device=device),
+                           torch.zeros(count, dtype=torch.float64, device=device),
+                           torch.zeros(count, dtype=torch.float64, device=device),
+                           torch.zeros(count, dtype=torch.int64, device=device),
+                           torch.zeros(count, dtype=torch.int64, device=device))
+
+    def to_c(self) -> A.zg_statistics:
+        s = A.zg_statistics()
+        s.n_solution_estimates = _ptr(self.n)
+        s.solution_mean = _ptr(self.mean)
+        s.solution_m2 = _ptr(self.m2)
+        s.total_walk_length = _ptr(self.walk_length)
+        s.total_splits = _ptr(self.splits)
+        return s
+
+    def variance(self):
+        torch = _torch()
+        return self.m2 / torch.clamp(self.n - 1, min=1).to(torch.float64)
+
+
+def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
+              opts: SolveOptions | None = None, stream=None) -> SolveReport:
+    opts = opts or SolveOptions()
+    sp = points.to_c()
+    st = stats.to_c()
+    oc = opts.to_c()
+    rep = A.zg_solve_report()
+    A.check(A.lib().zg_wos_solve(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st), C.byref(oc),
+                                 C.byref(rep), _stream_ptr(stream)))
+    return SolveReport.from_c(rep)
+
+
+def wost_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
+               opts: SolveOptions | None = None, stream=None) -> SolveReport:
+    opts = opts or SolveOptions()
+    sp = points.to_c()
+    st = stats.to_c()
+    oc = opts.to_c()
+    rep = A.zg_solve_report()
+    A.check(A.lib().zg_wost_solve(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st), C.byref(oc),
+                                  C.byref(rep), _stream_ptr(stream)))
+    return SolveReport.from_c(rep)

@@ -1,0 +1,323 @@
+"""Manufactured benchmark problems (host side, FP64 ground truth).
+
+Each builder returns a Problem with the boundary mesh, the bounding domain, the PDE
+descriptor (zg_pde), walk settings and the FP64 analytic solution.  Sizes follow the five
+benchmark configurations:
+
+  star2d     2D Laplace on r(t) = 1 + 0.3 cos 5t (1024 segments), g = e^x cos y, 256^2 grid
+  sphere3d   3D Poisson on a spherical-harmonic displaced icosphere (81,920 triangles),
+             u = sin 2x sin 2y sin 2z, f = 12 u, 1M interior points
+  mixed2d    2D Laplace, 4096-segment star curve + 32 interior slits, Neumann on the slits
+             and on the arc t in [0, pi], u = x^3 - 3 x y^2
+  screened   screened Poisson instances u = exp(a.x), |a|^2 = sigma, on geodesic spheres
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+from . import engine as E
+
+
+@dataclass
+class Problem:
+    name: str
+    dim: int
+    positions: np.ndarray          # [nV, dim] float32
+    indices: np.ndarray            # [nP, dim] int32
+    is_reflecting: np.ndarray | None
+    domain_min: np.ndarray
+    domain_max: np.ndarray
+    pde: A.zg_pde
+    settings: A.zg_walk_settings
+    n_walks: int
+    exact: callable                # FP64 analytic u(x), x [n, dim]
+    solve_double_sided: bool = False
+    meta: dict = field(default_factory=dict)
+
+
+# ------------------------------------------------------------------------------------
+# geometry helpers
+# ------------------------------------------------------------------------------------
+def star_curve(n_segments: int, a: float = 0.3, k: int = 5):
+    """Closed CCW polyline r(t) = 1 + a cos(k t); outward normals (s.y, -s.x)."""
+    t = 2.0 * np.pi * np.arange(n_segments) / n_segments
+    r = 1.0 + a * np.cos(k * t)
+    V = np.stack([r * np.cos(t), r * np.sin(t)], 1)
+    I = np.stack([np.arange(n_segments), (np.arange(n_segments) + 1) % n_segments], 1)
+    return V.astype(np.float32), I.astype(np.int32)
+
+
+def square_bounding_box(V: np.ndarray):
+    """computeBoundingBox(positions, makeSquare=true, scale=1) (fcpw_geometric_queries.h:291-317)."""
+    lo, hi = V.min(0).astype(np.float32), V.max(0).astype(np.float32)
+    c = 0.5 * (lo + hi)
+    half = 0.5 * float((hi - lo).max())
+    return (c - half).astype(np.float32), (c + half).astype(np.float32)
+
+
+def points_in_polygon(P: np.ndarray, V: np.ndarray, I: np.ndarray) -> np.ndarray:
+    """Even-odd rule against the segments (V[I[:,0]], V[I[:,1]]), vectorised."""
+    P = np.asarray(P, np.float64)
+    a = V[I[:, 0]].astype(np.float64)
+    b = V[I[:, 1]].astype(np.float64)
+    inside = np.zeros(len(P), bool)
+    chunk = 4096
+    for s in range(0, len(P), chunk):
+        x = P[s:s + chunk, 0:1]
+        y = P[s:s + chunk, 1:2]
+        ay, by = a[None, :, 1], b[None, :, 1]
+        ax, bx = a[None, :, 0], b[None, :, 0]
+        cond = (ay > y) != (by > y)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xi = ax + (y - ay) * (bx - ax) / (by - ay)
+        cross = cond & (x < xi)
+        inside[s:s + chunk] = (cross.sum(1) % 2) == 1
+    return inside
+
+
+def icosahedron():
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    return V, F
+
+
+def icosphere(level: int):
+    """Unit icosphere with 20*4^level outward-oriented triangles."""
+    V, F = icosahedron()
+    verts = [v for v in V]
+    for _ in range(level):
+        cache = {}
+        newF = []
+
+        def mid(i, j):
+            key = (i, j) if i < j else (j, i)
+            m = cache.get(key)
+            if m is None:
+                p = verts[i] + verts[j]
+                p = p / np.linalg.norm(p)
+                verts.append(p)
+                m = len(verts) - 1
+                cache[key] = m
+            return m
+
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            newF += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        F = np.array(newF, np.int64)
+    return np.array(verts), F
+
+
+def geodesic_sphere(freq: int):
+    """Unit geodesic sphere of frequency `freq` (20*freq^2 triangles), welded vertices."""
+    V0, F0 = icosahedron()
+    keys = {}
+    verts = []
+
+    def vid(p):
+        p = p / np.linalg.norm(p)
+        k = tuple(np.round(p * 1e9).astype(np.int64))
+        i = keys.get(k)
+        if i is None:
+            i = len(verts)
+            keys[k] = i
+            verts.append(p)
+        return i
+
+    F = []
+    n = freq
+    for a, b, c in F0:
+        A0, B0, C0 = V0[a], V0[b], V0[c]
+        grid = {}
+        for i in range(n + 1):
+            for j in range(n + 1 - i):
+                p = A0 + (B0 - A0) * (i / n) + (C0 - A0) * (j / n)
+                grid[(i, j)] = vid(p)
+        for i in range(n):
+            for j in range(n - i):
+                F.append([grid[(i, j)], grid[(i + 1, j)], grid[(i, j + 1)]])
+                if j + i + 1 < n:
+                    F.append([grid[(i + 1, j)], grid[(i + 1, j + 1)], grid[(i, j + 1)]])
+    return np.array(verts), np.array(F, np.int64)
+
+
+def real_sph_harm_basis(dirs: np.ndarray, lmax: int) -> np.ndarray:
+    """Real orthonormal spherical harmonics Y_lm(dirs) for l <= lmax, columns ordered (l, m)."""
+    from scipy.special import lpmv
+    x, y, z = dirs[:, 0], dirs[:, 1], dirs[:, 2]
+    theta = np.arccos(np.clip(z, -1.0, 1.0))
+    phi = np.arctan2(y, x)
+    ct = np.cos(theta)
+    cols = []
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            am = abs(m)
+            norm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - am) / math.factorial(l + am))
+            P = lpmv(am, l, ct)
+            if m == 0:
+                cols.append(norm * P)
+            elif m > 0:
+                cols.append(math.sqrt(2.0) * norm * P * np.cos(am * phi))
+            else:
+                cols.append(math.sqrt(2.0) * norm * P * np.sin(am * phi))
+    return np.stack(cols, 1)
+
+
+def sh_displace(V: np.ndarray, seed: int, amplitude: float = 0.15, lmax: int = 4) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    coeffs = []
+    for l in range(lmax + 1):
+        coeffs += list(rng.standard_normal(2 * l + 1) / (l + 1))
+    Y = real_sph_harm_basis(V, lmax)
+    r = 1.0 + amplitude * (Y @ np.array(coeffs))
+    if r.min() <= 0.05:
+        raise ValueError("displacement makes the surface non star-shaped")
+    return V * r[:, None]
+
+
+# ------------------------------------------------------------------------------------
+# analytic fields (FP64)
+# ------------------------------------------------------------------------------------
+def exp_cos(x):
+    return np.exp(x[:, 0]) * np.cos(x[:, 1])
+
+
+def sin_product(x, k=2.0):
+    return np.prod(np.sin(k * np.asarray(x, np.float64)), axis=1)
+
+
+def cubic_harmonic(x):
+    x = np.asarray(x, np.float64)
+    return x[:, 0] ** 3 - 3.0 * x[:, 0] * x[:, 1] ** 2
+
+
+def exp_linear(x, a):
+    return np.exp(np.asarray(x, np.float64) @ np.asarray(a, np.float64))
+
+
+# ------------------------------------------------------------------------------------
+# problems
+# ------------------------------------------------------------------------------------
+def star2d(n_segments=1024, grid_res=256, n_walks=512, eps=1e-3, max_len=1024) -> Problem:
+    V, I = star_curve(n_segments)
+    lo, hi = square_bounding_box(V)
+    g = E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_VALUE, (1.0, 1.0))
+    pde = E.make_pde(source=E.zero(), dirichlet=g, is_source_constant=True)
+    ws = E.walk_settings_host(eps, eps, max_len, False)
+    prob = Problem("star2d", 2, V, I, None, lo, hi, pde, ws, n_walks, exact_fn(exp_cos))
+    ext = hi - lo
+    ii, jj = np.meshgrid(np.arange(grid_res), np.arange(grid_res), indexing="ij")
+    gx = (ii.ravel() / np.float32(grid_res)).astype(np.float32) * ext[0] + lo[0]
+    gy = (jj.ravel() / np.float32(grid_res)).astype(np.float32) * ext[1] + lo[1]
+    G = np.stack([gx, gy], 1).astype(np.float32)
+    inside = points_in_polygon(G, V, I)
+    prob.meta["grid"] = G
+    prob.meta["points"] = G[inside]
+    prob.meta["grid_inside"] = inside
+    return prob
+
+
+def exact_fn(f, **kw):
+    def u(x):
+        return f(np.asarray(x, np.float64), **kw)
+    return u
+
+
+def sphere3d(level=6, seed=0x5EED, n_walks=256, eps=1e-4, max_len=1024, k=2.0) -> Problem:
+    V, F = icosphere(level)
+    V = sh_displace(V, seed)
+    lo, hi = V.min(0) - 1e-3, V.max(0) + 1e-3
+    u = E.function(A.ZG_FN_SIN_PRODUCT, A.ZG_EVAL_VALUE, (1.0, k))
+    f = E.function(A.ZG_FN_SIN_PRODUCT, A.ZG_EVAL_SCREENED_SOURCE, (1.0, k))
+    pde = E.make_pde(source=f, dirichlet=u, is_source_constant=False)
+    ws = E.walk_settings_host(eps, eps, max_len, False)
+    return Problem("sphere3d", 3, V.astype(np.float32), F.astype(np.int32), None,
+                   lo.astype(np.float32), hi.astype(np.float32), pde, ws, n_walks, exact_fn(sin_product, k=k))
+
+
+def mixed2d(n_segments=4096, n_slits=32, seed=0x5EED, n_walks=1024, eps=1e-3, max_len=1024) -> Problem:
+    V, I = star_curve(n_segments)
+    rng = np.random.default_rng(seed)
+    slits = []
+    tries = 0
+    while len(slits) < n_slits and tries < 100000:
+        tries += 1
+        c = rng.uniform(-0.6, 0.6, 2)
+        L = rng.uniform(0.05, 0.2)
+        ang = rng.uniform(0, np.pi)
+        d = 0.5 * L * np.array([np.cos(ang), np.sin(ang)])
+        p0, p1 = c - d, c + d
+        ok = True
+        for p in (p0, p1):
+            t = math.atan2(p[1], p[0])
+            if np.hypot(*p) > 0.9 * (1.0 + 0.3 * math.cos(5 * t)) - 0.05:
+                ok = False
+        for q0, q1 in slits:
+            if _segments_close(p0, p1, q0, q1, 0.02):
+                ok = False
+        if ok:
+            slits.append((p0, p1))
+    nv = len(V)
+    SV = np.array([p for s in slits for p in s], np.float32)
+    SI = np.arange(2 * len(slits), dtype=np.int32).reshape(-1, 2) + nv
+    mid = 0.5 * (V[I[:, 0]] + V[I[:, 1]])
+    theta = np.mod(np.arctan2(mid[:, 1], mid[:, 0]), 2 * np.pi)
+    refl_outer = theta <= np.pi
+    Vall = np.concatenate([V, SV]).astype(np.float32)
+    Iall = np.concatenate([I, SI]).astype(np.int32)
+    refl = np.concatenate([refl_outer, np.ones(len(SI), bool)]).astype(np.uint8)
+    lo, hi = square_bounding_box(V)
+    u = E.function(A.ZG_FN_CUBIC_HARMONIC, A.ZG_EVAL_VALUE, (1.0,))
+    h = E.function(A.ZG_FN_CUBIC_HARMONIC, A.ZG_EVAL_NORMAL_DERIVATIVE, (1.0,))
+    pde = E.make_pde(source=E.zero(), dirichlet=u, robin=h, is_source_constant=True)
+    ws = E.walk_settings_host(eps, eps, max_len, True)
+    prob = Problem("mixed2d", 2, Vall, Iall, refl, lo, hi, pde, ws, n_walks, exact_fn(cubic_harmonic),
+                   solve_double_sided=True)
+    prob.meta["outer"] = (V, I)
+    prob.meta["slits"] = slits
+    return prob
+
+
+def _segments_close(p0, p1, q0, q1, tol):
+    def seg_dist(a, b, c):
+        ab = b - a
+        t = np.clip(np.dot(c - a, ab) / max(np.dot(ab, ab), 1e-30), 0, 1)
+        return np.linalg.norm(a + t * ab - c)
+    d = min(seg_dist(p0, p1, q0), seg_dist(p0, p1, q1), seg_dist(q0, q1, p0), seg_dist(q0, q1, p1))
+    # proper intersection test
+    def orient(a, b, c):
+        return np.sign((b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]))
+    inter = orient(p0, p1, q0) != orient(p0, p1, q1) and orient(q0, q1, p0) != orient(q0, q1, p1)
+    return inter or d < tol
+
+
+def screened_instance(n_tris: int, sigma: float, direction, seed: int, n_walks=128, eps=1e-4,
+                      max_len=1024) -> Problem:
+    freq = max(1, int(round(math.sqrt(n_tris / 20.0))))
+    V, F = geodesic_sphere(freq)
+    V = sh_displace(V, seed)
+    lo, hi = V.min(0) - 1e-3, V.max(0) + 1e-3
+    a = math.sqrt(sigma) * np.asarray(direction, np.float64) / np.linalg.norm(direction)
+    u = E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, a[0], a[1], a[2]))
+    pde = E.make_pde(source=E.zero(), dirichlet=u, absorption=sigma, is_source_constant=True)
+    ws = E.walk_settings_host(eps, eps, max_len, False)
+    prob = Problem("screened", 3, V.astype(np.float32), F.astype(np.int32), None,
+                   lo.astype(np.float32), hi.astype(np.float32), pde, ws, n_walks, exact_fn(exp_linear, a=a))
+    prob.meta["sigma"] = sigma
+    prob.meta["a"] = a
+    return prob
+
+
+def sample_box(lo, hi, n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, size=(n, len(lo))).astype(np.float32)
