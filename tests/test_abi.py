@@ -1,0 +1,70 @@
+"""The C-ABI: the library loads without a GPU, exports every symbol include/zombie_amd.h
+declares, and the ctypes structures match the C layouts (checked with gcc)."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from zombie_amd import _abi as A
+
+REPO = Path(__file__).resolve().parents[1]
+HEADER = REPO / "include" / "zombie_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:zg_status|void|int32_t|const char\*)\s+(zg_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = A.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in A.SIGNATURES, f"{n} missing from the ctypes signature table"
+    assert lib.zg_version().decode() == "0.1.0"
+
+
+def test_defaults_match_walk_settings_constructor():
+    ws = A.zg_walk_settings()
+    A.lib().zg_walk_settings_default(C.byref(ws), 1e-3, 2e-3, 512, 1)
+    assert ws.silhouette_precision == pytest.approx(1e-3)
+    assert ws.russian_roulette_threshold == 0.0
+    assert ws.splitting_threshold == pytest.approx(A.FLT_MAX)
+    assert ws.steps_before_using_maximal_spheres == 512 and ws.steps_before_applying_tikhonov == 0
+    assert ws.solve_double_sided == 1 and ws.use_gradient_control_variates == 1
+    assert ws.ignore_source_contribution == 0
+
+
+def test_invalid_arguments_are_reported_not_fatal():
+    h = C.c_void_p()
+    st = A.lib().zg_scene_create(4, None, 0, None, 0, None, None, None, 0, 1e-3, None, C.byref(h))
+    assert st == A.ZG_ERR_INVALID_ARGUMENT
+    assert "dim" in A.last_error()
+
+
+STRUCTS = ["zg_walk_settings", "zg_function", "zg_pde", "zg_build_options", "zg_scene_info", "zg_sample_points",
+           "zg_statistics", "zg_solve_options", "zg_solve_report"]
+
+
+def test_ctypes_layouts_match_c(tmp_path):
+    src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){"]
+    for s in STRUCTS:
+        src.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for f, _ in getattr(A, s)._fields_:
+            src.append(f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(c), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line)
+    for s in STRUCTS:
+        cls = getattr(A, s)
+        assert int(got[s]) == C.sizeof(cls), s
+        for f, _ in cls._fields_:
+            assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, f"{s}.{f}"

@@ -186,3 +186,23 @@ def test_russian_roulette_and_screened_poisson(gpu, oracle_lib, ball3):
             assert rep.completion_codes[1] > 0 and orep["codes"][1] > 0
             ratio = rep.completion_codes[1] / orep["codes"][1]
             assert 0.9 < ratio < 1.1
+
+
+@pytest.mark.parametrize("tol", [0.1, 0.3])
+def test_relaxed_radius_stays_unbiased(gpu, oracle_lib, ball3, tol):
+    """A certified lower bound of the distance is a ball inside the domain: the estimator
+    must still agree with the oracle's exact-radius solve within the 4-SE rule, at the cost
+    of slightly longer walks."""
+    torch = gpu
+    O = oracle_lib
+    p, pts, d, osc = ball3
+    st, rep = _gpu_solve(torch, p, pts, d, 128, seed=15,
+                         opts=E.SolveOptions(seed=15, distance_tolerance=tol))
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    O.solve(osc, p.pde, p.settings, pts, d, 128, ost, seed=16, rng_mode=O.RNG_PER_POINT)
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, f"relaxed tol={tol}")
+    ml_g = g["len"].sum() / g["n"].sum()
+    ml_o = ost.walk_length.sum() / ost.n.sum()
+    assert ml_o * 0.98 < ml_g < ml_o * (1.0 + 2.5 * tol)

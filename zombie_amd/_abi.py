@@ -134,7 +134,7 @@ class zg_solve_options(C.Structure):
         ("regroup_interval", C.c_int32),
         ("waves_per_cu", C.c_int32),
         ("count_visits", C.c_int32),
-        ("reserved", C.c_int32),
+        ("distance_tolerance", C.c_float),
     ]
 
 

@@ -1,0 +1,91 @@
+"""Launch-shape autotuner for the persistent walk kernel.
+
+Tunes, on a sample of the caller's points and with the walk kernel's own HIP-event time:
+  - BVH width {4, 8} and leaf size {1, 2, 4} (scene layout),
+  - walks per lane (claim size) and the regroup interval (ballot/refill period),
+  - the certified distance tolerance of the closest-point query.
+The objective is completed walks per second: the number of walks needed for a target
+error is fixed by the estimator's variance, so walks/s (not raw steps/s, which a looser
+radius inflates) is what shortens the time to a given RMSE.  Coordinate descent: layout
+first, then tolerance, then the launch shape.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+from . import engine as E
+
+
+@dataclass
+class TuneResult:
+    bvh_width: int
+    leaf_size: int
+    walks_per_lane: int
+    regroup_interval: int
+    distance_tolerance: float
+    walks_per_s: float
+    steps_per_s: float
+    table: list = field(default_factory=list)
+
+
+def autotune(problem, points: E.DevicePoints, scenes: dict, widths=(4, 8), leaf_sizes=(1, 2, 4),
+             walks_per_lane=(1, 2, 4), regroups=(8, 16, 32, 64), tolerances=(0.0, 0.05, 0.1, 0.2),
+             sample: int = 65536, walks: int = 64, seed: int = 1234, solve=None) -> TuneResult:
+    """scenes: (width, leaf) -> E.Scene built from `problem` (built on demand when missing)."""
+    import torch
+    solve = solve or E.wos_solve
+    n = min(sample, points.n)
+    sub = E.DevicePoints(pt=points.pt[:n].contiguous(), dist_to_absorbing=points.dist_to_absorbing[:n].contiguous(),
+                         n_walks=torch.full((n,), walks, dtype=torch.int32, device=points.pt.device),
+                         n_walks_max=walks,
+                         type=None if points.type is None else points.type[:n].contiguous(),
+                         normal=None if points.normal is None else points.normal[:n].contiguous())
+    table = []
+
+    def scene_for(width, leaf):
+        key = (width, leaf)
+        if key not in scenes:
+            scenes[key] = E.Scene(problem.positions, problem.indices, problem.is_reflecting, problem.domain_min,
+                                  problem.domain_max, bvh_width=width, leaf_size=leaf,
+                                  solve_double_sided=problem.solve_double_sided)
+        return scenes[key]
+
+    def run(width, leaf, wpl, rg, tol):
+        sc = scene_for(width, leaf)
+        opts = E.SolveOptions(seed=seed, walks_per_lane=wpl, regroup_interval=rg, distance_tolerance=tol)
+        rep = solve(sc, problem.pde, problem.settings, sub, E.DeviceStats.zeros(n), opts)
+        t = max(rep.kernel_ms * 1e-3, 1e-9)
+        row = {"bvh_width": width, "leaf_size": leaf, "walks_per_lane": wpl, "regroup_interval": rg,
+               "distance_tolerance": tol, "walks_per_s": rep.total_walks / t, "steps_per_s": rep.total_steps / t,
+               "mean_walk_length": rep.total_steps / max(rep.total_walks, 1), "kernel_ms": rep.kernel_ms}
+        table.append(row)
+        return row
+
+    best = {"bvh_width": widths[0], "leaf_size": leaf_sizes[-1], "walks_per_lane": walks_per_lane[0],
+            "regroup_interval": regroups[min(2, len(regroups) - 1)], "distance_tolerance": tolerances[0]}
+    # warm every kernel variant once
+    for width in widths:
+        run(width, leaf_sizes[-1], 1, 16, 0.0)
+    table.clear()
+
+    def better(row, cur):
+        return cur is None or row["walks_per_s"] > cur["walks_per_s"]
+
+    cur = None
+    for width, leaf in itertools.product(widths, leaf_sizes):
+        row = run(width, leaf, best["walks_per_lane"], best["regroup_interval"], best["distance_tolerance"])
+        if better(row, cur):
+            cur = row
+    best.update(bvh_width=cur["bvh_width"], leaf_size=cur["leaf_size"])
+    for tol in tolerances:
+        row = run(best["bvh_width"], best["leaf_size"], best["walks_per_lane"], best["regroup_interval"], tol)
+        if better(row, cur):
+            cur = row
+    best.update(distance_tolerance=cur["distance_tolerance"])
+    for wpl, rg in itertools.product(walks_per_lane, regroups):
+        row = run(best["bvh_width"], best["leaf_size"], wpl, rg, best["distance_tolerance"])
+        if better(row, cur):
+            cur = row
+    return TuneResult(cur["bvh_width"], cur["leaf_size"], cur["walks_per_lane"], cur["regroup_interval"],
+                      cur["distance_tolerance"], cur["walks_per_s"], cur["steps_per_s"], table)

@@ -300,6 +300,8 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     if (opts) o = *opts;
     if (o.walks_per_lane < 1 || o.walks_per_lane > 64) return fail(ZG_ERR_INVALID_ARGUMENT, "walks_per_lane must be 1..64");
     if (o.regroup_interval < 1 || o.regroup_interval > 1024) return fail(ZG_ERR_INVALID_ARGUMENT, "regroup_interval must be 1..1024");
+    if (!(o.distance_tolerance >= 0.0f && o.distance_tolerance < 0.5f))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "distance_tolerance must be in [0, 0.5)");
     if ((uint64_t)o.stream_offset + (uint64_t)pts->n_walks_max > 0xffffffffull)
         return fail(ZG_ERR_INVALID_ARGUMENT, "stream_offset + n_walks_max must fit in 32 bits");
     hipStream_t strm = (hipStream_t)stream;
@@ -323,6 +325,11 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     P.chunksPerPoint = std::max(1, (pts->n_walks_max + o.walks_per_lane - 1) / o.walks_per_lane);
     P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
     P.seed = o.seed; P.streamOffset = o.stream_offset;
+    P.relax2 = (1.0f - o.distance_tolerance) * (1.0f - o.distance_tolerance);
+    {
+        float e2 = 2.0f * ws->epsilon_shell_for_absorbing_boundary;
+        P.guard2 = e2 * e2 / P.relax2;
+    }
     P.accSum = reinterpret_cast<double*>(wsp);
     P.accSq = reinterpret_cast<double*>(wsp + n);
     P.accN = reinterpret_cast<unsigned long long*>(wsp + 2 * n);

@@ -355,31 +355,50 @@ ZG_DEV bool isLeaf(int c) { return c < 0; }
 ZG_DEV int leafFirstD(int c) { return (~c) >> 4; }
 ZG_DEV int leafCountD(int c) { return ((~c) & 15) + 1; }
 
-// Traversal stack: the top kLds entries of a lane live in LDS (column tid of a
-// [kLds][blockDim] array, so lanes at the same depth hit consecutive banks); deeper
-// entries spill to a per-lane private array (scratch), used only by rare deep traversals.
+// Traversal stack: the top kLdsStack entries of a lane live in LDS (dynamic shared memory
+// laid out [depth][blockDim], so lanes at the same depth hit consecutive banks); deeper
+// entries spill to a per-lane private array (scratch) owned by the caller, touched only by
+// rare deep traversals.  Kernels that traverse launch with kStackLdsBytes of dynamic LDS.
 constexpr int kLdsStack = 16;
 constexpr int kSpillStack = 96;
+constexpr int kStackLdsBytes = kLdsStack * 256 * 8;
+
+extern __shared__ uint2 zgLdsStack[];
 
 struct Stack {
-    uint2* lds;       // &smem[0][threadIdx.x]
+    int col;          // threadIdx.x
     int stride;       // blockDim.x
     int sp;           // entries in LDS
     int ovf;          // entries in the spill array
-    uint2 spill[kSpillStack];
     bool overflow;
-    ZG_DEV void init(uint2* base, int s) { lds = base; stride = s; sp = 0; ovf = 0; overflow = false; }
+    unsigned long long* spill;  // caller's private array of kSpillStack entries
+    ZG_DEV void init(unsigned long long* spillArray) {
+        col = threadIdx.x; stride = blockDim.x; sp = 0; ovf = 0; overflow = false; spill = spillArray;
+    }
     ZG_DEV void push(int node, float d2) {
         uint2 e = make_uint2((unsigned)node, __float_as_uint(d2));
-        if (sp < kLdsStack) { lds[sp * stride] = e; sp++; }
-        else if (ovf < kSpillStack) { spill[ovf++] = e; }
+        if (sp < kLdsStack) { zgLdsStack[sp * stride + col] = e; sp++; }
+        else if (ovf < kSpillStack) {
+            reinterpret_cast<volatile unsigned long long*>(spill)[ovf] = ((unsigned long long)e.y << 32) | e.x;
+            ovf++;
+        }
         else overflow = true;
     }
     ZG_DEV bool pop(int& node, float& d2) {
         uint2 e;
-        if (ovf > 0) e = spill[--ovf];
-        else if (sp > 0) { sp--; e = lds[sp * stride]; }
-        else return false;
+        if (ovf > 0) {
+            // volatile keeps this private-memory load apart from the LDS load below (merged,
+            // the two would become one slow flat load on the hot path)
+            ovf--;
+            unsigned long long v = reinterpret_cast<volatile unsigned long long*>(spill)[ovf];
+            e.x = (unsigned)v;
+            e.y = (unsigned)(v >> 32);
+        } else if (sp > 0) {
+            sp--;
+            e = zgLdsStack[sp * stride + col];
+        } else {
+            return false;
+        }
         node = (int)e.x; d2 = __uint_as_float(e.y);
         return true;
     }
@@ -387,56 +406,135 @@ struct Stack {
 
 struct VisitCount { uint32_t nodes, prims; };
 
-// Closest point on one boundary part: exact nearest primitive under the squared-distance
-// bound best2 (pass FLT_MAX for an unbounded search).  Leaves met in a node are tested at
-// once (tightening best2 early), the nearest inner child is entered next and the other
-// inner children still within best2 are pushed.  Returns false when nothing lies within
-// the bound.  (Role of fcpw Aggregate::findClosestPoint, fcpw_geometric_queries.h:1069-1096.)
-template <int D, int W, bool COUNT>
-ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, Vec<D>& cp, int& prim,
-                         Stack& st, VisitCount& vc) {
-    bool found = false;
-    if (b.nNodes == 0) return false;
-    st.sp = 0; st.ovf = 0;
-    int node = 0;
-    for (;;) {
-        WideNode<D, W> nd;
-        nd.load(b.nodes, node);
-        if (COUNT) vc.nodes++;
-        float d2[W];
+// Squared distances from x to the W child boxes of node `idx`, streamed axis by axis
+// (two dwordx4 loads per bound and axis for W = 8) so that only the running sums stay live.
+template <int D, int W>
+ZG_DEV void childDistances(const float* nodes, int idx, const Vec<D>& x, float (&d2)[W], int (&ch)[W]) {
+    using L = NodeLayout<D, W>;
+    const float4* q = reinterpret_cast<const float4*>(nodes + (size_t)idx * L::kFloats);
 #pragma unroll
-        for (int k = 0; k < W; k++) d2[k] = nd.dist2(k, x);
+    for (int k = 0; k < W; k++) d2[k] = 0.0f;
 #pragma unroll
-        for (int k = 0; k < W; k++) {
-            if (isLeaf(nd.ch[k]) && d2[k] <= best2) {
-                int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
-                for (int i = first; i < first + cnt; i++) {
-                    Prim<D> pr; pr.load(b.prims, i);
-                    Vec<D> q;
-                    float dd = pr.closest2(x, q);
-                    if (COUNT) vc.prims++;
-                    if (dd < best2) { best2 = dd; cp = q; prim = i; found = true; }
-                }
+    for (int a = 0; a < D; a++) {
+#pragma unroll
+        for (int v = 0; v < W / 4; v++) {
+            float4 lo = q[a * (W / 4) + v];
+            float4 hi = q[(D + a) * (W / 4) + v];
+            float lov[4] = {lo.x, lo.y, lo.z, lo.w}, hiv[4] = {hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float d = fmaxf(fmaxf(lov[j] - x[a], x[a] - hiv[j]), 0.0f);
+                d2[4 * v + j] += d * d;
             }
         }
-        int next = -1; float nextD = 0.0f; int nk = -1;
+    }
+    const int4* c = reinterpret_cast<const int4*>(q + (2 * D * W) / 4);
+#pragma unroll
+    for (int v = 0; v < W / 4; v++) {
+        int4 cv = c[v];
+        ch[4 * v] = cv.x; ch[4 * v + 1] = cv.y; ch[4 * v + 2] = cv.z; ch[4 * v + 3] = cv.w;
+    }
+}
+
+// Closest point on one boundary part under the squared-distance bound best2 (pass FLT_MAX
+// for an unbounded search).  The children of a node are visited nearest first: leaves are
+// tested in ascending box distance while their box still lies within the pruning bound
+// (which shrinks as primitives are found), the nearest surviving inner child is entered
+// next and the other surviving inner children are pushed farthest-first so that the stack
+// pops them nearest-first.  (Role of fcpw Aggregate::findClosestPoint,
+// fcpw_geometric_queries.h:1069-1096.)
+//
+// relax2 = (1 - tau)^2 < 1 prunes boxes already within a factor (1 - tau) of the best
+// distance once best2 > guard2.  Every pruned box then lies at least (1 - tau) times the
+// final best distance away, so radius2 = min(best2, smallest pruned box distance^2) is a
+// certified lower bound of the true squared distance (exact when nothing was pruned that
+// way).  Walk on spheres stays unbiased with any ball inside the domain, so the walk uses
+// radius2 while cp stays the best boundary point found (distance^2 best2).
+// Returns false when nothing lies within the initial bound.
+template <int D, int W, bool COUNT>
+ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, Vec<D>& cp, int& prim,
+                         Stack& st, VisitCount& vc, float relax2 = 1.0f, float guard2 = 0.0f,
+                         float* radius2 = nullptr) {
+    bool found = false;
+    float lower2 = FLT_MAX;  // smallest box distance^2 pruned by the relaxation
+    if (b.nNodes == 0) { if (radius2) *radius2 = best2; return false; }
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    float prune2 = best2 > guard2 ? best2 * relax2 : best2;
+    for (;;) {
+        float d2[W];
+        int ch[W];
+        childDistances<D, W>(b.nodes, node, x, d2, ch);
+        if (COUNT) vc.nodes++;
+        uint32_t cand = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            if (!isLeaf(nd.ch[k]) && d2[k] <= best2 && (next < 0 || d2[k] < nextD)) { next = nd.ch[k]; nextD = d2[k]; nk = k; }
+            cand |= (d2[k] <= prune2 ? 1u : 0u) << k;
+            if (d2[k] > prune2 && d2[k] <= best2) lower2 = fminf(lower2, d2[k]);
         }
+        uint32_t inner = 0;
+        while (cand) {
+            float m = FLT_MAX; int bk = 0; int c = 0;
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                bool take = ((cand >> k) & 1u) && d2[k] <= m;
+                m = take ? d2[k] : m; bk = take ? k : bk; c = take ? ch[k] : c;
+            }
+            cand &= ~(1u << bk);
+            if (m > prune2) {
+                if (m <= best2) lower2 = fminf(lower2, m);
+                continue;
+            }
+            if (isLeaf(c)) {
+                int first = leafFirstD(c), cnt = leafCountD(c);
+                for (int i = first; i < first + cnt; i++) {
+                    Prim<D> pr; pr.load(b.prims, i);
+                    Vec<D> qq;
+                    float dd = pr.closest2(x, qq);
+                    if (COUNT) vc.prims++;
+                    if (dd < best2) {
+                        best2 = dd; cp = qq; prim = i; found = true;
+                        prune2 = best2 > guard2 ? best2 * relax2 : best2;
+                    }
+                }
+            } else {
+                inner |= 1u << bk;
+            }
+        }
+        // drop inner children pruned by the leaves just tested
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            if (!isLeaf(nd.ch[k]) && d2[k] <= best2 && k != nk) st.push(nd.ch[k], d2[k]);
+            if (((inner >> k) & 1u) && d2[k] > prune2) {
+                inner &= ~(1u << k);
+                if (d2[k] <= best2) lower2 = fminf(lower2, d2[k]);
+            }
         }
-        if (next < 0) {
+        int next = -1;
+        if (inner) {
+            // push all but the nearest, farthest first
+            while (__builtin_popcount(inner) > 1) {
+                float m = -1.0f; int bk = 0; int c = 0;
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    bool take = ((inner >> k) & 1u) && d2[k] > m;
+                    m = take ? d2[k] : m; bk = take ? k : bk; c = take ? ch[k] : c;
+                }
+                inner &= ~(1u << bk);
+                st.push(c, m);
+            }
+#pragma unroll
+            for (int k = 0; k < W; k++) if ((inner >> k) & 1u) next = ch[k];
+        } else {
             int n; float dn;
-            bool got = false;
-            while (st.pop(n, dn)) { if (dn <= best2) { got = true; break; } }
-            if (!got) break;
-            next = n;
+            while (st.pop(n, dn)) {
+                if (dn <= prune2) { next = n; break; }
+                if (dn <= best2) lower2 = fminf(lower2, dn);
+            }
         }
+        if (next < 0) break;
         node = next;
     }
+    if (radius2) *radius2 = fminf(best2, lower2);
     return found;
 }
 

@@ -33,6 +33,8 @@ struct WosParams {
     unsigned long long nClaims;
     uint64_t seed;
     uint32_t streamOffset;
+    float relax2;        // (1 - distance_tolerance)^2
+    float guard2;        // relaxation only when best^2 > guard2 (keeps the shell test exact)
     uint32_t pad2;
     double* accSum;
     double* accSq;

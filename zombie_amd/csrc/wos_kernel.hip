@@ -35,10 +35,10 @@ template <int D, int W>
 __global__ void __launch_bounds__(256) closestPointKernel(BoundaryView b, int64_t n, const float* __restrict__ pts,
                                                           float* __restrict__ dist, float* __restrict__ closest,
                                                           float* __restrict__ normal, int32_t* __restrict__ prim) {
-    __shared__ uint2 stk[kLdsStack][256];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
     VisitCount vc{0, 0};
     Vec<D> x = vload<D>(pts + i * D);
     float best2 = FLT_MAX;
@@ -62,10 +62,10 @@ __global__ void __launch_bounds__(256) closestPointKernel(BoundaryView b, int64_
 template <int D, int W>
 __global__ void __launch_bounds__(256) insideDomainKernel(SceneView s, int64_t n, const float* __restrict__ pts,
                                                           uint8_t* __restrict__ inside) {
-    __shared__ uint2 stk[kLdsStack][256];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
     Vec<D> x = vload<D>(pts + i * D);
     bool in = true;
     for (int a = 0; a < D; a++) {
@@ -80,10 +80,10 @@ __global__ void __launch_bounds__(256) rayKernel(BoundaryView b, int64_t n, cons
                                                  const float* __restrict__ d, const float* __restrict__ tmax,
                                                  uint8_t* __restrict__ hit, float* __restrict__ t, float* __restrict__ point,
                                                  float* __restrict__ normal) {
-    __shared__ uint2 stk[kLdsStack][256];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
     Vec<D> oo = vload<D>(o + i * D), dd = vload<D>(d + i * D);
     float th; int pi = -1;
     bool h = intersectRay<D, W>(b, oo, dd, tmax[i], th, pi, st);
@@ -114,8 +114,8 @@ ZG_DEV bool outsideDomain(const SceneView& s, const Vec<D>& x) {
 
 template <int D, int W, bool COUNT>
 __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
-    __shared__ uint2 stk[kLdsStack][256];
-    Stack st; st.init(&stk[0][threadIdx.x], blockDim.x);
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
     VisitCount vc{0, 0};
     const zg_walk_settings& ws = P.ws;
     const float eps = ws.epsilon_shell_for_absorbing_boundary;
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
     // claim
     int mode = 0;          // 0 idle, 1 stepping, 2 waiting for the projection query
     int pendingCode = 0;
-    int64_t p = 0;
+    int p = 0;
     int w = 0, wEnd = 0;
     // walk
     Vec<D> x = vzero<D>(), yc = vzero<D>();
@@ -139,11 +139,11 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
     double aSum = 0.0, aSq = 0.0;
     uint32_t aN = 0, aLen = 0;
     // per-lane counters
-    uint64_t cnt[4] = {0, 0, 0, 0};
-    uint64_t steps = 0, walks = 0, queries = 0;
+    uint32_t cnt0 = 0, cnt1 = 0, cnt2 = 0, cnt3 = 0;  // walks per completion code
+    uint32_t steps = 0, walks = 0, queries = 0;
 
     auto startWalk = [&]() {
-        x = vload<D>(P.pt + p * D);
+        x = vload<D>(P.pt + (size_t)p * D);
         dist = P.dist0[p];
         T = 1.0f; S = 0.0f; len = 0;
         ball.yukawa = sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
     };
 
     auto finishWalk = [&](int code) {
-        cnt[code]++;
+        cnt0 += code == 0; cnt1 += code == 1; cnt2 += code == 2; cnt3 += code == 3;
         if (code != ZG_ESCAPED_DOMAIN) {
             float g = 0.0f;
             if ((code == ZG_REACHED_ABSORBING_BOUNDARY || code == ZG_EXCEEDED_MAX_WALK_LENGTH) &&
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
             aSq += (double)c * (double)c;
             aN++;
             aLen += (uint32_t)len;
-            steps += (uint64_t)len;
+            steps += (uint32_t)len;
         }
         w++;
         if (w < wEnd) startWalk();
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
                 if (mode == 0) {
                     unsigned long long c = base + mbcnt64(idle);
                     if (c < P.nClaims) {
-                        p = (int64_t)(c / (unsigned long long)P.chunksPerPoint);
+                        p = (int)(c / (unsigned long long)P.chunksPerPoint);
                         int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
                         int nw = P.nWalks[p];
                         uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
@@ -280,21 +280,27 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
                 float r2 = 2.0f * dist;
                 best2 = r2 * r2 * 1.0001f + 1e-30f;
             }
+            // the terminal projection (mode 2) is always exact
+            float relax2 = mode == 2 ? 1.0f : P.relax2;
             Vec<D> q = yc; int qp = -1;
-            bool found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc);
+            float rad2 = best2;
+            bool found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc, relax2, P.guard2, &rad2);
             if (!found) {
                 best2 = FLT_MAX;
-                found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc);
+                found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc, relax2, P.guard2, &rad2);
             }
             queries++;
-            if (found) { yc = q; ycPrim = qp; ycValid = true; dist = __builtin_sqrtf(best2); }
+            if (found) { yc = q; ycPrim = qp; ycValid = true; dist = __builtin_sqrtf(rad2); }
             else { dist = FLT_MAX; ycValid = false; }
             if (mode == 2) finishWalk(pendingCode);
         }
     }
 
     // -------- per-lane counters --------
-    for (int k = 0; k < 4; k++) if (cnt[k]) atomicAdd(&P.counters[k], (unsigned long long)cnt[k]);
+    if (cnt0) atomicAdd(&P.counters[0], (unsigned long long)cnt0);
+    if (cnt1) atomicAdd(&P.counters[1], (unsigned long long)cnt1);
+    if (cnt2) atomicAdd(&P.counters[2], (unsigned long long)cnt2);
+    if (cnt3) atomicAdd(&P.counters[3], (unsigned long long)cnt3);
     if (steps) atomicAdd(&P.counters[4], (unsigned long long)steps);
     if (walks) atomicAdd(&P.counters[5], (unsigned long long)walks);
     if (COUNT) {
@@ -368,20 +374,20 @@ hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, cons
     const BoundaryView& b = boundary ? sv.refl : sv.abs;
     if (n <= 0) return hipSuccess;
     dim3 g(blocksFor(n)), bl(256);
-    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((closestPointKernel<2, 4>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
-    else if (sv.dim == 2) hipLaunchKernelGGL((closestPointKernel<2, 8>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
-    else if (sv.width == 4) hipLaunchKernelGGL((closestPointKernel<3, 4>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
-    else hipLaunchKernelGGL((closestPointKernel<3, 8>), g, bl, 0, s, b, n, pts, dist, closest, normal, prim);
+    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((closestPointKernel<2, 4>), g, bl, kStackLdsBytes, s, b, n, pts, dist, closest, normal, prim);
+    else if (sv.dim == 2) hipLaunchKernelGGL((closestPointKernel<2, 8>), g, bl, kStackLdsBytes, s, b, n, pts, dist, closest, normal, prim);
+    else if (sv.width == 4) hipLaunchKernelGGL((closestPointKernel<3, 4>), g, bl, kStackLdsBytes, s, b, n, pts, dist, closest, normal, prim);
+    else hipLaunchKernelGGL((closestPointKernel<3, 8>), g, bl, kStackLdsBytes, s, b, n, pts, dist, closest, normal, prim);
     return hipGetLastError();
 }
 
 hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, uint8_t* inside, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 g(blocksFor(n)), bl(256);
-    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((insideDomainKernel<2, 4>), g, bl, 0, s, sv, n, pts, inside);
-    else if (sv.dim == 2) hipLaunchKernelGGL((insideDomainKernel<2, 8>), g, bl, 0, s, sv, n, pts, inside);
-    else if (sv.width == 4) hipLaunchKernelGGL((insideDomainKernel<3, 4>), g, bl, 0, s, sv, n, pts, inside);
-    else hipLaunchKernelGGL((insideDomainKernel<3, 8>), g, bl, 0, s, sv, n, pts, inside);
+    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((insideDomainKernel<2, 4>), g, bl, kStackLdsBytes, s, sv, n, pts, inside);
+    else if (sv.dim == 2) hipLaunchKernelGGL((insideDomainKernel<2, 8>), g, bl, kStackLdsBytes, s, sv, n, pts, inside);
+    else if (sv.width == 4) hipLaunchKernelGGL((insideDomainKernel<3, 4>), g, bl, kStackLdsBytes, s, sv, n, pts, inside);
+    else hipLaunchKernelGGL((insideDomainKernel<3, 8>), g, bl, kStackLdsBytes, s, sv, n, pts, inside);
     return hipGetLastError();
 }
 
@@ -390,16 +396,16 @@ hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* 
     const BoundaryView& b = boundary ? sv.refl : sv.abs;
     if (n <= 0) return hipSuccess;
     dim3 g(blocksFor(n)), bl(256);
-    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((rayKernel<2, 4>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
-    else if (sv.dim == 2) hipLaunchKernelGGL((rayKernel<2, 8>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
-    else if (sv.width == 4) hipLaunchKernelGGL((rayKernel<3, 4>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
-    else hipLaunchKernelGGL((rayKernel<3, 8>), g, bl, 0, s, b, n, o, d, tmax, hit, t, point, normal);
+    if (sv.dim == 2 && sv.width == 4) hipLaunchKernelGGL((rayKernel<2, 4>), g, bl, kStackLdsBytes, s, b, n, o, d, tmax, hit, t, point, normal);
+    else if (sv.dim == 2) hipLaunchKernelGGL((rayKernel<2, 8>), g, bl, kStackLdsBytes, s, b, n, o, d, tmax, hit, t, point, normal);
+    else if (sv.width == 4) hipLaunchKernelGGL((rayKernel<3, 4>), g, bl, kStackLdsBytes, s, b, n, o, d, tmax, hit, t, point, normal);
+    else hipLaunchKernelGGL((rayKernel<3, 8>), g, bl, kStackLdsBytes, s, b, n, o, d, tmax, hit, t, point, normal);
     return hipGetLastError();
 }
 
 hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-#define ZG_WOS(D, Wd, C) hipLaunchKernelGGL((wosKernel<D, Wd, C>), g, bl, 0, s, P)
+#define ZG_WOS(D, Wd, C) hipLaunchKernelGGL((wosKernel<D, Wd, C>), g, bl, kStackLdsBytes, s, P)
     if (dim == 2) {
         if (width == 4) { if (count) ZG_WOS(2, 4, true); else ZG_WOS(2, 4, false); }
         else { if (count) ZG_WOS(2, 8, true); else ZG_WOS(2, 8, false); }

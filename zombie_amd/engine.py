@@ -101,6 +101,7 @@ class SolveOptions:
     regroup_interval: int = 16
     waves_per_cu: int = 0
     count_visits: bool = False
+    distance_tolerance: float = 0.0
 
     def to_c(self) -> A.zg_solve_options:
         o = A.zg_solve_options()
@@ -110,6 +111,7 @@ class SolveOptions:
         o.regroup_interval = self.regroup_interval
         o.waves_per_cu = self.waves_per_cu
         o.count_visits = 1 if self.count_visits else 0
+        o.distance_tolerance = self.distance_tolerance
         return o
 
 
