@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--walks", type=int, default=256)
     ap.add_argument("--bvh-width", type=int, default=8)
     ap.add_argument("--walks-per-lane", type=int, default=8)
-    ap.add_argument("--regroup", type=int, default=16)
+    ap.add_argument("--regroup", type=int, default=8)
+    ap.add_argument("--step-interval", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=4)
     ap.add_argument("--tolerance", type=float, default=0.0, help="distance tolerance when not autotuning")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
@@ -102,16 +103,20 @@ def main():
         tune = AT.autotune(prob, dp, scenes)
         for row in tune.table:
             log("tune: width {bvh_width} leaf {leaf_size} wpl {walks_per_lane:2d} regroup {regroup_interval:2d} "
-                "tol {distance_tolerance:.2f} -> {walks_per_s:10.4g} walks/s {steps_per_s:10.4g} steps/s "
+                "step {step_interval:2d} tol {distance_tolerance:.2f} -> {walks_per_s:10.4g} walks/s "
+                "{steps_per_s:10.4g} steps/s "
                 "len {mean_walk_length:.2f}".format(**row))
         args.bvh_width, leaf = tune.bvh_width, tune.leaf_size
         args.walks_per_lane, args.regroup, tol = tune.walks_per_lane, tune.regroup_interval, tune.distance_tolerance
+        args.step_interval = tune.step_interval
         scene = scenes[(args.bvh_width, leaf)]
-        log(f"tuned: width {args.bvh_width} leaf {leaf} wpl {args.walks_per_lane} regroup {args.regroup} tol {tol}")
+        log(f"tuned: width {args.bvh_width} leaf {leaf} wpl {args.walks_per_lane} regroup {args.regroup} "
+            f"step {args.step_interval} tol {tol}")
 
     def opts(step, count=False, tolerance=None):
         return E.SolveOptions(seed=args.seed, stream_offset=step * args.walks, walks_per_lane=args.walks_per_lane,
-                              regroup_interval=args.regroup, waves_per_cu=args.waves_per_cu, count_visits=count,
+                              regroup_interval=args.regroup, step_interval=args.step_interval,
+                              waves_per_cu=args.waves_per_cu, count_visits=count,
                               distance_tolerance=tol if tolerance is None else tolerance)
 
     # instrumented passes on a sample: BVH nodes / triangles visited per walk step, and the
@@ -175,8 +180,9 @@ def main():
     if prof.exists():
         pj = json.loads(prof.read_text())
         c = pj["config"]
-        if (c["bvh_width"], c["leaf_size"], c["walks_per_lane"], c["regroup_interval"], c["distance_tolerance"]) == \
-                (args.bvh_width, leaf, args.walks_per_lane, args.regroup, tol):
+        if (c["bvh_width"], c["leaf_size"], c["walks_per_lane"], c["regroup_interval"], c.get("step_interval"),
+                c["distance_tolerance"]) == (args.bvh_width, leaf, args.walks_per_lane, args.regroup,
+                                             args.step_interval, tol):
             traffic = pj["traffic_bytes"] / pj["raw_steps"] * (total_steps / len(reports))
     result = {
         "metric": METRIC,
@@ -194,7 +200,8 @@ def main():
         "config": {"workload": "3D Poisson WoS, SH-displaced icosphere 81,920 tris, 1M points, "
                                f"{args.walks} walks/pt/pass, eps 1e-4",
                    "bvh_width": args.bvh_width, "leaf_size": leaf, "walks_per_lane": args.walks_per_lane,
-                   "regroup_interval": args.regroup, "distance_tolerance": tol, "points": n,
+                   "regroup_interval": args.regroup, "step_interval": args.step_interval,
+                   "distance_tolerance": tol, "points": n,
                    "walks_per_point_per_step": args.walks, "autotuned": tune is not None},
         "value_definition": "completed walks/s x mean walk length with exact radii (reference walk-steps/s)",
         "walks_per_s": walks_per_s,

@@ -227,7 +227,8 @@ typedef struct zg_solve_options {
     uint64_t seed;            /* PCG32 initstate of every walk of the call              */
     uint32_t stream_offset;   /* added to the walk index in the PCG32 stream id          */
     int32_t walks_per_lane;   /* walks of one point a lane claims at a time (1..64)       */
-    int32_t regroup_interval; /* steps between wavefront ballots that refill idle lanes  */
+    int32_t regroup_interval; /* iterations between wavefront ballots that refill idle lanes */
+    int32_t step_interval;    /* iterations (one BVH node visit each) between walk steps   */
     int32_t waves_per_cu;     /* persistent grid size = CUs * waves_per_cu (0 = auto)    */
     int32_t count_visits;     /* 1: instrumented kernel counting BVH node/primitive visits */
     float distance_tolerance; /* tau in [0, 0.5): 0 = exact closest-point radii.  tau > 0 lets

@@ -132,6 +132,7 @@ class zg_solve_options(C.Structure):
         ("stream_offset", C.c_uint32),
         ("walks_per_lane", C.c_int32),
         ("regroup_interval", C.c_int32),
+        ("step_interval", C.c_int32),
         ("waves_per_cu", C.c_int32),
         ("count_visits", C.c_int32),
         ("distance_tolerance", C.c_float),

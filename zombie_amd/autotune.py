@@ -23,6 +23,7 @@ class TuneResult:
     leaf_size: int
     walks_per_lane: int
     regroup_interval: int
+    step_interval: int
     distance_tolerance: float
     walks_per_s: float
     steps_per_s: float
@@ -30,7 +31,8 @@ class TuneResult:
 
 
 def autotune(problem, points: E.DevicePoints, scenes: dict, widths=(4, 8), leaf_sizes=(1, 2, 4),
-             walks_per_lane=(1, 2, 4), regroups=(8, 16, 32, 64), tolerances=(0.0, 0.05, 0.1, 0.2),
+             walks_per_lane=(1, 4), regroups=(4, 8, 16, 32), step_intervals=(1, 2, 4, 8),
+             tolerances=(0.0, 0.1, 0.2, 0.3),
              sample: int = 65536, walks: int = 64, seed: int = 1234, solve=None) -> TuneResult:
     """scenes: (width, leaf) -> E.Scene built from `problem` (built on demand when missing)."""
     import torch
@@ -51,41 +53,54 @@ def autotune(problem, points: E.DevicePoints, scenes: dict, widths=(4, 8), leaf_
                                   solve_double_sided=problem.solve_double_sided)
         return scenes[key]
 
-    def run(width, leaf, wpl, rg, tol):
+    def run(width, leaf, wpl, rg, si, tol):
         sc = scene_for(width, leaf)
-        opts = E.SolveOptions(seed=seed, walks_per_lane=wpl, regroup_interval=rg, distance_tolerance=tol)
+        opts = E.SolveOptions(seed=seed, walks_per_lane=wpl, regroup_interval=rg, step_interval=si,
+                              distance_tolerance=tol)
         rep = solve(sc, problem.pde, problem.settings, sub, E.DeviceStats.zeros(n), opts)
         t = max(rep.kernel_ms * 1e-3, 1e-9)
         row = {"bvh_width": width, "leaf_size": leaf, "walks_per_lane": wpl, "regroup_interval": rg,
-               "distance_tolerance": tol, "walks_per_s": rep.total_walks / t, "steps_per_s": rep.total_steps / t,
+               "step_interval": si, "distance_tolerance": tol, "walks_per_s": rep.total_walks / t, "steps_per_s": rep.total_steps / t,
                "mean_walk_length": rep.total_steps / max(rep.total_walks, 1), "kernel_ms": rep.kernel_ms}
         table.append(row)
         return row
 
     best = {"bvh_width": widths[0], "leaf_size": leaf_sizes[-1], "walks_per_lane": walks_per_lane[0],
-            "regroup_interval": regroups[min(2, len(regroups) - 1)], "distance_tolerance": tolerances[0]}
+            "regroup_interval": regroups[min(1, len(regroups) - 1)], "step_interval": step_intervals[min(2, len(step_intervals) - 1)],
+            "distance_tolerance": tolerances[min(2, len(tolerances) - 1)]}
     # warm every kernel variant once
     for width in widths:
-        run(width, leaf_sizes[-1], 1, 16, 0.0)
+        run(width, leaf_sizes[-1], 1, 16, 4, 0.0)
     table.clear()
+
+    def cfg(**kw):
+        c = dict(best)
+        c.update(kw)
+        return run(c["bvh_width"], c["leaf_size"], c["walks_per_lane"], c["regroup_interval"], c["step_interval"],
+                   c["distance_tolerance"])
 
     def better(row, cur):
         return cur is None or row["walks_per_s"] > cur["walks_per_s"]
 
     cur = None
     for width, leaf in itertools.product(widths, leaf_sizes):
-        row = run(width, leaf, best["walks_per_lane"], best["regroup_interval"], best["distance_tolerance"])
+        row = cfg(bvh_width=width, leaf_size=leaf)
         if better(row, cur):
             cur = row
     best.update(bvh_width=cur["bvh_width"], leaf_size=cur["leaf_size"])
     for tol in tolerances:
-        row = run(best["bvh_width"], best["leaf_size"], best["walks_per_lane"], best["regroup_interval"], tol)
+        row = cfg(distance_tolerance=tol)
         if better(row, cur):
             cur = row
     best.update(distance_tolerance=cur["distance_tolerance"])
+    for si in step_intervals:
+        row = cfg(step_interval=si)
+        if better(row, cur):
+            cur = row
+    best.update(step_interval=cur["step_interval"])
     for wpl, rg in itertools.product(walks_per_lane, regroups):
-        row = run(best["bvh_width"], best["leaf_size"], wpl, rg, best["distance_tolerance"])
+        row = cfg(walks_per_lane=wpl, regroup_interval=rg)
         if better(row, cur):
             cur = row
     return TuneResult(cur["bvh_width"], cur["leaf_size"], cur["walks_per_lane"], cur["regroup_interval"],
-                      cur["distance_tolerance"], cur["walks_per_s"], cur["steps_per_s"], table)
+                      cur["step_interval"], cur["distance_tolerance"], cur["walks_per_s"], cur["steps_per_s"], table)

@@ -153,6 +153,7 @@ void zg_solve_options_default(zg_solve_options* o) {
     o->seed = 0x5EEDull;
     o->walks_per_lane = 8;
     o->regroup_interval = 16;
+    o->step_interval = 4;
     o->waves_per_cu = 0;
 }
 
@@ -300,6 +301,7 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     if (opts) o = *opts;
     if (o.walks_per_lane < 1 || o.walks_per_lane > 64) return fail(ZG_ERR_INVALID_ARGUMENT, "walks_per_lane must be 1..64");
     if (o.regroup_interval < 1 || o.regroup_interval > 1024) return fail(ZG_ERR_INVALID_ARGUMENT, "regroup_interval must be 1..1024");
+    if (o.step_interval < 1 || o.step_interval > 1024) return fail(ZG_ERR_INVALID_ARGUMENT, "step_interval must be 1..1024");
     if (!(o.distance_tolerance >= 0.0f && o.distance_tolerance < 0.5f))
         return fail(ZG_ERR_INVALID_ARGUMENT, "distance_tolerance must be in [0, 0.5)");
     if ((uint64_t)o.stream_offset + (uint64_t)pts->n_walks_max > 0xffffffffull)
@@ -321,7 +323,7 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     P.scene = s->view; P.pde = *pde; P.ws = *ws;
     P.pt = pts->pt; P.normal = pts->normal; P.type = pts->type; P.aligned = pts->estimate_boundary_normal_aligned;
     P.dist0 = pts->dist_to_absorbing_boundary; P.nWalks = pts->n_walks; P.nPoints = n;
-    P.wpl = o.walks_per_lane; P.regroup = o.regroup_interval;
+    P.wpl = o.walks_per_lane; P.regroup = o.regroup_interval; P.stepInterval = o.step_interval;
     P.chunksPerPoint = std::max(1, (pts->n_walks_max + o.walks_per_lane - 1) / o.walks_per_lane);
     P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
     P.seed = o.seed; P.streamOffset = o.stream_offset;

@@ -436,32 +436,65 @@ ZG_DEV void childDistances(const float* nodes, int idx, const Vec<D>& x, float (
     }
 }
 
-// Closest point on one boundary part under the squared-distance bound best2 (pass FLT_MAX
-// for an unbounded search).  The children of a node are visited nearest first: leaves are
-// tested in ascending box distance while their box still lies within the pruning bound
-// (which shrinks as primitives are found), the nearest surviving inner child is entered
-// next and the other surviving inner children are pushed farthest-first so that the stack
-// pops them nearest-first.  (Role of fcpw Aggregate::findClosestPoint,
-// fcpw_geometric_queries.h:1069-1096.)
+// Resumable closest-point query on one boundary part.  A query visits one BVH node per call
+// of visit(), so a persistent kernel can interleave the traversals of its lanes with their
+// walk steps instead of making every lane of a wave wait for the slowest traversal.
 //
-// relax2 = (1 - tau)^2 < 1 prunes boxes already within a factor (1 - tau) of the best
-// distance once best2 > guard2.  Every pruned box then lies at least (1 - tau) times the
-// final best distance away, so radius2 = min(best2, smallest pruned box distance^2) is a
-// certified lower bound of the true squared distance (exact when nothing was pruned that
-// way).  Walk on spheres stays unbiased with any ball inside the domain, so the walk uses
-// radius2 while cp stays the best boundary point found (distance^2 best2).
-// Returns false when nothing lies within the initial bound.
-template <int D, int W, bool COUNT>
-ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, Vec<D>& cp, int& prim,
-                         Stack& st, VisitCount& vc, float relax2 = 1.0f, float guard2 = 0.0f,
-                         float* radius2 = nullptr) {
-    bool found = false;
-    float lower2 = FLT_MAX;  // smallest box distance^2 pruned by the relaxation
-    if (b.nNodes == 0) { if (radius2) *radius2 = best2; return false; }
-    st.sp = 0; st.ovf = 0;
-    int node = 0;
-    float prune2 = best2 > guard2 ? best2 * relax2 : best2;
-    for (;;) {
+// Order: the children of a node are handled nearest first -- leaves are tested in ascending
+// box distance while their box still lies within the pruning bound (which shrinks as
+// primitives are found), the nearest surviving inner child is visited next and the other
+// surviving inner children are pushed farthest-first so that the stack pops them
+// nearest-first.  (Role of fcpw Aggregate::findClosestPoint, fcpw_geometric_queries.h:
+// 1069-1096.)
+//
+// Bound: best2 starts at the caller's bound (FLT_MAX for an unbounded search); a query that
+// finds nothing within it is restarted once without bound.
+//
+// Certified relaxation: relax2 = (1 - tau)^2 < 1 prunes boxes already within a factor
+// (1 - tau) of the best distance once best2 > guard2.  Every box pruned that way lies at least
+// (1 - tau) times the final best distance away, so radius2 = min(best2, smallest such box
+// distance^2) is a certified lower bound of the true squared distance (exact when nothing was
+// pruned that way).  Walk on spheres stays unbiased with any ball inside the domain, so the
+// walk uses radius2 while cp stays the best boundary point found.
+template <int D>
+struct ClosestQuery {
+    Vec<D> cp;       // best point found
+    float best2;     // its squared distance (or the bound)
+    float prune2;    // current pruning bound
+    float lower2;    // smallest box distance^2 pruned by the relaxation
+    float relax2, guard2;
+    int node;        // node to visit next, -1 = pop
+    int prim;        // packed index of the best primitive
+    bool found;
+    bool retried;
+
+    ZG_DEV void begin(float bound2, float relax2_, float guard2_) {
+        best2 = bound2; relax2 = relax2_; guard2 = guard2_;
+        prune2 = best2 > guard2 ? best2 * relax2 : best2;
+        lower2 = FLT_MAX; node = 0; prim = -1; found = false; retried = false;
+    }
+    ZG_DEV float radius2() const { return fminf(best2, lower2); }
+
+    // visits one node; returns true when the query is complete
+    template <int W, bool COUNT>
+    ZG_DEV bool visit(const BoundaryView& b, const Vec<D>& x, Stack& st, VisitCount& vc) {
+        if (b.nNodes == 0) return true;
+        if (node < 0) {
+            int n; float dn;
+            bool got = false;
+            while (st.pop(n, dn)) {
+                if (dn <= prune2) { got = true; break; }
+                if (dn <= best2) lower2 = fminf(lower2, dn);
+            }
+            if (!got) {
+                if (found || retried) return true;
+                // nothing within the initial bound: search again without it
+                retried = true;
+                best2 = FLT_MAX; prune2 = FLT_MAX; lower2 = FLT_MAX;
+                n = 0;
+            }
+            node = n;
+        }
         float d2[W];
         int ch[W];
         childDistances<D, W>(b.nodes, node, x, d2, ch);
@@ -501,7 +534,6 @@ ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, V
                 inner |= 1u << bk;
             }
         }
-        // drop inner children pruned by the leaves just tested
 #pragma unroll
         for (int k = 0; k < W; k++) {
             if (((inner >> k) & 1u) && d2[k] > prune2) {
@@ -509,9 +541,8 @@ ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, V
                 if (d2[k] <= best2) lower2 = fminf(lower2, d2[k]);
             }
         }
-        int next = -1;
+        node = -1;
         if (inner) {
-            // push all but the nearest, farthest first
             while (__builtin_popcount(inner) > 1) {
                 float m = -1.0f; int bk = 0; int c = 0;
 #pragma unroll
@@ -523,19 +554,25 @@ ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, V
                 st.push(c, m);
             }
 #pragma unroll
-            for (int k = 0; k < W; k++) if ((inner >> k) & 1u) next = ch[k];
-        } else {
-            int n; float dn;
-            while (st.pop(n, dn)) {
-                if (dn <= prune2) { next = n; break; }
-                if (dn <= best2) lower2 = fminf(lower2, dn);
-            }
+            for (int k = 0; k < W; k++) if ((inner >> k) & 1u) node = ch[k];
         }
-        if (next < 0) break;
-        node = next;
+        return false;
     }
-    if (radius2) *radius2 = fminf(best2, lower2);
-    return found;
+};
+
+// Complete closest-point query (loops ClosestQuery::visit); see ClosestQuery for the rules.
+template <int D, int W, bool COUNT>
+ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, Vec<D>& cp, int& prim,
+                         Stack& st, VisitCount& vc, float relax2 = 1.0f, float guard2 = 0.0f,
+                         float* radius2 = nullptr) {
+    ClosestQuery<D> q;
+    q.begin(best2, relax2, guard2);
+    q.cp = cp;
+    st.sp = 0; st.ovf = 0;
+    while (!q.template visit<W, COUNT>(b, x, st, vc)) {}
+    best2 = q.best2; cp = q.cp; prim = q.prim;
+    if (radius2) *radius2 = q.radius2();
+    return q.found;
 }
 
 // First hit of a ray with one boundary part, 0 <= t <= tMax.

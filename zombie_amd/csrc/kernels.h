@@ -29,7 +29,7 @@ struct WosParams {
     int32_t chunksPerPoint;
     int32_t wpl;
     int32_t regroup;
-    int32_t pad;
+    int32_t stepInterval;
     unsigned long long nClaims;
     uint64_t seed;
     uint32_t streamOffset;

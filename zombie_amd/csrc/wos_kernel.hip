@@ -112,8 +112,17 @@ ZG_DEV bool outsideDomain(const SceneView& s, const Vec<D>& x) {
     return o;
 }
 
+#ifndef ZG_WOS_WAVES_PER_EU
+#define ZG_WOS_WAVES_PER_EU 4
+#endif
+
+// One loop iteration = one BVH node visit for every lane whose distance query is in flight.
+// Lanes whose query has finished take their walk step every `stepInterval` iterations (or at
+// once when no lane of the wave is traversing), and idle lanes are refilled with new claims
+// every `regroup` iterations, so a long traversal on one lane no longer stalls the others.
 template <int D, int W, bool COUNT>
-__global__ void __launch_bounds__(256) wosKernel(WosParams P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZG_WOS_WAVES_PER_EU, 8)))
+wosKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
     VisitCount vc{0, 0};
@@ -122,12 +131,11 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
     const float sigma = P.pde.absorption_coeff;
     const int lane = __lane_id();
 
-    // claim
-    int mode = 0;          // 0 idle, 1 stepping, 2 waiting for the projection query
-    int pendingCode = 0;
+    // lane state: 0 idle, 1 step pending (dist holds the radius at x), 2 distance query in flight
+    int mode = 0;
+    int pendingCode = -1;   // >= 0: the walk ends with this code once the query (projection) is done
     int p = 0;
     int w = 0, wEnd = 0;
-    // walk
     Vec<D> x = vzero<D>(), yc = vzero<D>();
     int ycPrim = -1;
     bool ycValid = false;
@@ -135,12 +143,21 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
     int len = 0;
     Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = __builtin_sqrtf(sigma);
     Pcg32 rng; rng.state = 0; rng.inc = 1;
-    // per-claim FP64 accumulators
-    double aSum = 0.0, aSq = 0.0;
-    uint32_t aN = 0, aLen = 0;
-    // per-lane counters
-    uint32_t cnt0 = 0, cnt1 = 0, cnt2 = 0, cnt3 = 0;  // walks per completion code
-    uint32_t steps = 0, walks = 0, queries = 0;
+    ClosestQuery<D> q;
+    q.begin(FLT_MAX, 1.0f, 0.0f);
+    // block counters in LDS: walks per completion code, steps, walks started
+    __shared__ unsigned long long blk[6];
+    if (threadIdx.x < 6) blk[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t queries = 0;
+
+    auto beginQuery = [&](float bound2, float relax2, int code) {
+        q.begin(bound2, relax2, P.guard2);
+        q.cp = yc;
+        st.sp = 0; st.ovf = 0;
+        pendingCode = code;
+        mode = 2;
+    };
 
     auto startWalk = [&]() {
         x = vload<D>(P.pt + (size_t)p * D);
@@ -149,23 +166,13 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
         ball.yukawa = sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
         ycValid = false;
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(P.streamOffset + (uint32_t)w));
-        walks++;
-        if (dist > eps) { mode = 1; }
-        else { mode = 2; pendingCode = ZG_REACHED_ABSORBING_BOUNDARY; }
-    };
-
-    auto flushClaim = [&]() {
-        if (aN > 0) {
-            atomicAdd(&P.accSum[p], aSum);
-            atomicAdd(&P.accSq[p], aSq);
-            atomicAdd(&P.accN[p], (unsigned long long)aN);
-            atomicAdd(&P.accLen[p], (unsigned long long)aLen);
-        }
-        aSum = 0.0; aSq = 0.0; aN = 0; aLen = 0;
+        atomicAdd(&blk[5], 1ull);
+        if (dist > eps) { mode = 1; pendingCode = -1; }
+        else beginQuery(FLT_MAX, 1.0f, ZG_REACHED_ABSORBING_BOUNDARY);  // walk ends at once: project
     };
 
     auto finishWalk = [&](int code) {
-        cnt0 += code == 0; cnt1 += code == 1; cnt2 += code == 2; cnt3 += code == 3;
+        atomicAdd(&blk[code], 1ull);
         if (code != ZG_ESCAPED_DOMAIN) {
             float g = 0.0f;
             if ((code == ZG_REACHED_ABSORBING_BOUNDARY || code == ZG_EXCEEDED_MAX_WALK_LENGTH) &&
@@ -179,16 +186,17 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
                 }
                 g = evalField<D>(P.pde.dirichlet, yc, nrm, aligned, sigma);
             }
-            float c = T * g + S;
-            aSum += (double)c;
-            aSq += (double)c * (double)c;
-            aN++;
-            aLen += (uint32_t)len;
-            steps += (uint32_t)len;
+            // FP64 per-point accumulation of the walk estimate
+            double c = (double)(T * g + S);
+            atomicAdd(&P.accSum[p], c);
+            atomicAdd(&P.accSq[p], c * c);
+            atomicAdd(&P.accN[p], 1ull);
+            atomicAdd(&P.accLen[p], (unsigned long long)len);
+            atomicAdd(&blk[4], (unsigned long long)len);
         }
         w++;
         if (w < wEnd) startWalk();
-        else { flushClaim(); mode = 0; }
+        else mode = 0;
     };
 
     bool workLeft = true;
@@ -224,85 +232,59 @@ __global__ void __launch_bounds__(256) wosKernel(WosParams P) {
         }
         iter++;
 
-        // -------- walk step (walk_on_spheres.h:237-285) --------
-        bool query = false;
-        if (mode == 1 && dist <= eps) finishWalk(ZG_REACHED_ABSORBING_BOUNDARY);
-        if (mode == 1) {
-            float R = dist;
-            Vec<D> c = x;
-            ball.update(R);
-            if (!ws.ignore_source_contribution) {
-                float f;
-                if (P.pde.is_source_constant) {
-                    f = evalField<D>(P.pde.source, x, vzero<D>(), false, sigma);
-                } else {
-                    Vec<D> sdir = sampleUnitSphere<D>(rng);
-                    float r = ball.sampleRadius(rng);
-                    f = evalField<D>(P.pde.source, c + sdir * r, vzero<D>(), false, sigma);
-                }
-                S += T * ball.norm() * f;
-            }
-            Vec<D> dir = sampleUnitSphere<D>(rng);
-            x = c + dir * R;
-            if (ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
-            int code = -1;
-            if (outsideDomain<D>(P.scene, x)) {
-                code = ZG_ESCAPED_DOMAIN;
-            } else {
-                if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
-                    float survival = T / ws.russian_roulette_threshold;
-                    if (survival < rng.nextFloat()) { T = 0.0f; code = ZG_TERMINATED_WITH_RUSSIAN_ROULETTE; }
-                    else T = ws.russian_roulette_threshold;
-                }
-                if (code < 0) {
-                    len++;
-                    if (len >= ws.max_walk_length) {
-                        code = ZG_EXCEEDED_MAX_WALK_LENGTH;
-                    } else if (sigma > 0.0f && ws.steps_before_applying_tikhonov == len) {
-                        ball.yukawa = true;
-                    }
-                }
-            }
-            if (code == ZG_EXCEEDED_MAX_WALK_LENGTH && !ws.ignore_absorbing_boundary_contribution) {
-                mode = 2; pendingCode = code;      // terminal needs the projection of x
-            } else if (code >= 0) {
-                finishWalk(code);
-            } else {
-                query = true;
-            }
-        }
-        // -------- closest-point query: next radius, or the terminal projection --------
-        if (query || mode == 2) {
-            float best2 = FLT_MAX;
-            if (ycValid) {
-                best2 = norm2(x - yc) * 1.0001f + 1e-30f;
-            } else if (mode == 1) {
-                float r2 = 2.0f * dist;
-                best2 = r2 * r2 * 1.0001f + 1e-30f;
-            }
-            // the terminal projection (mode 2) is always exact
-            float relax2 = mode == 2 ? 1.0f : P.relax2;
-            Vec<D> q = yc; int qp = -1;
-            float rad2 = best2;
-            bool found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc, relax2, P.guard2, &rad2);
-            if (!found) {
-                best2 = FLT_MAX;
-                found = closestPoint<D, W, COUNT>(P.scene.abs, x, best2, q, qp, st, vc, relax2, P.guard2, &rad2);
-            }
+        // -------- one node of every distance query in flight --------
+        if (mode == 2 && q.template visit<W, COUNT>(P.scene.abs, x, st, vc)) {
             queries++;
-            if (found) { yc = q; ycPrim = qp; ycValid = true; dist = __builtin_sqrtf(rad2); }
+            if (q.found) { yc = q.cp; ycPrim = q.prim; ycValid = true; dist = __builtin_sqrtf(q.radius2()); }
             else { dist = FLT_MAX; ycValid = false; }
-            if (mode == 2) finishWalk(pendingCode);
+            if (pendingCode >= 0) finishWalk(pendingCode);  // terminal projection done
+            else mode = 1;
         }
+
+        // -------- walk steps (walk_on_spheres.h:237-287) --------
+        if (iter % P.stepInterval != 0 && __any(mode == 2)) continue;
+        if (mode != 1) continue;
+        if (dist <= eps) { finishWalk(ZG_REACHED_ABSORBING_BOUNDARY); continue; }
+        float R = dist;
+        Vec<D> c = x;
+        ball.update(R);
+        if (!ws.ignore_source_contribution) {
+            float f;
+            if (P.pde.is_source_constant) {
+                f = evalField<D>(P.pde.source, x, vzero<D>(), false, sigma);
+            } else {
+                Vec<D> sdir = sampleUnitSphere<D>(rng);
+                float r = ball.sampleRadius(rng);
+                f = evalField<D>(P.pde.source, c + sdir * r, vzero<D>(), false, sigma);
+            }
+            S += T * ball.norm() * f;
+        }
+        Vec<D> dir = sampleUnitSphere<D>(rng);
+        x = c + dir * R;
+        if (ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
+        if (outsideDomain<D>(P.scene, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
+        if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
+            float survival = T / ws.russian_roulette_threshold;
+            if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
+            T = ws.russian_roulette_threshold;
+        }
+        len++;
+        if (len >= ws.max_walk_length) {
+            if (ws.ignore_absorbing_boundary_contribution) finishWalk(ZG_EXCEEDED_MAX_WALK_LENGTH);
+            else beginQuery(FLT_MAX, 1.0f, ZG_EXCEEDED_MAX_WALK_LENGTH);  // terminal projection, exact
+            continue;
+        }
+        if (sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
+        // next radius: the new distance is at most |x - y_prev| (y_prev lies on the boundary)
+        float bound2;
+        if (ycValid) bound2 = norm2(x - yc) * 1.0001f + 1e-30f;
+        else { float r2 = 2.0f * R; bound2 = r2 * r2 * 1.0001f + 1e-30f; }
+        beginQuery(bound2, P.relax2, -1);
     }
 
-    // -------- per-lane counters --------
-    if (cnt0) atomicAdd(&P.counters[0], (unsigned long long)cnt0);
-    if (cnt1) atomicAdd(&P.counters[1], (unsigned long long)cnt1);
-    if (cnt2) atomicAdd(&P.counters[2], (unsigned long long)cnt2);
-    if (cnt3) atomicAdd(&P.counters[3], (unsigned long long)cnt3);
-    if (steps) atomicAdd(&P.counters[4], (unsigned long long)steps);
-    if (walks) atomicAdd(&P.counters[5], (unsigned long long)walks);
+    // -------- counters --------
+    __syncthreads();
+    if (threadIdx.x < 6 && blk[threadIdx.x]) atomicAdd(&P.counters[threadIdx.x], blk[threadIdx.x]);
     if (COUNT) {
         atomicAdd(&P.counters[6], (unsigned long long)vc.nodes);
         atomicAdd(&P.counters[7], (unsigned long long)vc.prims);

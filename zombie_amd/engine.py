@@ -99,6 +99,7 @@ class SolveOptions:
     stream_offset: int = 0
     walks_per_lane: int = 8
     regroup_interval: int = 16
+    step_interval: int = 4
     waves_per_cu: int = 0
     count_visits: bool = False
     distance_tolerance: float = 0.0
@@ -109,6 +110,7 @@ class SolveOptions:
         o.stream_offset = self.stream_offset
         o.walks_per_lane = self.walks_per_lane
         o.regroup_interval = self.regroup_interval
+        o.step_interval = self.step_interval
         o.waves_per_cu = self.waves_per_cu
         o.count_visits = 1 if self.count_visits else 0
         o.distance_tolerance = self.distance_tolerance
