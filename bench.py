@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Benchmark: 3D Poisson Walk-on-Spheres on one MI355X.
+"""Benchmark: 3D Poisson Walk-on-Spheres on one MI355X, plain and with the learned control variate.
 
 Workload (the headline configuration): icosphere level 6 (81,920 triangles) radially
 displaced by real spherical harmonics l <= 4, u = sin 2x sin 2y sin 2z, f = 12 u,
@@ -7,15 +7,16 @@ Dirichlet g = u, 1M interior points, 256 walks per point per pass, epsilon 1e-4,
 FP32 geometry, FP64 per-point accumulation.  A "step" is one 256-walk pass over all
 points (one zg_wos_solve call; inputs resident in HBM).
 
-Prints one JSON line: walk-steps/s (whole job), the roofline of the walk kernel, the RMSE
-against the analytic solution, the projected time to RMSE 1e-3, and the CPU baseline (the
-oracle restatement of the reference estimator on the host cores, on a bounded sample).
+Phases: setup (mesh, 1M interior points, distances) -> launch-shape autotune -> instrumented
+sample (BVH visits per step, exact-radius walk length) -> timed plain passes -> learned
+control variate (2M training pairs of 32-walk estimates, 5k Adam steps at batch 65,536, then
+the same timed passes with the variate) -> CPU baseline (the oracle on the host cores, on a
+bounded sample).  Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -26,6 +27,7 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "s to RMSE 1e-3 vs analytic, 1M pts: plain vs learned-CV WoS; walk-steps/s"
 HBM_PEAK_GBS = 8000.0
+TARGET_RMSE = 1e-3
 
 
 def parse():
@@ -35,17 +37,23 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--walks", type=int, default=256)
-    ap.add_argument("--bvh-width", type=int, default=8)
-    ap.add_argument("--walks-per-lane", type=int, default=8)
-    ap.add_argument("--regroup", type=int, default=8)
+    ap.add_argument("--bvh-width", type=int, default=4)
+    ap.add_argument("--leaf-size", type=int, default=2)
+    ap.add_argument("--walks-per-lane", type=int, default=1)
+    ap.add_argument("--regroup", type=int, default=32)
     ap.add_argument("--step-interval", type=int, default=4)
-    ap.add_argument("--leaf-size", type=int, default=4)
-    ap.add_argument("--tolerance", type=float, default=0.0, help="distance tolerance when not autotuning")
+    ap.add_argument("--tolerance", type=float, default=0.1, help="distance tolerance when not autotuning")
+    ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--no-learned-cv", action="store_true")
+    ap.add_argument("--cv-train-points", type=int, default=2_000_000)
+    ap.add_argument("--cv-train-walks", type=int, default=32)
+    ap.add_argument("--cv-steps", type=int, default=5000)
+    ap.add_argument("--cv-batch", type=int, default=65536)
+    ap.add_argument("--cv-grid", type=int, default=96)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for the PMC byte counters)")
-    ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-points", type=int, default=4000)
+    ap.add_argument("--cpu-points", type=int, default=16000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0x5EED)
     return ap.parse_args()
@@ -53,6 +61,18 @@ def parse():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def interior_points(torch, scene, prob, n, seed, dev):
+    from zombie_amd import problems as PB
+    pts, have = [], 0
+    while have < n:
+        cand = torch.from_numpy(PB.sample_box(prob.domain_min, prob.domain_max, max(4 * n, 1024), seed)).to(dev)
+        seed += 1
+        keep = cand[scene.inside_domain(cand).bool()]
+        pts.append(keep)
+        have += keep.shape[0]
+    return torch.cat(pts)[:n].contiguous()
 
 
 def main():
@@ -65,6 +85,7 @@ def main():
     import torch
     from zombie_amd import autotune as AT
     from zombie_amd import engine as E
+    from zombie_amd import learned_cv as LC
     from zombie_amd import problems as PB
 
     if not torch.cuda.is_available():
@@ -76,19 +97,7 @@ def main():
                     bvh_width=args.bvh_width, leaf_size=args.leaf_size, sah_bins=16)
     log(f"scene: {scene.info.n_absorbing_prims} tris, {scene.info.n_absorbing_nodes} nodes, "
         f"depth {scene.info.absorbing_depth}, {scene.info.device_bytes/1e6:.2f} MB")
-    # 1M interior points: uniform candidates in the box, kept by the GPU inside test
-    pts = []
-    have = 0
-    rng_seed = args.seed
-    while have < args.points:
-        cand = PB.sample_box(prob.domain_min, prob.domain_max, 4 * args.points, rng_seed)
-        rng_seed += 1
-        c = torch.from_numpy(cand).to(dev)
-        ins = scene.inside_domain(c).bool()
-        keep = c[ins]
-        pts.append(keep)
-        have += keep.shape[0]
-    P = torch.cat(pts)[: args.points].contiguous()
+    P = interior_points(torch, scene, prob, args.points, args.seed, dev)
     dist, _, _, _ = scene.closest_point(P, want_point=False)
     n = P.shape[0]
     dp = E.DevicePoints(pt=P, dist_to_absorbing=dist.contiguous(),
@@ -96,6 +105,8 @@ def main():
     exact = prob.exact(P.cpu().numpy())
     torch.cuda.synchronize()
     log(f"setup {time.time()-t_setup:.1f}s, {n} points")
+
+    # ---------------- launch shape ----------------
     tune = None
     scenes = {(args.bvh_width, args.leaf_size): scene}
     leaf, tol = args.leaf_size, args.tolerance
@@ -104,8 +115,7 @@ def main():
         for row in tune.table:
             log("tune: width {bvh_width} leaf {leaf_size} wpl {walks_per_lane:2d} regroup {regroup_interval:2d} "
                 "step {step_interval:2d} tol {distance_tolerance:.2f} -> {walks_per_s:10.4g} walks/s "
-                "{steps_per_s:10.4g} steps/s "
-                "len {mean_walk_length:.2f}".format(**row))
+                "{steps_per_s:10.4g} steps/s len {mean_walk_length:.2f}".format(**row))
         args.bvh_width, leaf = tune.bvh_width, tune.leaf_size
         args.walks_per_lane, args.regroup, tol = tune.walks_per_lane, tune.regroup_interval, tune.distance_tolerance
         args.step_interval = tune.step_interval
@@ -113,14 +123,13 @@ def main():
         log(f"tuned: width {args.bvh_width} leaf {leaf} wpl {args.walks_per_lane} regroup {args.regroup} "
             f"step {args.step_interval} tol {tol}")
 
-    def opts(step, count=False, tolerance=None):
-        return E.SolveOptions(seed=args.seed, stream_offset=step * args.walks, walks_per_lane=args.walks_per_lane,
-                              regroup_interval=args.regroup, step_interval=args.step_interval,
-                              waves_per_cu=args.waves_per_cu, count_visits=count,
+    def opts(step, count=False, tolerance=None, seed=None):
+        return E.SolveOptions(seed=args.seed if seed is None else seed, stream_offset=step * args.walks,
+                              walks_per_lane=args.walks_per_lane, regroup_interval=args.regroup,
+                              step_interval=args.step_interval, waves_per_cu=args.waves_per_cu, count_visits=count,
                               distance_tolerance=tol if tolerance is None else tolerance)
 
-    # instrumented passes on a sample: BVH nodes / triangles visited per walk step, and the
-    # mean walk length of exact radii (the reference's step count for the same walks)
+    # ---------------- instrumented sample ----------------
     sub = 65536
     dps = E.DevicePoints(pt=P[:sub].contiguous(), dist_to_absorbing=dp.dist_to_absorbing[:sub].contiguous(),
                          n_walks=dp.n_walks[:sub].contiguous(), n_walks_max=args.walks)
@@ -140,37 +149,38 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del src, dst
-    stats = E.DeviceStats.zeros(n)
-    for w in range(args.warmup):
-        E.wos_solve(scene, prob.pde, prob.settings, dp, E.DeviceStats.zeros(n), opts(1000 + w))
-    torch.cuda.synchronize()
-    reports = []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        reports.append(E.wos_solve(scene, prob.pde, prob.settings, dp, stats, opts(s)))
-        log(f"pass {s}: {reports[-1].total_steps} steps, {reports[-1].total_walks} walks, "
-            f"kernel {reports[-1].kernel_ms:.1f} ms")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+
+    # ---------------- timed passes ----------------
+    def timed(points, label, seed):
+        stats = E.DeviceStats.zeros(n)
+        for w in range(args.warmup):
+            E.wos_solve(scene, prob.pde, prob.settings, points, E.DeviceStats.zeros(n), opts(1000 + w, seed=seed))
+        torch.cuda.synchronize()
+        reports = []
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            reports.append(E.wos_solve(scene, prob.pde, prob.settings, points, stats, opts(s, seed=seed)))
+            log(f"{label} pass {s}: {reports[-1].total_steps} steps, {reports[-1].total_walks} walks, "
+                f"kernel {reports[-1].kernel_ms:.1f} ms")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        return dict(reports=reports, elapsed=elapsed, mean=stats.mean.cpu().numpy(),
+                    var=stats.variance().cpu().numpy(), n=stats.n.cpu().numpy())
+
+    plain = timed(dp, "plain", args.seed)
+    reports, elapsed = plain["reports"], plain["elapsed"]
     total_steps = sum(r.total_steps for r in reports)
     total_walks = sum(r.total_walks for r in reports)
     kernel_ms = sum(r.kernel_ms for r in reports) / len(reports)
     walks_per_s = total_walks / elapsed
     raw_steps_per_s = total_steps / elapsed
-    # reported in the reference's unit: the walks completed per second times the mean walk
-    # length that exact radii give (equal to the raw rate when the tolerance is 0)
+    # reported in the reference's unit: walks completed per second times the mean walk length
+    # that exact radii give (equal to the raw rate when the tolerance is 0)
     steps_per_s = walks_per_s * len_exact if tol > 0 else raw_steps_per_s
-    mean = stats.mean.cpu().numpy()
-    var = stats.variance().cpu().numpy()
-    nn = stats.n.cpu().numpy()
-    err = mean - exact
-    rmse = float(np.sqrt(np.mean(err ** 2)))
-    se_rmse = float(np.sqrt(np.mean(var / np.maximum(nn, 1))))
-    # projected walks per point for RMSE 1e-3: mean per-walk variance / 1e-6
-    walks_needed = float(np.mean(var)) / 1e-6
-    mean_len = total_steps / max(total_walks, 1)
-    s_to_rmse = walks_needed * n / walks_per_s
+    rmse = float(np.sqrt(np.mean((plain["mean"] - exact) ** 2)))
+    se_rmse = float(np.sqrt(np.mean(plain["var"] / np.maximum(plain["n"], 1))))
+    var_plain = float(np.mean(plain["var"]))
+    s_to_rmse_plain = var_plain / TARGET_RMSE ** 2 * n / walks_per_s
 
     achieved_gbs = bytes_per_step * (total_steps / len(reports)) / (kernel_ms * 1e-3) / 1e9
     # HBM/MALL traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
@@ -180,10 +190,55 @@ def main():
     if prof.exists():
         pj = json.loads(prof.read_text())
         c = pj["config"]
-        if (c["bvh_width"], c["leaf_size"], c["walks_per_lane"], c["regroup_interval"], c.get("step_interval"),
-                c["distance_tolerance"]) == (args.bvh_width, leaf, args.walks_per_lane, args.regroup,
-                                             args.step_interval, tol):
+        # traffic depends on the scene layout and the radius tolerance, not on the launch shape
+        if (c["bvh_width"], c["leaf_size"], c["distance_tolerance"]) == (args.bvh_width, leaf, tol):
             traffic = pj["traffic_bytes"] / pj["raw_steps"] * (total_steps / len(reports))
+
+    # ---------------- learned control variate ----------------
+    learned = None
+    if not args.no_learned_cv:
+        t0 = time.perf_counter()
+        TX = interior_points(torch, scene, prob, args.cv_train_points, args.seed + 7919, dev)
+        td, _, _, _ = scene.closest_point(TX, want_point=False)
+        tp = E.DevicePoints(pt=TX, dist_to_absorbing=td.contiguous(), n_walks=None, n_walks_max=0)
+        X, Y = LC.generate_training_data(scene, prob, tp, walks=args.cv_train_walks, opts=opts(0, seed=args.seed + 1))
+        torch.cuda.synchronize()
+        t_data = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        net = LC.MLP(LC.MLPConfig(in_dim=3, seed=args.seed), max_batch=args.cv_batch)
+        losses = LC.train(net, X, Y, steps=args.cv_steps, batch=args.cv_batch, lr=1e-3, seed=args.seed,
+                          log_every=max(1, args.cv_steps // 5))
+        _, g0 = net.evaluate(P, value=False)
+        grid = LC.bake_gradient_grid(net, prob.domain_min, prob.domain_max, args.cv_grid)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        log(f"learned cv: data {t_data:.1f}s, train+bake {t_train:.1f}s, losses {losses}")
+        dcv = E.DevicePoints(pt=P, dist_to_absorbing=dp.dist_to_absorbing, n_walks=dp.n_walks,
+                             n_walks_max=args.walks, cv_gradient=g0.contiguous(), cv_gradient_grid=grid)
+        cv = timed(dcv, "cv", args.seed + 2)   # independent streams from the plain passes
+        cv_walks = sum(r.total_walks for r in cv["reports"])
+        cv_wps = cv_walks / cv["elapsed"]
+        var_cv = float(np.mean(cv["var"]))
+        s_to_rmse_cv = var_cv / TARGET_RMSE ** 2 * n / cv_wps + t_data + t_train
+        # 4-SE agreement of the two estimators on a held-out 10k-point set
+        h = slice(n - 10000, n)
+        se = np.sqrt(plain["var"][h] / np.maximum(plain["n"][h], 1) + cv["var"][h] / np.maximum(cv["n"][h], 1))
+        ok = se > 0
+        z = (plain["mean"][h][ok] - cv["mean"][h][ok]) / se[ok]
+        learned = {
+            "train_data_s": t_data, "train_s": t_train, "final_loss": losses[-1][1] if losses else None,
+            "per_walk_variance_plain": var_plain, "per_walk_variance_cv": var_cv,
+            "variance_reduction": var_plain / var_cv, "walks_per_s_cv": cv_wps,
+            "rmse_vs_analytic_cv": float(np.sqrt(np.mean((cv["mean"] - exact) ** 2))),
+            "projected_s_to_rmse_1e-3_cv": s_to_rmse_cv,
+            "speedup_vs_plain_at_equal_rmse": s_to_rmse_plain / s_to_rmse_cv,
+            "heldout_points": int(ok.sum()), "heldout_beyond_4se": int((np.abs(z) > 4).sum()),
+            "heldout_z_mean": float(z.mean()), "heldout_z_std": float(z.std()),
+            "variate": "first step: exact grad c(x0) per point; later steps: grad c baked on a "
+                       f"{args.cv_grid}^3 grid (trilinear); MLP 3 -> 64 Fourier features -> 4x128 SiLU -> 1, "
+                       "bf16 MFMA, Adam lr 1e-3",
+        }
+
     result = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -209,19 +264,22 @@ def main():
         "mean_walk_length_exact": len_exact,
         "rmse_vs_analytic": rmse,
         "estimated_se_rmse": se_rmse,
-        "walks_per_point_done": int(nn.mean()),
-        "mean_walk_length": mean_len,
-        "projected_s_to_rmse_1e-3_plain": s_to_rmse,
+        "walks_per_point_done": int(plain["n"].mean()),
+        "mean_walk_length": total_steps / max(total_walks, 1),
+        "projected_s_to_rmse_1e-3_plain": s_to_rmse_plain,
+        "projected_s_to_rmse_1e-3_learned_cv": learned["projected_s_to_rmse_1e-3_cv"] if learned else None,
+        "learned_cv": learned,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": bytes_per_step, "nodes_per_step": nodes_per_step,
                      "prims_per_step": prims_per_step, "kernel_ms": kernel_ms,
-                     "note": "algorithmic bytes = BVH node + triangle bytes touched per step; the 4 MB "
-                             "scene is cache-resident, so HBM is not the binding level"},
+                     "note": "algorithmic bytes = BVH node + triangle bytes touched per step; the scene is "
+                             "cache-resident (L2/MALL), so HBM is not the binding level"},
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, P[: args.cpu_points].cpu().numpy(),
                                               dist[: args.cpu_points].cpu().numpy(), args.walks)
+        result["speedup_vs_cpu_baseline"] = steps_per_s / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)
 
 
@@ -237,7 +295,7 @@ def cpu_baseline(prob, pts, dist, walks):
     el = time.perf_counter() - t
     return {"value": rep["total_steps"] / el, "unit": "walk-steps/s", "cores": threads, "kind": "port",
             "sample": f"{len(pts)} points x {walks} walks of the same workload ({rep['total_steps']} steps, "
-                      f"{el:.1f} s)"}
+                      f"{el:.1f} s, exact radii, one PCG32 per point as in the reference)"}
 
 
 if __name__ == "__main__":

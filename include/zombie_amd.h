@@ -192,6 +192,15 @@ zg_status zg_star_radius(const zg_scene* scene, int64_t n, const float* pts,
 #define ZG_SAMPLE_ON_ABSORBING_BOUNDARY 1
 #define ZG_SAMPLE_ON_REFLECTING_BOUNDARY 2
 
+/* A vector field sampled on a regular grid over [lo, hi] (res[0] x res[1] x res[2] nodes,
+ * x fastest; 4 floats per node, the 4th unused), read with (bi/tri)linear interpolation
+ * clamped to the box.  Used as a control-variate gradient field (see cv_gradient_grid). */
+typedef struct zg_grid_field {
+    const float* data;   /* device, res[0]*res[1]*res[2]*4 floats (res[2] = 1 in 2D) */
+    int32_t res[3];
+    float lo[3], hi[3];
+} zg_grid_field;
+
 /* SamplePoint (common.h:151-195), structure of arrays, device pointers. */
 typedef struct zg_sample_points {
     int64_t n;
@@ -204,6 +213,13 @@ typedef struct zg_sample_points {
     float* first_sphere_radius;              /* [n] nullable; written when a point has no  */
                                              /*     previous estimates                     */
     const int32_t* n_walks;                  /* [n] walks per point                        */
+    const float* cv_gradient;                /* [n*dim] nullable: gradient of a control     */
+                                             /* variate c at the point; every walk adds the */
+                                             /* zero-mean term -grad c(x0).(x1 - x0) of its */
+                                             /* first step (x1 uniform on the first sphere) */
+    const zg_grid_field* cv_gradient_grid;   /* host struct, nullable: gradient field G of a */
+                                             /* control variate; every later step k adds    */
+                                             /* -T_k G(x_k).(x_{k+1} - x_k), zero mean too  */
     int32_t n_walks_max;                     /* host-side upper bound of n_walks[]         */
     int32_t reserved;
 } zg_sample_points;
@@ -266,6 +282,72 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,
                         zg_solve_report* report_host, void* stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* Batched solves: many independent instances (scene + PDE each) walked by one persistent */
+/* launch.  The reference solves one scene per call (walk_on_spheres.h:131-168); a batch  */
+/* is the same per-point estimator with a per-point instance index.  3D scenes of one BVH */
+/* width; the scenes must outlive the batch.                                              */
+/* ------------------------------------------------------------------------------------ */
+typedef struct zg_batch zg_batch;
+typedef struct zg_graph zg_graph;
+
+zg_status zg_batch_create(int32_t n_instances, const zg_scene* const* scenes_host, const zg_pde* pdes_host,
+                          zg_batch** out);
+zg_status zg_batch_destroy(zg_batch* batch);
+/* points of all instances concatenated; instance_of_point [n] (device) names each point's instance */
+zg_status zg_batch_wos_solve(const zg_batch* batch, const zg_walk_settings* settings_host,
+                             const zg_sample_points* points_host, const int32_t* instance_of_point,
+                             zg_statistics* stats, const zg_solve_options* opts_host,
+                             zg_solve_report* report_host, void* stream);
+/* The same solve captured once in a hipGraph (fill, walk and merge kernels over a workspace the
+ * graph owns; no memset nodes).  Each zg_graph_launch replays it with fresh PCG32 streams (the
+ * merge kernel advances a device-side stream offset by n_walks_max) and accumulates into the
+ * same statistics arrays, which must stay allocated.  zg_graph_read_report synchronises
+ * `stream` and returns the counters of the last replay (kernel_ms = 0). */
+zg_status zg_batch_wos_graph_create(const zg_batch* batch, const zg_walk_settings* settings_host,
+                                    const zg_sample_points* points_host, const int32_t* instance_of_point,
+                                    zg_statistics* stats, const zg_solve_options* opts_host, zg_graph** out);
+zg_status zg_graph_launch(zg_graph* graph, void* stream);
+zg_status zg_graph_read_report(zg_graph* graph, zg_solve_report* report_host, void* stream);
+zg_status zg_graph_destroy(zg_graph* graph);
+
+/* ------------------------------------------------------------------------------------ */
+/* Learned control variate: Fourier-feature MLP trained on the GPU.  No counterpart in the */
+/* reference; the nearest one is the running-mean gradient control variate of            */
+/* estimateSolutionAndGradient (walk_on_spheres.h:444-450).  Its input gradient feeds     */
+/* zg_sample_points.cv_gradient.                                                          */
+/* ------------------------------------------------------------------------------------ */
+typedef struct zg_mlp zg_mlp;
+
+typedef struct zg_mlp_config {
+    int32_t in_dim;           /* 2 or 3                                                  */
+    int32_t n_frequencies;    /* F Gaussian frequency vectors: features [sin(Bx), cos(Bx)] */
+    int32_t hidden;           /* units per hidden layer (multiple of 64, <= 512)         */
+    int32_t n_hidden_layers;  /* SiLU layers, 1..8                                       */
+    float frequency_sigma;    /* B entries ~ N(0, sigma^2)                               */
+    float reserved;
+    uint64_t seed;            /* frequencies and initial weights                         */
+} zg_mlp_config;
+
+/* max_batch: largest row count of one train/evaluate call (work buffers are sized for it) */
+zg_status zg_mlp_create(const zg_mlp_config* cfg, int64_t max_batch, zg_mlp** out);
+zg_status zg_mlp_destroy(zg_mlp* mlp);
+zg_status zg_mlp_parameter_count(const zg_mlp* mlp, int64_t* count);
+/* fp32 master parameters, per layer l = 0..n_hidden_layers: W_l (out x in, row-major) then b_l */
+zg_status zg_mlp_get_parameters(const zg_mlp* mlp, float* params_host, int64_t count);
+zg_status zg_mlp_set_parameters(zg_mlp* mlp, const float* params_host, int64_t count);
+/* frequency matrix B, n_frequencies x in_dim */
+zg_status zg_mlp_get_frequencies(const zg_mlp* mlp, float* freq_host, int64_t count);
+/* loss 0.5 mean (c(x) - y)^2 and its parameter gradient (device arrays x [batch*in_dim],
+ * y [batch]); grad_host/loss_host nullable (copied back when given, which synchronises). */
+zg_status zg_mlp_gradients(zg_mlp* mlp, const float* x, const float* y, int64_t batch, float* grad_host,
+                           float* loss_host, void* stream);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on the gradient of that loss */
+zg_status zg_mlp_train_step(zg_mlp* mlp, const float* x, const float* y, int64_t batch, float learning_rate,
+                            float* loss_host, void* stream);
+/* c(x) and grad_x c(x) for n points (any n; processed in chunks of max_batch); outputs nullable */
+zg_status zg_mlp_evaluate(zg_mlp* mlp, const float* x, int64_t n, float* value, float* gradient, void* stream);
 
 /* ------------------------------------------------------------------------------------ */
 /* Device memory helpers for callers without their own HIP runtime (ctypes, cgo, ...).   */

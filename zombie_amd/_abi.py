@@ -100,6 +100,10 @@ class zg_scene_info(C.Structure):
     ]
 
 
+class zg_grid_field(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("res", C.c_int32 * 3), ("lo", C.c_float * 3), ("hi", C.c_float * 3)]
+
+
 class zg_sample_points(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -111,6 +115,8 @@ class zg_sample_points(C.Structure):
         ("estimate_boundary_normal_aligned", C.c_void_p),
         ("first_sphere_radius", C.c_void_p),
         ("n_walks", C.c_void_p),
+        ("cv_gradient", C.c_void_p),
+        ("cv_gradient_grid", C.POINTER(zg_grid_field)),
         ("n_walks_max", C.c_int32),
         ("reserved", C.c_int32),
     ]
@@ -152,6 +158,18 @@ class zg_solve_report(C.Structure):
     ]
 
 
+class zg_mlp_config(C.Structure):
+    _fields_ = [
+        ("in_dim", C.c_int32),
+        ("n_frequencies", C.c_int32),
+        ("hidden", C.c_int32),
+        ("n_hidden_layers", C.c_int32),
+        ("frequency_sigma", C.c_float),
+        ("reserved", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/zombie_amd.h
 _P = C.c_void_p
 SIGNATURES = {
@@ -169,6 +187,26 @@ SIGNATURES = {
                                  C.POINTER(zg_statistics), C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),
     "zg_wost_solve": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points),
                                   C.POINTER(zg_statistics), C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),
+    "zg_batch_create": (C.c_int32, [C.c_int32, _P, C.POINTER(zg_pde), C.POINTER(C.c_void_p)]),
+    "zg_batch_destroy": (C.c_int32, [_P]),
+    "zg_batch_wos_solve": (C.c_int32, [_P, C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points), _P,
+                                       C.POINTER(zg_statistics), C.POINTER(zg_solve_options),
+                                       C.POINTER(zg_solve_report), _P]),
+    "zg_batch_wos_graph_create": (C.c_int32, [_P, C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points), _P,
+                                              C.POINTER(zg_statistics), C.POINTER(zg_solve_options),
+                                              C.POINTER(C.c_void_p)]),
+    "zg_graph_launch": (C.c_int32, [_P, _P]),
+    "zg_graph_read_report": (C.c_int32, [_P, C.POINTER(zg_solve_report), _P]),
+    "zg_graph_destroy": (C.c_int32, [_P]),
+    "zg_mlp_create": (C.c_int32, [C.POINTER(zg_mlp_config), C.c_int64, C.POINTER(C.c_void_p)]),
+    "zg_mlp_destroy": (C.c_int32, [_P]),
+    "zg_mlp_parameter_count": (C.c_int32, [_P, C.POINTER(C.c_int64)]),
+    "zg_mlp_get_parameters": (C.c_int32, [_P, _P, C.c_int64]),
+    "zg_mlp_set_parameters": (C.c_int32, [_P, _P, C.c_int64]),
+    "zg_mlp_get_frequencies": (C.c_int32, [_P, _P, C.c_int64]),
+    "zg_mlp_gradients": (C.c_int32, [_P, _P, _P, C.c_int64, _P, _P, _P]),
+    "zg_mlp_train_step": (C.c_int32, [_P, _P, _P, C.c_int64, C.c_float, _P, _P]),
+    "zg_mlp_evaluate": (C.c_int32, [_P, _P, C.c_int64, _P, _P, _P]),
     "zg_device_count": (C.c_int32, [C.POINTER(C.c_int32)]),
     "zg_malloc": (C.c_int32, [C.POINTER(C.c_void_p), C.c_size_t]),
     "zg_free": (C.c_int32, [_P]),

@@ -104,3 +104,45 @@ def autotune(problem, points: E.DevicePoints, scenes: dict, widths=(4, 8), leaf_
             cur = row
     return TuneResult(cur["bvh_width"], cur["leaf_size"], cur["walks_per_lane"], cur["regroup_interval"],
                       cur["step_interval"], cur["distance_tolerance"], cur["walks_per_s"], cur["steps_per_s"], table)
+
+
+def autotune_batch(problems, points: E.DevicePoints, instance_of_point, leaf_size: int = 2, widths=(4, 8),
+                   walks_per_lane=(1, 2, 4), regroups=(8, 16, 32), step_interval: int = 4,
+                   distance_tolerance: float = 0.1, sample: int = 65536, walks: int = 32, seed: int = 4321,
+                   batches: dict | None = None) -> TuneResult:
+    """Exhaustive search over BVH width x walks per lane x regroup interval for a batch of
+    instances (one E.Problem each) solved by one launch.  `batches` caches width -> (scenes,
+    E.Batch) across calls.  The sample is the first `sample` points of the concatenation."""
+    import torch
+    batches = {} if batches is None else batches
+    n = min(sample, points.n)
+    sub = E.DevicePoints(pt=points.pt[:n].contiguous(), dist_to_absorbing=points.dist_to_absorbing[:n].contiguous(),
+                         n_walks=torch.full((n,), walks, dtype=torch.int32, device=points.pt.device),
+                         n_walks_max=walks)
+    inst = instance_of_point[:n].contiguous()
+
+    def batch_for(width):
+        if width not in batches:
+            scenes = [E.Scene(p.positions, p.indices, None, p.domain_min, p.domain_max, bvh_width=width,
+                              leaf_size=leaf_size) for p in problems]
+            batches[width] = (scenes, E.Batch(scenes, [p.pde for p in problems]))
+        return batches[width][1]
+
+    table = []
+    settings = problems[0].settings
+    for width in widths:   # warm every kernel variant once
+        E.batch_wos_solve(batch_for(width), settings, sub, inst, E.DeviceStats.zeros(n), E.SolveOptions(seed=seed))
+    best = None
+    for width, wpl, rg in itertools.product(widths, walks_per_lane, regroups):
+        opts = E.SolveOptions(seed=seed, walks_per_lane=wpl, regroup_interval=rg, step_interval=step_interval,
+                              distance_tolerance=distance_tolerance)
+        rep = E.batch_wos_solve(batch_for(width), settings, sub, inst, E.DeviceStats.zeros(n), opts)
+        t = max(rep.kernel_ms * 1e-3, 1e-9)
+        row = {"bvh_width": width, "leaf_size": leaf_size, "walks_per_lane": wpl, "regroup_interval": rg,
+               "step_interval": step_interval, "distance_tolerance": distance_tolerance,
+               "walks_per_s": rep.total_walks / t, "steps_per_s": rep.total_steps / t, "kernel_ms": rep.kernel_ms}
+        table.append(row)
+        if best is None or row["walks_per_s"] > best["walks_per_s"]:
+            best = row
+    return TuneResult(best["bvh_width"], leaf_size, best["walks_per_lane"], best["regroup_interval"], step_interval,
+                      distance_tolerance, best["walks_per_s"], best["steps_per_s"], table)

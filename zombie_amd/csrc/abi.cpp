@@ -15,6 +15,9 @@
 #include "../../include/zombie_amd.h"
 #include "bvh_build.h"
 #include "kernels.h"
+#include "mlp.h"
+
+#include <random>
 
 using namespace zg;
 
@@ -56,6 +59,13 @@ int cuCount(int device) {
 }
 
 }  // namespace
+
+struct zg_mlp {
+    MlpDevice d{};
+    zg_mlp_config cfg{};
+    int device = 0;
+    std::vector<void*> allocations;
+};
 
 struct DeviceBoundary {
     float* nodes = nullptr;
@@ -281,19 +291,61 @@ zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const f
     return fail(ZG_ERR_UNSUPPORTED, "zg_star_radius is not implemented yet");
 }
 
-zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_settings* ws, const zg_sample_points* pts,
-                       zg_statistics* stats, const zg_solve_options* opts, zg_solve_report* report, void* stream) {
-    if (!s || !pde || !pts || !stats) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+}  // extern "C"
+
+// A batch of instances walked by one persistent launch (the scenes stay owned by the caller).
+struct zg_batch {
+    int n = 0, dim = 3, width = 8, device = 0, maxDepth = 0;
+    std::vector<const zg_scene*> scenes;
+    std::vector<zg_pde> pdes;
+    InstanceDesc* table = nullptr;
+};
+
+// A solve captured in a hipGraph: fill, walk and merge kernels over a library-owned workspace.
+struct zg_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t* workspace = nullptr;
+    uint32_t* streamOffset = nullptr;
+    int64_t n = 0;
+    int depth = 0, width = 0;
+};
+
+namespace {
+
+struct SolveSetup {
+    WosParams P;
+    MergeParams M;
+    int dim = 3, width = 8, blocks = 1, depth = 0;
+    bool batch = false, count = false;
+    int64_t n = 0;
+    int nWalksMax = 0;
+    size_t words = 0;
+};
+
+zg_status checkPde(const zg_pde* pde, int dim) {
+    if (pde->absorption_coeff < 0.0f) return fail(ZG_ERR_INVALID_ARGUMENT, "absorption_coeff must be >= 0");
+    if (pde->absorption_coeff > 0.0f && dim == 2)
+        return fail(ZG_ERR_UNSUPPORTED, "screened Poisson (absorption > 0) is supported in 3D only");
+    return ZG_OK;
+}
+
+// validation and kernel parameters of a walk-on-spheres solve (no allocation, no launch)
+zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, const zg_walk_settings* ws,
+                       const zg_sample_points* pts, const int32_t* instanceOf, zg_statistics* stats,
+                       const zg_solve_options* opts, SolveSetup& S) {
+    if ((!s && !b) || !pts || !stats || (s && !pde)) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
     zg_status st = checkSettings(ws);
     if (st != ZG_OK) return st;
-    if (pde->absorption_coeff < 0.0f) return fail(ZG_ERR_INVALID_ARGUMENT, "absorption_coeff must be >= 0");
-    if (pde->absorption_coeff > 0.0f && s->dim == 2)
-        return fail(ZG_ERR_UNSUPPORTED, "screened Poisson (absorption > 0) is supported in 3D only");
+    int dim = s ? s->dim : b->dim;
+    int sceneDevice = s ? s->device : b->device;
+    if (s) { st = checkPde(pde, dim); if (st != ZG_OK) return st; }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))
         return fail(ZG_ERR_INVALID_ARGUMENT, "pt, dist_to_absorbing_boundary and n_walks are required");
     if (pts->n > 0 && (!stats->n_solution_estimates || !stats->solution_mean || !stats->solution_m2))
         return fail(ZG_ERR_INVALID_ARGUMENT, "statistics arrays are required");
+    if (b && pts->n > 0 && !instanceOf) return fail(ZG_ERR_INVALID_ARGUMENT, "instance_of_point is required");
     if (pts->n_walks_max < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "n_walks_max must be >= 0");
     if (pts->n >= (1ll << 31)) return fail(ZG_ERR_INVALID_ARGUMENT, "at most 2^31-1 points per call");
     zg_solve_options o;
@@ -306,62 +358,117 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
         return fail(ZG_ERR_INVALID_ARGUMENT, "distance_tolerance must be in [0, 0.5)");
     if ((uint64_t)o.stream_offset + (uint64_t)pts->n_walks_max > 0xffffffffull)
         return fail(ZG_ERR_INVALID_ARGUMENT, "stream_offset + n_walks_max must fit in 32 bits");
-    hipStream_t strm = (hipStream_t)stream;
     int dev = 0;
     ZG_HIP(hipGetDevice(&dev));
-    if (dev != s->device) return fail(ZG_ERR_INVALID_ARGUMENT, "scene lives on device %d, current device is %d", s->device, dev);
-    int64_t n = pts->n;
-    if (n == 0) { if (report) std::memset(report, 0, sizeof(*report)); return ok(); }
+    if (dev != sceneDevice) return fail(ZG_ERR_INVALID_ARGUMENT, "scene lives on device %d, current device is %d", sceneDevice, dev);
 
-    // workspace: per-point batch sums + counters (stream-ordered allocation)
-    size_t words = (size_t)n * 4 + kCntSlots;
-    uint64_t* wsp = nullptr;
-    ZG_HIP(hipMallocAsync((void**)&wsp, words * 8, strm));
-    hipError_t e = launchFillZero(wsp, (int64_t)words, strm);
-    WosParams P;
+    WosParams& P = S.P;
     std::memset(&P, 0, sizeof(P));
-    P.scene = s->view; P.pde = *pde; P.ws = *ws;
+    // control-variate gradient grid
+    if (pts->cv_gradient_grid && pts->cv_gradient_grid->data) {
+        const zg_grid_field& gf = *pts->cv_gradient_grid;
+        for (int a = 0; a < 3; a++) {
+            int r = gf.res[a];
+            if (r < 1 || (a < dim && r < 2) || (a >= dim && r != 1))
+                return fail(ZG_ERR_INVALID_ARGUMENT, "cv_gradient_grid res[%d] = %d is invalid", a, r);
+            if (a < dim && !(gf.hi[a] > gf.lo[a])) return fail(ZG_ERR_INVALID_ARGUMENT, "cv_gradient_grid box is empty");
+            P.cvRes[a] = r;
+            P.cvLo[a] = gf.lo[a];
+            P.cvScale[a] = a < dim ? (float)(r - 1) / (gf.hi[a] - gf.lo[a]) : 0.0f;
+        }
+        P.cvGrid = reinterpret_cast<const float4*>(gf.data);
+    }
+    if (s) { P.scene = s->view; P.pde = *pde; }
+    else { P.instances = b->table; P.instanceOf = instanceOf; P.nInstances = b->n; }
+    P.ws = *ws;
+    int64_t n = pts->n;
     P.pt = pts->pt; P.normal = pts->normal; P.type = pts->type; P.aligned = pts->estimate_boundary_normal_aligned;
     P.dist0 = pts->dist_to_absorbing_boundary; P.nWalks = pts->n_walks; P.nPoints = n;
+    P.cvGrad = pts->cv_gradient;
     P.wpl = o.walks_per_lane; P.regroup = o.regroup_interval; P.stepInterval = o.step_interval;
     P.chunksPerPoint = std::max(1, (pts->n_walks_max + o.walks_per_lane - 1) / o.walks_per_lane);
     P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
     P.seed = o.seed; P.streamOffset = o.stream_offset;
     P.relax2 = (1.0f - o.distance_tolerance) * (1.0f - o.distance_tolerance);
-    {
-        float e2 = 2.0f * ws->epsilon_shell_for_absorbing_boundary;
-        P.guard2 = e2 * e2 / P.relax2;
-    }
-    P.accSum = reinterpret_cast<double*>(wsp);
-    P.accSq = reinterpret_cast<double*>(wsp + n);
-    P.accN = reinterpret_cast<unsigned long long*>(wsp + 2 * n);
-    P.accLen = reinterpret_cast<unsigned long long*>(wsp + 3 * n);
-    P.counters = reinterpret_cast<unsigned long long*>(wsp + 4 * n);
-    P.claimCounter = P.counters + kCntClaims;
-    int wavesPerCu = o.waves_per_cu > 0 ? o.waves_per_cu : 32;
-    int blocks = std::max(1, cuCount(dev) * wavesPerCu / 4);
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (e == hipSuccess && report) {
-        e = hipEventCreate(&ev0);
-        if (e == hipSuccess) e = hipEventCreate(&ev1);
-        if (e == hipSuccess) e = hipEventRecord(ev0, strm);
-    }
-    if (e == hipSuccess && pts->n_walks_max > 0) e = launchWos(P, s->dim, s->width, o.count_visits != 0, blocks, strm);
-    if (e == hipSuccess && report) e = hipEventRecord(ev1, strm);
-    MergeParams M;
+    float e2 = 2.0f * ws->epsilon_shell_for_absorbing_boundary;
+    P.guard2 = e2 * e2 / P.relax2;
+
+    MergeParams& M = S.M;
     std::memset(&M, 0, sizeof(M));
     M.n = n; M.pt = pts->pt; M.type = pts->type; M.aligned = pts->estimate_boundary_normal_aligned;
     M.dist0 = pts->dist_to_absorbing_boundary;
-    M.accSum = P.accSum; M.accSq = P.accSq; M.accN = P.accN; M.accLen = P.accLen;
     M.sN = stats->n_solution_estimates; M.sMean = stats->solution_mean; M.sM2 = stats->solution_m2;
     M.sLen = stats->total_walk_length; M.firstSphereRadius = pts->first_sphere_radius;
-    M.dirichlet = pde->dirichlet; M.absorption = pde->absorption_coeff;
+    if (s) { M.dirichlet = pde->dirichlet; M.absorption = pde->absorption_coeff; }
+    else { M.instances = b->table; M.instanceOf = instanceOf; M.nInstances = b->n; }
     M.ignoreAbsorbing = ws->ignore_absorbing_boundary_contribution;
     M.solveDoubleSided = ws->solve_double_sided;
     M.writeFirstSphereRadius = 1;
-    if (e == hipSuccess) e = launchMergeStats(M, s->dim, strm);
+
+    S.dim = dim; S.width = s ? s->width : b->width; S.depth = s ? s->info.absorbing_depth : b->maxDepth;
+    S.batch = b != nullptr; S.count = o.count_visits != 0; S.n = n; S.nWalksMax = pts->n_walks_max;
+    S.words = (size_t)n * 4 + kCntSlots;
+    int wavesPerCu = o.waves_per_cu > 0 ? o.waves_per_cu : 32;
+    S.blocks = std::max(1, cuCount(dev) * wavesPerCu / 4);
+    return ZG_OK;
+}
+
+void bindWorkspace(SolveSetup& S, uint64_t* wsp) {
+    int64_t n = S.n;
+    S.P.accSum = reinterpret_cast<double*>(wsp);
+    S.P.accSq = reinterpret_cast<double*>(wsp + n);
+    S.P.accN = reinterpret_cast<unsigned long long*>(wsp + 2 * n);
+    S.P.accLen = reinterpret_cast<unsigned long long*>(wsp + 3 * n);
+    S.P.counters = reinterpret_cast<unsigned long long*>(wsp + 4 * n);
+    S.P.claimCounter = S.P.counters + kCntClaims;
+    S.M.accSum = S.P.accSum; S.M.accSq = S.P.accSq; S.M.accN = S.P.accN; S.M.accLen = S.P.accLen;
+}
+
+// fill (a kernel, so the sequence can be captured in a graph), walk, merge
+hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent_t ev0, hipEvent_t ev1) {
+    bindWorkspace(S, wsp);
+    hipError_t e = launchFillZero(wsp, (int64_t)S.words, strm);
+    if (e == hipSuccess && ev0) e = hipEventRecord(ev0, strm);
+    if (e == hipSuccess && S.nWalksMax > 0)
+        e = S.batch ? launchWosBatch(S.P, S.width, S.count, S.blocks, strm)
+                    : launchWos(S.P, S.dim, S.width, S.count, S.blocks, strm);
+    if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
+    if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
+    return e;
+}
+
+zg_status checkErrorFlags(unsigned long long flags, int depth, int width) {
+    if (flags & 2u)
+        return fail(ZG_ERR_INVALID_ARGUMENT, "instance_of_point holds an index outside [0, n_instances); those points were skipped");
+    if (flags & 1u) return fail(ZG_ERR_DEVICE, "BVH traversal stack overflow (depth %d, width %d)", depth, width);
+    return ZG_OK;
+}
+
+void fillReport(const unsigned long long* cnt, float ms, zg_solve_report* report) {
+    for (int k = 0; k < 4; k++) report->completion_codes[k] = (int64_t)cnt[k];
+    report->total_steps = (int64_t)cnt[kCntSteps];
+    report->total_walks = (int64_t)cnt[kCntWalks];
+    report->node_visits = (int64_t)cnt[kCntNodes];
+    report->prim_visits = (int64_t)cnt[kCntPrims];
+    report->queries = (int64_t)cnt[kCntQueries];
+    report->kernel_ms = ms;
+    report->reserved = 0.0f;
+}
+
+// eager solve: stream-ordered workspace, launch, counters back, synchronise
+zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
+    if (S.n == 0) { if (report) std::memset(report, 0, sizeof(*report)); return ok(); }
+    uint64_t* wsp = nullptr;
+    ZG_HIP(hipMallocAsync((void**)&wsp, S.words * 8, strm));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipError_t e = hipSuccess;
+    if (report) {
+        e = hipEventCreate(&ev0);
+        if (e == hipSuccess) e = hipEventCreate(&ev1);
+    }
+    if (e == hipSuccess) e = enqueueSolve(S, wsp, strm, ev0, ev1);
     unsigned long long cnt[kCntSlots] = {0};
-    if (e == hipSuccess) e = hipMemcpyAsync(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost, strm);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt, S.P.counters, sizeof(cnt), hipMemcpyDeviceToHost, strm);
     if (e == hipSuccess) e = hipStreamSynchronize(strm);
     float ms = 0.0f;
     if (e == hipSuccess && report) e = hipEventElapsedTime(&ms, ev0, ev1);
@@ -370,17 +477,319 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     hipError_t ef = hipFreeAsync(wsp, strm);
     if (e == hipSuccess) e = ef;
     if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "walk-on-spheres launch failed: %s", hipGetErrorString(e));
-    if (cnt[kCntError] & 1u)
-        return fail(ZG_ERR_DEVICE, "BVH traversal stack overflow (depth %d, width %d)", s->info.absorbing_depth, s->width);
-    if (report) {
-        for (int k = 0; k < 4; k++) report->completion_codes[k] = (int64_t)cnt[k];
-        report->total_steps = (int64_t)cnt[kCntSteps];
-        report->total_walks = (int64_t)cnt[kCntWalks];
-        report->node_visits = (int64_t)cnt[kCntNodes];
-        report->prim_visits = (int64_t)cnt[kCntPrims];
-        report->queries = (int64_t)cnt[kCntQueries];
-        report->kernel_ms = ms;
-        report->reserved = 0.0f;
+    zg_status est = checkErrorFlags(cnt[kCntError], S.depth, S.width);
+    if (est != ZG_OK) return est;
+    if (report) fillReport(cnt, ms, report);
+    return ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_settings* ws, const zg_sample_points* pts,
+                       zg_statistics* stats, const zg_solve_options* opts, zg_solve_report* report, void* stream) {
+    if (!s) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL scene");
+    SolveSetup S;
+    zg_status st = prepareSolve(s, nullptr, pde, ws, pts, nullptr, stats, opts, S);
+    if (st != ZG_OK) return st;
+    return runSolve(S, report, (hipStream_t)stream);
+}
+
+zg_status zg_batch_create(int32_t nInstances, const zg_scene* const* scenes, const zg_pde* pdes, zg_batch** out) {
+    if (!out || !scenes || !pdes || nInstances < 1) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    *out = nullptr;
+    zg_batch* b = new zg_batch();
+    b->n = nInstances;
+    std::vector<InstanceDesc> host(nInstances);
+    for (int i = 0; i < nInstances; i++) {
+        const zg_scene* s = scenes[i];
+        if (!s) { delete b; return fail(ZG_ERR_INVALID_ARGUMENT, "scene %d is NULL", i); }
+        if (s->dim != 3) { delete b; return fail(ZG_ERR_UNSUPPORTED, "batched solves are 3D"); }
+        if (i > 0 && (s->width != b->width || s->device != b->device)) {
+            delete b;
+            return fail(ZG_ERR_INVALID_ARGUMENT, "all scenes of a batch need the same BVH width and device");
+        }
+        zg_status st = checkPde(&pdes[i], 3);
+        if (st != ZG_OK) { delete b; return st; }
+        b->width = s->width; b->device = s->device; b->maxDepth = std::max(b->maxDepth, s->info.absorbing_depth);
+        InstanceDesc& d = host[i];
+        std::memset(&d, 0, sizeof(d));
+        d.nodes = s->view.abs.nodes; d.prims = s->view.abs.prims; d.nNodes = s->view.abs.nNodes; d.nPrims = s->view.abs.nPrims;
+        for (int a = 0; a < 3; a++) { d.dmin[a] = s->view.dmin[a]; d.dmax[a] = s->view.dmax[a]; }
+        d.pde = pdes[i];
+        b->scenes.push_back(s);
+        b->pdes.push_back(pdes[i]);
+    }
+    hipError_t e = hipMalloc((void**)&b->table, sizeof(InstanceDesc) * nInstances);
+    if (e == hipSuccess) e = hipMemcpy(b->table, host.data(), sizeof(InstanceDesc) * nInstances, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (b->table) (void)hipFree(b->table);
+        delete b;
+        return fail(ZG_ERR_DEVICE, "zg_batch_create: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return ok();
+}
+
+zg_status zg_batch_destroy(zg_batch* b) {
+    if (!b) return ok();
+    if (b->table) (void)hipFree(b->table);
+    delete b;
+    return ok();
+}
+
+zg_status zg_batch_wos_solve(const zg_batch* b, const zg_walk_settings* ws, const zg_sample_points* pts,
+                             const int32_t* instanceOf, zg_statistics* stats, const zg_solve_options* opts,
+                             zg_solve_report* report, void* stream) {
+    if (!b) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL batch");
+    SolveSetup S;
+    zg_status st = prepareSolve(nullptr, b, nullptr, ws, pts, instanceOf, stats, opts, S);
+    if (st != ZG_OK) return st;
+    return runSolve(S, report, (hipStream_t)stream);
+}
+
+zg_status zg_batch_wos_graph_create(const zg_batch* b, const zg_walk_settings* ws, const zg_sample_points* pts,
+                                    const int32_t* instanceOf, zg_statistics* stats, const zg_solve_options* opts,
+                                    zg_graph** out) {
+    if (!b || !out) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    SolveSetup S;
+    zg_status st = prepareSolve(nullptr, b, nullptr, ws, pts, instanceOf, stats, opts, S);
+    if (st != ZG_OK) return st;
+    if (S.n == 0) return fail(ZG_ERR_INVALID_ARGUMENT, "empty point set");
+    zg_graph* g = new zg_graph();
+    g->n = S.n; g->depth = S.depth; g->width = S.width;
+    hipStream_t cs = nullptr;
+    hipError_t e = hipMalloc((void**)&g->workspace, S.words * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&g->streamOffset, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(g->streamOffset, &S.P.streamOffset, sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        // every replay reads its PCG32 stream offset from device memory and the merge kernel
+        // advances it by n_walks_max, so successive replays draw fresh streams
+        S.P.streamOffsetDev = g->streamOffset;
+        S.M.streamOffsetDev = g->streamOffset;
+        S.M.streamAdvance = (uint32_t)S.nWalksMax;
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t ek = enqueueSolve(S, g->workspace, cs, nullptr, nullptr);
+            e = hipStreamEndCapture(cs, &g->graph);
+            if (e == hipSuccess) e = ek;
+        }
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->workspace) (void)hipFree(g->workspace);
+        if (g->streamOffset) (void)hipFree(g->streamOffset);
+        delete g;
+        return fail(ZG_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return ok();
+}
+
+zg_status zg_graph_launch(zg_graph* g, void* stream) {
+    if (!g) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL graph");
+    ZG_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return ok();
+}
+
+zg_status zg_graph_read_report(zg_graph* g, zg_solve_report* report, void* stream) {
+    if (!g || !report) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    unsigned long long cnt[kCntSlots] = {0};
+    hipStream_t s = (hipStream_t)stream;
+    ZG_HIP(hipMemcpyAsync(cnt, g->workspace + 4 * g->n, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    ZG_HIP(hipStreamSynchronize(s));
+    zg_status est = checkErrorFlags(cnt[kCntError], g->depth, g->width);
+    if (est != ZG_OK) return est;
+    fillReport(cnt, 0.0f, report);
+    return ok();
+}
+
+zg_status zg_graph_destroy(zg_graph* g) {
+    if (!g) return ok();
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->workspace) (void)hipFree(g->workspace);
+    if (g->streamOffset) (void)hipFree(g->streamOffset);
+    delete g;
+    return ok();
+}
+
+
+// ------------------------------------------------------------------------------------
+// learned control variate (MLP)
+// ------------------------------------------------------------------------------------
+static void mlpFree(zg_mlp* m) {
+    for (void* p : m->allocations) (void)hipFree(p);
+    m->allocations.clear();
+}
+
+}  // extern "C"
+
+template <class T>
+static hipError_t mlpAlloc(zg_mlp* m, T** p, int64_t count) {
+    *p = nullptr;
+    hipError_t e = hipMalloc((void**)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) m->allocations.push_back((void*)*p);
+    return e;
+}
+
+extern "C" {
+
+zg_status zg_mlp_create(const zg_mlp_config* cfg, int64_t maxBatch, zg_mlp** out) {
+    if (!cfg || !out) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (cfg->in_dim != 2 && cfg->in_dim != 3) return fail(ZG_ERR_INVALID_ARGUMENT, "in_dim must be 2 or 3");
+    if (cfg->n_frequencies < 1 || cfg->n_frequencies > 256) return fail(ZG_ERR_INVALID_ARGUMENT, "n_frequencies must be 1..256");
+    if (cfg->hidden < 1 || cfg->hidden > 512) return fail(ZG_ERR_INVALID_ARGUMENT, "hidden must be 1..512");
+    if (cfg->n_hidden_layers < 1 || cfg->n_hidden_layers > kMaxHidden) return fail(ZG_ERR_INVALID_ARGUMENT, "n_hidden_layers must be 1..8");
+    if (maxBatch < 1 || maxBatch > (1ll << 26)) return fail(ZG_ERR_INVALID_ARGUMENT, "max_batch must be 1..2^26");
+    zg_mlp* m = new zg_mlp();
+    m->cfg = *cfg;
+    MlpDevice& d = m->d;
+    d.shape = MlpShape{cfg->in_dim, cfg->n_frequencies, cfg->hidden, cfg->n_hidden_layers};
+    int L = cfg->n_hidden_layers, H = cfg->hidden, F2 = 2 * cfg->n_frequencies;
+    int64_t off = 0;
+    for (int l = 0; l <= L; l++) {
+        int in = l == 0 ? F2 : H, outd = l == L ? 1 : H;
+        d.wOffset[l] = off; off += (int64_t)in * outd;
+        d.bOffset[l] = off; off += outd;
+    }
+    d.nParams = off;
+    d.maxRows = maxBatch;
+    // host initialisation: PyTorch-style U(-1/sqrt(in), 1/sqrt(in)) weights and biases,
+    // Gaussian frequencies
+    std::mt19937_64 gen(cfg->seed);
+    std::vector<float> theta(off), freq((size_t)cfg->n_frequencies * cfg->in_dim);
+    std::vector<int64_t> bfIndex(off, -1);
+    for (int l = 0; l <= L; l++) {
+        int in = l == 0 ? F2 : H, outd = l == L ? 1 : H;
+        std::uniform_real_distribution<float> u(-1.0f / std::sqrt((float)in), 1.0f / std::sqrt((float)in));
+        for (int64_t i = 0; i < (int64_t)in * outd; i++) {
+            theta[d.wOffset[l] + i] = u(gen);
+            if (l < L) bfIndex[d.wOffset[l] + i] = d.wOffset[l] + i;
+        }
+        for (int i = 0; i < outd; i++) theta[d.bOffset[l] + i] = u(gen);
+    }
+    std::normal_distribution<float> nd(0.0f, cfg->frequency_sigma);
+    for (auto& f : freq) f = nd(gen);
+    hipError_t e = hipGetDevice(&m->device);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.theta, off);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.grad, off);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.adamM, off);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.adamV, off);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.wBf, off);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.bfIndex, off);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.freq, (int64_t)freq.size());
+    if (e == hipSuccess) e = mlpAlloc(m, &d.phi, maxBatch * F2);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.dphi, maxBatch * F2);
+    for (int l = 0; l < L && e == hipSuccess; l++) {
+        e = mlpAlloc(m, &d.pre[l], maxBatch * H);
+        if (e == hipSuccess) e = mlpAlloc(m, &d.act[l], maxBatch * H);
+    }
+    if (e == hipSuccess) e = mlpAlloc(m, &d.dz[0], maxBatch * H);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.dz[1], maxBatch * H);
+    if (e == hipSuccess) e = mlpAlloc(m, &d.loss, 1);
+    if (e == hipSuccess) e = hipMemcpy(d.theta, theta.data(), off * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.bfIndex, bfIndex.data(), off * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.freq, freq.data(), freq.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d.adamM, 0, off * 4);
+    if (e == hipSuccess) e = hipMemset(d.adamV, 0, off * 4);
+    if (e == hipSuccess) e = mlpCast(d, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        mlpFree(m);
+        delete m;
+        return fail(e == hipErrorOutOfMemory ? ZG_ERR_OUT_OF_MEMORY : ZG_ERR_DEVICE, "zg_mlp_create: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return ok();
+}
+
+zg_status zg_mlp_destroy(zg_mlp* m) {
+    if (!m) return ok();
+    mlpFree(m);
+    delete m;
+    return ok();
+}
+
+zg_status zg_mlp_parameter_count(const zg_mlp* m, int64_t* count) {
+    if (!m || !count) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    *count = m->d.nParams;
+    return ok();
+}
+
+zg_status zg_mlp_get_parameters(const zg_mlp* m, float* host, int64_t count) {
+    if (!m || !host || count != m->d.nParams) return fail(ZG_ERR_INVALID_ARGUMENT, "count must equal the parameter count");
+    ZG_HIP(hipMemcpy(host, m->d.theta, count * 4, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+zg_status zg_mlp_set_parameters(zg_mlp* m, const float* host, int64_t count) {
+    if (!m || !host || count != m->d.nParams) return fail(ZG_ERR_INVALID_ARGUMENT, "count must equal the parameter count");
+    ZG_HIP(hipMemcpy(m->d.theta, host, count * 4, hipMemcpyHostToDevice));
+    ZG_HIP(mlpCast(m->d, 0));
+    ZG_HIP(hipDeviceSynchronize());
+    return ok();
+}
+
+zg_status zg_mlp_get_frequencies(const zg_mlp* m, float* host, int64_t count) {
+    if (!m || !host || count != (int64_t)m->cfg.n_frequencies * m->cfg.in_dim)
+        return fail(ZG_ERR_INVALID_ARGUMENT, "count must equal n_frequencies * in_dim");
+    ZG_HIP(hipMemcpy(host, m->d.freq, count * 4, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+static zg_status mlpGradientsImpl(zg_mlp* m, const float* x, const float* y, int64_t batch, hipStream_t s) {
+    if (!m || !x || !y) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (batch < 1 || batch > m->d.maxRows) return fail(ZG_ERR_INVALID_ARGUMENT, "batch must be 1..max_batch");
+    ZG_HIP(mlpZero(m->d.grad, m->d.nParams, s));
+    ZG_HIP(mlpZero(m->d.loss, 1, s));
+    ZG_HIP(mlpForward(m->d, x, (int)batch, nullptr, s));
+    ZG_HIP(mlpBackward(m->d, y, (int)batch, 1, m->d.loss, nullptr, s));
+    return ZG_OK;
+}
+
+zg_status zg_mlp_gradients(zg_mlp* m, const float* x, const float* y, int64_t batch, float* gradHost, float* lossHost,
+                           void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    zg_status st = mlpGradientsImpl(m, x, y, batch, s);
+    if (st != ZG_OK) return st;
+    if (gradHost) { ZG_HIP(hipMemcpyAsync(gradHost, m->d.grad, m->d.nParams * 4, hipMemcpyDeviceToHost, s)); }
+    if (lossHost) { ZG_HIP(hipMemcpyAsync(lossHost, m->d.loss, 4, hipMemcpyDeviceToHost, s)); }
+    if (gradHost || lossHost) { ZG_HIP(hipStreamSynchronize(s)); }
+    return ok();
+}
+
+zg_status zg_mlp_train_step(zg_mlp* m, const float* x, const float* y, int64_t batch, float lr, float* lossHost,
+                            void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    zg_status st = mlpGradientsImpl(m, x, y, batch, s);
+    if (st != ZG_OK) return st;
+    ZG_HIP(mlpAdam(m->d, lr, s));
+    if (lossHost) {
+        ZG_HIP(hipMemcpyAsync(lossHost, m->d.loss, 4, hipMemcpyDeviceToHost, s));
+        ZG_HIP(hipStreamSynchronize(s));
+    }
+    return ok();
+}
+
+zg_status zg_mlp_evaluate(zg_mlp* m, const float* x, int64_t n, float* value, float* gradient, void* stream) {
+    if (!m || (n > 0 && !x)) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    int d = m->cfg.in_dim;
+    for (int64_t r0 = 0; r0 < n; r0 += m->d.maxRows) {
+        int rows = (int)std::min<int64_t>(m->d.maxRows, n - r0);
+        const float* xr = x + r0 * d;
+        ZG_HIP(mlpForward(m->d, xr, rows, nullptr, s));
+        // mode 2 writes c(x) and back-propagates dc/dc = 1 down to the Fourier features
+        ZG_HIP(mlpBackward(m->d, nullptr, rows, gradient ? 2 : 0, nullptr, value ? value + r0 : nullptr, s));
+        if (gradient) { ZG_HIP(mlpInputGrad(m->d, xr, rows, gradient + r0 * d, s)); }
     }
     return ok();
 }

@@ -475,9 +475,10 @@ struct ClosestQuery {
     }
     ZG_DEV float radius2() const { return fminf(best2, lower2); }
 
-    // visits one node; returns true when the query is complete
-    template <int W, bool COUNT>
-    ZG_DEV bool visit(const BoundaryView& b, const Vec<D>& x, Stack& st, VisitCount& vc) {
+    // visits one node of the view b (anything with nodes, prims, nNodes); returns true when
+    // the query is complete
+    template <int W, bool COUNT, class View>
+    ZG_DEV bool visit(const View& b, const Vec<D>& x, Stack& st, VisitCount& vc) {
         if (b.nNodes == 0) return true;
         if (node < 0) {
             int n; float dn;

@@ -15,6 +15,16 @@ enum CounterSlot {
     kCntClaims = 9, kCntError = 10, kCntSlots = 16
 };
 
+// one instance of a batched solve: scene part walked (absorbing boundary), bounding domain, PDE
+struct InstanceDesc {
+    const float* nodes;
+    const float* prims;
+    int32_t nNodes;
+    int32_t nPrims;
+    float dmin[3], dmax[3];
+    zg_pde pde;
+};
+
 struct WosParams {
     SceneView scene;
     zg_pde pde;
@@ -25,6 +35,10 @@ struct WosParams {
     const uint8_t* aligned;
     const float* dist0;
     const int32_t* nWalks;
+    const float* cvGrad;
+    const float4* cvGrid;       // gradient field nodes (x fastest), null when absent
+    int32_t cvRes[3];
+    float cvLo[3], cvScale[3];  // node coordinate = (x - lo) * scale
     int64_t nPoints;
     int32_t chunksPerPoint;
     int32_t wpl;
@@ -35,13 +49,16 @@ struct WosParams {
     uint32_t streamOffset;
     float relax2;        // (1 - distance_tolerance)^2
     float guard2;        // relaxation only when best^2 > guard2 (keeps the shell test exact)
-    uint32_t pad2;
+    int32_t nInstances;  // batched solves: size of the instance table
     double* accSum;
     double* accSq;
     unsigned long long* accN;
     unsigned long long* accLen;
     unsigned long long* claimCounter;
     unsigned long long* counters;
+    const InstanceDesc* instances;    // batched solves: instance table
+    const int32_t* instanceOf;        // batched solves: instance of each point
+    const uint32_t* streamOffsetDev;  // when set, replaces streamOffset (graph replays advance it)
 };
 
 struct MergeParams {
@@ -64,6 +81,11 @@ struct MergeParams {
     int32_t ignoreAbsorbing;
     int32_t solveDoubleSided;
     int32_t writeFirstSphereRadius;
+    const InstanceDesc* instances;
+    const int32_t* instanceOf;
+    int32_t nInstances;
+    uint32_t* streamOffsetDev;        // advanced by streamAdvance once the merge is done
+    uint32_t streamAdvance;
 };
 
 hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s);
@@ -73,6 +95,7 @@ hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, 
 hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
                      uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
 hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchWosBatch(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
 
 }  // namespace zg

@@ -105,12 +105,19 @@ ZG_DEV uint32_t mbcnt64(uint64_t mask) {
 }
 
 template <int D>
-ZG_DEV bool outsideDomain(const SceneView& s, const Vec<D>& x) {
+ZG_DEV bool outsideDomain(const float* dmin, const float* dmax, const Vec<D>& x) {
     bool o = false;
 #pragma unroll
-    for (int a = 0; a < D; a++) o = o || x[a] < s.dmin[a] || x[a] > s.dmax[a];
+    for (int a = 0; a < D; a++) o = o || x[a] < dmin[a] || x[a] > dmax[a];
     return o;
 }
+
+// the part of a scene a walk traverses (per lane in batched solves)
+struct WalkView {
+    const float* nodes;
+    const float* prims;
+    int nNodes;
+};
 
 #ifndef ZG_WOS_WAVES_PER_EU
 #define ZG_WOS_WAVES_PER_EU 4
@@ -120,7 +127,38 @@ ZG_DEV bool outsideDomain(const SceneView& s, const Vec<D>& x) {
 // Lanes whose query has finished take their walk step every `stepInterval` iterations (or at
 // once when no lane of the wave is traversing), and idle lanes are refilled with new claims
 // every `regroup` iterations, so a long traversal on one lane no longer stalls the others.
-template <int D, int W, bool COUNT>
+// control-variate gradient field at x: (bi/tri)linear interpolation of the grid nodes,
+// clamped to the grid box
+template <int D>
+ZG_DEV Vec<D> cvField(const WosParams& P, const Vec<D>& x) {
+    int i0[3], i1[3];
+    float t[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (a < D) {
+            float u = fminf(fmaxf((x[a < D ? a : 0] - P.cvLo[a]) * P.cvScale[a], 0.0f), (float)(P.cvRes[a] - 1));
+            int k = min((int)u, P.cvRes[a] - 2);
+            i0[a] = k; i1[a] = k + 1; t[a] = u - (float)k;
+        } else {
+            i0[a] = 0; i1[a] = 0; t[a] = 0.0f;
+        }
+    }
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int c = 0; c < (D == 3 ? 8 : 4); c++) {
+        int ix = (c & 1) ? i1[0] : i0[0], iy = (c & 2) ? i1[1] : i0[1], iz = (c & 4) ? i1[2] : i0[2];
+        float wgt = ((c & 1) ? t[0] : 1.0f - t[0]) * ((c & 2) ? t[1] : 1.0f - t[1]);
+        if (D == 3) wgt *= (c & 4) ? t[2] : 1.0f - t[2];
+        float4 v = P.cvGrid[((size_t)iz * P.cvRes[1] + iy) * P.cvRes[0] + ix];
+        acc.x += wgt * v.x; acc.y += wgt * v.y; acc.z += wgt * v.z;
+    }
+    Vec<D> g;
+    g[0] = acc.x; g[1] = acc.y;
+    if (D == 3) g[D - 1] = acc.z;
+    return g;
+}
+
+template <int D, int W, bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZG_WOS_WAVES_PER_EU, 8)))
 wosKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
@@ -128,8 +166,13 @@ wosKernel(WosParams P) {
     VisitCount vc{0, 0};
     const zg_walk_settings& ws = P.ws;
     const float eps = ws.epsilon_shell_for_absorbing_boundary;
-    const float sigma = P.pde.absorption_coeff;
     const int lane = __lane_id();
+    const uint32_t streamOffset = P.streamOffsetDev ? *P.streamOffsetDev : P.streamOffset;
+    // scene / PDE of the current walk: uniform, or the lane's instance in batched solves
+    const InstanceDesc* inst = nullptr;
+    WalkView view{P.scene.abs.nodes, P.scene.abs.prims, P.scene.abs.nNodes};
+    float sigma = P.pde.absorption_coeff;
+    auto pde = [&]() -> const zg_pde& { return BATCH ? inst->pde : P.pde; };
 
     // lane state: 0 idle, 1 step pending (dist holds the radius at x), 2 distance query in flight
     int mode = 0;
@@ -160,12 +203,18 @@ wosKernel(WosParams P) {
     };
 
     auto startWalk = [&]() {
+        if (BATCH) {
+            inst = P.instances + P.instanceOf[p];
+            view.nodes = inst->nodes; view.prims = inst->prims; view.nNodes = inst->nNodes;
+            sigma = inst->pde.absorption_coeff;
+            ball.sqrtLambda = __builtin_sqrtf(sigma);
+        }
         x = vload<D>(P.pt + (size_t)p * D);
         dist = P.dist0[p];
         T = 1.0f; S = 0.0f; len = 0;
         ball.yukawa = sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
         ycValid = false;
-        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(P.streamOffset + (uint32_t)w));
+        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
         atomicAdd(&blk[5], 1ull);
         if (dist > eps) { mode = 1; pendingCode = -1; }
         else beginQuery(FLT_MAX, 1.0f, ZG_REACHED_ABSORBING_BOUNDARY);  // walk ends at once: project
@@ -180,11 +229,11 @@ wosKernel(WosParams P) {
                 Vec<D> nrm = vzero<D>();
                 bool aligned = false;
                 if (ws.solve_double_sided && ycPrim >= 0) {
-                    Prim<D> pr; pr.load(P.scene.abs.prims, ycPrim);
+                    Prim<D> pr; pr.load(view.prims, ycPrim);
                     nrm = pr.normal();
                     aligned = dot(x - yc, nrm) > 0.0f;
                 }
-                g = evalField<D>(P.pde.dirichlet, yc, nrm, aligned, sigma);
+                g = evalField<D>(pde().dirichlet, yc, nrm, aligned, sigma);
             }
             // FP64 per-point accumulation of the walk estimate
             double c = (double)(T * g + S);
@@ -222,6 +271,10 @@ wosKernel(WosParams P) {
                         uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
                         if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
                         else if (P.dist0[p] <= eps) nw = min(nw, 1);
+                        if (BATCH && (uint32_t)P.instanceOf[p] >= (uint32_t)P.nInstances) {
+                            nw = 0;  // bad instance index: no walks, error flag for the host
+                            if (k == 0) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 2u);
+                        }
                         w = k * P.wpl;
                         wEnd = min(w + P.wpl, nw);
                         if (w < wEnd) startWalk();
@@ -233,7 +286,7 @@ wosKernel(WosParams P) {
         iter++;
 
         // -------- one node of every distance query in flight --------
-        if (mode == 2 && q.template visit<W, COUNT>(P.scene.abs, x, st, vc)) {
+        if (mode == 2 && q.template visit<W, COUNT>(view, x, st, vc)) {
             queries++;
             if (q.found) { yc = q.cp; ycPrim = q.prim; ycValid = true; dist = __builtin_sqrtf(q.radius2()); }
             else { dist = FLT_MAX; ycValid = false; }
@@ -250,19 +303,29 @@ wosKernel(WosParams P) {
         ball.update(R);
         if (!ws.ignore_source_contribution) {
             float f;
-            if (P.pde.is_source_constant) {
-                f = evalField<D>(P.pde.source, x, vzero<D>(), false, sigma);
+            if (pde().is_source_constant) {
+                f = evalField<D>(pde().source, x, vzero<D>(), false, sigma);
             } else {
                 Vec<D> sdir = sampleUnitSphere<D>(rng);
                 float r = ball.sampleRadius(rng);
-                f = evalField<D>(P.pde.source, c + sdir * r, vzero<D>(), false, sigma);
+                f = evalField<D>(pde().source, c + sdir * r, vzero<D>(), false, sigma);
             }
             S += T * ball.norm() * f;
         }
         Vec<D> dir = sampleUnitSphere<D>(rng);
         x = c + dir * R;
+        if (len == 0 && P.cvGrad) {
+            // first-step control variate: E[dir] = 0, so -grad c(x0).(x1 - x0) has zero mean
+            S -= dot(vload<D>(P.cvGrad + (size_t)p * D), dir) * R;
+        } else if (P.cvGrid) {
+            // later steps: -T G(x_k).(x_{k+1} - x_k) with G a fixed field, zero mean as well
+            S -= T * dot(cvField<D>(P, c), dir) * R;
+        }
         if (ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
-        if (outsideDomain<D>(P.scene, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
+        if (outsideDomain<D>(BATCH ? inst->dmin : P.scene.dmin, BATCH ? inst->dmax : P.scene.dmax, x)) {
+            finishWalk(ZG_ESCAPED_DOMAIN);
+            continue;
+        }
         if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
             float survival = T / ws.russian_roulette_threshold;
             if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
@@ -290,7 +353,7 @@ wosKernel(WosParams P) {
         atomicAdd(&P.counters[7], (unsigned long long)vc.prims);
         atomicAdd(&P.counters[8], (unsigned long long)queries);
     }
-    if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[10]), 1u);
+    if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 1u);
 }
 
 // per-point batch -> caller statistics (Chan merge, common.h:268-310); sample points on the
@@ -298,7 +361,9 @@ wosKernel(WosParams P) {
 template <int D>
 __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && M.streamOffsetDev) *M.streamOffsetDev += M.streamAdvance;  // next replay: fresh streams
     if (i >= M.n) return;
+    if (M.instances && (uint32_t)M.instanceOf[i] >= (uint32_t)M.nInstances) return;  // flagged by the walk kernel
     int64_t N1 = M.sN[i];
     bool hasPrev = N1 > 0;
     uint8_t ty = M.type ? M.type[i] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
@@ -309,7 +374,9 @@ __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
             float g = 0.0f;
             if (!M.ignoreAbsorbing) {
                 Vec<D> x = vload<D>(M.pt + i * D);
-                g = evalField<D>(M.dirichlet, x, vzero<D>(), M.solveDoubleSided && M.aligned && M.aligned[i], M.absorption);
+                const zg_function& gd = M.instances ? M.instances[M.instanceOf[i]].pde.dirichlet : M.dirichlet;
+                float sig = M.instances ? M.instances[M.instanceOf[i]].pde.absorption_coeff : M.absorption;
+                g = evalField<D>(gd, x, vzero<D>(), M.solveDoubleSided && M.aligned && M.aligned[i], sig);
             }
             sum = g; sq = (double)g * g; nb = 1;
         }
@@ -387,7 +454,7 @@ hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* 
 
 hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-#define ZG_WOS(D, Wd, C) hipLaunchKernelGGL((wosKernel<D, Wd, C>), g, bl, kStackLdsBytes, s, P)
+#define ZG_WOS(D, Wd, C) hipLaunchKernelGGL((wosKernel<D, Wd, C, false>), g, bl, kStackLdsBytes, s, P)
     if (dim == 2) {
         if (width == 4) { if (count) ZG_WOS(2, 4, true); else ZG_WOS(2, 4, false); }
         else { if (count) ZG_WOS(2, 8, true); else ZG_WOS(2, 8, false); }
@@ -396,6 +463,18 @@ hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gri
         else { if (count) ZG_WOS(3, 8, true); else ZG_WOS(3, 8, false); }
     }
 #undef ZG_WOS
+    return hipGetLastError();
+}
+
+hipError_t launchWosBatch(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s) {
+    dim3 g(gridBlocks), bl(256);
+    if (width == 4) {
+        if (count) hipLaunchKernelGGL((wosKernel<3, 4, true, true>), g, bl, kStackLdsBytes, s, P);
+        else hipLaunchKernelGGL((wosKernel<3, 4, false, true>), g, bl, kStackLdsBytes, s, P);
+    } else {
+        if (count) hipLaunchKernelGGL((wosKernel<3, 8, true, true>), g, bl, kStackLdsBytes, s, P);
+        else hipLaunchKernelGGL((wosKernel<3, 8, false, true>), g, bl, kStackLdsBytes, s, P);
+    }
     return hipGetLastError();
 }
 

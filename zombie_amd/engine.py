@@ -223,6 +223,8 @@ class DevicePoints:
     aligned: object = None
     dist_to_reflecting: object = None
     first_sphere_radius: object = None
+    cv_gradient: object = None
+    cv_gradient_grid: object = None     # GridField
 
     @property
     def n(self) -> int:
@@ -239,8 +241,30 @@ class DevicePoints:
         s.estimate_boundary_normal_aligned = _ptr(self.aligned)
         s.first_sphere_radius = _ptr(self.first_sphere_radius)
         s.n_walks = _ptr(self.n_walks)
+        s.cv_gradient = _ptr(self.cv_gradient)
+        if self.cv_gradient_grid is not None:
+            self._grid_c = self.cv_gradient_grid.to_c()   # keep the struct alive for the call
+            s.cv_gradient_grid = C.pointer(self._grid_c)
         s.n_walks_max = int(self.n_walks_max)
         return s
+
+
+@dataclass
+class GridField:
+    """Vector field on a regular grid: data [res_z, res_y, res_x, 4] float32 on the device."""
+    data: object
+    lo: tuple
+    hi: tuple
+
+    def to_c(self) -> A.zg_grid_field:
+        g = A.zg_grid_field()
+        g.data = _ptr(self.data)
+        rz, ry, rx = self.data.shape[0], self.data.shape[1], self.data.shape[2]
+        g.res[0], g.res[1], g.res[2] = rx, ry, rz
+        for i in range(3):
+            g.lo[i] = float(self.lo[i]) if i < len(self.lo) else 0.0
+            g.hi[i] = float(self.hi[i]) if i < len(self.hi) else 0.0
+        return g
 
 
 @dataclass
@@ -284,6 +308,88 @@ def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: Devic
     A.check(A.lib().zg_wos_solve(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st), C.byref(oc),
                                  C.byref(rep), _stream_ptr(stream)))
     return SolveReport.from_c(rep)
+
+
+class Batch:
+    """Many independent 3D instances (scene + PDE each) solved by one persistent launch.
+
+    Points of all instances are concatenated; `instance_of_point` (int32 device tensor) names
+    each point's instance.  The scenes are referenced, not copied, and must outlive the batch.
+    """
+
+    def __init__(self, scenes, pdes):
+        if len(scenes) != len(pdes) or not scenes:
+            raise ValueError("need one PDE per scene and at least one instance")
+        arr = (C.c_void_p * len(scenes))(*[s.handle.value for s in scenes])
+        parr = (A.zg_pde * len(pdes))(*pdes)
+        h = C.c_void_p()
+        A.check(A.lib().zg_batch_create(len(scenes), arr, parr, C.byref(h)))
+        self._h = h
+        self.scenes = list(scenes)
+        self.pdes = list(pdes)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            A.lib().zg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_wos_solve(batch: Batch, ws: A.zg_walk_settings, points: DevicePoints, instance_of_point,
+                    stats: DeviceStats, opts: SolveOptions | None = None, stream=None) -> SolveReport:
+    opts = opts or SolveOptions()
+    sp = points.to_c()
+    st = stats.to_c()
+    oc = opts.to_c()
+    rep = A.zg_solve_report()
+    A.check(A.lib().zg_batch_wos_solve(batch.handle, C.byref(ws), C.byref(sp), _ptr(instance_of_point),
+                                       C.byref(st), C.byref(oc), C.byref(rep), _stream_ptr(stream)))
+    return SolveReport.from_c(rep)
+
+
+class SolveGraph:
+    """A batched solve captured once in a hipGraph; every launch() replays it with fresh
+    PCG32 streams and accumulates into the same statistics (kept alive by this object)."""
+
+    def __init__(self, batch: Batch, ws: A.zg_walk_settings, points: DevicePoints, instance_of_point,
+                 stats: DeviceStats, opts: SolveOptions | None = None):
+        opts = opts or SolveOptions()
+        self._keep = (batch, points, instance_of_point, stats)
+        sp = points.to_c()
+        st = stats.to_c()
+        oc = opts.to_c()
+        h = C.c_void_p()
+        A.check(A.lib().zg_batch_wos_graph_create(batch.handle, C.byref(ws), C.byref(sp), _ptr(instance_of_point),
+                                                  C.byref(st), C.byref(oc), C.byref(h)))
+        self._h = h
+
+    def launch(self, stream=None):
+        A.check(A.lib().zg_graph_launch(self._h, _stream_ptr(stream)))
+
+    def report(self, stream=None) -> SolveReport:
+        rep = A.zg_solve_report()
+        A.check(A.lib().zg_graph_read_report(self._h, C.byref(rep), _stream_ptr(stream)))
+        return SolveReport.from_c(rep)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            A.lib().zg_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def wost_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,

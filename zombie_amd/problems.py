@@ -119,36 +119,33 @@ def icosphere(level: int):
 
 
 def geodesic_sphere(freq: int):
-    """Unit geodesic sphere of frequency `freq` (20*freq^2 triangles), welded vertices."""
+    """Unit geodesic sphere of frequency `freq` (20*freq^2 outward triangles), welded vertices."""
     V0, F0 = icosahedron()
-    keys = {}
-    verts = []
-
-    def vid(p):
-        p = p / np.linalg.norm(p)
-        k = tuple(np.round(p * 1e9).astype(np.int64))
-        i = keys.get(k)
-        if i is None:
-            i = len(verts)
-            keys[k] = i
-            verts.append(p)
-        return i
-
-    F = []
     n = freq
-    for a, b, c in F0:
+    ii, jj = [a.ravel() for a in np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")]
+    keep = ii + jj <= n
+    ii, jj = ii[keep], jj[keep]
+    lid = -np.ones((n + 1, n + 1), np.int64)
+    lid[ii, jj] = np.arange(len(ii))
+    # local triangles of one face: up (i,j),(i+1,j),(i,j+1); down (i+1,j),(i+1,j+1),(i,j+1)
+    ui, uj = [a.ravel() for a in np.meshgrid(np.arange(n), np.arange(n), indexing="ij")]
+    up = ui + uj < n
+    down = ui + uj + 1 < n
+    tri_up = np.stack([lid[ui[up], uj[up]], lid[ui[up] + 1, uj[up]], lid[ui[up], uj[up] + 1]], 1)
+    tri_dn = np.stack([lid[ui[down] + 1, uj[down]], lid[ui[down] + 1, uj[down] + 1], lid[ui[down], uj[down] + 1]], 1)
+    tri_local = np.concatenate([tri_up, tri_dn])
+    pts, tris = [], []
+    nloc = len(ii)
+    for f, (a, b, c) in enumerate(F0):
         A0, B0, C0 = V0[a], V0[b], V0[c]
-        grid = {}
-        for i in range(n + 1):
-            for j in range(n + 1 - i):
-                p = A0 + (B0 - A0) * (i / n) + (C0 - A0) * (j / n)
-                grid[(i, j)] = vid(p)
-        for i in range(n):
-            for j in range(n - i):
-                F.append([grid[(i, j)], grid[(i + 1, j)], grid[(i, j + 1)]])
-                if j + i + 1 < n:
-                    F.append([grid[(i + 1, j)], grid[(i + 1, j + 1)], grid[(i, j + 1)]])
-    return np.array(verts), np.array(F, np.int64)
+        P = A0[None] + (B0 - A0)[None] * (ii / n)[:, None] + (C0 - A0)[None] * (jj / n)[:, None]
+        pts.append(P / np.linalg.norm(P, axis=1, keepdims=True))
+        tris.append(tri_local + f * nloc)
+    P = np.concatenate(pts)
+    T = np.concatenate(tris)
+    key = np.round(P * 1e9).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    return P[first], inv.reshape(-1)[T]
 
 
 def real_sph_harm_basis(dirs: np.ndarray, lmax: int) -> np.ndarray:
