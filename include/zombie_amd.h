@@ -184,6 +184,23 @@ zg_status zg_star_radius(const zg_scene* scene, int64_t n, const float* pts,
                          float silhouette_precision, const uint8_t* flip_normal_orientation,
                          float* radius, void* stream);
 
+/* intersectsWithReflectingBoundary's visibility test: hasLineOfSight of the reflecting
+ * aggregate (fcpw_geometric_queries.h:1218-1226) between p1[i] and p2[i] (offsets, if any,
+ * applied by the caller).  visible[i] = 1 when no reflecting segment crosses the open
+ * segment.  2D only. */
+zg_status zg_line_of_sight(const zg_scene* scene, int64_t n, const float* p1, const float* p2,
+                           uint8_t* visible, void* stream);
+
+/* sampleReflectingBoundary (fcpw_geometric_queries.h:1227-1242): a point on the reflecting
+ * boundary near the ball |y - x| <= radius drawn with the random numbers u[2i], u[2i+1] by
+ * a stochastic descent of the reflecting BVH (children overlapping the ball weighted by
+ * branchTraversalWeight of their box centre, segments by that weight times their length).
+ * ok[i] = 0 when nothing overlaps; pdf[i] is the density of the sample per unit length.
+ * pt / normal / pdf nullable.  2D only. */
+zg_status zg_sample_reflecting(const zg_scene* scene, int64_t n, const float* x, const float* radius,
+                               const float* u, uint8_t* ok, float* pt, float* normal, float* pdf,
+                               void* stream);
+
 /* ------------------------------------------------------------------------------------ */
 /* Solves.                                                                               */
 /* ------------------------------------------------------------------------------------ */
@@ -277,7 +294,11 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        zg_solve_report* report_host, void* stream);
 
 /* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
- * walk :345-490); 2D scenes with a Neumann (reflecting) boundary. */
+ * walk :345-490); 2D scenes, pure Neumann reflecting conditions (h = pde.robin, Robin
+ * coefficients 0), harmonic Green's function (absorption 0).  points_host->normal, type and
+ * estimate_boundary_normal_aligned describe points on the reflecting boundary;
+ * first_sphere_radius (nullable) records the star radius of each point's first step on the
+ * first call and is reused while the point has estimates. */
 zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,

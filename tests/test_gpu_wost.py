@@ -1,0 +1,243 @@
+"""Walk on stars on the GPU (2D, Neumann reflecting boundary): the reflecting-boundary
+queries against the oracle and against closed forms, and the estimator's statistical parity
+with the oracle and the manufactured solution u = x^3 - 3xy^2."""
+import numpy as np
+import pytest
+
+from zombie_amd import _abi as A
+from zombie_amd import engine as E
+from zombie_amd import problems as PB
+
+from helpers import check_four_se, z_scores
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mixed(oracle_lib):
+    p = PB.mixed2d(n_segments=1024, n_slits=16)
+    osc = oracle_lib.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    V, I = p.meta["outer"]
+    cand = PB.sample_box(p.domain_min, p.domain_max, 4000, 9)
+    pts = cand[PB.points_in_polygon(cand, V, I)][:1200]
+    d, *_ = osc.closest_point(pts, boundary=0)
+    return p, osc, pts, d
+
+
+def _scene(p, width):
+    return E.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, bvh_width=width,
+                   leaf_size=4, solve_double_sided=True)
+
+
+def _t(torch, a, dtype=np.float32):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype)).cuda()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_star_radius_matches_oracle(gpu, mixed, width):
+    torch = gpu
+    p, osc, pts, d = mixed
+    sc = _scene(p, width)
+    eps = p.settings.epsilon_shell_for_reflecting_boundary
+    mn = np.full(len(pts), eps, np.float32)
+    g = sc.star_radius(_t(torch, pts), _t(torch, mn), _t(torch, d)).cpu().numpy()
+    o = osc.star_radius(pts, mn, d)
+    close = np.isclose(g, o, rtol=1e-5, atol=1e-6)
+    assert close.mean() > 0.995, f"{(~close).sum()} of {len(pts)} star radii differ"
+    # result in [min(eps, d), max(eps, d)] (min radius above max radius returns the max radius)
+    assert (g <= np.maximum(d, eps) * (1 + 1e-6)).all() and (g >= np.minimum(d, eps) * (1 - 1e-6)).all()
+    # invariant: no silhouette vertex lies strictly inside a radius above the clamp
+    for i in range(0, len(pts), 40):
+        v, s = osc.silhouette_candidates(pts[i])
+        ds = np.linalg.norm(v[s] - pts[i], axis=1)
+        if g[i] > eps * 1.001:
+            assert (ds >= g[i] * (1 - 1e-5)).all()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_line_of_sight_matches_oracle(gpu, mixed, width):
+    torch = gpu
+    p, osc, pts, d = mixed
+    sc = _scene(p, width)
+    rng = np.random.default_rng(3)
+    a = pts[rng.integers(0, len(pts), 1500)]
+    b = a + rng.normal(scale=0.3, size=a.shape).astype(np.float32)
+    g = sc.line_of_sight(_t(torch, a), _t(torch, b)).cpu().numpy().astype(bool)
+    o = np.array([osc.line_of_sight(a[i], b[i]) for i in range(len(a))])
+    assert (g == o).mean() > 0.999
+    assert 0.05 < g.mean() < 0.95          # both outcomes occur
+
+
+def _reflecting_segments(p):
+    sel = p.is_reflecting.astype(bool)
+    return p.positions[p.indices[sel, 0]].astype(np.float64), p.positions[p.indices[sel, 1]].astype(np.float64)
+
+
+def _clipped_length(a, b, x, R):
+    """Length of each segment a->b inside the disk |y - x| < R."""
+    d = b - a
+    f = a - x
+    A_ = (d * d).sum(1)
+    B = 2 * (f * d).sum(1)
+    Cc = (f * f).sum(1) - R * R
+    disc = B * B - 4 * A_ * Cc
+    out = np.zeros(len(a))
+    m = disc > 0
+    sq = np.sqrt(disc[m])
+    t0 = np.clip((-B[m] - sq) / (2 * A_[m]), 0, 1)
+    t1 = np.clip((-B[m] + sq) / (2 * A_[m]), 0, 1)
+    out[m] = np.maximum(t1 - t0, 0) * np.sqrt(A_[m])
+    return out
+
+
+def _box_overlap_length(a, b, x, R):
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    dd = np.maximum(np.maximum(lo - x, x - hi), 0)
+    return np.linalg.norm(b - a, axis=1)[(dd * dd).sum(1) <= R * R].sum()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_sample_reflecting_is_unbiased(gpu, mixed, width):
+    """E[1/pdf] is the length of the segments whose box meets the ball (the support of the
+    sampler), and E[1{|y-x|<R}/pdf] the reflecting length inside the ball: both pin the
+    exact density of the stochastic descent."""
+    torch = gpu
+    p, osc, pts, d = mixed
+    sc = _scene(p, width)
+    a, b = _reflecting_segments(p)
+    rng = np.random.default_rng(11)
+    ns = 200_000
+    checked = 0
+    for i in range(0, len(pts), 97):
+        x = pts[i].astype(np.float64)
+        R = float(rng.uniform(0.05, 0.6))
+        xs = np.broadcast_to(pts[i], (ns, 2))
+        u = rng.random((ns, 2), dtype=np.float32)
+        ok, y, nr, pdf = sc.sample_reflecting(_t(torch, xs), _t(torch, np.full(ns, R)), _t(torch, u))
+        ok = ok.cpu().numpy().astype(bool); y = y.cpu().numpy().astype(np.float64); pdf = pdf.cpu().numpy()
+        support = _box_overlap_length(a, b, x, R)
+        if support == 0:
+            assert not ok.any()
+            continue
+        checked += 1
+        assert (pdf[ok] > 0).all()
+        w = np.where(ok, 1.0 / np.where(ok, pdf, 1.0), 0.0)
+        se = w.std() / np.sqrt(ns)
+        assert abs(w.mean() - support) < 4 * se + 1e-6 * support, (i, w.mean(), support, se)
+        inside = ok & (np.linalg.norm(y - x, axis=1) < R)
+        wi = np.where(inside, w, 0.0)
+        clipped = _clipped_length(a, b, x, R).sum()
+        sei = wi.std() / np.sqrt(ns)
+        assert abs(wi.mean() - clipped) < 4 * sei + 1e-6, (i, wi.mean(), clipped, sei)
+    assert checked >= 5
+
+
+def _device_points(torch, pts, dist, n_walks, types=None, normals=None, fsr=None):
+    n = len(pts)
+    return E.DevicePoints(pt=_t(torch, pts), dist_to_absorbing=_t(torch, dist),
+                          n_walks=torch.full((n,), n_walks, dtype=torch.int32, device="cuda"), n_walks_max=n_walks,
+                          type=None if types is None else _t(torch, types, np.uint8),
+                          normal=None if normals is None else _t(torch, normals),
+                          first_sphere_radius=fsr)
+
+
+def _host(st):
+    return dict(n=st.n.cpu().numpy(), mean=st.mean.cpu().numpy(), var=st.variance().cpu().numpy(),
+                len=st.walk_length.cpu().numpy(), m2=st.m2.cpu().numpy())
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_wost_parity_with_oracle_and_closed_form(gpu, oracle_lib, mixed, width):
+    torch = gpu
+    O = oracle_lib
+    p, osc, pts, d = mixed
+    pts, d = pts[:400], d[:400]
+    sc = _scene(p, width)
+    st = E.DeviceStats.zeros(len(pts))
+    fsr = torch.zeros(len(pts), dtype=torch.float32, device="cuda")
+    rep = E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, 256, fsr=fsr), st, E.SolveOptions(seed=21))
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    ofsr = np.zeros(len(pts), np.float32)
+    orep = O.solve(osc, p.pde, p.settings, pts, d, 256, ost, seed=22, wost=True, rng_mode=O.RNG_PER_POINT,
+                   first_sphere_radius=ofsr)
+    assert (g["n"] == ost.n).all()
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, f"wost gpu vs oracle (width {width})")
+    exact = p.exact(pts)
+    pooled = (g["mean"] - exact).sum() / np.sqrt((g["var"] / g["n"]).sum())
+    assert abs(pooled) < 4.0, pooled
+    # the recorded first sphere radius is the oracle's (same silhouette query and shrink)
+    np.testing.assert_allclose(fsr.cpu().numpy(), ofsr, rtol=1e-5, atol=1e-6)
+    # same estimator: walk lengths and completion codes agree in distribution
+    ml_g, ml_o = g["len"].sum() / g["n"].sum(), ost.walk_length.sum() / ost.n.sum()
+    assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)
+    assert rep.completion_codes[0] > 0
+    assert rep.total_walks == 256 * len(pts)
+
+
+def test_wost_boundary_sample_points(gpu, oracle_lib, mixed):
+    """Points on the reflecting boundary (both sides of the double-sided slits) and on the
+    absorbing boundary."""
+    torch = gpu
+    O = oracle_lib
+    p, osc, _, _ = mixed
+    sc = _scene(p, 8)
+    refl = p.is_reflecting.astype(bool)
+    a, b = p.positions[p.indices[:, 0]], p.positions[p.indices[:, 1]]
+    mid = 0.5 * (a + b)
+    s = b - a
+    nrm = np.stack([s[:, 1], -s[:, 0]], 1) / np.linalg.norm(s, axis=1, keepdims=True)
+    ridx = np.nonzero(refl)[0][::4][:120]
+    aidx = np.nonzero(~refl)[0][::8][:60]
+    pts = np.concatenate([mid[ridx], mid[aidx]]).astype(np.float32)
+    normals = np.concatenate([nrm[ridx], nrm[aidx]]).astype(np.float32)
+    types = np.concatenate([np.full(len(ridx), A.ZG_SAMPLE_ON_REFLECTING_BOUNDARY),
+                            np.full(len(aidx), A.ZG_SAMPLE_ON_ABSORBING_BOUNDARY)]).astype(np.uint8)
+    d, *_ = osc.closest_point(pts, boundary=0)
+    d[len(ridx):] = 0.0
+    st = E.DeviceStats.zeros(len(pts))
+    E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, 256, types, normals), st, E.SolveOptions(seed=5))
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    O.solve(osc, p.pde, p.settings, pts, d, 256, ost, seed=6, wost=True, rng_mode=O.RNG_PER_POINT, types=types,
+            normals=normals)
+    nr = len(ridx)
+    assert (g["n"][nr:] == 1).all()
+    np.testing.assert_allclose(g["mean"][nr:], p.exact(pts[nr:]), rtol=1e-5, atol=1e-5)
+    z, ok = z_scores(g["mean"][:nr], g["var"][:nr], g["n"][:nr], ost.mean[:nr], ost.variance()[:nr], ost.n[:nr])
+    check_four_se(z, ok, "wost reflecting-boundary points")
+
+
+def test_wost_progressive_calls_equal_one_long_call(gpu, mixed):
+    torch = gpu
+    p, osc, pts, d = mixed
+    pts, d = pts[:200], d[:200]
+    sc = _scene(p, 4)
+    a = E.DeviceStats.zeros(len(pts))
+    fsr = torch.zeros(len(pts), dtype=torch.float32, device="cuda")
+    dp = _device_points(torch, pts, d, 32, fsr=fsr)
+    E.wost_solve(sc, p.pde, p.settings, dp, a, E.SolveOptions(seed=9, stream_offset=0))
+    E.wost_solve(sc, p.pde, p.settings, dp, a, E.SolveOptions(seed=9, stream_offset=32))
+    b = E.DeviceStats.zeros(len(pts))
+    E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, 64), b, E.SolveOptions(seed=9))
+    ha, hb = _host(a), _host(b)
+    m = d > p.settings.epsilon_shell_for_absorbing_boundary
+    np.testing.assert_array_equal(ha["n"][m], hb["n"][m])
+    np.testing.assert_array_equal(ha["len"][m], hb["len"][m])
+    np.testing.assert_allclose(ha["mean"][m], hb["mean"][m], rtol=1e-9, atol=1e-12)
+
+
+def test_wost_argument_errors(gpu, mixed):
+    torch = gpu
+    p, osc, pts, d = mixed
+    sc = _scene(p, 4)
+    dp = _device_points(torch, pts[:8], d[:8], 4)
+    robin = E.make_pde(dirichlet=p.pde.dirichlet, robin=p.pde.robin)
+    robin.are_robin_conditions_pure_neumann = 0
+    with pytest.raises(A.ZombieError):
+        E.wost_solve(sc, robin, p.settings, dp, E.DeviceStats.zeros(8))
+    s3 = PB.sphere3d(level=2)
+    sc3 = E.Scene(s3.positions, s3.indices, None, s3.domain_min, s3.domain_max)
+    with pytest.raises(A.ZombieError):
+        E.wost_solve(sc3, s3.pde, s3.settings, dp, E.DeviceStats.zeros(8))

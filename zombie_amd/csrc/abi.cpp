@@ -287,8 +287,27 @@ zg_status zg_intersect_ray(const zg_scene* s, int32_t boundary, int64_t n, const
 
 zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const float* minR, const float* maxR,
                          float prec, const uint8_t* flip, float* radius, void* stream) {
-    (void)s; (void)n; (void)pts; (void)minR; (void)maxR; (void)prec; (void)flip; (void)radius; (void)stream;
-    return fail(ZG_ERR_UNSUPPORTED, "zg_star_radius is not implemented yet");
+    (void)flip;  // the 2D silhouette-vertex test is symmetric in the normal orientation
+    if (!s || n < 0 || (n > 0 && (!pts || !minR || !maxR || !radius))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "star radius queries run on 2D scenes");
+    ZG_HIP(launchStarRadius(s->view, n, pts, minR, maxR, prec, radius, (hipStream_t)stream));
+    return ok();
+}
+
+zg_status zg_line_of_sight(const zg_scene* s, int64_t n, const float* p1, const float* p2, uint8_t* visible,
+                           void* stream) {
+    if (!s || n < 0 || (n > 0 && (!p1 || !p2 || !visible))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "line-of-sight queries run on 2D scenes");
+    ZG_HIP(launchLineOfSight(s->view, n, p1, p2, visible, (hipStream_t)stream));
+    return ok();
+}
+
+zg_status zg_sample_reflecting(const zg_scene* s, int64_t n, const float* x, const float* radius, const float* u,
+                               uint8_t* ok_, float* pt, float* normal, float* pdf, void* stream) {
+    if (!s || n < 0 || (n > 0 && (!x || !radius || !u || !ok_))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "reflecting-boundary sampling runs on 2D scenes");
+    ZG_HIP(launchSampleReflecting(s->view, n, x, radius, u, ok_, pt, normal, pdf, (hipStream_t)stream));
+    return ok();
 }
 
 }  // extern "C"
@@ -317,10 +336,11 @@ struct SolveSetup {
     WosParams P;
     MergeParams M;
     int dim = 3, width = 8, blocks = 1, depth = 0;
-    bool batch = false, count = false;
+    bool batch = false, count = false, wost = false;
     int64_t n = 0;
     int nWalksMax = 0;
     size_t words = 0;
+    const float* callerFsr = nullptr;   // walk on stars: caller's recorded first sphere radii
 };
 
 zg_status checkPde(const zg_pde* pde, int dim) {
@@ -333,13 +353,18 @@ zg_status checkPde(const zg_pde* pde, int dim) {
 // validation and kernel parameters of a walk-on-spheres solve (no allocation, no launch)
 zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, const zg_walk_settings* ws,
                        const zg_sample_points* pts, const int32_t* instanceOf, zg_statistics* stats,
-                       const zg_solve_options* opts, SolveSetup& S) {
+                       const zg_solve_options* opts, SolveSetup& S, bool wost = false) {
     if ((!s && !b) || !pts || !stats || (s && !pde)) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
     zg_status st = checkSettings(ws);
     if (st != ZG_OK) return st;
     int dim = s ? s->dim : b->dim;
     int sceneDevice = s ? s->device : b->device;
     if (s) { st = checkPde(pde, dim); if (st != ZG_OK) return st; }
+    if (wost) {
+        if (!s || dim != 2) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on 2D scenes");
+        if (!pde->are_robin_conditions_pure_neumann)
+            return fail(ZG_ERR_UNSUPPORTED, "walk on stars supports pure Neumann reflecting conditions (Robin coefficients 0)");
+    }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))
         return fail(ZG_ERR_INVALID_ARGUMENT, "pt, dist_to_absorbing_boundary and n_walks are required");
@@ -407,7 +432,12 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
 
     S.dim = dim; S.width = s ? s->width : b->width; S.depth = s ? s->info.absorbing_depth : b->maxDepth;
     S.batch = b != nullptr; S.count = o.count_visits != 0; S.n = n; S.nWalksMax = pts->n_walks_max;
-    S.words = (size_t)n * 4 + kCntSlots;
+    S.words = (size_t)n * 4 + kCntSlots + (wost ? (size_t)(n + 1) / 2 : 0);
+    S.wost = wost;
+    if (wost) {
+        S.callerFsr = pts->first_sphere_radius;
+        P.prevN = stats->n_solution_estimates;
+    }
     int wavesPerCu = o.waves_per_cu > 0 ? o.waves_per_cu : 32;
     S.blocks = std::max(1, cuCount(dev) * wavesPerCu / 4);
     return ZG_OK;
@@ -422,6 +452,11 @@ void bindWorkspace(SolveSetup& S, uint64_t* wsp) {
     S.P.counters = reinterpret_cast<unsigned long long*>(wsp + 4 * n);
     S.P.claimCounter = S.P.counters + kCntClaims;
     S.M.accSum = S.P.accSum; S.M.accSq = S.P.accSq; S.M.accN = S.P.accN; S.M.accLen = S.P.accLen;
+    if (S.wost) {
+        float* fsr = reinterpret_cast<float*>(wsp + 4 * n + kCntSlots);
+        S.P.fsr = fsr;
+        S.M.fsrSrc = fsr;
+    }
 }
 
 // fill (a kernel, so the sequence can be captured in a graph), walk, merge
@@ -429,9 +464,11 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     bindWorkspace(S, wsp);
     hipError_t e = launchFillZero(wsp, (int64_t)S.words, strm);
     if (e == hipSuccess && ev0) e = hipEventRecord(ev0, strm);
+    if (e == hipSuccess && S.wost) e = launchWostFirstSphere(S.P, S.width, S.callerFsr, const_cast<float*>(S.P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
-        e = S.batch ? launchWosBatch(S.P, S.width, S.count, S.blocks, strm)
-                    : launchWos(S.P, S.dim, S.width, S.count, S.blocks, strm);
+        e = S.wost ? launchWost(S.P, S.width, S.count, S.blocks, strm)
+                   : S.batch ? launchWosBatch(S.P, S.width, S.count, S.blocks, strm)
+                             : launchWos(S.P, S.dim, S.width, S.count, S.blocks, strm);
     if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
     if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
     return e;
@@ -796,8 +833,11 @@ zg_status zg_mlp_evaluate(zg_mlp* m, const float* x, int64_t n, float* value, fl
 
 zg_status zg_wost_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_settings* ws, const zg_sample_points* pts,
                         zg_statistics* stats, const zg_solve_options* opts, zg_solve_report* report, void* stream) {
-    (void)s; (void)pde; (void)ws; (void)pts; (void)stats; (void)opts; (void)report; (void)stream;
-    return fail(ZG_ERR_UNSUPPORTED, "zg_wost_solve is not implemented yet");
+    if (!s) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL scene");
+    SolveSetup S;
+    zg_status st = prepareSolve(s, nullptr, pde, ws, pts, nullptr, stats, opts, S, true);
+    if (st != ZG_OK) return st;
+    return runSolve(S, report, (hipStream_t)stream);
 }
 
 }  // extern "C"

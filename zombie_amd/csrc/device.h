@@ -671,4 +671,237 @@ ZG_DEV int countAxisHits(const BoundaryView& b, const Vec<D>& x, int axis, Stack
     return hits;
 }
 
+
+// lanes of the wave below this one that are set in mask
+ZG_DEV uint32_t mbcnt64(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// outsideBoundingDomain (core/geometric_queries.h:376-379)
+template <int D>
+ZG_DEV bool outsideDomain(const float* dmin, const float* dmax, const Vec<D>& x) {
+    bool o = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) o = o || x[a] < dmin[a] || x[a] > dmax[a];
+    return o;
+}
+
+// ------------------------------------------------------------------------------------
+// Walk-on-stars queries on the 2D reflecting boundary
+// ------------------------------------------------------------------------------------
+
+// offsetPointAlongDirection (core/geometric_queries.h:175-207): the Ray Tracing Gems ch. 6
+// offset, a few float ulps along n in integer space away from the origin, a fixed step near it.
+template <int D>
+ZG_DEV Vec<D> offsetPoint(const Vec<D>& p, const Vec<D>& n) {
+    const float origin = 1.0f / 32.0f, floatScale = 1.0f / 65536.0f, intScale = 256.0f;
+    Vec<D> r;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        int nOff = (int)(n[i] * intScale);
+        float pOff = __int_as_float(__float_as_int(p[i]) + (p[i] < 0.0f ? -nOff : nOff));
+        r[i] = fabsf(p[i]) < origin ? p[i] + floatScale * n[i] : pOff;
+    }
+    return r;
+}
+
+// Point where the ray o + t dir crosses segment i (the parametric point on the segment, as the
+// reference reports hits).
+ZG_DEV Vec<2> segmentHitPoint(const float* prims, int i, const Vec<2>& o, const Vec<2>& dir, float t) {
+    Prim<2> pr; pr.load(prims, i);
+    Vec<2> u = pr.a - o, v = pr.b - pr.a;
+    float dv = cross2(dir, v);
+    if (dv == 0.0f) return o + dir * t;
+    float s = cross2(u, dir) / dv;
+    return pr.a + v * s;
+}
+
+// isSilhouetteVertex for a 2D candidate (v, n0, n1, single): a vertex with one neighbour is
+// always a silhouette; otherwise it is one unless both adjacent segments face x, or both
+// face away, by more than the precision.
+ZG_DEV bool isSilhouette2(const float4& c0, const float4& c1, const Vec<2>& x, float precision) {
+    if (c1.z != 0.0f) return true;
+    Vec<2> view; view[0] = x[0] - c0.x; view[1] = x[1] - c0.y;
+    float l2 = norm2(view);
+    if (l2 <= 0.0f) return true;
+    float inv = 1.0f / __builtin_sqrtf(l2);
+    float f0 = (c0.z * view[0] + c0.w * view[1]) * inv;
+    float f1 = (c1.x * view[0] + c1.y * view[1]) * inv;
+    bool bothFront = f0 > precision && f1 > precision;
+    bool bothBack = f0 < -precision && f1 < -precision;
+    return !(bothFront || bothBack);
+}
+
+// computeStarRadiusForReflectingBoundary (fcpw_geometric_queries.h:1254-1268): distance to
+// the closest silhouette vertex within maxRadius, clamped below by minRadius; maxRadius
+// clamped below by minRadius when there is none.  Nearest-first traversal of the
+// silhouette-candidate BVH, stopped once a silhouette closer than minRadius is found.
+template <int W>
+ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius, float maxRadius, float precision,
+                         Stack& st) {
+    if (minRadius > maxRadius) return maxRadius;
+    if (b.nSilNodes == 0) return fmaxf(maxRadius, minRadius);
+    float best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
+    const float min2 = minRadius * minRadius;
+    bool found = false;
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        float d2[W];
+        int ch[W];
+        childDistances<2, W>(b.silNodes, node, x, d2, ch);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (d2[k] > best2 || !isLeaf(ch[k])) continue;
+            int first = leafFirstD(ch[k]), cnt = leafCountD(ch[k]);
+            for (int i = first; i < first + cnt; i++) {
+                const float4* c = reinterpret_cast<const float4*>(b.sil) + 2 * i;
+                float4 c0 = c[0], c1 = c[1];
+                float dx = x[0] - c0.x, dy = x[1] - c0.y;
+                float dd = dx * dx + dy * dy;
+                if (dd < best2 && isSilhouette2(c0, c1, x, precision)) { best2 = dd; found = true; }
+            }
+        }
+        if (found && best2 <= min2) break;
+        int next = -1; float nd = 0.0f; int nk = -1;
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            if (!isLeaf(ch[k]) && d2[k] <= best2 && (next < 0 || d2[k] < nd)) { next = ch[k]; nd = d2[k]; nk = k; }
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            if (!isLeaf(ch[k]) && d2[k] <= best2 && k != nk) st.push(ch[k], d2[k]);
+        if (next < 0) {
+            int n; float dn;
+            bool got = false;
+            while (st.pop(n, dn)) { if (dn <= best2) { got = true; break; } }
+            if (!got) break;
+            next = n;
+        }
+        node = next;
+    }
+    return found ? fmaxf(__builtin_sqrtf(best2), minRadius) : fmaxf(maxRadius, minRadius);
+}
+
+// hasLineOfSight: true unless a segment of b crosses the open segment p1 -> p2.
+template <int W>
+ZG_DEV bool lineOfSight2(const BoundaryView& b, const Vec<2>& p1, const Vec<2>& p2, Stack& st) {
+    if (b.nNodes == 0) return true;
+    Vec<2> d = p2 - p1;
+    float L = __builtin_sqrtf(norm2(d));
+    if (L <= 0.0f) return true;
+    Vec<2> dir = d * (1.0f / L);
+    Vec<2> inv; inv[0] = 1.0f / dir[0]; inv[1] = 1.0f / dir[1];
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<2, W> nd;
+        nd.load(b.nodes, node);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float t0 = 0.0f, t1 = L;
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                float ta = (nd.lo[a][k] - p1[a]) * inv[a], tb = (nd.hi[a][k] - p1[a]) * inv[a];
+                t0 = fmaxf(t0, fminf(ta, tb));
+                t1 = fminf(t1, fmaxf(ta, tb));
+            }
+            if (t0 > t1) continue;
+            if (isLeaf(nd.ch[k])) {
+                int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
+                for (int i = first; i < first + cnt; i++) {
+                    Prim<2> pr; pr.load(b.prims, i);
+                    float t;
+                    if (pr.ray(p1, dir, L, t) && t > 0.0f && t < L) return false;
+                }
+            } else {
+                st.push(nd.ch[k], t0);
+            }
+        }
+        int n; float tn;
+        if (!st.pop(n, tn)) break;
+        node = n;
+    }
+    return true;
+}
+
+// branchTraversalWeight (fcpw_geometric_queries.h:1391-1398): |G| of the 3D free-space
+// harmonic Green's function at max(r, 1e-2).
+ZG_DEV float branchWeight(float r2) {
+    float r = fmaxf(__builtin_sqrtf(r2), 1e-2f);
+    return 1.0f / (4.0f * kPi * r);
+}
+
+// sampleReflectingBoundary (fcpw_geometric_queries.h:1227-1242): one stochastic descent of
+// the wide BVH.  At an inner node a child whose box overlaps the ball |y - x| <= radius is
+// chosen with probability proportional to branchWeight(|box centre - x|^2); in the leaf, a
+// segment whose box overlaps the ball with probability proportional to branchWeight x length;
+// then a uniform point on it.  u0 is rescaled at every level; pdf is the exact density of the
+// sample.  Every segment whose box overlaps the ball has positive density, so the estimator
+// alpha G h / pdf is unbiased.
+template <int W>
+ZG_DEV bool sampleReflecting2(const BoundaryView& b, const Vec<2>& x, float radius, float u0, float u1,
+                              Vec<2>& pt, Vec<2>& nrm, float& pdf) {
+    if (b.nNodes == 0) return false;
+    const float R2 = radius * radius;
+    float prob = 1.0f;
+    float u = u0;
+    int node = 0;
+    for (int depth = 0; depth < 64; depth++) {
+        WideNode<2, W> nd;
+        nd.load(b.nodes, node);
+        float w[W];
+        float total = 0.0f;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float cx = 0.5f * (nd.lo[0][k] + nd.hi[0][k]) - x[0];
+            float cy = 0.5f * (nd.lo[1][k] + nd.hi[1][k]) - x[1];
+            w[k] = nd.dist2(k, x) <= R2 ? branchWeight(cx * cx + cy * cy) : 0.0f;
+            total += w[k];
+        }
+        if (!(total > 0.0f)) return false;
+        // inverse CDF: the last overlapping child whose interval starts at or below t
+        float t = u * total, acc = 0.0f;
+        int sel = -1;
+        float wSel = 0.0f, accSel = 0.0f;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (w[k] > 0.0f) {
+                if (sel < 0 || t >= acc) { sel = k; wSel = w[k]; accSel = acc; }
+                acc += w[k];
+            }
+        }
+        prob *= wSel / total;
+        u = fminf(fmaxf((t - accSel) / wSel, 0.0f), 0.99999994f);
+        int c = nd.ch[sel];
+        if (!isLeaf(c)) { node = c; continue; }
+        int first = leafFirstD(c), cnt = leafCountD(c);
+        float lw[16];
+        float ltotal = 0.0f;
+        for (int j = 0; j < cnt; j++) {
+            Prim<2> pr; pr.load(b.prims, first + j);
+            float lox = fminf(pr.a[0], pr.b[0]), hix = fmaxf(pr.a[0], pr.b[0]);
+            float loy = fminf(pr.a[1], pr.b[1]), hiy = fmaxf(pr.a[1], pr.b[1]);
+            float dx = fmaxf(fmaxf(lox - x[0], x[0] - hix), 0.0f), dy = fmaxf(fmaxf(loy - x[1], x[1] - hiy), 0.0f);
+            float mx = 0.5f * (lox + hix) - x[0], my = 0.5f * (loy + hiy) - x[1];
+            float len = __builtin_sqrtf(norm2(pr.b - pr.a));
+            lw[j] = dx * dx + dy * dy <= R2 ? branchWeight(mx * mx + my * my) * len : 0.0f;
+            ltotal += lw[j];
+        }
+        if (!(ltotal > 0.0f)) return false;
+        float lt = u * ltotal, lacc = 0.0f;
+        int ps = -1;
+        for (int j = 0; j < cnt; j++) {
+            if (lw[j] > 0.0f && (ps < 0 || lt >= lacc)) ps = j;
+            lacc += lw[j];
+        }
+        Prim<2> pr; pr.load(b.prims, first + ps);
+        float len = __builtin_sqrtf(norm2(pr.b - pr.a));
+        pt = pr.a + (pr.b - pr.a) * u1;
+        nrm = pr.normal();
+        pdf = prob * (lw[ps] / ltotal) / len;
+        return true;
+    }
+    return false;
+}
+
 }  // namespace zg

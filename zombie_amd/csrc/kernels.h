@@ -59,6 +59,8 @@ struct WosParams {
     const InstanceDesc* instances;    // batched solves: instance table
     const int32_t* instanceOf;        // batched solves: instance of each point
     const uint32_t* streamOffsetDev;  // when set, replaces streamOffset (graph replays advance it)
+    const float* fsr;                 // walk on stars: first sphere radius per point
+    const int64_t* prevN;             // walk on stars: caller's estimate counts (first call when 0)
 };
 
 struct MergeParams {
@@ -86,6 +88,7 @@ struct MergeParams {
     int32_t nInstances;
     uint32_t* streamOffsetDev;        // advanced by streamAdvance once the merge is done
     uint32_t streamAdvance;
+    const float* fsrSrc;              // walk on stars: first sphere radius to record (else dist0)
 };
 
 hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s);
@@ -97,5 +100,14 @@ hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* 
 hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s);
 hipError_t launchWosBatch(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
+// walk on stars (2D)
+hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s);
+hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
+                            float prec, float* radius, hipStream_t s);
+hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, const float* p2, uint8_t* visible,
+                             hipStream_t s);
+hipError_t launchSampleReflecting(const SceneView& sv, int64_t n, const float* x, const float* radius, const float* u,
+                                  uint8_t* ok, float* pt, float* normal, float* pdf, hipStream_t s);
 
 }  // namespace zg

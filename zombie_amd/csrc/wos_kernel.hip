@@ -100,18 +100,6 @@ __global__ void __launch_bounds__(256) rayKernel(BoundaryView b, int64_t n, cons
 // ------------------------------------------------------------------------------------
 // walk-on-spheres
 // ------------------------------------------------------------------------------------
-ZG_DEV uint32_t mbcnt64(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-template <int D>
-ZG_DEV bool outsideDomain(const float* dmin, const float* dmax, const Vec<D>& x) {
-    bool o = false;
-#pragma unroll
-    for (int a = 0; a < D; a++) o = o || x[a] < dmin[a] || x[a] > dmax[a];
-    return o;
-}
-
 // the part of a scene a walk traverses (per lane in batched solves)
 struct WalkView {
     const float* nodes;
@@ -382,7 +370,7 @@ __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
         }
         if (M.firstSphereRadius && !hasPrev) M.firstSphereRadius[i] = 0.0f;
     } else if (M.firstSphereRadius && !hasPrev && M.writeFirstSphereRadius) {
-        M.firstSphereRadius[i] = M.dist0[i];
+        M.firstSphereRadius[i] = M.fsrSrc ? M.fsrSrc[i] : M.dist0[i];
     }
     if (nb > 0) {
         double meanB = sum / (double)nb;

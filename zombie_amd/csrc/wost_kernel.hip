@@ -1,0 +1,370 @@
+// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D, Neumann reflecting boundary) for
+// gfx950, its per-point first-sphere prologue, and the batched star-radius, line-of-sight and
+// reflecting-boundary sampling queries.
+//
+// Work decomposition is the one of the walk-on-spheres kernel: claims of `wpl` walks of one
+// point, handed to the idle lanes of a wave by ballot + mbcnt every `regroup` iterations (or
+// as soon as the whole wave is idle), FP64 per-point atomics, a merge kernel for the caller's
+// statistics.  One iteration advances every active lane by one full walk-on-stars step; the
+// queries of a step run to completion inside it:
+//   star radius     closest silhouette vertex of the reflecting boundary (silhouette BVH)
+//   ray             first reflecting-boundary hit within the star radius
+//   boundary sample stochastic descent of the reflecting BVH + line-of-sight test
+//   distance        closest point of the absorbing boundary for the next step
+// Per walk the lane restates WalkOnStars::walk (point_estimation/walk_on_stars.h:345-490)
+// with estimateSolution's per-point rules (:518-662), computeReflectingBoundaryContribution
+// (:189-250), computeSourceContribution (:252-275) and computeWalkStepThroughput (:277-307)
+// for pure Neumann conditions and the harmonic Green's function (reflectance 1, Poisson
+// kernel 1).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace zg {
+
+// star radius of a walk: closest silhouette within [epsR, distA], shrunk by 1 % for numerical
+// robustness when the reflecting boundary is not empty (walk_on_stars.h:370-392,576-590)
+template <int W>
+ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, const Vec<2>& x, float distA, Stack& st) {
+    float r = distA;
+    if (sc.refl.nPrims > 0) {
+        r = starRadius2<W>(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, st);
+        if (ws.epsilon_shell_for_reflecting_boundary <= distA)
+            r = fmaxf(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
+    }
+    return r;
+}
+
+// first sphere radius per point (estimateSolution :566-590); a point with earlier estimates
+// keeps the caller's recorded radius
+template <int W>
+__global__ void __launch_bounds__(256) wostFirstSphereKernel(WosParams P, const float* __restrict__ callerFsr,
+                                                             float* __restrict__ fsr) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nPoints) return;
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
+    const zg_walk_settings& ws = P.ws;
+    uint8_t ty = P.type ? P.type[i] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+    float d = P.dist0[i];
+    float r;
+    if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) r = 0.0f;
+    else if (callerFsr && P.prevN && P.prevN[i] > 0) r = callerFsr[i];
+    else if (d <= ws.epsilon_shell_for_absorbing_boundary || ws.steps_before_using_maximal_spheres == 0) r = d;
+    else r = wostStarRadius<W>(P.scene, ws, vload<2>(P.pt + 2 * i), d, st);
+    fsr[i] = r;
+}
+
+template <int W, bool COUNT>
+__global__ void __launch_bounds__(256) wostKernel(WosParams P) {
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
+    VisitCount vc{0, 0};
+    const zg_walk_settings& ws = P.ws;
+    const zg_pde& pde = P.pde;
+    const SceneView& sc = P.scene;
+    const float epsA = ws.epsilon_shell_for_absorbing_boundary;
+    const float prec = ws.silhouette_precision;
+    const bool hasRefl = sc.refl.nPrims > 0;
+    const int lane = __lane_id();
+    const uint32_t streamOffset = P.streamOffsetDev ? *P.streamOffsetDev : P.streamOffset;
+
+    __shared__ unsigned long long blk[6];
+    if (threadIdx.x < 6) blk[threadIdx.x] = 0;
+    __syncthreads();
+
+    // walk state (WalkState, common.h:93-130)
+    bool active = false;
+    int p = 0, w = 0, wEnd = 0;
+    Vec<2> x = vzero<2>(), nrm = vzero<2>(), prevDir = vzero<2>();
+    float prevDist = FLT_MAX, T = 1.0f, Sref = 0.0f, Ssrc = 0.0f, distA = 0.0f, fsr0 = 0.0f;
+    int len = 0;
+    bool onRefl = false, flip = false, firstStep = true;
+    Ball<2> ball; ball.yukawa = false; ball.sqrtLambda = 0.0f; ball.R = 0.0f;
+    Pcg32 rng; rng.state = 0; rng.inc = 1;
+
+    auto startWalk = [&]() {
+        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+        Vec<2> n = P.normal ? vload<2>(P.normal + 2 * (size_t)p) : vzero<2>();
+        bool al = P.aligned ? P.aligned[p] != 0 : false;
+        x = vload<2>(P.pt + 2 * (size_t)p);
+        nrm = n; prevDir = n; prevDist = FLT_MAX; flip = false;
+        if (ws.solve_double_sided && ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && al) {
+            nrm = n * -1.0f; prevDir = n * -1.0f; flip = true;
+        }
+        onRefl = ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY;
+        distA = P.dist0[p];
+        fsr0 = P.fsr[p];
+        firstStep = true;
+        T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
+        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+        atomicAdd(&blk[5], 1ull);
+        active = true;
+    };
+
+    auto finishWalk = [&](int code) {
+        atomicAdd(&blk[code], 1ull);
+        if (code != ZG_ESCAPED_DOMAIN) {
+            float term = 0.0f;
+            if (code == ZG_REACHED_ABSORBING_BOUNDARY && !ws.ignore_absorbing_boundary_contribution) {
+                // project to the absorbing boundary (estimateSolution :627-645)
+                float b2 = FLT_MAX;
+                Vec<2> cp = x;
+                int prim = -1;
+                Vec<2> xp = x, nn = nrm;
+                bool al = false;
+                if (closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc)) {
+                    Prim<2> pr; pr.load(sc.abs.prims, prim);
+                    Vec<2> n = pr.normal();
+                    float sd = dot(x - cp, n);
+                    xp = cp; nn = n;
+                    al = ws.solve_double_sided && sd > 0.0f;
+                }
+                term = evalField<2>(pde.dirichlet, xp, nn, al, 0.0f);
+            }
+            double c = (double)(T * term + Sref + Ssrc);
+            atomicAdd(&P.accSum[p], c);
+            atomicAdd(&P.accSq[p], c * c);
+            atomicAdd(&P.accN[p], 1ull);
+            atomicAdd(&P.accLen[p], (unsigned long long)len);
+            atomicAdd(&blk[4], (unsigned long long)len);
+        }
+        w++;
+        if (w < wEnd) startWalk();
+        else active = false;
+    };
+
+    bool workLeft = true;
+    int iter = 0;
+    for (;;) {
+        // -------- regroup: refill idle lanes with new claims --------
+        if (iter % P.regroup == 0 || !__any(active)) {
+            for (int round = 0; round < 4; round++) {
+                uint64_t idle = __ballot(!active);
+                if (!workLeft || idle == 0) break;
+                uint32_t nIdle = (uint32_t)__popcll(idle);
+                int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(P.claimCounter, (unsigned long long)nIdle);
+                base = __shfl(base, leader);
+                if (base + nIdle >= P.nClaims) workLeft = false;
+                if (!active) {
+                    unsigned long long c = base + mbcnt64(idle);
+                    if (c < P.nClaims) {
+                        p = (int)(c / (unsigned long long)P.chunksPerPoint);
+                        int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
+                        int nw = P.nWalks[p];
+                        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+                        if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
+                        else if (P.dist0[p] <= epsA) nw = min(nw, 1);
+                        w = k * P.wpl;
+                        wEnd = min(w + P.wpl, nw);
+                        if (w < wEnd) startWalk();
+                    }
+                }
+            }
+            if (!workLeft && !__any(active)) break;
+        }
+        iter++;
+        if (!active) continue;
+
+        // -------- one walk-on-stars step (walk_on_stars.h:366-486) --------
+        if (!(distA > epsA)) { finishWalk(ZG_REACHED_ABSORBING_BOUNDARY); continue; }
+        float R;
+        if (firstStep && fsr0 > 0.0f) {
+            R = fsr0;
+        } else {
+            flip = false;
+            if (ws.solve_double_sided && onRefl && prevDist > 0.0f && dot(prevDir, nrm) < 0.0f) {
+                nrm = nrm * -1.0f;
+                flip = true;
+            }
+            R = ws.steps_before_using_maximal_spheres <= len ? distA : wostStarRadius<W>(sc, ws, x, distA, st);
+        }
+        ball.update(R);
+        Vec<2> dir = sampleUnitSphere<2>(rng);
+        if (onRefl && dot(nrm, dir) > 0.0f) dir = dir * -1.0f;
+
+        // first reflecting-boundary hit within the star (intersectReflectingBoundary)
+        Vec<2> o = onRefl ? offsetPoint<2>(x, nrm * -1.0f) : x;
+        float tHit = R;
+        int hitPrim = -1;
+        bool hit = hasRefl && intersectRay<2, W>(sc.refl, o, dir, R, tHit, hitPrim, st);
+        Vec<2> ipt, inrm = vzero<2>();
+        float idist;
+        if (hit) {
+            ipt = segmentHitPoint(sc.refl.prims, hitPrim, o, dir, tHit);
+            Prim<2> pr; pr.load(sc.refl.prims, hitPrim);
+            inrm = pr.normal();
+            idist = tHit;
+        } else {
+            ipt = o + dir * R;
+            idist = R;
+        }
+
+        // reflecting boundary contribution (computeReflectingBoundaryContribution :189-250)
+        if (hasRefl && !ws.ignore_reflecting_boundary_contribution) {
+            float u0 = rng.nextFloat();
+            float u1 = rng.nextFloat();
+            Vec<2> bpt, bn;
+            float bpdf = 0.0f;
+            if (sampleReflecting2<W>(sc.refl, x, R, u0, u1, bpt, bn, bpdf)) {
+                Vec<2> toS = bpt - x;
+                float distS = __builtin_sqrtf(norm2(toS));
+                float alpha = onRefl ? 2.0f : 1.0f;
+                bool al = false;
+                if (ws.solve_double_sided) {
+                    toS = toS * (1.0f / distS);
+                    if (flip) {
+                        al = true;
+                    } else if (dot(toS, bn) < -prec) {
+                        bool flipS = alpha > 1.0f ? dot(toS, nrm) < -prec : true;
+                        if (flipS) al = true;
+                    }
+                }
+                Vec<2> bnS = bn * (al ? -1.0f : 1.0f);
+                if (bpdf > 0.0f && distS < R) {
+                    Vec<2> p1 = onRefl ? offsetPoint<2>(x, nrm * -1.0f) : x;
+                    Vec<2> p2 = offsetPoint<2>(bpt, bnS * -1.0f);
+                    if (lineOfSight2<W>(sc.refl, p1, p2, st)) {
+                        float G = ball.evaluate(fmaxf(1e-4f, distS));
+                        float h = evalField<2>(pde.robin, bpt, bn, ws.solve_double_sided && al, 0.0f);
+                        Sref += T * alpha * G * h / bpdf;
+                    }
+                }
+            }
+        }
+
+        // source contribution along the walk direction (computeSourceContribution :252-275)
+        if (!ws.ignore_source_contribution) {
+            float r = ball.sampleRadius(rng);
+            if (r <= idist) Ssrc += T * ball.norm() * evalField<2>(pde.source, x + dir * r, vzero<2>(), false, 0.0f);
+        }
+
+        // move; throughput is 1 for pure Neumann conditions and the harmonic kernel
+        prevDist = idist;
+        prevDir = dir;
+        x = ipt;
+        nrm = inrm;
+        onRefl = hit;
+        if (!onRefl && outsideDomain<2>(sc.dmin, sc.dmax, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
+        if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :309-343
+            float survival = T / ws.russian_roulette_threshold;
+            if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
+            T = ws.russian_roulette_threshold;
+        }
+        len++;
+        if (len >= ws.max_walk_length) { finishWalk(ZG_EXCEEDED_MAX_WALK_LENGTH); continue; }
+        // distance to the absorbing boundary for the next step
+        float b2 = FLT_MAX;
+        Vec<2> cp = x;
+        int prim = -1;
+        distA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc) ? __builtin_sqrtf(b2) : FLT_MAX;
+        firstStep = false;
+    }
+
+    __syncthreads();
+    if (threadIdx.x < 6 && blk[threadIdx.x]) atomicAdd(&P.counters[threadIdx.x], blk[threadIdx.x]);
+    if (COUNT) {
+        atomicAdd(&P.counters[kCntNodes], (unsigned long long)vc.nodes);
+        atomicAdd(&P.counters[kCntPrims], (unsigned long long)vc.prims);
+    }
+    if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 1u);
+}
+
+// ------------------------------------------------------------------------------------
+// batched reflecting-boundary queries
+// ------------------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(256) starRadiusKernel(SceneView sv, int64_t n, const float* __restrict__ pts,
+                                                        const float* __restrict__ minR, const float* __restrict__ maxR,
+                                                        float prec, float* __restrict__ radius) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
+    radius[i] = sv.refl.nPrims == 0 ? maxR[i] : starRadius2<W>(sv.refl, vload<2>(pts + 2 * i), minR[i], maxR[i], prec, st);
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) lineOfSightKernel(SceneView sv, int64_t n, const float* __restrict__ p1,
+                                                         const float* __restrict__ p2, uint8_t* __restrict__ visible) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
+    visible[i] = lineOfSight2<W>(sv.refl, vload<2>(p1 + 2 * i), vload<2>(p2 + 2 * i), st) ? 1 : 0;
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) sampleReflectingKernel(SceneView sv, int64_t n, const float* __restrict__ x,
+                                                              const float* __restrict__ radius, const float* __restrict__ u,
+                                                              uint8_t* __restrict__ ok, float* __restrict__ pt,
+                                                              float* __restrict__ normal, float* __restrict__ pdf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Vec<2> q = vzero<2>(), nn = vzero<2>();
+    float pd = 0.0f;
+    bool got = sampleReflecting2<W>(sv.refl, vload<2>(x + 2 * i), radius[i], u[2 * i], u[2 * i + 1], q, nn, pd);
+    ok[i] = got ? 1 : 0;
+    if (pt) vstore<2>(pt + 2 * i, q);
+    if (normal) vstore<2>(normal + 2 * i, nn);
+    if (pdf) pdf[i] = got ? pd : 0.0f;
+}
+
+// ------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------
+static inline unsigned wostBlocksFor(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s) {
+    if (P.nPoints <= 0) return hipSuccess;
+    dim3 g(wostBlocksFor(P.nPoints)), bl(256);
+    if (width == 4) hipLaunchKernelGGL((wostFirstSphereKernel<4>), g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
+    else hipLaunchKernelGGL((wostFirstSphereKernel<8>), g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
+    return hipGetLastError();
+}
+
+hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s) {
+    dim3 g(gridBlocks), bl(256);
+    if (width == 4) {
+        if (count) hipLaunchKernelGGL((wostKernel<4, true>), g, bl, kStackLdsBytes, s, P);
+        else hipLaunchKernelGGL((wostKernel<4, false>), g, bl, kStackLdsBytes, s, P);
+    } else {
+        if (count) hipLaunchKernelGGL((wostKernel<8, true>), g, bl, kStackLdsBytes, s, P);
+        else hipLaunchKernelGGL((wostKernel<8, false>), g, bl, kStackLdsBytes, s, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
+                            float prec, float* radius, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 g(wostBlocksFor(n)), bl(256);
+    if (sv.width == 4) hipLaunchKernelGGL((starRadiusKernel<4>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, radius);
+    else hipLaunchKernelGGL((starRadiusKernel<8>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, radius);
+    return hipGetLastError();
+}
+
+hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, const float* p2, uint8_t* visible,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 g(wostBlocksFor(n)), bl(256);
+    if (sv.width == 4) hipLaunchKernelGGL((lineOfSightKernel<4>), g, bl, kStackLdsBytes, s, sv, n, p1, p2, visible);
+    else hipLaunchKernelGGL((lineOfSightKernel<8>), g, bl, kStackLdsBytes, s, sv, n, p1, p2, visible);
+    return hipGetLastError();
+}
+
+hipError_t launchSampleReflecting(const SceneView& sv, int64_t n, const float* x, const float* radius, const float* u,
+                                  uint8_t* ok, float* pt, float* normal, float* pdf, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 g(wostBlocksFor(n)), bl(256);
+    if (sv.width == 4) hipLaunchKernelGGL((sampleReflectingKernel<4>), g, bl, 0, s, sv, n, x, radius, u, ok, pt, normal, pdf);
+    else hipLaunchKernelGGL((sampleReflectingKernel<8>), g, bl, 0, s, sv, n, x, radius, u, ok, pt, normal, pdf);
+    return hipGetLastError();
+}
+
+}  // namespace zg

@@ -208,6 +208,32 @@ class Scene:
                                          _ptr(t), _ptr(p), _ptr(nr), _stream_ptr()))
         return hit, t, p, nr
 
+    # walk-on-stars queries on the reflecting boundary (2D)
+    def star_radius(self, pts, min_r, max_r, precision: float = 1e-3):
+        torch = _torch()
+        out = torch.empty(pts.shape[0], dtype=torch.float32, device=pts.device)
+        A.check(A.lib().zg_star_radius(self._h, pts.shape[0], _ptr(pts), _ptr(min_r), _ptr(max_r), float(precision),
+                                       None, _ptr(out), _stream_ptr()))
+        return out
+
+    def line_of_sight(self, p1, p2):
+        torch = _torch()
+        out = torch.empty(p1.shape[0], dtype=torch.uint8, device=p1.device)
+        A.check(A.lib().zg_line_of_sight(self._h, p1.shape[0], _ptr(p1), _ptr(p2), _ptr(out), _stream_ptr()))
+        return out
+
+    def sample_reflecting(self, x, radius, u):
+        torch = _torch()
+        n = x.shape[0]
+        dev = x.device
+        ok = torch.empty(n, dtype=torch.uint8, device=dev)
+        pt = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        nr = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        pdf = torch.empty(n, dtype=torch.float32, device=dev)
+        A.check(A.lib().zg_sample_reflecting(self._h, n, _ptr(x), _ptr(radius), _ptr(u), _ptr(ok), _ptr(pt), _ptr(nr),
+                                             _ptr(pdf), _stream_ptr()))
+        return ok, pt, nr, pdf
+
 
 # ------------------------------------------------------------------------------------
 # points + statistics on the device
