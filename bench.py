@@ -35,8 +35,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sphere3d", choices=["sphere3d", "star2d", "mixed2d", "batch"],
+                    help="sphere3d: the headline 3D Poisson workload (default); star2d: 2D Laplace on the star "
+                         "domain; mixed2d: 2D walk on stars with Neumann slits; batch: 256 screened-Poisson "
+                         "instances in one graph-captured launch")
     ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--walks", type=int, default=256)
+    ap.add_argument("--walks", type=int, default=None, help="walks per point per pass (workload default)")
+    ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
     ap.add_argument("--bvh-width", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=2)
     ap.add_argument("--walks-per-lane", type=int, default=1)
@@ -91,6 +96,12 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
+    if args.workload != "sphere3d":
+        result = {"star2d": bench_star2d, "mixed2d": bench_mixed2d, "batch": bench_batch}[args.workload](args, torch, dev)
+        print(json.dumps(result), flush=True)
+        return
+    if args.walks is None:
+        args.walks = 256
     t_setup = time.time()
     prob = PB.sphere3d(seed=args.seed, n_walks=args.walks)
     scene = E.Scene(prob.positions, prob.indices, None, prob.domain_min, prob.domain_max,
@@ -296,6 +307,275 @@ def cpu_baseline(prob, pts, dist, walks):
     return {"value": rep["total_steps"] / el, "unit": "walk-steps/s", "cores": threads, "kind": "port",
             "sample": f"{len(pts)} points x {walks} walks of the same workload ({rep['total_steps']} steps, "
                       f"{el:.1f} s, exact radii, one PCG32 per point as in the reference)"}
+
+
+# ------------------------------------------------------------------------------------
+# secondary workloads (parity-test configurations measured at their full sizes)
+# ------------------------------------------------------------------------------------
+def _result(metric_label, workload, value, elapsed, args, extra):
+    r = {"metric": f"walk-steps/s ({metric_label})", "value": value, "unit": "walk-steps/s", "n_gpus": 1,
+         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 geometry, f64 accumulation",
+         "data": "synthetic (manufactured solution, seeded geometry and points)", "config": workload}
+    r.update(extra)
+    return r
+
+
+def _timed(torch, args, run):
+    for w in range(args.warmup):
+        r = run(1000 + w, None)
+        log(f"warmup pass {w}: {r.total_walks} walks, {r.total_steps} steps, kernel {r.kernel_ms:.1f} ms")
+    torch.cuda.synchronize()
+    reps = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        reps.append(run(s, s))
+        log(f"pass {s}: {reps[-1].total_walks} walks, {reps[-1].total_steps} steps, kernel {reps[-1].kernel_ms:.1f} ms")
+    torch.cuda.synchronize()
+    return reps, time.perf_counter() - t0
+
+
+def _roofline(rep_count, node_bytes, prim_bytes, kernel_ms_per_pass, steps_per_pass, note):
+    nodes = rep_count.node_visits / max(rep_count.total_steps, 1)
+    prims = rep_count.prim_visits / max(rep_count.total_steps, 1)
+    bps = nodes * node_bytes + prims * prim_bytes + 48.0
+    ach = bps * steps_per_pass / (kernel_ms_per_pass * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "bytes_per_step": bps, "nodes_per_step": nodes, "prims_per_step": prims,
+            "kernel_ms": kernel_ms_per_pass, "note": note}
+
+
+def _oracle_baseline(label, run, cores):
+    t = time.perf_counter()
+    rep = run()
+    el = time.perf_counter() - t
+    return {"value": rep["total_steps"] / el, "unit": "walk-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{label} ({rep['total_steps']} steps, {el:.1f} s, one PCG32 per point as in the reference)"}
+
+
+def bench_star2d(args, torch, dev):
+    """2D Laplace on the star domain r = 1 + 0.3 cos 5t (1,024 segments), g = e^x cos y,
+    the 34,627 interior points of a 256^2 grid, 512 walks, epsilon 1e-3."""
+    import numpy as np
+    import oracle as O
+    from zombie_amd import engine as E
+    from zombie_amd import problems as PB
+    walks = args.walks or 512
+    prob = PB.star2d(n_walks=walks)
+    scene = E.Scene(prob.positions, prob.indices, None, prob.domain_min, prob.domain_max, bvh_width=args.bvh_width,
+                    leaf_size=args.leaf_size)
+    P = torch.from_numpy(prob.meta["points"]).to(dev)
+    n = P.shape[0]
+    dist, _, _, _ = scene.closest_point(P, want_point=False)
+    dp = E.DevicePoints(pt=P, dist_to_absorbing=dist.contiguous(),
+                        n_walks=torch.full((n,), walks, dtype=torch.int32, device=dev), n_walks_max=walks)
+    stats = E.DeviceStats.zeros(n)
+
+    def opts(s, count=False):
+        return E.SolveOptions(seed=args.seed, stream_offset=s * walks, walks_per_lane=args.walks_per_lane,
+                              regroup_interval=args.regroup, step_interval=args.step_interval, count_visits=count)
+
+    def run(s, keep):
+        return E.wos_solve(scene, prob.pde, prob.settings, dp, stats if keep is not None else E.DeviceStats.zeros(n),
+                           opts(s))
+    reps, el = _timed(torch, args, run)
+    steps = sum(r.total_steps for r in reps)
+    walks_done = sum(r.total_walks for r in reps)
+    kms = sum(r.kernel_ms for r in reps) / len(reps)
+    rc = E.wos_solve(scene, prob.pde, prob.settings, dp, E.DeviceStats.zeros(n), opts(99, True))
+    mean = stats.mean.cpu().numpy()
+    rmse = float(np.sqrt(np.mean((mean - prob.exact(prob.meta["points"])) ** 2)))
+    res = _result("2D Laplace WoS, star domain", {
+        "workload": f"2D Laplace WoS, star domain 1,024 segments, {n} grid points, {walks} walks/pt/pass, eps 1e-3",
+        "bvh_width": args.bvh_width, "leaf_size": args.leaf_size, "walks_per_lane": args.walks_per_lane,
+        "regroup_interval": args.regroup, "step_interval": args.step_interval, "points": n}, steps / el, el, args,
+        {"walks_per_s": walks_done / el, "mean_walk_length": steps / max(walks_done, 1), "rmse_vs_analytic": rmse,
+         "roofline": _roofline(rc, scene.info.node_bytes, scene.info.prim_bytes, kms, steps / len(reps),
+                               "closest-point bytes touched per step; the scene is cache-resident")})
+    if not args.no_cpu_baseline:
+        pts, d = prob.meta["points"], dist.cpu().numpy()
+        threads = min(os.cpu_count() or 1, 16)
+        osc = O.Scene(prob.positions, prob.indices, None, prob.domain_min, prob.domain_max)
+        res["cpu_baseline"] = _oracle_baseline(f"{len(pts)} points x {walks} walks", lambda: O.solve(
+            osc, prob.pde, prob.settings, pts, d, walks, O.Stats(len(pts)), seed=7, rng_mode=O.RNG_PER_POINT,
+            threads=threads), threads)
+        res["speedup_vs_cpu_baseline"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
+
+
+def bench_mixed2d(args, torch, dev):
+    """2D walk on stars: the star curve with 4,096 segments, Neumann on the upper half and on
+    32 interior slits (double-sided), u = x^3 - 3xy^2; 200k interior points, 1,024 walks."""
+    import numpy as np
+    import oracle as O
+    from zombie_amd import engine as E
+    from zombie_amd import problems as PB
+    walks = args.walks or 1024
+    t0 = time.time()
+    prob = PB.mixed2d(n_walks=walks)
+    scene = E.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,
+                    bvh_width=args.bvh_width, leaf_size=args.leaf_size, solve_double_sided=True)
+    V, I = prob.meta["outer"]
+    n_target = args.points if args.points != 1_000_000 else 200_000
+    cand = PB.sample_box(prob.domain_min, prob.domain_max, 2 * n_target, args.seed)
+    pts = cand[PB.points_in_polygon(cand, V, I)][:n_target]
+    P = torch.from_numpy(pts).to(dev)
+    n = P.shape[0]
+    dist, _, _, _ = scene.closest_point(P, boundary=0, want_point=False)
+    dp = E.DevicePoints(pt=P, dist_to_absorbing=dist.contiguous(),
+                        n_walks=torch.full((n,), walks, dtype=torch.int32, device=dev), n_walks_max=walks)
+    stats = E.DeviceStats.zeros(n)
+    log(f"mixed2d setup {time.time() - t0:.1f}s: {n} points, {walks} walks")
+
+    def opts(s, count=False):
+        return E.SolveOptions(seed=args.seed, stream_offset=s * walks, walks_per_lane=args.walks_per_lane,
+                              regroup_interval=args.regroup, count_visits=count)
+
+    def run(s, keep):
+        return E.wost_solve(scene, prob.pde, prob.settings, dp, stats if keep is not None else E.DeviceStats.zeros(n),
+                            opts(s))
+    reps, el = _timed(torch, args, run)
+    steps = sum(r.total_steps for r in reps)
+    walks_done = sum(r.total_walks for r in reps)
+    kms = sum(r.kernel_ms for r in reps) / len(reps)
+    sub = 16384
+    dps = E.DevicePoints(pt=P[:sub].contiguous(), dist_to_absorbing=dist[:sub].contiguous(),
+                         n_walks=torch.full((sub,), 64, dtype=torch.int32, device=dev), n_walks_max=64)
+    rc = E.wost_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(99, True))
+    mean = stats.mean.cpu().numpy()
+    rmse = float(np.sqrt(np.mean((mean - prob.exact(pts)) ** 2)))
+    res = _result("2D mixed Dirichlet/Neumann walk on stars", {
+        "workload": f"2D WoSt, star curve 4,096 segments + 32 Neumann slits, {n} points, {walks} walks/pt/pass, "
+                    "eps 1e-3", "bvh_width": args.bvh_width, "leaf_size": args.leaf_size,
+        "walks_per_lane": args.walks_per_lane, "regroup_interval": args.regroup, "points": n}, steps / el, el, args,
+        {"walks_per_s": walks_done / el, "mean_walk_length": steps / max(walks_done, 1), "rmse_vs_analytic": rmse,
+         "completion_codes": [int(sum(r.completion_codes[k] for r in reps)) for k in range(4)],
+         "roofline": _roofline(rc, scene.info.node_bytes, scene.info.prim_bytes, kms, steps / len(reps),
+                               "absorbing-boundary closest-point bytes only; the reflecting queries (silhouette, "
+                               "ray, sampling, visibility) are not counted")})
+    if not args.no_cpu_baseline:
+        k = 200
+        threads = min(os.cpu_count() or 1, 16)
+        osc = O.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,
+                      solve_double_sided=True)
+        res["cpu_baseline"] = _oracle_baseline(f"{k} points x {walks} walks", lambda: O.solve(
+            osc, prob.pde, prob.settings, pts[:k], dist[:k].cpu().numpy(), walks, O.Stats(k), seed=7, wost=True,
+            rng_mode=O.RNG_PER_POINT, threads=threads), threads)
+        res["speedup_vs_cpu_baseline"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
+
+
+def bench_batch(args, torch, dev):
+    """256 screened-Poisson instances (SH-displaced geodesic spheres, triangles log-uniform in
+    [2k, 200k], points log-uniform in [1k, 1e5], sigma ~ U[0, 10], u = exp(a.x) with
+    |a|^2 = sigma), 128 walks, one persistent launch per pass replayed from a hipGraph."""
+    import concurrent.futures as cf
+    import numpy as np
+    import oracle as O
+    from zombie_amd import autotune as AT
+    from zombie_amd import engine as E
+    from zombie_amd import problems as PB
+    walks = args.walks or 128
+    t0 = time.time()
+    spec = PB.screened_batch_spec(args.instances, seed=args.seed)
+
+    def build(sp):
+        pr = PB.screened_instance(sp["n_tris"], sp["sigma"], sp["direction"], sp["seed"], n_walks=walks)
+        return pr
+    with cf.ThreadPoolExecutor(16) as ex:
+        probs = list(ex.map(build, spec))
+
+    def scenes_for(width, leaf):
+        with cf.ThreadPoolExecutor(16) as ex:
+            return list(ex.map(lambda pr: E.Scene(pr.positions, pr.indices, None, pr.domain_min, pr.domain_max,
+                                                  bvh_width=width, leaf_size=leaf), probs))
+    scenes = scenes_for(args.bvh_width, args.leaf_size)
+    pts, dists, inst, exact = [], [], [], []
+    for i, (sp, pr, sc) in enumerate(zip(spec, probs, scenes)):
+        need, got, seed = sp["n_points"], [], args.seed + 100 * i
+        while sum(len(g) for g in got) < need:
+            c = torch.from_numpy(PB.sample_box(pr.domain_min, pr.domain_max, 2 * need + 64, seed)).to(dev)
+            seed += 1
+            got.append(c[sc.inside_domain(c).bool()])
+        p = torch.cat(got)[:need].contiguous()
+        d, _, _, _ = sc.closest_point(p, want_point=False)
+        pts.append(p); dists.append(d); inst.append(torch.full((need,), i, dtype=torch.int32, device=dev))
+        exact.append(PB.exp_linear(p.cpu().numpy(), pr.meta["a"]))
+    P = torch.cat(pts).contiguous(); D = torch.cat(dists).contiguous(); IOP = torch.cat(inst).contiguous()
+    exact = np.concatenate(exact)
+    n = P.shape[0]
+    n_tris = sum(int(pr.indices.shape[0]) for pr in probs)
+    log(f"batch setup {time.time()-t0:.1f}s: {len(probs)} instances, {n_tris} triangles, {n} points")
+    dp = E.DevicePoints(pt=P, dist_to_absorbing=D, n_walks=torch.full((n,), walks, dtype=torch.int32, device=dev),
+                        n_walks_max=walks)
+    batches = {args.bvh_width: (scenes, E.Batch(scenes, [pr.pde for pr in probs]))}
+    wpl, rg, width = args.walks_per_lane, args.regroup, args.bvh_width
+    tune = None
+    if not args.no_tune:
+        # the sample must cover many instances: a strided gather of the concatenation
+        idx = torch.arange(0, n, max(1, n // 131072), device=dev)[:131072]
+        sdp = E.DevicePoints(pt=P[idx].contiguous(), dist_to_absorbing=D[idx].contiguous(), n_walks=None,
+                             n_walks_max=0)
+        for w in (4, 8):
+            if w not in batches:
+                sc_w = scenes_for(w, args.leaf_size)
+                batches[w] = (sc_w, E.Batch(sc_w, [pr.pde for pr in probs]))
+        tune = AT.autotune_batch(probs, sdp, IOP[idx].contiguous(), leaf_size=args.leaf_size, batches=batches,
+                                 distance_tolerance=args.tolerance, step_interval=args.step_interval)
+        for row in tune.table:
+            log("tune: width {bvh_width} wpl {walks_per_lane} regroup {regroup_interval:2d} -> "
+                "{walks_per_s:10.4g} walks/s".format(**row))
+        width, wpl, rg = tune.bvh_width, tune.walks_per_lane, tune.regroup_interval
+    batch = batches[width][1]
+    stats = E.DeviceStats.zeros(n)
+    o = E.SolveOptions(seed=args.seed, stream_offset=0, walks_per_lane=wpl, regroup_interval=rg,
+                       step_interval=args.step_interval, distance_tolerance=args.tolerance)
+    graph = E.SolveGraph(batch, probs[0].settings, dp, IOP, stats, o)
+    warm_stats = E.DeviceStats.zeros(n)
+    for w in range(args.warmup):
+        E.batch_wos_solve(batch, probs[0].settings, dp, IOP, warm_stats, E.SolveOptions(
+            seed=args.seed + 1, walks_per_lane=wpl, regroup_interval=rg, step_interval=args.step_interval,
+            distance_tolerance=args.tolerance))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    steps = walks_done = 0
+    for s in range(args.steps):
+        graph.launch()
+        r = graph.report()          # synchronises: counters of this replay
+        steps += r.total_steps; walks_done += r.total_walks
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    graph.close()
+    rc = E.batch_wos_solve(batch, probs[0].settings, dp, IOP, E.DeviceStats.zeros(n), E.SolveOptions(
+        seed=args.seed + 3, walks_per_lane=wpl, regroup_interval=rg, step_interval=args.step_interval,
+        distance_tolerance=args.tolerance, count_visits=True))
+    mean = stats.mean.cpu().numpy()
+    var = stats.variance().cpu().numpy()
+    nn = np.maximum(stats.n.cpu().numpy(), 1)
+    err = mean - exact
+    pooled = float(err.sum() / np.sqrt((var / nn).sum()))
+    res = _result("batched screened Poisson, one graph-captured launch", {
+        "workload": f"{len(probs)} screened-Poisson instances, {n_tris} triangles, {n} points, {walks} walks/pt/pass, "
+                    "eps 1e-4", "bvh_width": width, "leaf_size": args.leaf_size, "walks_per_lane": wpl,
+        "regroup_interval": rg, "step_interval": args.step_interval, "distance_tolerance": args.tolerance,
+        "points": n, "instances": len(probs), "autotuned": tune is not None}, steps / el, el, args,
+        {"walks_per_s": walks_done / el, "mean_walk_length": steps / max(walks_done, 1),
+         "rmse_vs_analytic": float(np.sqrt(np.mean(err ** 2))), "pooled_z_vs_analytic": pooled,
+         "roofline": _roofline(rc, scenes[0].info.node_bytes, scenes[0].info.prim_bytes, rc.kernel_ms,
+                               rc.total_steps, "closest-point bytes touched per step (instrumented pass)")})
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        i0 = int(np.argmin([abs(sp["n_tris"] - 43000) for sp in spec]))   # an instance of typical size
+        sel = (IOP == i0).nonzero().flatten()[:2000]
+        pr = probs[i0]
+        osc = O.Scene(pr.positions, pr.indices, None, pr.domain_min, pr.domain_max)
+        pts0, d0 = P[sel].cpu().numpy(), D[sel].cpu().numpy()
+        res["cpu_baseline"] = _oracle_baseline(
+            f"instance {i0} ({spec[i0]['n_tris']} triangles, sigma {spec[i0]['sigma']:.2f}): {len(pts0)} points x "
+            f"{walks} walks, exact radii", lambda: O.solve(osc, pr.pde, pr.settings, pts0, d0, walks, O.Stats(len(pts0)),
+                                                         seed=7, rng_mode=O.RNG_PER_POINT, threads=threads), threads)
+        res["speedup_vs_cpu_baseline"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
 
 
 if __name__ == "__main__":

@@ -44,8 +44,8 @@ def lib():
         L.orc_closest_point.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, C.c_int, P, C.c_int]
         L.orc_intersect_ray.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, P, C.c_int]
         L.orc_inside_domain.argtypes = [P, C.c_int64, P, P, C.c_int]
-        L.orc_star_radius.argtypes = [P, C.c_int64, P, P, P, C.c_float, P]
-        L.orc_silhouette_candidates.argtypes = [P, P, C.c_float, P, P]
+        L.orc_star_radius.argtypes = [P, C.c_int64, P, P, P, C.c_float, P, P]
+        L.orc_silhouette_candidates.argtypes = [P, P, C.c_float, C.c_int, P, P]
         L.orc_silhouette_candidates.restype = C.c_int64
         L.orc_sample_reflecting.argtypes = [P, P, C.c_float, P, P, P, P]
         L.orc_sample_reflecting.restype = C.c_int
@@ -170,21 +170,22 @@ class Scene:
         lib().orc_inside_domain(self.h, len(pts), _p(pts), _p(out), threads or default_threads())
         return out.astype(bool)
 
-    def star_radius(self, pts, min_r, max_r, precision=1e-3):
+    def star_radius(self, pts, min_r, max_r, precision=1e-3, flip=None):
         pts = np.ascontiguousarray(pts, np.float32)
         n = len(pts)
         mn = np.ascontiguousarray(np.broadcast_to(min_r, n), np.float32)
         mx = np.ascontiguousarray(np.broadcast_to(max_r, n), np.float32)
+        fl = None if flip is None else np.ascontiguousarray(np.broadcast_to(flip, n), np.uint8)
         out = np.empty(n, np.float32)
-        lib().orc_star_radius(self.h, n, _p(pts), _p(mn), _p(mx), precision, _p(out))
+        lib().orc_star_radius(self.h, n, _p(pts), _p(mn), _p(mx), precision, _p(fl), _p(out))
         return out
 
-    def silhouette_candidates(self, x, precision=1e-3):
+    def silhouette_candidates(self, x, precision=1e-3, flip=False):
         x = np.ascontiguousarray(x, np.float32)
-        n = lib().orc_silhouette_candidates(self.h, _p(x), precision, None, None)
+        n = lib().orc_silhouette_candidates(self.h, _p(x), precision, int(flip), None, None)
         v = np.empty((n, 2), np.float32)
         s = np.empty(n, np.uint8)
-        lib().orc_silhouette_candidates(self.h, _p(x), precision, _p(v), _p(s))
+        lib().orc_silhouette_candidates(self.h, _p(x), precision, int(flip), _p(v), _p(s))
         return v, s.astype(bool)
 
     def sample_reflecting(self, x, radius, u):

@@ -588,17 +588,27 @@ struct Mesh {
         }
     }
 
-    bool isSilhouette(const SilVertex& s, const V<2>& x, float precision) const {
+    // Silhouette vertex seen from x.  With d0, d1 the cosines between the adjacent segments'
+    // normals and the unit direction v -> x: a vertex whose neighbours lie on the same side of
+    // the line (x, v) is a silhouette, i.e. d0 * d1 < 0.  When x lies (within the precision)
+    // on the line of one adjacent segment, the vertex is a silhouette only when the other
+    // segment turns towards the far side, sign * d_other > precision: a concave corner seen
+    // from the walker's side, whose grazing ray bounds a region the walker cannot see.  sign
+    // = +1 for normals pointing away from the walker (outward normals of a domain walked from
+    // inside), -1 when the walker's normal orientation is flipped.  A vertex with one
+    // neighbour (an open end) and a query at the vertex itself are always silhouettes.
+    bool isSilhouette(const SilVertex& s, const V<2>& x, float precision, bool flip) const {
         if (s.p1 < 0) return true;
         V<2> v = verts[s.v];
         V<2> view = x - v;
         float l = norm(view);
         if (l <= 0.0f) return true;
         view = view * (1.0f / l);
-        float f0 = dot(primNormal(s.p0), view), f1 = dot(primNormal(s.p1), view);
-        bool bothFront = f0 > precision && f1 > precision;
-        bool bothBack = f0 < -precision && f1 < -precision;
-        return !(bothFront || bothBack);
+        float sign = flip ? -1.0f : 1.0f;
+        float d0 = dot(primNormal(s.p0), view), d1 = dot(primNormal(s.p1), view);
+        if (std::fabs(d0) <= precision) return sign * d1 > precision;
+        if (std::fabs(d1) <= precision) return sign * d0 > precision;
+        return d0 * d1 < 0.0f;
     }
 };
 
@@ -698,14 +708,15 @@ int countHits(const Mesh<D>& m, const V<D>& o, const V<D>& dir) {
 }
 
 // closest silhouette point (2D), fcpw_geometric_queries.h:1254-1268
-inline float starRadius2(const Mesh<2>& m, const V<2>& x, float minRadius, float maxRadius, float precision) {
+inline float starRadius2(const Mesh<2>& m, const V<2>& x, float minRadius, float maxRadius, float precision,
+                         bool flip) {
     if (minRadius > maxRadius) return maxRadius;
     float best = FLT_MAX; bool found = false;
     for (const auto& s : m.sil) {
         float d = norm(x - m.verts[s.v]);
         if (d > maxRadius) continue;
         if (d >= best) continue;
-        if (m.isSilhouette(s, x, precision)) { best = d; found = true; }
+        if (m.isSilhouette(s, x, precision, flip)) { best = d; found = true; }
     }
     return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
 }
@@ -962,10 +973,9 @@ void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBa
 template <int D>
 float wostStarRadius(const Scene<D>& sc, const SolveCtx& cx, const V<D>& x, float distA, bool flip) {
     const zg_walk_settings& ws = *cx.ws;
-    (void)flip;
     float r = distA;
     if constexpr (D == 2) {
-        if (sc.hasRefl()) r = starRadius2(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision);
+        if (sc.hasRefl()) r = starRadius2(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip);
     }
     if (sc.hasRefl() && ws.epsilon_shell_for_reflecting_boundary <= distA)
         r = std::max(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
@@ -1317,20 +1327,23 @@ void orc_inside_domain(void* hv, int64_t n, const float* pts, uint8_t* inside, i
     else insideImpl<3>(*h->s3, n, pts, inside, nThreads);
 }
 
-void orc_star_radius(void* hv, int64_t n, const float* pts, const float* minR, const float* maxR, float prec, float* out) {
+void orc_star_radius(void* hv, int64_t n, const float* pts, const float* minR, const float* maxR, float prec,
+                     const uint8_t* flip, float* out) {
     SceneHandle* h = (SceneHandle*)hv;
     if (h->dim != 2) { for (int64_t i = 0; i < n; i++) out[i] = maxR[i]; return; }
-    for (int64_t i = 0; i < n; i++) out[i] = h->s2->hasRefl() ? starRadius2(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec) : maxR[i];
+    for (int64_t i = 0; i < n; i++)
+        out[i] = h->s2->hasRefl() ? starRadius2(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec,
+                                                flip && flip[i] != 0) : maxR[i];
 }
 
 // silhouette test of every candidate vertex seen from x; out[k] = 1 when candidate k is a silhouette
-int64_t orc_silhouette_candidates(void* hv, const float* x, float prec, float* verts_out, uint8_t* is_sil_out) {
+int64_t orc_silhouette_candidates(void* hv, const float* x, float prec, int flip, float* verts_out, uint8_t* is_sil_out) {
     SceneHandle* h = (SceneHandle*)hv;
     if (h->dim != 2) return 0;
     const auto& m = h->s2->refl;
     for (size_t k = 0; k < m.sil.size(); k++) {
         if (verts_out) store<2>(verts_out + 2 * k, m.verts[m.sil[k].v]);
-        if (is_sil_out) is_sil_out[k] = m.isSilhouette(m.sil[k], load<2>(x), prec);
+        if (is_sil_out) is_sil_out[k] = m.isSilhouette(m.sil[k], load<2>(x), prec, flip != 0);
     }
     return (int64_t)m.sil.size();
 }

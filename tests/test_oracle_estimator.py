@@ -97,7 +97,7 @@ def test_screened_poisson_matches_exact_and_rr_unbiased(oracle_lib):
 
 def test_mixed_2d_walk_on_stars_matches_exact(oracle_lib):
     O = oracle_lib
-    p = PB.mixed2d(n_segments=1024, n_slits=16)
+    p = PB.mixed2d(n_segments=1024, n_slits=8)
     sc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
     V, I = p.meta["outer"]
     cand = PB.sample_box(p.domain_min, p.domain_max, 1500, 9)

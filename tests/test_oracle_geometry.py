@@ -113,6 +113,66 @@ def test_silhouette_star_radius_invariants(oracle_lib):
         assert np.linalg.norm(verts[k] - e) < 1e-6 and sil[k]
 
 
+def _outer_reflecting_chain(prob):
+    """Vertices of the reflecting half of the star curve with both neighbours reflecting:
+    (vertex, previous vertex, next vertex) in the curve's counter-clockwise order."""
+    V, I = prob.meta["outer"]
+    refl = prob.is_reflecting[: len(I)].astype(bool)
+    out = []
+    for k in range(len(I)):
+        kn = (k + 1) % len(I)
+        if refl[k] and refl[kn]:
+            out.append((V[I[k, 1]], V[I[k, 0]], V[I[kn, 1]]))
+    return out
+
+
+def test_silhouette_rule_matches_visibility_geometry(oracle_lib):
+    """Generic views: a vertex is a silhouette exactly when its two neighbours lie on the same
+    side of the line through the viewer and the vertex.  Views along an adjacent segment
+    (the walker on the boundary): the vertex is a silhouette exactly when the corner is
+    concave (a right turn of the counter-clockwise curve), with normals pointing away from
+    the walker; flipping the orientation flips the answer."""
+    prob = PB.mixed2d(n_segments=256, n_slits=1)
+    sc = oracle_lib.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,
+                          solve_double_sided=True)
+    chain = _outer_reflecting_chain(prob)
+    assert len(chain) > 100
+
+    def cross(a, b):
+        return a[0] * b[1] - a[1] * b[0]
+
+    rng = np.random.default_rng(12)
+    Q = rng.uniform(-0.5, 0.5, size=(40, 2)).astype(np.float32)
+    checked = 0
+    for q in Q:
+        verts, sil = sc.silhouette_candidates(q)
+        for v, u0, u1 in chain[::5]:
+            k = np.argmin(np.linalg.norm(verts - v, axis=1))
+            assert np.linalg.norm(verts[k] - v) < 1e-6
+            d = (q - v).astype(np.float64)
+            s0 = cross(d, (u0 - v).astype(np.float64)) / (np.linalg.norm(d) * np.linalg.norm(u0 - v))
+            s1 = cross(d, (u1 - v).astype(np.float64)) / (np.linalg.norm(d) * np.linalg.norm(u1 - v))
+            if min(abs(s0), abs(s1)) < 1e-2:
+                continue                               # near-grazing views: covered below
+            assert sil[k] == (s0 * s1 > 0), (q, v)
+            checked += 1
+    assert checked > 500
+    # the walker on a segment, looking at the segment's end vertex
+    tangent = 0
+    for v, u0, u1 in chain:
+        x = (0.5 * (u0 + v)).astype(np.float32)        # on the segment u0 -> v
+        turn = cross((v - u0).astype(np.float64), (u1 - v).astype(np.float64))
+        if abs(turn) < 1e-4 * np.linalg.norm(v - u0) * np.linalg.norm(u1 - v):
+            continue
+        for flip in (False, True):
+            verts, sil = sc.silhouette_candidates(x, flip=flip)
+            k = np.argmin(np.linalg.norm(verts - v, axis=1))
+            concave = turn < 0
+            assert sil[k] == (concave != flip), (v, turn, flip)
+        tangent += 1
+    assert tangent > 50
+
+
 def test_reflecting_sampler_is_unbiased_for_boundary_length(oracle_lib):
     """E[1/pdf * 1{sample in ball}] equals the length of reflecting boundary inside the ball."""
     prob = PB.mixed2d(n_segments=512, n_slits=8)

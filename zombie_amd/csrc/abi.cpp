@@ -287,10 +287,9 @@ zg_status zg_intersect_ray(const zg_scene* s, int32_t boundary, int64_t n, const
 
 zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const float* minR, const float* maxR,
                          float prec, const uint8_t* flip, float* radius, void* stream) {
-    (void)flip;  // the 2D silhouette-vertex test is symmetric in the normal orientation
     if (!s || n < 0 || (n > 0 && (!pts || !minR || !maxR || !radius))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
     if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "star radius queries run on 2D scenes");
-    ZG_HIP(launchStarRadius(s->view, n, pts, minR, maxR, prec, radius, (hipStream_t)stream));
+    ZG_HIP(launchStarRadius(s->view, n, pts, minR, maxR, prec, flip, radius, (hipStream_t)stream));
     return ok();
 }
 

@@ -716,20 +716,26 @@ ZG_DEV Vec<2> segmentHitPoint(const float* prims, int i, const Vec<2>& o, const 
     return pr.a + v * s;
 }
 
-// isSilhouetteVertex for a 2D candidate (v, n0, n1, single): a vertex with one neighbour is
-// always a silhouette; otherwise it is one unless both adjacent segments face x, or both
-// face away, by more than the precision.
-ZG_DEV bool isSilhouette2(const float4& c0, const float4& c1, const Vec<2>& x, float precision) {
+// Silhouette test of a 2D candidate (v, n0, n1, single) seen from x.  With d0, d1 the cosines
+// between the adjacent segments' normals and the unit direction v -> x, a vertex whose
+// neighbours lie on the same side of the line (x, v) is a silhouette: d0 * d1 < 0.  When x
+// lies (within the precision) on the line of one adjacent segment, the vertex is a
+// silhouette only when the other segment turns to the far side, sign * d_other > precision
+// (a concave corner seen from the walker's side); sign = +1 for normals pointing away from
+// the walker, -1 when the walker's normal orientation is flipped.  Open ends (one neighbour)
+// and a query at the vertex itself are always silhouettes.
+ZG_DEV bool isSilhouette2(const float4& c0, const float4& c1, const Vec<2>& x, float precision, bool flip) {
     if (c1.z != 0.0f) return true;
     Vec<2> view; view[0] = x[0] - c0.x; view[1] = x[1] - c0.y;
-    float l2 = norm2(view);
-    if (l2 <= 0.0f) return true;
-    float inv = 1.0f / __builtin_sqrtf(l2);
-    float f0 = (c0.z * view[0] + c0.w * view[1]) * inv;
-    float f1 = (c1.x * view[0] + c1.y * view[1]) * inv;
-    bool bothFront = f0 > precision && f1 > precision;
-    bool bothBack = f0 < -precision && f1 < -precision;
-    return !(bothFront || bothBack);
+    float l = __builtin_sqrtf(norm2(view));
+    if (l <= 0.0f) return true;
+    view = view * (1.0f / l);
+    float sign = flip ? -1.0f : 1.0f;
+    float d0 = c0.z * view[0] + c0.w * view[1];
+    float d1 = c1.x * view[0] + c1.y * view[1];
+    if (fabsf(d0) <= precision) return sign * d1 > precision;
+    if (fabsf(d1) <= precision) return sign * d0 > precision;
+    return d0 * d1 < 0.0f;
 }
 
 // computeStarRadiusForReflectingBoundary (fcpw_geometric_queries.h:1254-1268): distance to
@@ -738,7 +744,7 @@ ZG_DEV bool isSilhouette2(const float4& c0, const float4& c1, const Vec<2>& x, f
 // silhouette-candidate BVH, stopped once a silhouette closer than minRadius is found.
 template <int W>
 ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius, float maxRadius, float precision,
-                         Stack& st) {
+                         bool flip, Stack& st) {
     if (minRadius > maxRadius) return maxRadius;
     if (b.nSilNodes == 0) return fmaxf(maxRadius, minRadius);
     float best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
@@ -759,7 +765,7 @@ ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius
                 float4 c0 = c[0], c1 = c[1];
                 float dx = x[0] - c0.x, dy = x[1] - c0.y;
                 float dd = dx * dx + dy * dy;
-                if (dd < best2 && isSilhouette2(c0, c1, x, precision)) { best2 = dd; found = true; }
+                if (dd < best2 && isSilhouette2(c0, c1, x, precision, flip)) { best2 = dd; found = true; }
             }
         }
         if (found && best2 <= min2) break;

@@ -104,7 +104,7 @@ hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
 hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s);
 hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
 hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
-                            float prec, float* radius, hipStream_t s);
+                            float prec, const uint8_t* flip, float* radius, hipStream_t s);
 hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, const float* p2, uint8_t* visible,
                              hipStream_t s);
 hipError_t launchSampleReflecting(const SceneView& sv, int64_t n, const float* x, const float* radius, const float* u,

@@ -28,10 +28,11 @@ namespace zg {
 // star radius of a walk: closest silhouette within [epsR, distA], shrunk by 1 % for numerical
 // robustness when the reflecting boundary is not empty (walk_on_stars.h:370-392,576-590)
 template <int W>
-ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, const Vec<2>& x, float distA, Stack& st) {
+ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, const Vec<2>& x, float distA, bool flip,
+                            Stack& st) {
     float r = distA;
     if (sc.refl.nPrims > 0) {
-        r = starRadius2<W>(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, st);
+        r = starRadius2<W>(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip, st);
         if (ws.epsilon_shell_for_reflecting_boundary <= distA)
             r = fmaxf(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
     }
@@ -54,7 +55,10 @@ __global__ void __launch_bounds__(256) wostFirstSphereKernel(WosParams P, const 
     if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) r = 0.0f;
     else if (callerFsr && P.prevN && P.prevN[i] > 0) r = callerFsr[i];
     else if (d <= ws.epsilon_shell_for_absorbing_boundary || ws.steps_before_using_maximal_spheres == 0) r = d;
-    else r = wostStarRadius<W>(P.scene, ws, vload<2>(P.pt + 2 * i), d, st);
+    else {
+        bool flip = ws.solve_double_sided && ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && P.aligned && P.aligned[i];
+        r = wostStarRadius<W>(P.scene, ws, vload<2>(P.pt + 2 * i), d, flip, st);
+    }
     fsr[i] = r;
 }
 
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 nrm = nrm * -1.0f;
                 flip = true;
             }
-            R = ws.steps_before_using_maximal_spheres <= len ? distA : wostStarRadius<W>(sc, ws, x, distA, st);
+            R = ws.steps_before_using_maximal_spheres <= len ? distA : wostStarRadius<W>(sc, ws, x, distA, flip, st);
         }
         ball.update(R);
         Vec<2> dir = sampleUnitSphere<2>(rng);
@@ -281,12 +285,14 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
 template <int W>
 __global__ void __launch_bounds__(256) starRadiusKernel(SceneView sv, int64_t n, const float* __restrict__ pts,
                                                         const float* __restrict__ minR, const float* __restrict__ maxR,
-                                                        float prec, float* __restrict__ radius) {
+                                                        float prec, const uint8_t* __restrict__ flip,
+                                                        float* __restrict__ radius) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
-    radius[i] = sv.refl.nPrims == 0 ? maxR[i] : starRadius2<W>(sv.refl, vload<2>(pts + 2 * i), minR[i], maxR[i], prec, st);
+    bool fl = flip && flip[i] != 0;
+    radius[i] = sv.refl.nPrims == 0 ? maxR[i] : starRadius2<W>(sv.refl, vload<2>(pts + 2 * i), minR[i], maxR[i], prec, fl, st);
 }
 
 template <int W>
@@ -341,11 +347,11 @@ hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks,
 }
 
 hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
-                            float prec, float* radius, hipStream_t s) {
+                            float prec, const uint8_t* flip, float* radius, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 g(wostBlocksFor(n)), bl(256);
-    if (sv.width == 4) hipLaunchKernelGGL((starRadiusKernel<4>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, radius);
-    else hipLaunchKernelGGL((starRadiusKernel<8>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, radius);
+    if (sv.width == 4) hipLaunchKernelGGL((starRadiusKernel<4>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, flip, radius);
+    else hipLaunchKernelGGL((starRadiusKernel<8>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, flip, radius);
     return hipGetLastError();
 }
 

@@ -242,7 +242,11 @@ def sphere3d(level=6, seed=0x5EED, n_walks=256, eps=1e-4, max_len=1024, k=2.0) -
                    lo.astype(np.float32), hi.astype(np.float32), pde, ws, n_walks, exact_fn(sin_product, k=k))
 
 
-def mixed2d(n_segments=4096, n_slits=32, seed=0x5EED, n_walks=1024, eps=1e-3, max_len=1024) -> Problem:
+def mixed2d(n_segments=4096, n_slits=32, seed=0x5EED, n_walks=1024, eps=1e-3, max_len=16384) -> Problem:
+    """Star curve (Dirichlet on its lower half, Neumann on the upper half) with n_slits interior
+    Neumann slits (double-sided), u = x^3 - 3xy^2.  Walks bounce between the slits (about 1,000
+    steps on average with 32 slits); a walk that stops at max_len loses its terminal
+    contribution, so the default max_len leaves no walk truncated."""
     V, I = star_curve(n_segments)
     rng = np.random.default_rng(seed)
     slits = []
@@ -313,6 +317,20 @@ def screened_instance(n_tris: int, sigma: float, direction, seed: int, n_walks=1
     prob.meta["sigma"] = sigma
     prob.meta["a"] = a
     return prob
+
+
+def screened_batch_spec(n_instances: int = 256, seed: int = 0x5EED, tri_range=(2000, 200000),
+                        point_range=(1000, 100000), sigma_max: float = 10.0):
+    """Instance parameters of the batched screened-Poisson workload: triangle and point
+    counts log-uniform over the ranges, sigma ~ U[0, sigma_max], a uniform direction for
+    a = sqrt(sigma) * direction.  Returns dicts (n_tris, n_points, sigma, direction, seed)."""
+    rng = np.random.default_rng(seed)
+    lt = rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]), n_instances)
+    lp = rng.uniform(math.log(point_range[0]), math.log(point_range[1]), n_instances)
+    sig = rng.uniform(0.0, sigma_max, n_instances)
+    dirs = rng.normal(size=(n_instances, 3))
+    return [dict(n_tris=int(round(math.exp(lt[i]))), n_points=int(round(math.exp(lp[i]))), sigma=float(sig[i]),
+                 direction=dirs[i], seed=int(seed + 1 + i)) for i in range(n_instances)]
 
 
 def sample_box(lo, hi, n, seed):
