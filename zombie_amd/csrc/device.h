@@ -436,16 +436,21 @@ ZG_DEV void childDistances(const float* nodes, int idx, const Vec<D>& x, float (
     }
 }
 
-// Resumable closest-point query on one boundary part.  A query visits one BVH node per call
-// of visit(), so a persistent kernel can interleave the traversals of its lanes with their
-// walk steps instead of making every lane of a wave wait for the slowest traversal.
+// Resumable closest-point query on one boundary part.  A query advances by one work item per
+// call of visit(), so a persistent kernel can interleave the traversals of its lanes with
+// their walk steps instead of making every lane of a wave wait for the slowest traversal.
 //
-// Order: the children of a node are handled nearest first -- leaves are tested in ascending
-// box distance while their box still lies within the pruning bound (which shrinks as
-// primitives are found), the nearest surviving inner child is visited next and the other
-// surviving inner children are pushed farthest-first so that the stack pops them
-// nearest-first.  (Role of fcpw Aggregate::findClosestPoint, fcpw_geometric_queries.h:
-// 1069-1096.)
+// Work items are either an inner node (the squared distances to its W child boxes) or one
+// primitive of a leaf: leaves are not tested when their parent is visited but queued like
+// inner children, and a leaf of n primitives takes n items.  Both kinds of item cost about
+// the same number of instructions, so the lanes of a wave stay in step (testing all leaf
+// children of a node at once made one lane's visit cost up to W * leaf-size triangle tests
+// while its neighbours idled).
+//
+// Order: the surviving children of a node (inner or leaf, box distance within the pruning
+// bound) are sorted by a compare-exchange network; the nearest becomes the next item, the
+// others are pushed farthest-first so the stack pops them nearest-first.  (Role of fcpw
+// Aggregate::findClosestPoint, fcpw_geometric_queries.h:1069-1096.)
 //
 // Bound: best2 starts at the caller's bound (FLT_MAX for an unbounded search); a query that
 // finds nothing within it is restarted once without bound.
@@ -463,100 +468,95 @@ struct ClosestQuery {
     float prune2;    // current pruning bound
     float lower2;    // smallest box distance^2 pruned by the relaxation
     float relax2, guard2;
-    int node;        // node to visit next, -1 = pop
+    float itemD2;    // box distance^2 of the current item
+    int item;        // current item: inner node index (>= 0) or leaf code (< 0)
     int prim;        // packed index of the best primitive
+    bool has;        // an item is held (else pop one)
     bool found;
     bool retried;
 
     ZG_DEV void begin(float bound2, float relax2_, float guard2_) {
         best2 = bound2; relax2 = relax2_; guard2 = guard2_;
         prune2 = best2 > guard2 ? best2 * relax2 : best2;
-        lower2 = FLT_MAX; node = 0; prim = -1; found = false; retried = false;
+        lower2 = FLT_MAX; item = 0; itemD2 = 0.0f; has = true; prim = -1; found = false; retried = false;
     }
     ZG_DEV float radius2() const { return fminf(best2, lower2); }
+    ZG_DEV void prunedBox(float d2) { if (d2 <= best2) lower2 = fminf(lower2, d2); }
 
-    // visits one node of the view b (anything with nodes, prims, nNodes); returns true when
-    // the query is complete
+    template <int W>
+    ZG_DEV static void cmpx(float (&k)[W], int (&v)[W], int i, int j) {
+        bool sw = k[j] < k[i];
+        float ki = k[i], kj = k[j];
+        int vi = v[i], vj = v[j];
+        k[i] = sw ? kj : ki; k[j] = sw ? ki : kj;
+        v[i] = sw ? vj : vi; v[j] = sw ? vi : vj;
+    }
+    // ascending sort network (Batcher odd-even merge for 8, optimal for 4)
+    template <int W>
+    ZG_DEV static void sortAscending(float (&k)[W], int (&v)[W]) {
+        if constexpr (W == 4) {
+            cmpx<4>(k, v, 0, 1); cmpx<4>(k, v, 2, 3); cmpx<4>(k, v, 0, 2); cmpx<4>(k, v, 1, 3); cmpx<4>(k, v, 1, 2);
+        } else {
+            cmpx<8>(k, v, 0, 1); cmpx<8>(k, v, 2, 3); cmpx<8>(k, v, 4, 5); cmpx<8>(k, v, 6, 7);
+            cmpx<8>(k, v, 0, 2); cmpx<8>(k, v, 1, 3); cmpx<8>(k, v, 4, 6); cmpx<8>(k, v, 5, 7);
+            cmpx<8>(k, v, 1, 2); cmpx<8>(k, v, 5, 6);
+            cmpx<8>(k, v, 0, 4); cmpx<8>(k, v, 1, 5); cmpx<8>(k, v, 2, 6); cmpx<8>(k, v, 3, 7);
+            cmpx<8>(k, v, 2, 4); cmpx<8>(k, v, 3, 5);
+            cmpx<8>(k, v, 1, 2); cmpx<8>(k, v, 3, 4); cmpx<8>(k, v, 5, 6);
+        }
+    }
+
+    // advances the query on the view b (anything with nodes, prims, nNodes) by one item;
+    // returns true when the query is complete
     template <int W, bool COUNT, class View>
     ZG_DEV bool visit(const View& b, const Vec<D>& x, Stack& st, VisitCount& vc) {
         if (b.nNodes == 0) return true;
-        if (node < 0) {
+        if (!has) {
             int n; float dn;
             bool got = false;
             while (st.pop(n, dn)) {
                 if (dn <= prune2) { got = true; break; }
-                if (dn <= best2) lower2 = fminf(lower2, dn);
+                prunedBox(dn);
             }
             if (!got) {
                 if (found || retried) return true;
                 // nothing within the initial bound: search again without it
                 retried = true;
                 best2 = FLT_MAX; prune2 = FLT_MAX; lower2 = FLT_MAX;
-                n = 0;
+                n = 0; dn = 0.0f;
             }
-            node = n;
+            item = n; itemD2 = dn; has = true;
+        }
+        if (isLeaf(item)) {
+            // one primitive of a leaf
+            int first = leafFirstD(item), cnt = leafCountD(item);
+            Prim<D> pr; pr.load(b.prims, first);
+            Vec<D> qq;
+            float dd = pr.closest2(x, qq);
+            if (COUNT) vc.prims++;
+            if (dd < best2) {
+                best2 = dd; cp = qq; prim = first; found = true;
+                prune2 = best2 > guard2 ? best2 * relax2 : best2;
+            }
+            has = cnt > 1 && itemD2 <= prune2;
+            if (cnt > 1 && !has) prunedBox(itemD2);
+            item = ~(((first + 1) << 4) | (cnt - 2));
+            return false;
         }
         float d2[W];
         int ch[W];
-        childDistances<D, W>(b.nodes, node, x, d2, ch);
+        childDistances<D, W>(b.nodes, item, x, d2, ch);
         if (COUNT) vc.nodes++;
-        uint32_t cand = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            cand |= (d2[k] <= prune2 ? 1u : 0u) << k;
-            if (d2[k] > prune2 && d2[k] <= best2) lower2 = fminf(lower2, d2[k]);
+            if (d2[k] > prune2) { prunedBox(d2[k]); d2[k] = FLT_MAX; }
         }
-        uint32_t inner = 0;
-        while (cand) {
-            float m = FLT_MAX; int bk = 0; int c = 0;
+        sortAscending<W>(d2, ch);
+        has = d2[0] < FLT_MAX;
+        item = ch[0]; itemD2 = d2[0];
 #pragma unroll
-            for (int k = 0; k < W; k++) {
-                bool take = ((cand >> k) & 1u) && d2[k] <= m;
-                m = take ? d2[k] : m; bk = take ? k : bk; c = take ? ch[k] : c;
-            }
-            cand &= ~(1u << bk);
-            if (m > prune2) {
-                if (m <= best2) lower2 = fminf(lower2, m);
-                continue;
-            }
-            if (isLeaf(c)) {
-                int first = leafFirstD(c), cnt = leafCountD(c);
-                for (int i = first; i < first + cnt; i++) {
-                    Prim<D> pr; pr.load(b.prims, i);
-                    Vec<D> qq;
-                    float dd = pr.closest2(x, qq);
-                    if (COUNT) vc.prims++;
-                    if (dd < best2) {
-                        best2 = dd; cp = qq; prim = i; found = true;
-                        prune2 = best2 > guard2 ? best2 * relax2 : best2;
-                    }
-                }
-            } else {
-                inner |= 1u << bk;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            if (((inner >> k) & 1u) && d2[k] > prune2) {
-                inner &= ~(1u << k);
-                if (d2[k] <= best2) lower2 = fminf(lower2, d2[k]);
-            }
-        }
-        node = -1;
-        if (inner) {
-            while (__builtin_popcount(inner) > 1) {
-                float m = -1.0f; int bk = 0; int c = 0;
-#pragma unroll
-                for (int k = 0; k < W; k++) {
-                    bool take = ((inner >> k) & 1u) && d2[k] > m;
-                    m = take ? d2[k] : m; bk = take ? k : bk; c = take ? ch[k] : c;
-                }
-                inner &= ~(1u << bk);
-                st.push(c, m);
-            }
-#pragma unroll
-            for (int k = 0; k < W; k++) if ((inner >> k) & 1u) node = ch[k];
-        }
+        for (int k = W - 1; k >= 1; k--)
+            if (d2[k] < FLT_MAX) st.push(ch[k], d2[k]);
         return false;
     }
 };
