@@ -359,7 +359,7 @@ ZG_DEV int leafCountD(int c) { return ((~c) & 15) + 1; }
 // laid out [depth][blockDim], so lanes at the same depth hit consecutive banks); deeper
 // entries spill to a per-lane private array (scratch) owned by the caller, touched only by
 // rare deep traversals.  Kernels that traverse launch with kStackLdsBytes of dynamic LDS.
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 12;
 constexpr int kSpillStack = 96;
 constexpr int kStackLdsBytes = kLdsStack * 256 * 8;
 
@@ -482,6 +482,14 @@ struct ClosestQuery {
     }
     ZG_DEV float radius2() const { return fminf(best2, lower2); }
     ZG_DEV void prunedBox(float d2) { if (d2 <= best2) lower2 = fminf(lower2, d2); }
+    // a known point of the boundary (squared distance d2 from the query) tightens the bound
+    // before the traversal starts
+    ZG_DEV void seed(const Vec<D>& p, int primIdx, float d2) {
+        if (d2 < best2) {
+            best2 = d2; cp = p; prim = primIdx; found = true;
+            prune2 = best2 > guard2 ? best2 * relax2 : best2;
+        }
+    }
 
     template <int W>
     ZG_DEV static void cmpx(float (&k)[W], int (&v)[W], int i, int j) {
@@ -527,8 +535,27 @@ struct ClosestQuery {
             }
             item = n; itemD2 = dn; has = true;
         }
+        if (!isLeaf(item)) {
+            // an inner node: its nearest surviving child becomes the next item
+            float d2[W];
+            int ch[W];
+            childDistances<D, W>(b.nodes, item, x, d2, ch);
+            if (COUNT) vc.nodes++;
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                if (d2[k] > prune2) { prunedBox(d2[k]); d2[k] = FLT_MAX; }
+            }
+            sortAscending<W>(d2, ch);
+            has = d2[0] < FLT_MAX;
+            item = ch[0]; itemD2 = d2[0];
+#pragma unroll
+            for (int k = W - 1; k >= 1; k--)
+                if (d2[k] < FLT_MAX) st.push(ch[k], d2[k]);
+            if (!has) return false;
+        }
         if (isLeaf(item)) {
-            // one primitive of a leaf
+            // one primitive of a leaf (in the same call as the node that led to it, so a call
+            // costs at most one node and one primitive)
             int first = leafFirstD(item), cnt = leafCountD(item);
             Prim<D> pr; pr.load(b.prims, first);
             Vec<D> qq;
@@ -541,22 +568,7 @@ struct ClosestQuery {
             has = cnt > 1 && itemD2 <= prune2;
             if (cnt > 1 && !has) prunedBox(itemD2);
             item = ~(((first + 1) << 4) | (cnt - 2));
-            return false;
         }
-        float d2[W];
-        int ch[W];
-        childDistances<D, W>(b.nodes, item, x, d2, ch);
-        if (COUNT) vc.nodes++;
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            if (d2[k] > prune2) { prunedBox(d2[k]); d2[k] = FLT_MAX; }
-        }
-        sortAscending<W>(d2, ch);
-        has = d2[0] < FLT_MAX;
-        item = ch[0]; itemD2 = d2[0];
-#pragma unroll
-        for (int k = W - 1; k >= 1; k--)
-            if (d2[k] < FLT_MAX) st.push(ch[k], d2[k]);
         return false;
     }
 };
