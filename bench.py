@@ -198,6 +198,7 @@ def main():
     # HBM/MALL traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # the same kernel configuration, scaled to this launch's step count
     traffic = None
+    issue = {}
     prof = REPO / "profiles" / "wos3d_sphere81920_1M.json"
     if prof.exists():
         pj = json.loads(prof.read_text())
@@ -205,6 +206,8 @@ def main():
         # traffic depends on the scene layout and the radius tolerance, not on the launch shape
         if (c["bvh_width"], c["leaf_size"], c["distance_tolerance"]) == (args.bvh_width, leaf, tol):
             traffic = pj["traffic_bytes"] / pj["raw_steps"] * (total_steps / len(reports))
+            issue = {"valu_busy": pj.get("valu_busy"), "valu_lane_utilisation": pj.get("valu_lane_utilisation"),
+                     "issue_source": pj["source"]}
 
     # ---------------- learned control variate ----------------
     learned = None
@@ -284,9 +287,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": bytes_per_step, "nodes_per_step": nodes_per_step,
-                     "prims_per_step": prims_per_step, "kernel_ms": kernel_ms,
+                     "prims_per_step": prims_per_step, "kernel_ms": kernel_ms, **issue,
                      "note": "algorithmic bytes = BVH node + triangle bytes touched per step; the scene is "
-                             "cache-resident (L2/MALL), so HBM is not the binding level"},
+                             "cache-resident (L2/MALL), so HBM is not the binding level: the kernel is bound by "
+                             "VALU instruction issue under lane divergence (valu_busy, valu_lane_utilisation "
+                             "from the committed SQ counter profile)"},
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, P[: args.cpu_points].cpu().numpy(),
