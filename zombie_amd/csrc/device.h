@@ -163,6 +163,11 @@ struct Ball {
 // ------------------------------------------------------------------------------------
 // manufactured PDE data: zg_function descriptors (stand-ins for the PDE<T,DIM> callbacks)
 // ------------------------------------------------------------------------------------
+// The manufactured fields use the hardware sin/cos/exp (v_sin_f32, v_cos_f32, v_exp_f32;
+// absolute error near 1e-6 at the arguments these fields see, far below the Monte Carlo
+// error): the accurate library versions cost 5% of the walk kernel's time.
+ZG_DEV void fieldSinCos(float a, float* s, float* c) { __sincosf(a, s, c); }
+ZG_DEV float fieldExp(float a) { return __expf(a); }
 template <int D>
 ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, bool aligned, float absorption) {
     const float* p = f.params;
@@ -170,9 +175,9 @@ ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, b
     switch (f.kind) {
     case ZG_FN_CONSTANT: u = p[0]; break;
     case ZG_FN_EXP_COS: {
-        float e = expf(p[1] * x[0]);
+        float e = fieldExp(p[1] * x[0]);
         float s, c;
-        sincosf(p[1] * x[1], &s, &c);
+        fieldSinCos(p[1] * x[1], &s, &c);
         u = p[0] * e * c;
         g0 = p[0] * p[1] * e * c;
         g1 = -p[0] * p[1] * e * s;
@@ -180,7 +185,7 @@ ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, b
     }
     case ZG_FN_SIN_PRODUCT: {
         float s[3] = {1.0f, 1.0f, 1.0f}, c[3] = {0.0f, 0.0f, 0.0f};
-        for (int i = 0; i < D; i++) sincosf(p[1] * x[i], &s[i], &c[i]);
+        for (int i = 0; i < D; i++) fieldSinCos(p[1] * x[i], &s[i], &c[i]);
         u = p[0] * s[0] * s[1] * s[2];
         g0 = p[0] * p[1] * c[0] * s[1] * s[2];
         g1 = p[0] * p[1] * s[0] * c[1] * s[2];
@@ -197,7 +202,7 @@ ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, b
     }
     case ZG_FN_EXP_LINEAR: {
         float a = p[1] * x[0] + p[2] * x[1] + (D == 3 ? p[3] * x[D == 3 ? 2 : 0] : 0.0f);
-        u = p[0] * expf(a);
+        u = p[0] * fieldExp(a);
         g0 = p[1] * u; g1 = p[2] * u; g2 = D == 3 ? p[3] * u : 0.0f;
         float a2 = p[1] * p[1] + p[2] * p[2] + (D == 3 ? p[3] * p[3] : 0.0f);
         negLap = -a2 * u;
@@ -520,11 +525,13 @@ struct ClosestQuery {
     ZG_DEV bool visit(const View& b, const Vec<D>& x, Stack& st, VisitCount& vc) {
         if (b.nNodes == 0) return true;
         if (!has) {
+            // one pop per call: a pruned entry ends the call, so a lane draining a stack of
+            // pruned entries does not hold the wave in a loop of dependent LDS reads
             int n; float dn;
             bool got = false;
-            while (st.pop(n, dn)) {
-                if (dn <= prune2) { got = true; break; }
-                prunedBox(dn);
+            if (st.pop(n, dn)) {
+                if (dn > prune2) { prunedBox(dn); return false; }
+                got = true;
             }
             if (!got) {
                 if (found || retried) return true;
@@ -574,13 +581,20 @@ struct ClosestQuery {
 };
 
 // Complete closest-point query (loops ClosestQuery::visit); see ClosestQuery for the rules.
+// seedPrim >= 0 names a primitive (packed index) likely to be close, tested first.
 template <int D, int W, bool COUNT>
 ZG_DEV bool closestPoint(const BoundaryView& b, const Vec<D>& x, float& best2, Vec<D>& cp, int& prim,
                          Stack& st, VisitCount& vc, float relax2 = 1.0f, float guard2 = 0.0f,
-                         float* radius2 = nullptr) {
+                         float* radius2 = nullptr, int seedPrim = -1) {
     ClosestQuery<D> q;
     q.begin(best2, relax2, guard2);
     q.cp = cp;
+    if (seedPrim >= 0) {
+        Prim<D> pr; pr.load(b.prims, seedPrim);
+        Vec<D> qq;
+        q.seed(qq, seedPrim, pr.closest2(x, qq));
+        if (COUNT) vc.prims++;
+    }
     st.sp = 0; st.ovf = 0;
     while (!q.template visit<W, COUNT>(b, x, st, vc)) {}
     best2 = q.best2; cp = q.cp; prim = q.prim;

@@ -165,7 +165,8 @@ wosKernel(WosParams P) {
     float sigma = P.pde.absorption_coeff;
     auto pde = [&]() -> const zg_pde& { return BATCH ? inst->pde : P.pde; };
 
-    // lane state: 0 idle, 1 step pending (dist holds the radius at x), 2 distance query in flight
+    // lane state: 0 idle, 1 step pending (dist holds the radius at x), 2 distance query in flight,
+    // 3 the claim's next walk is to be started
     int mode = 0;
     int pendingCode = -1;   // >= 0: the walk ends with this code once the query (projection) is done
     int p = 0;
@@ -235,8 +236,7 @@ wosKernel(WosParams P) {
             atomicAdd(&blk[4], (unsigned long long)len);
         }
         w++;
-        if (w < wEnd) startWalk();
-        else mode = 0;
+        mode = w < wEnd ? 3 : 0;   // 3: the claim's next walk starts at the top of the loop
     };
 
     bool workLeft = true;
@@ -268,13 +268,14 @@ wosKernel(WosParams P) {
                         }
                         w = k * P.wpl;
                         wEnd = min(w + P.wpl, nw);
-                        if (w < wEnd) startWalk();
+                        if (w < wEnd) mode = 3;
                     }
                 }
             }
             if (!workLeft && !__any(mode != 0)) break;
         }
         iter++;
+        if (mode == 3) startWalk();
 
         // -------- one node of every distance query in flight --------
         if (mode == 2 && q.template visit<W, COUNT>(view, x, st, vc)) {

@@ -86,6 +86,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     Vec<2> x = vzero<2>(), nrm = vzero<2>(), prevDir = vzero<2>();
     float prevDist = FLT_MAX, T = 1.0f, Sref = 0.0f, Ssrc = 0.0f, distA = 0.0f, fsr0 = 0.0f;
     int len = 0;
+    int absPrim = -1;   // closest absorbing primitive of the last distance query (seeds the next)
     bool onRefl = false, flip = false, firstStep = true;
     Ball<2> ball; ball.yukawa = false; ball.sqrtLambda = 0.0f; ball.R = 0.0f;
     Pcg32 rng; rng.state = 0; rng.inc = 1;
@@ -103,6 +104,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         distA = P.dist0[p];
         fsr0 = P.fsr[p];
         firstStep = true;
+        absPrim = -1;
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
         atomicAdd(&blk[5], 1ull);
@@ -120,7 +122,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 int prim = -1;
                 Vec<2> xp = x, nn = nrm;
                 bool al = false;
-                if (closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc)) {
+                if (closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, 1.0f, 0.0f, nullptr, absPrim)) {
                     Prim<2> pr; pr.load(sc.abs.prims, prim);
                     Vec<2> n = pr.normal();
                     float sd = dot(x - cp, n);
@@ -266,7 +268,9 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         float b2 = FLT_MAX;
         Vec<2> cp = x;
         int prim = -1;
-        distA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc) ? __builtin_sqrtf(b2) : FLT_MAX;
+        bool gotA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, 1.0f, 0.0f, nullptr, absPrim);
+        distA = gotA ? __builtin_sqrtf(b2) : FLT_MAX;
+        absPrim = gotA ? prim : -1;
         firstStep = false;
     }
 
