@@ -514,15 +514,14 @@ def bench_mixed2d(args, torch, dev):
                                "absorbing-boundary closest-point bytes only; the reflecting queries (silhouette, "
                                "ray, sampling, visibility) are not counted")})
     if not args.no_cpu_baseline:
-        k, kw = 32, 256
+        k, kw = 2048, 64
         threads = min(os.cpu_count() or 1, 16)
         osc = O.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,
                       solve_double_sided=True)
         res["cpu_baseline"] = _oracle_baseline(
-            f"{k} points x {kw} walks; the oracle scans the reflecting boundary by brute force for the star radius, "
-            "sampling and visibility queries", lambda: O.solve(
-                osc, prob.pde, prob.settings, pts[:k], dist[:k].cpu().numpy(), kw, O.Stats(k), seed=7, wost=True,
-                rng_mode=O.RNG_PER_POINT, threads=threads), threads)
+            f"{k} points x {kw} walks (binary-BVH star radius, sampling, visibility and distance queries)",
+            lambda: O.solve(osc, prob.pde, prob.settings, pts[:k], dist[:k].cpu().numpy(), kw, O.Stats(k), seed=7,
+                            wost=True, rng_mode=O.RNG_PER_POINT, threads=threads), threads)
         res["speedup_vs_cpu_baseline"] = res["value"] / res["cpu_baseline"]["value"]
     return res
 

@@ -45,6 +45,9 @@ def lib():
         L.orc_intersect_ray.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, P, C.c_int]
         L.orc_inside_domain.argtypes = [P, C.c_int64, P, P, C.c_int]
         L.orc_star_radius.argtypes = [P, C.c_int64, P, P, P, C.c_float, P, P]
+        L.orc_star_radius_bvh.argtypes = [P, C.c_int64, P, P, P, C.c_float, P, P]
+        L.orc_sample_reflecting_bvh.argtypes = [P, P, C.c_float, P, P, P, P]
+        L.orc_line_of_sight_bvh.argtypes = [P, P, P]
         L.orc_silhouette_candidates.argtypes = [P, P, C.c_float, C.c_int, P, P]
         L.orc_silhouette_candidates.restype = C.c_int64
         L.orc_sample_reflecting.argtypes = [P, P, C.c_float, P, P, P, P]
@@ -170,14 +173,15 @@ class Scene:
         lib().orc_inside_domain(self.h, len(pts), _p(pts), _p(out), threads or default_threads())
         return out.astype(bool)
 
-    def star_radius(self, pts, min_r, max_r, precision=1e-3, flip=None):
+    def star_radius(self, pts, min_r, max_r, precision=1e-3, flip=None, bvh=False):
         pts = np.ascontiguousarray(pts, np.float32)
         n = len(pts)
         mn = np.ascontiguousarray(np.broadcast_to(min_r, n), np.float32)
         mx = np.ascontiguousarray(np.broadcast_to(max_r, n), np.float32)
         fl = None if flip is None else np.ascontiguousarray(np.broadcast_to(flip, n), np.uint8)
         out = np.empty(n, np.float32)
-        lib().orc_star_radius(self.h, n, _p(pts), _p(mn), _p(mx), precision, _p(fl), _p(out))
+        fn = lib().orc_star_radius_bvh if bvh else lib().orc_star_radius
+        fn(self.h, n, _p(pts), _p(mn), _p(mx), precision, _p(fl), _p(out))
         return out
 
     def silhouette_candidates(self, x, precision=1e-3, flip=False):
@@ -188,18 +192,19 @@ class Scene:
         lib().orc_silhouette_candidates(self.h, _p(x), precision, int(flip), _p(v), _p(s))
         return v, s.astype(bool)
 
-    def sample_reflecting(self, x, radius, u):
+    def sample_reflecting(self, x, radius, u, bvh=False):
         x = np.ascontiguousarray(x, np.float32)
         u = np.ascontiguousarray(u, np.float32)
         pt = np.empty(2, np.float32)
         nr = np.empty(2, np.float32)
         pdf = np.empty(1, np.float32)
-        ok = lib().orc_sample_reflecting(self.h, _p(x), radius, _p(u), _p(pt), _p(nr), _p(pdf))
+        fn = lib().orc_sample_reflecting_bvh if bvh else lib().orc_sample_reflecting
+        ok = fn(self.h, _p(x), radius, _p(u), _p(pt), _p(nr), _p(pdf))
         return bool(ok), pt, nr, float(pdf[0])
 
-    def line_of_sight(self, p1, p2):
-        return bool(lib().orc_line_of_sight(self.h, _p(np.ascontiguousarray(p1, np.float32)),
-                                            _p(np.ascontiguousarray(p2, np.float32))))
+    def line_of_sight(self, p1, p2, bvh=False):
+        fn = lib().orc_line_of_sight_bvh if bvh else lib().orc_line_of_sight
+        return bool(fn(self.h, _p(np.ascontiguousarray(p1, np.float32)), _p(np.ascontiguousarray(p2, np.float32))))
 
 
 class Stats:

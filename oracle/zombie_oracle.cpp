@@ -530,6 +530,7 @@ struct Mesh {
     // 2D silhouette candidates: vertex -> adjacent primitive(s)
     struct SilVertex { int v; int p0; int p1; };  // p1 = -1 when the vertex has one neighbour
     std::vector<SilVertex> sil;
+    Bvh<D> silBvh;                                 // over the silhouette candidates (point boxes)
 
     int nPrims() const { return (int)origId.size(); }
     V<D> vtx(int p, int k) const { return verts[idx[p * D + k]]; }
@@ -585,6 +586,9 @@ struct Mesh {
                 }
                 sil.push_back({v, first[v], second[v]});
             }
+            std::vector<Box<2>> pb(sil.size());
+            for (size_t k = 0; k < sil.size(); k++) pb[k].expand(verts[sil[k].v]);
+            silBvh.build(pb, 4, 8);
         }
     }
 
@@ -721,6 +725,67 @@ inline float starRadius2(const Mesh<2>& m, const V<2>& x, float minRadius, float
     return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
 }
 
+// the same query over the silhouette-candidate BVH (nearest-first, pruned by the best
+// distance and max radius, stopped once a silhouette within min radius is found); equal to
+// starRadius2 (checked by tests)
+inline float starRadius2Bvh(const Mesh<2>& m, const V<2>& x, float minRadius, float maxRadius, float precision,
+                            bool flip) {
+    if (minRadius > maxRadius) return maxRadius;
+    if (m.sil.empty()) return std::max(maxRadius, minRadius);
+    float best = FLT_MAX, best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
+    bool found = false;
+    const auto& nodes = m.silBvh.nodes;
+    struct E { int n; float d2; };
+    E stack[128];
+    int sp = 0;
+    stack[sp++] = {0, nodes[0].box.dist2(x)};
+    while (sp > 0) {
+        E e = stack[--sp];
+        if (e.d2 > best2) continue;
+        const auto& nd = nodes[e.n];
+        if (nd.left < 0) {
+            for (int i = nd.start; i < nd.start + nd.count; i++) {
+                const auto& sv = m.sil[m.silBvh.order[i]];
+                float d = norm(x - m.verts[sv.v]);
+                if (d > maxRadius || d >= best) continue;
+                if (m.isSilhouette(sv, x, precision, flip)) { best = d; best2 = d * d; found = true; }
+            }
+            if (found && best <= minRadius) break;
+            continue;
+        }
+        float dl = nodes[nd.left].box.dist2(x), dr = nodes[nd.right].box.dist2(x);
+        int nearN = nd.left, farN = nd.right; float dn = dl, df = dr;
+        if (dr < dl) { nearN = nd.right; farN = nd.left; dn = dr; df = dl; }
+        if (df <= best2) stack[sp++] = {farN, df};
+        if (dn <= best2) stack[sp++] = {nearN, dn};
+    }
+    return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
+}
+
+// line of sight over the BVH: any reflecting segment crossing the open segment p1 -> p2
+inline bool hasLineOfSightBvh(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) {
+    V<2> d = p2 - p1; float L = norm(d);
+    if (L <= 0.0f || m.nPrims() == 0) return true;
+    V<2> dir = d * (1.0f / L);
+    V<2> inv; for (int i = 0; i < 2; i++) inv[i] = 1.0f / dir[i];
+    int stack[128]; int sp = 0; stack[sp++] = 0;
+    const auto& nodes = m.bvh.nodes;
+    while (sp > 0) {
+        const auto& nd = nodes[stack[--sp]];
+        if (!nd.box.rayOverlap(p1, inv, L)) continue;
+        if (nd.left < 0) {
+            for (int i = nd.start; i < nd.start + nd.count; i++) {
+                int p = m.bvh.order[i];
+                float t, s;
+                if (raySegment(p1, dir, L, m.vtx(p, 0), m.vtx(p, 1), t, s) && t > 0.0f && t < L) return false;
+            }
+            continue;
+        }
+        stack[sp++] = nd.left; stack[sp++] = nd.right;
+    }
+    return true;
+}
+
 // line of sight (2D): no reflecting segment crosses the open segment p1 -> p2
 inline bool hasLineOfSight(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) {
     V<2> d = p2 - p1; float L = norm(d);
@@ -763,6 +828,54 @@ inline bool sampleReflecting2(const Mesh<2>& m, const V<2>& x, float radius, con
     nrm = m.primNormal(chosen);
     pdf = (float)(wch / total) / m.primArea(chosen);
     return true;
+}
+
+// sampleReflectingBoundary by one stochastic descent of the binary BVH: at an inner node a
+// child whose box meets the ball is taken with probability proportional to
+// branchWeight(|box centre - x|^2); in the leaf a segment whose box meets the ball with
+// probability proportional to branchWeight x length; u[0] is rescaled at every level.  pdf is
+// the exact density of the sample (the product of the choice probabilities / length).
+inline bool sampleReflecting2Bvh(const Mesh<2>& m, const V<2>& x, float radius, const float* u,
+                                 V<2>& pt, V<2>& nrm, float& pdf) {
+    if (m.nPrims() == 0) return false;
+    const float R2 = radius * radius;
+    const auto& nodes = m.bvh.nodes;
+    if (nodes[0].box.dist2(x) > R2) return false;
+    double prob = 1.0;
+    double uu = u[0];
+    int n = 0;
+    for (;;) {
+        const auto& nd = nodes[n];
+        if (nd.left < 0) {
+            double total = 0.0;
+            double w[64]; int cnt = std::min(nd.count, 64);
+            for (int j = 0; j < cnt; j++) {
+                int p = m.bvh.order[nd.start + j];
+                w[j] = m.boxes[p].dist2(x) <= R2 ? (double)branchWeight(norm2(m.boxes[p].centroid() - x)) * m.primArea(p) : 0.0;
+                total += w[j];
+            }
+            if (!(total > 0.0)) return false;
+            double t = uu * total, acc = 0.0; int sel = -1;
+            for (int j = 0; j < cnt; j++) {
+                if (w[j] > 0.0 && (sel < 0 || t >= acc)) sel = j;
+                acc += w[j];
+            }
+            int p = m.bvh.order[nd.start + sel];
+            V<2> pa = m.vtx(p, 0), pb = m.vtx(p, 1);
+            pt = pa + (pb - pa) * u[1];
+            nrm = m.primNormal(p);
+            pdf = (float)(prob * (w[sel] / total) / m.primArea(p));
+            return true;
+        }
+        const auto& L = nodes[nd.left].box; const auto& Rb = nodes[nd.right].box;
+        double wl = L.dist2(x) <= R2 ? (double)branchWeight(norm2(L.centroid() - x)) : 0.0;
+        double wr = Rb.dist2(x) <= R2 ? (double)branchWeight(norm2(Rb.centroid() - x)) : 0.0;
+        double total = wl + wr;
+        if (!(total > 0.0)) return false;
+        double t = uu * total;
+        if (wl > 0.0 && (wr <= 0.0 || t < wl)) { prob *= wl / total; uu = std::min(t / wl, 0.99999999); n = nd.left; }
+        else { prob *= wr / total; uu = std::min(std::max((t - wl) / wr, 0.0), 0.99999999); n = nd.right; }
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -939,7 +1052,7 @@ void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBa
     float u[D]; for (int i = 0; i < D; i++) u[i] = rng.nextFloat();
     V<D> bpt, bn; float bpdf;
     bool ok = false;
-    if constexpr (D == 2) ok = sampleReflecting2(sc.refl, st.currentPt, starRadius, u, bpt, bn, bpdf);
+    if constexpr (D == 2) ok = sampleReflecting2Bvh(sc.refl, st.currentPt, starRadius, u, bpt, bn, bpdf);
     if (!ok) return;
     V<D> toS = bpt - st.currentPt;
     float distS = norm(toS);
@@ -959,7 +1072,7 @@ void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBa
         V<D> p1 = st.onReflectingBoundary ? offsetPoint<D>(st.currentPt, st.currentNormal * -1.0f) : st.currentPt;
         V<D> p2 = offsetPoint<D>(bpt, bnS * -1.0f);
         bool blocked = false;
-        if constexpr (D == 2) blocked = !hasLineOfSight(sc.refl, p1, p2);
+        if constexpr (D == 2) blocked = !hasLineOfSightBvh(sc.refl, p1, p2);
         if (!blocked) {
             float r = std::max(gf.rClamp, distS);
             float G = gf.evaluate(r);
@@ -975,7 +1088,7 @@ float wostStarRadius(const Scene<D>& sc, const SolveCtx& cx, const V<D>& x, floa
     const zg_walk_settings& ws = *cx.ws;
     float r = distA;
     if constexpr (D == 2) {
-        if (sc.hasRefl()) r = starRadius2(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip);
+        if (sc.hasRefl()) r = starRadius2Bvh(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip);
     }
     if (sc.hasRefl() && ws.epsilon_shell_for_reflecting_boundary <= distA)
         r = std::max(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
@@ -1361,6 +1474,32 @@ int orc_line_of_sight(void* hv, const float* p1, const float* p2) {
     SceneHandle* h = (SceneHandle*)hv;
     if (h->dim != 2) return 1;
     return hasLineOfSight(h->s2->refl, load<2>(p1), load<2>(p2));
+}
+
+// BVH-accelerated forms used by the estimator (equal to the brute-force forms above for the
+// star radius and line of sight; a different but exact density for the sampler)
+void orc_star_radius_bvh(void* hv, int64_t n, const float* pts, const float* minR, const float* maxR, float prec,
+                         const uint8_t* flip, float* out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) { for (int64_t i = 0; i < n; i++) out[i] = maxR[i]; return; }
+    for (int64_t i = 0; i < n; i++)
+        out[i] = h->s2->hasRefl() ? starRadius2Bvh(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec,
+                                                   flip && flip[i] != 0) : maxR[i];
+}
+
+int orc_sample_reflecting_bvh(void* hv, const float* x, float radius, const float* u, float* pt, float* nrm, float* pdf) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) return 0;
+    V<2> p, nn; float pd;
+    bool ok = sampleReflecting2Bvh(h->s2->refl, load<2>(x), radius, u, p, nn, pd);
+    if (ok) { store<2>(pt, p); store<2>(nrm, nn); *pdf = pd; }
+    return ok;
+}
+
+int orc_line_of_sight_bvh(void* hv, const float* p1, const float* p2) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) return 1;
+    return hasLineOfSightBvh(h->s2->refl, load<2>(p1), load<2>(p2));
 }
 
 // Solves.  rng_mode 0: one generator per point consumed across its walks (the reference's

@@ -198,6 +198,46 @@ def test_reflecting_sampler_is_unbiased_for_boundary_length(oracle_lib):
     assert abs(est - length) < 4 * se + 1e-3
 
 
+def test_bvh_reflecting_queries_equal_brute_force(oracle_lib):
+    """The estimator's BVH forms of the star radius and line of sight return exactly what the
+    brute-force scans return; the BVH sampler's density is exact: E[1/pdf] is the length of
+    the segments whose box meets the ball and E[1/pdf 1{in ball}] the length inside it."""
+    prob = PB.mixed2d(n_segments=512, n_slits=8)
+    sc = oracle_lib.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,
+                          solve_double_sided=True)
+    rng = np.random.default_rng(21)
+    Q = rng.uniform(-1.0, 1.0, size=(600, 2)).astype(np.float32)
+    mx = rng.uniform(0.01, 1.5, size=600).astype(np.float32)
+    for flip in (False, True):
+        a = sc.star_radius(Q, 1e-3, mx, flip=flip)
+        b = sc.star_radius(Q, 1e-3, mx, flip=flip, bvh=True)
+        np.testing.assert_array_equal(a, b)
+    P2 = Q + rng.normal(scale=0.4, size=Q.shape).astype(np.float32)
+    los = [sc.line_of_sight(p, q) for p, q in zip(Q, P2)]
+    los_b = [sc.line_of_sight(p, q, bvh=True) for p, q in zip(Q, P2)]
+    assert los == los_b and 0 < sum(los) < len(los)
+    V, I = prob.positions, prob.indices[prob.is_reflecting.astype(bool)]
+    A, B = V[I[:, 0]].astype(np.float64), V[I[:, 1]].astype(np.float64)
+    for x, R in (((0.1, 0.8), 0.5), ((-0.3, 0.1), 0.3)):
+        x = np.array(x, np.float32)
+        lo, hi = np.minimum(A, B), np.maximum(A, B)
+        dd = np.maximum(np.maximum(lo - x, x - hi), 0)
+        support = np.linalg.norm(B - A, axis=1)[(dd * dd).sum(1) <= R * R].sum()
+        s = np.linspace(0, 1, 2001)
+        clipped = 0.0
+        for a, b in zip(A, B):
+            pts = a[None] + s[:, None] * (b - a)[None]
+            clipped += np.linalg.norm(b - a) * (np.linalg.norm(pts - x, axis=1) < R)[:-1].mean()
+        w, wi = [], []
+        for _ in range(20000):
+            ok, pt, nr, pdf = sc.sample_reflecting(x, R, rng.random(2).astype(np.float32), bvh=True)
+            w.append(1.0 / pdf if ok else 0.0)
+            wi.append(1.0 / pdf if ok and np.linalg.norm(pt - x) < R else 0.0)
+        w, wi = np.array(w), np.array(wi)
+        assert abs(w.mean() - support) < 4 * w.std() / np.sqrt(len(w)) + 1e-6 * support
+        assert abs(wi.mean() - clipped) < 4 * wi.std() / np.sqrt(len(wi)) + 1e-3
+
+
 def test_line_of_sight_brute_force(oracle_lib):
     prob = PB.mixed2d(n_segments=256, n_slits=8)
     sc = oracle_lib.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,

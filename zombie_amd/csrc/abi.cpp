@@ -335,7 +335,7 @@ struct SolveSetup {
     WosParams P;
     MergeParams M;
     int dim = 3, width = 8, blocks = 1, depth = 0;
-    bool batch = false, count = false, wost = false;
+    bool batch = false, count = false, wost = false, screened = false;
     int64_t n = 0;
     int nWalksMax = 0;
     size_t words = 0;
@@ -431,6 +431,8 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
 
     S.dim = dim; S.width = s ? s->width : b->width; S.depth = s ? s->info.absorbing_depth : b->maxDepth;
     S.batch = b != nullptr; S.count = o.count_visits != 0; S.n = n; S.nWalksMax = pts->n_walks_max;
+    if (s) S.screened = pde->absorption_coeff > 0.0f;
+    else for (const zg_pde& ip : b->pdes) S.screened = S.screened || ip.absorption_coeff > 0.0f;
     S.words = (size_t)n * 4 + kCntSlots + (wost ? (size_t)(n + 1) / 2 : 0);
     S.wost = wost;
     if (wost) {
@@ -466,8 +468,8 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     if (e == hipSuccess && S.wost) e = launchWostFirstSphere(S.P, S.width, S.callerFsr, const_cast<float*>(S.P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
         e = S.wost ? launchWost(S.P, S.width, S.count, S.blocks, strm)
-                   : S.batch ? launchWosBatch(S.P, S.width, S.count, S.blocks, strm)
-                             : launchWos(S.P, S.dim, S.width, S.count, S.blocks, strm);
+                   : S.batch ? launchWosBatch(S.P, S.width, S.count, S.screened, S.blocks, strm)
+                             : launchWos(S.P, S.dim, S.width, S.count, S.screened, S.blocks, strm);
     if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
     if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
     return e;

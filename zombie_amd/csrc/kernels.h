@@ -97,8 +97,8 @@ hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, cons
 hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);
 hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
                      uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
-hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s);
-hipError_t launchWosBatch(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, int gridBlocks, hipStream_t s);
+hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, int gridBlocks, hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
 // walk on stars (2D)
 hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s);

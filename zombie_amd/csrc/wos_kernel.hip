@@ -149,7 +149,8 @@ ZG_DEV Vec<D> cvField(const WosParams& P, const Vec<D>& x) {
     return g;
 }
 
-template <int D, int W, bool COUNT, bool BATCH>
+// SCREENED = false compiles the Yukawa (screened Poisson) terms out of the walk.
+template <int D, int W, bool COUNT, bool BATCH, bool SCREENED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WosOccupancy<W>::kWaves, 8)))
 wosKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
@@ -204,7 +205,7 @@ wosKernel(WosParams P) {
         x = vload<D>(P.pt + (size_t)p * D);
         dist = P.dist0[p];
         T = 1.0f; S = 0.0f; len = 0;
-        ball.yukawa = sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
+        ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
         ycValid = false;
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
         atomicAdd(&blk[5], 1ull);
@@ -313,7 +314,7 @@ wosKernel(WosParams P) {
             // later steps: -T G(x_k).(x_{k+1} - x_k) with G a fixed field, zero mean as well
             S -= T * dot(cvField<D>(P, c), dir) * R;
         }
-        if (ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
+        if (SCREENED && ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
         if (outsideDomain<D>(BATCH ? inst->dmin : P.scene.dmin, BATCH ? inst->dmax : P.scene.dmax, x)) {
             finishWalk(ZG_ESCAPED_DOMAIN);
             continue;
@@ -329,7 +330,7 @@ wosKernel(WosParams P) {
             else beginQuery(FLT_MAX, 1.0f, ZG_EXCEEDED_MAX_WALK_LENGTH);  // terminal projection, exact
             continue;
         }
-        if (sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
+        if (SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
         // next radius: the new distance is at most |x - y_prev| (y_prev lies on the boundary)
         float bound2;
         if (ycValid) bound2 = norm2(x - yc) * 1.0001f + 1e-30f;
@@ -452,29 +453,34 @@ hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* 
     return hipGetLastError();
 }
 
-hipError_t launchWos(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s) {
+hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-#define ZG_WOS(D, Wd, C) hipLaunchKernelGGL((wosKernel<D, Wd, C, false>), g, bl, kStackLdsBytes, s, P)
-    if (dim == 2) {
-        if (width == 4) { if (count) ZG_WOS(2, 4, true); else ZG_WOS(2, 4, false); }
-        else { if (count) ZG_WOS(2, 8, true); else ZG_WOS(2, 8, false); }
+#define ZG_WOS(D, Wd, C, S) hipLaunchKernelGGL((wosKernel<D, Wd, C, false, S>), g, bl, kStackLdsBytes, s, P)
+    if (dim == 2) {  // 2D walks are harmonic (screened Poisson is 3D only)
+        if (width == 4) { if (count) ZG_WOS(2, 4, true, false); else ZG_WOS(2, 4, false, false); }
+        else { if (count) ZG_WOS(2, 8, true, false); else ZG_WOS(2, 8, false, false); }
+    } else if (screened) {
+        if (width == 4) { if (count) ZG_WOS(3, 4, true, true); else ZG_WOS(3, 4, false, true); }
+        else { if (count) ZG_WOS(3, 8, true, true); else ZG_WOS(3, 8, false, true); }
     } else {
-        if (width == 4) { if (count) ZG_WOS(3, 4, true); else ZG_WOS(3, 4, false); }
-        else { if (count) ZG_WOS(3, 8, true); else ZG_WOS(3, 8, false); }
+        if (width == 4) { if (count) ZG_WOS(3, 4, true, false); else ZG_WOS(3, 4, false, false); }
+        else { if (count) ZG_WOS(3, 8, true, false); else ZG_WOS(3, 8, false, false); }
     }
 #undef ZG_WOS
     return hipGetLastError();
 }
 
-hipError_t launchWosBatch(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s) {
+hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-    if (width == 4) {
-        if (count) hipLaunchKernelGGL((wosKernel<3, 4, true, true>), g, bl, kStackLdsBytes, s, P);
-        else hipLaunchKernelGGL((wosKernel<3, 4, false, true>), g, bl, kStackLdsBytes, s, P);
+#define ZG_WOSB(Wd, C, S) hipLaunchKernelGGL((wosKernel<3, Wd, C, true, S>), g, bl, kStackLdsBytes, s, P)
+    if (screened) {
+        if (width == 4) { if (count) ZG_WOSB(4, true, true); else ZG_WOSB(4, false, true); }
+        else { if (count) ZG_WOSB(8, true, true); else ZG_WOSB(8, false, true); }
     } else {
-        if (count) hipLaunchKernelGGL((wosKernel<3, 8, true, true>), g, bl, kStackLdsBytes, s, P);
-        else hipLaunchKernelGGL((wosKernel<3, 8, false, true>), g, bl, kStackLdsBytes, s, P);
+        if (width == 4) { if (count) ZG_WOSB(4, true, false); else ZG_WOSB(4, false, false); }
+        else { if (count) ZG_WOSB(8, true, false); else ZG_WOSB(8, false, false); }
     }
+#undef ZG_WOSB
     return hipGetLastError();
 }
 
