@@ -199,6 +199,11 @@ def main():
     s_to_rmse_plain = var_plain / TARGET_RMSE ** 2 * n / walks_per_s
 
     achieved_gbs = bytes_per_step * (total_steps / len(reports)) / (kernel_ms * 1e-3) / 1e9
+    # measured read ceilings: a 16 MB buffer stays in the XCDs' L2 caches, a 4 GiB one streams HBM
+    E.probe_read_bandwidth(16 << 20, 20)
+    l2_gbs = E.probe_read_bandwidth(16 << 20, 400)
+    hbm_gbs = E.probe_read_bandwidth(4 << 30, 4)
+    log(f"read probe: L2-resident {l2_gbs:.0f} GB/s, HBM {hbm_gbs:.0f} GB/s")
     # HBM/MALL traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # the same kernel configuration, scaled to this launch's step count
     traffic = None
@@ -297,6 +302,7 @@ def main():
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": bytes_per_step, "nodes_per_step": nodes_per_step,
                      "prims_per_step": prims_per_step, "kernel_ms": kernel_ms, **issue,
+                     "l2_read_measured": l2_gbs, "hbm_read_measured": hbm_gbs, "frac_of_l2_read": achieved_gbs / l2_gbs,
                      "note": "algorithmic bytes = BVH node + triangle bytes touched per step; the scene is "
                              "cache-resident (L2/MALL), so HBM is not the binding level: the kernel is bound by "
                              "VALU instruction issue under lane divergence (valu_busy, valu_lane_utilisation "

@@ -304,6 +304,12 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,
                         zg_solve_report* report_host, void* stream);
 
+/* Measurement utility (no counterpart in the reference): reads `bytes` of device memory at
+ * `buf` `reps` times with float4 loads over every CU and returns the elapsed milliseconds.
+ * A buffer that fits the L2 caches gives the cache read ceiling, a large one the HBM
+ * ceiling, the two rooflines the walk kernels are reported against. */
+zg_status zg_probe_read_bandwidth(const void* buf, int64_t bytes, int32_t reps, float* ms_out, void* stream);
+
 /* ------------------------------------------------------------------------------------ */
 /* Batched solves: many independent instances (scene + PDE each) walked by one persistent */
 /* launch.  The reference solves one scene per call (walk_on_spheres.h:131-168); a batch  */

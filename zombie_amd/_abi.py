@@ -183,6 +183,7 @@ SIGNATURES = {
     "zg_inside_domain": (C.c_int32, [_P, C.c_int64, _P, _P, _P]),
     "zg_intersect_ray": (C.c_int32, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_star_radius": (C.c_int32, [_P, C.c_int64, _P, _P, _P, C.c_float, _P, _P, _P]),
+    "zg_probe_read_bandwidth": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_float), _P]),
     "zg_line_of_sight": (C.c_int32, [_P, C.c_int64, _P, _P, _P, _P]),
     "zg_sample_reflecting": (C.c_int32, [_P, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_wos_solve": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points),

@@ -293,6 +293,28 @@ zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const f
     return ok();
 }
 
+zg_status zg_probe_read_bandwidth(const void* buf, int64_t bytes, int32_t reps, float* ms_out, void* stream) {
+    if (!buf || bytes < 16 || reps < 1 || !ms_out) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    int dev = 0;
+    ZG_HIP(hipGetDevice(&dev));
+    hipStream_t s = (hipStream_t)stream;
+    float* sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMallocAsync((void**)&sink, sizeof(float), s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    if (e == hipSuccess) e = launchReadProbe(buf, bytes, reps, cuCount(dev) * 8, sink, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms_out, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFreeAsync(sink, s);
+    if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "read probe failed: %s", hipGetErrorString(e));
+    return ok();
+}
+
 zg_status zg_line_of_sight(const zg_scene* s, int64_t n, const float* p1, const float* p2, uint8_t* visible,
                            void* stream) {
     if (!s || n < 0 || (n > 0 && (!p1 || !p2 || !visible))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");

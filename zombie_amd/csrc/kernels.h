@@ -100,6 +100,8 @@ hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* 
 hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, int gridBlocks, hipStream_t s);
 hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, int gridBlocks, hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
+// read-bandwidth probe
+hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks, float* sink, hipStream_t s);
 // walk on stars (2D)
 hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s);
 hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);

@@ -336,6 +336,16 @@ def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: Devic
     return SolveReport.from_c(rep)
 
 
+def probe_read_bandwidth(nbytes: int, reps: int, device="cuda", stream=None) -> float:
+    """Read bandwidth in GB/s of a float4 streaming read of an nbytes buffer, repeated reps times
+    (an L2-sized buffer measures the cache ceiling, a large one the HBM ceiling)."""
+    torch = _torch()
+    buf = torch.ones(nbytes // 4, dtype=torch.float32, device=device)
+    ms = C.c_float()
+    A.check(A.lib().zg_probe_read_bandwidth(_ptr(buf), buf.numel() * 4, reps, C.byref(ms), _stream_ptr(stream)))
+    return buf.numel() * 4 * reps / (ms.value * 1e-3) / 1e9
+
+
 class Batch:
     """Many independent 3D instances (scene + PDE each) solved by one persistent launch.
 
