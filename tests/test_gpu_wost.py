@@ -44,10 +44,18 @@ def test_star_radius_matches_oracle(gpu, mixed, width):
     o = osc.star_radius(pts, mn, d)
     close = np.isclose(g, o, rtol=1e-5, atol=1e-6)
     assert close.mean() > 0.995, f"{(~close).sum()} of {len(pts)} star radii differ"
+    # a silhouette missed by the GPU traversal (its normal-cone pruning) would give a larger radius
+    assert (g > o * (1 + 1e-4) + 1e-5).sum() == 0
+    # the walker's flipped orientation
+    fl = torch.ones(len(pts), dtype=torch.uint8, device="cuda")
+    gf = sc.star_radius(_t(torch, pts), _t(torch, mn), _t(torch, d), flip=fl).cpu().numpy()
+    of = osc.star_radius(pts, mn, d, flip=np.ones(len(pts), np.uint8))
+    assert np.isclose(gf, of, rtol=1e-5, atol=1e-6).mean() > 0.995
+    assert (gf > of * (1 + 1e-4) + 1e-5).sum() == 0
     # result in [min(eps, d), max(eps, d)] (min radius above max radius returns the max radius)
     assert (g <= np.maximum(d, eps) * (1 + 1e-6)).all() and (g >= np.minimum(d, eps) * (1 - 1e-6)).all()
     # invariant: no silhouette vertex lies strictly inside a radius above the clamp
-    for i in range(0, len(pts), 40):
+    for i in range(0, len(pts), 4):
         v, s = osc.silhouette_candidates(pts[i])
         ds = np.linalg.norm(v[s] - pts[i], axis=1)
         if g[i] > eps * 1.001:

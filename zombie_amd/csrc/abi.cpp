@@ -73,6 +73,8 @@ struct DeviceBoundary {
     int32_t* primIds = nullptr;
     float* sil = nullptr;
     float* silNodes = nullptr;
+    float* silCones = nullptr;
+    float silConePrecision = 0.0f;
     BuiltBoundary host;  // sizes only after upload (vectors cleared)
     int64_t bytes = 0;
 };
@@ -103,6 +105,7 @@ void freeBoundary(DeviceBoundary& b) {
     if (b.primIds) (void)hipFree(b.primIds);
     if (b.sil) (void)hipFree(b.sil);
     if (b.silNodes) (void)hipFree(b.silNodes);
+    if (b.silCones) (void)hipFree(b.silCones);
     b = DeviceBoundary();
 }
 
@@ -113,7 +116,9 @@ zg_status uploadBoundary(DeviceBoundary& d, BoundaryView& v) {
     ZG_HIP(upload(&d.primIds, d.host.primIds, d.bytes));
     ZG_HIP(upload(&d.sil, d.host.sil, d.bytes));
     ZG_HIP(upload(&d.silNodes, d.host.silNodes, d.bytes));
+    ZG_HIP(upload(&d.silCones, d.host.silCones, d.bytes));
     v.nodes = d.nodes; v.prims = d.prims; v.primIds = d.primIds; v.sil = d.sil; v.silNodes = d.silNodes;
+    v.silCones = d.silCones; v.silConePrecision = d.silConePrecision;
     v.nNodes = d.host.nNodes; v.nPrims = d.host.nPrims; v.nSil = d.host.nSil; v.nSilNodes = d.host.nSilNodes;
     return ZG_OK;
 }
@@ -217,7 +222,10 @@ zg_status zg_scene_create(int32_t dim, const float* V, int64_t nV, const int32_t
     try {
         buildBoundary(dim, o.bvh_width, o.leaf_size, o.sah_bins, V, I, selA, s->abs.host);
         buildBoundary(dim, o.bvh_width, o.leaf_size, o.sah_bins, V, I, selR, s->refl.host);
-        if (dim == 2 && !selR.empty()) buildSilhouettes2D(o.bvh_width, V, nV, I, selR, dbl != 0, prec, s->refl.host);
+        if (dim == 2 && !selR.empty()) {
+            buildSilhouettes2D(o.bvh_width, V, nV, I, selR, dbl != 0, prec, s->refl.host);
+            s->refl.silConePrecision = prec;   // the cones' margin covers queries up to this precision
+        }
     } catch (const std::exception& e) {
         delete s;
         return fail(ZG_ERR_INVALID_ARGUMENT, "scene build failed: %s", e.what());

@@ -11,6 +11,8 @@
 #include "bvh_build.h"
 
 #include <algorithm>
+#include <climits>
+#include <cmath>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -240,6 +242,75 @@ void buildBoundary(int dim, int width, int leafSize, int bins, const float* V, c
     else buildDim<3>(width, leafSize, bins, V, I, sel, out);
 }
 
+// Normal cones of the silhouette BVH (the role of the spatialized normal cone hierarchy behind
+// fcpw's findClosestSilhouettePoint): for every child subtree, the smallest arc of directions
+// containing the normals of all segments adjacent to its candidates, widened by the angular
+// margin asin(precision) + 1e-4.  A subtree seen from x under directions whose angle to every
+// such normal stays away from 90 degrees by more than the margin holds only vertices whose two
+// segments both face x, or both face away, by more than the precision: no silhouette.
+// Subtrees holding an open end (always a silhouette) get a full cone.  Stored per child as
+// (axis.x, axis.y, sin(half + margin), cos(half + margin)); a cone of half-angle + margin
+// >= 90 degrees is stored as (0, 0, 1, -1), which never prunes.
+template <int W>
+static void buildSilhouetteCones(BuiltBoundary& out, float precision) {
+    using L = NodeLayout<2, W>;
+    const int nn = out.nSilNodes;
+    out.silCones.assign((size_t)nn * W * 4, 0.0f);
+    if (nn == 0) return;
+    const double margin = std::asin(std::min(1.0, (double)std::max(precision, 0.0f))) + 1e-4;
+    // candidate range [lo, hi) of every node (children are leaves or later nodes)
+    std::vector<int> rlo(nn, INT32_MAX), rhi(nn, -1);
+    auto childRange = [&](int c, int& lo, int& hi) {
+        if (c < 0) { lo = leafFirst(c); hi = lo + leafCount(c); } else { lo = rlo[c]; hi = rhi[c]; }
+    };
+    for (int i = nn - 1; i >= 0; i--) {
+        const int32_t* ch = reinterpret_cast<const int32_t*>(&out.silNodes[(size_t)i * L::kFloats + L::kChildOffset]);
+        for (int k = 0; k < W; k++) {
+            const float* lo0 = &out.silNodes[(size_t)i * L::kFloats];
+            if (lo0[k] >= 1e29f) continue;  // empty slot
+            int lo, hi;
+            childRange(ch[k], lo, hi);
+            if (hi <= lo) continue;
+            rlo[i] = std::min(rlo[i], lo); rhi[i] = std::max(rhi[i], hi);
+        }
+    }
+    std::vector<double> ang;
+    for (int i = 0; i < nn; i++) {
+        const int32_t* ch = reinterpret_cast<const int32_t*>(&out.silNodes[(size_t)i * L::kFloats + L::kChildOffset]);
+        const float* lo0 = &out.silNodes[(size_t)i * L::kFloats];
+        for (int k = 0; k < W; k++) {
+            float* cone = &out.silCones[((size_t)i * W + k) * 4];
+            cone[0] = 0.0f; cone[1] = 0.0f; cone[2] = 1.0f; cone[3] = -1.0f;   // never prunes
+            if (lo0[k] >= 1e29f) continue;
+            int lo, hi;
+            childRange(ch[k], lo, hi);
+            if (hi <= lo) continue;
+            ang.clear();
+            bool full = false;
+            for (int c = lo; c < hi && !full; c++) {
+                const float* cd = &out.sil[(size_t)c * kSilFloats];
+                if (cd[6] != 0.0f) { full = true; break; }
+                ang.push_back(std::atan2((double)cd[3], (double)cd[2]));
+                ang.push_back(std::atan2((double)cd[5], (double)cd[4]));
+            }
+            if (full || ang.empty()) continue;
+            std::sort(ang.begin(), ang.end());
+            // the smallest covering arc is the complement of the largest gap between neighbours
+            double gap = ang.front() + 2.0 * M_PI - ang.back(), start = ang.front();
+            for (size_t j = 1; j < ang.size(); j++)
+                if (ang[j] - ang[j - 1] > gap) { gap = ang[j] - ang[j - 1]; start = ang[j]; }
+            double arc = 2.0 * M_PI - gap;
+            double half = 0.5 * arc + margin;
+            if (half >= 0.5 * M_PI) continue;
+            double axis = start + 0.5 * arc;
+            cone[0] = (float)std::cos(axis); cone[1] = (float)std::sin(axis);
+            // sin rounded up, cos rounded down: the stored cone contains the exact one
+            cone[2] = std::nextafter((float)std::sin(half), 2.0f);
+            cone[3] = std::nextafter((float)std::cos(half), -2.0f);
+        }
+    }
+}
+
 // 2D silhouette candidates of a reflecting polyline: every vertex with its adjacent
 // segments; a vertex with a single neighbour (an open end) is always a silhouette.  With
 // one-sided problems convex vertices are dropped, as getIgnoreCandidateSilhouetteCallback
@@ -294,6 +365,8 @@ void buildSilhouettes2D(int width, const float* V, int64_t nV, const int32_t* I,
     out.sil.resize(cand.size());
     for (int i = 0; i < n; i++)
         std::memcpy(&out.sil[(size_t)i * kSilFloats], &cand[(size_t)order[i] * kSilFloats], kSilFloats * 4);
+    if (width == 4) buildSilhouetteCones<4>(out, precision);
+    else buildSilhouetteCones<8>(out, precision);
 }
 
 }  // namespace zg

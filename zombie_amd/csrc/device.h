@@ -768,6 +768,30 @@ ZG_DEV bool isSilhouette2(const float4& c0, const float4& c1, const Vec<2>& x, f
 // the closest silhouette vertex within maxRadius, clamped below by minRadius; maxRadius
 // clamped below by minRadius when there is none.  Nearest-first traversal of the
 // silhouette-candidate BVH, stopped once a silhouette closer than minRadius is found.
+// Normal-cone pruning: a child whose box is seen from x under directions that make an angle
+// with every normal of its subtree (its cone, widened by the build's precision margin) away
+// from 90 degrees can hold no silhouette and is skipped.  With u the direction from the box
+// centre to x, sin(phi) = box radius / distance the angular radius of the box, and (a, theta)
+// the cone, every normal-to-view angle lies within alpha +- (theta + phi), alpha = angle(a, u):
+// the child is skipped when |cos alpha| > sin(theta + phi), theta + phi < 90 degrees.
+template <int W>
+ZG_DEV bool coneMaySeeSilhouette(const float4& cone, float lox, float loy, float hix, float hiy, const Vec<2>& x) {
+    float cx = 0.5f * (lox + hix), cy = 0.5f * (loy + hiy);
+    float ux = x[0] - cx, uy = x[1] - cy;
+    float d2 = ux * ux + uy * uy;
+    float ex = 0.5f * (hix - lox), ey = 0.5f * (hiy - loy);
+    float r2 = ex * ex + ey * ey;
+    if (d2 <= r2 * 1.0001f || cone.w < 0.0f) return true;   // x near or inside the box, or a wide cone
+    float invd = 1.0f / __builtin_sqrtf(d2);
+    float sphi = __builtin_sqrtf(r2) * invd * 1.0001f;
+    float cphi = __builtin_sqrtf(fmaxf(0.0f, 1.0f - sphi * sphi));
+    float sinSum = cone.z * cphi + cone.w * sphi;            // sin(theta + phi)
+    float cosSum = cone.w * cphi - cone.z * sphi;            // cos(theta + phi)
+    if (cosSum <= 0.0f) return true;
+    float calpha = (cone.x * ux + cone.y * uy) * invd;
+    return fabsf(calpha) <= sinSum * 1.0001f + 1e-6f;
+}
+
 template <int W>
 ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius, float maxRadius, float precision,
                          bool flip, Stack& st) {
@@ -775,17 +799,25 @@ ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius
     if (b.nSilNodes == 0) return fmaxf(maxRadius, minRadius);
     float best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
     const float min2 = minRadius * minRadius;
+    const bool cones = b.silCones != nullptr && precision <= b.silConePrecision;
     bool found = false;
     st.sp = 0; st.ovf = 0;
     int node = 0;
     for (;;) {
-        float d2[W];
-        int ch[W];
-        childDistances<2, W>(b.silNodes, node, x, d2, ch);
+        WideNode<2, W> nd;
+        nd.load(b.silNodes, node);
+        const float4* cn = reinterpret_cast<const float4*>(b.silCones) + (size_t)node * W;
+        bool keep[W];
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            if (d2[k] > best2 || !isLeaf(ch[k])) continue;
-            int first = leafFirstD(ch[k]), cnt = leafCountD(ch[k]);
+            keep[k] = nd.dist2(k, x) <= best2;
+            if (cones && keep[k])
+                keep[k] = coneMaySeeSilhouette<W>(cn[k], nd.lo[0][k], nd.lo[1][k], nd.hi[0][k], nd.hi[1][k], x);
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (!keep[k] || !isLeaf(nd.ch[k])) continue;
+            int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
             for (int i = first; i < first + cnt; i++) {
                 const float4* c = reinterpret_cast<const float4*>(b.sil) + 2 * i;
                 float4 c0 = c[0], c1 = c[1];
@@ -795,13 +827,17 @@ ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius
             }
         }
         if (found && best2 <= min2) break;
-        int next = -1; float nd = 0.0f; int nk = -1;
+        int next = -1; float ndist = 0.0f; int nk = -1;
 #pragma unroll
-        for (int k = 0; k < W; k++)
-            if (!isLeaf(ch[k]) && d2[k] <= best2 && (next < 0 || d2[k] < nd)) { next = ch[k]; nd = d2[k]; nk = k; }
+        for (int k = 0; k < W; k++) {
+            float dk = nd.dist2(k, x);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && (next < 0 || dk < ndist)) { next = nd.ch[k]; ndist = dk; nk = k; }
+        }
 #pragma unroll
-        for (int k = 0; k < W; k++)
-            if (!isLeaf(ch[k]) && d2[k] <= best2 && k != nk) st.push(ch[k], d2[k]);
+        for (int k = 0; k < W; k++) {
+            float dk = nd.dist2(k, x);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && k != nk) st.push(nd.ch[k], dk);
+        }
         if (next < 0) {
             int n; float dn;
             bool got = false;

@@ -49,6 +49,8 @@ struct BoundaryView {
     const int32_t* primIds;// 2D: caller index per packed primitive
     const float* sil;      // 2D reflecting: silhouette candidates (kSilFloats each)
     const float* silNodes; // 2D reflecting: wide BVH over silhouette candidates
+    const float* silCones; // per silhouette-BVH child: normal cone (axis.xy, sin, cos of half-angle + margin)
+    float silConePrecision;// the silhouette precision the cones' margin covers
     int32_t nNodes;
     int32_t nPrims;
     int32_t nSil;

@@ -209,11 +209,11 @@ class Scene:
         return hit, t, p, nr
 
     # walk-on-stars queries on the reflecting boundary (2D)
-    def star_radius(self, pts, min_r, max_r, precision: float = 1e-3):
+    def star_radius(self, pts, min_r, max_r, precision: float = 1e-3, flip=None):
         torch = _torch()
         out = torch.empty(pts.shape[0], dtype=torch.float32, device=pts.device)
         A.check(A.lib().zg_star_radius(self._h, pts.shape[0], _ptr(pts), _ptr(min_r), _ptr(max_r), float(precision),
-                                       None, _ptr(out), _stream_ptr()))
+                                       _ptr(flip), _ptr(out), _stream_ptr()))
         return out
 
     def line_of_sight(self, p1, p2):
