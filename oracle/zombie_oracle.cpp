@@ -531,6 +531,8 @@ struct Mesh {
     struct SilVertex { int v; int p0; int p1; };  // p1 = -1 when the vertex has one neighbour
     std::vector<SilVertex> sil;
     Bvh<D> silBvh;                                 // over the silhouette candidates (point boxes)
+    std::vector<std::array<float, 4>> silCone;     // per silBvh node: axis.xy, sin, cos of half-angle + margin
+    float silConePrecision = 0.0f;
 
     int nPrims() const { return (int)origId.size(); }
     V<D> vtx(int p, int k) const { return verts[idx[p * D + k]]; }
@@ -589,6 +591,8 @@ struct Mesh {
             std::vector<Box<2>> pb(sil.size());
             for (size_t k = 0; k < sil.size(); k++) pb[k].expand(verts[sil[k].v]);
             silBvh.build(pb, 4, 8);
+            buildSilhouetteCones(precision);
+            silConePrecision = precision;
         }
     }
 
@@ -601,6 +605,60 @@ struct Mesh {
     // = +1 for normals pointing away from the walker (outward normals of a domain walked from
     // inside), -1 when the walker's normal orientation is flipped.  A vertex with one
     // neighbour (an open end) and a query at the vertex itself are always silhouettes.
+    // normal cone of every silhouette-BVH node (the spatialized normal cone hierarchy of fcpw's
+    // findClosestSilhouettePoint): the smallest arc containing the normals of the segments
+    // adjacent to the node's candidates, widened by asin(precision) + 1e-4; full for nodes
+    // holding an open end.  Stored as (axis.x, axis.y, sin, cos) of the widened half-angle;
+    // (0, 0, 1, -1) never prunes.
+    void buildSilhouetteCones(float precision) {
+        const auto& nodes = silBvh.nodes;
+        int nn = (int)nodes.size();
+        silCone.assign(nn, {0.0f, 0.0f, 1.0f, -1.0f});
+        std::vector<int> lo(nn), hi(nn);
+        for (int i = nn - 1; i >= 0; i--) {   // children follow their parent in build order
+            if (nodes[i].left < 0) { lo[i] = nodes[i].start; hi[i] = nodes[i].start + nodes[i].count; }
+            else { lo[i] = std::min(lo[nodes[i].left], lo[nodes[i].right]); hi[i] = std::max(hi[nodes[i].left], hi[nodes[i].right]); }
+        }
+        double margin = std::asin(std::min(1.0, (double)std::max(precision, 0.0f))) + 1e-4;
+        std::vector<double> ang;
+        for (int i = 0; i < nn; i++) {
+            ang.clear();
+            bool full = false;
+            for (int j = lo[i]; j < hi[i]; j++) {
+                const SilVertex& sv = sil[silBvh.order[j]];
+                if (sv.p1 < 0) { full = true; break; }
+                V<2> n0 = primNormal(sv.p0), n1 = primNormal(sv.p1);
+                ang.push_back(std::atan2((double)n0[1], (double)n0[0]));
+                ang.push_back(std::atan2((double)n1[1], (double)n1[0]));
+            }
+            if (full || ang.empty()) continue;
+            std::sort(ang.begin(), ang.end());
+            double gap = ang.front() + 2.0 * M_PI - ang.back(), start = ang.front();
+            for (size_t j = 1; j < ang.size(); j++)
+                if (ang[j] - ang[j - 1] > gap) { gap = ang[j] - ang[j - 1]; start = ang[j]; }
+            double arc = 2.0 * M_PI - gap, half = 0.5 * arc + margin;
+            if (half >= 0.5 * M_PI) continue;
+            double axis = start + 0.5 * arc;
+            silCone[i] = {(float)std::cos(axis), (float)std::sin(axis), std::nextafter((float)std::sin(half), 2.0f),
+                          std::nextafter((float)std::cos(half), -2.0f)};
+        }
+    }
+    // whether a silhouette-BVH node may hold a silhouette seen from x (see buildSilhouetteCones)
+    bool coneMaySee(int node, const V<2>& x) const {
+        const auto& c = silCone[node];
+        if (c[3] < 0.0f) return true;
+        const Box<2>& b = silBvh.nodes[node].box;
+        V<2> ctr = b.centroid(), u = x - ctr, e = (b.hi - b.lo) * 0.5f;
+        float d2 = norm2(u), r2 = norm2(e);
+        if (d2 <= r2 * 1.0001f) return true;
+        float invd = 1.0f / std::sqrt(d2), sphi = std::sqrt(r2) * invd * 1.0001f;
+        float cphi = std::sqrt(std::max(0.0f, 1.0f - sphi * sphi));
+        float sinSum = c[2] * cphi + c[3] * sphi, cosSum = c[3] * cphi - c[2] * sphi;
+        if (cosSum <= 0.0f) return true;
+        float ca = (c[0] * u[0] + c[1] * u[1]) * invd;
+        return std::fabs(ca) <= sinSum * 1.0001f + 1e-6f;
+    }
+
     bool isSilhouette(const SilVertex& s, const V<2>& x, float precision, bool flip) const {
         if (s.p1 < 0) return true;
         V<2> v = verts[s.v];
@@ -732,6 +790,7 @@ inline float starRadius2Bvh(const Mesh<2>& m, const V<2>& x, float minRadius, fl
                             bool flip) {
     if (minRadius > maxRadius) return maxRadius;
     if (m.sil.empty()) return std::max(maxRadius, minRadius);
+    const bool cones = precision <= m.silConePrecision;
     float best = FLT_MAX, best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
     bool found = false;
     const auto& nodes = m.silBvh.nodes;
@@ -754,6 +813,8 @@ inline float starRadius2Bvh(const Mesh<2>& m, const V<2>& x, float minRadius, fl
             continue;
         }
         float dl = nodes[nd.left].box.dist2(x), dr = nodes[nd.right].box.dist2(x);
+        if (cones && !m.coneMaySee(nd.left, x)) dl = FLT_MAX;
+        if (cones && !m.coneMaySee(nd.right, x)) dr = FLT_MAX;
         int nearN = nd.left, farN = nd.right; float dn = dl, df = dr;
         if (dr < dl) { nearN = nd.right; farN = nd.left; dn = dr; df = dl; }
         if (df <= best2) stack[sp++] = {farN, df};
