@@ -474,9 +474,10 @@ def bench_mixed2d(args, torch, dev):
     stats = E.DeviceStats.zeros(n)
     log(f"mixed2d setup {time.time() - t0:.1f}s: {n} points, {walks} walks")
 
-    def opts(s, count=False):
+    def opts(s, count=False, tol=None):
         return E.SolveOptions(seed=args.seed, stream_offset=s * walks, walks_per_lane=args.walks_per_lane,
-                              regroup_interval=args.regroup, count_visits=count)
+                              regroup_interval=args.regroup, count_visits=count,
+                              distance_tolerance=args.tolerance if tol is None else tol)
 
     # a pass of `walks` walks per point runs as progressive calls of `chunk` walks (fresh stream
     # offsets; equal to one call of `walks` walks), so a long pass reports its progress
@@ -508,13 +509,20 @@ def bench_mixed2d(args, torch, dev):
     dps = E.DevicePoints(pt=P[:sub].contiguous(), dist_to_absorbing=dist[:sub].contiguous(),
                          n_walks=torch.full((sub,), 64, dtype=torch.int32, device=dev), n_walks_max=64)
     rc = E.wost_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(99, True))
+    # exact-distance walk length on the sample: the value is reported in the reference's unit
+    rx = E.wost_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(98, tol=0.0))
+    len_exact = rx.total_steps / max(rx.total_walks, 1)
     mean = stats.mean.cpu().numpy()
     rmse = float(np.sqrt(np.mean((mean - prob.exact(pts)) ** 2)))
+    value = walks_done / el * len_exact if args.tolerance > 0 else steps / el
     res = _result("2D mixed Dirichlet/Neumann walk on stars", {
         "workload": f"2D WoSt, star curve 4,096 segments + 32 Neumann slits, {n} points, {walks} walks/pt/pass, "
                     "eps 1e-3", "bvh_width": args.bvh_width, "leaf_size": args.leaf_size,
-        "walks_per_lane": args.walks_per_lane, "regroup_interval": args.regroup, "points": n}, steps / el, el, args,
-        {"walks_per_s": walks_done / el, "mean_walk_length": steps / max(walks_done, 1), "rmse_vs_analytic": rmse,
+        "walks_per_lane": args.walks_per_lane, "regroup_interval": args.regroup,
+        "distance_tolerance": args.tolerance, "points": n}, value, el, args,
+        {"value_definition": "completed walks/s x mean walk length with exact distances (reference walk-steps/s)",
+         "walks_per_s": walks_done / el, "raw_walk_steps_per_s": steps / el, "mean_walk_length_exact": len_exact,
+         "mean_walk_length": steps / max(walks_done, 1), "rmse_vs_analytic": rmse,
          "completion_codes": [int(sum(r.completion_codes[k] for r in reps)) for k in range(4)],
          "roofline": _roofline(rc, scene.info.node_bytes, scene.info.prim_bytes, kms, steps / len(reps),
                                "absorbing-boundary closest-point bytes only; the reflecting queries (silhouette, "

@@ -189,6 +189,25 @@ def test_wost_parity_with_oracle_and_closed_form(gpu, oracle_lib, mixed, width):
     assert rep.total_walks == walks
 
 
+def test_wost_relaxed_distance_stays_unbiased(gpu, oracle_lib, mixed):
+    """A certified lower bound of the absorbing distance only shrinks the maximal star radius:
+    the estimates agree with the exact-distance oracle within the 4-SE rule."""
+    torch = gpu
+    O = oracle_lib
+    p, osc, pts, d = mixed
+    pts, d = pts[:400], d[:400]
+    sc = _scene(p, 4)
+    st = E.DeviceStats.zeros(len(pts))
+    rep = E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, 256), st,
+                       E.SolveOptions(seed=31, distance_tolerance=0.2))
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    orep = O.solve(osc, p.pde, p.settings, pts, d, 256, ost, seed=32, wost=True, rng_mode=O.RNG_PER_POINT)
+    assert rep.completion_codes[3] <= 1e-4 * 256 * len(pts)
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, "wost relaxed distance vs oracle")
+
+
 def test_wost_boundary_sample_points(gpu, oracle_lib, mixed):
     """Points on the reflecting boundary (both sides of the double-sided slits) and on the
     absorbing boundary."""

@@ -269,8 +269,11 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         float b2 = FLT_MAX;
         Vec<2> cp = x;
         int prim = -1;
-        bool gotA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, 1.0f, 0.0f, nullptr, absPrim);
-        distA = gotA ? __builtin_sqrtf(b2) : FLT_MAX;
+        // with distance_tolerance > 0 the query returns a certified lower bound of the distance
+        // (exact within 2 epsilon of the boundary): a smaller maximal star radius, never a larger one
+        float rad2 = FLT_MAX;
+        bool gotA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, P.relax2, P.guard2, &rad2, absPrim);
+        distA = gotA ? __builtin_sqrtf(rad2) : FLT_MAX;
         absPrim = gotA ? prim : -1;
         firstStep = false;
     }
