@@ -653,7 +653,7 @@ def bench_batch(args, torch, dev):
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         i0 = int(np.argmin([abs(sp["n_tris"] - 43000) for sp in spec]))   # an instance of typical size
-        sel = (IOP == i0).nonzero().flatten()[:2000]
+        sel = (IOP == i0).nonzero().flatten()[:20000]
         pr = probs[i0]
         osc = O.Scene(pr.positions, pr.indices, None, pr.domain_min, pr.domain_max)
         pts0, d0 = P[sel].cpu().numpy(), D[sel].cpu().numpy()
