@@ -188,6 +188,47 @@ def test_russian_roulette_and_screened_poisson(gpu, oracle_lib, ball3):
             assert 0.9 < ratio < 1.1
 
 
+@pytest.mark.parametrize("screened", [False, True])
+def test_walk_splitting_matches_oracle(gpu, oracle_lib, ball3, screened):
+    """Splitting below throughput 1 (applyWeightWindow): every walk splits at its first step
+    into parts of throughput 0.4 (Yukawa dampening then splits nothing more); the estimate of a
+    walk is the sum over its parts.  GPU and oracle agree within the 4-SE rule, report split
+    parts per point of the same mean, and stay unbiased against the closed form."""
+    torch = gpu
+    O = oracle_lib
+    p0, pts, d, osc = ball3
+    if screened:
+        sigma = 4.0
+        a = np.sqrt(sigma) * np.array([0.6, 0.0, 0.8])
+        pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a)),
+                         absorption=sigma, is_source_constant=True)
+    else:
+        pde = p0.pde
+    ws = E.walk_settings_host(1e-4, 1e-4, 1024, splitting_threshold=0.4, russian_roulette_threshold=0.1)
+    prob = PB.Problem(**{**p0.__dict__, "pde": pde, "settings": ws})
+    n = len(pts)
+    st = E.DeviceStats.zeros(n)
+    rep = E.wos_solve(E.Scene(prob.positions, prob.indices, None, prob.domain_min, prob.domain_max), pde, ws,
+                      _device_points(torch, pts, d, 128), st, E.SolveOptions(seed=41))
+    g = _host(st)
+    g_splits = st.splits.cpu().numpy()
+    ost = O.Stats(n)
+    O.solve(osc, pde, ws, pts, d, 128, ost, seed=42, rng_mode=O.RNG_PER_POINT)
+    inside = d > 1e-4
+    assert (g["n"][inside] == 128).all() and (ost.n[inside] == 128).all()
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, f"splitting (screened={screened})")
+    # extra parts per walk: 1.5 for the harmonic walk at threshold 0.4 (0.6 left over: one copy,
+    # then a second with probability 0.5), fewer once Yukawa dampening has lowered the
+    # throughput; the same law on both sides
+    gs, os_ = g_splits[inside].sum() / (128 * inside.sum()), ost.splits[inside].sum() / (128 * inside.sum())
+    assert (1.4 < gs < 1.6 if not screened else 0.2 < gs < 1.6) and abs(gs / os_ - 1) < 0.05, (gs, os_)
+    exact = PB.exp_linear(pts, a) if screened else prob.exact(pts)
+    pooled = (g["mean"] - exact).sum() / np.sqrt((g["var"] / np.maximum(g["n"], 1)).sum())
+    assert abs(pooled) < 4.0, pooled
+    assert sum(rep.completion_codes) > rep.total_walks   # every part ends with a completion code
+
+
 @pytest.mark.parametrize("tol", [0.1, 0.3])
 def test_relaxed_radius_stays_unbiased(gpu, oracle_lib, ball3, tol):
     """A certified lower bound of the distance is a ball inside the domain: the estimator

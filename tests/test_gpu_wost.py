@@ -269,6 +269,9 @@ def test_wost_argument_errors(gpu, mixed):
     robin.are_robin_conditions_pure_neumann = 0
     with pytest.raises(A.ZombieError):
         E.wost_solve(sc, robin, p.settings, dp, E.DeviceStats.zeros(8))
+    with pytest.raises(A.ZombieError):   # splitting runs in walk on spheres only
+        E.wost_solve(sc, p.pde, E.walk_settings_host(1e-3, 1e-3, 1024, True, splitting_threshold=0.5), dp,
+                     E.DeviceStats.zeros(8))
     s3 = PB.sphere3d(level=2)
     sc3 = E.Scene(s3.positions, s3.indices, None, s3.domain_min, s3.domain_max)
     with pytest.raises(A.ZombieError):

@@ -133,12 +133,10 @@ zg_status checkSettings(const zg_walk_settings* ws) {
     if (ws->max_walk_length < 1) return fail(ZG_ERR_INVALID_ARGUMENT, "max_walk_length must be >= 1");
     if (!(ws->epsilon_shell_for_absorbing_boundary >= 0.0f))
         return fail(ZG_ERR_INVALID_ARGUMENT, "epsilon_shell_for_absorbing_boundary must be >= 0");
-    // walk throughput never exceeds max(1, rr threshold) here: splitting below that level
-    // would need per-walk state queues, which the GPU path does not implement
-    float tmax = std::max(1.0f, ws->russian_roulette_threshold);
-    if (ws->splitting_threshold < tmax)
-        return fail(ZG_ERR_UNSUPPORTED, "splitting_threshold %g < %g: walk splitting is not supported on the GPU",
-                    ws->splitting_threshold, tmax);
+    if (!(ws->splitting_threshold > 0.0f))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "splitting_threshold must be > 0");
+    if (!(ws->russian_roulette_threshold >= 0.0f))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "russian_roulette_threshold must be >= 0");
     return ZG_OK;
 }
 
@@ -365,7 +363,7 @@ struct SolveSetup {
     WosParams P;
     MergeParams M;
     int dim = 3, width = 8, blocks = 1, depth = 0;
-    bool batch = false, count = false, wost = false, screened = false;
+    bool batch = false, count = false, wost = false, screened = false, split = false;
     int64_t n = 0;
     int nWalksMax = 0;
     size_t words = 0;
@@ -389,7 +387,13 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     int dim = s ? s->dim : b->dim;
     int sceneDevice = s ? s->device : b->device;
     if (s) { st = checkPde(pde, dim); if (st != ZG_OK) return st; }
+    // walk throughputs start at 1 and never grow for these PDEs (harmonic and Yukawa kernels,
+    // pure Neumann reflectance), so walks split only below a threshold of 1
+    bool split = ws->splitting_threshold < 1.0f;
     if (wost) {
+        if (split)
+            return fail(ZG_ERR_UNSUPPORTED, "splitting_threshold %g < 1: walk splitting runs in walk on spheres only",
+                        ws->splitting_threshold);
         if (!s || dim != 2) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on 2D scenes");
         if (!pde->are_robin_conditions_pure_neumann)
             return fail(ZG_ERR_UNSUPPORTED, "walk on stars supports pure Neumann reflecting conditions (Robin coefficients 0)");
@@ -463,8 +467,10 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     S.batch = b != nullptr; S.count = o.count_visits != 0; S.n = n; S.nWalksMax = pts->n_walks_max;
     if (s) S.screened = pde->absorption_coeff > 0.0f;
     else for (const zg_pde& ip : b->pdes) S.screened = S.screened || ip.absorption_coeff > 0.0f;
-    S.words = (size_t)n * 4 + kCntSlots + (wost ? (size_t)(n + 1) / 2 : 0);
+    S.words = (size_t)n * 4 + kCntSlots + (wost ? (size_t)(n + 1) / 2 : 0) + (split ? (size_t)n : 0);
     S.wost = wost;
+    S.split = split;
+    M.sSplits = stats->total_splits;
     if (wost) {
         S.callerFsr = pts->first_sphere_radius;
         P.prevN = stats->n_solution_estimates;
@@ -483,6 +489,10 @@ void bindWorkspace(SolveSetup& S, uint64_t* wsp) {
     S.P.counters = reinterpret_cast<unsigned long long*>(wsp + 4 * n);
     S.P.claimCounter = S.P.counters + kCntClaims;
     S.M.accSum = S.P.accSum; S.M.accSq = S.P.accSq; S.M.accN = S.P.accN; S.M.accLen = S.P.accLen;
+    if (S.split) {   // after the walk-on-stars radii (never both: splitting runs in walk on spheres only)
+        S.P.accSplits = reinterpret_cast<unsigned long long*>(wsp + 4 * n + kCntSlots + (S.wost ? (n + 1) / 2 : 0));
+        S.M.accSplits = S.P.accSplits;
+    }
     if (S.wost) {
         float* fsr = reinterpret_cast<float*>(wsp + 4 * n + kCntSlots);
         S.P.fsr = fsr;
@@ -498,8 +508,8 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     if (e == hipSuccess && S.wost) e = launchWostFirstSphere(S.P, S.width, S.callerFsr, const_cast<float*>(S.P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
         e = S.wost ? launchWost(S.P, S.width, S.count, S.blocks, strm)
-                   : S.batch ? launchWosBatch(S.P, S.width, S.count, S.screened, S.blocks, strm)
-                             : launchWos(S.P, S.dim, S.width, S.count, S.screened, S.blocks, strm);
+                   : S.batch ? launchWosBatch(S.P, S.width, S.count, S.screened, S.split, S.blocks, strm)
+                             : launchWos(S.P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
     if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
     if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
     return e;

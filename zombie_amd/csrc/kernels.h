@@ -61,7 +61,11 @@ struct WosParams {
     const uint32_t* streamOffsetDev;  // when set, replaces streamOffset (graph replays advance it)
     const float* fsr;                 // walk on stars: first sphere radius per point
     const int64_t* prevN;             // walk on stars: caller's estimate counts (first call when 0)
+    unsigned long long* accSplits;    // walk splitting: split parts per point
 };
+
+// capacity of the per-lane queue of split walk parts
+constexpr int kSplitQueue = 8;
 
 struct MergeParams {
     int64_t n;
@@ -89,6 +93,8 @@ struct MergeParams {
     uint32_t* streamOffsetDev;        // advanced by streamAdvance once the merge is done
     uint32_t streamAdvance;
     const float* fsrSrc;              // walk on stars: first sphere radius to record (else dist0)
+    const unsigned long long* accSplits;
+    int64_t* sSplits;                 // caller's total split counts (nullable)
 };
 
 hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s);
@@ -97,8 +103,10 @@ hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, cons
 hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);
 hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* o, const float* d, const float* tmax,
                      uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
-hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, int gridBlocks, hipStream_t s);
-hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, int gridBlocks, hipStream_t s);
+hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, bool split, int gridBlocks,
+                     hipStream_t s);
+hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, bool split, int gridBlocks,
+                          hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
 // read-bandwidth probe
 hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks, float* sink, hipStream_t s);

@@ -150,7 +150,12 @@ ZG_DEV Vec<D> cvField(const WosParams& P, const Vec<D>& x) {
 }
 
 // SCREENED = false compiles the Yukawa (screened Poisson) terms out of the walk.
-template <int D, int W, bool COUNT, bool BATCH, bool SCREENED>
+// SPLIT = true adds walk splitting (applyWeightWindow, walk_on_spheres.h:192-226): a walk whose
+// throughput exceeds splitting_threshold continues at the threshold and queues copies of its
+// state (throughput = threshold, length 0, no source sum) in a per-lane private queue; the
+// estimate of the original walk is the sum over its parts.  When the queue cannot take the
+// copies the walk is not split (an unsplit walk is just as unbiased).
+template <int D, int W, bool COUNT, bool BATCH, bool SCREENED, bool SPLIT = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WosOccupancy<W>::kWaves, 8)))
 wosKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
@@ -179,6 +184,11 @@ wosKernel(WosParams P) {
     int len = 0;
     Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = __builtin_sqrtf(sigma);
     Pcg32 rng; rng.state = 0; rng.inc = 1;
+    // splitting: parts of the current walk still to run, the running sum of its parts
+    float4 splitQ[SPLIT ? kSplitQueue : 1];
+    int nq = 0, nSplits = 0;
+    float partSum = 0.0f;
+    bool partOk = false;
     ClosestQuery<D> q;
     q.begin(FLT_MAX, 1.0f, 0.0f);
     // block counters in LDS: walks per completion code, steps, walks started
@@ -208,6 +218,7 @@ wosKernel(WosParams P) {
         ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
         ycValid = false;
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+        if (SPLIT) { nq = 0; nSplits = 0; partSum = 0.0f; partOk = false; }
         atomicAdd(&blk[5], 1ull);
         if (dist > eps) { mode = 1; pendingCode = -1; }
         else beginQuery(FLT_MAX, 1.0f, ZG_REACHED_ABSORBING_BOUNDARY);  // walk ends at once: project
@@ -228,13 +239,40 @@ wosKernel(WosParams P) {
                 }
                 g = evalField<D>(pde().dirichlet, yc, nrm, aligned, sigma);
             }
-            // FP64 per-point accumulation of the walk estimate
-            double c = (double)(T * g + S);
-            atomicAdd(&P.accSum[p], c);
-            atomicAdd(&P.accSq[p], c * c);
-            atomicAdd(&P.accN[p], 1ull);
+            if (SPLIT) {
+                partSum += T * g + S;
+                partOk = true;
+            } else {
+                // FP64 per-point accumulation of the walk estimate
+                double c = (double)(T * g + S);
+                atomicAdd(&P.accSum[p], c);
+                atomicAdd(&P.accSq[p], c * c);
+                atomicAdd(&P.accN[p], 1ull);
+            }
             atomicAdd(&P.accLen[p], (unsigned long long)len);
             atomicAdd(&blk[4], (unsigned long long)len);
+        }
+        if (SPLIT) {
+            if (nq > 0) {
+                // next part of the same walk: its state, an exact distance query at its point
+                nq--;
+                float4 e = splitQ[nq];
+                x[0] = e.x; x[1] = e.y;
+                if (D == 3) x[D - 1] = e.z;
+                T = e.w; S = 0.0f; len = 0;
+                ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
+                ycValid = false; ycPrim = -1;
+                nSplits++;
+                beginQuery(FLT_MAX, 1.0f, -1);
+                return;
+            }
+            if (partOk) {
+                double c = (double)partSum;
+                atomicAdd(&P.accSum[p], c);
+                atomicAdd(&P.accSq[p], c * c);
+                atomicAdd(&P.accN[p], 1ull);
+                if (nSplits) atomicAdd(&P.accSplits[p], (unsigned long long)nSplits);
+            }
         }
         w++;
         mode = w < wEnd ? 3 : 0;   // 3: the claim's next walk starts at the top of the loop
@@ -319,7 +357,18 @@ wosKernel(WosParams P) {
             finishWalk(ZG_ESCAPED_DOMAIN);
             continue;
         }
-        if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
+        if (SPLIT && T > ws.splitting_threshold) {  // applyWeightWindow :198-213
+            const float thr = ws.splitting_threshold;
+            float left = T - thr;
+            int copies = 0;
+            for (float l = left; l > thr && copies <= kSplitQueue; l -= thr) copies++;
+            if (nq + copies + 1 <= kSplitQueue) {
+                T = thr;
+                float4 e = make_float4(x[0], x[1], D == 3 ? x[D - 1] : 0.0f, thr);
+                while (left > thr) { left -= thr; splitQ[nq++] = e; }
+                if (rng.nextFloat() < left / thr) splitQ[nq++] = e;
+            }
+        } else if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
             float survival = T / ws.russian_roulette_threshold;
             if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
             T = ws.russian_roulette_threshold;
@@ -399,6 +448,7 @@ __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
         M.sN[i] = N1 + nb;
     }
     if (M.sLen) M.sLen[i] += (int64_t)M.accLen[i];
+    if (M.sSplits && M.accSplits) M.sSplits[i] += (int64_t)M.accSplits[i];
 }
 
 __global__ void fillZeroKernel(uint64_t* __restrict__ p, int64_t n) {
@@ -453,32 +503,41 @@ hipError_t launchRay(const SceneView& sv, int boundary, int64_t n, const float* 
     return hipGetLastError();
 }
 
-hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, int gridBlocks, hipStream_t s) {
+hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool screened, bool split, int gridBlocks,
+                     hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-#define ZG_WOS(D, Wd, C, S) hipLaunchKernelGGL((wosKernel<D, Wd, C, false, S>), g, bl, kStackLdsBytes, s, P)
-    if (dim == 2) {  // 2D walks are harmonic (screened Poisson is 3D only)
-        if (width == 4) { if (count) ZG_WOS(2, 4, true, false); else ZG_WOS(2, 4, false, false); }
-        else { if (count) ZG_WOS(2, 8, true, false); else ZG_WOS(2, 8, false, false); }
+#define ZG_WOS(D, Wd, C, S, SP) hipLaunchKernelGGL((wosKernel<D, Wd, C, false, S, SP>), g, bl, kStackLdsBytes, s, P)
+    if (split) {   // no visit counting with splitting
+        if (dim == 2) { if (width == 4) ZG_WOS(2, 4, false, false, true); else ZG_WOS(2, 8, false, false, true); }
+        else if (screened) { if (width == 4) ZG_WOS(3, 4, false, true, true); else ZG_WOS(3, 8, false, true, true); }
+        else { if (width == 4) ZG_WOS(3, 4, false, false, true); else ZG_WOS(3, 8, false, false, true); }
+    } else if (dim == 2) {  // 2D walks are harmonic (screened Poisson is 3D only)
+        if (width == 4) { if (count) ZG_WOS(2, 4, true, false, false); else ZG_WOS(2, 4, false, false, false); }
+        else { if (count) ZG_WOS(2, 8, true, false, false); else ZG_WOS(2, 8, false, false, false); }
     } else if (screened) {
-        if (width == 4) { if (count) ZG_WOS(3, 4, true, true); else ZG_WOS(3, 4, false, true); }
-        else { if (count) ZG_WOS(3, 8, true, true); else ZG_WOS(3, 8, false, true); }
+        if (width == 4) { if (count) ZG_WOS(3, 4, true, true, false); else ZG_WOS(3, 4, false, true, false); }
+        else { if (count) ZG_WOS(3, 8, true, true, false); else ZG_WOS(3, 8, false, true, false); }
     } else {
-        if (width == 4) { if (count) ZG_WOS(3, 4, true, false); else ZG_WOS(3, 4, false, false); }
-        else { if (count) ZG_WOS(3, 8, true, false); else ZG_WOS(3, 8, false, false); }
+        if (width == 4) { if (count) ZG_WOS(3, 4, true, false, false); else ZG_WOS(3, 4, false, false, false); }
+        else { if (count) ZG_WOS(3, 8, true, false, false); else ZG_WOS(3, 8, false, false, false); }
     }
 #undef ZG_WOS
     return hipGetLastError();
 }
 
-hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, int gridBlocks, hipStream_t s) {
+hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, bool split, int gridBlocks,
+                          hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-#define ZG_WOSB(Wd, C, S) hipLaunchKernelGGL((wosKernel<3, Wd, C, true, S>), g, bl, kStackLdsBytes, s, P)
-    if (screened) {
-        if (width == 4) { if (count) ZG_WOSB(4, true, true); else ZG_WOSB(4, false, true); }
-        else { if (count) ZG_WOSB(8, true, true); else ZG_WOSB(8, false, true); }
+#define ZG_WOSB(Wd, C, S, SP) hipLaunchKernelGGL((wosKernel<3, Wd, C, true, S, SP>), g, bl, kStackLdsBytes, s, P)
+    if (split) {
+        if (screened) { if (width == 4) ZG_WOSB(4, false, true, true); else ZG_WOSB(8, false, true, true); }
+        else { if (width == 4) ZG_WOSB(4, false, false, true); else ZG_WOSB(8, false, false, true); }
+    } else if (screened) {
+        if (width == 4) { if (count) ZG_WOSB(4, true, true, false); else ZG_WOSB(4, false, true, false); }
+        else { if (count) ZG_WOSB(8, true, true, false); else ZG_WOSB(8, false, true, false); }
     } else {
-        if (width == 4) { if (count) ZG_WOSB(4, true, false); else ZG_WOSB(4, false, false); }
-        else { if (count) ZG_WOSB(8, true, false); else ZG_WOSB(8, false, false); }
+        if (width == 4) { if (count) ZG_WOSB(4, true, false, false); else ZG_WOSB(4, false, false, false); }
+        else { if (count) ZG_WOSB(8, true, false, false); else ZG_WOSB(8, false, false, false); }
     }
 #undef ZG_WOSB
     return hipGetLastError();
