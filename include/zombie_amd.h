@@ -84,16 +84,29 @@ void zg_walk_settings_default(zg_walk_settings* ws, float epsilon_shell_absorbin
 #define ZG_FN_SIN_PRODUCT 3     /* p0 * prod_{i<DIM} sin(p1*x_i)        (-Lap u = DIM p1^2 u) */
 #define ZG_FN_CUBIC_HARMONIC 4  /* p0 * (x^3 - 3 x y^2)                 (harmonic)          */
 #define ZG_FN_EXP_LINEAR 5      /* p0 * exp(p1*x + p2*y + p3*z)         (Lap u = |a|^2 u)   */
+#define ZG_FN_DENSE_GRID 6      /* values on a cell-centred grid (see zg_function.grid)    */
 
 /* which quantity of the manufactured field u a descriptor evaluates */
 #define ZG_EVAL_VALUE 0              /* u(x)                                                 */
 #define ZG_EVAL_NORMAL_DERIVATIVE 1  /* s * grad u(x) . n, s = -1 on the normal-aligned side  */
 #define ZG_EVAL_SCREENED_SOURCE 2    /* -Lap u(x) + absorption * u(x)                        */
 
+/* ZG_FN_DENSE_GRID: DenseGrid<float,1,DIM> (utils/dense_grid.h:30-356) with the semantics of
+ * its getDenseGridCallback0: grid[] holds shape[0] x shape[1] (x shape[2]) cell values, the
+ * last axis fastest, for cells centred at grid_min + (i + 1/2) (grid_max - grid_min) / shape;
+ * interpolate = 0 returns the value of the cell containing x (clamped to the grid),
+ * 1 bi/trilinear interpolation between cell centres, clamped to the edge values.  The value is
+ * returned whatever `eval` says.  grid points to device memory for the GPU solvers (host memory
+ * for a CPU caller), unused dimensions of a 2D grid have shape 1. */
 typedef struct zg_function {
     int32_t kind;      /* ZG_FN_*   */
     int32_t eval;      /* ZG_EVAL_* */
     float params[6];
+    const float* grid;
+    int32_t shape[3];
+    int32_t interpolate;
+    float grid_min[3];
+    float grid_max[3];
 } zg_function;
 
 typedef struct zg_pde {
@@ -303,6 +316,14 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,
                         zg_solve_report* report_host, void* stream);
+
+/* Evaluates a PDE data descriptor at n points (the PDE<T,DIM> callbacks source / dirichlet /
+ * robin of pde.h:30-44, as the solvers evaluate them): out[i] = f(pts[i], normals[i],
+ * aligned[i]) with the absorption coefficient used by ZG_EVAL_SCREENED_SOURCE.  normals and
+ * aligned nullable; device arrays. */
+zg_status zg_evaluate_function(int32_t dim, const zg_function* f_host, int64_t n, const float* pts,
+                               const float* normals, const uint8_t* aligned, float absorption, float* out,
+                               void* stream);
 
 /* Measurement utility (no counterpart in the reference): reads `bytes` of device memory at
  * `buf` `reps` times with float4 loads over every CU and returns the elapsed milliseconds.

@@ -259,8 +259,46 @@ struct GreensBall {
 // manufactured PDE data (zg_function descriptors), evaluated in float like the
 // reference's T=float callbacks
 // ------------------------------------------------------------------------------------
+// DenseGrid<float,1,DIM>::operator() (utils/dense_grid.h:264-356): cell-centred values over
+// [grid_min, grid_max], last axis fastest; nearest cell, or bi/trilinear interpolation between
+// cell centres with the blend fractions clamped to [0, 1] (clamp to edge)
+template <int D>
+float gridValue(const zg_function& f, const V<D>& x) {
+    float l[3]; int sh[3];
+    for (int a = 0; a < D; a++) { l[a] = (x[a] - f.grid_min[a]) / (f.grid_max[a] - f.grid_min[a]); sh[a] = f.shape[a]; }
+    if (!f.interpolate) {
+        size_t flat = 0, stride = 1;
+        for (int a = D - 1; a >= 0; a--) {
+            int idx = std::clamp((int)std::floor(l[a] * sh[a]), 0, sh[a] - 1);
+            flat += (size_t)idx * stride; stride *= (size_t)sh[a];
+        }
+        return f.grid[flat];
+    }
+    int i0[3], i1[3]; float t[3];
+    for (int a = 0; a < D; a++) {
+        float u = sh[a] == 1 ? 0.0f : l[a] * sh[a] - 0.5f;
+        i0[a] = std::clamp((int)std::floor(u), 0, sh[a] - 1);
+        i1[a] = std::clamp(i0[a] + 1, 0, sh[a] - 1);
+        t[a] = std::clamp(u - (float)i0[a], 0.0f, 1.0f);
+    }
+    const float* g = f.grid;
+    if constexpr (D == 2) {
+        return g[i0[0] * sh[1] + i0[1]] * (1 - t[0]) * (1 - t[1]) + g[i1[0] * sh[1] + i0[1]] * t[0] * (1 - t[1]) +
+               g[i0[0] * sh[1] + i1[1]] * (1 - t[0]) * t[1] + g[i1[0] * sh[1] + i1[1]] * t[0] * t[1];
+    } else {
+        float r = 0.0f;
+        for (int c = 0; c < 8; c++) {
+            int ci = (c >> 2) & 1, cj = (c >> 1) & 1, ck = c & 1;
+            float w = (ci ? t[0] : 1 - t[0]) * (cj ? t[1] : 1 - t[1]) * (ck ? t[2] : 1 - t[2]);
+            r += w * g[((ci ? i1[0] : i0[0]) * sh[1] + (cj ? i1[1] : i0[1])) * sh[2] + (ck ? i1[2] : i0[2])];
+        }
+        return r;
+    }
+}
+
 template <int D>
 float evalField(const zg_function& f, const V<D>& x, const V<D>& n, bool normalAligned, float absorption) {
+    if (f.kind == ZG_FN_DENSE_GRID) return gridValue<D>(f, x);
     const float* p = f.params;
     float z = D == 3 ? x[2] : 0.0f;
     float u = 0.0f;

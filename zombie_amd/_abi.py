@@ -23,6 +23,7 @@ ZG_FN_EXP_COS = 2
 ZG_FN_SIN_PRODUCT = 3
 ZG_FN_CUBIC_HARMONIC = 4
 ZG_FN_EXP_LINEAR = 5
+ZG_FN_DENSE_GRID = 6
 
 ZG_EVAL_VALUE = 0
 ZG_EVAL_NORMAL_DERIVATIVE = 1
@@ -62,7 +63,9 @@ class zg_walk_settings(C.Structure):
 
 
 class zg_function(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("eval", C.c_int32), ("params", C.c_float * 6)]
+    _fields_ = [("kind", C.c_int32), ("eval", C.c_int32), ("params", C.c_float * 6), ("grid", C.c_void_p),
+                ("shape", C.c_int32 * 3), ("interpolate", C.c_int32), ("grid_min", C.c_float * 3),
+                ("grid_max", C.c_float * 3)]
 
 
 class zg_pde(C.Structure):
@@ -183,6 +186,8 @@ SIGNATURES = {
     "zg_inside_domain": (C.c_int32, [_P, C.c_int64, _P, _P, _P]),
     "zg_intersect_ray": (C.c_int32, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_star_radius": (C.c_int32, [_P, C.c_int64, _P, _P, _P, C.c_float, _P, _P, _P]),
+    "zg_evaluate_function": (C.c_int32, [C.c_int32, C.POINTER(zg_function), C.c_int64, _P, _P, _P, C.c_float, _P,
+                                         _P]),
     "zg_probe_read_bandwidth": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_float), _P]),
     "zg_line_of_sight": (C.c_int32, [_P, C.c_int64, _P, _P, _P, _P]),
     "zg_sample_reflecting": (C.c_int32, [_P, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),

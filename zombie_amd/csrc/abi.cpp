@@ -128,6 +128,18 @@ int64_t nodeBytes(int dim, int width) {
     return width == 4 ? NodeLayout<3, 4>::kBytes : NodeLayout<3, 8>::kBytes;
 }
 
+zg_status checkFunction(const zg_function& f, int dim) {
+    if (f.kind < ZG_FN_ZERO || f.kind > ZG_FN_DENSE_GRID) return fail(ZG_ERR_INVALID_ARGUMENT, "unknown function kind %d", f.kind);
+    if (f.kind != ZG_FN_DENSE_GRID) return ZG_OK;
+    if (!f.grid) return fail(ZG_ERR_INVALID_ARGUMENT, "dense grid without data");
+    for (int a = 0; a < 3; a++) {
+        if (f.shape[a] < 1 || (a >= dim && f.shape[a] != 1))
+            return fail(ZG_ERR_INVALID_ARGUMENT, "dense grid shape[%d] = %d is invalid", a, f.shape[a]);
+        if (a < dim && !(f.grid_max[a] > f.grid_min[a])) return fail(ZG_ERR_INVALID_ARGUMENT, "dense grid box is empty");
+    }
+    return ZG_OK;
+}
+
 zg_status checkSettings(const zg_walk_settings* ws) {
     if (!ws) return fail(ZG_ERR_INVALID_ARGUMENT, "walk settings are NULL");
     if (ws->max_walk_length < 1) return fail(ZG_ERR_INVALID_ARGUMENT, "max_walk_length must be >= 1");
@@ -299,6 +311,16 @@ zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const f
     return ok();
 }
 
+zg_status zg_evaluate_function(int32_t dim, const zg_function* f, int64_t n, const float* pts, const float* normals,
+                               const uint8_t* aligned, float absorption, float* out, void* stream) {
+    if (!f || (dim != 2 && dim != 3) || n < 0 || (n > 0 && (!pts || !out)))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    zg_status st = checkFunction(*f, dim);
+    if (st != ZG_OK) return st;
+    ZG_HIP(launchEvalFunction(*f, dim, n, pts, normals, aligned, absorption, out, (hipStream_t)stream));
+    return ok();
+}
+
 zg_status zg_probe_read_bandwidth(const void* buf, int64_t bytes, int32_t reps, float* ms_out, void* stream) {
     if (!buf || bytes < 16 || reps < 1 || !ms_out) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
     int dev = 0;
@@ -371,6 +393,8 @@ struct SolveSetup {
 };
 
 zg_status checkPde(const zg_pde* pde, int dim) {
+    const zg_function* fs[4] = {&pde->source, &pde->dirichlet, &pde->robin, &pde->robin_coeff};
+    for (const zg_function* f : fs) { zg_status st = checkFunction(*f, dim); if (st != ZG_OK) return st; }
     if (pde->absorption_coeff < 0.0f) return fail(ZG_ERR_INVALID_ARGUMENT, "absorption_coeff must be >= 0");
     if (pde->absorption_coeff > 0.0f && dim == 2)
         return fail(ZG_ERR_UNSUPPORTED, "screened Poisson (absorption > 0) is supported in 3D only");

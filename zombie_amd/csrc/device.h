@@ -168,8 +168,60 @@ struct Ball {
 // error): the accurate library versions cost 5% of the walk kernel's time.
 ZG_DEV void fieldSinCos(float a, float* s, float* c) { __sincosf(a, s, c); }
 ZG_DEV float fieldExp(float a) { return __expf(a); }
+// DenseGrid<float,1,DIM>::operator() (utils/dense_grid.h:264-356): cell-centred values over
+// [grid_min, grid_max], last axis fastest; nearest cell, or bi/trilinear interpolation between
+// cell centres with the blend fractions clamped to [0, 1] (clamp to edge).
+template <int D>
+ZG_DEV float gridValue(const zg_function& f, const Vec<D>& x) {
+    float l[3];
+    int sh[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        float ext = f.grid_max[a] - f.grid_min[a];
+        l[a] = (x[a] - f.grid_min[a]) / ext;
+        sh[a] = f.shape[a];
+    }
+    if (!f.interpolate) {
+        int flat = 0, stride = 1;
+#pragma unroll
+        for (int a = D - 1; a >= 0; a--) {
+            int idx = min(max((int)floorf(l[a] * (float)sh[a]), 0), sh[a] - 1);
+            flat += idx * stride;
+            stride *= sh[a];
+        }
+        return f.grid[flat];
+    }
+    int i0[3], i1[3];
+    float t[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        float u = sh[a] == 1 ? 0.0f : l[a] * (float)sh[a] - 0.5f;
+        i0[a] = min(max((int)floorf(u), 0), sh[a] - 1);
+        i1[a] = min(max(i0[a] + 1, 0), sh[a] - 1);
+        t[a] = fminf(fmaxf(u - (float)i0[a], 0.0f), 1.0f);
+    }
+    if (D == 2) {
+        const float* g = f.grid;
+        float f00 = g[i0[0] * sh[1] + i0[1]], f01 = g[i0[0] * sh[1] + i1[1]];
+        float f10 = g[i1[0] * sh[1] + i0[1]], f11 = g[i1[0] * sh[1] + i1[1]];
+        return f00 * (1.0f - t[0]) * (1.0f - t[1]) + f10 * t[0] * (1.0f - t[1]) + f01 * (1.0f - t[0]) * t[1] +
+               f11 * t[0] * t[1];
+    }
+    const float* g = f.grid;
+    auto at = [&](int i, int j, int k) { return g[(i * sh[1] + j) * sh[2] + k]; };
+    float r = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        int ci = (c >> 2) & 1, cj = (c >> 1) & 1, ck = c & 1;
+        float wgt = (ci ? t[0] : 1.0f - t[0]) * (cj ? t[1] : 1.0f - t[1]) * (ck ? t[2] : 1.0f - t[2]);
+        r += wgt * at(ci ? i1[0] : i0[0], cj ? i1[1] : i0[1], ck ? i1[2] : i0[2]);
+    }
+    return r;
+}
+
 template <int D>
 ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, bool aligned, float absorption) {
+    if (f.kind == ZG_FN_DENSE_GRID) return gridValue<D>(f, x);
     const float* p = f.params;
     float u = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, negLap = 0.0f;
     switch (f.kind) {

@@ -108,6 +108,8 @@ hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool sc
 hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, bool split, int gridBlocks,
                           hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
+hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const float* pts, const float* normals,
+                              const uint8_t* aligned, float absorption, float* out, hipStream_t s);
 // read-bandwidth probe
 hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks, float* sink, hipStream_t s);
 // walk on stars (2D)

@@ -451,6 +451,19 @@ __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
     if (M.sSplits && M.accSplits) M.sSplits[i] += (int64_t)M.accSplits[i];
 }
 
+// batched evaluation of a PDE data descriptor
+template <int D>
+__global__ void __launch_bounds__(256) evalFunctionKernel(zg_function f, int64_t n, const float* __restrict__ pts,
+                                                          const float* __restrict__ normals,
+                                                          const uint8_t* __restrict__ aligned, float absorption,
+                                                          float* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Vec<D> x = vload<D>(pts + i * D);
+    Vec<D> nr = normals ? vload<D>(normals + i * D) : vzero<D>();
+    out[i] = evalField<D>(f, x, nr, aligned && aligned[i], absorption);
+}
+
 __global__ void fillZeroKernel(uint64_t* __restrict__ p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -540,6 +553,15 @@ hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screen
         else { if (count) ZG_WOSB(8, true, false, false); else ZG_WOSB(8, false, false, false); }
     }
 #undef ZG_WOSB
+    return hipGetLastError();
+}
+
+hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const float* pts, const float* normals,
+                              const uint8_t* aligned, float absorption, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 g((unsigned)((n + 255) / 256)), bl(256);
+    if (dim == 2) hipLaunchKernelGGL(evalFunctionKernel<2>, g, bl, 0, s, f, n, pts, normals, aligned, absorption, out);
+    else hipLaunchKernelGGL(evalFunctionKernel<3>, g, bl, 0, s, f, n, pts, normals, aligned, absorption, out);
     return hipGetLastError();
 }
 

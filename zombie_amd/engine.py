@@ -41,6 +41,46 @@ def function(kind: int, eval_: int = A.ZG_EVAL_VALUE, params=()) -> A.zg_functio
     return f
 
 
+class GridFunction:
+    """A ZG_FN_DENSE_GRID descriptor (DenseGrid<float,1,DIM>, utils/dense_grid.h) with its value
+    array kept alive: values[i, j(, k)] is the value of the cell centred at
+    lo + (index + 1/2) (hi - lo) / shape.  `fn` is the zg_function to put in a zg_pde; the object
+    must outlive every solve that uses it.  device=None keeps the values in host memory (for the
+    CPU oracle)."""
+
+    def __init__(self, values, lo, hi, interpolate: bool = True, device="cuda"):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if v.ndim not in (2, 3):
+            raise ValueError("values must be a 2D or 3D array")
+        if device is None:
+            self.values = v
+            ptr = v.ctypes.data
+        else:
+            torch = _torch()
+            self.values = torch.from_numpy(v).to(device).contiguous()
+            ptr = self.values.data_ptr()
+        f = A.zg_function()
+        f.kind = A.ZG_FN_DENSE_GRID
+        f.eval = A.ZG_EVAL_VALUE
+        f.grid = ptr
+        for a in range(3):
+            f.shape[a] = v.shape[a] if a < v.ndim else 1
+            f.grid_min[a] = float(lo[a]) if a < v.ndim else 0.0
+            f.grid_max[a] = float(hi[a]) if a < v.ndim else 1.0
+        f.interpolate = 1 if interpolate else 0
+        self.fn = f
+
+
+def evaluate_function(dim: int, f: A.zg_function, pts, normals=None, aligned=None, absorption: float = 0.0,
+                      stream=None):
+    """f at the device points pts [n, dim] (zg_evaluate_function)."""
+    torch = _torch()
+    out = torch.empty(pts.shape[0], dtype=torch.float32, device=pts.device)
+    A.check(A.lib().zg_evaluate_function(dim, C.byref(f), pts.shape[0], _ptr(pts), _ptr(normals), _ptr(aligned),
+                                         float(absorption), _ptr(out), _stream_ptr(stream)))
+    return out
+
+
 def constant(value: float) -> A.zg_function:
     return function(A.ZG_FN_CONSTANT, A.ZG_EVAL_VALUE, (value,))
 
