@@ -222,6 +222,11 @@ zg_status zg_sample_reflecting(const zg_scene* scene, int64_t n, const float* x,
 #define ZG_SAMPLE_ON_ABSORBING_BOUNDARY 1
 #define ZG_SAMPLE_ON_REFLECTING_BOUNDARY 2
 
+/* EstimationQuantity (common.h:145-149) */
+#define ZG_ESTIMATE_SOLUTION 0
+#define ZG_ESTIMATE_SOLUTION_AND_GRADIENT 1
+#define ZG_ESTIMATE_SKIP 2
+
 /* A vector field sampled on a regular grid over [lo, hi] (res[0] x res[1] x res[2] nodes,
  * x fastest; 4 floats per node, the 4th unused), read with (bi/tri)linear interpolation
  * clamped to the box.  Used as a control-variate gradient field (see cv_gradient_grid). */
@@ -252,6 +257,9 @@ typedef struct zg_sample_points {
                                              /* -T_k G(x_k).(x_{k+1} - x_k), zero mean too  */
     int32_t n_walks_max;                     /* host-side upper bound of n_walks[]         */
     int32_t reserved;
+    const uint8_t* estimation_quantity;      /* [n] nullable (all ZG_ESTIMATE_SOLUTION)     */
+    const float* direction_for_derivative;   /* [n*dim] nullable (e_0): direction of the    */
+                                             /* directional derivative of gradient points  */
 } zg_sample_points;
 
 /* SampleStatistics (common.h:200-395) in FP64, structure of arrays, device pointers. */
@@ -261,6 +269,12 @@ typedef struct zg_statistics {
     double* solution_m2;
     int64_t* total_walk_length;
     int64_t* total_splits;
+    /* gradient members, required when a point asks for ZG_ESTIMATE_SOLUTION_AND_GRADIENT */
+    int64_t* n_gradient_estimates;
+    double* gradient_mean;                    /* [n*dim]                                    */
+    double* gradient_m2;                      /* [n*dim]                                    */
+    double* total_first_source_contribution;  /* [n]                                        */
+    double* total_derivative_contribution;    /* [n]                                        */
 } zg_statistics;
 
 /* WalkCompletionCode (common.h:132-137) */
@@ -300,7 +314,15 @@ typedef struct zg_solve_report {
 
 /* WalkOnSpheres<float,DIM>::solve (walk_on_spheres.h:131-168 -> estimateSolution :320-414,
  * walk :228-291, getTerminalContribution :293-318) for every point in one persistent launch.
- * report_host nullable; when given, the call synchronises `stream` and fills it. */
+ * report_host nullable; when given, the call synchronises `stream` and fills it.
+ * Points with estimation_quantity ZG_ESTIMATE_SKIP are left alone; points asking for
+ * ZG_ESTIMATE_SOLUTION_AND_GRADIENT run estimateSolutionAndGradient (:416-593): first sphere of
+ * radius 0.99 * dist_to_absorbing_boundary, Latin hypercube directions, antithetic pairs,
+ * cosine-weighted boundary directions, gradient control variates (the point's solution mean and
+ * mean first-step source term).  The pairs of a call run in stages [0,1), [1,2), [2,4), ...;
+ * the control variates are read from the statistics when a stage starts (the reference updates
+ * them before every pair).  With gradient points the call copies estimation_quantity to the host
+ * and synchronises `stream`. */
 zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                        zg_statistics* stats, const zg_solve_options* opts_host,
@@ -316,6 +338,14 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,
                         zg_solve_report* report_host, void* stream);
+
+/* The per-point Latin hypercube tables of a gradient solve (generateStratifiedSamples<DIM-1>,
+ * core/sampling.h:435-458): table t holds n_samples rows of `dims` floats and is drawn from
+ * pcg32(~seed, (t << 32) | stream_offset).  A solve with seed s and stream offset o uses the
+ * table of point p with 2 * n_pairs rows (n_pairs = n_walks / 2 with antithetic variates, else
+ * n_walks).  out: device array of n_tables * n_samples * dims floats. */
+zg_status zg_stratified_samples(int32_t dims, int64_t n_tables, int32_t n_samples, uint64_t seed,
+                                uint32_t stream_offset, float* out, void* stream);
 
 /* Evaluates a PDE data descriptor at n points (the PDE<T,DIM> callbacks source / dirichlet /
  * robin of pde.h:30-44, as the solvers evaluate them): out[i] = f(pts[i], normals[i],

@@ -57,6 +57,14 @@ def lib():
         L.orc_solve.argtypes = [C.c_int, P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings), C.c_int64, P, P, P, P,
                                 P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, P, P, P, P, P, P, P]
         L.orc_solve.restype = C.c_int
+        L.orc_solve_grad.argtypes = [P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings), C.c_int64, P, P, P, P, P,
+                                     P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P,
+                                     P, P, P, P]
+        L.orc_solve_grad.restype = C.c_int
+        L.orc_stratified_samples.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_uint64, C.c_uint32, P]
+        L.orc_pcg32_bounded.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, P]
+        L.orc_greens_ball_gradient.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, P]
+        L.orc_gradient_direction.argtypes = [C.c_int, C.c_int, P, P, P]
         _lib = L
     return _lib
 
@@ -81,6 +89,34 @@ def pcg32_default_u32(n):
     out = np.empty(n, np.uint32)
     lib().orc_pcg32_default_u32(n, _p(out))
     return out
+
+
+def pcg32_bounded(initstate, initseq, bound, n):
+    out = np.empty(n, np.uint32)
+    lib().orc_pcg32_bounded(initstate, initseq, bound, n, _p(out))
+    return out
+
+
+def stratified_samples(dims, n_tables, n_samples, seed, stream_offset=0):
+    out = np.empty((n_tables, n_samples, dims), np.float32)
+    lib().orc_stratified_samples(dims, n_tables, n_samples, seed, stream_offset, _p(out))
+    return out
+
+
+def greens_ball_gradient(dim, R, lam, r):
+    out = np.empty(2, np.float32)
+    lib().orc_greens_ball_gradient(dim, R, lam, r, _p(out))
+    return dict(gradient_norm=float(out[0]), poisson_kernel_gradient=float(out[1]))
+
+
+def gradient_direction(dim, cosine, u, n=None):
+    """Boundary direction of the gradient path from uniforms u (dim-1 values, plus the sign
+    coin when cosine); returns (direction, pdf)."""
+    u = np.ascontiguousarray(u, np.float32)
+    nn = np.zeros(dim, np.float32) if n is None else np.ascontiguousarray(n, np.float32)
+    out = np.empty(dim + 1, np.float32)
+    lib().orc_gradient_direction(dim, 1 if cosine else 0, _p(u), _p(nn), _p(out))
+    return out[:dim].copy(), float(out[dim])
 
 
 def sample_unit_sphere(dim, seed, n):
@@ -217,6 +253,52 @@ class Stats:
 
     def variance(self):
         return self.m2 / np.maximum(self.n - 1, 1)
+
+
+class GradStats(Stats):
+    """Stats plus the gradient members of SampleStatistics."""
+
+    def __init__(self, n, dim):
+        super().__init__(n)
+        self.dim = dim
+        self.n_grad = np.zeros(n, np.int64)
+        self.grad_mean = np.zeros((n, dim), np.float64)
+        self.grad_m2 = np.zeros((n, dim), np.float64)
+        self.first_source = np.zeros(n, np.float64)
+        self.derivative = np.zeros(n, np.float64)
+
+    def grad_variance(self):
+        return self.grad_m2 / np.maximum(self.n_grad - 1, 1)[:, None]
+
+    def derivative_mean(self):
+        return self.derivative / np.maximum(self.n, 1)
+
+
+CV_PER_PAIR = 0   # the reference: control variates refreshed before every antithetic pair
+CV_STAGED = 1     # the GPU: refreshed when a stage [0,1), [1,2), [2,4), ... starts
+
+
+def solve_grad(scene: Scene, pde, ws, pts, dist, n_walks, stats: GradStats, quantity=None, direction=None,
+               seed=0x5EED, stream_offset=0, rng_mode=RNG_PER_WALK, cv_mode=CV_PER_PAIR, threads=None,
+               types=None, first_sphere_radius=None):
+    """Walk on spheres with per-point estimation quantities (gradient points included)."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    n = len(pts)
+    dist = np.ascontiguousarray(dist, np.float32)
+    nw = np.ascontiguousarray(np.broadcast_to(np.asarray(n_walks, np.int32), n), np.int32)
+    q = np.ascontiguousarray(np.broadcast_to(np.asarray(1 if quantity is None else quantity, np.uint8), n), np.uint8)
+    d = None if direction is None else np.ascontiguousarray(np.broadcast_to(np.asarray(direction, np.float32),
+                                                                            (n, scene.dim)), np.float32)
+    types = None if types is None else np.ascontiguousarray(types, np.uint8)
+    rep = np.zeros(8, np.int64)
+    st = lib().orc_solve_grad(scene.h, C.byref(pde), C.byref(ws), n, _p(pts), _p(types), _p(dist), _p(nw), _p(q),
+                              _p(d), seed, stream_offset, rng_mode, cv_mode, threads or default_threads(),
+                              _p(stats.n), _p(stats.mean), _p(stats.m2), _p(stats.walk_length), _p(stats.splits),
+                              _p(stats.n_grad), _p(stats.grad_mean), _p(stats.grad_m2), _p(stats.first_source),
+                              _p(stats.derivative), _p(first_sphere_radius), _p(rep))
+    if st != 0:
+        raise RuntimeError(f"oracle gradient solve failed with status {st}")
+    return dict(codes=rep[:4].copy(), total_steps=int(rep[4]))
 
 
 def solve(scene: Scene, pde, ws, pts, dist, n_walks, stats: Stats, seed=0x5EED, stream_offset=0,

@@ -22,6 +22,12 @@
 //   SampleStatistics (FP64 here; float in the reference) point_estimation/common.h:200-395
 //   WalkOnSpheres::estimateSolution / walk / weight window / terminal
 //                                                        point_estimation/walk_on_spheres.h:170-414
+//   WalkOnSpheres::estimateSolutionAndGradient (stratified first step, antithetic pairs,
+//     gradient control variates, cosine-weighted directions)
+//                                                        point_estimation/walk_on_spheres.h:416-593
+//   generateStratifiedSamples, sampleUnitHemisphereCosine, transformCoordinates
+//                                                        core/sampling.h:124-157,185-247,435-458
+//   Green's function / Poisson kernel gradients          core/distributions.h:493-550,600-659,904-994
 //   WalkOnStars::estimateSolution / walk / reflecting + source contributions / throughput
 //                                                        point_estimation/walk_on_stars.h:189-662
 //   tbb::parallel_for over points -> std::thread pool    walk_on_spheres.h:153-166
@@ -70,6 +76,16 @@ struct Pcg32 {
         uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
         uint32_t rot = (uint32_t)(old >> 59u);
         return (xorshifted >> rot) | (xorshifted << ((~rot + 1u) & 31u));
+    }
+    // uniform integer in [0, bound): draws below (2^32 mod bound) are rejected so that every
+    // residue is equally likely (pcg32.h nextUInt(bound) of the pcg32 dependency, absent;
+    // same rule as O'Neill's pcg32_boundedrand_r)
+    uint32_t nextUInt(uint32_t bound) {
+        uint32_t threshold = (~bound + 1u) % bound;
+        for (;;) {
+            uint32_t r = nextUInt();
+            if (r >= threshold) return r % bound;
+        }
     }
     // 23-bit mantissa construction: a float in [1,2) minus 1
     float nextFloat() {
@@ -130,6 +146,76 @@ inline V<3> sampleUnitSphere3(Pcg32& rng) {
 template <int D> V<D> sampleUnitSphere(Pcg32& rng);
 template <> V<2> sampleUnitSphere<2>(Pcg32& rng) { return sampleUnitSphere2(rng); }
 template <> V<3> sampleUnitSphere<3>(Pcg32& rng) { return sampleUnitSphere3(rng); }
+
+// the same maps from given uniforms u[0..DIM-2] (core/sampling.h:104-107,164-170)
+template <int D> V<D> sphereFromU(const float* u);
+template <> V<2> sphereFromU<2>(const float* u) {
+    float phi = (float)(2.0 * M_PI * u[0]);
+    V<2> d; d[0] = std::cos(phi); d[1] = std::sin(phi);
+    return d;
+}
+template <> V<3> sphereFromU<3>(const float* u) {
+    float z = 1.0f - 2.0f * u[0];
+    float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+    float phi = (float)(2.0 * M_PI * u[1]);
+    V<3> d; d[0] = r * std::cos(phi); d[1] = r * std::sin(phi); d[2] = z;
+    return d;
+}
+
+// cosine-weighted hemisphere about +e_{DIM-1} (core/sampling.h:124-129 in 2D; 3D via Shirley's
+// concentric disk map :185-217) and its pdf (:146-149, :232-235)
+template <int D> V<D> hemisphereCosineFromU(const float* u);
+template <> V<2> hemisphereCosineFromU<2>(const float* u) {
+    float u1 = 2.0f * u[0] - 1.0f;
+    V<2> d; d[0] = u1; d[1] = std::sqrt(std::max(0.0f, 1.0f - u1 * u1));
+    return d;
+}
+template <> V<3> hemisphereCosineFromU<3>(const float* u) {
+    float u1 = 2.0f * u[0] - 1.0f, u2 = 2.0f * u[1] - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (!(u1 == 0.0f && u2 == 0.0f)) {
+        float theta, r;
+        if (std::fabs(u1) > std::fabs(u2)) { r = u1; theta = (float)(0.25 * M_PI * (u2 / u1)); }
+        else { r = u2; theta = (float)(0.5 * M_PI * (1.0f - 0.5f * (u1 / u2))); }
+        dx = r * std::cos(theta); dy = r * std::sin(theta);
+    }
+    V<3> d; d[0] = dx; d[1] = dy; d[2] = std::sqrt(std::max(0.0f, 1.0f - (dx * dx + dy * dy)));
+    return d;
+}
+template <int D> float pdfHemisphereCosine(float cosTheta) {
+    return D == 2 ? cosTheta / 2.0f : (float)(cosTheta / M_PI);
+}
+
+// rotate d from the frame whose last axis is n into world coordinates (core/sampling.h:151-157,
+// 237-247; the 3D basis is Duff et al., "Building an Orthonormal Basis, Revisited", JCGT 2017)
+inline void transformCoordinates(const V<2>& n, V<2>& d) {
+    V<2> s; s[0] = n[1]; s[1] = -n[0];
+    d = d[0] * s + d[1] * n;
+}
+inline void transformCoordinates(const V<3>& n, V<3>& d) {
+    float sign = std::copysign(1.0f, n[2]);
+    float a = -1.0f / (sign + n[2]);
+    float b = n[0] * n[1] * a;
+    V<3> b1; b1[0] = 1.0f + sign * n[0] * n[0] * a; b1[1] = sign * b; b1[2] = -sign * n[0];
+    V<3> b2; b2[0] = b; b2[1] = sign + n[1] * n[1] * a; b2[2] = -n[1];
+    d = d[0] * b1 + d[1] * b2 + d[2] * n;
+}
+
+// Latin hypercube samples (core/sampling.h:435-458): nSamples points of `dims` coordinates,
+// row i starts on the diagonal stratum (i + U)/nSamples, then every coordinate column gets its
+// own Fisher-Yates shuffle driven by bounded draws
+inline void stratifiedSamples(int dims, int nSamples, Pcg32& rng, std::vector<float>& s) {
+    const float oneMinusEps = 1.0f - FLT_EPSILON;
+    float inv = 1.0f / (float)nSamples;
+    s.assign((size_t)dims * nSamples, 0.0f);
+    for (int i = 0; i < nSamples; i++)
+        for (int j = 0; j < dims; j++) s[(size_t)dims * i + j] = std::min(((float)i + rng.nextFloat()) * inv, oneMinusEps);
+    for (int i = 0; i < dims; i++)
+        for (int j = 0; j < nSamples; j++) {
+            int other = j + (int)rng.nextUInt((uint32_t)(nSamples - j));
+            std::swap(s[(size_t)dims * j + i], s[(size_t)dims * other + i]);
+        }
+}
 
 // ------------------------------------------------------------------------------------
 // Green's functions of the ball (core/distributions.h:323-665, 835-1005)
@@ -204,6 +290,37 @@ struct GreensBall {
     float reflectance(float r) const {
         if (!yukawa) return 1.0f;
         return poissonKernelDampeningFactor(r);
+    }
+
+    // gradient of G(x, y) in y for the centred ball: (y - c) * gradientNorm(r)
+    // (distributions.h:493-502 harmonic 2D, :600-609 harmonic 3D, :904-922 Yukawa 3D)
+    float gradientNorm(float r) const {
+        if (!yukawa) {
+            if (D == 2) return (1.0f / (r * r) - 1.0f / (R * R)) / (2.0f * PI_F);
+            return (1.0f / (r * r * r) - 1.0f / (R * R * R)) / (4.0f * PI_F);
+        }
+        float mur = r * sqrtLambda;
+        float expmur = std::exp(-mur);
+        float exp2mur = expmur * expmur;
+        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
+        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
+        float K32mur = expmur * (1.0f + 1.0f / mur);
+        float I32mur = coshmur - sinhmur / mur;
+        float Qr = sqrtLambda * (K32mur - I32mur * K32muR / I32muR);
+        return Qr / (4.0f * PI_F * r * r);
+    }
+    V<D> gradient(float r, const V<D>& y) const { return (y - c) * gradientNorm(r); }
+
+    // gradient of the Poisson kernel at y on the sphere (distributions.h:545-550, :654-659, :987-994)
+    V<D> poissonKernelGradient(const V<D>& y) const {
+        V<D> d = y - c;
+        if (!yukawa) {
+            if (D == 2) return 2.0f * d * (1.0f / (2.0f * PI_F * R * R));
+            return 3.0f * d * (1.0f / (4.0f * PI_F * R * R));
+        }
+        float lambda = sqrtLambda * sqrtLambda;
+        float QR = lambda / I32muR;
+        return d * QR * (1.0f / (4.0f * PI_F));
     }
 
     float pdfSampleSphereUniform(float r) const {
@@ -1142,6 +1259,157 @@ void wosEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>& 
     }
 }
 
+// gradient statistics of one point (SampleStatistics gradient members, common.h:226-247,
+// 288-309), FP64
+template <int D>
+struct GradStats {
+    int64_t n = 0;
+    double mean[D] = {}, m2[D] = {};
+    double firstSource = 0.0, derivative = 0.0;
+    void add(const double* g) {
+        n++;
+        for (int i = 0; i < D; i++) {
+            double d = g[i] - mean[i];
+            mean[i] += d / (double)n;
+            m2[i] += d * (g[i] - mean[i]);
+        }
+    }
+};
+
+// control variates of the gradient estimate for the next antithetic pair
+struct GradCv {
+    int mode;                 // 0: the running means before every pair; 1: frozen per stage
+    int64_t prevN; double prevMean, prevFirst;
+    float b = 0.0f, s = 0.0f; // current values
+    int stageEnd = 0;
+    // the solution mean and the mean first-step source term over prevN + o.n estimates
+    void refresh(const Stats& o, double first) {
+        int64_t N = prevN + o.n;
+        double mean = N > 0 ? (prevMean * (double)prevN + o.mean * (double)o.n) / (double)N : 0.0;
+        b = (float)mean;
+        s = (float)((prevFirst + first) / (double)std::max<int64_t>(1, N));
+    }
+    void atPair(int w, const Stats& o, double first) {
+        if (mode == 0) { refresh(o, first); return; }
+        // stages [0,1), [1,2), [2,4), [4,8), ...: the values are refreshed when a stage starts
+        if (w == 0 || w >= stageEnd) { refresh(o, first); stageEnd = w == 0 ? 1 : 2 * w; }
+    }
+};
+
+// estimateSolutionAndGradient (walk_on_spheres.h:416-593).  rng_mode 0 consumes one generator
+// per point in the reference's order (stratified table, then every walk; the antithetic half
+// replays the generator saved after the first step).  rng_mode 1 is the GPU's scheme: the table
+// comes from pcg32(~seed, (p << 32) | stream_offset) and both halves of pair w start from
+// pcg32(seed, (p << 32) | (stream_offset + w)), the second half redrawing the first step before
+// mirroring it (the same numbers, so the same walk continuation as the reference's replay).
+template <int D>
+void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t p, const V<D>& pt, float distA,
+                     int nWalks, const V<D>& dirDeriv, Stats& out, GradStats<D>& g, int64_t codes[4],
+                     QueryCount* qc, float* firstSphereRadius) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    int nHalves = 1;
+    if (ws.use_gradient_antithetic_variates) { nWalks = std::max(1, nWalks / 2); nHalves = 2; }
+    const float R0 = 0.99f * distA;  // ZOMBIE_RADIUS_SHRINK_PERCENTAGE
+    if (firstSphereRadius) *firstSphereRadius = R0;
+    Pcg32 pointRng = makeRng(cx, p, 0);
+    std::vector<float> strat;
+    if (cx.rngMode == 1) { Pcg32 t(~cx.seed, ((uint64_t)p << 32) | (uint64_t)cx.streamOffset); stratifiedSamples(D - 1, 2 * nWalks, t, strat); }
+    else stratifiedSamples(D - 1, 2 * nWalks, pointRng, strat);
+    std::deque<WalkState<D>> q;
+    for (int w = 0; w < nWalks; w++) {
+        float sourceRadius = 0.0f, sourcePdf = 1.0f, boundaryPdf = 1.0f;
+        V<D> sourcePt = V<D>::zero(), boundaryPt = V<D>::zero();
+        Pcg32 savedRng;
+        float cvB = 0.0f, cvS = 0.0f;
+        if (ws.use_gradient_control_variates) { cv.atPair(w, out, g.firstSource); cvB = cv.b; cvS = cv.s; }
+        for (int half = 0; half < nHalves; half++) {
+            Pcg32 pairRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
+            Pcg32& rng = cx.rngMode == 1 ? pairRng : pointRng;
+            // the first step: GPU scheme redraws it in the second half, then mirrors
+            bool draw = half == 0 || cx.rngMode == 1;
+            WalkState<D> st; st.currentPt = pt; st.currentNormal = V<D>::zero(); st.prevDirection = V<D>::zero();
+            GreensBall<D> gf;
+            if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
+            gf.updateBall(st.currentPt, R0);
+            float firstSource = 0.0f;
+            V<D> sourceDir = V<D>::zero();
+            if (!ws.ignore_source_contribution) {
+                float gNorm = gf.norm();
+                V<D> nz = V<D>::zero();
+                if (pde.is_source_constant) {
+                    firstSource = gNorm * evalField<D>(pde.source, st.currentPt, nz, false, pde.absorption_coeff);
+                } else {
+                    if (draw) {
+                        V<D> sd = sphereFromU<D>(&strat[(size_t)(D - 1) * (2 * w + 0)]);
+                        sourcePt = gf.sampleVolumeDir(sd, rng, sourceRadius, sourcePdf);
+                    }
+                    if (half == 1) { V<D> sdv = sourcePt - st.currentPt; sourcePt = st.currentPt - sdv; }
+                    firstSource = gNorm * evalField<D>(pde.source, sourcePt, nz, false, pde.absorption_coeff);
+                    sourceDir = gf.gradient(sourceRadius, sourcePt) * (1.0f / (sourcePdf * gNorm));
+                }
+                st.totalSourceContribution += st.throughput * firstSource;
+            }
+            if (draw) {
+                const float* u = &strat[(size_t)(D - 1) * (2 * w + 1)];
+                V<D> bd;
+                if (ws.use_cosine_sampling_for_derivatives) {
+                    bd = hemisphereCosineFromU<D>(u);
+                    if (rng.nextFloat() < 0.5f) bd[D - 1] *= -1.0f;
+                    boundaryPdf = 0.5f * pdfHemisphereCosine<D>(std::fabs(bd[D - 1]));
+                    transformCoordinates(dirDeriv, bd);
+                } else {
+                    bd = sphereFromU<D>(u);
+                    boundaryPdf = gf.pdfSampleSphereUniform(1.0f);
+                }
+                boundaryPt = gf.c + gf.R * bd;
+            }
+            if (half == 1) { V<D> bdv = boundaryPt - st.currentPt; boundaryPt = st.currentPt - bdv; }
+            st.currentPt = boundaryPt;
+            st.throughput *= gf.poissonKernel() / boundaryPdf;
+            V<D> boundaryDir = gf.poissonKernelGradient(boundaryPt) * (1.0f / (boundaryPdf * st.throughput));
+            if (cx.rngMode == 0) { if (half == 0) savedRng = rng; else rng = savedRng; }
+
+            q.push_back(st);
+            int splits = -1; float total = 0.0f; bool success = false;
+            while (!q.empty()) {
+                WalkState<D> s2 = q.front(); q.pop_front();
+                splits++;
+                if (splits > 0) {
+                    gf = GreensBall<D>();
+                    if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
+                }
+                float dist; V<D> cp; int prim;
+                closestPoint(sc.abs, s2.currentPt, dist, cp, prim, qc);
+                Code code = wosWalk<D>(sc, cx, dist, gf, rng, s2, q, qc);
+                codes[code]++;
+                if (code != Escaped) {
+                    float term = wosTerminal<D>(sc, cx, code, s2, qc);
+                    total += s2.throughput * term + s2.totalSourceContribution;
+                    out.len += s2.walkLength;
+                    success = true;
+                }
+            }
+            if (success) {
+                float boundaryContribution = total - firstSource;
+                double est[D];
+                float dd = 0.0f;
+                for (int i = 0; i < D; i++) {
+                    float bi = (boundaryContribution - cvB) * boundaryDir[i];
+                    float si = (firstSource - cvS) * sourceDir[i];
+                    dd += bi * dirDeriv[i];
+                    dd += si * dirDeriv[i];
+                    est[i] = (double)(bi + si);
+                }
+                out.add((double)total);
+                g.firstSource += firstSource;
+                g.add(est);
+                g.derivative += dd;
+                out.splits += splits;
+            }
+        }
+    }
+}
+
 // ----- Walk on Stars (walk_on_stars.h:189-662), 2D -----
 template <int D>
 void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBall<D>& gf, float starRadius,
@@ -1428,8 +1696,13 @@ static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_s
                      const float* pts, const float* normals, const uint8_t* types, const uint8_t* aligned,
                      const float* distA, const int32_t* nWalks, uint64_t seed, uint32_t streamOffset, int rngMode,
                      int nThreads, int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits,
-                     float* firstSphereRadius, int64_t* report) {
+                     float* firstSphereRadius, int64_t* report, const uint8_t* quantity = nullptr,
+                     const float* dirDeriv = nullptr, int cvMode = 0, int64_t* gN = nullptr, double* gMean = nullptr,
+                     double* gM2 = nullptr, double* gFirst = nullptr, double* gDeriv = nullptr) {
     if (pde->absorption_coeff > 0.0f && D == 2) return ZG_ERR_UNSUPPORTED;
+    bool anyGrad = false;
+    if (quantity) for (int64_t i = 0; i < n; i++) anyGrad = anyGrad || quantity[i] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT;
+    if (anyGrad && (wost || !gN || !gMean || !gM2 || !gFirst || !gDeriv)) return ZG_ERR_INVALID_ARGUMENT;
     SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
     int nt = std::max(1, nThreads);
     std::vector<std::array<int64_t, 8>> per(nt);
@@ -1439,6 +1712,35 @@ static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_s
         V<D> nr = normals ? load<D>(normals + i * D) : V<D>::zero();
         uint8_t ty = types ? types[i] : 0;
         Stats st; int64_t codes[4] = {0, 0, 0, 0}; QueryCount qc;
+        uint8_t quant = quantity ? quantity[i] : (uint8_t)ZG_ESTIMATE_SOLUTION;
+        if (quant == ZG_ESTIMATE_SKIP) return;
+        if (quant == ZG_ESTIMATE_SOLUTION_AND_GRADIENT) {
+            V<D> dd = V<D>::zero(); dd[0] = 1.0f;
+            if (dirDeriv) dd = load<D>(dirDeriv + i * D);
+            GradCv cv{cvMode, sN[i], sMean[i], gFirst[i]};
+            GradStats<D> g;
+            wosEstimateGrad<D>(sc, cx, cv, i, x, distA[i], nWalks[i], dd, st, g, codes, &qc,
+                               firstSphereRadius ? firstSphereRadius + i : nullptr);
+            mergeInto(sN[i], sMean[i], sM2[i], st);
+            if (g.n > 0) {
+                int64_t N1 = gN[i], N2 = g.n, N = N1 + N2;
+                for (int k = 0; k < D; k++) {
+                    double& m = gMean[i * D + k]; double& m2 = gM2[i * D + k];
+                    if (N1 == 0) { m = g.mean[k]; m2 = g.m2[k]; }
+                    else {
+                        double d = g.mean[k] - m;
+                        m += d * ((double)N2 / (double)N);
+                        m2 += g.m2[k] + d * d * ((double)N1 * (double)N2 / (double)N);
+                    }
+                }
+                gN[i] = N;
+            }
+            gFirst[i] += g.firstSource; gDeriv[i] += g.derivative;
+            sLen[i] += st.len; sSplits[i] += st.splits;
+            for (int k = 0; k < 4; k++) per[t][k] += codes[k];
+            per[t][4] += st.len; per[t][5] += qc.nodes; per[t][6] += qc.prims;
+            return;
+        }
         if (wost) wostEstimate<D>(sc, cx, i, x, nr, ty, aligned ? aligned[i] != 0 : false, distA[i], nWalks[i], st, codes, &qc, sN[i], firstSphereRadius ? firstSphereRadius + i : nullptr);
         else wosEstimate<D>(sc, cx, i, x, ty, distA[i], nWalks[i], st, codes, &qc, sN[i], firstSphereRadius ? firstSphereRadius + i : nullptr);
         mergeInto(sN[i], sMean[i], sM2[i], st);
@@ -1613,6 +1915,63 @@ int orc_solve(int wost, void* hv, const zg_pde* pde, const zg_walk_settings* ws,
     SceneHandle* h = (SceneHandle*)hv;
     if (h->dim == 2) return solveImpl<2>(wost != 0, *h->s2, pde, ws, n, pts, normals, types, aligned, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report);
     return solveImpl<3>(wost != 0, *h->s3, pde, ws, n, pts, normals, types, aligned, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report);
+}
+
+// solve with per-point estimation quantities (EstimationQuantity, common.h:145-149) and the
+// gradient statistics; cv_mode 0 refreshes the gradient control variates before every pair
+// (the reference), 1 once per stage [0,1), [1,2), [2,4), ... (the GPU)
+int orc_solve_grad(void* hv, const zg_pde* pde, const zg_walk_settings* ws, int64_t n, const float* pts,
+                   const uint8_t* types, const float* distA, const int32_t* nWalks, const uint8_t* quantity,
+                   const float* dirDeriv, uint64_t seed, uint32_t streamOffset, int rngMode, int cvMode, int nThreads,
+                   int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits, int64_t* gN,
+                   double* gMean, double* gM2, double* gFirst, double* gDeriv, float* firstSphereRadius,
+                   int64_t* report) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) return solveImpl<2>(false, *h->s2, pde, ws, n, pts, nullptr, types, nullptr, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report, quantity, dirDeriv, cvMode, gN, gMean, gM2, gFirst, gDeriv);
+    return solveImpl<3>(false, *h->s3, pde, ws, n, pts, nullptr, types, nullptr, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report, quantity, dirDeriv, cvMode, gN, gMean, gM2, gFirst, gDeriv);
+}
+
+// the per-point Latin hypercube tables of a gradient solve: table t (2 * n_pairs rows of
+// `dims` coordinates) is drawn from pcg32(~seed, (t << 32) | stream_offset)
+void orc_stratified_samples(int dims, int64_t nTables, int nSamples, uint64_t seed, uint32_t streamOffset, float* out) {
+    std::vector<float> s;
+    for (int64_t t = 0; t < nTables; t++) {
+        Pcg32 r(~seed, ((uint64_t)t << 32) | (uint64_t)streamOffset);
+        stratifiedSamples(dims, nSamples, r, s);
+        std::memcpy(out + (size_t)t * dims * nSamples, s.data(), s.size() * sizeof(float));
+    }
+}
+void orc_pcg32_bounded(uint64_t initstate, uint64_t initseq, uint32_t bound, int64_t n, uint32_t* out) {
+    Pcg32 r(initstate, initseq);
+    for (int64_t i = 0; i < n; i++) out[i] = r.nextUInt(bound);
+}
+// Green's function gradient quantities of a centred ball of radius R at distance r along e_0:
+// out = [gradientNorm(r), poissonKernelGradient(R e_0)[0]]
+void orc_greens_ball_gradient(int dim, float R, float lambda, float r, float* out) {
+    if (dim == 2) {
+        GreensBall<2> g; if (lambda > 0) g.makeYukawa(lambda); g.updateBall(V<2>::zero(), R);
+        V<2> y = V<2>::zero(); y[0] = R;
+        out[0] = g.gradientNorm(r); out[1] = g.poissonKernelGradient(y)[0];
+    } else {
+        GreensBall<3> g; if (lambda > 0) g.makeYukawa(lambda); g.updateBall(V<3>::zero(), R);
+        V<3> y = V<3>::zero(); y[0] = R;
+        out[0] = g.gradientNorm(r); out[1] = g.poissonKernelGradient(y)[0];
+    }
+}
+// boundary-direction samplers of the gradient path from given uniforms: mode 0 uniform sphere,
+// 1 cosine hemisphere (+ the sign flip given in u[dims]) rotated to direction n; out = [dir, pdf]
+void orc_gradient_direction(int dim, int mode, const float* u, const float* n, float* out) {
+    if (dim == 2) {
+        V<2> d; float pdf;
+        if (mode == 0) { d = sphereFromU<2>(u); pdf = 1.0f / (2.0f * PI_F); }
+        else { d = hemisphereCosineFromU<2>(u); if (u[1] < 0.5f) d[1] *= -1.0f; pdf = 0.5f * pdfHemisphereCosine<2>(std::fabs(d[1])); transformCoordinates(load<2>(n), d); }
+        store<2>(out, d); out[2] = pdf;
+    } else {
+        V<3> d; float pdf;
+        if (mode == 0) { d = sphereFromU<3>(u); pdf = 1.0f / (4.0f * PI_F); }
+        else { d = hemisphereCosineFromU<3>(u); if (u[2] < 0.5f) d[2] *= -1.0f; pdf = 0.5f * pdfHemisphereCosine<3>(std::fabs(d[2])); transformCoordinates(load<3>(n), d); }
+        store<3>(out, d); out[3] = pdf;
+    }
 }
 
 }  // extern "C"

@@ -32,6 +32,9 @@ ZG_EVAL_SCREENED_SOURCE = 2
 ZG_SAMPLE_IN_DOMAIN = 0
 ZG_SAMPLE_ON_ABSORBING_BOUNDARY = 1
 ZG_SAMPLE_ON_REFLECTING_BOUNDARY = 2
+ZG_ESTIMATE_SOLUTION = 0
+ZG_ESTIMATE_SOLUTION_AND_GRADIENT = 1
+ZG_ESTIMATE_SKIP = 2
 
 ZG_REACHED_ABSORBING_BOUNDARY = 0
 ZG_TERMINATED_WITH_RUSSIAN_ROULETTE = 1
@@ -122,6 +125,8 @@ class zg_sample_points(C.Structure):
         ("cv_gradient_grid", C.POINTER(zg_grid_field)),
         ("n_walks_max", C.c_int32),
         ("reserved", C.c_int32),
+        ("estimation_quantity", C.c_void_p),
+        ("direction_for_derivative", C.c_void_p),
     ]
 
 
@@ -132,6 +137,11 @@ class zg_statistics(C.Structure):
         ("solution_m2", C.c_void_p),
         ("total_walk_length", C.c_void_p),
         ("total_splits", C.c_void_p),
+        ("n_gradient_estimates", C.c_void_p),
+        ("gradient_mean", C.c_void_p),
+        ("gradient_m2", C.c_void_p),
+        ("total_first_source_contribution", C.c_void_p),
+        ("total_derivative_contribution", C.c_void_p),
     ]
 
 
@@ -186,6 +196,7 @@ SIGNATURES = {
     "zg_inside_domain": (C.c_int32, [_P, C.c_int64, _P, _P, _P]),
     "zg_intersect_ray": (C.c_int32, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_star_radius": (C.c_int32, [_P, C.c_int64, _P, _P, _P, C.c_float, _P, _P, _P]),
+    "zg_stratified_samples": (C.c_int32, [C.c_int32, C.c_int64, C.c_int32, C.c_uint64, C.c_uint32, _P, _P]),
     "zg_evaluate_function": (C.c_int32, [C.c_int32, C.POINTER(zg_function), C.c_int64, _P, _P, _P, C.c_float, _P,
                                          _P]),
     "zg_probe_read_bandwidth": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_float), _P]),

@@ -390,6 +390,7 @@ struct SolveSetup {
     int nWalksMax = 0;
     size_t words = 0;
     const float* callerFsr = nullptr;   // walk on stars: caller's recorded first sphere radii
+    zg_statistics stats{};              // caller's statistics (gradient members checked per call)
 };
 
 zg_status checkPde(const zg_pde* pde, int dim) {
@@ -486,12 +487,19 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     M.ignoreAbsorbing = ws->ignore_absorbing_boundary_contribution;
     M.solveDoubleSided = ws->solve_double_sided;
     M.writeFirstSphereRadius = 1;
+    if (pts->estimation_quantity && b)
+        return fail(ZG_ERR_UNSUPPORTED, "estimation_quantity is supported by zg_wos_solve and zg_wost_solve only");
+    P.quantity = pts->estimation_quantity; M.quantity = pts->estimation_quantity;
+    M.want = ZG_ESTIMATE_SOLUTION;
+    P.dirDeriv = pts->direction_for_derivative;
+    S.stats = *stats;
 
     S.dim = dim; S.width = s ? s->width : b->width; S.depth = s ? s->info.absorbing_depth : b->maxDepth;
     S.batch = b != nullptr; S.count = o.count_visits != 0; S.n = n; S.nWalksMax = pts->n_walks_max;
     if (s) S.screened = pde->absorption_coeff > 0.0f;
     else for (const zg_pde& ip : b->pdes) S.screened = S.screened || ip.absorption_coeff > 0.0f;
     S.words = (size_t)n * 4 + kCntSlots + (wost ? (size_t)(n + 1) / 2 : 0) + (split ? (size_t)n : 0);
+    if (P.quantity) S.words += (size_t)(n + 1) / 2;   // masked walk counts, last in the workspace
     S.wost = wost;
     S.split = split;
     M.sSplits = stats->total_splits;
@@ -529,11 +537,19 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     bindWorkspace(S, wsp);
     hipError_t e = launchFillZero(wsp, (int64_t)S.words, strm);
     if (e == hipSuccess && ev0) e = hipEventRecord(ev0, strm);
-    if (e == hipSuccess && S.wost) e = launchWostFirstSphere(S.P, S.width, S.callerFsr, const_cast<float*>(S.P.fsr), strm);
+    WosParams P = S.P;
+    if (e == hipSuccess && P.quantity) {
+        // the walk kernels see walk counts with the non-solution points zeroed
+        int32_t* masked = reinterpret_cast<int32_t*>(wsp + S.words - (S.n + 1) / 2);
+        e = launchMaskWalks(P.nWalks, P.quantity, S.n, masked, strm);
+        P.nWalks = masked;
+        P.quantity = nullptr;
+    }
+    if (e == hipSuccess && S.wost) e = launchWostFirstSphere(P, S.width, S.callerFsr, const_cast<float*>(P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
-        e = S.wost ? launchWost(S.P, S.width, S.count, S.blocks, strm)
-                   : S.batch ? launchWosBatch(S.P, S.width, S.count, S.screened, S.split, S.blocks, strm)
-                             : launchWos(S.P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
+        e = S.wost ? launchWost(P, S.width, S.count, S.blocks, strm)
+                   : S.batch ? launchWosBatch(P, S.width, S.count, S.screened, S.split, S.blocks, strm)
+                             : launchWos(P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
     if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
     if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
     return e;
@@ -557,18 +573,119 @@ void fillReport(const unsigned long long* cnt, float ms, zg_solve_report* report
     report->reserved = 0.0f;
 }
 
+// gradient layout after the solve workspace: sums and squares of the gradient estimates
+// (n*dim each), first-step source and derivative totals (n each), then the stratified tables
+struct GradLayout {
+    int halves = 1, pairsMax = 0;
+    int64_t stride = 0;     // floats per stratified table
+    size_t words = 0;
+};
+
+GradLayout gradLayout(const SolveSetup& S) {
+    GradLayout L;
+    L.halves = S.P.ws.use_gradient_antithetic_variates ? 2 : 1;
+    L.pairsMax = L.halves == 2 ? std::max(1, S.nWalksMax / 2) : S.nWalksMax;
+    L.stride = (int64_t)(S.dim - 1) * 2 * L.pairsMax;
+    L.words = (size_t)S.n * (2 * S.dim + 2) + ((size_t)S.n * (size_t)L.stride + 1) / 2;
+    return L;
+}
+
+// estimateSolutionAndGradient for the points asking for it: stratified tables, then stages of
+// antithetic pairs [0,1), [1,2), [2,4), ... (one stage without control variates); every stage
+// reads the control variates from the statistics the previous stage merged
+hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipStream_t strm) {
+    const int64_t n = S.n;
+    const int D = S.dim;
+    GradLayout L = gradLayout(S);
+    uint64_t* g = wsp + S.words;
+    WosParams P = S.P;
+    MergeParams M = S.M;
+    P.cvGrad = nullptr; P.cvGrid = nullptr;
+    P.accG = reinterpret_cast<double*>(g);
+    P.accG2 = reinterpret_cast<double*>(g + (size_t)n * D);
+    P.accFirst = reinterpret_cast<double*>(g + (size_t)n * 2 * D);
+    P.accDeriv = reinterpret_cast<double*>(g + (size_t)n * (2 * D + 1));
+    P.strat = reinterpret_cast<const float*>(g + (size_t)n * (2 * D + 2));
+    P.stratStride = L.stride;
+    P.gradHalves = L.halves;
+    P.cvMean = S.stats.solution_mean; P.cvN = S.stats.n_solution_estimates;
+    P.cvFirst = S.stats.total_first_source_contribution;
+    M.want = ZG_ESTIMATE_SOLUTION_AND_GRADIENT;
+    M.accG = P.accG; M.accG2 = P.accG2; M.accFirst = P.accFirst; M.accDeriv = P.accDeriv;
+    M.gN = S.stats.n_gradient_estimates; M.gMean = S.stats.gradient_mean; M.gM2 = S.stats.gradient_m2;
+    M.gFirst = S.stats.total_first_source_contribution; M.gDeriv = S.stats.total_derivative_contribution;
+    M.streamOffsetDev = nullptr;
+    hipError_t e = hipSuccess;
+    if (fillCounters) e = launchFillZero(reinterpret_cast<uint64_t*>(P.counters), kCntSlots, strm);
+    if (e == hipSuccess)
+        e = launchStratified(D - 1, const_cast<float*>(P.strat), n, L.stride, 2 * L.pairsMax, P.nWalks, P.quantity,
+                             L.halves, P.seed, P.streamOffset, strm);
+    const bool cv = S.P.ws.use_gradient_control_variates != 0;
+    for (int u0 = 0; e == hipSuccess && u0 < L.pairsMax;) {
+        int u1 = cv ? (u0 == 0 ? 1 : std::min(2 * u0, L.pairsMax)) : L.pairsMax;
+        e = launchFillZero(wsp, n * 4, strm);   // solution sums, counts, lengths
+        if (e == hipSuccess) e = launchFillZero(reinterpret_cast<uint64_t*>(P.claimCounter), 1, strm);
+        if (e == hipSuccess) e = launchFillZero(g, n * (2 * D + 2), strm);
+        if (e == hipSuccess && S.split) e = launchFillZero(reinterpret_cast<uint64_t*>(P.accSplits), n, strm);
+        P.gradW0 = u0 * L.halves;
+        P.gradW1 = u1 * L.halves;
+        P.chunksPerPoint = std::max(1, (P.gradW1 - P.gradW0 + P.wpl - 1) / P.wpl);
+        P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
+        if (e == hipSuccess) e = launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
+        if (e == hipSuccess) e = launchMergeStats(M, D, strm);
+        u0 = u1;
+    }
+    return e;
+}
+
 // eager solve: stream-ordered workspace, launch, counters back, synchronise
 zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
     if (S.n == 0) { if (report) std::memset(report, 0, sizeof(*report)); return ok(); }
+    // estimation quantities: which launches the call needs
+    int64_t nSolve = S.n, nGrad = 0;
+    if (S.P.quantity) {
+        std::vector<uint8_t> hq((size_t)S.n);
+        ZG_HIP(hipMemcpyAsync(hq.data(), S.P.quantity, (size_t)S.n, hipMemcpyDeviceToHost, strm));
+        ZG_HIP(hipStreamSynchronize(strm));
+        nSolve = 0;
+        for (uint8_t q : hq) {
+            if (q > ZG_ESTIMATE_SKIP) return fail(ZG_ERR_INVALID_ARGUMENT, "estimation_quantity value %d is invalid", (int)q);
+            nSolve += q == ZG_ESTIMATE_SOLUTION;
+            nGrad += q == ZG_ESTIMATE_SOLUTION_AND_GRADIENT;
+        }
+    }
+    size_t words = S.words;
+    if (nGrad > 0) {
+        if (S.wost) return fail(ZG_ERR_UNSUPPORTED, "gradient estimation runs in walk on spheres only");
+        const zg_statistics& st = S.stats;
+        if (!st.n_gradient_estimates || !st.gradient_mean || !st.gradient_m2 || !st.total_first_source_contribution ||
+            !st.total_derivative_contribution)
+            return fail(ZG_ERR_INVALID_ARGUMENT, "gradient points need the gradient members of zg_statistics");
+        words += gradLayout(S).words;
+    }
     uint64_t* wsp = nullptr;
-    ZG_HIP(hipMallocAsync((void**)&wsp, S.words * 8, strm));
+    ZG_HIP(hipMallocAsync((void**)&wsp, words * 8, strm));
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipError_t e = hipSuccess;
     if (report) {
         e = hipEventCreate(&ev0);
         if (e == hipSuccess) e = hipEventCreate(&ev1);
     }
-    if (e == hipSuccess) e = enqueueSolve(S, wsp, strm, ev0, ev1);
+    if (e == hipSuccess && nSolve > 0) e = enqueueSolve(S, wsp, strm, ev0, nGrad > 0 ? nullptr : ev1);
+    if (e == hipSuccess && nGrad > 0) {
+        if (nSolve == 0) {
+            bindWorkspace(S, wsp);
+            if (ev0) e = hipEventRecord(ev0, strm);
+        }
+        if (e == hipSuccess) e = enqueueGradient(S, wsp, nSolve == 0, strm);
+        if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
+    }
+    if (e == hipSuccess && nSolve == 0 && nGrad == 0) {   // every point skipped
+        bindWorkspace(S, wsp);
+        e = launchFillZero(reinterpret_cast<uint64_t*>(S.P.counters), kCntSlots, strm);
+        if (e == hipSuccess && ev0) e = hipEventRecord(ev0, strm);
+        if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
+    }
     unsigned long long cnt[kCntSlots] = {0};
     if (e == hipSuccess) e = hipMemcpyAsync(cnt, S.P.counters, sizeof(cnt), hipMemcpyDeviceToHost, strm);
     if (e == hipSuccess) e = hipStreamSynchronize(strm);
@@ -596,6 +713,18 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     zg_status st = prepareSolve(s, nullptr, pde, ws, pts, nullptr, stats, opts, S);
     if (st != ZG_OK) return st;
     return runSolve(S, report, (hipStream_t)stream);
+}
+
+zg_status zg_stratified_samples(int32_t dims, int64_t nTables, int32_t nSamples, uint64_t seed, uint32_t streamOffset,
+                                float* out, void* stream) {
+    if (dims < 1 || dims > 2) return fail(ZG_ERR_INVALID_ARGUMENT, "dims must be 1 or 2");
+    if (nTables < 0 || nSamples < 1) return fail(ZG_ERR_INVALID_ARGUMENT, "n_tables >= 0 and n_samples >= 1 required");
+    if (nTables >= (1ll << 31)) return fail(ZG_ERR_INVALID_ARGUMENT, "at most 2^31-1 tables");
+    if (nTables > 0 && !out) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL output");
+    hipError_t e = launchStratified(dims, out, nTables, (int64_t)dims * nSamples, nSamples, nullptr, nullptr, 1, seed,
+                                    streamOffset, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "zg_stratified_samples: %s", hipGetErrorString(e));
+    return ok();
 }
 
 zg_status zg_batch_create(int32_t nInstances, const zg_scene* const* scenes, const zg_pde* pdes, zg_batch** out) {

@@ -11,6 +11,9 @@ namespace zg {
 
 #define ZG_DEV __device__ __forceinline__
 
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kPiF = kPi;
+
 // ------------------------------------------------------------------------------------
 // small vectors
 // ------------------------------------------------------------------------------------
@@ -54,6 +57,15 @@ struct Pcg32 {
         next();
     }
     ZG_DEV float nextFloat() { return __uint_as_float((next() >> 9) | 0x3f800000u) - 1.0f; }
+    // uniform integer in [0, bound): draws below 2^32 mod bound are rejected (every residue
+    // equally likely; pcg32 nextUInt(bound))
+    ZG_DEV uint32_t nextBounded(uint32_t bound) {
+        uint32_t threshold = (0u - bound) % bound;
+        for (;;) {
+            uint32_t r = next();
+            if (r >= threshold) return r % bound;
+        }
+    }
 };
 
 // ------------------------------------------------------------------------------------
@@ -75,10 +87,62 @@ template <> ZG_DEV Vec<3> sampleUnitSphere<3>(Pcg32& rng) {
     return d;
 }
 
+// the same maps from given uniforms (the stratified directions of the gradient path)
+template <int D> ZG_DEV Vec<D> sphereFromU(const float* u);
+template <> ZG_DEV Vec<2> sphereFromU<2>(const float* u) {
+    Vec<2> d; d[0] = __builtin_amdgcn_cosf(u[0]); d[1] = __builtin_amdgcn_sinf(u[0]);
+    return d;
+}
+template <> ZG_DEV Vec<3> sphereFromU<3>(const float* u) {
+    float z = 1.0f - 2.0f * u[0];
+    float r = __builtin_sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    Vec<3> d; d[0] = r * __builtin_amdgcn_cosf(u[1]); d[1] = r * __builtin_amdgcn_sinf(u[1]); d[2] = z;
+    return d;
+}
+
+// cosine-weighted hemisphere about +e_{DIM-1} (core/sampling.h:124-129; 3D through Shirley's
+// concentric disk map :185-217) and its density per solid angle (:146-149, :232-235)
+template <int D> ZG_DEV Vec<D> hemisphereCosineFromU(const float* u);
+template <> ZG_DEV Vec<2> hemisphereCosineFromU<2>(const float* u) {
+    float u1 = 2.0f * u[0] - 1.0f;
+    Vec<2> d; d[0] = u1; d[1] = __builtin_sqrtf(fmaxf(0.0f, 1.0f - u1 * u1));
+    return d;
+}
+template <> ZG_DEV Vec<3> hemisphereCosineFromU<3>(const float* u) {
+    float u1 = 2.0f * u[0] - 1.0f, u2 = 2.0f * u[1] - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (!(u1 == 0.0f && u2 == 0.0f)) {
+        // angle in revolutions: pi/4 (u2/u1) or pi/2 (1 - u1/(2 u2))
+        float rev, r;
+        if (fabsf(u1) > fabsf(u2)) { r = u1; rev = 0.125f * (u2 / u1); }
+        else { r = u2; rev = 0.25f * (1.0f - 0.5f * (u1 / u2)); }
+        dx = r * __builtin_amdgcn_cosf(rev); dy = r * __builtin_amdgcn_sinf(rev);
+    }
+    Vec<3> d; d[0] = dx; d[1] = dy; d[2] = __builtin_sqrtf(fmaxf(0.0f, 1.0f - (dx * dx + dy * dy)));
+    return d;
+}
+template <int D> ZG_DEV float pdfHemisphereCosine(float cosTheta) { return D == 2 ? 0.5f * cosTheta : cosTheta * (1.0f / kPiF); }
+
+// rotate d from the frame whose last axis is n to world coordinates (core/sampling.h:151-157;
+// the 3D basis of :237-247 is Duff et al., "Building an Orthonormal Basis, Revisited", 2017)
+ZG_DEV Vec<2> fromFrame(const Vec<2>& n, const Vec<2>& d) {
+    Vec<2> r; r[0] = d[0] * n[1] + d[1] * n[0]; r[1] = -d[0] * n[0] + d[1] * n[1];
+    return r;
+}
+ZG_DEV Vec<3> fromFrame(const Vec<3>& n, const Vec<3>& d) {
+    float sign = copysignf(1.0f, n[2]);
+    float a = -1.0f / (sign + n[2]);
+    float b = n[0] * n[1] * a;
+    Vec<3> r;
+    r[0] = d[0] * (1.0f + sign * n[0] * n[0] * a) + d[1] * b + d[2] * n[0];
+    r[1] = d[0] * (sign * b) + d[1] * (sign + n[1] * n[1] * a) + d[2] * n[1];
+    r[2] = d[0] * (-sign * n[0]) + d[1] * (-n[1]) + d[2] * n[2];
+    return r;
+}
+
 // ------------------------------------------------------------------------------------
 // Green's function of the ball (core/distributions.h:453-665 harmonic, 835-1005 Yukawa 3D)
 // ------------------------------------------------------------------------------------
-constexpr float kPi = 3.14159265358979323846f;
 
 template <int D>
 struct Ball {
@@ -123,8 +187,44 @@ struct Ball {
     }
     ZG_DEV float pdfSphere(float r) const { return D == 2 ? 1.0f / (2.0f * kPi * r) : 1.0f / (4.0f * kPi * r * r); }
 
+    // gradient of G(c, y) in the centre: (y - c) gradientNorm(r) (distributions.h:493-502,
+    // 600-609, 904-922)
+    ZG_DEV float gradientNorm(float r) const {
+        if (!yukawa) {
+            if (D == 2) return (1.0f / (r * r) - 1.0f / (R * R)) / (2.0f * kPi);
+            return (1.0f / (r * r * r) - 1.0f / (R * R * R)) / (4.0f * kPi);
+        }
+        float mur = r * sqrtLambda;
+        float expmur = __expf(-mur);
+        float exp2mur = expmur * expmur;
+        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
+        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
+        float K32mur = expmur * (1.0f + 1.0f / mur);
+        float I32mur = coshmur - sinhmur / mur;
+        float Qr = sqrtLambda * (K32mur - I32mur * K32R() / I32R());
+        return Qr / (4.0f * kPi * r * r);
+    }
+    ZG_DEV float K32R() const { return expmuR * (1.0f + 1.0f / muR); }
+    ZG_DEV float I32R() const {
+        float coshmuR = (1.0f + expmuR * expmuR) / (2.0f * expmuR);
+        return coshmuR - sinhmuR / muR;
+    }
+    // Poisson kernel of the centred ball per unit solid angle, and its gradient in the centre
+    // at y on the sphere (distributions.h:512-515,545-550, 628-631,654-659, 934-936,987-994)
+    ZG_DEV float poissonKernel() const {
+        if (!yukawa) return D == 2 ? 1.0f / (2.0f * kPi) : 1.0f / (4.0f * kPi);
+        return muR / (4.0f * kPi * sinhmuR);
+    }
+    ZG_DEV float poissonKernelGradientScale() const {  // grad P = (y - c) * scale
+        if (!yukawa) return D == 2 ? 2.0f / (2.0f * kPi * R * R) : 3.0f / (4.0f * kPi * R * R);
+        return sqrtLambda * sqrtLambda / I32R() / (4.0f * kPi);
+    }
+
     // sampleVolume(dir, rng, r, pdf) radius part; returns r (distributions.h:460-470,565-577,855-864,419-440)
-    ZG_DEV float sampleRadius(Pcg32& rng) const {
+    ZG_DEV float sampleRadius(Pcg32& rng) const { float pdf; return sampleRadiusPdf(rng, pdf); }
+    // ... and the density of the sampled point (evaluate(r) / norm; before the clamp of r for
+    // the Yukawa rejection sampler, after it for the harmonic ones, as the reference computes it)
+    ZG_DEV float sampleRadiusPdf(Pcg32& rng, float& pdfOut) const {
         float r;
         if (!yukawa) {
             if (D == 2) {
@@ -148,14 +248,15 @@ struct Ball {
             do {
                 float u = rng.nextFloat();
                 r = rng.nextFloat() * R;
-                float pdf = evaluate(r) / fnNorm;
-                float pdfRadius = pdf / pdfSphere(r);
+                pdfOut = evaluate(r) / fnNorm;
+                float pdfRadius = pdfOut / pdfSphere(r);
                 iter++;
                 if (u < pdfRadius / bound) break;
             } while (iter < 1000);
         }
         r = fmaxf(1e-4f, r);
         if (r > R) r = R * 0.5f;
+        if (!yukawa) pdfOut = evaluate(r) / norm();
         return r;
     }
 };

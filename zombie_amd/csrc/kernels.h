@@ -62,6 +62,22 @@ struct WosParams {
     const float* fsr;                 // walk on stars: first sphere radius per point
     const int64_t* prevN;             // walk on stars: caller's estimate counts (first call when 0)
     unsigned long long* accSplits;    // walk splitting: split parts per point
+    // per-point estimation quantity (null: all ZG_ESTIMATE_SOLUTION).  Gradient launches walk the
+    // points asking for gradients; solution launches get walk counts with the others zeroed.
+    const uint8_t* quantity;
+    // gradient launches (estimateSolutionAndGradient)
+    const float* dirDeriv;            // direction of the directional derivative (null: e_0)
+    const float* strat;               // per-point Latin hypercube tables
+    int64_t stratStride;              // floats per table
+    int32_t gradW0, gradW1;           // walk indices [gradW0, gradW1) of this stage
+    int32_t gradHalves;               // 2 with antithetic pairs (walk w is half w & 1 of pair w >> 1)
+    const double* cvMean;             // caller's solution means (boundary-term control variate)
+    const int64_t* cvN;               // caller's solution counts
+    const double* cvFirst;            // caller's first-step source totals (source-term control variate)
+    double* accG;                     // [n*dim] sums of the gradient estimates
+    double* accG2;                    // [n*dim] sums of their squares
+    double* accFirst;                 // [n] first-step source terms
+    double* accDeriv;                 // [n] directional derivative contributions
 };
 
 // capacity of the per-lane queue of split walk parts
@@ -95,9 +111,22 @@ struct MergeParams {
     const float* fsrSrc;              // walk on stars: first sphere radius to record (else dist0)
     const unsigned long long* accSplits;
     int64_t* sSplits;                 // caller's total split counts (nullable)
+    const uint8_t* quantity;          // per-point estimation quantity (null: all solution)
+    int32_t want;                     // quantity of the points this merge folds in
+    // gradient statistics (want == ZG_ESTIMATE_SOLUTION_AND_GRADIENT)
+    const double* accG;
+    const double* accG2;
+    const double* accFirst;
+    const double* accDeriv;
+    int64_t* gN;
+    double* gMean;
+    double* gM2;
+    double* gFirst;
+    double* gDeriv;
 };
 
 hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s);
+hipError_t launchMaskWalks(const int32_t* nWalks, const uint8_t* quantity, int64_t n, int32_t* out, hipStream_t s);
 hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, const float* pts, float* dist,
                               float* closest, float* normal, int32_t* prim, hipStream_t s);
 hipError_t launchInsideDomain(const SceneView& sv, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);
@@ -108,6 +137,12 @@ hipError_t launchWos(const WosParams& P, int dim, int width, bool count, bool sc
 hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screened, bool split, int gridBlocks,
                           hipStream_t s);
 hipError_t launchMergeStats(const MergeParams& M, int dim, hipStream_t s);
+// gradient solves: walk launch of one stage, and the per-point stratified tables
+hipError_t launchWosGrad(const WosParams& P, int dim, int width, bool screened, bool split, int gridBlocks,
+                         hipStream_t s);
+hipError_t launchStratified(int dims, float* out, int64_t nTables, int64_t stride, int32_t nSamples,
+                            const int32_t* nWalks, const uint8_t* quantity, int32_t halves, uint64_t seed,
+                            uint32_t streamOffset, hipStream_t s);
 hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const float* pts, const float* normals,
                               const uint8_t* aligned, float absorption, float* out, hipStream_t s);
 // read-bandwidth probe

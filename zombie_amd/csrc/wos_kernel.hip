@@ -155,7 +155,16 @@ ZG_DEV Vec<D> cvField(const WosParams& P, const Vec<D>& x) {
 // state (throughput = threshold, length 0, no source sum) in a per-lane private queue; the
 // estimate of the original walk is the sum over its parts.  When the queue cannot take the
 // copies the walk is not split (an unsplit walk is just as unbiased).
-template <int D, int W, bool COUNT, bool BATCH, bool SCREENED, bool SPLIT = false>
+// GRAD = true runs estimateSolutionAndGradient (walk_on_spheres.h:416-593) for the points asking
+// for it: walk w of a point is half (w & 1) of antithetic pair w >> 1 (or pair w without
+// antithetic variates).  The first step is explicit: a ball of radius 0.99 * d(x0), a source
+// sample and a boundary point from the point's stratified table (directions) and the pair's
+// generator pcg32(seed, (p << 32) | (stream_offset + pair)) (source radius, cosine-sampling
+// sign), mirrored through x0 for the second half, which therefore continues with exactly the
+// random numbers of the first (the reference replays its saved generator).  The walk then goes
+// on from the boundary point as a plain walk; its estimate feeds the solution statistics and the
+// gradient estimate (boundary term - cv) * grad P / P + (first source - cv) * grad G / (pdf |G|).
+template <int D, int W, bool COUNT, bool BATCH, bool SCREENED, bool SPLIT = false, bool GRAD = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WosOccupancy<W>::kWaves, 8)))
 wosKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
@@ -189,6 +198,9 @@ wosKernel(WosParams P) {
     int nq = 0, nSplits = 0;
     float partSum = 0.0f;
     bool partOk = false;
+    // gradient walks: boundary / source gradient directions, first-step source term, control variates
+    Vec<D> gB = vzero<D>(), gS = vzero<D>();
+    float fsc = 0.0f, cvB = 0.0f, cvS = 0.0f;
     ClosestQuery<D> q;
     q.begin(FLT_MAX, 1.0f, 0.0f);
     // block counters in LDS: walks per completion code, steps, walks started
@@ -205,7 +217,94 @@ wosKernel(WosParams P) {
         mode = 2;
     };
 
+    // first step of a gradient walk (walk_on_spheres.h:453-531); ends with the exact distance
+    // query at the boundary point
+    auto startGradWalk = [&]() {
+        const int pair = P.gradHalves == 2 ? (w >> 1) : w;
+        const bool mirror = P.gradHalves == 2 && (w & 1);
+        const Vec<D> c0 = vload<D>(P.pt + (size_t)p * D);
+        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
+        T = 1.0f; S = 0.0f; len = 0;
+        ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
+        ball.update(0.99f * P.dist0[p]);
+        const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + (size_t)(D - 1) * 2 * (size_t)pair;
+        fsc = 0.0f; gS = vzero<D>();
+        if (!ws.ignore_source_contribution) {
+            float gNorm = ball.norm();
+            if (pde().is_source_constant) {
+                fsc = gNorm * evalField<D>(pde().source, c0, vzero<D>(), false, sigma);
+            } else {
+                float pdf;
+                Vec<D> sd = sphereFromU<D>(tab);
+                float r = ball.sampleRadiusPdf(rng, pdf);
+                Vec<D> ys = c0 + sd * r;
+                if (mirror) ys = c0 - (ys - c0);
+                fsc = gNorm * evalField<D>(pde().source, ys, vzero<D>(), false, sigma);
+                gS = (ys - c0) * (ball.gradientNorm(r) / (pdf * gNorm));
+            }
+            S += fsc;
+        }
+        Vec<D> bd;
+        float bpdf;
+        if (ws.use_cosine_sampling_for_derivatives) {
+            bd = hemisphereCosineFromU<D>(tab + (D - 1));
+            if (rng.nextFloat() < 0.5f) bd[D - 1] = -bd[D - 1];
+            bpdf = 0.5f * pdfHemisphereCosine<D>(fabsf(bd[D - 1]));
+            Vec<D> nd = vzero<D>();
+            nd[0] = 1.0f;
+            if (P.dirDeriv) nd = vload<D>(P.dirDeriv + (size_t)p * D);
+            bd = fromFrame(nd, bd);
+        } else {
+            bd = sphereFromU<D>(tab + (D - 1));
+            bpdf = D == 2 ? 1.0f / (2.0f * kPi) : 1.0f / (4.0f * kPi);
+        }
+        Vec<D> yb = c0 + bd * ball.R;
+        if (mirror) yb = c0 - (yb - c0);
+        x = yb;
+        T *= ball.poissonKernel() / bpdf;
+        gB = (yb - c0) * (ball.poissonKernelGradientScale() / (bpdf * T));
+        cvB = 0.0f; cvS = 0.0f;
+        if (ws.use_gradient_control_variates) {
+            int64_t N = P.cvN[p];
+            cvB = (float)P.cvMean[p];
+            cvS = (float)(P.cvFirst[p] / (double)(N > 1 ? N : 1));
+        }
+        ycValid = false; ycPrim = -1;
+        if (SPLIT) { nq = 0; nSplits = 0; partSum = 0.0f; partOk = false; }
+        atomicAdd(&blk[5], 1ull);
+        beginQuery(FLT_MAX, 1.0f, -1);
+    };
+
+    // FP64 per-point accumulation of one walk's estimate (the sum over its parts when split)
+    auto accumulate = [&](float total) {
+        double c = (double)total;
+        atomicAdd(&P.accSum[p], c);
+        atomicAdd(&P.accSq[p], c * c);
+        atomicAdd(&P.accN[p], 1ull);
+        if (GRAD) {
+            // gradient estimate and directional derivative (walk_on_spheres.h:566-586)
+            float bc = total - fsc;
+            Vec<D> nd = vzero<D>();
+            nd[0] = 1.0f;
+            if (P.dirDeriv) nd = vload<D>(P.dirDeriv + (size_t)p * D);
+            float dd = 0.0f;
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                float bi = (bc - cvB) * gB[i];
+                float si = (fsc - cvS) * gS[i];
+                dd += bi * nd[i];
+                dd += si * nd[i];
+                double e = (double)(bi + si);
+                atomicAdd(&P.accG[(size_t)p * D + i], e);
+                atomicAdd(&P.accG2[(size_t)p * D + i], e * e);
+            }
+            atomicAdd(&P.accFirst[p], (double)fsc);
+            atomicAdd(&P.accDeriv[p], (double)dd);
+        }
+    };
+
     auto startWalk = [&]() {
+        if (GRAD) { startGradWalk(); return; }
         if (BATCH) {
             inst = P.instances + P.instanceOf[p];
             view.nodes = inst->nodes; view.prims = inst->prims; view.nNodes = inst->nNodes;
@@ -243,11 +342,7 @@ wosKernel(WosParams P) {
                 partSum += T * g + S;
                 partOk = true;
             } else {
-                // FP64 per-point accumulation of the walk estimate
-                double c = (double)(T * g + S);
-                atomicAdd(&P.accSum[p], c);
-                atomicAdd(&P.accSq[p], c * c);
-                atomicAdd(&P.accN[p], 1ull);
+                accumulate(T * g + S);
             }
             atomicAdd(&P.accLen[p], (unsigned long long)len);
             atomicAdd(&blk[4], (unsigned long long)len);
@@ -267,10 +362,7 @@ wosKernel(WosParams P) {
                 return;
             }
             if (partOk) {
-                double c = (double)partSum;
-                atomicAdd(&P.accSum[p], c);
-                atomicAdd(&P.accSq[p], c * c);
-                atomicAdd(&P.accN[p], 1ull);
+                accumulate(partSum);
                 if (nSplits) atomicAdd(&P.accSplits[p], (unsigned long long)nSplits);
             }
         }
@@ -298,14 +390,21 @@ wosKernel(WosParams P) {
                         p = (int)(c / (unsigned long long)P.chunksPerPoint);
                         int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
                         int nw = P.nWalks[p];
-                        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
-                        if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
-                        else if (P.dist0[p] <= eps) nw = min(nw, 1);
+                        if (GRAD) {
+                            // walks of the stage: [gradW0, min(gradW1, halves * pairs))
+                            int pairs = P.gradHalves == 2 ? max(1, nw / 2) : nw;
+                            nw = P.quantity[p] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT ? min(pairs * P.gradHalves, P.gradW1) : 0;
+                            w = P.gradW0 + k * P.wpl;
+                        } else {
+                            uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+                            if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
+                            else if (P.dist0[p] <= eps) nw = min(nw, 1);
+                            w = k * P.wpl;
+                        }
                         if (BATCH && (uint32_t)P.instanceOf[p] >= (uint32_t)P.nInstances) {
                             nw = 0;  // bad instance index: no walks, error flag for the host
                             if (k == 0) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 2u);
                         }
-                        w = k * P.wpl;
                         wEnd = min(w + P.wpl, nw);
                         if (w < wEnd) mode = 3;
                     }
@@ -414,12 +513,37 @@ __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
     if (i == 0 && M.streamOffsetDev) *M.streamOffsetDev += M.streamAdvance;  // next replay: fresh streams
     if (i >= M.n) return;
     if (M.instances && (uint32_t)M.instanceOf[i] >= (uint32_t)M.nInstances) return;  // flagged by the walk kernel
+    if (M.quantity && M.quantity[i] != (uint8_t)M.want) return;
     int64_t N1 = M.sN[i];
     bool hasPrev = N1 > 0;
     uint8_t ty = M.type ? M.type[i] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
     double sum = M.accSum[i], sq = M.accSq[i];
     int64_t nb = (int64_t)M.accN[i];
-    if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
+    if (M.want == ZG_ESTIMATE_SOLUTION_AND_GRADIENT) {
+        // gradient points: first sphere 0.99 d (walk_on_spheres.h:430), no boundary shortcut;
+        // gradient components merged like the solution (common.h:288-309)
+        if (M.firstSphereRadius) M.firstSphereRadius[i] = 0.99f * M.dist0[i];
+        if (nb > 0) {
+            int64_t G1 = M.gN[i];
+            double N = (double)(G1 + nb);
+            for (int k = 0; k < D; k++) {
+                double gs = M.accG[(size_t)i * D + k], gq = M.accG2[(size_t)i * D + k];
+                double meanB = gs / (double)nb;
+                double m2B = fmax(0.0, gq - gs * meanB);
+                double& gm = M.gMean[(size_t)i * D + k];
+                double& g2 = M.gM2[(size_t)i * D + k];
+                if (G1 == 0) { gm = meanB; g2 = m2B; }
+                else {
+                    double delta = meanB - gm;
+                    gm = gm + delta * ((double)nb / N);
+                    g2 = g2 + m2B + delta * delta * ((double)G1 * (double)nb / N);
+                }
+            }
+            M.gN[i] = G1 + nb;
+        }
+        M.gFirst[i] += M.accFirst[i];
+        M.gDeriv[i] += M.accDeriv[i];
+    } else if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
         if (!hasPrev) {
             float g = 0.0f;
             if (!M.ignoreAbsorbing) {
@@ -464,6 +588,50 @@ __global__ void __launch_bounds__(256) evalFunctionKernel(zg_function f, int64_t
     out[i] = evalField<D>(f, x, nr, aligned && aligned[i], absorption);
 }
 
+// Latin hypercube tables (generateStratifiedSamples<DIMS>, core/sampling.h:435-458), one lane per
+// table: table t is drawn from pcg32(~seed, (t << 32) | streamOffset).  With nWalks given, table
+// p holds 2 * pairs(p) rows (at most nSamples, the rows a table has room for) for the gradient
+// points only; otherwise every table has nSamples rows.
+template <int DIMS>
+__global__ void __launch_bounds__(256) stratifiedKernel(float* __restrict__ out, int64_t nTables, int64_t stride,
+                                                        int32_t nSamples, const int32_t* __restrict__ nWalks,
+                                                        const uint8_t* __restrict__ quantity, int32_t halves,
+                                                        uint64_t seed, uint32_t streamOffset) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nTables) return;
+    int ns = nSamples;
+    if (nWalks) {
+        if (quantity[t] != ZG_ESTIMATE_SOLUTION_AND_GRADIENT) return;
+        int nw = nWalks[t];
+        ns = min(2 * (halves == 2 ? max(1, nw / 2) : nw), nSamples);
+    }
+    if (ns <= 0) return;
+    float* s = out + (size_t)t * (size_t)stride;
+    Pcg32 rng;
+    rng.seed(~seed, ((uint64_t)t << 32) | (uint64_t)streamOffset);
+    const float inv = 1.0f / (float)ns;
+    const float oneMinusEps = 1.0f - FLT_EPSILON;
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j < DIMS; j++) s[(size_t)DIMS * i + j] = fminf(((float)i + rng.nextFloat()) * inv, oneMinusEps);
+    for (int j = 0; j < DIMS; j++)
+        for (int i = 0; i < ns; i++) {
+            int other = i + (int)rng.nextBounded((uint32_t)(ns - i));
+            float a = s[(size_t)DIMS * i + j];
+            s[(size_t)DIMS * i + j] = s[(size_t)DIMS * other + j];
+            s[(size_t)DIMS * other + j] = a;
+        }
+}
+
+// walk counts of the solution points (others 0) for the solution launch of a call with
+// per-point estimation quantities
+__global__ void __launch_bounds__(256) maskWalksKernel(const int32_t* __restrict__ nWalks,
+                                                       const uint8_t* __restrict__ quantity, int64_t n,
+                                                       int32_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = quantity[i] == ZG_ESTIMATE_SOLUTION ? nWalks[i] : 0;
+}
+
 __global__ void fillZeroKernel(uint64_t* __restrict__ p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -479,6 +647,12 @@ hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s) {
     if (nWords <= 0) return hipSuccess;
     unsigned g = (unsigned)std::min<int64_t>(4096, (nWords + 255) / 256);
     hipLaunchKernelGGL(fillZeroKernel, dim3(g), dim3(256), 0, s, p, nWords);
+    return hipGetLastError();
+}
+
+hipError_t launchMaskWalks(const int32_t* nWalks, const uint8_t* quantity, int64_t n, int32_t* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(maskWalksKernel, dim3(blocksFor(n)), dim3(256), 0, s, nWalks, quantity, n, out);
     return hipGetLastError();
 }
 
@@ -553,6 +727,34 @@ hipError_t launchWosBatch(const WosParams& P, int width, bool count, bool screen
         else { if (count) ZG_WOSB(8, true, false, false); else ZG_WOSB(8, false, false, false); }
     }
 #undef ZG_WOSB
+    return hipGetLastError();
+}
+
+hipError_t launchWosGrad(const WosParams& P, int dim, int width, bool screened, bool split, int gridBlocks,
+                         hipStream_t s) {
+    dim3 g(gridBlocks), bl(256);
+#define ZG_WOSG(D, Wd, S, SP) hipLaunchKernelGGL((wosKernel<D, Wd, false, false, S, SP, true>), g, bl, kStackLdsBytes, s, P)
+    if (dim == 2) {
+        if (split) { if (width == 4) ZG_WOSG(2, 4, false, true); else ZG_WOSG(2, 8, false, true); }
+        else { if (width == 4) ZG_WOSG(2, 4, false, false); else ZG_WOSG(2, 8, false, false); }
+    } else if (screened) {
+        if (split) { if (width == 4) ZG_WOSG(3, 4, true, true); else ZG_WOSG(3, 8, true, true); }
+        else { if (width == 4) ZG_WOSG(3, 4, true, false); else ZG_WOSG(3, 8, true, false); }
+    } else {
+        if (split) { if (width == 4) ZG_WOSG(3, 4, false, true); else ZG_WOSG(3, 8, false, true); }
+        else { if (width == 4) ZG_WOSG(3, 4, false, false); else ZG_WOSG(3, 8, false, false); }
+    }
+#undef ZG_WOSG
+    return hipGetLastError();
+}
+
+hipError_t launchStratified(int dims, float* out, int64_t nTables, int64_t stride, int32_t nSamples,
+                            const int32_t* nWalks, const uint8_t* quantity, int32_t halves, uint64_t seed,
+                            uint32_t streamOffset, hipStream_t s) {
+    if (nTables <= 0) return hipSuccess;
+    dim3 g(blocksFor(nTables)), bl(256);
+    if (dims == 1) hipLaunchKernelGGL(stratifiedKernel<1>, g, bl, 0, s, out, nTables, stride, nSamples, nWalks, quantity, halves, seed, streamOffset);
+    else hipLaunchKernelGGL(stratifiedKernel<2>, g, bl, 0, s, out, nTables, stride, nSamples, nWalks, quantity, halves, seed, streamOffset);
     return hipGetLastError();
 }
 

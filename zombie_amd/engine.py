@@ -291,6 +291,8 @@ class DevicePoints:
     first_sphere_radius: object = None
     cv_gradient: object = None
     cv_gradient_grid: object = None     # GridField
+    estimation_quantity: object = None  # uint8 [n]: ZG_ESTIMATE_SOLUTION / _SOLUTION_AND_GRADIENT / _SKIP
+    direction_for_derivative: object = None   # float32 [n, dim], default e_0
 
     @property
     def n(self) -> int:
@@ -312,6 +314,8 @@ class DevicePoints:
             self._grid_c = self.cv_gradient_grid.to_c()   # keep the struct alive for the call
             s.cv_gradient_grid = C.pointer(self._grid_c)
         s.n_walks_max = int(self.n_walks_max)
+        s.estimation_quantity = _ptr(self.estimation_quantity)
+        s.direction_for_derivative = _ptr(self.direction_for_derivative)
         return s
 
 
@@ -340,15 +344,24 @@ class DeviceStats:
     m2: object
     walk_length: object
     splits: object
+    # gradient members (SampleStatistics, common.h:226-247), present with zeros(..., dim=d)
+    n_grad: object = None
+    grad_mean: object = None     # [n, dim]
+    grad_m2: object = None       # [n, dim]
+    first_source: object = None
+    derivative: object = None
 
     @staticmethod
-    def zeros(count: int, device="cuda") -> "DeviceStats":
+    def zeros(count: int, device="cuda", dim: int = 0) -> "DeviceStats":
+        """dim > 0 also allocates the gradient members (needed by gradient points)."""
         torch = _torch()
-        return DeviceStats(torch.zeros(count, dtype=torch.int64, device=device),
-                           torch.zeros(count, dtype=torch.float64, device=device),
-                           torch.zeros(count, dtype=torch.float64, device=device),
-                           torch.zeros(count, dtype=torch.int64, device=device),
-                           torch.zeros(count, dtype=torch.int64, device=device))
+        z64 = lambda *s: torch.zeros(*s, dtype=torch.float64, device=device)
+        i64 = lambda *s: torch.zeros(*s, dtype=torch.int64, device=device)
+        st = DeviceStats(i64(count), z64(count), z64(count), i64(count), i64(count))
+        if dim > 0:
+            st.n_grad, st.grad_mean, st.grad_m2 = i64(count), z64(count, dim), z64(count, dim)
+            st.first_source, st.derivative = z64(count), z64(count)
+        return st
 
     def to_c(self) -> A.zg_statistics:
         s = A.zg_statistics()
@@ -357,11 +370,35 @@ class DeviceStats:
         s.solution_m2 = _ptr(self.m2)
         s.total_walk_length = _ptr(self.walk_length)
         s.total_splits = _ptr(self.splits)
+        s.n_gradient_estimates = _ptr(self.n_grad)
+        s.gradient_mean = _ptr(self.grad_mean)
+        s.gradient_m2 = _ptr(self.grad_m2)
+        s.total_first_source_contribution = _ptr(self.first_source)
+        s.total_derivative_contribution = _ptr(self.derivative)
         return s
 
     def variance(self):
         torch = _torch()
         return self.m2 / torch.clamp(self.n - 1, min=1).to(torch.float64)
+
+    def grad_variance(self):
+        torch = _torch()
+        return self.grad_m2 / torch.clamp(self.n_grad - 1, min=1).to(torch.float64)[:, None]
+
+    def derivative_mean(self):
+        """getEstimatedDerivative (common.h:353-357): total / max(1, solution estimates)"""
+        torch = _torch()
+        return self.derivative / torch.clamp(self.n, min=1).to(torch.float64)
+
+
+def stratified_samples(dims: int, n_tables: int, n_samples: int, seed: int, stream_offset: int = 0,
+                       device="cuda", stream=None):
+    """The per-point Latin hypercube tables of gradient solves: [n_tables, n_samples, dims]."""
+    torch = _torch()
+    out = torch.empty((n_tables, n_samples, dims), dtype=torch.float32, device=device)
+    A.check(A.lib().zg_stratified_samples(dims, n_tables, n_samples, seed, stream_offset, _ptr(out),
+                                          _stream_ptr(stream)))
+    return out
 
 
 def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
