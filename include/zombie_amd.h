@@ -333,7 +333,10 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
  * coefficients 0), harmonic Green's function (absorption 0).  points_host->normal, type and
  * estimate_boundary_normal_aligned describe points on the reflecting boundary;
  * first_sphere_radius (nullable) records the star radius of each point's first step on the
- * first call and is reused while the point has estimates. */
+ * first call and is reused while the point has estimates.  Gradient points run
+ * estimateSolutionAndGradient (:664-848) as in zg_wos_solve, with a first ball of radius
+ * 0.99 min(dist_to_absorbing_boundary, dist_to_reflecting_boundary) (the reflecting distance is
+ * queried when dist_to_reflecting_boundary is NULL). */
 zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,

@@ -57,9 +57,9 @@ def lib():
         L.orc_solve.argtypes = [C.c_int, P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings), C.c_int64, P, P, P, P,
                                 P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, P, P, P, P, P, P, P]
         L.orc_solve.restype = C.c_int
-        L.orc_solve_grad.argtypes = [P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings), C.c_int64, P, P, P, P, P,
-                                     P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P,
-                                     P, P, P, P]
+        L.orc_solve_grad.argtypes = [C.c_int, P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings), C.c_int64, P, P,
+                                     P, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P, P, P, P,
+                                     P, P, P, P, P, P, P]
         L.orc_solve_grad.restype = C.c_int
         L.orc_stratified_samples.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_uint64, C.c_uint32, P]
         L.orc_pcg32_bounded.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, P]
@@ -280,8 +280,9 @@ CV_STAGED = 1     # the GPU: refreshed when a stage [0,1), [1,2), [2,4), ... sta
 
 def solve_grad(scene: Scene, pde, ws, pts, dist, n_walks, stats: GradStats, quantity=None, direction=None,
                seed=0x5EED, stream_offset=0, rng_mode=RNG_PER_WALK, cv_mode=CV_PER_PAIR, threads=None,
-               types=None, first_sphere_radius=None):
-    """Walk on spheres with per-point estimation quantities (gradient points included)."""
+               types=None, first_sphere_radius=None, wost=False, dist_reflecting=None):
+    """Walk on spheres (or stars) with per-point estimation quantities (gradient points included).
+    dist_reflecting: distances to the reflecting boundary (computed when None)."""
     pts = np.ascontiguousarray(pts, np.float32)
     n = len(pts)
     dist = np.ascontiguousarray(dist, np.float32)
@@ -291,7 +292,9 @@ def solve_grad(scene: Scene, pde, ws, pts, dist, n_walks, stats: GradStats, quan
                                                                             (n, scene.dim)), np.float32)
     types = None if types is None else np.ascontiguousarray(types, np.uint8)
     rep = np.zeros(8, np.int64)
-    st = lib().orc_solve_grad(scene.h, C.byref(pde), C.byref(ws), n, _p(pts), _p(types), _p(dist), _p(nw), _p(q),
+    dr = None if dist_reflecting is None else np.ascontiguousarray(dist_reflecting, np.float32)
+    st = lib().orc_solve_grad(1 if wost else 0, scene.h, C.byref(pde), C.byref(ws), n, _p(pts), _p(types), _p(dist),
+                              _p(dr), _p(nw), _p(q),
                               _p(d), seed, stream_offset, rng_mode, cv_mode, threads or default_threads(),
                               _p(stats.n), _p(stats.mean), _p(stats.m2), _p(stats.walk_length), _p(stats.splits),
                               _p(stats.n_grad), _p(stats.grad_mean), _p(stats.grad_m2), _p(stats.first_source),

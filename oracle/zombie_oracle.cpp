@@ -1259,6 +1259,29 @@ void wosEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>& 
     }
 }
 
+template <int D>
+Code wostWalk(const Scene<D>& sc, const SolveCtx& cx, float distA, float firstSphereRadius, bool flip,
+              GreensBall<D>& gf, Pcg32& rng, WalkState<D>& st, std::deque<WalkState<D>>& q, QueryCount* qc);
+
+// terminal value of a walk-on-stars walk (estimateSolution :627-645): projection to the
+// absorbing boundary when it reached the boundary
+template <int D>
+float wostTerminal(const Scene<D>& sc, const SolveCtx& cx, Code code, WalkState<D>& st, QueryCount* qc) {
+    const zg_walk_settings& ws = *cx.ws;
+    if (code == Reached && !ws.ignore_absorbing_boundary_contribution) {
+        float d; V<D> cp; int prim;
+        bool aligned = false;
+        if (closestPoint(sc.abs, st.currentPt, d, cp, prim, qc)) {
+            V<D> n = sc.abs.primNormal(prim);
+            float sd = dot(st.currentPt - cp, n);
+            st.currentPt = cp; st.currentNormal = n;
+            aligned = ws.solve_double_sided && sd > 0.0f;
+        }
+        return evalField<D>(cx.pde->dirichlet, st.currentPt, st.currentNormal, aligned, cx.pde->absorption_coeff);
+    }
+    return 0.0f;
+}
+
 // gradient statistics of one point (SampleStatistics gradient members, common.h:226-247,
 // 288-309), FP64
 template <int D>
@@ -1296,7 +1319,9 @@ struct GradCv {
     }
 };
 
-// estimateSolutionAndGradient (walk_on_spheres.h:416-593).  rng_mode 0 consumes one generator
+// estimateSolutionAndGradient (walk_on_spheres.h:416-593; walk on stars :664-848, whose first
+// ball has radius 0.99 min(d_absorbing, d_reflecting) and whose walk goes on as a walk-on-stars
+// walk from the boundary point).  rng_mode 0 consumes one generator
 // per point in the reference's order (stratified table, then every walk; the antithetic half
 // replays the generator saved after the first step).  rng_mode 1 is the GPU's scheme: the table
 // comes from pcg32(~seed, (p << 32) | stream_offset) and both halves of pair w start from
@@ -1305,11 +1330,11 @@ struct GradCv {
 template <int D>
 void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t p, const V<D>& pt, float distA,
                      int nWalks, const V<D>& dirDeriv, Stats& out, GradStats<D>& g, int64_t codes[4],
-                     QueryCount* qc, float* firstSphereRadius) {
+                     QueryCount* qc, float* firstSphereRadius, bool wost = false, float distR = FLT_MAX) {
     const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
     int nHalves = 1;
     if (ws.use_gradient_antithetic_variates) { nWalks = std::max(1, nWalks / 2); nHalves = 2; }
-    const float R0 = 0.99f * distA;  // ZOMBIE_RADIUS_SHRINK_PERCENTAGE
+    const float R0 = 0.99f * (wost ? std::min(distA, distR) : distA);  // ZOMBIE_RADIUS_SHRINK_PERCENTAGE
     if (firstSphereRadius) *firstSphereRadius = R0;
     Pcg32 pointRng = makeRng(cx, p, 0);
     std::vector<float> strat;
@@ -1364,6 +1389,8 @@ void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t 
                 boundaryPt = gf.c + gf.R * bd;
             }
             if (half == 1) { V<D> bdv = boundaryPt - st.currentPt; boundaryPt = st.currentPt - bdv; }
+            st.prevDistance = gf.R;
+            st.prevDirection = (boundaryPt - st.currentPt) * (1.0f / gf.R);
             st.currentPt = boundaryPt;
             st.throughput *= gf.poissonKernel() / boundaryPdf;
             V<D> boundaryDir = gf.poissonKernelGradient(boundaryPt) * (1.0f / (boundaryPdf * st.throughput));
@@ -1380,11 +1407,12 @@ void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t 
                 }
                 float dist; V<D> cp; int prim;
                 closestPoint(sc.abs, s2.currentPt, dist, cp, prim, qc);
-                Code code = wosWalk<D>(sc, cx, dist, gf, rng, s2, q, qc);
+                Code code = wost ? wostWalk<D>(sc, cx, dist, 0.0f, false, gf, rng, s2, q, qc)
+                                 : wosWalk<D>(sc, cx, dist, gf, rng, s2, q, qc);
                 codes[code]++;
                 if (code != Escaped) {
-                    float term = wosTerminal<D>(sc, cx, code, s2, qc);
-                    total += s2.throughput * term + s2.totalSourceContribution;
+                    float term = wost ? wostTerminal<D>(sc, cx, code, s2, qc) : wosTerminal<D>(sc, cx, code, s2, qc);
+                    total += s2.throughput * term + s2.totalReflectingBoundaryContribution + s2.totalSourceContribution;
                     out.len += s2.walkLength;
                     success = true;
                 }
@@ -1578,18 +1606,7 @@ void wostEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>&
             Code code = wostWalk<D>(sc, cx, dA, fs, flip, gf, rng, st, q, qc);
             codes[code]++;
             if (code != Escaped) {
-                float term = 0.0f;
-                if (code == Reached && !ws.ignore_absorbing_boundary_contribution) {
-                    float d; V<D> cp; int prim;
-                    bool aligned = false;
-                    if (closestPoint(sc.abs, st.currentPt, d, cp, prim, qc)) {
-                        V<D> n = sc.abs.primNormal(prim);
-                        float sd = dot(st.currentPt - cp, n);
-                        st.currentPt = cp; st.currentNormal = n;
-                        aligned = ws.solve_double_sided && sd > 0.0f;
-                    }
-                    term = evalField<D>(pde.dirichlet, st.currentPt, st.currentNormal, aligned, pde.absorption_coeff);
-                }
+                float term = wostTerminal<D>(sc, cx, code, st, qc);
                 total += st.throughput * term + st.totalReflectingBoundaryContribution + st.totalSourceContribution;
                 out.len += st.walkLength;
                 success = true;
@@ -1698,11 +1715,12 @@ static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_s
                      int nThreads, int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits,
                      float* firstSphereRadius, int64_t* report, const uint8_t* quantity = nullptr,
                      const float* dirDeriv = nullptr, int cvMode = 0, int64_t* gN = nullptr, double* gMean = nullptr,
-                     double* gM2 = nullptr, double* gFirst = nullptr, double* gDeriv = nullptr) {
+                     double* gM2 = nullptr, double* gFirst = nullptr, double* gDeriv = nullptr,
+                     const float* distR = nullptr) {
     if (pde->absorption_coeff > 0.0f && D == 2) return ZG_ERR_UNSUPPORTED;
     bool anyGrad = false;
     if (quantity) for (int64_t i = 0; i < n; i++) anyGrad = anyGrad || quantity[i] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT;
-    if (anyGrad && (wost || !gN || !gMean || !gM2 || !gFirst || !gDeriv)) return ZG_ERR_INVALID_ARGUMENT;
+    if (anyGrad && (!gN || !gMean || !gM2 || !gFirst || !gDeriv)) return ZG_ERR_INVALID_ARGUMENT;
     SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
     int nt = std::max(1, nThreads);
     std::vector<std::array<int64_t, 8>> per(nt);
@@ -1719,8 +1737,13 @@ static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_s
             if (dirDeriv) dd = load<D>(dirDeriv + i * D);
             GradCv cv{cvMode, sN[i], sMean[i], gFirst[i]};
             GradStats<D> g;
+            float dR = FLT_MAX;
+            if (wost) {
+                if (distR) dR = distR[i];
+                else if (sc.hasRefl()) { V<D> cp; int pr; closestPoint(sc.refl, x, dR, cp, pr, &qc); }
+            }
             wosEstimateGrad<D>(sc, cx, cv, i, x, distA[i], nWalks[i], dd, st, g, codes, &qc,
-                               firstSphereRadius ? firstSphereRadius + i : nullptr);
+                               firstSphereRadius ? firstSphereRadius + i : nullptr, wost, dR);
             mergeInto(sN[i], sMean[i], sM2[i], st);
             if (g.n > 0) {
                 int64_t N1 = gN[i], N2 = g.n, N = N1 + N2;
@@ -1920,15 +1943,15 @@ int orc_solve(int wost, void* hv, const zg_pde* pde, const zg_walk_settings* ws,
 // solve with per-point estimation quantities (EstimationQuantity, common.h:145-149) and the
 // gradient statistics; cv_mode 0 refreshes the gradient control variates before every pair
 // (the reference), 1 once per stage [0,1), [1,2), [2,4), ... (the GPU)
-int orc_solve_grad(void* hv, const zg_pde* pde, const zg_walk_settings* ws, int64_t n, const float* pts,
-                   const uint8_t* types, const float* distA, const int32_t* nWalks, const uint8_t* quantity,
-                   const float* dirDeriv, uint64_t seed, uint32_t streamOffset, int rngMode, int cvMode, int nThreads,
-                   int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits, int64_t* gN,
-                   double* gMean, double* gM2, double* gFirst, double* gDeriv, float* firstSphereRadius,
+int orc_solve_grad(int wost, void* hv, const zg_pde* pde, const zg_walk_settings* ws, int64_t n, const float* pts,
+                   const uint8_t* types, const float* distA, const float* distR, const int32_t* nWalks,
+                   const uint8_t* quantity, const float* dirDeriv, uint64_t seed, uint32_t streamOffset, int rngMode,
+                   int cvMode, int nThreads, int64_t* sN, double* sMean, double* sM2, int64_t* sLen, int64_t* sSplits,
+                   int64_t* gN, double* gMean, double* gM2, double* gFirst, double* gDeriv, float* firstSphereRadius,
                    int64_t* report) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim == 2) return solveImpl<2>(false, *h->s2, pde, ws, n, pts, nullptr, types, nullptr, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report, quantity, dirDeriv, cvMode, gN, gMean, gM2, gFirst, gDeriv);
-    return solveImpl<3>(false, *h->s3, pde, ws, n, pts, nullptr, types, nullptr, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report, quantity, dirDeriv, cvMode, gN, gMean, gM2, gFirst, gDeriv);
+    if (h->dim == 2) return solveImpl<2>(wost != 0, *h->s2, pde, ws, n, pts, nullptr, types, nullptr, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report, quantity, dirDeriv, cvMode, gN, gMean, gM2, gFirst, gDeriv, distR);
+    return solveImpl<3>(wost != 0, *h->s3, pde, ws, n, pts, nullptr, types, nullptr, distA, nWalks, seed, streamOffset, rngMode, nThreads, sN, sMean, sM2, sLen, sSplits, firstSphereRadius, report, quantity, dirDeriv, cvMode, gN, gMean, gM2, gFirst, gDeriv, distR);
 }
 
 // the per-point Latin hypercube tables of a gradient solve: table t (2 * n_pairs rows of

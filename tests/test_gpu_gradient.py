@@ -257,6 +257,39 @@ def test_quantities_and_progressive_calls(gpu, oracle_lib, sphere):
     assert np.all(np.abs(_pooled(h2.grad_mean[q == 1], h2.grad_var[q == 1], h2.n_grad[q == 1], g)) < 4.0)
 
 
+def test_walk_on_stars_gradient(gpu, oracle_lib):
+    """walk_on_stars.h:664-848 on the mixed Dirichlet/Neumann problem with interior slits:
+    first ball 0.99 min(d_absorbing, d_reflecting), then walk-on-stars steps"""
+    torch = gpu
+    O = oracle_lib
+    p = PB.mixed2d(n_segments=1024, n_slits=8)
+    osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    V, I = p.meta["outer"]
+    cand = PB.sample_box(p.domain_min, p.domain_max, 3000, 29)
+    pts = cand[PB.points_in_polygon(cand, V, I)][:400]
+    d, *_ = osc.closest_point(pts, boundary=0)
+    dr, *_ = osc.closest_point(pts, boundary=1)
+    x = pts.astype(np.float64)
+    g = np.stack([3 * x[:, 0] ** 2 - 3 * x[:, 1] ** 2, -6 * x[:, 0] * x[:, 1]], 1)
+    sc = E.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    ws = p.settings
+    for give_dr in (True, False):
+        dp = _points(torch, pts, d, 256)
+        if give_dr:
+            dp.dist_to_reflecting = torch.from_numpy(dr.astype(np.float32)).cuda()
+        st = E.DeviceStats.zeros(len(pts), dim=2)
+        rep = E.wost_solve(sc, p.pde, ws, dp, st, E.SolveOptions(seed=31 + give_dr))
+        h = _Host(st)
+        # rare escapes through the slit ends (numerical leaks) are dropped, as in the reference
+        assert np.array_equal(h.n_grad, h.n) and h.n.min() >= 250
+        assert rep.completion_codes[3] <= 1e-4 * rep.total_walks + 2
+        assert np.allclose(dp.first_sphere_radius.cpu().numpy(), np.float32(0.99) * np.minimum(d, dr), rtol=1e-5)
+        assert np.all(np.abs(_pooled(h.grad_mean, h.grad_var, h.n_grad, g)) < 4.0), give_dr
+    ost = O.GradStats(len(pts), 2)
+    O.solve_grad(osc, p.pde, ws, pts, d, 256, ost, wost=True, seed=41, cv_mode=O.CV_STAGED, dist_reflecting=dr)
+    _vs_oracle(h, ost, "wost")
+
+
 def test_gradient_argument_errors(gpu, sphere):
     torch = gpu
     V, F, lo, hi, osc = sphere

@@ -385,3 +385,34 @@ def test_quantities_skip_and_solution(oracle_lib):
     assert np.array_equal(st.mean[s0], ref.mean[s0]) and np.array_equal(st.n[s0], ref.n[s0])
     assert np.all(st.n[q == 2] == 0) and np.all(st.n_grad[q != 1] == 0)
     assert np.all(st.n_grad[q == 1] == 64)
+
+
+def _cubic_grad(x):
+    x = x.astype(np.float64)
+    return np.stack([3 * x[:, 0] ** 2 - 3 * x[:, 1] ** 2, -6 * x[:, 0] * x[:, 1]], 1)
+
+
+def test_walk_on_stars_gradient_matches_exact(oracle_lib):
+    """walk_on_stars.h:664-848: the first ball avoids both boundaries (radius
+    0.99 min(d_absorbing, d_reflecting)), then a walk-on-stars walk with Neumann reflections"""
+    O = oracle_lib
+    p = PB.mixed2d(n_segments=1024, n_slits=8)
+    sc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    V, I = p.meta["outer"]
+    cand = PB.sample_box(p.domain_min, p.domain_max, 1500, 19)
+    pts = cand[PB.points_in_polygon(cand, V, I)][:160]
+    d, *_ = sc.closest_point(pts, boundary=0)
+    dr, *_ = sc.closest_point(pts, boundary=1)
+    g = _cubic_grad(pts)
+    for kw, rng_mode in ((dict(), O.RNG_PER_WALK), (dict(use_gradient_control_variates=0), O.RNG_PER_POINT)):
+        ws = E.walk_settings_host(1e-3, 1e-3, 16384, True)
+        for k, v in kw.items():
+            setattr(ws, k, v)
+        st = O.GradStats(len(pts), 2)
+        fsr = np.zeros(len(pts), np.float32)
+        O.solve_grad(sc, p.pde, ws, pts, d, 256, st, wost=True, rng_mode=rng_mode, first_sphere_radius=fsr,
+                     dist_reflecting=dr if rng_mode == O.RNG_PER_WALK else None)
+        z, pooled = _gz(st, g)
+        assert np.all(np.abs(pooled) < 4.0) and 0.2 < z.std() < 1.2, (kw, pooled, z.std())
+        assert np.allclose(fsr, np.float32(0.99) * np.minimum(d, dr), rtol=1e-6)
+        assert np.array_equal(st.n_grad, st.n) and np.all(st.n == 256)

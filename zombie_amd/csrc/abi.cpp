@@ -492,6 +492,7 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     P.quantity = pts->estimation_quantity; M.quantity = pts->estimation_quantity;
     M.want = ZG_ESTIMATE_SOLUTION;
     P.dirDeriv = pts->direction_for_derivative;
+    P.distR = pts->dist_to_reflecting_boundary;
     S.stats = *stats;
 
     S.dim = dim; S.width = s ? s->width : b->width; S.depth = s ? s->info.absorbing_depth : b->maxDepth;
@@ -620,6 +621,7 @@ hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipS
     if (e == hipSuccess)
         e = launchStratified(D - 1, const_cast<float*>(P.strat), n, L.stride, 2 * L.pairsMax, P.nWalks, P.quantity,
                              L.halves, P.seed, P.streamOffset, strm);
+    if (e == hipSuccess && S.wost) e = launchWostGradFirstSphere(P, S.width, const_cast<float*>(P.fsr), strm);
     const bool cv = S.P.ws.use_gradient_control_variates != 0;
     for (int u0 = 0; e == hipSuccess && u0 < L.pairsMax;) {
         int u1 = cv ? (u0 == 0 ? 1 : std::min(2 * u0, L.pairsMax)) : L.pairsMax;
@@ -631,7 +633,9 @@ hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipS
         P.gradW1 = u1 * L.halves;
         P.chunksPerPoint = std::max(1, (P.gradW1 - P.gradW0 + P.wpl - 1) / P.wpl);
         P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
-        if (e == hipSuccess) e = launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
+        if (e == hipSuccess)
+            e = S.wost ? launchWostGrad(P, S.width, S.blocks, strm)
+                       : launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
         if (e == hipSuccess) e = launchMergeStats(M, D, strm);
         u0 = u1;
     }
@@ -656,7 +660,6 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
     }
     size_t words = S.words;
     if (nGrad > 0) {
-        if (S.wost) return fail(ZG_ERR_UNSUPPORTED, "gradient estimation runs in walk on spheres only");
         const zg_statistics& st = S.stats;
         if (!st.n_gradient_estimates || !st.gradient_mean || !st.gradient_m2 || !st.total_first_source_contribution ||
             !st.total_derivative_contribution)

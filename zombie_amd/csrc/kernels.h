@@ -78,6 +78,7 @@ struct WosParams {
     double* accG2;                    // [n*dim] sums of their squares
     double* accFirst;                 // [n] first-step source terms
     double* accDeriv;                 // [n] directional derivative contributions
+    const float* distR;               // walk on stars: caller's distances to the reflecting boundary (nullable)
 };
 
 // capacity of the per-lane queue of split walk parts
@@ -150,6 +151,8 @@ hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks,
 // walk on stars (2D)
 hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s);
 hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchWostGradFirstSphere(const WosParams& P, int width, float* fsr, hipStream_t s);
+hipError_t launchWostGrad(const WosParams& P, int width, int gridBlocks, hipStream_t s);
 hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
                             float prec, const uint8_t* flip, float* radius, hipStream_t s);
 hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, const float* p2, uint8_t* visible,

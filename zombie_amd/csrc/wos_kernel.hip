@@ -520,9 +520,10 @@ __global__ void __launch_bounds__(256) mergeStatsKernel(MergeParams M) {
     double sum = M.accSum[i], sq = M.accSq[i];
     int64_t nb = (int64_t)M.accN[i];
     if (M.want == ZG_ESTIMATE_SOLUTION_AND_GRADIENT) {
-        // gradient points: first sphere 0.99 d (walk_on_spheres.h:430), no boundary shortcut;
-        // gradient components merged like the solution (common.h:288-309)
-        if (M.firstSphereRadius) M.firstSphereRadius[i] = 0.99f * M.dist0[i];
+        // gradient points: first sphere 0.99 d (walk_on_spheres.h:430; walk on stars records
+        // 0.99 min(d_absorbing, d_reflecting), :680-681), no boundary shortcut; gradient
+        // components merged like the solution (common.h:288-309)
+        if (M.firstSphereRadius) M.firstSphereRadius[i] = M.fsrSrc ? M.fsrSrc[i] : 0.99f * M.dist0[i];
         if (nb > 0) {
             int64_t G1 = M.gN[i];
             double N = (double)(G1 + nb);

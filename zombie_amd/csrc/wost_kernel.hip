@@ -62,8 +62,32 @@ __global__ void __launch_bounds__(256) wostFirstSphereKernel(WosParams P, const 
     fsr[i] = r;
 }
 
-// 4 waves per SIMD (106 VGPRs); forcing 5 or 6 waves (spills) measured no faster
-template <int W, bool COUNT>
+// first ball of the gradient walks of walk on stars (walk_on_stars.h:678-681): radius
+// 0.99 min(d_absorbing, d_reflecting), the reflecting distance taken from the caller or queried
+template <int W>
+__global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, float* __restrict__ fsr) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nPoints || P.quantity[i] != ZG_ESTIMATE_SOLUTION_AND_GRADIENT) return;
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
+    float dR = FLT_MAX;
+    if (P.distR) {
+        dR = P.distR[i];
+    } else if (P.scene.refl.nPrims > 0) {
+        VisitCount vc{0, 0};
+        float b2 = FLT_MAX;
+        Vec<2> cp = vzero<2>();
+        int prim = -1;
+        if (closestPoint<2, W, false>(P.scene.refl, vload<2>(P.pt + 2 * i), b2, cp, prim, st, vc)) dR = __builtin_sqrtf(b2);
+    }
+    fsr[i] = 0.99f * fminf(P.dist0[i], dR);
+}
+
+// 4 waves per SIMD (106 VGPRs); forcing 5 or 6 waves (spills) measured no faster.
+// GRAD = true: estimateSolutionAndGradient (walk_on_stars.h:664-848) for the points asking for
+// it -- the explicit first step of the walk-on-spheres gradient walks inside the first ball (no
+// boundary in it), then walk-on-stars steps from the boundary point.
+template <int W, bool COUNT, bool GRAD = false>
 __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
@@ -91,8 +115,80 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     bool onRefl = false, flip = false, firstStep = true;
     Ball<2> ball; ball.yukawa = false; ball.sqrtLambda = 0.0f; ball.R = 0.0f;
     Pcg32 rng; rng.state = 0; rng.inc = 1;
+    // gradient walks: boundary / source gradient directions, first-step source term, control variates
+    Vec<2> gB = vzero<2>(), gS = vzero<2>();
+    float fsc = 0.0f, cvB = 0.0f, cvS = 0.0f;
+
+    // first step of a gradient walk (walk_on_stars.h:705-791)
+    auto startGradWalk = [&]() {
+        const int pair = P.gradHalves == 2 ? (w >> 1) : w;
+        const bool mirror = P.gradHalves == 2 && (w & 1);
+        const Vec<2> c0 = vload<2>(P.pt + 2 * (size_t)p);
+        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
+        T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
+        ball.update(P.fsr[p]);
+        const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + 2 * (size_t)pair;
+        fsc = 0.0f; gS = vzero<2>();
+        if (!ws.ignore_source_contribution) {
+            float gNorm = ball.norm();
+            if (pde.is_source_constant) {
+                fsc = gNorm * evalField<2>(pde.source, c0, vzero<2>(), false, 0.0f);
+            } else {
+                float pdf;
+                Vec<2> sd = sphereFromU<2>(tab);
+                float r = ball.sampleRadiusPdf(rng, pdf);
+                Vec<2> ys = c0 + sd * r;
+                if (mirror) ys = c0 - (ys - c0);
+                fsc = gNorm * evalField<2>(pde.source, ys, vzero<2>(), false, 0.0f);
+                gS = (ys - c0) * (ball.gradientNorm(r) / (pdf * gNorm));
+            }
+            Ssrc += fsc;
+        }
+        Vec<2> bd;
+        float bpdf;
+        if (ws.use_cosine_sampling_for_derivatives) {
+            bd = hemisphereCosineFromU<2>(tab + 1);
+            if (rng.nextFloat() < 0.5f) bd[1] = -bd[1];
+            bpdf = 0.5f * pdfHemisphereCosine<2>(fabsf(bd[1]));
+            Vec<2> nd = vzero<2>();
+            nd[0] = 1.0f;
+            if (P.dirDeriv) nd = vload<2>(P.dirDeriv + 2 * (size_t)p);
+            bd = fromFrame(nd, bd);
+        } else {
+            bd = sphereFromU<2>(tab + 1);
+            bpdf = 1.0f / (2.0f * kPi);
+        }
+        Vec<2> yb = c0 + bd * ball.R;
+        if (mirror) yb = c0 - (yb - c0);
+        prevDist = ball.R;
+        prevDir = (yb - c0) * (1.0f / ball.R);
+        x = yb;
+        nrm = vzero<2>();
+        onRefl = false; flip = false;
+        T *= ball.poissonKernel() / bpdf;
+        gB = (yb - c0) * (ball.poissonKernelGradientScale() / (bpdf * T));
+        cvB = 0.0f; cvS = 0.0f;
+        if (ws.use_gradient_control_variates) {
+            int64_t N = P.cvN[p];
+            cvB = (float)P.cvMean[p];
+            cvS = (float)(P.cvFirst[p] / (double)(N > 1 ? N : 1));
+        }
+        // the walk goes on from the boundary point with an exact absorbing distance and a star
+        // radius computed at its first step
+        float b2 = FLT_MAX;
+        Vec<2> cp = x;
+        int prim = -1;
+        bool gotA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc);
+        distA = gotA ? __builtin_sqrtf(b2) : FLT_MAX;
+        absPrim = gotA ? prim : -1;
+        fsr0 = 0.0f;
+        firstStep = true;
+        atomicAdd(&blk[5], 1ull);
+        active = true;
+    };
 
     auto startWalk = [&]() {
+        if (GRAD) { startGradWalk(); return; }
         uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
         Vec<2> n = P.normal ? vload<2>(P.normal + 2 * (size_t)p) : vzero<2>();
         bool al = P.aligned ? P.aligned[p] != 0 : false;
@@ -132,10 +228,31 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 }
                 term = evalField<2>(pde.dirichlet, xp, nn, al, 0.0f);
             }
-            double c = (double)(T * term + Sref + Ssrc);
+            float total = T * term + Sref + Ssrc;
+            double c = (double)total;
             atomicAdd(&P.accSum[p], c);
             atomicAdd(&P.accSq[p], c * c);
             atomicAdd(&P.accN[p], 1ull);
+            if (GRAD) {
+                // gradient estimate and directional derivative (walk_on_stars.h:818-838)
+                float bc = total - fsc;
+                Vec<2> nd = vzero<2>();
+                nd[0] = 1.0f;
+                if (P.dirDeriv) nd = vload<2>(P.dirDeriv + 2 * (size_t)p);
+                float dd = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    float bi = (bc - cvB) * gB[i];
+                    float si = (fsc - cvS) * gS[i];
+                    dd += bi * nd[i];
+                    dd += si * nd[i];
+                    double e = (double)(bi + si);
+                    atomicAdd(&P.accG[2 * (size_t)p + i], e);
+                    atomicAdd(&P.accG2[2 * (size_t)p + i], e * e);
+                }
+                atomicAdd(&P.accFirst[p], (double)fsc);
+                atomicAdd(&P.accDeriv[p], (double)dd);
+            }
             atomicAdd(&P.accLen[p], (unsigned long long)len);
             atomicAdd(&blk[4], (unsigned long long)len);
         }
@@ -164,10 +281,17 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                         p = (int)(c / (unsigned long long)P.chunksPerPoint);
                         int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
                         int nw = P.nWalks[p];
-                        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
-                        if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
-                        else if (P.dist0[p] <= epsA) nw = min(nw, 1);
-                        w = k * P.wpl;
+                        if (GRAD) {
+                            // walks of the stage: [gradW0, min(gradW1, halves * pairs))
+                            int pairs = P.gradHalves == 2 ? max(1, nw / 2) : nw;
+                            nw = P.quantity[p] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT ? min(pairs * P.gradHalves, P.gradW1) : 0;
+                            w = P.gradW0 + k * P.wpl;
+                        } else {
+                            uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+                            if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
+                            else if (P.dist0[p] <= epsA) nw = min(nw, 1);
+                            w = k * P.wpl;
+                        }
                         wEnd = min(w + P.wpl, nw);
                         if (w < wEnd) startWalk();
                     }
@@ -339,6 +463,21 @@ hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* cal
     dim3 g(wostBlocksFor(P.nPoints)), bl(256);
     if (width == 4) hipLaunchKernelGGL((wostFirstSphereKernel<4>), g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
     else hipLaunchKernelGGL((wostFirstSphereKernel<8>), g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
+    return hipGetLastError();
+}
+
+hipError_t launchWostGradFirstSphere(const WosParams& P, int width, float* fsr, hipStream_t s) {
+    if (P.nPoints <= 0) return hipSuccess;
+    dim3 g((unsigned)((P.nPoints + 255) / 256)), bl(256);
+    if (width == 4) hipLaunchKernelGGL((wostGradFirstSphereKernel<4>), g, bl, kStackLdsBytes, s, P, fsr);
+    else hipLaunchKernelGGL((wostGradFirstSphereKernel<8>), g, bl, kStackLdsBytes, s, P, fsr);
+    return hipGetLastError();
+}
+
+hipError_t launchWostGrad(const WosParams& P, int width, int gridBlocks, hipStream_t s) {
+    dim3 g(gridBlocks), bl(256);
+    if (width == 4) hipLaunchKernelGGL((wostKernel<4, false, true>), g, bl, kStackLdsBytes, s, P);
+    else hipLaunchKernelGGL((wostKernel<8, false, true>), g, bl, kStackLdsBytes, s, P);
     return hipGetLastError();
 }
 
