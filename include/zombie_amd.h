@@ -90,6 +90,8 @@ void zg_walk_settings_default(zg_walk_settings* ws, float epsilon_shell_absorbin
 #define ZG_EVAL_VALUE 0              /* u(x)                                                 */
 #define ZG_EVAL_NORMAL_DERIVATIVE 1  /* s * grad u(x) . n, s = -1 on the normal-aligned side  */
 #define ZG_EVAL_SCREENED_SOURCE 2    /* -Lap u(x) + absorption * u(x)                        */
+#define ZG_EVAL_ROBIN 3              /* s * grad u(x) . n + params[5] * u(x): Robin data h for */
+                                     /* du/dn + kappa u = h with kappa = params[5]            */
 
 /* ZG_FN_DENSE_GRID: DenseGrid<float,1,DIM> (utils/dense_grid.h:30-356) with the semantics of
  * its getDenseGridCallback0: grid[] holds shape[0] x shape[1] (x shape[2]) cell values, the
@@ -162,6 +164,16 @@ zg_status zg_scene_create(int32_t dim, const float* positions_host, int64_t n_ve
                           int32_t solve_double_sided, float silhouette_precision,
                           const zg_build_options* opts, zg_scene** out);
 zg_status zg_scene_destroy(zg_scene* scene);
+
+/* Robin coefficient bounds of the reflecting primitives (FcpwRobinBoundaryHandler<2>::
+ * buildAccelerationStructure / updateCoefficientValues, fcpw_geometric_queries.h:726-831):
+ * min_coeff_host[p] <= max_coeff_host[p] for every primitive p of the scene (entries of
+ * absorbing primitives are ignored); both NULL clears them (Neumann conditions).  With bounds
+ * set, star radii (zg_star_radius, zg_wost_solve) also respect the Robin bound of every segment
+ * with max coefficient > 0 (reflectance_boundary_bvh/geometry.h:86-140, robin_bounds.h:12-35).
+ * The pointwise coefficient of the walk throughput is pde.robin_coeff.  2D scenes only. */
+zg_status zg_scene_set_robin_coefficients(zg_scene* scene, const float* min_coeff_host,
+                                          const float* max_coeff_host);
 zg_status zg_scene_get_info(const zg_scene* scene, zg_scene_info* info_host);
 
 /* ------------------------------------------------------------------------------------ */
@@ -329,8 +341,10 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        zg_solve_report* report_host, void* stream);
 
 /* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
- * walk :345-490); 2D scenes, pure Neumann reflecting conditions (h = pde.robin, Robin
- * coefficients 0), harmonic Green's function (absorption 0).  points_host->normal, type and
+ * walk :345-490); 2D scenes, Neumann or Robin reflecting conditions (h = pde.robin; with
+ * are_robin_conditions_pure_neumann = 0 the throughput takes the Robin reflectance with
+ * kappa = pde.robin_coeff, and zg_scene_set_robin_coefficients gives the per-segment bounds of
+ * the star radius), harmonic Green's function (absorption 0).  points_host->normal, type and
  * estimate_boundary_normal_aligned describe points on the reflecting boundary;
  * first_sphere_radius (nullable) records the star radius of each point's first step on the
  * first call and is reused while the point has estimates.  Gradient points run

@@ -40,6 +40,8 @@ def lib():
         L.orc_scene_create.argtypes = [C.c_int, P, C.c_int64, P, C.c_int64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_float]
         L.orc_scene_create.restype = P
         L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_set_robin.argtypes = [P, P, P]
+        L.orc_robin_radius.argtypes = [P, C.c_int64, P, P, C.c_int, P]
         L.orc_scene_stats.argtypes = [P, P]
         L.orc_closest_point.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, C.c_int, P, C.c_int]
         L.orc_intersect_ray.argtypes = [P, C.c_int, C.c_int64, P, P, P, P, P, P, C.c_int]
@@ -173,6 +175,22 @@ class Scene:
         if getattr(self, "h", None):
             lib().orc_scene_destroy(self.h)
             self.h = None
+
+    def set_robin(self, min_coeff, max_coeff):
+        """Per-primitive Robin coefficient bounds (arrays over all primitives; None clears)."""
+        if min_coeff is None:
+            lib().orc_scene_set_robin(self.h, None, None)
+            return
+        self._rmin = np.ascontiguousarray(min_coeff, np.float32)
+        self._rmax = np.ascontiguousarray(max_coeff, np.float32)
+        lib().orc_scene_set_robin(self.h, _p(self._rmin), _p(self._rmax))
+
+    def robin_radius(self, pts, max_r, brute=False):
+        pts = np.ascontiguousarray(pts, np.float32)
+        mr = np.ascontiguousarray(np.broadcast_to(np.asarray(max_r, np.float32), len(pts)), np.float32)
+        out = np.empty(len(pts), np.float32)
+        lib().orc_robin_radius(self.h, len(pts), _p(pts), _p(mr), 1 if brute else 0, _p(out))
+        return out
 
     def stats(self):
         out = np.zeros(5, np.int64)

@@ -286,10 +286,27 @@ struct GreensBall {
         return poissonKernelDampeningFactor(r);
     }
 
-    // reflectance for pure Neumann conditions (robinCoeff = 0): 1 (harmonic), Q (Yukawa)
-    float reflectance(float r) const {
-        if (!yukawa) return 1.0f;
-        return poissonKernelDampeningFactor(r);
+    // reflectance (distributions.h:536-543 harmonic 2D, :635-645 harmonic 3D, :957-971 Yukawa 3D)
+    // for a step of length r along dir onto a boundary with normal n and Robin coefficient kappa
+    float reflectance(float r, const V<D>& dir, const V<D>& n, float kappa) const {
+        if (!yukawa) {
+            if (std::fabs(kappa) > 0.0f) {
+                float P = D == 2 ? dot(n, dir) / r : dot(n, dir) / (r * r);
+                float G = D == 2 ? std::log(R / r) : 1.0f / r - 1.0f / R;
+                return 1.0f - kappa * G / P;
+            }
+            return 1.0f;
+        }
+        float Q = poissonKernelDampeningFactor(r);
+        if (std::fabs(kappa) > 0.0f) {
+            float P = dot(n, dir) / (r * r);
+            float mur = r * sqrtLambda;
+            float expmur = std::exp(-mur);
+            float sinhmur = (1.0f - expmur * expmur) / (2.0f * expmur);
+            float G = (expmur - expmuR * sinhmur / sinhmuR) / r;
+            return Q - kappa * G / P;
+        }
+        return Q;
     }
 
     // gradient of G(x, y) in y for the centred ball: (y - c) * gradientNorm(r)
@@ -463,10 +480,11 @@ float evalField(const zg_function& f, const V<D>& x, const V<D>& n, bool normalA
     default: break;
     }
     if (f.eval == ZG_EVAL_VALUE) return u;
-    if (f.eval == ZG_EVAL_NORMAL_DERIVATIVE) {
+    if (f.eval == ZG_EVAL_NORMAL_DERIVATIVE || f.eval == ZG_EVAL_ROBIN) {
         float dn = 0.0f;
         for (int i = 0; i < D; i++) dn += g[i] * n[i];
-        return normalAligned ? -dn : dn;
+        dn = normalAligned ? -dn : dn;
+        return f.eval == ZG_EVAL_ROBIN ? dn + f.params[5] * u : dn;
     }
     return negLap + absorption * u;
 }
@@ -688,6 +706,10 @@ struct Mesh {
     Bvh<D> silBvh;                                 // over the silhouette candidates (point boxes)
     std::vector<std::array<float, 4>> silCone;     // per silBvh node: axis.xy, sin, cos of half-angle + margin
     float silConePrecision = 0.0f;
+    // Robin coefficient bounds per primitive (FcpwRobinBoundaryHandler::buildAccelerationStructure /
+    // updateCoefficientValues, fcpw_geometric_queries.h:726-831); empty = 0 (pure Neumann)
+    std::vector<float> robinMin, robinMax;
+    bool hasRobin() const { for (float v : robinMax) if (v > FLT_EPSILON) return true; return false; }
 
     int nPrims() const { return (int)origId.size(); }
     V<D> vtx(int p, int k) const { return verts[idx[p * D + k]]; }
@@ -976,6 +998,47 @@ inline float starRadius2Bvh(const Mesh<2>& m, const V<2>& x, float minRadius, fl
         if (dn <= best2) stack[sp++] = {nearN, dn};
     }
     return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
+}
+
+// Robin part of the star radius of a segment (ReflectanceLineSegment::computeSquaredStarRadius,
+// reflectance_boundary_bvh/geometry.h:86-140, with RobinLineSegmentBound, robin_bounds.h:12-35):
+// inside the radius (h / c) exp(c^2 / (h kappa_max)), c = sqrt(h kappa_max)/sqrt(2) clamped to the
+// view-angle cosines of the segment's closest and farthest points, the reflectance of every
+// point of the segment seen from x stays non-negative.  FCPW's epsilon is taken as FLT_EPSILON.
+inline void robinSegmentBound(const Mesh<2>& m, int p, const V<2>& x, float& r2) {
+    float kmax = m.robinMax.empty() ? 0.0f : m.robinMax[p];
+    if (!(kmax > FLT_EPSILON)) return;            // perfectly reflecting: silhouettes only
+    V<2> pa = m.vtx(p, 0), pb = m.vtx(p, 1), cp;
+    float cd = closestPointSegment(pa, pb, x, cp);
+    float cd2 = cd * cd;
+    if (cd2 > r2) return;
+    if (cd < FLT_EPSILON) { r2 = std::min(r2, cd); return; }
+    V<2> n0 = segmentNormal(pa, pb);
+    float h = std::fabs(dot(n0, pa - x));
+    if (!(h > 0.0f)) { r2 = std::min(r2, cd2); return; }
+    float da = norm(x - pa), db = norm(x - pb);
+    V<2> fp = da >= db ? pa : pb;
+    float fd = std::max(da, db);
+    float cosUpper = std::fabs(dot((x - cp) * (1.0f / cd), n0));
+    float cosLower = std::fabs(dot((x - fp) * (1.0f / fd), n0));
+    float cosLine = std::sqrt(h * kmax) / std::sqrt(2.0f);
+    float c = std::min(std::max(cosLine, cosLower), cosUpper);
+    float rad = (h / c) * std::exp(c * c / (h * kmax));
+    r2 = std::min(r2, rad * rad);
+}
+// minimum over the reflecting segments within sqrt(r2) of x (BVH pruned by box distance)
+inline float robinRadius2(const Mesh<2>& m, const V<2>& x, float r2, bool brute = false) {
+    if (m.nPrims() == 0 || m.robinMax.empty()) return r2;
+    if (brute) { for (int p = 0; p < m.nPrims(); p++) robinSegmentBound(m, p, x, r2); return r2; }
+    const auto& nodes = m.bvh.nodes;
+    int stack[128]; int sp = 0; stack[sp++] = 0;
+    while (sp > 0) {
+        const auto& nd = nodes[stack[--sp]];
+        if (nd.box.dist2(x) > r2) continue;
+        if (nd.left < 0) { for (int i = nd.start; i < nd.start + nd.count; i++) robinSegmentBound(m, m.bvh.order[i], x, r2); continue; }
+        stack[sp++] = nd.right; stack[sp++] = nd.left;
+    }
+    return r2;
 }
 
 // line of sight over the BVH: any reflecting segment crossing the open segment p1 -> p2
@@ -1483,11 +1546,29 @@ float wostStarRadius(const Scene<D>& sc, const SolveCtx& cx, const V<D>& x, floa
     const zg_walk_settings& ws = *cx.ws;
     float r = distA;
     if constexpr (D == 2) {
-        if (sc.hasRefl()) r = starRadius2Bvh(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip);
+        const float minR = ws.epsilon_shell_for_reflecting_boundary;
+        if (sc.hasRefl()) {
+            r = starRadius2Bvh(sc.refl, x, minR, distA, ws.silhouette_precision, flip);
+            // Robin boundaries also bound the radius (populateStarRadiusQueryForRobinBoundary,
+            // fcpw_geometric_queries.h:1272-1289)
+            if (minR <= distA && sc.refl.hasRobin()) {
+                float r2 = robinRadius2(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX);
+                r = std::max(std::min(r, std::sqrt(r2)), minR);
+            }
+        }
     }
     if (sc.hasRefl() && ws.epsilon_shell_for_reflecting_boundary <= distA)
         r = std::max(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
     return r;
+}
+
+// PDE::hasNonZeroRobinCoeff (pde.h:62-78)
+template <int D>
+bool hasNonZeroRobinCoeff(const zg_pde& pde, const V<D>& x) {
+    if (pde.are_robin_conditions_pure_neumann) return false;
+    V<D> n = V<D>::zero(); n[0] = 1.0f;
+    return std::fabs(evalField<D>(pde.robin_coeff, x, n, true, pde.absorption_coeff)) > 0.0f ||
+           std::fabs(evalField<D>(pde.robin_coeff, x, n, false, pde.absorption_coeff)) > 0.0f;
 }
 
 template <int D>
@@ -1507,6 +1588,8 @@ Code wostWalk(const Scene<D>& sc, const SolveCtx& cx, float distA, float firstSp
                 }
             }
             if (ws.steps_before_using_maximal_spheres <= st.walkLength) starRadius = distA;
+            else if (st.onReflectingBoundary && hasNonZeroRobinCoeff<D>(pde, st.currentPt))
+                starRadius = ws.epsilon_shell_for_reflecting_boundary;   // reflectance vanishes on the boundary
             else starRadius = wostStarRadius<D>(sc, cx, st.currentPt, distA, flip);
         }
         gf.updateBall(st.currentPt, starRadius);
@@ -1538,10 +1621,19 @@ Code wostWalk(const Scene<D>& sc, const SolveCtx& cx, float distA, float firstSp
         st.currentPt = ipt;
         st.currentNormal = inrm;
         st.onReflectingBoundary = hit;
-        // computeWalkStepThroughput (:277-307), pure Neumann
+        // computeWalkStepThroughput (:277-307)
         float thr;
         if (st.onReflectingBoundary && st.prevDistance > FLT_EPSILON) {
-            thr = std::min(std::max(gf.reflectance(st.prevDistance), 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
+            float kappa = 0.0f;
+            V<D> nr = st.currentNormal;
+            if (!pde.are_robin_conditions_pure_neumann) {
+                bool fl = false;
+                if (ws.solve_double_sided) { fl = dot(st.prevDirection, st.currentNormal) < 0.0f; if (fl) nr = nr * -1.0f; }
+                kappa = evalField<D>(pde.robin_coeff, st.currentPt, st.currentNormal, ws.solve_double_sided && fl,
+                                     pde.absorption_coeff);
+            }
+            thr = std::min(std::max(gf.reflectance(st.prevDistance, st.prevDirection, nr, kappa), 0.0f),
+                           pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
         } else {
             thr = gf.directionSampledPoissonKernel(st.currentPt);
         }
@@ -1584,6 +1676,8 @@ void wostEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>&
     float fsr = firstSphereRadiusIO ? *firstSphereRadiusIO : 0.0f;
     if (!hasPrev) {
         if (distA <= ws.epsilon_shell_for_absorbing_boundary || ws.steps_before_using_maximal_spheres == 0) fsr = distA;
+        else if (type == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && hasNonZeroRobinCoeff<D>(pde, pt))
+            fsr = ws.epsilon_shell_for_reflecting_boundary;
         else fsr = wostStarRadius<D>(sc, cx, pt, distA, flip);
         if (firstSphereRadiusIO) *firstSphereRadiusIO = fsr;
     }
@@ -1837,6 +1931,25 @@ void* orc_scene_create(int dim, const float* V_, int64_t nV, const int32_t* I, i
     return h;
 }
 void orc_scene_destroy(void* h) { delete (SceneHandle*)h; }
+// per-primitive Robin coefficient bounds over the caller's primitives (reflecting ones are read);
+// NULL clears them (pure Neumann)
+void orc_scene_set_robin(void* hv, const float* minC, const float* maxC) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim != 2) return;
+    Mesh<2>& m = h->s2->refl;
+    m.robinMin.clear(); m.robinMax.clear();
+    if (!minC || !maxC) return;
+    for (int p = 0; p < m.nPrims(); p++) { m.robinMin.push_back(minC[m.origId[p]]); m.robinMax.push_back(maxC[m.origId[p]]); }
+}
+// Robin radius bound sqrt(min(maxR^2, Robin bounds)) at n points (brute: every segment)
+void orc_robin_radius(void* hv, int64_t n, const float* pts, const float* maxR, int brute, float* out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    for (int64_t i = 0; i < n; i++) {
+        if (h->dim != 2) { out[i] = maxR[i]; continue; }
+        float r2 = maxR[i] < FLT_MAX ? maxR[i] * maxR[i] : FLT_MAX;
+        out[i] = std::sqrt(robinRadius2(h->s2->refl, load<2>(pts + 2 * i), r2, brute != 0));
+    }
+}
 // out: [absNodes, absDepth, reflNodes, reflDepth, nSilhouetteCandidates]
 void orc_scene_stats(void* hv, int64_t* out) {
     SceneHandle* h = (SceneHandle*)hv;

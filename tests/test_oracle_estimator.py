@@ -146,3 +146,27 @@ def test_sample_types_and_shell_rules(oracle_lib):
     assert (st.n[2:] == 16).all()
     O.solve(sc, p.pde, p.settings, pts, dd, 16, st, types=types)
     assert st.n[0] == 1      # the boundary value is recorded once
+
+
+def test_robin_walk_on_stars_matches_exact(oracle_lib):
+    """Robin conditions du/dn + kappa u = h on the reflecting segments (walk_on_stars.h:277-307
+    reflectance, :372-376 epsilon radius on the boundary, Robin star-radius bound): the estimate
+    matches u = x^3 - 3xy^2; treating the same data as Neumann (reflectance 1) is detected."""
+    O = oracle_lib
+    p = PB.mixed2d_robin(kappa=2.0, n_segments=1024, n_slits=8)
+    sc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    sc.set_robin(*p.meta["robin"])
+    V, I = p.meta["outer"]
+    cand = PB.sample_box(p.domain_min, p.domain_max, 1500, 21)
+    pts = cand[PB.points_in_polygon(cand, V, I)][:200]
+    d, *_ = sc.closest_point(pts, boundary=0)
+    st = O.Stats(len(pts))
+    rep = O.solve(sc, p.pde, p.settings, pts, d, 256, st, wost=True)
+    u = p.exact(pts)
+    z = _z_vs_exact(st, u)
+    pz = _pooled_z(st, u)
+    assert abs(pz) < 4.0 and 0.8 < z.std() < 1.3, (pz, z.std())
+    pde_n = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True)
+    st2 = O.Stats(len(pts))
+    O.solve(sc, pde_n, p.settings, pts, d, 256, st2, wost=True)
+    assert abs(_pooled_z(st2, u)) > 8.0

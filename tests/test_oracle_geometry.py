@@ -253,3 +253,63 @@ def test_line_of_sight_brute_force(oracle_lib):
         p, q = rng.uniform(-0.9, 0.9, size=(2, 2)).astype(np.float32)
         blocked = any(crosses(p, q, V[i], V[j]) for i, j in I)
         assert sc.line_of_sight(p, q) == (not blocked)
+
+
+# ------------------------------------------------------------------------------------
+# Robin star-radius bound (reflectance_boundary_bvh/geometry.h:86-140, robin_bounds.h:12-35)
+# ------------------------------------------------------------------------------------
+def _robin_scene(O, kappa):
+    p = PB.mixed2d_robin(kappa=kappa, n_segments=512, n_slits=8)
+    sc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    sc.set_robin(*p.meta["robin"])
+    return p, sc
+
+
+def test_robin_bound_bvh_equals_brute_force(oracle_lib):
+    O = oracle_lib
+    p, sc = _robin_scene(O, 3.0)
+    pts = PB.sample_box(p.domain_min, p.domain_max, 3000, 12)
+    a = sc.robin_radius(pts, 10.0)
+    b = sc.robin_radius(pts, 10.0, brute=True)
+    np.testing.assert_array_equal(a, b)
+    assert np.all(a <= 10.0) and np.all(a > 0)
+    sc.set_robin(None, None)
+    np.testing.assert_array_equal(sc.robin_radius(pts, 10.0), np.full(len(pts), 10.0, np.float32))
+
+
+@pytest.mark.parametrize("kappa", [0.5, 3.0, 20.0])
+def test_robin_bound_keeps_reflectance_nonnegative(oracle_lib, kappa):
+    """Inside the bound R every point y of a Robin segment at distance r < R has
+    |cos(n, y - x)| >= kappa r log(R / r), i.e. the reflectance 1 - kappa G / P of a step from x
+    to y is >= 0 -- the property the bound exists for -- and the bound is tight: some segment
+    point of the minimising segment comes within a few percent of equality."""
+    O = oracle_lib
+    p, sc = _robin_scene(O, kappa)
+    V, I = p.positions.astype(np.float64), p.indices
+    refl = p.is_reflecting.astype(bool)
+    segs = [(V[I[k, 0]], V[I[k, 1]]) for k in np.nonzero(refl)[0]]
+    rng = np.random.default_rng(5)
+    Vo, Io = p.meta["outer"]
+    cand = PB.sample_box(p.domain_min, p.domain_max, 600, 13)
+    pts = cand[PB.points_in_polygon(cand, Vo, Io)][:120]
+    R = sc.robin_radius(pts, 10.0).astype(np.float64)
+    t = np.linspace(0.0, 1.0, 65)
+    worst_slack = []
+    for x, Rx in zip(pts.astype(np.float64), R):
+        slack_min = np.inf
+        for a, b in segs:
+            y = a[None, :] + t[:, None] * (b - a)[None, :]
+            d = y - x[None, :]
+            r = np.linalg.norm(d, axis=1)
+            m = (r < Rx) & (r > 1e-9)
+            if not m.any():
+                continue
+            n = np.array([(b - a)[1], -(b - a)[0]]) / np.linalg.norm(b - a)
+            cos = np.abs(d[m] @ n) / r[m]
+            need = kappa * r[m] * np.log(Rx / r[m])
+            assert np.all(cos >= need - 1e-4 * (1 + need)), (x, Rx, float(np.min(cos - need)))
+            slack_min = min(slack_min, float(np.min(cos - need)))
+        worst_slack.append(slack_min)
+    # tightness: for most points the minimising segment nearly touches the constraint
+    ws = np.array([s for s in worst_slack if np.isfinite(s)])
+    assert np.median(ws) < 0.05, np.median(ws)

@@ -27,6 +27,7 @@ ZG_FN_DENSE_GRID = 6
 
 ZG_EVAL_VALUE = 0
 ZG_EVAL_NORMAL_DERIVATIVE = 1
+ZG_EVAL_ROBIN = 3
 ZG_EVAL_SCREENED_SOURCE = 2
 
 ZG_SAMPLE_IN_DOMAIN = 0
@@ -191,6 +192,7 @@ SIGNATURES = {
     "zg_scene_create": (C.c_int32, [C.c_int32, _P, C.c_int64, _P, C.c_int64, _P, _P, _P, C.c_int32, C.c_float,
                                     C.POINTER(zg_build_options), C.POINTER(C.c_void_p)]),
     "zg_scene_destroy": (C.c_int32, [_P]),
+    "zg_scene_set_robin_coefficients": (C.c_int32, [_P, _P, _P]),
     "zg_scene_get_info": (C.c_int32, [_P, C.POINTER(zg_scene_info)]),
     "zg_closest_point": (C.c_int32, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P]),
     "zg_inside_domain": (C.c_int32, [_P, C.c_int64, _P, _P, _P]),

@@ -75,12 +75,14 @@ struct DeviceBoundary {
     float* silNodes = nullptr;
     float* silCones = nullptr;
     float silConePrecision = 0.0f;
-    BuiltBoundary host;  // sizes only after upload (vectors cleared)
+    float* robinMax = nullptr;   // largest Robin coefficient per packed primitive
+    BuiltBoundary host;  // sizes only after upload (vectors cleared; reflecting primIds kept)
     int64_t bytes = 0;
 };
 
 struct zg_scene {
     int dim = 3, width = 8, device = 0;
+    int64_t nPrims = 0;
     DeviceBoundary abs, refl;
     SceneView view{};
     zg_scene_info info{};
@@ -106,6 +108,7 @@ void freeBoundary(DeviceBoundary& b) {
     if (b.sil) (void)hipFree(b.sil);
     if (b.silNodes) (void)hipFree(b.silNodes);
     if (b.silCones) (void)hipFree(b.silCones);
+    if (b.robinMax) (void)hipFree(b.robinMax);
     b = DeviceBoundary();
 }
 
@@ -130,6 +133,7 @@ int64_t nodeBytes(int dim, int width) {
 
 zg_status checkFunction(const zg_function& f, int dim) {
     if (f.kind < ZG_FN_ZERO || f.kind > ZG_FN_DENSE_GRID) return fail(ZG_ERR_INVALID_ARGUMENT, "unknown function kind %d", f.kind);
+    if (f.eval < ZG_EVAL_VALUE || f.eval > ZG_EVAL_ROBIN) return fail(ZG_ERR_INVALID_ARGUMENT, "unknown eval mode %d", f.eval);
     if (f.kind != ZG_FN_DENSE_GRID) return ZG_OK;
     if (!f.grid) return fail(ZG_ERR_INVALID_ARGUMENT, "dense grid without data");
     for (int a = 0; a < 3; a++) {
@@ -260,10 +264,37 @@ zg_status zg_scene_create(int32_t dim, const float* V, int64_t nV, const int32_t
     // host copies are no longer needed
     for (DeviceBoundary* b : {&s->abs, &s->refl}) {
         std::vector<float>().swap(b->host.nodes); std::vector<float>().swap(b->host.prims);
-        std::vector<int32_t>().swap(b->host.primIds); std::vector<float>().swap(b->host.sil);
-        std::vector<float>().swap(b->host.silNodes);
+        std::vector<float>().swap(b->host.sil); std::vector<float>().swap(b->host.silNodes);
     }
+    std::vector<int32_t>().swap(s->abs.host.primIds);   // the reflecting ones map Robin coefficients
+    s->nPrims = nP;
     *out = s;
+    return ok();
+}
+
+zg_status zg_scene_set_robin_coefficients(zg_scene* s, const float* minCoeff, const float* maxCoeff) {
+    if (!s) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL scene");
+    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "Robin conditions run in 2D walk on stars");
+    if ((minCoeff == nullptr) != (maxCoeff == nullptr))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "give both coefficient arrays or neither");
+    const std::vector<int32_t>& ids = s->refl.host.primIds;
+    std::vector<float> mx(ids.size());
+    if (minCoeff) {
+        for (size_t i = 0; i < ids.size(); i++) {
+            int32_t p = ids[i];
+            if (p < 0 || p >= s->nPrims) return fail(ZG_ERR_DEVICE, "reflecting primitive map is corrupt");
+            float lo = minCoeff[p], hi = maxCoeff[p];
+            if (!(lo <= hi) || !std::isfinite(lo) || !std::isfinite(hi))
+                return fail(ZG_ERR_INVALID_ARGUMENT, "Robin coefficient bounds of primitive %d are invalid", (int)p);
+            mx[i] = hi;
+        }
+    }
+    if (s->refl.robinMax) { ZG_HIP(hipFree(s->refl.robinMax)); s->refl.robinMax = nullptr; }
+    s->view.refl.robinMax = nullptr;
+    if (!minCoeff || ids.empty()) return ok();
+    int64_t bytes = 0;
+    ZG_HIP(upload(&s->refl.robinMax, mx, bytes));
+    s->view.refl.robinMax = s->refl.robinMax;
     return ok();
 }
 
@@ -420,8 +451,6 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
             return fail(ZG_ERR_UNSUPPORTED, "splitting_threshold %g < 1: walk splitting runs in walk on spheres only",
                         ws->splitting_threshold);
         if (!s || dim != 2) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on 2D scenes");
-        if (!pde->are_robin_conditions_pure_neumann)
-            return fail(ZG_ERR_UNSUPPORTED, "walk on stars supports pure Neumann reflecting conditions (Robin coefficients 0)");
     }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))

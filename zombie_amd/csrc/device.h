@@ -364,9 +364,10 @@ ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, b
     default: break;
     }
     if (f.eval == ZG_EVAL_VALUE) return u;
-    if (f.eval == ZG_EVAL_NORMAL_DERIVATIVE) {
+    if (f.eval == ZG_EVAL_NORMAL_DERIVATIVE || f.eval == ZG_EVAL_ROBIN) {
         float dn = g0 * n[0] + g1 * n[1] + (D == 3 ? g2 * n[D == 3 ? 2 : 0] : 0.0f);
-        return aligned ? -dn : dn;
+        dn = aligned ? -dn : dn;
+        return f.eval == ZG_EVAL_ROBIN ? dn + p[5] * u : dn;
     }
     return negLap + absorption * u;
 }
@@ -1001,6 +1002,64 @@ ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius
         node = next;
     }
     return found ? fmaxf(__builtin_sqrtf(best2), minRadius) : fmaxf(maxRadius, minRadius);
+}
+
+// Robin bound of the star radius (ReflectanceLineSegment::computeSquaredStarRadius with
+// RobinLineSegmentBound, reflectance_boundary_bvh/geometry.h:86-140, robin_bounds.h:12-35): for
+// every segment with kappa_max > 0 within sqrt(r2) of x, r2 shrinks to the square of
+// (h / c) exp(c^2 / (h kappa_max)), h the distance to the segment's line and c = sqrt(h kappa_max
+// / 2) clamped to the view-angle cosines of its closest and farthest points; inside that radius
+// the Robin reflectance of the segment seen from x stays non-negative.  Pruned by box distance.
+ZG_DEV void robinSegmentBound(const Prim<2>& pr, float kmax, const Vec<2>& x, float& r2) {
+    if (!(kmax > FLT_EPSILON)) return;
+    Vec<2> cp;
+    float cd2 = pr.closest2(x, cp);
+    if (cd2 > r2) return;
+    float cd = __builtin_sqrtf(cd2);
+    if (cd < FLT_EPSILON) { r2 = fminf(r2, cd); return; }
+    Vec<2> n0 = pr.normal();
+    float h = fabsf(dot(n0, pr.a - x));
+    if (!(h > 0.0f)) { r2 = fminf(r2, cd2); return; }
+    float da = __builtin_sqrtf(norm2(x - pr.a)), db = __builtin_sqrtf(norm2(x - pr.b));
+    Vec<2> fp = da >= db ? pr.a : pr.b;
+    float fd = fmaxf(da, db);
+    float cosUpper = fabsf(dot(x - cp, n0)) / cd;
+    float cosLower = fabsf(dot(x - fp, n0)) / fd;
+    float cosLine = __builtin_sqrtf(h * kmax) * 0.70710678f;
+    float c = fminf(fmaxf(cosLine, cosLower), cosUpper);
+    float rad = (h / c) * __expf(c * c / (h * kmax));
+    r2 = fminf(r2, rad * rad);
+}
+
+template <int W>
+ZG_DEV float robinRadius2(const BoundaryView& b, const Vec<2>& x, float r2, Stack& st) {
+    if (b.nNodes == 0 || !b.robinMax) return r2;
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<2, W> nd;
+        nd.load(b.nodes, node);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (nd.dist2(k, x) > r2) continue;
+            int c = nd.ch[k];
+            if (isLeaf(c)) {
+                int first = leafFirstD(c), cnt = leafCountD(c);
+                for (int j = first; j < first + cnt; j++) {
+                    Prim<2> pr; pr.load(b.prims, j);
+                    robinSegmentBound(pr, b.robinMax[j], x, r2);
+                }
+            } else {
+                st.push(c, nd.dist2(k, x));
+            }
+        }
+        int n; float dn;
+        bool got = false;
+        while (st.pop(n, dn)) { if (dn <= r2) { got = true; break; } }
+        if (!got) break;
+        node = n;
+    }
+    return r2;
 }
 
 // hasLineOfSight: true unless a segment of b crosses the open segment p1 -> p2.

@@ -51,6 +51,7 @@ struct BoundaryView {
     const float* silNodes; // 2D reflecting: wide BVH over silhouette candidates
     const float* silCones; // per silhouette-BVH child: normal cone (axis.xy, sin, cos of half-angle + margin)
     float silConePrecision;// the silhouette precision the cones' margin covers
+    const float* robinMax; // 2D reflecting: largest Robin coefficient per packed primitive (null: 0)
     int32_t nNodes;
     int32_t nPrims;
     int32_t nSil;

@@ -1,4 +1,4 @@
-// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D, Neumann reflecting boundary) for
+// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D, Neumann or Robin reflecting boundary) for
 // gfx950, its per-point first-sphere prologue, and the batched star-radius, line-of-sight and
 // reflecting-boundary sampling queries.
 //
@@ -14,8 +14,9 @@
 // Per walk the lane restates WalkOnStars::walk (point_estimation/walk_on_stars.h:345-490)
 // with estimateSolution's per-point rules (:518-662), computeReflectingBoundaryContribution
 // (:189-250), computeSourceContribution (:252-275) and computeWalkStepThroughput (:277-307)
-// for pure Neumann conditions and the harmonic Green's function (reflectance 1, Poisson
-// kernel 1).
+// for the harmonic Green's function: reflectance 1 under Neumann conditions, 1 - kappa G / P
+// under Robin conditions (the star radius then also respects the Robin bound of every segment
+// and shrinks to the reflecting epsilon on the Robin boundary).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -27,16 +28,28 @@ namespace zg {
 
 // star radius of a walk: closest silhouette within [epsR, distA], shrunk by 1 % for numerical
 // robustness when the reflecting boundary is not empty (walk_on_stars.h:370-392,576-590)
-template <int W>
+template <int W, bool ROBIN = true>
 ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, const Vec<2>& x, float distA, bool flip,
                             Stack& st) {
     float r = distA;
     if (sc.refl.nPrims > 0) {
-        r = starRadius2<W>(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip, st);
-        if (ws.epsilon_shell_for_reflecting_boundary <= distA)
-            r = fmaxf(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
+        const float minR = ws.epsilon_shell_for_reflecting_boundary;
+        r = starRadius2<W>(sc.refl, x, minR, distA, ws.silhouette_precision, flip, st);
+        // Robin segments bound the radius too (fcpw_geometric_queries.h:1272-1289)
+        if (ROBIN && sc.refl.robinMax && minR <= distA)
+            r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), minR);
+        if (minR <= distA) r = fmaxf(0.99f * r, minR);
     }
     return r;
+}
+
+// PDE::hasNonZeroRobinCoeff (pde.h:62-78)
+ZG_DEV bool hasNonZeroRobinCoeff(const zg_pde& pde, const Vec<2>& x) {
+    if (pde.are_robin_conditions_pure_neumann) return false;
+    Vec<2> n = vzero<2>();
+    n[0] = 1.0f;
+    return fabsf(evalField<2>(pde.robin_coeff, x, n, true, 0.0f)) > 0.0f ||
+           fabsf(evalField<2>(pde.robin_coeff, x, n, false, 0.0f)) > 0.0f;
 }
 
 // first sphere radius per point (estimateSolution :566-590); a point with earlier estimates
@@ -55,6 +68,8 @@ __global__ void __launch_bounds__(256) wostFirstSphereKernel(WosParams P, const 
     if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) r = 0.0f;
     else if (callerFsr && P.prevN && P.prevN[i] > 0) r = callerFsr[i];
     else if (d <= ws.epsilon_shell_for_absorbing_boundary || ws.steps_before_using_maximal_spheres == 0) r = d;
+    else if (ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && hasNonZeroRobinCoeff(P.pde, vload<2>(P.pt + 2 * i)))
+        r = ws.epsilon_shell_for_reflecting_boundary;
     else {
         bool flip = ws.solve_double_sided && ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && P.aligned && P.aligned[i];
         r = wostStarRadius<W>(P.scene, ws, vload<2>(P.pt + 2 * i), d, flip, st);
@@ -87,7 +102,8 @@ __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, fl
 // GRAD = true: estimateSolutionAndGradient (walk_on_stars.h:664-848) for the points asking for
 // it -- the explicit first step of the walk-on-spheres gradient walks inside the first ball (no
 // boundary in it), then walk-on-stars steps from the boundary point.
-template <int W, bool COUNT, bool GRAD = false>
+// ROBIN = true compiles the Robin reflectance and star-radius bound in (non-Neumann PDEs).
+template <int W, bool COUNT, bool GRAD = false, bool ROBIN = false>
 __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
@@ -98,6 +114,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     const float epsA = ws.epsilon_shell_for_absorbing_boundary;
     const float prec = ws.silhouette_precision;
     const bool hasRefl = sc.refl.nPrims > 0;
+    const bool robin = ROBIN && !pde.are_robin_conditions_pure_neumann;
     const int lane = __lane_id();
     const uint32_t streamOffset = P.streamOffsetDev ? *P.streamOffsetDev : P.streamOffset;
 
@@ -313,7 +330,9 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 nrm = nrm * -1.0f;
                 flip = true;
             }
-            R = ws.steps_before_using_maximal_spheres <= len ? distA : wostStarRadius<W>(sc, ws, x, distA, flip, st);
+            if (ws.steps_before_using_maximal_spheres <= len) R = distA;
+            else if (robin && onRefl && hasNonZeroRobinCoeff(pde, x)) R = ws.epsilon_shell_for_reflecting_boundary;
+            else R = wostStarRadius<W, ROBIN>(sc, ws, x, distA, flip, st);
         }
         ball.update(R);
         Vec<2> dir = sampleUnitSphere<2>(rng);
@@ -375,12 +394,25 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             if (r <= idist) Ssrc += T * ball.norm() * evalField<2>(pde.source, x + dir * r, vzero<2>(), false, 0.0f);
         }
 
-        // move; throughput is 1 for pure Neumann conditions and the harmonic kernel
+        // move; throughput (computeWalkStepThroughput :277-307): 1 for Neumann conditions and
+        // the harmonic kernel, the clamped Robin reflectance 1 - kappa G / P after a boundary hit
         prevDist = idist;
         prevDir = dir;
         x = ipt;
         nrm = inrm;
         onRefl = hit;
+        if (robin && onRefl && prevDist > FLT_EPSILON) {
+            bool fl = ws.solve_double_sided && dot(prevDir, nrm) < 0.0f;
+            Vec<2> nr = fl ? nrm * -1.0f : nrm;
+            float kappa = evalField<2>(pde.robin_coeff, x, nrm, fl, 0.0f);
+            float refl = 1.0f;
+            if (fabsf(kappa) > 0.0f) {
+                float Pk = dot(nr, prevDir) / prevDist;
+                float G = __logf(R / prevDist);
+                refl = 1.0f - kappa * G / Pk;
+            }
+            T *= fminf(fmaxf(refl, 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
+        }
         if (!onRefl && outsideDomain<2>(sc.dmin, sc.dmax, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
         if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :309-343
             float survival = T / ws.russian_roulette_threshold;
@@ -424,7 +456,13 @@ __global__ void __launch_bounds__(256) starRadiusKernel(SceneView sv, int64_t n,
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
     bool fl = flip && flip[i] != 0;
-    radius[i] = sv.refl.nPrims == 0 ? maxR[i] : starRadius2<W>(sv.refl, vload<2>(pts + 2 * i), minR[i], maxR[i], prec, fl, st);
+    Vec<2> x = vload<2>(pts + 2 * i);
+    float r = sv.refl.nPrims == 0 ? maxR[i] : starRadius2<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
+    if (sv.refl.robinMax && minR[i] <= maxR[i]) {
+        float m = maxR[i];
+        r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sv.refl, x, m < FLT_MAX ? m * m : FLT_MAX, st))), minR[i]);
+    }
+    radius[i] = r;
 }
 
 template <int W>
@@ -476,20 +514,29 @@ hipError_t launchWostGradFirstSphere(const WosParams& P, int width, float* fsr, 
 
 hipError_t launchWostGrad(const WosParams& P, int width, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-    if (width == 4) hipLaunchKernelGGL((wostKernel<4, false, true>), g, bl, kStackLdsBytes, s, P);
-    else hipLaunchKernelGGL((wostKernel<8, false, true>), g, bl, kStackLdsBytes, s, P);
+    const bool robin = !P.pde.are_robin_conditions_pure_neumann;
+    if (width == 4) {
+        if (robin) hipLaunchKernelGGL((wostKernel<4, false, true, true>), g, bl, kStackLdsBytes, s, P);
+        else hipLaunchKernelGGL((wostKernel<4, false, true, false>), g, bl, kStackLdsBytes, s, P);
+    } else {
+        if (robin) hipLaunchKernelGGL((wostKernel<8, false, true, true>), g, bl, kStackLdsBytes, s, P);
+        else hipLaunchKernelGGL((wostKernel<8, false, true, false>), g, bl, kStackLdsBytes, s, P);
+    }
     return hipGetLastError();
 }
 
 hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
+    const bool robin = !P.pde.are_robin_conditions_pure_neumann;
+#define ZG_WOST(Wd, C, R) hipLaunchKernelGGL((wostKernel<Wd, C, false, R>), g, bl, kStackLdsBytes, s, P)
     if (width == 4) {
-        if (count) hipLaunchKernelGGL((wostKernel<4, true>), g, bl, kStackLdsBytes, s, P);
-        else hipLaunchKernelGGL((wostKernel<4, false>), g, bl, kStackLdsBytes, s, P);
+        if (robin) { if (count) ZG_WOST(4, true, true); else ZG_WOST(4, false, true); }
+        else { if (count) ZG_WOST(4, true, false); else ZG_WOST(4, false, false); }
     } else {
-        if (count) hipLaunchKernelGGL((wostKernel<8, true>), g, bl, kStackLdsBytes, s, P);
-        else hipLaunchKernelGGL((wostKernel<8, false>), g, bl, kStackLdsBytes, s, P);
+        if (robin) { if (count) ZG_WOST(8, true, true); else ZG_WOST(8, false, true); }
+        else { if (count) ZG_WOST(8, true, false); else ZG_WOST(8, false, false); }
     }
+#undef ZG_WOST
     return hipGetLastError();
 }
 

@@ -90,16 +90,18 @@ def zero() -> A.zg_function:
 
 
 def make_pde(source=None, dirichlet=None, robin=None, absorption: float = 0.0,
-             is_source_constant: bool = False) -> A.zg_pde:
+             is_source_constant: bool = False, robin_coeff=None) -> A.zg_pde:
+    """PDE<float,DIM> (pde.h:19-45).  A robin_coeff descriptor makes the reflecting conditions
+    Robin (are_robin_conditions_pure_neumann = 0; coefficients taken as non-negative)."""
     p = A.zg_pde()
     p.absorption_coeff = float(absorption)
     p.is_source_constant = 1 if is_source_constant else 0
-    p.are_robin_conditions_pure_neumann = 1
+    p.are_robin_conditions_pure_neumann = 1 if robin_coeff is None else 0
     p.are_robin_coeffs_nonnegative = 1
     p.source = source if source is not None else zero()
     p.dirichlet = dirichlet if dirichlet is not None else zero()
     p.robin = robin if robin is not None else zero()
-    p.robin_coeff = zero()
+    p.robin_coeff = robin_coeff if robin_coeff is not None else zero()
     return p
 
 
@@ -247,6 +249,15 @@ class Scene:
         A.check(A.lib().zg_intersect_ray(self._h, boundary, n, _ptr(origins), _ptr(dirs), _ptr(t_max), _ptr(hit),
                                          _ptr(t), _ptr(p), _ptr(nr), _stream_ptr()))
         return hit, t, p, nr
+
+    def set_robin_coefficients(self, min_coeff, max_coeff):
+        """Per-primitive Robin coefficient bounds (host arrays over all primitives; None clears)."""
+        if min_coeff is None:
+            A.check(A.lib().zg_scene_set_robin_coefficients(self.handle, None, None))
+            return
+        lo = np.ascontiguousarray(min_coeff, np.float32)
+        hi = np.ascontiguousarray(max_coeff, np.float32)
+        A.check(A.lib().zg_scene_set_robin_coefficients(self.handle, lo.ctypes.data, hi.ctypes.data))
 
     # walk-on-stars queries on the reflecting boundary (2D)
     def star_radius(self, pts, min_r, max_r, precision: float = 1e-3, flip=None):

@@ -289,6 +289,22 @@ def mixed2d(n_segments=4096, n_slits=32, seed=0x5EED, n_walks=1024, eps=1e-3, ma
     return prob
 
 
+def mixed2d_robin(kappa=2.0, n_segments=1024, n_slits=8, seed=0x5EED, n_walks=256, eps=1e-3,
+                  max_len=16384) -> Problem:
+    """mixed2d with Robin conditions du/dn + kappa u = h on every reflecting segment (the upper
+    half of the star curve and both sides of the slits), u = x^3 - 3xy^2.  meta['robin'] holds
+    the per-primitive coefficient bounds (kappa on reflecting primitives)."""
+    p = mixed2d(n_segments=n_segments, n_slits=n_slits, seed=seed, n_walks=n_walks, eps=eps, max_len=max_len)
+    h = E.function(A.ZG_FN_CUBIC_HARMONIC, A.ZG_EVAL_ROBIN, (1.0, 0.0, 0.0, 0.0, 0.0, kappa))
+    p.pde = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=h, is_source_constant=True,
+                       robin_coeff=E.constant(kappa))
+    k = np.where(p.is_reflecting.astype(bool), np.float32(kappa), np.float32(0.0)).astype(np.float32)
+    p.meta["robin"] = (k, k)
+    p.meta["kappa"] = kappa
+    p.name = "mixed2d_robin"
+    return p
+
+
 def _segments_close(p0, p1, q0, q1, tol):
     def seg_dist(a, b, c):
         ab = b - a
