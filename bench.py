@@ -50,7 +50,10 @@ def parse():
     ap.add_argument("--step-interval", type=int, default=16)
     ap.add_argument("--tolerance", type=float, default=0.2, help="distance tolerance when not autotuning")
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the launch-shape autotuner first (the defaults are its usual choice for this "
+                         "workload, and the configuration of the committed kernel profile)")
+    ap.add_argument("--no-tune", action="store_true", help="(default) use the launch shape given by the flags")
     ap.add_argument("--no-learned-cv", action="store_true")
     ap.add_argument("--cv-train-points", type=int, default=2_000_000)
     ap.add_argument("--cv-train-walks", type=int, default=32)
@@ -126,7 +129,7 @@ def main():
     tune = None
     scenes = {(args.bvh_width, args.leaf_size): scene}
     leaf, tol = args.leaf_size, args.tolerance
-    if not args.no_tune:
+    if args.tune and not args.no_tune:
         tune = AT.autotune(prob, dp, scenes)
         for row in tune.table:
             log("tune: width {bvh_width} leaf {leaf_size} wpl {walks_per_lane:2d} regroup {regroup_interval:2d} "

@@ -364,6 +364,60 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
 zg_status zg_stratified_samples(int32_t dims, int64_t n_tables, int32_t n_samples, uint64_t seed,
                                 uint32_t stream_offset, float* out, void* stream);
 
+/* ------------------------------------------------------------------------------------ */
+/* Boundary value caching (variance_reduction/boundary_value_caching.h): splatting cached  */
+/* boundary and domain samples into evaluation points with the free-space kernels.         */
+/* ------------------------------------------------------------------------------------ */
+/* A sample cache (SamplePoint fields the splat reads), structure of arrays, device pointers. */
+typedef struct zg_bvc_samples {
+    int64_t n;
+    const float* pt;                 /* [n*dim]                                               */
+    const float* normal;             /* [n*dim] unit normals of boundary samples               */
+    const uint8_t* type;             /* [n] ZG_SAMPLE_ON_ABSORBING_BOUNDARY, _ON_REFLECTING_    */
+                                     /*     BOUNDARY, or ZG_SAMPLE_IN_DOMAIN (source sample)   */
+    const uint8_t* normal_aligned;   /* [n] nullable: estimateBoundaryNormalAligned           */
+    const float* pdf;                /* [n] sampling density of the sample                     */
+    const float* solution;           /* [n] estimated (or known) solution at the sample         */
+    const float* normal_derivative;  /* [n] estimated or known normal derivative               */
+    const float* contribution;       /* [n] Robin data h (reflecting) or source value (domain)  */
+    const float* robin_coeff;        /* [n] nullable (0)                                      */
+} zg_bvc_samples;
+
+/* Evaluation points (EvaluationPoint, boundary_value_caching.h:18-56), device pointers. */
+typedef struct zg_bvc_eval_points {
+    int64_t n;
+    const float* pt;                 /* [n*dim]                                               */
+    const uint8_t* type;             /* [n] nullable (all ZG_SAMPLE_IN_DOMAIN)                  */
+    const float* dist_to_absorbing_boundary;   /* [n]                                        */
+    const float* dist_to_reflecting_boundary;  /* [n] nullable (infinite)                     */
+} zg_bvc_eval_points;
+
+/* Per evaluation point and category, FP64, Chan-merged across calls; categories in the order
+ * absorbing, absorbing normal-aligned, reflecting, reflecting normal-aligned, source (the five
+ * SampleStatistics of EvaluationPoint).  The solution at a point is the sum of the five means
+ * (getEstimatedSolution), its gradient the sum of the five gradient means. */
+#define ZG_BVC_CATEGORIES 5
+typedef struct zg_bvc_statistics {
+    int64_t* n;                      /* [n_eval*5]                                            */
+    double* solution_mean;           /* [n_eval*5]                                            */
+    double* solution_m2;             /* [n_eval*5]                                            */
+    double* gradient_mean;           /* [n_eval*5*dim]                                        */
+    double* gradient_m2;             /* [n_eval*5*dim]                                        */
+} zg_bvc_statistics;
+
+/* BoundaryValueCaching::splat over every sample for every evaluation point
+ * (boundary_value_caching.h:370-402,683-810): free-space harmonic Green's function G, Poisson
+ * kernel P and their gradients at r = max(radius_clamp, |y - x|), optional kernel regularization
+ * (distributions.h:12-55), the Robin / Neumann / Dirichlet forms of the boundary estimate,
+ * alpha = 2 for evaluation points on the boundary (gradient 0 there); evaluation points closer
+ * than the cutoffs to the boundaries are skipped; non-finite kernels skip the sample.  One
+ * thread per evaluation point, samples staged through LDS.  absorption must be 0 (the harmonic
+ * kernels; ZG_ERR_UNSUPPORTED otherwise). */
+zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* samples_host,
+                       const zg_bvc_eval_points* eval_host, float radius_clamp, float kernel_regularization,
+                       float robin_coeff_cutoff_for_normal_derivative, float cutoff_dist_to_absorbing_boundary,
+                       float cutoff_dist_to_reflecting_boundary, zg_bvc_statistics* stats_host, void* stream);
+
 /* Evaluates a PDE data descriptor at n points (the PDE<T,DIM> callbacks source / dirichlet /
  * robin of pde.h:30-44, as the solvers evaluate them): out[i] = f(pts[i], normals[i],
  * aligned[i]) with the absorption coefficient used by ZG_EVAL_SCREENED_SOURCE.  normals and

@@ -63,6 +63,9 @@ def lib():
                                      P, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P, P, P, P,
                                      P, P, P, P, P, P, P]
         L.orc_solve_grad.restype = C.c_int
+        L.orc_bvc_splat.argtypes = [C.c_int, C.POINTER(A.zg_bvc_samples), C.POINTER(A.zg_bvc_eval_points), C.c_float,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P, P, C.c_int]
+        L.orc_free_space_kernels.argtypes = [C.c_int, P, P, P, C.c_float, C.c_float, P]
         L.orc_stratified_samples.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_uint64, C.c_uint32, P]
         L.orc_pcg32_bounded.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, P]
         L.orc_greens_ball_gradient.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, P]
@@ -119,6 +122,62 @@ def gradient_direction(dim, cosine, u, n=None):
     out = np.empty(dim + 1, np.float32)
     lib().orc_gradient_direction(dim, 1 if cosine else 0, _p(u), _p(nn), _p(out))
     return out[:dim].copy(), float(out[dim])
+
+
+def free_space_kernels(dim, x, y, n, r, kreg=0.0):
+    """[G, P, dG, dP, G_reg, P_reg] of the free-space harmonic kernels (pole x, sample y, normal n)."""
+    out = np.zeros(2 + 2 * dim + 2, np.float32)
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    xx, yy, nn = f(x), f(y), f(n)
+    lib().orc_free_space_kernels(dim, _p(xx), _p(yy), _p(nn), r, kreg, _p(out))
+    return dict(G=float(out[0]), P=float(out[1]), dG=out[2:2 + dim].copy(), dP=out[2 + dim:2 + 2 * dim].copy(),
+                G_reg=float(out[2 + 2 * dim]), P_reg=float(out[3 + 2 * dim]))
+
+
+class BvcStats:
+    def __init__(self, n, dim):
+        self.dim = dim
+        self.n = np.zeros((n, 5), np.int64)
+        self.mean = np.zeros((n, 5), np.float64)
+        self.m2 = np.zeros((n, 5), np.float64)
+        self.grad_mean = np.zeros((n, 5, dim), np.float64)
+        self.grad_m2 = np.zeros((n, 5, dim), np.float64)
+
+    def solution(self):
+        return self.mean.sum(1)
+
+    def gradient(self):
+        return self.grad_mean.sum(1)
+
+
+def bvc_splat(dim, samples: dict, evals: dict, stats: BvcStats, radius_clamp=1e-3, kernel_regularization=0.0,
+              robin_cutoff=np.inf, cutoff_abs=0.0, cutoff_refl=0.0, threads=None):
+    """samples: pt, normal, type, normal_aligned, pdf, solution, normal_derivative, contribution, robin_coeff
+    (numpy; the last two and normal_aligned optional); evals: pt, type, dist_abs, dist_refl."""
+    keep = []
+    def arr(a, dt):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+    S = A.zg_bvc_samples()
+    S.n = len(samples["pt"])
+    S.pt = arr(samples["pt"], np.float32); S.normal = arr(samples["normal"], np.float32)
+    S.type = arr(samples["type"], np.uint8); S.normal_aligned = arr(samples.get("normal_aligned"), np.uint8)
+    S.pdf = arr(samples["pdf"], np.float32); S.solution = arr(samples["solution"], np.float32)
+    S.normal_derivative = arr(samples["normal_derivative"], np.float32)
+    S.contribution = arr(samples.get("contribution"), np.float32)
+    S.robin_coeff = arr(samples.get("robin_coeff"), np.float32)
+    Ev = A.zg_bvc_eval_points()
+    Ev.n = len(evals["pt"])
+    Ev.pt = arr(evals["pt"], np.float32); Ev.type = arr(evals.get("type"), np.uint8)
+    Ev.dist_to_absorbing_boundary = arr(evals["dist_abs"], np.float32)
+    Ev.dist_to_reflecting_boundary = arr(evals.get("dist_refl"), np.float32)
+    lib().orc_bvc_splat(dim, C.byref(S), C.byref(Ev), radius_clamp, kernel_regularization,
+                        float(min(robin_cutoff, 3.0e38)), cutoff_abs, cutoff_refl, _p(stats.n), _p(stats.mean),
+                        _p(stats.m2), _p(stats.grad_mean), _p(stats.grad_m2), threads or default_threads())
+    return stats
 
 
 def sample_unit_sphere(dim, seed, n):

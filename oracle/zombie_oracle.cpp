@@ -1756,6 +1756,98 @@ void buildScene(Scene<D>& sc, const float* V_, int64_t nV, const int32_t* I, int
     sc.dmin = load<D>(dmin); sc.dmax = load<D>(dmax);
 }
 
+// ------------------------------------------------------------------------------------
+// boundary value caching: free-space harmonic kernels (distributions.h:84-177), kernel
+// regularization (:12-55) and the splat (variance_reduction/boundary_value_caching.h:370-402,
+// 683-810)
+// ------------------------------------------------------------------------------------
+template <int D>
+struct FreeSpaceKernels {
+    float G, P;
+    V<D> dG, dP;
+    // pole x, sample y with normal n, r = clamped distance
+    void eval(const V<D>& x, const V<D>& y, const V<D>& n, float r) {
+        V<D> xy = x - y;
+        float r2 = r * r;
+        if (D == 2) {
+            G = -std::log(r) / (2.0f * PI_F);
+            dG = xy * (-1.0f / (2.0f * PI_F * r2));
+            P = dot(n, xy) / (2.0f * PI_F * r2);
+            dP = (n - xy * (2.0f * (dot(n, xy) / r2))) * (1.0f / (2.0f * PI_F * r2));
+        } else {
+            float r3 = r2 * r;
+            G = 1.0f / (4.0f * PI_F * r);
+            dG = xy * (-1.0f / (4.0f * PI_F * r3));
+            P = dot(n, xy) / (4.0f * PI_F * r3);
+            dP = (n - xy * (3.0f * (dot(n, xy) / r2))) * (1.0f / (4.0f * PI_F * r3));
+        }
+    }
+};
+template <int D> float regularizationG(float r) { return D == 2 ? 1.0f : std::erf(r); }
+template <int D> float regularizationP(float r) {
+    if (D == 2) return 1.0f - std::exp(-r * r);
+    return std::erf(r) - 2.0f * r * std::exp(-r * r) / std::sqrt((float)M_PI);
+}
+
+struct BvcAcc {   // Welford per category
+    int64_t n = 0; double mean = 0.0, m2 = 0.0; double gm[3] = {0, 0, 0}, g2[3] = {0, 0, 0};
+    void add(double x, const double* g, int D) {
+        n++;
+        double d = x - mean; mean += d / (double)n; m2 += d * (x - mean);
+        for (int i = 0; i < D; i++) { double e = g[i] - gm[i]; gm[i] += e / (double)n; g2[i] += e * (g[i] - gm[i]); }
+    }
+};
+
+template <int D>
+void bvcSplatPoint(const zg_bvc_samples& S, const V<D>& x, bool evalOnBoundary, float radiusClamp, float kreg,
+                   float robinCutoff, BvcAcc acc[5]) {
+    const float alpha = evalOnBoundary ? 2.0f : 1.0f;
+    const float alphaG = evalOnBoundary ? 0.0f : alpha;
+    for (int64_t j = 0; j < S.n; j++) {
+        V<D> y = load<D>(S.pt + j * D);
+        uint8_t ty = S.type[j];
+        float pdf = S.pdf[j];
+        bool aligned = S.normal_aligned && S.normal_aligned[j];
+        float r = std::max(radiusClamp, norm(y - x));
+        FreeSpaceKernels<D> k;
+        V<D> n = V<D>::zero();
+        if (ty != ZG_SAMPLE_IN_DOMAIN) n = load<D>(S.normal + j * D) * (aligned ? -1.0f : 1.0f);
+        k.eval(x, y, n, r);
+        double est, grad[3];
+        if (ty == ZG_SAMPLE_IN_DOMAIN) {
+            float dGn = norm(k.dG);
+            if (std::isinf(k.G) || std::isnan(k.G) || std::isinf(dGn) || std::isnan(dGn)) continue;
+            float G = k.G;
+            if (kreg > 0.0f) G *= regularizationG<D>(r / kreg);
+            float src = S.contribution[j];
+            est = alpha * G * src / pdf;
+            for (int i = 0; i < D; i++) grad[i] = alphaG * k.dG[i] * src / pdf;
+            acc[4].add(est, grad, D);
+            continue;
+        }
+        float dGn = norm(k.dG), dPn = norm(k.dP);
+        if (std::isinf(k.G) || std::isinf(k.P) || std::isinf(dGn) || std::isinf(dPn) ||
+            std::isnan(k.G) || std::isnan(k.P) || std::isnan(dGn) || std::isnan(dPn)) continue;
+        float G = k.G, P = k.P;
+        if (kreg > 0.0f) { float rr = r / kreg; G *= regularizationG<D>(rr); P *= regularizationP<D>(rr); }
+        float u = S.solution[j], dn = S.normal_derivative[j];
+        float h = S.contribution ? S.contribution[j] : 0.0f;
+        float kappa = S.robin_coeff ? S.robin_coeff[j] : 0.0f;
+        if (std::fabs(kappa) > robinCutoff) {
+            est = alpha * ((G + P / kappa) * dn - P * h / kappa) / pdf;
+            for (int i = 0; i < D; i++) grad[i] = alphaG * ((k.dG[i] + k.dP[i] / kappa) * dn - k.dP[i] * h / kappa) / pdf;
+        } else if (std::fabs(kappa) > 0.0f) {
+            est = alpha * (G * h - (P + kappa * G) * u) / pdf;
+            for (int i = 0; i < D; i++) grad[i] = alphaG * (k.dG[i] * h - (k.dP[i] + kappa * k.dG[i]) * u) / pdf;
+        } else {
+            est = alpha * (G * dn - P * u) / pdf;
+            for (int i = 0; i < D; i++) grad[i] = alphaG * (k.dG[i] * dn - k.dP[i] * u) / pdf;
+        }
+        int cat = (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY ? 0 : 2) + (aligned ? 1 : 0);
+        acc[cat].add(est, grad, D);
+    }
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1931,6 +2023,52 @@ void* orc_scene_create(int dim, const float* V_, int64_t nV, const int32_t* I, i
     return h;
 }
 void orc_scene_destroy(void* h) { delete (SceneHandle*)h; }
+// BoundaryValueCaching::splat of a sample cache into evaluation points (host arrays); stats are
+// Chan-merged like zg_bvc_statistics
+int orc_bvc_splat(int dim, const zg_bvc_samples* S, const zg_bvc_eval_points* E, float radiusClamp, float kreg,
+                  float robinCutoff, float cutoffA, float cutoffR, int64_t* sn, double* smean, double* sm2,
+                  double* sgm, double* sg2, int nThreads) {
+    parallelFor(E->n, std::max(1, nThreads), [&](int64_t i, int) {
+        float dA = E->dist_to_absorbing_boundary[i];
+        float dR = E->dist_to_reflecting_boundary ? E->dist_to_reflecting_boundary[i] : FLT_MAX;
+        if (dA < cutoffA || dR < cutoffR) return;
+        uint8_t ty = E->type ? E->type[i] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+        bool onB = ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY || ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY;
+        BvcAcc acc[5];
+        if (dim == 2) bvcSplatPoint<2>(*S, load<2>(E->pt + 2 * i), onB, radiusClamp, kreg, robinCutoff, acc);
+        else bvcSplatPoint<3>(*S, load<3>(E->pt + 3 * i), onB, radiusClamp, kreg, robinCutoff, acc);
+        for (int c = 0; c < 5; c++) {
+            const BvcAcc& a = acc[c];
+            if (a.n == 0) continue;
+            size_t k = (size_t)i * 5 + c;
+            int64_t N1 = sn[k], N = N1 + a.n;
+            double f2 = (double)a.n / (double)N, f12 = (double)N1 * (double)a.n / (double)N;
+            if (N1 == 0) { smean[k] = a.mean; sm2[k] = a.m2; }
+            else { double d = a.mean - smean[k]; smean[k] += d * f2; sm2[k] += a.m2 + d * d * f12; }
+            for (int g = 0; g < dim; g++) {
+                size_t kg = k * dim + g;
+                if (N1 == 0) { sgm[kg] = a.gm[g]; sg2[kg] = a.g2[g]; }
+                else { double d = a.gm[g] - sgm[kg]; sgm[kg] += d * f2; sg2[kg] += a.g2[g] + d * d * f12; }
+            }
+            sn[k] = N;
+        }
+    });
+    return 0;
+}
+// free-space kernels at (x, y, n, r) for pins: out = [G, P, dG..., dP...], regularized G and P in
+// out[2 + 2 dim], out[3 + 2 dim] (kreg > 0)
+void orc_free_space_kernels(int dim, const float* x, const float* y, const float* n, float r, float kreg, float* out) {
+    if (dim == 2) {
+        FreeSpaceKernels<2> k; k.eval(load<2>(x), load<2>(y), load<2>(n), r);
+        out[0] = k.G; out[1] = k.P; store<2>(out + 2, k.dG); store<2>(out + 4, k.dP);
+        out[6] = kreg > 0 ? k.G * regularizationG<2>(r / kreg) : k.G; out[7] = kreg > 0 ? k.P * regularizationP<2>(r / kreg) : k.P;
+    } else {
+        FreeSpaceKernels<3> k; k.eval(load<3>(x), load<3>(y), load<3>(n), r);
+        out[0] = k.G; out[1] = k.P; store<3>(out + 2, k.dG); store<3>(out + 5, k.dP);
+        out[8] = kreg > 0 ? k.G * regularizationG<3>(r / kreg) : k.G; out[9] = kreg > 0 ? k.P * regularizationP<3>(r / kreg) : k.P;
+    }
+}
+
 // per-primitive Robin coefficient bounds over the caller's primitives (reflecting ones are read);
 // NULL clears them (pure Neumann)
 void orc_scene_set_robin(void* hv, const float* minC, const float* maxC) {

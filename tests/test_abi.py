@@ -47,7 +47,8 @@ def test_invalid_arguments_are_reported_not_fatal():
 
 
 STRUCTS = ["zg_walk_settings", "zg_function", "zg_pde", "zg_build_options", "zg_scene_info", "zg_sample_points",
-           "zg_statistics", "zg_solve_options", "zg_solve_report", "zg_grid_field", "zg_mlp_config"]
+           "zg_statistics", "zg_solve_options", "zg_solve_report", "zg_grid_field", "zg_mlp_config",
+           "zg_bvc_samples", "zg_bvc_eval_points", "zg_bvc_statistics"]
 
 
 def test_ctypes_layouts_match_c(tmp_path):

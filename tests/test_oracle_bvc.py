@@ -1,0 +1,137 @@
+"""Pins of the oracle's boundary-value-caching splat (variance_reduction/boundary_value_caching.h:
+370-402, 683-810) against mathematics: the free-space kernels against finite differences and
+closed forms, and the splat of exact boundary data against Green's representation formula
+u(x) = oint (G du/dn - u dG/dn_y) + int G f on the unit disk and ball."""
+import math
+
+import numpy as np
+import pytest
+
+from zombie_amd import _abi as A
+
+
+def _fd_grad(f, x, h=1e-3):
+    g = np.zeros(len(x))
+    for a in range(len(x)):
+        e = np.zeros(len(x)); e[a] = h
+        g[a] = (f(x + e) - f(x - e)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_free_space_kernels(oracle_lib, dim):
+    """G is the fundamental solution (-log r / 2 pi, 1 / 4 pi r); P = n . grad_y G; the gradients
+    in the pole x match central differences; the regularizations are the published forms."""
+    O = oracle_lib
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        x = rng.normal(size=dim) * 0.5
+        y = rng.normal(size=dim) * 0.5
+        n = rng.normal(size=dim); n /= np.linalg.norm(n)
+        r = float(np.linalg.norm(x - y))
+        k = O.free_space_kernels(dim, x, y, n, r)
+        Gx = (lambda z: -math.log(np.linalg.norm(z - y)) / (2 * math.pi)) if dim == 2 else \
+             (lambda z: 1.0 / (4 * math.pi * np.linalg.norm(z - y)))
+        assert abs(k["G"] - Gx(x)) < 1e-5 * (1 + abs(Gx(x)))
+        # P = n . grad_y G(x, y)
+        Gy = (lambda z: -math.log(np.linalg.norm(x - z)) / (2 * math.pi)) if dim == 2 else \
+             (lambda z: 1.0 / (4 * math.pi * np.linalg.norm(x - z)))
+        P = float(np.dot(n, _fd_grad(Gy, y)))
+        assert abs(k["P"] - P) < 2e-3 * (1 + abs(P)), (k["P"], P)
+        np.testing.assert_allclose(k["dG"], _fd_grad(Gx, x), rtol=3e-3, atol=1e-4)
+        Px = lambda z: float(O.free_space_kernels(dim, z, y, n, float(np.linalg.norm(z - y)))["P"])
+        np.testing.assert_allclose(k["dP"], _fd_grad(Px, x, 1e-3), rtol=5e-3, atol=2e-3)
+    # regularization factors at r / kreg
+    k = O.free_space_kernels(dim, np.zeros(dim), np.ones(dim) * 0.1, np.eye(dim)[0], 0.3, 0.5)
+    k0 = O.free_space_kernels(dim, np.zeros(dim), np.ones(dim) * 0.1, np.eye(dim)[0], 0.3)
+    rr = 0.3 / 0.5
+    if dim == 2:
+        assert abs(k["G_reg"] - k0["G"]) < 1e-7
+        assert abs(k["P_reg"] - k0["P"] * (1 - math.exp(-rr * rr))) < 1e-6
+    else:
+        assert abs(k["G_reg"] - k0["G"] * math.erf(rr)) < 1e-6
+        assert abs(k["P_reg"] - k0["P"] * (math.erf(rr) - 2 * rr * math.exp(-rr * rr) / math.sqrt(math.pi))) < 1e-6
+
+
+def _sphere_samples(dim, n):
+    """quasi-uniform points on the unit circle / sphere (Fibonacci lattice) with outward normals"""
+    if dim == 2:
+        t = (np.arange(n) + 0.5) / n * 2 * np.pi
+        y = np.stack([np.cos(t), np.sin(t)], 1)
+        return y, 1.0 / (2 * np.pi)
+    i = np.arange(n) + 0.5
+    z = 1 - 2 * i / n
+    phi = np.pi * (1 + 5 ** 0.5) * i
+    rr = np.sqrt(1 - z * z)
+    y = np.stack([rr * np.cos(phi), rr * np.sin(phi), z], 1)
+    return y, 1.0 / (4 * np.pi)
+
+
+def _u_exp_cos(x):
+    return np.exp(x[:, 0]) * np.cos(x[:, 1])
+
+
+def _grad_exp_cos(x):
+    e = np.exp(x[:, 0])
+    g = np.zeros_like(x)
+    g[:, 0] = e * np.cos(x[:, 1]); g[:, 1] = -e * np.sin(x[:, 1])
+    return g
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("form", ["dirichlet", "robin_small", "robin_large"])
+def test_splat_reproduces_green_representation(oracle_lib, dim, form):
+    O = oracle_lib
+    y, pdf = _sphere_samples(dim, 4000 if dim == 2 else 40000)
+    u, g = _u_exp_cos(y), _grad_exp_cos(y)
+    dn = np.sum(g * y, 1)                      # outward normal = y
+    kappa = {"dirichlet": 0.0, "robin_small": 0.5, "robin_large": 4.0}[form]
+    samples = dict(pt=y, normal=y, type=np.full(len(y), A.ZG_SAMPLE_ON_ABSORBING_BOUNDARY if kappa == 0 else
+                                                 A.ZG_SAMPLE_ON_REFLECTING_BOUNDARY, np.uint8),
+                   pdf=np.full(len(y), pdf), solution=u, normal_derivative=dn, contribution=dn + kappa * u,
+                   robin_coeff=np.full(len(y), kappa))
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(200, dim)); x = x / np.linalg.norm(x, axis=1, keepdims=True) * rng.random((200, 1)) * 0.7
+    evals = dict(pt=x, dist_abs=1 - np.linalg.norm(x, axis=1))
+    st = O.BvcStats(len(x), dim)
+    O.bvc_splat(dim, samples, evals, st, radius_clamp=1e-4, robin_cutoff=1.0)
+    cat = 0 if kappa == 0 else 2
+    assert np.all(st.n[:, cat] == len(y)) and np.all(st.n.sum(1) == len(y))
+    np.testing.assert_allclose(st.solution(), _u_exp_cos(x), atol=2e-3)
+    np.testing.assert_allclose(st.gradient(), _grad_exp_cos(x), atol=5e-3)
+    # a flipped sample orientation (normal_aligned) lands in the aligned category
+    samples["normal_aligned"] = np.ones(len(y), np.uint8)
+    samples["normal"] = -y
+    st2 = O.BvcStats(len(x), dim)
+    O.bvc_splat(dim, samples, evals, st2, radius_clamp=1e-4, robin_cutoff=1.0)
+    assert np.all(st2.n[:, cat + 1] == len(y))
+    np.testing.assert_allclose(st2.solution(), st.solution(), rtol=1e-6, atol=1e-7)
+
+
+def test_splat_source_term_and_cutoffs(oracle_lib):
+    """-Lap u = f with u = 1 - |x|^2 / 6 on the unit ball (f = 1, u = 5/6 on the sphere): the
+    boundary part gives 5/6 (harmonic part), the domain samples add int G f = (1 - |x|^2) / 6;
+    evaluation points on the boundary use alpha = 2; points within the cutoffs are skipped."""
+    O = oracle_lib
+    dim = 3
+    y, pdf = _sphere_samples(dim, 20000)
+    samples_b = dict(pt=y, normal=y, type=np.full(len(y), A.ZG_SAMPLE_ON_ABSORBING_BOUNDARY, np.uint8),
+                     pdf=np.full(len(y), pdf), solution=np.full(len(y), 5 / 6), normal_derivative=-np.ones(len(y)) / 3)
+    rng = np.random.default_rng(9)
+    m = 400000
+    z = rng.normal(size=(m, 3)); z = z / np.linalg.norm(z, axis=1, keepdims=True) * rng.random((m, 1)) ** (1 / 3)
+    samples_d = dict(pt=z, normal=np.zeros_like(z), type=np.full(m, A.ZG_SAMPLE_IN_DOMAIN, np.uint8),
+                     pdf=np.full(m, 3 / (4 * np.pi)), solution=np.zeros(m), normal_derivative=np.zeros(m),
+                     contribution=np.ones(m))
+    x = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.1], [0.5, 0.1, 0.0], [0.0, 0.0, 0.05]])
+    evals = dict(pt=x, dist_abs=np.array([1.0, 0.6, 0.49, 0.02]))
+    st = O.BvcStats(len(x), dim)
+    O.bvc_splat(dim, samples_b, evals, st, radius_clamp=1e-3, cutoff_abs=0.03)
+    O.bvc_splat(dim, samples_d, evals, st, radius_clamp=1e-3, cutoff_abs=0.03)
+    exact = 1 - np.sum(x ** 2, 1) / 6
+    got = st.solution()
+    assert np.all(np.abs(got[:3] - exact[:3]) < 0.01), (got, exact)
+    assert np.all(st.n[3] == 0)                        # inside the cutoff: not splatted
+    # se of the source estimate is the statistic's own
+    se = np.sqrt(st.m2[:3, 4] / (st.n[:3, 4] - 1) / st.n[:3, 4])
+    assert np.all(np.abs(got[:3] - exact[:3]) < 4 * se + 2e-3)

@@ -146,6 +146,44 @@ class zg_statistics(C.Structure):
     ]
 
 
+class zg_bvc_samples(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("pt", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("type", C.c_void_p),
+        ("normal_aligned", C.c_void_p),
+        ("pdf", C.c_void_p),
+        ("solution", C.c_void_p),
+        ("normal_derivative", C.c_void_p),
+        ("contribution", C.c_void_p),
+        ("robin_coeff", C.c_void_p),
+    ]
+
+
+class zg_bvc_eval_points(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("pt", C.c_void_p),
+        ("type", C.c_void_p),
+        ("dist_to_absorbing_boundary", C.c_void_p),
+        ("dist_to_reflecting_boundary", C.c_void_p),
+    ]
+
+
+class zg_bvc_statistics(C.Structure):
+    _fields_ = [
+        ("n", C.c_void_p),
+        ("solution_mean", C.c_void_p),
+        ("solution_m2", C.c_void_p),
+        ("gradient_mean", C.c_void_p),
+        ("gradient_m2", C.c_void_p),
+    ]
+
+
+ZG_BVC_CATEGORIES = 5
+
+
 class zg_solve_options(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64),
@@ -198,6 +236,9 @@ SIGNATURES = {
     "zg_inside_domain": (C.c_int32, [_P, C.c_int64, _P, _P, _P]),
     "zg_intersect_ray": (C.c_int32, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_star_radius": (C.c_int32, [_P, C.c_int64, _P, _P, _P, C.c_float, _P, _P, _P]),
+    "zg_bvc_splat": (C.c_int32, [C.c_int32, C.c_float, C.POINTER(zg_bvc_samples), C.POINTER(zg_bvc_eval_points),
+                                 C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(zg_bvc_statistics),
+                                 _P]),
     "zg_stratified_samples": (C.c_int32, [C.c_int32, C.c_int64, C.c_int32, C.c_uint64, C.c_uint32, _P, _P]),
     "zg_evaluate_function": (C.c_int32, [C.c_int32, C.POINTER(zg_function), C.c_int64, _P, _P, _P, C.c_float, _P,
                                          _P]),

@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["bvh_build.cpp", "abi.cpp", "wos_kernel.hip", "wost_kernel.hip", "mlp_kernels.hip", "probe_kernel.hip"]
+SOURCES = ["bvh_build.cpp", "abi.cpp", "wos_kernel.hip", "wost_kernel.hip", "mlp_kernels.hip", "probe_kernel.hip",
+           "bvc_kernel.hip"]
 
 
 def _hipcc() -> str:

@@ -747,6 +747,26 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     return runSolve(S, report, (hipStream_t)stream);
 }
 
+zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* S, const zg_bvc_eval_points* E,
+                       float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR,
+                       zg_bvc_statistics* out, void* stream) {
+    if (!S || !E || !out) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (dim != 2 && dim != 3) return fail(ZG_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
+    if (absorption != 0.0f) return fail(ZG_ERR_UNSUPPORTED, "the splat uses the harmonic free-space kernels (absorption 0)");
+    if (S->n < 0 || E->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative counts");
+    if (E->n >= (1ll << 31)) return fail(ZG_ERR_INVALID_ARGUMENT, "at most 2^31-1 evaluation points");
+    if (S->n > 0 && (!S->pt || !S->normal || !S->type || !S->pdf || !S->solution || !S->normal_derivative))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "sample pt, normal, type, pdf, solution and normal_derivative are required");
+    if (E->n > 0 && (!E->pt || !E->dist_to_absorbing_boundary))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "evaluation pt and dist_to_absorbing_boundary are required");
+    if (E->n > 0 && (!out->n || !out->solution_mean || !out->solution_m2 || !out->gradient_mean || !out->gradient_m2))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "statistics arrays are required");
+    if (!(radiusClamp >= 0.0f) || !(kreg >= 0.0f)) return fail(ZG_ERR_INVALID_ARGUMENT, "radius clamp and kernel regularization must be >= 0");
+    hipError_t e = launchBvcSplat(dim, *S, *E, *out, radiusClamp, kreg, robinCutoff, cutoffA, cutoffR, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "zg_bvc_splat: %s", hipGetErrorString(e));
+    return ok();
+}
+
 zg_status zg_stratified_samples(int32_t dims, int64_t nTables, int32_t nSamples, uint64_t seed, uint32_t streamOffset,
                                 float* out, void* stream) {
     if (dims < 1 || dims > 2) return fail(ZG_ERR_INVALID_ARGUMENT, "dims must be 1 or 2");

@@ -146,6 +146,9 @@ hipError_t launchStratified(int dims, float* out, int64_t nTables, int64_t strid
                             uint32_t streamOffset, hipStream_t s);
 hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const float* pts, const float* normals,
                               const uint8_t* aligned, float absorption, float* out, hipStream_t s);
+// boundary value caching splat
+hipError_t launchBvcSplat(int dim, const zg_bvc_samples& S, const zg_bvc_eval_points& E, const zg_bvc_statistics& out,
+                          float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR, hipStream_t s);
 // read-bandwidth probe
 hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks, float* sink, hipStream_t s);
 // walk on stars (2D)

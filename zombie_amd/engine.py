@@ -526,3 +526,86 @@ def wost_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: Devi
     A.check(A.lib().zg_wost_solve(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st), C.byref(oc),
                                   C.byref(rep), _stream_ptr(stream)))
     return SolveReport.from_c(rep)
+
+
+# ------------------------------------------------------------------------------------
+# boundary value caching splat
+# ------------------------------------------------------------------------------------
+@dataclass
+class BvcSamples:
+    """A sample cache on the device (SamplePoint fields the splat reads)."""
+    pt: object
+    normal: object
+    type: object                     # uint8: ZG_SAMPLE_ON_ABSORBING_BOUNDARY / _ON_REFLECTING_ / _IN_DOMAIN
+    pdf: object
+    solution: object
+    normal_derivative: object
+    contribution: object = None      # Robin data h (reflecting) or source value (domain)
+    robin_coeff: object = None
+    normal_aligned: object = None
+
+    def to_c(self) -> A.zg_bvc_samples:
+        s = A.zg_bvc_samples()
+        s.n = int(self.pt.shape[0])
+        s.pt, s.normal, s.type, s.pdf = _ptr(self.pt), _ptr(self.normal), _ptr(self.type), _ptr(self.pdf)
+        s.solution, s.normal_derivative = _ptr(self.solution), _ptr(self.normal_derivative)
+        s.contribution, s.robin_coeff = _ptr(self.contribution), _ptr(self.robin_coeff)
+        s.normal_aligned = _ptr(self.normal_aligned)
+        return s
+
+
+@dataclass
+class BvcEvalPoints:
+    pt: object
+    dist_to_absorbing: object
+    dist_to_reflecting: object = None
+    type: object = None
+
+    def to_c(self) -> A.zg_bvc_eval_points:
+        e = A.zg_bvc_eval_points()
+        e.n = int(self.pt.shape[0])
+        e.pt, e.type = _ptr(self.pt), _ptr(self.type)
+        e.dist_to_absorbing_boundary, e.dist_to_reflecting_boundary = _ptr(self.dist_to_absorbing), _ptr(self.dist_to_reflecting)
+        return e
+
+
+@dataclass
+class BvcStats:
+    """Five SampleStatistics per evaluation point (absorbing, absorbing aligned, reflecting,
+    reflecting aligned, source) in FP64."""
+    n: object
+    mean: object
+    m2: object
+    grad_mean: object
+    grad_m2: object
+
+    @staticmethod
+    def zeros(count: int, dim: int, device="cuda") -> "BvcStats":
+        torch = _torch()
+        z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=device)
+        return BvcStats(torch.zeros(count, 5, dtype=torch.int64, device=device), z(count, 5), z(count, 5),
+                        z(count, 5, dim), z(count, 5, dim))
+
+    def to_c(self) -> A.zg_bvc_statistics:
+        s = A.zg_bvc_statistics()
+        s.n, s.solution_mean, s.solution_m2 = _ptr(self.n), _ptr(self.mean), _ptr(self.m2)
+        s.gradient_mean, s.gradient_m2 = _ptr(self.grad_mean), _ptr(self.grad_m2)
+        return s
+
+    def solution(self):
+        """EvaluationPoint::getEstimatedSolution: the sum of the five means"""
+        return self.mean.sum(1)
+
+    def gradient(self):
+        return self.grad_mean.sum(1)
+
+
+def bvc_splat(dim: int, samples: BvcSamples, evals: BvcEvalPoints, stats: BvcStats, radius_clamp: float = 1e-3,
+              kernel_regularization: float = 0.0, robin_coeff_cutoff: float = float("inf"),
+              cutoff_dist_to_absorbing: float = 0.0, cutoff_dist_to_reflecting: float = 0.0, absorption: float = 0.0,
+              stream=None) -> None:
+    """BoundaryValueCaching::splat (boundary_value_caching.h:370-402) on the GPU."""
+    s, e, st = samples.to_c(), evals.to_c(), stats.to_c()
+    A.check(A.lib().zg_bvc_splat(dim, absorption, C.byref(s), C.byref(e), radius_clamp, kernel_regularization,
+                                 min(robin_coeff_cutoff, 3.0e38), cutoff_dist_to_absorbing, cutoff_dist_to_reflecting,
+                                 C.byref(st), _stream_ptr(stream)))
