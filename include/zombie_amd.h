@@ -165,6 +165,19 @@ zg_status zg_scene_create(int32_t dim, const float* positions_host, int64_t n_ve
                           const zg_build_options* opts, zg_scene** out);
 zg_status zg_scene_destroy(zg_scene* scene);
 
+/* SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h:87-190 SdfGrid,
+ * :247-272 populateGeometricQueriesForDirichletBoundary): the whole boundary is absorbing and
+ * given by signed distances at the cell centres of a grid over [grid_min, grid_max]
+ * (values[shape[0] x shape[1] (x shape[2])], last axis fastest, host memory; copied), read with
+ * bi/trilinear interpolation clamped to the edge.  zg_closest_point returns |sdf|, the
+ * steepest-descent projection onto the zero level set and its normal (prim -1);
+ * zg_inside_domain tests sdf < 0; zg_wos_solve walks with radii |sdf| (no walk splitting,
+ * gradient points or control-variate gradients).  Walks leaving [domain_min, domain_max]
+ * (NULL: the grid box) escape. */
+zg_status zg_sdf_scene_create(int32_t dim, const float* values_host, const int32_t* shape_host,
+                              const float* grid_min_host, const float* grid_max_host,
+                              const float* domain_min_host, const float* domain_max_host, zg_scene** out);
+
 /* Robin coefficient bounds of the reflecting primitives (FcpwRobinBoundaryHandler<2>::
  * buildAccelerationStructure / updateCoefficientValues, fcpw_geometric_queries.h:726-831):
  * min_coeff_host[p] <= max_coeff_host[p] for every primitive p of the scene (entries of

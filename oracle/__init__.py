@@ -63,6 +63,10 @@ def lib():
                                      P, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P, P, P, P,
                                      P, P, P, P, P, P, P]
         L.orc_solve_grad.restype = C.c_int
+        L.orc_sdf_solve.argtypes = [C.c_int, P, P, P, P, P, P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings),
+                                    C.c_int64, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, P, P, P, P, P]
+        L.orc_sdf_solve.restype = C.c_int
+        L.orc_sdf_query.argtypes = [C.c_int, P, P, P, P, C.c_int64, P, P, P, P]
         L.orc_bvc_splat.argtypes = [C.c_int, C.POINTER(A.zg_bvc_samples), C.POINTER(A.zg_bvc_eval_points), C.c_float,
                                     C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P, P, C.c_int]
         L.orc_free_space_kernels.argtypes = [C.c_int, P, P, P, C.c_float, C.c_float, P]
@@ -122,6 +126,41 @@ def gradient_direction(dim, cosine, u, n=None):
     out = np.empty(dim + 1, np.float32)
     lib().orc_gradient_direction(dim, 1 if cosine else 0, _p(u), _p(nn), _p(out))
     return out[:dim].copy(), float(out[dim])
+
+
+class SdfGrid:
+    """SDF-grid Dirichlet geometry (values [shape...] cell-centred over [lo, hi], last axis fastest)."""
+
+    def __init__(self, values, lo, hi, domain_min=None, domain_max=None):
+        self.values = np.ascontiguousarray(values, np.float32)
+        self.dim = self.values.ndim
+        self.shape = np.ascontiguousarray(self.values.shape, np.int32)
+        self.lo = np.ascontiguousarray(lo, np.float32)
+        self.hi = np.ascontiguousarray(hi, np.float32)
+        self.dmin = np.ascontiguousarray(self.lo if domain_min is None else domain_min, np.float32)
+        self.dmax = np.ascontiguousarray(self.hi if domain_max is None else domain_max, np.float32)
+
+    def query(self, pts):
+        pts = np.ascontiguousarray(pts, np.float32)
+        n = len(pts)
+        d = np.empty(n, np.float32); pr = np.empty_like(pts); nr = np.empty_like(pts)
+        lib().orc_sdf_query(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), n, _p(pts), _p(d),
+                            _p(pr), _p(nr))
+        return d, pr, nr
+
+    def solve(self, pde, ws, pts, dist, n_walks, stats: "Stats", seed=0x5EED, stream_offset=0, rng_mode=RNG_PER_WALK,
+              threads=None, types=None):
+        pts = np.ascontiguousarray(pts, np.float32)
+        n = len(pts)
+        dist = np.ascontiguousarray(dist, np.float32)
+        nw = np.ascontiguousarray(np.broadcast_to(np.asarray(n_walks, np.int32), n), np.int32)
+        types = None if types is None else np.ascontiguousarray(types, np.uint8)
+        rep = np.zeros(5, np.int64)
+        lib().orc_sdf_solve(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), _p(self.dmin),
+                            _p(self.dmax), C.byref(pde), C.byref(ws), n, _p(pts), _p(types), _p(dist), _p(nw), seed,
+                            stream_offset, rng_mode, threads or default_threads(), _p(stats.n), _p(stats.mean),
+                            _p(stats.m2), _p(stats.walk_length), _p(rep))
+        return dict(codes=rep[:4].copy(), total_steps=int(rep[4]))
 
 
 def free_space_kernels(dim, x, y, n, r, kreg=0.0):

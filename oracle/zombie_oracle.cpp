@@ -48,6 +48,7 @@
 #include <deque>
 #include <memory>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../include/zombie_amd.h"
@@ -1757,6 +1758,101 @@ void buildScene(Scene<D>& sc, const float* V_, int64_t nV, const int32_t* I, int
 }
 
 // ------------------------------------------------------------------------------------
+// SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h:87-190, 247-272): the
+// distance is |sdf(x)| of a trilinearly interpolated grid, the projection steepest descent on
+// the SDF with central-difference gradients, clamped into the grid
+// ------------------------------------------------------------------------------------
+template <int D>
+struct SdfGeom {
+    zg_function f;          // the grid (ZG_FN_DENSE_GRID, interpolated)
+    V<D> dmin, dmax;        // bounding domain of the walks
+    float value(const V<D>& x) const { return gridValue<D>(f, x); }
+    V<D> clampIn(V<D> x) const {
+        for (int a = 0; a < D; a++) x[a] = std::clamp(x[a], f.grid_min[a] + 1e-6f, f.grid_max[a] - 1e-6f);
+        return x;
+    }
+    V<D> gradient(const V<D>& x) const {
+        V<D> g = V<D>::zero();
+        for (int a = 0; a < D; a++) {
+            float spacing = (f.grid_max[a] - f.grid_min[a]) / (float)f.shape[a];
+            V<D> o = V<D>::zero(); o[a] = spacing;
+            V<D> xm = clampIn(x - o), xp = clampIn(x + o);
+            float denom = xp[a] - xm[a];
+            g[a] = denom > 0.0f ? (value(xp) - value(xm)) / denom : 0.0f;
+        }
+        return g;
+    }
+    V<D> normal(const V<D>& x) const { V<D> g = gradient(x); return g * (1.0f / (norm(g) + 1e-6f)); }
+    void project(V<D>& x, V<D>& n, int maxIter = 8, float eps = 1e-5f) const {
+        for (int i = 0; i < maxIter; i++) {
+            float s = value(x);
+            if (std::fabs(s) < eps) break;
+            V<D> g = gradient(x);
+            x = clampIn(x - g * (s / (norm(g) + 1e-6f)));
+        }
+        n = normal(x);
+    }
+    bool outside(const V<D>& x) const {
+        for (int a = 0; a < D; a++) if (x[a] < dmin[a] || x[a] > dmax[a]) return true;
+        return false;
+    }
+};
+
+// estimateSolution + walk (walk_on_spheres.h:228-414) over the SDF geometry
+template <int D>
+void wosEstimateSdf(const SdfGeom<D>& g, const SolveCtx& cx, int64_t p, const V<D>& pt, uint8_t type, float distA,
+                    int nWalks, Stats& out, int64_t codes[4], int64_t prevN) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    auto terminal = [&](V<D> x) {
+        float sd = g.value(x);
+        V<D> n; g.project(x, n);
+        return evalField<D>(pde.dirichlet, x, n, ws.solve_double_sided && sd > 0.0f, pde.absorption_coeff);
+    };
+    if (type == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
+        if (prevN == 0) out.add(ws.ignore_absorbing_boundary_contribution ? 0.0 :
+                                (double)evalField<D>(pde.dirichlet, pt, V<D>::zero(), false, pde.absorption_coeff));
+        return;
+    } else if (distA <= ws.epsilon_shell_for_absorbing_boundary) {
+        nWalks = std::min(nWalks, 1);
+    }
+    Pcg32 pointRng = makeRng(cx, p, 0);
+    for (int w = 0; w < nWalks; w++) {
+        Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
+        Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
+        GreensBall<D> gf;
+        if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
+        WalkState<D> st; st.currentPt = pt;
+        float dist = distA;
+        Code code = Reached;
+        std::deque<WalkState<D>> q;
+        while (dist > ws.epsilon_shell_for_absorbing_boundary) {
+            gf.updateBall(st.currentPt, dist);
+            if (!ws.ignore_source_contribution) {
+                float sc_;
+                if (pde.is_source_constant) sc_ = gf.norm() * evalField<D>(pde.source, st.currentPt, V<D>::zero(), false, pde.absorption_coeff);
+                else { float r, pdf; V<D> y = gf.sampleVolume(rng, r, pdf); sc_ = gf.norm() * evalField<D>(pde.source, y, V<D>::zero(), false, pde.absorption_coeff); }
+                st.totalSourceContribution += st.throughput * sc_;
+            }
+            V<D> dir = sampleUnitSphere<D>(rng);
+            st.currentPt = st.currentPt + dist * dir;
+            st.throughput *= gf.directionSampledPoissonKernel(st.currentPt);
+            if (g.outside(st.currentPt)) { code = Escaped; break; }
+            if (applyWeightWindow(ws, rng, st, q)) { code = RR; break; }
+            st.walkLength++;
+            if (st.walkLength >= ws.max_walk_length) { code = MaxLen; break; }
+            if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == st.walkLength) gf.makeYukawa(pde.absorption_coeff);
+            dist = std::fabs(g.value(st.currentPt));
+        }
+        codes[code]++;
+        if (code == Escaped) continue;
+        float term = 0.0f;
+        if ((code == Reached || code == MaxLen) && !ws.ignore_absorbing_boundary_contribution) term = terminal(st.currentPt);
+        out.add((double)(st.throughput * term + st.totalSourceContribution));
+        out.len += st.walkLength;
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // boundary value caching: free-space harmonic kernels (distributions.h:84-177), kernel
 // regularization (:12-55) and the splat (variance_reduction/boundary_value_caching.h:370-402,
 // 683-810)
@@ -2023,6 +2119,50 @@ void* orc_scene_create(int dim, const float* V_, int64_t nV, const int32_t* I, i
     return h;
 }
 void orc_scene_destroy(void* h) { delete (SceneHandle*)h; }
+// walk on spheres over an SDF grid (values with the DenseGrid layout, trilinear); rng_mode as orc_solve
+int orc_sdf_solve(int dim, const float* grid, const int32_t* shape, const float* gmin, const float* gmax,
+                  const float* dmin, const float* dmax, const zg_pde* pde, const zg_walk_settings* ws, int64_t n,
+                  const float* pts, const uint8_t* types, const float* distA, const int32_t* nWalks, uint64_t seed,
+                  uint32_t streamOffset, int rngMode, int nThreads, int64_t* sN, double* sMean, double* sM2,
+                  int64_t* sLen, int64_t* report) {
+    zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
+    for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
+    SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
+    int nt = std::max(1, nThreads);
+    std::vector<std::array<int64_t, 5>> per(nt);
+    for (auto& a : per) a.fill(0);
+    auto run = [&](auto dimTag) {
+        constexpr int D = decltype(dimTag)::value;
+        SdfGeom<D> g; g.f = f; g.dmin = load<D>(dmin); g.dmax = load<D>(dmax);
+        parallelFor(n, nt, [&](int64_t i, int t) {
+            Stats st; int64_t codes[4] = {0, 0, 0, 0};
+            wosEstimateSdf<D>(g, cx, i, load<D>(pts + i * D), types ? types[i] : 0, distA[i], nWalks[i], st, codes, sN[i]);
+            mergeInto(sN[i], sMean[i], sM2[i], st);
+            sLen[i] += st.len;
+            for (int k = 0; k < 4; k++) per[t][k] += codes[k];
+            per[t][4] += st.len;
+        });
+    };
+    if (dim == 2) run(std::integral_constant<int, 2>{}); else run(std::integral_constant<int, 3>{});
+    if (report) { for (int k = 0; k < 5; k++) { report[k] = 0; for (auto& a : per) report[k] += a[k]; } }
+    return 0;
+}
+// SDF queries: |sdf|, projected point and normal (projectToAbsorbingBoundary for an SdfGrid)
+void orc_sdf_query(int dim, const float* grid, const int32_t* shape, const float* gmin, const float* gmax, int64_t n,
+                   const float* pts, float* dist, float* proj, float* nrm) {
+    zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
+    for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
+    for (int64_t i = 0; i < n; i++) {
+        if (dim == 2) {
+            SdfGeom<2> g; g.f = f; V<2> x = load<2>(pts + 2 * i), nn; dist[i] = std::fabs(g.value(x));
+            g.project(x, nn); store<2>(proj + 2 * i, x); store<2>(nrm + 2 * i, nn);
+        } else {
+            SdfGeom<3> g; g.f = f; V<3> x = load<3>(pts + 3 * i), nn; dist[i] = std::fabs(g.value(x));
+            g.project(x, nn); store<3>(proj + 3 * i, x); store<3>(nrm + 3 * i, nn);
+        }
+    }
+}
+
 // BoundaryValueCaching::splat of a sample cache into evaluation points (host arrays); stats are
 // Chan-merged like zg_bvc_statistics
 int orc_bvc_splat(int dim, const zg_bvc_samples* S, const zg_bvc_eval_points* E, float radiusClamp, float kreg,

@@ -1,0 +1,48 @@
+"""The oracle's SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h:87-190, 247-272)
+against the exact sphere: distances, projections and normals of an interpolated signed-distance
+grid, and walk on spheres over it against a harmonic closed form."""
+import numpy as np
+import pytest
+
+from zombie_amd import _abi as A
+from zombie_amd import engine as E
+from zombie_amd import problems as PB
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sdf_queries_match_the_sphere(oracle_lib, dim):
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(dim, 96 if dim == 2 else 48)
+    g = O.SdfGrid(vals, lo, hi)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(500, dim)); x = x / np.linalg.norm(x, axis=1, keepdims=True) * rng.uniform(0.2, 1.15, (500, 1))
+    d, proj, nrm = g.query(x)
+    r = np.linalg.norm(x, axis=1)
+    h = (hi[0] - lo[0]) / vals.shape[0]
+    assert np.max(np.abs(d - np.abs(r - 1))) < 0.2 * h           # trilinear interpolation of |x| - 1
+    assert np.max(np.abs(np.linalg.norm(proj, axis=1) - 1)) < 1e-3
+    u = x / r[:, None]
+    assert np.min(np.sum(nrm * u, 1)) > 0.995
+    np.testing.assert_allclose(np.linalg.norm(nrm, axis=1), 1.0, atol=1e-4)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sdf_walk_on_spheres_matches_exact(oracle_lib, dim):
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(dim, 96 if dim == 2 else 48)
+    g = O.SdfGrid(vals, lo, hi)
+    pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_VALUE, (1.0, 1.0)))
+    ws = E.walk_settings_host(1e-3, 1e-3, 1024)
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(300, dim)); x = x / np.linalg.norm(x, axis=1, keepdims=True) * rng.random((300, 1)) * 0.8
+    d, _, _ = g.query(x)
+    st = O.Stats(len(x))
+    rep = g.solve(pde, ws, x, d, 256, st)
+    u = np.exp(x[:, 0].astype(np.float64)) * np.cos(x[:, 1].astype(np.float64))
+    se2 = st.variance() / st.n
+    pooled = np.sum(st.mean - u) / np.sqrt(np.sum(se2))
+    z = (st.mean - u) / np.sqrt(se2)
+    assert abs(pooled) < 4.0 and 0.8 < z.std() < 1.25, (pooled, z.std())
+    assert rep["codes"][0] == 256 * len(x)
+    # the walk length of walk on spheres in a ball grows like log(1 / eps)
+    assert 5 < rep["total_steps"] / st.n.sum() < 60

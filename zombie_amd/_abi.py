@@ -231,6 +231,7 @@ SIGNATURES = {
                                     C.POINTER(zg_build_options), C.POINTER(C.c_void_p)]),
     "zg_scene_destroy": (C.c_int32, [_P]),
     "zg_scene_set_robin_coefficients": (C.c_int32, [_P, _P, _P]),
+    "zg_sdf_scene_create": (C.c_int32, [C.c_int32, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_void_p)]),
     "zg_scene_get_info": (C.c_int32, [_P, C.POINTER(zg_scene_info)]),
     "zg_closest_point": (C.c_int32, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P]),
     "zg_inside_domain": (C.c_int32, [_P, C.c_int64, _P, _P, _P]),

@@ -83,6 +83,8 @@ struct DeviceBoundary {
 struct zg_scene {
     int dim = 3, width = 8, device = 0;
     int64_t nPrims = 0;
+    zg_function sdf{};          // SDF-grid scenes: device grid in sdf.grid (kind ZG_FN_DENSE_GRID, interpolated)
+    float* sdfGrid = nullptr;
     DeviceBoundary abs, refl;
     SceneView view{};
     zg_scene_info info{};
@@ -298,8 +300,48 @@ zg_status zg_scene_set_robin_coefficients(zg_scene* s, const float* minCoeff, co
     return ok();
 }
 
+zg_status zg_sdf_scene_create(int32_t dim, const float* values, const int32_t* shape, const float* gridMin,
+                              const float* gridMax, const float* domainMin, const float* domainMax, zg_scene** out) {
+    if (!out) return fail(ZG_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(ZG_ERR_INVALID_ARGUMENT, "dim must be 2 or 3, got %d", dim);
+    if (!values || !shape || !gridMin || !gridMax) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL grid argument");
+    size_t count = 1;
+    for (int a = 0; a < dim; a++) {
+        if (shape[a] < 2) return fail(ZG_ERR_INVALID_ARGUMENT, "SDF grid shape[%d] = %d must be >= 2", a, shape[a]);
+        if (!(gridMax[a] > gridMin[a])) return fail(ZG_ERR_INVALID_ARGUMENT, "SDF grid box is empty");
+        count *= (size_t)shape[a];
+    }
+    int dev = 0;
+    ZG_HIP(hipGetDevice(&dev));
+    zg_scene* s = new zg_scene();
+    s->dim = dim; s->width = 4; s->device = dev;
+    hipError_t e = hipMalloc((void**)&s->sdfGrid, count * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(s->sdfGrid, values, count * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (s->sdfGrid) (void)hipFree(s->sdfGrid);
+        delete s;
+        return fail(ZG_ERR_DEVICE, "zg_sdf_scene_create: %s", hipGetErrorString(e));
+    }
+    zg_function& f = s->sdf;
+    f.kind = ZG_FN_DENSE_GRID; f.eval = ZG_EVAL_VALUE; f.grid = s->sdfGrid; f.interpolate = 1;
+    for (int a = 0; a < 3; a++) {
+        f.shape[a] = a < dim ? shape[a] : 1;
+        f.grid_min[a] = a < dim ? gridMin[a] : 0.0f;
+        f.grid_max[a] = a < dim ? gridMax[a] : 1.0f;
+        s->view.dmin[a] = a < dim ? (domainMin ? domainMin[a] : gridMin[a]) : -FLT_MAX;
+        s->view.dmax[a] = a < dim ? (domainMax ? domainMax[a] : gridMax[a]) : FLT_MAX;
+    }
+    s->view.dim = dim; s->view.width = 4;
+    s->info.dim = dim; s->info.bvh_width = 4;
+    s->info.device_bytes = (int64_t)(count * sizeof(float));
+    *out = s;
+    return ok();
+}
+
 zg_status zg_scene_destroy(zg_scene* s) {
     if (!s) return ok();
+    if (s->sdfGrid) (void)hipFree(s->sdfGrid);
     freeBoundary(s->abs);
     freeBoundary(s->refl);
     delete s;
@@ -316,12 +358,18 @@ zg_status zg_closest_point(const zg_scene* s, int32_t boundary, int64_t n, const
                            float* closest, float* normal, int32_t* prim, void* stream) {
     if (!s || n < 0 || (n > 0 && (!pts || !dist))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
     if (boundary != 0 && boundary != 1) return fail(ZG_ERR_INVALID_ARGUMENT, "boundary must be 0 or 1");
+    if (s->sdfGrid) {
+        if (boundary != 0) return fail(ZG_ERR_UNSUPPORTED, "SDF-grid scenes have an absorbing boundary only");
+        ZG_HIP(launchSdfQuery(s->sdf, s->dim, n, pts, dist, closest, normal, prim, (hipStream_t)stream));
+        return ok();
+    }
     ZG_HIP(launchClosestPoint(s->view, boundary, n, pts, dist, closest, normal, prim, (hipStream_t)stream));
     return ok();
 }
 
 zg_status zg_inside_domain(const zg_scene* s, int64_t n, const float* pts, uint8_t* inside, void* stream) {
     if (!s || n < 0 || (n > 0 && (!pts || !inside))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
+    if (s->sdfGrid) { ZG_HIP(launchSdfInside(s->sdf, s->dim, n, pts, inside, (hipStream_t)stream)); return ok(); }
     ZG_HIP(launchInsideDomain(s->view, n, pts, inside, (hipStream_t)stream));
     return ok();
 }
@@ -330,6 +378,7 @@ zg_status zg_intersect_ray(const zg_scene* s, int32_t boundary, int64_t n, const
                            const float* tmax, uint8_t* hit, float* t, float* point, float* normal, void* stream) {
     if (!s || n < 0 || (n > 0 && (!o || !d || !tmax || !hit))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
     if (boundary != 0 && boundary != 1) return fail(ZG_ERR_INVALID_ARGUMENT, "boundary must be 0 or 1");
+    if (s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "ray queries are not available on SDF-grid scenes");
     ZG_HIP(launchRay(s->view, boundary, n, o, d, tmax, hit, t, point, normal, (hipStream_t)stream));
     return ok();
 }
@@ -337,7 +386,7 @@ zg_status zg_intersect_ray(const zg_scene* s, int32_t boundary, int64_t n, const
 zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const float* minR, const float* maxR,
                          float prec, const uint8_t* flip, float* radius, void* stream) {
     if (!s || n < 0 || (n > 0 && (!pts || !minR || !maxR || !radius))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
-    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "star radius queries run on 2D scenes");
+    if (s->dim != 2 || s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "star radius queries run on 2D mesh scenes");
     ZG_HIP(launchStarRadius(s->view, n, pts, minR, maxR, prec, flip, radius, (hipStream_t)stream));
     return ok();
 }
@@ -422,6 +471,7 @@ struct SolveSetup {
     size_t words = 0;
     const float* callerFsr = nullptr;   // walk on stars: caller's recorded first sphere radii
     zg_statistics stats{};              // caller's statistics (gradient members checked per call)
+    bool sdf = false;                   // SDF-grid scene
 };
 
 zg_status checkPde(const zg_pde* pde, int dim) {
@@ -491,6 +541,14 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
         P.cvGrid = reinterpret_cast<const float4*>(gf.data);
     }
     if (s) { P.scene = s->view; P.pde = *pde; }
+    if (s && s->sdfGrid) {
+        if (wost) return fail(ZG_ERR_UNSUPPORTED, "walk on stars needs a mesh scene");
+        if (split) return fail(ZG_ERR_UNSUPPORTED, "walk splitting is not available on SDF-grid scenes");
+        if (pts->cv_gradient || pts->cv_gradient_grid)
+            return fail(ZG_ERR_UNSUPPORTED, "control-variate gradients are not available on SDF-grid scenes");
+        P.sdf = s->sdf;
+        S.sdf = true;
+    }
     else { P.instances = b->table; P.instanceOf = instanceOf; P.nInstances = b->n; }
     P.ws = *ws;
     int64_t n = pts->n;
@@ -578,8 +636,9 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     if (e == hipSuccess && S.wost) e = launchWostFirstSphere(P, S.width, S.callerFsr, const_cast<float*>(P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
         e = S.wost ? launchWost(P, S.width, S.count, S.blocks, strm)
-                   : S.batch ? launchWosBatch(P, S.width, S.count, S.screened, S.split, S.blocks, strm)
-                             : launchWos(P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
+            : S.batch ? launchWosBatch(P, S.width, S.count, S.screened, S.split, S.blocks, strm)
+            : S.sdf ? launchWosSdf(P, S.dim, S.screened, S.blocks, strm)
+                    : launchWos(P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
     if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
     if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
     return e;
@@ -689,6 +748,7 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
     }
     size_t words = S.words;
     if (nGrad > 0) {
+        if (S.sdf) return fail(ZG_ERR_UNSUPPORTED, "gradient estimation is not available on SDF-grid scenes");
         const zg_statistics& st = S.stats;
         if (!st.n_gradient_estimates || !st.gradient_mean || !st.gradient_m2 || !st.total_first_source_contribution ||
             !st.total_derivative_contribution)
@@ -788,6 +848,7 @@ zg_status zg_batch_create(int32_t nInstances, const zg_scene* const* scenes, con
     for (int i = 0; i < nInstances; i++) {
         const zg_scene* s = scenes[i];
         if (!s) { delete b; return fail(ZG_ERR_INVALID_ARGUMENT, "scene %d is NULL", i); }
+        if (s->sdfGrid) { delete b; return fail(ZG_ERR_UNSUPPORTED, "batched solves take mesh scenes"); }
         if (s->dim != 3) { delete b; return fail(ZG_ERR_UNSUPPORTED, "batched solves are 3D"); }
         if (i > 0 && (s->width != b->width || s->device != b->device)) {
             delete b;

@@ -373,6 +373,47 @@ ZG_DEV float evalField(const zg_function& f, const Vec<D>& x, const Vec<D>& n, b
 }
 
 // ------------------------------------------------------------------------------------
+// SDF-grid geometry (utils/sdf_grid_geometric_queries.h:87-190): value by trilinear
+// interpolation of the grid, central-difference gradient with the stencil clamped into the grid,
+// steepest-descent projection to the zero level set (at most 8 steps, |sdf| < 1e-5 stops)
+// ------------------------------------------------------------------------------------
+template <int D>
+ZG_DEV Vec<D> sdfClampIn(const zg_function& f, Vec<D> x) {
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = fminf(fmaxf(x[a], f.grid_min[a] + 1e-6f), f.grid_max[a] - 1e-6f);
+    return x;
+}
+template <int D>
+ZG_DEV Vec<D> sdfGradient(const zg_function& f, const Vec<D>& x) {
+    Vec<D> g;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const float spacing = (f.grid_max[a] - f.grid_min[a]) / (float)f.shape[a];
+        Vec<D> xm = x, xp = x;
+        xm[a] -= spacing; xp[a] += spacing;
+        xm = sdfClampIn<D>(f, xm); xp = sdfClampIn<D>(f, xp);
+        const float denom = xp[a] - xm[a];
+        g[a] = denom > 0.0f ? (gridValue<D>(f, xp) - gridValue<D>(f, xm)) / denom : 0.0f;
+    }
+    return g;
+}
+template <int D>
+ZG_DEV Vec<D> sdfNormal(const zg_function& f, const Vec<D>& x) {
+    Vec<D> g = sdfGradient<D>(f, x);
+    return g * (1.0f / (__builtin_sqrtf(norm2(g)) + 1e-6f));
+}
+template <int D>
+ZG_DEV void sdfProject(const zg_function& f, Vec<D>& x, Vec<D>& n) {
+    for (int i = 0; i < 8; i++) {
+        const float v = gridValue<D>(f, x);
+        if (fabsf(v) < 1e-5f) break;
+        Vec<D> g = sdfGradient<D>(f, x);
+        x = sdfClampIn<D>(f, x - g * (v / (__builtin_sqrtf(norm2(g)) + 1e-6f)));
+    }
+    n = sdfNormal<D>(f, x);
+}
+
+// ------------------------------------------------------------------------------------
 // primitives (Ericson, "Real-Time Collision Detection" 5.1.2 / 5.1.5; the same closed forms
 // as core/geometry_helpers.h:28-49,94-161)
 // ------------------------------------------------------------------------------------

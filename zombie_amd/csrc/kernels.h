@@ -79,6 +79,7 @@ struct WosParams {
     double* accFirst;                 // [n] first-step source terms
     double* accDeriv;                 // [n] directional derivative contributions
     const float* distR;               // walk on stars: caller's distances to the reflecting boundary (nullable)
+    zg_function sdf;                  // SDF-grid scenes: the interpolated signed-distance grid (grid != null)
 };
 
 // capacity of the per-lane queue of split walk parts
@@ -146,6 +147,11 @@ hipError_t launchStratified(int dims, float* out, int64_t nTables, int64_t strid
                             uint32_t streamOffset, hipStream_t s);
 hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const float* pts, const float* normals,
                               const uint8_t* aligned, float absorption, float* out, hipStream_t s);
+// SDF-grid scenes: walk on spheres and the distance / projection query
+hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s);
+hipError_t launchSdfQuery(const zg_function& sdf, int dim, int64_t n, const float* pts, float* dist, float* closest,
+                          float* normal, int32_t* prim, hipStream_t s);
+hipError_t launchSdfInside(const zg_function& sdf, int dim, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);
 // boundary value caching splat
 hipError_t launchBvcSplat(int dim, const zg_bvc_samples& S, const zg_bvc_eval_points& E, const zg_bvc_statistics& out,
                           float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR, hipStream_t s);

@@ -402,6 +402,31 @@ class DeviceStats:
         return self.derivative / torch.clamp(self.n, min=1).to(torch.float64)
 
 
+class SdfScene(Scene):
+    """SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h): signed distances at the
+    cell centres of a grid over [lo, hi] (values[shape], last axis fastest), trilinear.  Used like
+    a Scene by closest_point (|sdf| + projection), inside_domain and wos_solve."""
+
+    def __init__(self, values, lo, hi, domain_min=None, domain_max=None):
+        vals = np.ascontiguousarray(values, dtype=np.float32)
+        self.dim = vals.ndim
+        shape = np.ascontiguousarray(vals.shape, dtype=np.int32)
+        lo = np.ascontiguousarray(lo, dtype=np.float32)
+        hi = np.ascontiguousarray(hi, dtype=np.float32)
+        dmin = None if domain_min is None else np.ascontiguousarray(domain_min, dtype=np.float32)
+        dmax = None if domain_max is None else np.ascontiguousarray(domain_max, dtype=np.float32)
+        h = C.c_void_p()
+        A.check(A.lib().zg_sdf_scene_create(self.dim, vals.ctypes.data, shape.ctypes.data, lo.ctypes.data,
+                                            hi.ctypes.data, None if dmin is None else dmin.ctypes.data,
+                                            None if dmax is None else dmax.ctypes.data, C.byref(h)))
+        self._h = h
+        self.domain_min = lo if dmin is None else dmin
+        self.domain_max = hi if dmax is None else dmax
+        info = A.zg_scene_info()
+        A.check(A.lib().zg_scene_get_info(self._h, C.byref(info)))
+        self.info = info
+
+
 def stratified_samples(dims: int, n_tables: int, n_samples: int, seed: int, stream_offset: int = 0,
                        device="cuda", stream=None):
     """The per-point Latin hypercube tables of gradient solves: [n_tables, n_samples, dims]."""

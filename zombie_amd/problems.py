@@ -305,6 +305,16 @@ def mixed2d_robin(kappa=2.0, n_segments=1024, n_slits=8, seed=0x5EED, n_walks=25
     return p
 
 
+def sphere_sdf_grid(dim=3, res=64, radius=1.0, margin=0.2):
+    """Signed distance |x| - radius of a ball sampled at the cell centres of a res^dim grid over
+    [-(radius + margin), radius + margin]^dim (DenseGrid layout, last axis fastest)."""
+    h = radius + margin
+    c = (np.arange(res) + 0.5) / res * 2 * h - h
+    mesh = np.meshgrid(*([c] * dim), indexing="ij")
+    r = np.sqrt(sum(m.astype(np.float64) ** 2 for m in mesh))
+    return (r - radius).astype(np.float32), np.full(dim, -h, np.float32), np.full(dim, h, np.float32)
+
+
 def _segments_close(p0, p1, q0, q1, tol):
     def seg_dist(a, b, c):
         ab = b - a
