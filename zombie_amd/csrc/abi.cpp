@@ -541,6 +541,7 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
         P.cvGrid = reinterpret_cast<const float4*>(gf.data);
     }
     if (s) { P.scene = s->view; P.pde = *pde; }
+    else { P.instances = b->table; P.instanceOf = instanceOf; P.nInstances = b->n; }
     if (s && s->sdfGrid) {
         if (wost) return fail(ZG_ERR_UNSUPPORTED, "walk on stars needs a mesh scene");
         if (split) return fail(ZG_ERR_UNSUPPORTED, "walk splitting is not available on SDF-grid scenes");
@@ -549,7 +550,6 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
         P.sdf = s->sdf;
         S.sdf = true;
     }
-    else { P.instances = b->table; P.instanceOf = instanceOf; P.nInstances = b->n; }
     P.ws = *ws;
     int64_t n = pts->n;
     P.pt = pts->pt; P.normal = pts->normal; P.type = pts->type; P.aligned = pts->estimate_boundary_normal_aligned;
