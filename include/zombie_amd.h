@@ -144,7 +144,7 @@ typedef struct zg_scene_info {
     int64_t node_bytes;          /* bytes of one packed node                              */
     int64_t prim_bytes;          /* bytes of one packed primitive                         */
     int64_t device_bytes;        /* total HBM held by the scene                           */
-    int64_t n_silhouette_vertices;
+    int64_t n_silhouette_vertices;  /* silhouette candidates: vertices (2D) / edges (3D) */
 } zg_scene_info;
 
 /* Builds the device scene from a host mesh (positions_host [n_vertices*dim], indices_host
@@ -155,8 +155,9 @@ typedef struct zg_scene_info {
  * (geometric_queries.h:376-379).  Replaces FcpwDirichletBoundaryHandler /
  * FcpwNeumannBoundaryHandler::buildAccelerationStructure (fcpw_geometric_queries.h:537-679)
  * and populateGeometricQueriesFor{Dirichlet,Neumann}Boundary (:1062-1304); for the
- * reflecting part (2D) the silhouette candidates of computeSilhouettes (:631) are built with
- * getIgnoreCandidateSilhouetteCallback(solve_double_sided, silhouette_precision) (:1029-1038). */
+ * reflecting part the silhouette candidates of computeSilhouettes (:631, :667) -- vertices in
+ * 2D, edges in 3D -- are built with getIgnoreCandidateSilhouetteCallback(solve_double_sided,
+ * silhouette_precision) (:1029-1038; 3D dihedral angle as :955-967 measures it). */
 zg_status zg_scene_create(int32_t dim, const float* positions_host, int64_t n_vertices,
                           const int32_t* indices_host, int64_t n_prims,
                           const uint8_t* is_reflecting_host,
@@ -216,7 +217,8 @@ zg_status zg_intersect_ray(const zg_scene* scene, int32_t boundary, int64_t n,
 
 /* computeStarRadiusForReflectingBoundary for a Neumann boundary (fcpw_geometric_queries.h
  * :1254-1268): the distance to the closest silhouette point of the reflecting boundary
- * within max_radius, clamped below by min_radius.  2D only. */
+ * (vertex in 2D, point on an edge in 3D) within max_radius, clamped below by min_radius.
+ * Mesh scenes. */
 zg_status zg_star_radius(const zg_scene* scene, int64_t n, const float* pts,
                          const float* min_radius, const float* max_radius,
                          float silhouette_precision, const uint8_t* flip_normal_orientation,
@@ -224,17 +226,18 @@ zg_status zg_star_radius(const zg_scene* scene, int64_t n, const float* pts,
 
 /* intersectsWithReflectingBoundary's visibility test: hasLineOfSight of the reflecting
  * aggregate (fcpw_geometric_queries.h:1218-1226) between p1[i] and p2[i] (offsets, if any,
- * applied by the caller).  visible[i] = 1 when no reflecting segment crosses the open
- * segment.  2D only. */
+ * applied by the caller).  visible[i] = 1 when no reflecting segment (2D) or triangle (3D)
+ * crosses the open segment.  Mesh scenes. */
 zg_status zg_line_of_sight(const zg_scene* scene, int64_t n, const float* p1, const float* p2,
                            uint8_t* visible, void* stream);
 
 /* sampleReflectingBoundary (fcpw_geometric_queries.h:1227-1242): a point on the reflecting
- * boundary near the ball |y - x| <= radius drawn with the random numbers u[2i], u[2i+1] by
- * a stochastic descent of the reflecting BVH (children overlapping the ball weighted by
- * branchTraversalWeight of their box centre, segments by that weight times their length).
- * ok[i] = 0 when nothing overlaps; pdf[i] is the density of the sample per unit length.
- * pt / normal / pdf nullable.  2D only. */
+ * boundary near the ball |y - x| <= radius drawn with the DIM random numbers u[DIM*i ..] by a
+ * stochastic descent of the reflecting BVH (children overlapping the ball weighted by
+ * branchTraversalWeight of their box centre, primitives by that weight times their length or
+ * area), then a uniform point on the primitive from the remaining DIM - 1 numbers.  ok[i] = 0
+ * when nothing overlaps; pdf[i] is the density of the sample per unit length (2D) or area
+ * (3D).  pt / normal / pdf nullable.  Mesh scenes. */
 zg_status zg_sample_reflecting(const zg_scene* scene, int64_t n, const float* x, const float* radius,
                                const float* u, uint8_t* ok, float* pt, float* normal, float* pdf,
                                void* stream);
@@ -354,10 +357,11 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        zg_solve_report* report_host, void* stream);
 
 /* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
- * walk :345-490); 2D scenes, Neumann or Robin reflecting conditions (h = pde.robin; with
- * are_robin_conditions_pure_neumann = 0 the throughput takes the Robin reflectance with
- * kappa = pde.robin_coeff, and zg_scene_set_robin_coefficients gives the per-segment bounds of
- * the star radius), harmonic Green's function (absorption 0).  points_host->normal, type and
+ * walk :345-490); mesh scenes, harmonic Green's function (absorption 0).  2D: Neumann or Robin
+ * reflecting conditions (h = pde.robin; with are_robin_conditions_pure_neumann = 0 the
+ * throughput takes the Robin reflectance with kappa = pde.robin_coeff, and
+ * zg_scene_set_robin_coefficients gives the per-segment bounds of the star radius).  3D:
+ * Neumann conditions (silhouette edges bound the star radius; Robin data is rejected).  points_host->normal, type and
  * estimate_boundary_normal_aligned describe points on the reflecting boundary;
  * first_sphere_radius (nullable) records the star radius of each point's first step on the
  * first call and is reused while the point has estimates.  Gradient points run

@@ -339,7 +339,8 @@ class Scene:
     def silhouette_candidates(self, x, precision=1e-3, flip=False):
         x = np.ascontiguousarray(x, np.float32)
         n = lib().orc_silhouette_candidates(self.h, _p(x), precision, int(flip), None, None)
-        v = np.empty((n, 2), np.float32)
+        # 2D: the candidate vertices [n, 2]; 3D: the candidate edges' end points [n, 2, 3]
+        v = np.empty((n, 2) if self.dim == 2 else (n, 2, 3), np.float32)
         s = np.empty(n, np.uint8)
         lib().orc_silhouette_candidates(self.h, _p(x), precision, int(flip), _p(v), _p(s))
         return v, s.astype(bool)
@@ -347,8 +348,8 @@ class Scene:
     def sample_reflecting(self, x, radius, u, bvh=False):
         x = np.ascontiguousarray(x, np.float32)
         u = np.ascontiguousarray(u, np.float32)
-        pt = np.empty(2, np.float32)
-        nr = np.empty(2, np.float32)
+        pt = np.empty(self.dim, np.float32)
+        nr = np.empty(self.dim, np.float32)
         pdf = np.empty(1, np.float32)
         fn = lib().orc_sample_reflecting_bvh if bvh else lib().orc_sample_reflecting
         ok = fn(self.h, _p(x), radius, _p(u), _p(pt), _p(nr), _p(pdf))
@@ -356,7 +357,9 @@ class Scene:
 
     def line_of_sight(self, p1, p2, bvh=False):
         fn = lib().orc_line_of_sight_bvh if bvh else lib().orc_line_of_sight
-        return bool(fn(self.h, _p(np.ascontiguousarray(p1, np.float32)), _p(np.ascontiguousarray(p2, np.float32))))
+        a = np.ascontiguousarray(p1, np.float32)   # held until the call returns (_p takes a raw address)
+        b = np.ascontiguousarray(p2, np.float32)
+        return bool(fn(self.h, _p(a), _p(b)))
 
 
 class Stats:

@@ -18,6 +18,8 @@
 //     through utils/reflectance_boundary_bvh/bvh.h:38-42,369-506 and the call sites
 //     utils/fcpw_geometric_queries.h:1069-1242
 //   silhouette candidates + closest silhouette point     fcpw_geometric_queries.h:1029-1038,1254-1268
+//     (2D vertices; 3D edges with the dihedral angle of reflectance_boundary_bvh setup,
+//     fcpw_geometric_queries.h:955-967, and FCPW's edge test restated from the WoSt paper)
 //   branchTraversalWeight                                 fcpw_geometric_queries.h:1391-1398
 //   SampleStatistics (FP64 here; float in the reference) point_estimation/common.h:200-395
 //   WalkOnSpheres::estimateSolution / walk / weight window / terminal
@@ -46,6 +48,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <thread>
 #include <type_traits>
@@ -504,6 +507,17 @@ inline float closestPointSegment(const V<2>& pa, const V<2>& pb, const V<2>& x, 
     return norm(x - pt);
 }
 
+template <int D>
+inline float closestPointSegmentD(const V<D>& pa, const V<D>& pb, const V<D>& x, V<D>& pt) {
+    V<D> u = pb - pa, v = x - pa;
+    float c1 = dot(u, v);
+    if (c1 <= 0.0f) { pt = pa; return norm(x - pa); }
+    float c2 = dot(u, u);
+    if (c2 <= c1) { pt = pb; return norm(x - pb); }
+    pt = pa + u * (c1 / c2);
+    return norm(x - pt);
+}
+
 inline float closestPointTriangle(const V<3>& pa, const V<3>& pb, const V<3>& pc, const V<3>& x, V<3>& pt) {
     V<3> ab = pb - pa, ac = pc - pa, ax = x - pa;
     float d1 = dot(ab, ax), d2 = dot(ac, ax);
@@ -704,6 +718,9 @@ struct Mesh {
     // 2D silhouette candidates: vertex -> adjacent primitive(s)
     struct SilVertex { int v; int p0; int p1; };  // p1 = -1 when the vertex has one neighbour
     std::vector<SilVertex> sil;
+    // 3D silhouette candidates: edge (va, vb) with its adjacent triangles (f1 = -1: boundary edge)
+    struct SilEdge { int va; int vb; int f0; int f1; };
+    std::vector<SilEdge> silE;
     Bvh<D> silBvh;                                 // over the silhouette candidates (point boxes)
     std::vector<std::array<float, 4>> silCone;     // per silBvh node: axis.xy, sin, cos of half-angle + margin
     float silConePrecision = 0.0f;
@@ -715,6 +732,15 @@ struct Mesh {
     int nPrims() const { return (int)origId.size(); }
     V<D> vtx(int p, int k) const { return verts[idx[p * D + k]]; }
 
+    // uniform point on primitive p from D - 1 uniform numbers (the triangle by the square-root
+    // warp: (1 - sqrt(u1)) a + sqrt(u1) (1 - u2) b + sqrt(u1) u2 c)
+    V<D> primSample(int p, const float* u) const {
+        if constexpr (D == 2) return vtx(p, 0) + (vtx(p, 1) - vtx(p, 0)) * u[0];
+        else {
+            float su = std::sqrt(u[0]);
+            return vtx(p, 0) + (vtx(p, 1) - vtx(p, 0)) * (su * (1.0f - u[1])) + (vtx(p, 2) - vtx(p, 0)) * (su * u[1]);
+        }
+    }
     V<D> primNormal(int p) const {
         if constexpr (D == 2) return segmentNormal(vtx(p, 0), vtx(p, 1));
         else return triangleNormal(vtx(p, 0), vtx(p, 1), vtx(p, 2));
@@ -771,6 +797,37 @@ struct Mesh {
             silBvh.build(pb, 4, 8);
             buildSilhouetteCones(precision);
             silConePrecision = precision;
+        } else {
+            // every edge with its first two adjacent triangles (a third one of a non-manifold
+            // edge is not recorded); f0's winding gives the edge direction a -> b
+            std::map<std::pair<int, int>, std::array<int, 3>> edges;   // (min, max) -> f0, f1, j in f0
+            for (int p = 0; p < nPrims(); p++) {
+                for (int j = 0; j < 3; j++) {
+                    int a = idx[p * 3 + j], b = idx[p * 3 + (j + 1) % 3];
+                    auto key = std::make_pair(std::min(a, b), std::max(a, b));
+                    auto it = edges.find(key);
+                    if (it == edges.end()) edges[key] = {p, -1, j};
+                    else if (it->second[1] < 0) it->second[1] = p;
+                }
+            }
+            silE.clear();
+            for (const auto& kv : edges) {
+                int f0 = kv.second[0], f1 = kv.second[1], j = kv.second[2];
+                int a = idx[f0 * 3 + j], b = idx[f0 * 3 + (j + 1) % 3];
+                if (f1 >= 0 && !solveDoubleSided) {
+                    // dihedral angle atan2(e . (n1 x n0), n0 . n1), negative at convex edges of a
+                    // surface with outward normals: ignored below the precision for one-sided problems
+                    V<3> e = verts[b] - verts[a];
+                    e = e * (1.0f / norm(e));
+                    V<3> n0 = primNormal(f0), n1 = primNormal(f1);
+                    float ang = std::atan2(dot(e, cross3(n1, n0)), dot(n0, n1));
+                    if (ang < precision) continue;
+                }
+                silE.push_back({a, b, f0, f1});
+            }
+            std::vector<Box<3>> pb(silE.size());
+            for (size_t k = 0; k < silE.size(); k++) { pb[k].expand(verts[silE[k].va]); pb[k].expand(verts[silE[k].vb]); }
+            silBvh.build(pb, 4, 8);
         }
     }
 
@@ -835,6 +892,22 @@ struct Mesh {
         if (cosSum <= 0.0f) return true;
         float ca = (c[0] * u[0] + c[1] * u[1]) * invd;
         return std::fabs(ca) <= sinSum * 1.0001f + 1e-6f;
+    }
+
+    // Silhouette edge seen from x: the vertex rule with the unit direction from the edge's
+    // closest point cp to x (both triangle planes contain the edge line, so only the
+    // normalisation depends on cp).  Boundary edges and a query on the edge are silhouettes.
+    bool isSilhouetteEdge(const SilEdge& e, const V<3>& x, const V<3>& cp, float precision, bool flip) const {
+        if (e.f1 < 0) return true;
+        V<3> view = x - cp;
+        float l = norm(view);
+        if (l <= 0.0f) return true;
+        view = view * (1.0f / l);
+        float sign = flip ? -1.0f : 1.0f;
+        float d0 = dot(primNormal(e.f0), view), d1 = dot(primNormal(e.f1), view);
+        if (std::fabs(d0) <= precision) return sign * d1 > precision;
+        if (std::fabs(d1) <= precision) return sign * d0 > precision;
+        return d0 * d1 < 0.0f;
     }
 
     bool isSilhouette(const SilVertex& s, const V<2>& x, float precision, bool flip) const {
@@ -1001,6 +1074,57 @@ inline float starRadius2Bvh(const Mesh<2>& m, const V<2>& x, float minRadius, fl
     return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
 }
 
+// closest silhouette point (3D): the closest point of the nearest silhouette edge
+inline float starRadius3(const Mesh<3>& m, const V<3>& x, float minRadius, float maxRadius, float precision,
+                         bool flip) {
+    if (minRadius > maxRadius) return maxRadius;
+    float best = FLT_MAX; bool found = false;
+    for (const auto& e : m.silE) {
+        V<3> cp;
+        float d = closestPointSegmentD<3>(m.verts[e.va], m.verts[e.vb], x, cp);
+        if (d > maxRadius || d >= best) continue;
+        if (m.isSilhouetteEdge(e, x, cp, precision, flip)) { best = d; found = true; }
+    }
+    return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
+}
+
+// the same over the silhouette-edge BVH (nearest-first, pruned by the best distance and the
+// max radius, stopped once a silhouette within min radius is found); equal to starRadius3
+inline float starRadius3Bvh(const Mesh<3>& m, const V<3>& x, float minRadius, float maxRadius, float precision,
+                            bool flip) {
+    if (minRadius > maxRadius) return maxRadius;
+    if (m.silE.empty()) return std::max(maxRadius, minRadius);
+    float best = FLT_MAX, best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
+    bool found = false;
+    const auto& nodes = m.silBvh.nodes;
+    struct E { int n; float d2; };
+    E stack[128];
+    int sp = 0;
+    stack[sp++] = {0, nodes[0].box.dist2(x)};
+    while (sp > 0) {
+        E e = stack[--sp];
+        if (e.d2 > best2) continue;
+        const auto& nd = nodes[e.n];
+        if (nd.left < 0) {
+            for (int i = nd.start; i < nd.start + nd.count; i++) {
+                const auto& se = m.silE[m.silBvh.order[i]];
+                V<3> cp;
+                float d = closestPointSegmentD<3>(m.verts[se.va], m.verts[se.vb], x, cp);
+                if (d > maxRadius || d >= best) continue;
+                if (m.isSilhouetteEdge(se, x, cp, precision, flip)) { best = d; best2 = d * d; found = true; }
+            }
+            if (found && best <= minRadius) break;
+            continue;
+        }
+        float dl = nodes[nd.left].box.dist2(x), dr = nodes[nd.right].box.dist2(x);
+        int nearN = nd.left, farN = nd.right; float dn = dl, df = dr;
+        if (dr < dl) { nearN = nd.right; farN = nd.left; dn = dr; df = dl; }
+        if (df <= best2) stack[sp++] = {farN, df};
+        if (dn <= best2) stack[sp++] = {nearN, dn};
+    }
+    return found ? std::max(best, minRadius) : std::max(maxRadius, minRadius);
+}
+
 // Robin part of the star radius of a segment (ReflectanceLineSegment::computeSquaredStarRadius,
 // reflectance_boundary_bvh/geometry.h:86-140, with RobinLineSegmentBound, robin_bounds.h:12-35):
 // inside the radius (h / c) exp(c^2 / (h kappa_max)), c = sqrt(h kappa_max)/sqrt(2) clamped to the
@@ -1042,12 +1166,13 @@ inline float robinRadius2(const Mesh<2>& m, const V<2>& x, float r2, bool brute 
     return r2;
 }
 
-// line of sight over the BVH: any reflecting segment crossing the open segment p1 -> p2
-inline bool hasLineOfSightBvh(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) {
-    V<2> d = p2 - p1; float L = norm(d);
+// line of sight over the BVH: any reflecting primitive crossing the open segment p1 -> p2
+template <int D>
+bool hasLineOfSightBvh(const Mesh<D>& m, const V<D>& p1, const V<D>& p2) {
+    V<D> d = p2 - p1; float L = norm(d);
     if (L <= 0.0f || m.nPrims() == 0) return true;
-    V<2> dir = d * (1.0f / L);
-    V<2> inv; for (int i = 0; i < 2; i++) inv[i] = 1.0f / dir[i];
+    V<D> dir = d * (1.0f / L);
+    V<D> inv; for (int i = 0; i < D; i++) inv[i] = 1.0f / dir[i];
     int stack[128]; int sp = 0; stack[sp++] = 0;
     const auto& nodes = m.bvh.nodes;
     while (sp > 0) {
@@ -1056,8 +1181,8 @@ inline bool hasLineOfSightBvh(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) 
         if (nd.left < 0) {
             for (int i = nd.start; i < nd.start + nd.count; i++) {
                 int p = m.bvh.order[i];
-                float t, s;
-                if (raySegment(p1, dir, L, m.vtx(p, 0), m.vtx(p, 1), t, s) && t > 0.0f && t < L) return false;
+                float t;
+                if (m.primRay(p, p1, dir, L, t) && t > 0.0f && t < L) return false;
             }
             continue;
         }
@@ -1066,14 +1191,15 @@ inline bool hasLineOfSightBvh(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) 
     return true;
 }
 
-// line of sight (2D): no reflecting segment crosses the open segment p1 -> p2
-inline bool hasLineOfSight(const Mesh<2>& m, const V<2>& p1, const V<2>& p2) {
-    V<2> d = p2 - p1; float L = norm(d);
+// line of sight: no reflecting primitive crosses the open segment p1 -> p2
+template <int D>
+bool hasLineOfSight(const Mesh<D>& m, const V<D>& p1, const V<D>& p2) {
+    V<D> d = p2 - p1; float L = norm(d);
     if (L <= 0.0f) return true;
-    V<2> dir = d * (1.0f / L);
+    V<D> dir = d * (1.0f / L);
     for (int p = 0; p < m.nPrims(); p++) {
-        float t, s;
-        if (raySegment(p1, dir, L, m.vtx(p, 0), m.vtx(p, 1), t, s)) {
+        float t;
+        if (m.primRay(p, p1, dir, L, t)) {
             if (t > 0.0f && t < L) return false;
         }
     }
@@ -1089,8 +1215,9 @@ inline float branchWeight(float r2) {
     float r = std::max(std::sqrt(r2), 1e-2f);
     return std::fabs(1.0f / (4.0f * PI_F * r));  // HarmonicGreensFnFreeSpace<3>::evaluate
 }
-inline bool sampleReflecting2(const Mesh<2>& m, const V<2>& x, float radius, const float* u,
-                              V<2>& pt, V<2>& nrm, float& pdf) {
+template <int D>
+bool sampleReflecting(const Mesh<D>& m, const V<D>& x, float radius, const float* u,
+                      V<D>& pt, V<D>& nrm, float& pdf) {
     float R2 = radius * radius;
     double total = 0.0;
     std::vector<std::pair<int, double>> cand;
@@ -1103,8 +1230,7 @@ inline bool sampleReflecting2(const Mesh<2>& m, const V<2>& x, float radius, con
     if (cand.empty()) return false;
     double target = u[0] * total, acc = 0.0; int chosen = cand.back().first; double wch = cand.back().second;
     for (auto& c : cand) { acc += c.second; if (target < acc) { chosen = c.first; wch = c.second; break; } }
-    V<2> pa = m.vtx(chosen, 0), pb = m.vtx(chosen, 1);
-    pt = pa + (pb - pa) * u[1];
+    pt = m.primSample(chosen, u + 1);
     nrm = m.primNormal(chosen);
     pdf = (float)(wch / total) / m.primArea(chosen);
     return true;
@@ -1115,8 +1241,9 @@ inline bool sampleReflecting2(const Mesh<2>& m, const V<2>& x, float radius, con
 // branchWeight(|box centre - x|^2); in the leaf a segment whose box meets the ball with
 // probability proportional to branchWeight x length; u[0] is rescaled at every level.  pdf is
 // the exact density of the sample (the product of the choice probabilities / length).
-inline bool sampleReflecting2Bvh(const Mesh<2>& m, const V<2>& x, float radius, const float* u,
-                                 V<2>& pt, V<2>& nrm, float& pdf) {
+template <int D>
+bool sampleReflectingBvh(const Mesh<D>& m, const V<D>& x, float radius, const float* u,
+                         V<D>& pt, V<D>& nrm, float& pdf) {
     if (m.nPrims() == 0) return false;
     const float R2 = radius * radius;
     const auto& nodes = m.bvh.nodes;
@@ -1141,8 +1268,7 @@ inline bool sampleReflecting2Bvh(const Mesh<2>& m, const V<2>& x, float radius, 
                 acc += w[j];
             }
             int p = m.bvh.order[nd.start + sel];
-            V<2> pa = m.vtx(p, 0), pb = m.vtx(p, 1);
-            pt = pa + (pb - pa) * u[1];
+            pt = m.primSample(p, u + 1);
             nrm = m.primNormal(p);
             pdf = (float)(prob * (w[sel] / total) / m.primArea(p));
             return true;
@@ -1502,7 +1628,7 @@ void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t 
     }
 }
 
-// ----- Walk on Stars (walk_on_stars.h:189-662), 2D -----
+// ----- Walk on Stars (walk_on_stars.h:189-662), 2D and 3D -----
 template <int D>
 void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBall<D>& gf, float starRadius,
                                 bool flip, Pcg32& rng, WalkState<D>& st) {
@@ -1511,7 +1637,7 @@ void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBa
     float u[D]; for (int i = 0; i < D; i++) u[i] = rng.nextFloat();
     V<D> bpt, bn; float bpdf;
     bool ok = false;
-    if constexpr (D == 2) ok = sampleReflecting2Bvh(sc.refl, st.currentPt, starRadius, u, bpt, bn, bpdf);
+    ok = sampleReflectingBvh<D>(sc.refl, st.currentPt, starRadius, u, bpt, bn, bpdf);
     if (!ok) return;
     V<D> toS = bpt - st.currentPt;
     float distS = norm(toS);
@@ -1531,7 +1657,7 @@ void wostReflectingContribution(const Scene<D>& sc, const SolveCtx& cx, GreensBa
         V<D> p1 = st.onReflectingBoundary ? offsetPoint<D>(st.currentPt, st.currentNormal * -1.0f) : st.currentPt;
         V<D> p2 = offsetPoint<D>(bpt, bnS * -1.0f);
         bool blocked = false;
-        if constexpr (D == 2) blocked = !hasLineOfSightBvh(sc.refl, p1, p2);
+        blocked = !hasLineOfSightBvh<D>(sc.refl, p1, p2);
         if (!blocked) {
             float r = std::max(gf.rClamp, distS);
             float G = gf.evaluate(r);
@@ -1557,6 +1683,9 @@ float wostStarRadius(const Scene<D>& sc, const SolveCtx& cx, const V<D>& x, floa
                 r = std::max(std::min(r, std::sqrt(r2)), minR);
             }
         }
+    } else {
+        if (sc.hasRefl())
+            r = starRadius3Bvh(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip);
     }
     if (sc.hasRefl() && ws.epsilon_shell_for_reflecting_boundary <= distA)
         r = std::max(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
@@ -2000,6 +2129,8 @@ static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_s
                      double* gM2 = nullptr, double* gFirst = nullptr, double* gDeriv = nullptr,
                      const float* distR = nullptr) {
     if (pde->absorption_coeff > 0.0f && D == 2) return ZG_ERR_UNSUPPORTED;
+    // the Robin star-radius bound is restated for segments only
+    if (wost && D == 3 && !pde->are_robin_conditions_pure_neumann) return ZG_ERR_UNSUPPORTED;
     bool anyGrad = false;
     if (quantity) for (int64_t i = 0; i < n; i++) anyGrad = anyGrad || quantity[i] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT;
     if (anyGrad && (!gN || !gMean || !gM2 || !gFirst || !gDeriv)) return ZG_ERR_INVALID_ARGUMENT;
@@ -2232,7 +2363,7 @@ void orc_robin_radius(void* hv, int64_t n, const float* pts, const float* maxR, 
 void orc_scene_stats(void* hv, int64_t* out) {
     SceneHandle* h = (SceneHandle*)hv;
     if (h->dim == 2) { auto& s = *h->s2; out[0] = s.abs.bvh.nodes.size(); out[1] = s.abs.bvh.depth; out[2] = s.refl.bvh.nodes.size(); out[3] = s.refl.bvh.depth; out[4] = s.refl.sil.size(); }
-    else { auto& s = *h->s3; out[0] = s.abs.bvh.nodes.size(); out[1] = s.abs.bvh.depth; out[2] = s.refl.bvh.nodes.size(); out[3] = s.refl.bvh.depth; out[4] = 0; }
+    else { auto& s = *h->s3; out[0] = s.abs.bvh.nodes.size(); out[1] = s.abs.bvh.depth; out[2] = s.refl.bvh.nodes.size(); out[3] = s.refl.bvh.depth; out[4] = s.refl.silE.size(); }
 }
 
 void orc_closest_point(void* hv, int boundary, int64_t n, const float* pts, float* dist, float* cp, float* nrm,
@@ -2258,16 +2389,32 @@ void orc_inside_domain(void* hv, int64_t n, const float* pts, uint8_t* inside, i
 void orc_star_radius(void* hv, int64_t n, const float* pts, const float* minR, const float* maxR, float prec,
                      const uint8_t* flip, float* out) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) { for (int64_t i = 0; i < n; i++) out[i] = maxR[i]; return; }
-    for (int64_t i = 0; i < n; i++)
-        out[i] = h->s2->hasRefl() ? starRadius2(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec,
-                                                flip && flip[i] != 0) : maxR[i];
+    for (int64_t i = 0; i < n; i++) {
+        bool fl = flip && flip[i] != 0;
+        if (h->dim == 2)
+            out[i] = h->s2->hasRefl() ? starRadius2(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec, fl) : maxR[i];
+        else
+            out[i] = h->s3->hasRefl() ? starRadius3(h->s3->refl, load<3>(pts + 3 * i), minR[i], maxR[i], prec, fl) : maxR[i];
+    }
 }
 
-// silhouette test of every candidate vertex seen from x; out[k] = 1 when candidate k is a silhouette
+// silhouette test of every candidate seen from x; out[k] = 1 when candidate k is a silhouette.
+// verts_out: the vertex (2D, 2 floats) or the edge's two end points (3D, 6 floats) per candidate
 int64_t orc_silhouette_candidates(void* hv, const float* x, float prec, int flip, float* verts_out, uint8_t* is_sil_out) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) return 0;
+    if (h->dim == 3) {
+        const auto& m = h->s3->refl;
+        for (size_t k = 0; k < m.silE.size(); k++) {
+            const auto& e = m.silE[k];
+            if (verts_out) { store<3>(verts_out + 6 * k, m.verts[e.va]); store<3>(verts_out + 6 * k + 3, m.verts[e.vb]); }
+            if (is_sil_out) {
+                V<3> cp;
+                closestPointSegmentD<3>(m.verts[e.va], m.verts[e.vb], load<3>(x), cp);
+                is_sil_out[k] = m.isSilhouetteEdge(e, load<3>(x), cp, prec, flip != 0);
+            }
+        }
+        return (int64_t)m.silE.size();
+    }
     const auto& m = h->s2->refl;
     for (size_t k = 0; k < m.sil.size(); k++) {
         if (verts_out) store<2>(verts_out + 2 * k, m.verts[m.sil[k].v]);
@@ -2276,19 +2423,29 @@ int64_t orc_silhouette_candidates(void* hv, const float* x, float prec, int flip
     return (int64_t)m.sil.size();
 }
 
+extern "C++" {
+template <int D>
+static int sampleImpl(const Mesh<D>& m, bool bvh, const float* x, float radius, const float* u, float* pt, float* nrm,
+                      float* pdf) {
+    V<D> p, nn; float pd;
+    bool ok = bvh ? sampleReflectingBvh<D>(m, load<D>(x), radius, u, p, nn, pd)
+                  : sampleReflecting<D>(m, load<D>(x), radius, u, p, nn, pd);
+    if (ok) { store<D>(pt, p); store<D>(nrm, nn); *pdf = pd; }
+    return ok;
+}
+}
+
+// u holds DIM uniform numbers: one for the primitive choice, DIM - 1 for the point on it
 int orc_sample_reflecting(void* hv, const float* x, float radius, const float* u, float* pt, float* nrm, float* pdf) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) return 0;
-    V<2> p, nn; float pd;
-    bool ok = sampleReflecting2(h->s2->refl, load<2>(x), radius, u, p, nn, pd);
-    if (ok) { store<2>(pt, p); store<2>(nrm, nn); *pdf = pd; }
-    return ok;
+    return h->dim == 2 ? sampleImpl<2>(h->s2->refl, false, x, radius, u, pt, nrm, pdf)
+                       : sampleImpl<3>(h->s3->refl, false, x, radius, u, pt, nrm, pdf);
 }
 
 int orc_line_of_sight(void* hv, const float* p1, const float* p2) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) return 1;
-    return hasLineOfSight(h->s2->refl, load<2>(p1), load<2>(p2));
+    if (h->dim == 2) return hasLineOfSight<2>(h->s2->refl, load<2>(p1), load<2>(p2));
+    return hasLineOfSight<3>(h->s3->refl, load<3>(p1), load<3>(p2));
 }
 
 // BVH-accelerated forms used by the estimator (equal to the brute-force forms above for the
@@ -2296,25 +2453,25 @@ int orc_line_of_sight(void* hv, const float* p1, const float* p2) {
 void orc_star_radius_bvh(void* hv, int64_t n, const float* pts, const float* minR, const float* maxR, float prec,
                          const uint8_t* flip, float* out) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) { for (int64_t i = 0; i < n; i++) out[i] = maxR[i]; return; }
-    for (int64_t i = 0; i < n; i++)
-        out[i] = h->s2->hasRefl() ? starRadius2Bvh(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec,
-                                                   flip && flip[i] != 0) : maxR[i];
+    for (int64_t i = 0; i < n; i++) {
+        bool fl = flip && flip[i] != 0;
+        if (h->dim == 2)
+            out[i] = h->s2->hasRefl() ? starRadius2Bvh(h->s2->refl, load<2>(pts + 2 * i), minR[i], maxR[i], prec, fl) : maxR[i];
+        else
+            out[i] = h->s3->hasRefl() ? starRadius3Bvh(h->s3->refl, load<3>(pts + 3 * i), minR[i], maxR[i], prec, fl) : maxR[i];
+    }
 }
 
 int orc_sample_reflecting_bvh(void* hv, const float* x, float radius, const float* u, float* pt, float* nrm, float* pdf) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) return 0;
-    V<2> p, nn; float pd;
-    bool ok = sampleReflecting2Bvh(h->s2->refl, load<2>(x), radius, u, p, nn, pd);
-    if (ok) { store<2>(pt, p); store<2>(nrm, nn); *pdf = pd; }
-    return ok;
+    return h->dim == 2 ? sampleImpl<2>(h->s2->refl, true, x, radius, u, pt, nrm, pdf)
+                       : sampleImpl<3>(h->s3->refl, true, x, radius, u, pt, nrm, pdf);
 }
 
 int orc_line_of_sight_bvh(void* hv, const float* p1, const float* p2) {
     SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) return 1;
-    return hasLineOfSightBvh(h->s2->refl, load<2>(p1), load<2>(p2));
+    if (h->dim == 2) return hasLineOfSightBvh<2>(h->s2->refl, load<2>(p1), load<2>(p2));
+    return hasLineOfSightBvh<3>(h->s3->refl, load<3>(p1), load<3>(p2));
 }
 
 // Solves.  rng_mode 0: one generator per point consumed across its walks (the reference's

@@ -1,7 +1,7 @@
 """Boundary value caching on the GPU: the splat kernel against the oracle's restatement on the
 same inputs (every boundary form, both orientations, source samples, cutoffs, regularization,
 repeated calls), and the whole solver (boundary_value_caching.h:178-946) on manufactured
-Dirichlet problems in 2D and 3D.  The oracle's splat itself is pinned against Green's
+Dirichlet problems in 2D and 3D and a mixed Dirichlet / Neumann problem in 3D.  The oracle's splat itself is pinned against Green's
 representation formula in tests/test_oracle_bvc.py."""
 import numpy as np
 import pytest
@@ -166,3 +166,36 @@ def test_bvc_solver_2d_dirichlet_with_source(gpu, oracle_lib):
     print(f"bvc 2d: {keep.sum()} points, rmse {rmse:.4g}, mean error {err.mean():.3g}, median se {np.median(se):.3g}")
     assert rmse < 4 * np.sqrt(np.mean(se ** 2)) + 0.01
     assert st.n[:, 4].min().item() > 0          # the source cache contributed everywhere
+
+
+def test_bvc_solver_3d_mixed_neumann(gpu, oracle_lib):
+    """L-shaped prism with Dirichlet outer walls and Neumann re-entrant walls and top cap:
+    absorbing samples take solution and normal derivative from 3D walk-on-stars gradient walks,
+    reflecting samples the solution from 3D walk-on-stars walks and the Neumann data h; the
+    splat reproduces u = e^x cos y within its standard errors."""
+    torch = gpu
+    p = PB.lprism3d(plate=False)
+    refl = p.is_reflecting.astype(bool)
+    sc = E.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max)
+    solver = BV.BoundaryValueCachingSolver(sc, BV.UniformBoundarySampler(p.positions, p.indices[~refl]),
+                                           BV.UniformBoundarySampler(p.positions, p.indices[refl]), None, seed=11)
+    solver.generate_samples(30000, 20000, 0, 2e-3, 0.0, False)
+    assert sum(len(c) for c in solver.caches) >= 50000
+    solver.compute_sample_estimates(p.pde, p.settings, 64, 64)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-0.85, 0.85, (6000, 3)).astype(np.float32)
+    x = x[p.meta["inside"](x, 0.15)][:1500]
+    xt = torch.from_numpy(x).cuda()
+    dA = sc.closest_point(xt, boundary=0, want_point=False)[0].contiguous()
+    dR = sc.closest_point(xt, boundary=1, want_point=False)[0].contiguous()
+    ev = E.BvcEvalPoints(pt=xt, dist_to_absorbing=dA, dist_to_reflecting=dR)
+    st = E.BvcStats.zeros(len(x), 3)
+    solver.splat(ev, st, radius_clamp=1e-3)
+    n = st.n.cpu().numpy()
+    assert (n[:, 0] > 0).all() and (n[:, 2] > 0).all()     # absorbing and reflecting caches contributed
+    err = st.solution().cpu().numpy() - _exp_cos(x)
+    rmse = float(np.sqrt(np.mean(err ** 2)))
+    se = _splat_se(st)
+    print(f"bvc 3d mixed: rmse {rmse:.4g}, mean error {err.mean():.3g}, median se {np.median(se):.3g}")
+    assert rmse < 4 * np.sqrt(np.mean(se ** 2)) + 0.01
+    assert abs(err.mean()) < 3 * np.median(se) + 0.005

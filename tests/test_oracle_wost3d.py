@@ -1,0 +1,194 @@
+"""Pins of the oracle's 3D walk-on-stars geometry and estimator (walk_on_stars.h:189-662 over
+triangle meshes; FCPW's closest-silhouette-edge, stochastic sphere sampling and line-of-sight
+queries behind fcpw_geometric_queries.h:1218-1270, with the dihedral-angle rule of
+:955-967 and :1029-1038).  FCPW is absent, so the silhouette queries are pinned by what the
+mathematics fixes: the candidate set of a known polyhedron, the star-shapedness the radius
+exists for (no ray from x crosses the reflecting surface twice inside it, which the plain
+absorbing distance violates on the same inputs), an open rim's closed-form distance, and the
+BVH forms against brute force; the sampler by the surface-area identity E[1{|y - x| < R} / pdf]
+= area inside the ball; the estimator by a manufactured harmonic solution."""
+import math
+
+import numpy as np
+import pytest
+
+from zombie_amd import _abi as A
+from zombie_amd import engine as E
+from zombie_amd import problems as PB
+
+
+def _interior(p, n, seed, margin=0.02):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.95, 0.95, (4 * n, 3)).astype(np.float32)
+    return x[p.meta["inside"](x, margin)][:n]
+
+
+def _scene(O, p):
+    return O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max,
+                   solve_double_sided=p.solve_double_sided)
+
+
+def _edges(I):
+    e = np.sort(np.concatenate([I[:, [0, 1]], I[:, [1, 2]], I[:, [2, 0]]]), axis=1)
+    u, c = np.unique(e, axis=0, return_counts=True)
+    return u, c
+
+
+def test_silhouette_candidates_of_the_l_prism(oracle_lib):
+    """One-sided closed L-prism: of the reflecting triangles' edges, flat and convex ones are
+    dropped; the concave re-entrant edge (n_z pieces along (0, 0, z)) and the rim between the
+    reflecting and absorbing parts (edges with one reflecting triangle) remain.  Double-sided:
+    every edge is a candidate."""
+    O = oracle_lib
+    n_sub, n_z = 4, 4
+    p = PB.lprism3d(n_sub=n_sub, n_z=n_z, plate=False)
+    sc = _scene(O, p)
+    R = p.indices[p.is_reflecting.astype(bool)]
+    u, c = _edges(R)
+    rim = int((c == 1).sum())
+    V, _ = sc.silhouette_candidates(np.zeros(3, np.float32))
+    assert len(V) == rim + n_z
+    reentrant = np.all(np.abs(V[:, :, :2]) < 1e-6, axis=(1, 2))
+    assert reentrant.sum() == n_z
+    sc2 = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    assert len(sc2.silhouette_candidates(np.zeros(3, np.float32))[0]) == len(u)
+
+
+@pytest.mark.parametrize("plate", [False, True])
+def test_star_radius_is_star_shaped_and_bvh_equals_brute_force(oracle_lib, plate):
+    O = oracle_lib
+    p = PB.lprism3d(plate=plate)
+    sc = _scene(O, p)
+    x = _interior(p, 600, 1)
+    dA = sc.closest_point(x, boundary=0)[0]
+    r = sc.star_radius(x, 1e-3, dA)
+    assert np.array_equal(r, sc.star_radius(x, 1e-3, dA, bvh=True))
+    assert np.all(r <= dA) and np.mean(r < dA - 1e-6) > 0.1
+
+    rng = np.random.default_rng(2)
+
+    def max_crossings(radius, idx):
+        worst = 0
+        for i in idx:
+            d = rng.normal(size=(48, 3)).astype(np.float32)
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            for k in range(48):
+                t0, cnt = 0.0, 0
+                while t0 < radius[i]:
+                    o = (x[i] + d[k] * t0).astype(np.float32)
+                    hit, t, _ = sc.intersect_ray(o[None], d[k][None], np.array([radius[i] - t0], np.float32), boundary=1)
+                    if not hit[0]:
+                        break
+                    cnt += 1
+                    t0 += float(t[0]) + 1e-5
+                worst = max(worst, cnt)
+        return worst
+
+    idx = np.arange(0, len(x), 6)
+    assert max_crossings(r, idx) <= 1
+    # the absorbing distance alone is not star-shaped on the same inputs
+    assert max_crossings(dA, idx) >= 2
+
+
+def test_open_rim_radius_closed_form(oracle_lib):
+    """A flat open square [-a, a]^2 x {0} seen from (0, 0, h): flat interior edges are never
+    silhouettes, every rim edge is, and the nearest rim point is an edge midpoint at
+    sqrt(h^2 + a^2)."""
+    O = oracle_lib
+    m, a = 6, 0.5
+    g = np.linspace(-a, a, m + 1)
+    V = np.array([(x, y, 0.0) for y in g for x in g], np.float32)
+    T = []
+    for j in range(m):
+        for i in range(m):
+            v = j * (m + 1) + i
+            T += [(v, v + 1, v + m + 2), (v, v + m + 2, v + m + 1)]
+    I = np.array(T, np.int32)
+    sc = O.Scene(V, I, np.ones(len(I), np.uint8), np.full(3, -2.0), np.full(3, 2.0), solve_double_sided=True)
+    for h in (0.1, 0.3, -0.2):
+        x = np.array([[0.0, 0.0, h]], np.float32)
+        r = sc.star_radius(x, 1e-4, np.array([10.0], np.float32))[0]
+        assert r == pytest.approx(math.hypot(h, a), rel=1e-5)
+    # off-centre: the nearest rim edge (x = a)
+    x = np.array([[0.3, -0.1, 0.2]], np.float32)
+    assert sc.star_radius(x, 1e-4, np.array([10.0], np.float32))[0] == pytest.approx(math.hypot(0.2, 0.2), rel=1e-5)
+
+
+@pytest.mark.parametrize("bvh", [False, True])
+def test_sampler_area_identity(oracle_lib, bvh):
+    """E[1{|y - x| < R} / pdf] over the sampler's draws equals the area of the reflecting
+    surface inside the ball: a disc of radius sqrt(R^2 - h^2) on a large flat square."""
+    O = oracle_lib
+    m = 16
+    g = np.linspace(-1.0, 1.0, m + 1)
+    V = np.array([(x, y, 0.0) for y in g for x in g], np.float32)
+    T = []
+    for j in range(m):
+        for i in range(m):
+            v = j * (m + 1) + i
+            T += [(v, v + 1, v + m + 2), (v, v + m + 2, v + m + 1)]
+    I = np.array(T, np.int32)
+    sc = O.Scene(V, I, np.ones(len(I), np.uint8), np.full(3, -2.0), np.full(3, 2.0), solve_double_sided=True)
+    x = np.array([0.1, -0.2, 0.25], np.float32)
+    R = 0.6
+    rng = np.random.default_rng(4)
+    vals = []
+    for _ in range(20000):
+        ok, y, nr, pdf = sc.sample_reflecting(x, R, rng.random(3), bvh=bvh)
+        assert ok and abs(y[2]) < 1e-6 and abs(abs(nr[2]) - 1) < 1e-6
+        vals.append((np.linalg.norm(y - x) < R) / pdf)
+    vals = np.array(vals)
+    exact = math.pi * (R * R - 0.25 ** 2)
+    assert abs(vals.mean() - exact) < 4 * vals.std() / math.sqrt(len(vals)), (vals.mean(), exact)
+
+
+def test_line_of_sight_bvh_equals_brute_force_and_plate_blocks(oracle_lib):
+    O = oracle_lib
+    p = PB.lprism3d(plate=True)
+    sc = _scene(O, p)
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-1, 1, (400, 3)).astype(np.float32)
+    b = rng.uniform(-1, 1, (400, 3)).astype(np.float32)
+    assert [sc.line_of_sight(u, v) for u, v in zip(a, b)] == [sc.line_of_sight(u, v, bvh=True) for u, v in zip(a, b)]
+    # through the plate (z = -0.3, x, y in [-0.8, -0.3], away from its triangles' shared edges)
+    # and beside it
+    assert not sc.line_of_sight([-0.52, -0.45, -0.1], [-0.52, -0.45, -0.5])
+    assert sc.line_of_sight([-0.52, -0.45, -0.1], [-0.52, -0.45, -0.2])
+    assert sc.line_of_sight([-0.9, -0.9, -0.1], [-0.9, -0.9, -0.5])
+
+
+@pytest.mark.parametrize("plate", [False, True])
+def test_walk_on_stars_3d_matches_exact(oracle_lib, plate):
+    """u = e^x cos y on the L-prism with Neumann walls, cap (and plate): 4-SE agreement; the
+    same walks with the Neumann data dropped are detected."""
+    O = oracle_lib
+    p = PB.lprism3d(plate=plate)
+    sc = _scene(O, p)
+    x = _interior(p, 200, 3)
+    d = sc.closest_point(x, boundary=0)[0]
+    st = O.Stats(len(x))
+    rep = O.solve(sc, p.pde, p.settings, x, d, 256, st, wost=True)
+    u = p.exact(x)
+    se = np.sqrt(st.variance() / st.n)
+    z = (st.mean - u) / se
+    pooled = np.sum(st.mean - u) / np.sqrt(np.sum(se ** 2))
+    assert abs(pooled) < 4.0 and 0.8 < z.std() < 1.3, (pooled, z.std())
+    assert rep["codes"][0] == 256 * len(x)
+    bad = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=E.zero(), is_source_constant=True)
+    st0 = O.Stats(len(x))
+    O.solve(sc, bad, p.settings, x, d, 256, st0, wost=True)
+    se0 = np.sqrt(st0.variance() / st0.n)
+    assert abs(np.sum(st0.mean - u) / np.sqrt(np.sum(se0 ** 2))) > 10.0
+
+
+def test_robin_walk_on_stars_3d_is_unsupported(oracle_lib):
+    """the Robin star-radius bound is stated for segments only"""
+    O = oracle_lib
+    p = PB.lprism3d(plate=False)
+    sc = _scene(O, p)
+    x = _interior(p, 4, 6)
+    d = sc.closest_point(x, boundary=0)[0]
+    pde = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True,
+                     robin_coeff=E.constant(1.0))
+    with pytest.raises(RuntimeError):
+        O.solve(sc, pde, p.settings, x, d, 4, O.Stats(len(x)), wost=True)

@@ -227,7 +227,7 @@ class BoundaryValueCachingSolver:
             st = E.DeviceStats.zeros(n, dim=self.dim)
             opts = E.SolveOptions(seed=self.seed + 1000003 * (self.calls + 1))
             self.calls += 1
-            if self.dim == 2 and self.scene.info.n_reflecting_prims > 0:
+            if self.scene.info.n_reflecting_prims > 0:
                 E.wost_solve(self.scene, pde, ws, dp, st, opts)
             else:
                 E.wos_solve(self.scene, pde, ws, dp, st, opts)
@@ -282,7 +282,7 @@ class BoundaryValueCachingSolver:
         st = E.DeviceStats.zeros(n)
         opts = E.SolveOptions(seed=self.seed + 7777777 + self.calls)
         self.calls += 1
-        if self.dim == 2 and self.scene.info.n_reflecting_prims > 0:
+        if self.scene.info.n_reflecting_prims > 0:
             E.wost_solve(self.scene, pde, ws, dp, st, opts)
         else:
             E.wos_solve(self.scene, pde, ws, dp, st, opts)

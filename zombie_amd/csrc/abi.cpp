@@ -241,6 +241,8 @@ zg_status zg_scene_create(int32_t dim, const float* V, int64_t nV, const int32_t
         if (dim == 2 && !selR.empty()) {
             buildSilhouettes2D(o.bvh_width, V, nV, I, selR, dbl != 0, prec, s->refl.host);
             s->refl.silConePrecision = prec;   // the cones' margin covers queries up to this precision
+        } else if (dim == 3 && !selR.empty()) {
+            buildSilhouettes3D(o.bvh_width, V, nV, I, selR, dbl != 0, prec, s->refl.host);
         }
     } catch (const std::exception& e) {
         delete s;
@@ -386,7 +388,7 @@ zg_status zg_intersect_ray(const zg_scene* s, int32_t boundary, int64_t n, const
 zg_status zg_star_radius(const zg_scene* s, int64_t n, const float* pts, const float* minR, const float* maxR,
                          float prec, const uint8_t* flip, float* radius, void* stream) {
     if (!s || n < 0 || (n > 0 && (!pts || !minR || !maxR || !radius))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
-    if (s->dim != 2 || s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "star radius queries run on 2D mesh scenes");
+    if (s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "star radius queries run on mesh scenes");
     ZG_HIP(launchStarRadius(s->view, n, pts, minR, maxR, prec, flip, radius, (hipStream_t)stream));
     return ok();
 }
@@ -426,7 +428,7 @@ zg_status zg_probe_read_bandwidth(const void* buf, int64_t bytes, int32_t reps, 
 zg_status zg_line_of_sight(const zg_scene* s, int64_t n, const float* p1, const float* p2, uint8_t* visible,
                            void* stream) {
     if (!s || n < 0 || (n > 0 && (!p1 || !p2 || !visible))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
-    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "line-of-sight queries run on 2D scenes");
+    if (s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "line-of-sight queries run on mesh scenes");
     ZG_HIP(launchLineOfSight(s->view, n, p1, p2, visible, (hipStream_t)stream));
     return ok();
 }
@@ -434,7 +436,7 @@ zg_status zg_line_of_sight(const zg_scene* s, int64_t n, const float* p1, const 
 zg_status zg_sample_reflecting(const zg_scene* s, int64_t n, const float* x, const float* radius, const float* u,
                                uint8_t* ok_, float* pt, float* normal, float* pdf, void* stream) {
     if (!s || n < 0 || (n > 0 && (!x || !radius || !u || !ok_))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
-    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "reflecting-boundary sampling runs on 2D scenes");
+    if (s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "reflecting-boundary sampling runs on mesh scenes");
     ZG_HIP(launchSampleReflecting(s->view, n, x, radius, u, ok_, pt, normal, pdf, (hipStream_t)stream));
     return ok();
 }
@@ -500,7 +502,11 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
         if (split)
             return fail(ZG_ERR_UNSUPPORTED, "splitting_threshold %g < 1: walk splitting runs in walk on spheres only",
                         ws->splitting_threshold);
-        if (!s || dim != 2) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on 2D scenes");
+        if (!s) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on single scenes");
+        if (dim == 3 && pde->absorption_coeff > 0.0f)
+            return fail(ZG_ERR_UNSUPPORTED, "3D walk on stars runs the harmonic kernel (absorption 0)");
+        if (dim == 3 && !pde->are_robin_conditions_pure_neumann)
+            return fail(ZG_ERR_UNSUPPORTED, "Robin conditions run in 2D walk on stars");
     }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))
@@ -633,9 +639,9 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
         P.nWalks = masked;
         P.quantity = nullptr;
     }
-    if (e == hipSuccess && S.wost) e = launchWostFirstSphere(P, S.width, S.callerFsr, const_cast<float*>(P.fsr), strm);
+    if (e == hipSuccess && S.wost) e = launchWostFirstSphere(P, S.dim, S.width, S.callerFsr, const_cast<float*>(P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
-        e = S.wost ? launchWost(P, S.width, S.count, S.blocks, strm)
+        e = S.wost ? launchWost(P, S.dim, S.width, S.count, S.blocks, strm)
             : S.batch ? launchWosBatch(P, S.width, S.count, S.screened, S.split, S.blocks, strm)
             : S.sdf ? launchWosSdf(P, S.dim, S.screened, S.blocks, strm)
                     : launchWos(P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
@@ -709,7 +715,7 @@ hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipS
     if (e == hipSuccess)
         e = launchStratified(D - 1, const_cast<float*>(P.strat), n, L.stride, 2 * L.pairsMax, P.nWalks, P.quantity,
                              L.halves, P.seed, P.streamOffset, strm);
-    if (e == hipSuccess && S.wost) e = launchWostGradFirstSphere(P, S.width, const_cast<float*>(P.fsr), strm);
+    if (e == hipSuccess && S.wost) e = launchWostGradFirstSphere(P, S.dim, S.width, const_cast<float*>(P.fsr), strm);
     const bool cv = S.P.ws.use_gradient_control_variates != 0;
     for (int u0 = 0; e == hipSuccess && u0 < L.pairsMax;) {
         int u1 = cv ? (u0 == 0 ? 1 : std::min(2 * u0, L.pairsMax)) : L.pairsMax;
@@ -722,7 +728,7 @@ hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipS
         P.chunksPerPoint = std::max(1, (P.gradW1 - P.gradW0 + P.wpl - 1) / P.wpl);
         P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
         if (e == hipSuccess)
-            e = S.wost ? launchWostGrad(P, S.width, S.blocks, strm)
+            e = S.wost ? launchWostGrad(P, S.dim, S.width, S.blocks, strm)
                        : launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
         if (e == hipSuccess) e = launchMergeStats(M, D, strm);
         u0 = u1;

@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 
 namespace zg {
@@ -367,6 +368,77 @@ void buildSilhouettes2D(int width, const float* V, int64_t nV, const int32_t* I,
         std::memcpy(&out.sil[(size_t)i * kSilFloats], &cand[(size_t)order[i] * kSilFloats], kSilFloats * 4);
     if (width == 4) buildSilhouetteCones<4>(out, precision);
     else buildSilhouetteCones<8>(out, precision);
+}
+
+// 3D silhouette candidates of a reflecting triangle mesh: every edge with its (first two)
+// adjacent triangles; an edge with one triangle (an open edge) is always a silhouette.  With
+// one-sided problems an edge whose dihedral angle atan2(e . (n1 x n0), n0 . n1) -- e the edge
+// direction in the winding of its first triangle, n0 and n1 the unit normals of its first and
+// second triangles, as fcpw_geometric_queries.h:955-967 measures it -- is below the precision
+// (flat or convex) is dropped, as getIgnoreCandidateSilhouetteCallback does (:1029-1038).
+// Candidate layout: a.xyz, open; b.xyz, 0; n0.xyz, 0; n1.xyz, 0 (kSil3Floats).
+void buildSilhouettes3D(int width, const float* V, int64_t nV, const int32_t* I, const std::vector<int>& sel,
+                        bool solveDoubleSided, float precision, BuiltBoundary& out) {
+    (void)nV;
+    auto normal = [&](int p, float* n) {
+        const float* a = V + (size_t)I[(size_t)p * 3] * 3;
+        const float* b = V + (size_t)I[(size_t)p * 3 + 1] * 3;
+        const float* c = V + (size_t)I[(size_t)p * 3 + 2] * 3;
+        float u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        n[0] = u[1] * v[2] - u[2] * v[1]; n[1] = u[2] * v[0] - u[0] * v[2]; n[2] = u[0] * v[1] - u[1] * v[0];
+        float l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        n[0] /= l; n[1] /= l; n[2] /= l;
+    };
+    struct EdgeFaces { int f0, f1, a, b; };
+    std::map<std::pair<int, int>, EdgeFaces> edges;
+    for (int p : sel) {
+        for (int j = 0; j < 3; j++) {
+            int a = I[(size_t)p * 3 + j], b = I[(size_t)p * 3 + (j + 1) % 3];
+            auto key = std::make_pair(std::min(a, b), std::max(a, b));
+            auto it = edges.find(key);
+            if (it == edges.end()) edges[key] = {p, -1, a, b};
+            else if (it->second.f1 < 0) it->second.f1 = p;
+        }
+    }
+    std::vector<float> cand;
+    for (const auto& kv : edges) {
+        const EdgeFaces& e = kv.second;
+        float n0[3], n1[3] = {0.0f, 0.0f, 0.0f};
+        normal(e.f0, n0);
+        if (e.f1 >= 0) {
+            normal(e.f1, n1);
+            if (!solveDoubleSided) {
+                const float* pa = V + (size_t)e.a * 3;
+                const float* pb = V + (size_t)e.b * 3;
+                float d[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+                float l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                d[0] /= l; d[1] /= l; d[2] /= l;
+                float cr[3] = {n1[1] * n0[2] - n1[2] * n0[1], n1[2] * n0[0] - n1[0] * n0[2], n1[0] * n0[1] - n1[1] * n0[0]};
+                float ang = std::atan2(d[0] * cr[0] + d[1] * cr[1] + d[2] * cr[2], n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2]);
+                if (ang < precision) continue;
+            }
+        }
+        const float* pa = V + (size_t)e.a * 3;
+        const float* pb = V + (size_t)e.b * 3;
+        float c[kSil3Floats] = {pa[0], pa[1], pa[2], e.f1 < 0 ? 1.0f : 0.0f, pb[0], pb[1], pb[2], 0.0f,
+                                n0[0], n0[1], n0[2], 0.0f, n1[0], n1[1], n1[2], 0.0f};
+        cand.insert(cand.end(), c, c + kSil3Floats);
+    }
+    int n = (int)(cand.size() / kSil3Floats);
+    std::vector<Box<3>> boxes(n);
+    for (int i = 0; i < n; i++) {
+        boxes[i].expandPt(&cand[(size_t)i * kSil3Floats]);
+        boxes[i].expandPt(&cand[(size_t)i * kSil3Floats + 4]);
+    }
+    std::vector<int> order;
+    int depth = 0;
+    if (width == 4) buildWide<3, 4>(boxes, 4, 16, out.silNodes, order, out.nSilNodes, depth);
+    else buildWide<3, 8>(boxes, 4, 16, out.silNodes, order, out.nSilNodes, depth);
+    out.nSil = n;
+    out.sil.resize(cand.size());
+    for (int i = 0; i < n; i++)
+        std::memcpy(&out.sil[(size_t)i * kSil3Floats], &cand[(size_t)order[i] * kSil3Floats], kSil3Floats * 4);
+    out.silCones.clear();
 }
 
 }  // namespace zg

@@ -23,5 +23,7 @@ void buildBoundary(int dim, int width, int leafSize, int bins, const float* V, c
 
 void buildSilhouettes2D(int width, const float* V, int64_t nV, const int32_t* I, const std::vector<int>& sel,
                         bool solveDoubleSided, float precision, BuiltBoundary& out);
+void buildSilhouettes3D(int width, const float* V, int64_t nV, const int32_t* I, const std::vector<int>& sel,
+                        bool solveDoubleSided, float precision, BuiltBoundary& out);
 
 }  // namespace zg

@@ -1225,4 +1225,209 @@ ZG_DEV bool sampleReflecting2(const BoundaryView& b, const Vec<2>& x, float radi
     return false;
 }
 
+// ------------------------------------------------------------------------------------
+// Walk-on-stars queries on the 3D reflecting boundary (triangle meshes)
+// ------------------------------------------------------------------------------------
+
+ZG_DEV float closestOnSegment3(const Vec<3>& pa, const Vec<3>& pb, const Vec<3>& x, Vec<3>& pt) {
+    Vec<3> u = pb - pa, v = x - pa;
+    float c1 = dot(u, v);
+    float c2 = dot(u, u);
+    pt = c1 <= 0.0f ? pa : (c2 <= c1 ? pb : pa + u * (c1 / c2));
+    return norm2(x - pt);
+}
+
+// Silhouette test of a 3D candidate edge seen from x: the vertex rule of isSilhouette2 with
+// the unit direction from the edge's closest point cp to x and the normals of the edge's two
+// triangles (both planes contain the edge line).  Open edges (one triangle) and a query on
+// the edge are always silhouettes.
+ZG_DEV bool isSilhouette3(const float4& c0, const float4& c2, const float4& c3, const Vec<3>& x, const Vec<3>& cp,
+                          float precision, bool flip) {
+    if (c0.w != 0.0f) return true;
+    Vec<3> view = x - cp;
+    float l = __builtin_sqrtf(norm2(view));
+    if (l <= 0.0f) return true;
+    view = view * (1.0f / l);
+    float sign = flip ? -1.0f : 1.0f;
+    float d0 = c2.x * view[0] + c2.y * view[1] + c2.z * view[2];
+    float d1 = c3.x * view[0] + c3.y * view[1] + c3.z * view[2];
+    if (fabsf(d0) <= precision) return sign * d1 > precision;
+    if (fabsf(d1) <= precision) return sign * d0 > precision;
+    return d0 * d1 < 0.0f;
+}
+
+// computeStarRadiusForReflectingBoundary in 3D: distance to the closest point of the nearest
+// silhouette edge within maxRadius, clamped below by minRadius (maxRadius clamped below by
+// minRadius when there is none).  Nearest-first traversal of the candidate-edge BVH, stopped
+// once a silhouette closer than minRadius is found.
+template <int W>
+ZG_DEV float starRadius3(const BoundaryView& b, const Vec<3>& x, float minRadius, float maxRadius, float precision,
+                         bool flip, Stack& st) {
+    if (minRadius > maxRadius) return maxRadius;
+    if (b.nSilNodes == 0) return fmaxf(maxRadius, minRadius);
+    float best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
+    const float min2 = minRadius * minRadius;
+    bool found = false;
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<3, W> nd;
+        nd.load(b.silNodes, node);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (!isLeaf(nd.ch[k]) || nd.dist2(k, x) > best2) continue;
+            int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
+            for (int i = first; i < first + cnt; i++) {
+                const float4* c = reinterpret_cast<const float4*>(b.sil) + 4 * i;
+                float4 c0 = c[0], c1 = c[1];
+                Vec<3> pa, pb, cp;
+                pa[0] = c0.x; pa[1] = c0.y; pa[2] = c0.z;
+                pb[0] = c1.x; pb[1] = c1.y; pb[2] = c1.z;
+                float dd = closestOnSegment3(pa, pb, x, cp);
+                if (dd < best2 && isSilhouette3(c0, c[2], c[3], x, cp, precision, flip)) { best2 = dd; found = true; }
+            }
+        }
+        if (found && best2 <= min2) break;
+        int next = -1; float ndist = 0.0f; int nk = -1;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float dk = nd.dist2(k, x);
+            if (!isLeaf(nd.ch[k]) && dk <= best2 && (next < 0 || dk < ndist)) { next = nd.ch[k]; ndist = dk; nk = k; }
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float dk = nd.dist2(k, x);
+            if (!isLeaf(nd.ch[k]) && dk <= best2 && k != nk) st.push(nd.ch[k], dk);
+        }
+        if (next < 0) {
+            int n; float dn;
+            bool got = false;
+            while (st.pop(n, dn)) { if (dn <= best2) { got = true; break; } }
+            if (!got) break;
+            next = n;
+        }
+        node = next;
+    }
+    return found ? fmaxf(__builtin_sqrtf(best2), minRadius) : fmaxf(maxRadius, minRadius);
+}
+
+// hasLineOfSight in 3D: true unless a triangle of b crosses the open segment p1 -> p2.
+template <int W>
+ZG_DEV bool lineOfSight3(const BoundaryView& b, const Vec<3>& p1, const Vec<3>& p2, Stack& st) {
+    if (b.nNodes == 0) return true;
+    Vec<3> d = p2 - p1;
+    float L = __builtin_sqrtf(norm2(d));
+    if (L <= 0.0f) return true;
+    Vec<3> dir = d * (1.0f / L);
+    Vec<3> inv;
+#pragma unroll
+    for (int a = 0; a < 3; a++) inv[a] = 1.0f / dir[a];
+    st.sp = 0; st.ovf = 0;
+    int node = 0;
+    for (;;) {
+        WideNode<3, W> nd;
+        nd.load(b.nodes, node);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float t0 = 0.0f, t1 = L;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                float ta = (nd.lo[a][k] - p1[a]) * inv[a], tb = (nd.hi[a][k] - p1[a]) * inv[a];
+                t0 = fmaxf(t0, fminf(ta, tb));
+                t1 = fminf(t1, fmaxf(ta, tb));
+            }
+            if (t0 > t1) continue;
+            if (isLeaf(nd.ch[k])) {
+                int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
+                for (int i = first; i < first + cnt; i++) {
+                    Prim<3> pr; pr.load(b.prims, i);
+                    float t;
+                    if (pr.ray(p1, dir, L, t) && t > 0.0f && t < L) return false;
+                }
+            } else {
+                st.push(nd.ch[k], t0);
+            }
+        }
+        int n; float tn;
+        if (!st.pop(n, tn)) break;
+        node = n;
+    }
+    return true;
+}
+
+// sampleReflectingBoundary in 3D: the stochastic descent of sampleReflecting2 over the
+// triangle BVH (children and leaf triangles whose boxes overlap the ball, weighted by
+// branchWeight of the box-centre distance, triangles also by their area), then a uniform
+// point on the triangle from (u1, u2) by the square-root warp.  pdf is the exact density.
+template <int W>
+ZG_DEV bool sampleReflecting3(const BoundaryView& b, const Vec<3>& x, float radius, float u0, float u1, float u2,
+                              Vec<3>& pt, Vec<3>& nrm, float& pdf) {
+    if (b.nNodes == 0) return false;
+    const float R2 = radius * radius;
+    float prob = 1.0f;
+    float u = u0;
+    int node = 0;
+    for (int depth = 0; depth < 64; depth++) {
+        WideNode<3, W> nd;
+        nd.load(b.nodes, node);
+        float w[W];
+        float total = 0.0f;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float cx = 0.5f * (nd.lo[0][k] + nd.hi[0][k]) - x[0];
+            float cy = 0.5f * (nd.lo[1][k] + nd.hi[1][k]) - x[1];
+            float cz = 0.5f * (nd.lo[2][k] + nd.hi[2][k]) - x[2];
+            w[k] = nd.dist2(k, x) <= R2 ? branchWeight(cx * cx + cy * cy + cz * cz) : 0.0f;
+            total += w[k];
+        }
+        if (!(total > 0.0f)) return false;
+        float t = u * total, acc = 0.0f;
+        int sel = -1;
+        float wSel = 0.0f, accSel = 0.0f;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (w[k] > 0.0f) {
+                if (sel < 0 || t >= acc) { sel = k; wSel = w[k]; accSel = acc; }
+                acc += w[k];
+            }
+        }
+        prob *= wSel / total;
+        u = fminf(fmaxf((t - accSel) / wSel, 0.0f), 0.99999994f);
+        int c = nd.ch[sel];
+        if (!isLeaf(c)) { node = c; continue; }
+        int first = leafFirstD(c), cnt = leafCountD(c);
+        float lw[16];
+        float ltotal = 0.0f;
+        for (int j = 0; j < cnt; j++) {
+            Prim<3> pr; pr.load(b.prims, first + j);
+            float d2 = 0.0f, m2 = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                float lo = fminf(fminf(pr.a[a], pr.b[a]), pr.c[a]), hi = fmaxf(fmaxf(pr.a[a], pr.b[a]), pr.c[a]);
+                float dd = fmaxf(fmaxf(lo - x[a], x[a] - hi), 0.0f);
+                float mm = 0.5f * (lo + hi) - x[a];
+                d2 += dd * dd; m2 += mm * mm;
+            }
+            float area = 0.5f * __builtin_sqrtf(norm2(cross3(pr.b - pr.a, pr.c - pr.a)));
+            lw[j] = d2 <= R2 ? branchWeight(m2) * area : 0.0f;
+            ltotal += lw[j];
+        }
+        if (!(ltotal > 0.0f)) return false;
+        float lt = u * ltotal, lacc = 0.0f;
+        int ps = -1;
+        for (int j = 0; j < cnt; j++) {
+            if (lw[j] > 0.0f && (ps < 0 || lt >= lacc)) ps = j;
+            lacc += lw[j];
+        }
+        Prim<3> pr; pr.load(b.prims, first + ps);
+        float area = 0.5f * __builtin_sqrtf(norm2(cross3(pr.b - pr.a, pr.c - pr.a)));
+        float su = __builtin_sqrtf(u1);
+        pt = pr.a + (pr.b - pr.a) * (su * (1.0f - u2)) + (pr.c - pr.a) * (su * u2);
+        nrm = pr.normal();
+        pdf = prob * (lw[ps] / ltotal) / area;
+        return true;
+    }
+    return false;
+}
+
 }  // namespace zg

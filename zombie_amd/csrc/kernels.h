@@ -158,10 +158,10 @@ hipError_t launchBvcSplat(int dim, const zg_bvc_samples& S, const zg_bvc_eval_po
 // read-bandwidth probe
 hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks, float* sink, hipStream_t s);
 // walk on stars (2D)
-hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s);
-hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s);
-hipError_t launchWostGradFirstSphere(const WosParams& P, int width, float* fsr, hipStream_t s);
-hipError_t launchWostGrad(const WosParams& P, int width, int gridBlocks, hipStream_t s);
+hipError_t launchWostFirstSphere(const WosParams& P, int dim, int width, const float* callerFsr, float* fsr, hipStream_t s);
+hipError_t launchWost(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchWostGradFirstSphere(const WosParams& P, int dim, int width, float* fsr, hipStream_t s);
+hipError_t launchWostGrad(const WosParams& P, int dim, int width, int gridBlocks, hipStream_t s);
 hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
                             float prec, const uint8_t* flip, float* radius, hipStream_t s);
 hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, const float* p2, uint8_t* visible,

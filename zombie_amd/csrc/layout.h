@@ -36,7 +36,8 @@ struct PrimLayout {
 };
 
 // silhouette candidate (2D reflecting boundary): v.xy, n0.xy, n1.xy, single-neighbour flag, pad
-constexpr int kSilFloats = 8;
+constexpr int kSilFloats = 8;    // 2D candidate: vertex.xy, n0.xy, n1.xy, open end, pad
+constexpr int kSil3Floats = 16;  // 3D candidate edge: a.xyz, open edge; b.xyz, 0; n0.xyz, 0; n1.xyz, 0
 
 inline int leafFirst(int child) { return (~child) >> 4; }
 inline int leafCount(int child) { return ((~child) & 15) + 1; }
@@ -47,8 +48,8 @@ struct BoundaryView {
     const float* nodes;    // nNodes * NodeLayout::kFloats
     const float* prims;    // nPrims * PrimLayout::kFloats (leaf order)
     const int32_t* primIds;// 2D: caller index per packed primitive
-    const float* sil;      // 2D reflecting: silhouette candidates (kSilFloats each)
-    const float* silNodes; // 2D reflecting: wide BVH over silhouette candidates
+    const float* sil;      // reflecting: silhouette candidates (kSilFloats each in 2D, kSil3Floats in 3D)
+    const float* silNodes; // reflecting: wide BVH over the silhouette candidates
     const float* silCones; // per silhouette-BVH child: normal cone (axis.xy, sin, cos of half-angle + margin)
     float silConePrecision;// the silhouette precision the cones' margin covers
     const float* robinMax; // 2D reflecting: largest Robin coefficient per packed primitive (null: 0)

@@ -1,13 +1,13 @@
-// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D, Neumann or Robin reflecting boundary) for
-// gfx950, its per-point first-sphere prologue, and the batched star-radius, line-of-sight and
-// reflecting-boundary sampling queries.
+// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D Neumann or Robin, 3D Neumann reflecting
+// boundaries) for gfx950, its per-point first-sphere prologue, and the batched star-radius,
+// line-of-sight and reflecting-boundary sampling queries.
 //
 // Work decomposition is the one of the walk-on-spheres kernel: claims of `wpl` walks of one
 // point, handed to the idle lanes of a wave by ballot + mbcnt every `regroup` iterations (or
 // as soon as the whole wave is idle), FP64 per-point atomics, a merge kernel for the caller's
 // statistics.  One iteration advances every active lane by one full walk-on-stars step; the
 // queries of a step run to completion inside it:
-//   star radius     closest silhouette vertex of the reflecting boundary (silhouette BVH)
+//   star radius     closest silhouette vertex (2D) / edge (3D) of the reflecting boundary
 //   ray             first reflecting-boundary hit within the star radius
 //   boundary sample stochastic descent of the reflecting BVH + line-of-sight test
 //   distance        closest point of the absorbing boundary for the next step
@@ -28,33 +28,67 @@ namespace zg {
 
 // star radius of a walk: closest silhouette within [epsR, distA], shrunk by 1 % for numerical
 // robustness when the reflecting boundary is not empty (walk_on_stars.h:370-392,576-590)
-template <int W, bool ROBIN = true>
-ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, const Vec<2>& x, float distA, bool flip,
+template <int D, int W, bool ROBIN = true>
+ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, const Vec<D>& x, float distA, bool flip,
                             Stack& st) {
     float r = distA;
     if (sc.refl.nPrims > 0) {
         const float minR = ws.epsilon_shell_for_reflecting_boundary;
-        r = starRadius2<W>(sc.refl, x, minR, distA, ws.silhouette_precision, flip, st);
-        // Robin segments bound the radius too (fcpw_geometric_queries.h:1272-1289)
-        if (ROBIN && sc.refl.robinMax && minR <= distA)
-            r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), minR);
+        if constexpr (D == 2) {
+            r = starRadius2<W>(sc.refl, x, minR, distA, ws.silhouette_precision, flip, st);
+            // Robin segments bound the radius too (fcpw_geometric_queries.h:1272-1289)
+            if (ROBIN && sc.refl.robinMax && minR <= distA)
+                r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), minR);
+        } else {
+            r = starRadius3<W>(sc.refl, x, minR, distA, ws.silhouette_precision, flip, st);
+        }
         if (minR <= distA) r = fmaxf(0.99f * r, minR);
     }
     return r;
 }
 
 // PDE::hasNonZeroRobinCoeff (pde.h:62-78)
-ZG_DEV bool hasNonZeroRobinCoeff(const zg_pde& pde, const Vec<2>& x) {
+template <int D>
+ZG_DEV bool hasNonZeroRobinCoeff(const zg_pde& pde, const Vec<D>& x) {
     if (pde.are_robin_conditions_pure_neumann) return false;
-    Vec<2> n = vzero<2>();
+    Vec<D> n = vzero<D>();
     n[0] = 1.0f;
-    return fabsf(evalField<2>(pde.robin_coeff, x, n, true, 0.0f)) > 0.0f ||
-           fabsf(evalField<2>(pde.robin_coeff, x, n, false, 0.0f)) > 0.0f;
+    return fabsf(evalField<D>(pde.robin_coeff, x, n, true, 0.0f)) > 0.0f ||
+           fabsf(evalField<D>(pde.robin_coeff, x, n, false, 0.0f)) > 0.0f;
+}
+
+// first reflecting-boundary sample of a step: a point, its normal and density (DIM uniform
+// numbers, as sampleReflectingBoundary draws them), then the visibility test
+template <int D, int W>
+ZG_DEV bool sampleReflectingD(const BoundaryView& b, const Vec<D>& x, float radius, Pcg32& rng, Vec<D>& pt, Vec<D>& nrm,
+                              float& pdf) {
+    if constexpr (D == 2) {
+        float u0 = rng.nextFloat();
+        float u1 = rng.nextFloat();
+        return sampleReflecting2<W>(b, x, radius, u0, u1, pt, nrm, pdf);
+    } else {
+        float u0 = rng.nextFloat();
+        float u1 = rng.nextFloat();
+        float u2 = rng.nextFloat();
+        return sampleReflecting3<W>(b, x, radius, u0, u1, u2, pt, nrm, pdf);
+    }
+}
+template <int D, int W>
+ZG_DEV bool lineOfSightD(const BoundaryView& b, const Vec<D>& p1, const Vec<D>& p2, Stack& st) {
+    if constexpr (D == 2) return lineOfSight2<W>(b, p1, p2, st);
+    else return lineOfSight3<W>(b, p1, p2, st);
+}
+// point where the ray o + t dir meets reflecting primitive i: the parametric point on the
+// segment in 2D (as the reference reports hits), o + t dir on a triangle
+template <int D>
+ZG_DEV Vec<D> hitPointD(const float* prims, int i, const Vec<D>& o, const Vec<D>& dir, float t) {
+    if constexpr (D == 2) return segmentHitPoint(prims, i, o, dir, t);
+    else return o + dir * t;
 }
 
 // first sphere radius per point (estimateSolution :566-590); a point with earlier estimates
 // keeps the caller's recorded radius
-template <int W>
+template <int D, int W>
 __global__ void __launch_bounds__(256) wostFirstSphereKernel(WosParams P, const float* __restrict__ callerFsr,
                                                              float* __restrict__ fsr) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,18 +102,18 @@ __global__ void __launch_bounds__(256) wostFirstSphereKernel(WosParams P, const 
     if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) r = 0.0f;
     else if (callerFsr && P.prevN && P.prevN[i] > 0) r = callerFsr[i];
     else if (d <= ws.epsilon_shell_for_absorbing_boundary || ws.steps_before_using_maximal_spheres == 0) r = d;
-    else if (ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && hasNonZeroRobinCoeff(P.pde, vload<2>(P.pt + 2 * i)))
+    else if (ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && hasNonZeroRobinCoeff<D>(P.pde, vload<D>(P.pt + D * i)))
         r = ws.epsilon_shell_for_reflecting_boundary;
     else {
         bool flip = ws.solve_double_sided && ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && P.aligned && P.aligned[i];
-        r = wostStarRadius<W>(P.scene, ws, vload<2>(P.pt + 2 * i), d, flip, st);
+        r = wostStarRadius<D, W>(P.scene, ws, vload<D>(P.pt + D * i), d, flip, st);
     }
     fsr[i] = r;
 }
 
 // first ball of the gradient walks of walk on stars (walk_on_stars.h:678-681): radius
 // 0.99 min(d_absorbing, d_reflecting), the reflecting distance taken from the caller or queried
-template <int W>
+template <int D, int W>
 __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, float* __restrict__ fsr) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.nPoints || P.quantity[i] != ZG_ESTIMATE_SOLUTION_AND_GRADIENT) return;
@@ -91,9 +125,9 @@ __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, fl
     } else if (P.scene.refl.nPrims > 0) {
         VisitCount vc{0, 0};
         float b2 = FLT_MAX;
-        Vec<2> cp = vzero<2>();
+        Vec<D> cp = vzero<D>();
         int prim = -1;
-        if (closestPoint<2, W, false>(P.scene.refl, vload<2>(P.pt + 2 * i), b2, cp, prim, st, vc)) dR = __builtin_sqrtf(b2);
+        if (closestPoint<D, W, false>(P.scene.refl, vload<D>(P.pt + D * i), b2, cp, prim, st, vc)) dR = __builtin_sqrtf(b2);
     }
     fsr[i] = 0.99f * fminf(P.dist0[i], dR);
 }
@@ -103,8 +137,10 @@ __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, fl
 // it -- the explicit first step of the walk-on-spheres gradient walks inside the first ball (no
 // boundary in it), then walk-on-stars steps from the boundary point.
 // ROBIN = true compiles the Robin reflectance and star-radius bound in (non-Neumann PDEs).
-template <int W, bool COUNT, bool GRAD = false, bool ROBIN = false>
+// ROBIN runs in 2D only (the Robin star-radius bound is stated for segments).
+template <int D, int W, bool COUNT, bool GRAD = false, bool ROBIN = false>
 __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
+    static_assert(!ROBIN || D == 2, "Robin conditions run in 2D");
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
     VisitCount vc{0, 0};
@@ -125,62 +161,62 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     // walk state (WalkState, common.h:93-130)
     bool active = false;
     int p = 0, w = 0, wEnd = 0;
-    Vec<2> x = vzero<2>(), nrm = vzero<2>(), prevDir = vzero<2>();
+    Vec<D> x = vzero<D>(), nrm = vzero<D>(), prevDir = vzero<D>();
     float prevDist = FLT_MAX, T = 1.0f, Sref = 0.0f, Ssrc = 0.0f, distA = 0.0f, fsr0 = 0.0f;
     int len = 0;
     int absPrim = -1;   // closest absorbing primitive of the last distance query (seeds the next)
     bool onRefl = false, flip = false, firstStep = true;
-    Ball<2> ball; ball.yukawa = false; ball.sqrtLambda = 0.0f; ball.R = 0.0f;
+    Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = 0.0f; ball.R = 0.0f;
     Pcg32 rng; rng.state = 0; rng.inc = 1;
     // gradient walks: boundary / source gradient directions, first-step source term, control variates
-    Vec<2> gB = vzero<2>(), gS = vzero<2>();
+    Vec<D> gB = vzero<D>(), gS = vzero<D>();
     float fsc = 0.0f, cvB = 0.0f, cvS = 0.0f;
 
     // first step of a gradient walk (walk_on_stars.h:705-791)
     auto startGradWalk = [&]() {
         const int pair = P.gradHalves == 2 ? (w >> 1) : w;
         const bool mirror = P.gradHalves == 2 && (w & 1);
-        const Vec<2> c0 = vload<2>(P.pt + 2 * (size_t)p);
+        const Vec<D> c0 = vload<D>(P.pt + D * (size_t)p);
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
         ball.update(P.fsr[p]);
-        const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + 2 * (size_t)pair;
-        fsc = 0.0f; gS = vzero<2>();
+        const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + (size_t)(D - 1) * 2 * (size_t)pair;
+        fsc = 0.0f; gS = vzero<D>();
         if (!ws.ignore_source_contribution) {
             float gNorm = ball.norm();
             if (pde.is_source_constant) {
-                fsc = gNorm * evalField<2>(pde.source, c0, vzero<2>(), false, 0.0f);
+                fsc = gNorm * evalField<D>(pde.source, c0, vzero<D>(), false, 0.0f);
             } else {
                 float pdf;
-                Vec<2> sd = sphereFromU<2>(tab);
+                Vec<D> sd = sphereFromU<D>(tab);
                 float r = ball.sampleRadiusPdf(rng, pdf);
-                Vec<2> ys = c0 + sd * r;
+                Vec<D> ys = c0 + sd * r;
                 if (mirror) ys = c0 - (ys - c0);
-                fsc = gNorm * evalField<2>(pde.source, ys, vzero<2>(), false, 0.0f);
+                fsc = gNorm * evalField<D>(pde.source, ys, vzero<D>(), false, 0.0f);
                 gS = (ys - c0) * (ball.gradientNorm(r) / (pdf * gNorm));
             }
             Ssrc += fsc;
         }
-        Vec<2> bd;
+        Vec<D> bd;
         float bpdf;
         if (ws.use_cosine_sampling_for_derivatives) {
-            bd = hemisphereCosineFromU<2>(tab + 1);
-            if (rng.nextFloat() < 0.5f) bd[1] = -bd[1];
-            bpdf = 0.5f * pdfHemisphereCosine<2>(fabsf(bd[1]));
-            Vec<2> nd = vzero<2>();
+            bd = hemisphereCosineFromU<D>(tab + (D - 1));
+            if (rng.nextFloat() < 0.5f) bd[D - 1] = -bd[D - 1];
+            bpdf = 0.5f * pdfHemisphereCosine<D>(fabsf(bd[D - 1]));
+            Vec<D> nd = vzero<D>();
             nd[0] = 1.0f;
-            if (P.dirDeriv) nd = vload<2>(P.dirDeriv + 2 * (size_t)p);
+            if (P.dirDeriv) nd = vload<D>(P.dirDeriv + D * (size_t)p);
             bd = fromFrame(nd, bd);
         } else {
-            bd = sphereFromU<2>(tab + 1);
-            bpdf = 1.0f / (2.0f * kPi);
+            bd = sphereFromU<D>(tab + (D - 1));
+            bpdf = D == 2 ? 1.0f / (2.0f * kPi) : 1.0f / (4.0f * kPi);
         }
-        Vec<2> yb = c0 + bd * ball.R;
+        Vec<D> yb = c0 + bd * ball.R;
         if (mirror) yb = c0 - (yb - c0);
         prevDist = ball.R;
         prevDir = (yb - c0) * (1.0f / ball.R);
         x = yb;
-        nrm = vzero<2>();
+        nrm = vzero<D>();
         onRefl = false; flip = false;
         T *= ball.poissonKernel() / bpdf;
         gB = (yb - c0) * (ball.poissonKernelGradientScale() / (bpdf * T));
@@ -193,9 +229,9 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         // the walk goes on from the boundary point with an exact absorbing distance and a star
         // radius computed at its first step
         float b2 = FLT_MAX;
-        Vec<2> cp = x;
+        Vec<D> cp = x;
         int prim = -1;
-        bool gotA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc);
+        bool gotA = closestPoint<D, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc);
         distA = gotA ? __builtin_sqrtf(b2) : FLT_MAX;
         absPrim = gotA ? prim : -1;
         fsr0 = 0.0f;
@@ -207,9 +243,9 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     auto startWalk = [&]() {
         if (GRAD) { startGradWalk(); return; }
         uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
-        Vec<2> n = P.normal ? vload<2>(P.normal + 2 * (size_t)p) : vzero<2>();
+        Vec<D> n = P.normal ? vload<D>(P.normal + D * (size_t)p) : vzero<D>();
         bool al = P.aligned ? P.aligned[p] != 0 : false;
-        x = vload<2>(P.pt + 2 * (size_t)p);
+        x = vload<D>(P.pt + D * (size_t)p);
         nrm = n; prevDir = n; prevDist = FLT_MAX; flip = false;
         if (ws.solve_double_sided && ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && al) {
             nrm = n * -1.0f; prevDir = n * -1.0f; flip = true;
@@ -232,18 +268,18 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             if (code == ZG_REACHED_ABSORBING_BOUNDARY && !ws.ignore_absorbing_boundary_contribution) {
                 // project to the absorbing boundary (estimateSolution :627-645)
                 float b2 = FLT_MAX;
-                Vec<2> cp = x;
+                Vec<D> cp = x;
                 int prim = -1;
-                Vec<2> xp = x, nn = nrm;
+                Vec<D> xp = x, nn = nrm;
                 bool al = false;
-                if (closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, 1.0f, 0.0f, nullptr, absPrim)) {
-                    Prim<2> pr; pr.load(sc.abs.prims, prim);
-                    Vec<2> n = pr.normal();
+                if (closestPoint<D, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, 1.0f, 0.0f, nullptr, absPrim)) {
+                    Prim<D> pr; pr.load(sc.abs.prims, prim);
+                    Vec<D> n = pr.normal();
                     float sd = dot(x - cp, n);
                     xp = cp; nn = n;
                     al = ws.solve_double_sided && sd > 0.0f;
                 }
-                term = evalField<2>(pde.dirichlet, xp, nn, al, 0.0f);
+                term = evalField<D>(pde.dirichlet, xp, nn, al, 0.0f);
             }
             float total = T * term + Sref + Ssrc;
             double c = (double)total;
@@ -253,19 +289,19 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             if (GRAD) {
                 // gradient estimate and directional derivative (walk_on_stars.h:818-838)
                 float bc = total - fsc;
-                Vec<2> nd = vzero<2>();
+                Vec<D> nd = vzero<D>();
                 nd[0] = 1.0f;
-                if (P.dirDeriv) nd = vload<2>(P.dirDeriv + 2 * (size_t)p);
+                if (P.dirDeriv) nd = vload<D>(P.dirDeriv + D * (size_t)p);
                 float dd = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 2; i++) {
+                for (int i = 0; i < D; i++) {
                     float bi = (bc - cvB) * gB[i];
                     float si = (fsc - cvS) * gS[i];
                     dd += bi * nd[i];
                     dd += si * nd[i];
                     double e = (double)(bi + si);
-                    atomicAdd(&P.accG[2 * (size_t)p + i], e);
-                    atomicAdd(&P.accG2[2 * (size_t)p + i], e * e);
+                    atomicAdd(&P.accG[D * (size_t)p + i], e);
+                    atomicAdd(&P.accG2[D * (size_t)p + i], e * e);
                 }
                 atomicAdd(&P.accFirst[p], (double)fsc);
                 atomicAdd(&P.accDeriv[p], (double)dd);
@@ -331,23 +367,23 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 flip = true;
             }
             if (ws.steps_before_using_maximal_spheres <= len) R = distA;
-            else if (robin && onRefl && hasNonZeroRobinCoeff(pde, x)) R = ws.epsilon_shell_for_reflecting_boundary;
-            else R = wostStarRadius<W, ROBIN>(sc, ws, x, distA, flip, st);
+            else if (robin && onRefl && hasNonZeroRobinCoeff<D>(pde, x)) R = ws.epsilon_shell_for_reflecting_boundary;
+            else R = wostStarRadius<D, W, ROBIN>(sc, ws, x, distA, flip, st);
         }
         ball.update(R);
-        Vec<2> dir = sampleUnitSphere<2>(rng);
+        Vec<D> dir = sampleUnitSphere<D>(rng);
         if (onRefl && dot(nrm, dir) > 0.0f) dir = dir * -1.0f;
 
         // first reflecting-boundary hit within the star (intersectReflectingBoundary)
-        Vec<2> o = onRefl ? offsetPoint<2>(x, nrm * -1.0f) : x;
+        Vec<D> o = onRefl ? offsetPoint<D>(x, nrm * -1.0f) : x;
         float tHit = R;
         int hitPrim = -1;
-        bool hit = hasRefl && intersectRay<2, W>(sc.refl, o, dir, R, tHit, hitPrim, st);
-        Vec<2> ipt, inrm = vzero<2>();
+        bool hit = hasRefl && intersectRay<D, W>(sc.refl, o, dir, R, tHit, hitPrim, st);
+        Vec<D> ipt, inrm = vzero<D>();
         float idist;
         if (hit) {
-            ipt = segmentHitPoint(sc.refl.prims, hitPrim, o, dir, tHit);
-            Prim<2> pr; pr.load(sc.refl.prims, hitPrim);
+            ipt = hitPointD<D>(sc.refl.prims, hitPrim, o, dir, tHit);
+            Prim<D> pr; pr.load(sc.refl.prims, hitPrim);
             inrm = pr.normal();
             idist = tHit;
         } else {
@@ -357,12 +393,10 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
 
         // reflecting boundary contribution (computeReflectingBoundaryContribution :189-250)
         if (hasRefl && !ws.ignore_reflecting_boundary_contribution) {
-            float u0 = rng.nextFloat();
-            float u1 = rng.nextFloat();
-            Vec<2> bpt, bn;
+            Vec<D> bpt, bn;
             float bpdf = 0.0f;
-            if (sampleReflecting2<W>(sc.refl, x, R, u0, u1, bpt, bn, bpdf)) {
-                Vec<2> toS = bpt - x;
+            if (sampleReflectingD<D, W>(sc.refl, x, R, rng, bpt, bn, bpdf)) {
+                Vec<D> toS = bpt - x;
                 float distS = __builtin_sqrtf(norm2(toS));
                 float alpha = onRefl ? 2.0f : 1.0f;
                 bool al = false;
@@ -375,13 +409,13 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                         if (flipS) al = true;
                     }
                 }
-                Vec<2> bnS = bn * (al ? -1.0f : 1.0f);
+                Vec<D> bnS = bn * (al ? -1.0f : 1.0f);
                 if (bpdf > 0.0f && distS < R) {
-                    Vec<2> p1 = onRefl ? offsetPoint<2>(x, nrm * -1.0f) : x;
-                    Vec<2> p2 = offsetPoint<2>(bpt, bnS * -1.0f);
-                    if (lineOfSight2<W>(sc.refl, p1, p2, st)) {
+                    Vec<D> p1 = onRefl ? offsetPoint<D>(x, nrm * -1.0f) : x;
+                    Vec<D> p2 = offsetPoint<D>(bpt, bnS * -1.0f);
+                    if (lineOfSightD<D, W>(sc.refl, p1, p2, st)) {
                         float G = ball.evaluate(fmaxf(1e-4f, distS));
-                        float h = evalField<2>(pde.robin, bpt, bn, ws.solve_double_sided && al, 0.0f);
+                        float h = evalField<D>(pde.robin, bpt, bn, ws.solve_double_sided && al, 0.0f);
                         Sref += T * alpha * G * h / bpdf;
                     }
                 }
@@ -391,7 +425,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         // source contribution along the walk direction (computeSourceContribution :252-275)
         if (!ws.ignore_source_contribution) {
             float r = ball.sampleRadius(rng);
-            if (r <= idist) Ssrc += T * ball.norm() * evalField<2>(pde.source, x + dir * r, vzero<2>(), false, 0.0f);
+            if (r <= idist) Ssrc += T * ball.norm() * evalField<D>(pde.source, x + dir * r, vzero<D>(), false, 0.0f);
         }
 
         // move; throughput (computeWalkStepThroughput :277-307): 1 for Neumann conditions and
@@ -401,10 +435,10 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         x = ipt;
         nrm = inrm;
         onRefl = hit;
-        if (robin && onRefl && prevDist > FLT_EPSILON) {
+        if constexpr (ROBIN) if (robin && onRefl && prevDist > FLT_EPSILON) {
             bool fl = ws.solve_double_sided && dot(prevDir, nrm) < 0.0f;
-            Vec<2> nr = fl ? nrm * -1.0f : nrm;
-            float kappa = evalField<2>(pde.robin_coeff, x, nrm, fl, 0.0f);
+            Vec<D> nr = fl ? nrm * -1.0f : nrm;
+            float kappa = evalField<D>(pde.robin_coeff, x, nrm, fl, 0.0f);
             float refl = 1.0f;
             if (fabsf(kappa) > 0.0f) {
                 float Pk = dot(nr, prevDir) / prevDist;
@@ -413,7 +447,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             }
             T *= fminf(fmaxf(refl, 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
         }
-        if (!onRefl && outsideDomain<2>(sc.dmin, sc.dmax, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
+        if (!onRefl && outsideDomain<D>(sc.dmin, sc.dmax, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
         if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :309-343
             float survival = T / ws.russian_roulette_threshold;
             if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
@@ -423,12 +457,12 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         if (len >= ws.max_walk_length) { finishWalk(ZG_EXCEEDED_MAX_WALK_LENGTH); continue; }
         // distance to the absorbing boundary for the next step
         float b2 = FLT_MAX;
-        Vec<2> cp = x;
+        Vec<D> cp = x;
         int prim = -1;
         // with distance_tolerance > 0 the query returns a certified lower bound of the distance
         // (exact within 2 epsilon of the boundary): a smaller maximal star radius, never a larger one
         float rad2 = FLT_MAX;
-        bool gotA = closestPoint<2, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, P.relax2, P.guard2, &rad2, absPrim);
+        bool gotA = closestPoint<D, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc, P.relax2, P.guard2, &rad2, absPrim);
         distA = gotA ? __builtin_sqrtf(rad2) : FLT_MAX;
         absPrim = gotA ? prim : -1;
         firstStep = false;
@@ -446,7 +480,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
 // ------------------------------------------------------------------------------------
 // batched reflecting-boundary queries
 // ------------------------------------------------------------------------------------
-template <int W>
+template <int D, int W>
 __global__ void __launch_bounds__(256) starRadiusKernel(SceneView sv, int64_t n, const float* __restrict__ pts,
                                                         const float* __restrict__ minR, const float* __restrict__ maxR,
                                                         float prec, const uint8_t* __restrict__ flip,
@@ -456,38 +490,48 @@ __global__ void __launch_bounds__(256) starRadiusKernel(SceneView sv, int64_t n,
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
     bool fl = flip && flip[i] != 0;
-    Vec<2> x = vload<2>(pts + 2 * i);
-    float r = sv.refl.nPrims == 0 ? maxR[i] : starRadius2<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
-    if (sv.refl.robinMax && minR[i] <= maxR[i]) {
-        float m = maxR[i];
-        r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sv.refl, x, m < FLT_MAX ? m * m : FLT_MAX, st))), minR[i]);
+    Vec<D> x = vload<D>(pts + D * i);
+    float r;
+    if (sv.refl.nPrims == 0) {
+        r = maxR[i];
+    } else if constexpr (D == 2) {
+        r = starRadius2<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
+        if (sv.refl.robinMax && minR[i] <= maxR[i]) {
+            float m = maxR[i];
+            r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sv.refl, x, m < FLT_MAX ? m * m : FLT_MAX, st))), minR[i]);
+        }
+    } else {
+        r = starRadius3<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
     }
     radius[i] = r;
 }
 
-template <int W>
+template <int D, int W>
 __global__ void __launch_bounds__(256) lineOfSightKernel(SceneView sv, int64_t n, const float* __restrict__ p1,
                                                          const float* __restrict__ p2, uint8_t* __restrict__ visible) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
-    visible[i] = lineOfSight2<W>(sv.refl, vload<2>(p1 + 2 * i), vload<2>(p2 + 2 * i), st) ? 1 : 0;
+    visible[i] = lineOfSightD<D, W>(sv.refl, vload<D>(p1 + D * i), vload<D>(p2 + D * i), st) ? 1 : 0;
 }
 
-template <int W>
+// u: DIM uniform numbers per query (primitive choice, then the point on it)
+template <int D, int W>
 __global__ void __launch_bounds__(256) sampleReflectingKernel(SceneView sv, int64_t n, const float* __restrict__ x,
                                                               const float* __restrict__ radius, const float* __restrict__ u,
                                                               uint8_t* __restrict__ ok, float* __restrict__ pt,
                                                               float* __restrict__ normal, float* __restrict__ pdf) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Vec<2> q = vzero<2>(), nn = vzero<2>();
+    Vec<D> q = vzero<D>(), nn = vzero<D>();
     float pd = 0.0f;
-    bool got = sampleReflecting2<W>(sv.refl, vload<2>(x + 2 * i), radius[i], u[2 * i], u[2 * i + 1], q, nn, pd);
+    bool got;
+    if constexpr (D == 2) got = sampleReflecting2<W>(sv.refl, vload<2>(x + 2 * i), radius[i], u[2 * i], u[2 * i + 1], q, nn, pd);
+    else got = sampleReflecting3<W>(sv.refl, vload<3>(x + 3 * i), radius[i], u[3 * i], u[3 * i + 1], u[3 * i + 2], q, nn, pd);
     ok[i] = got ? 1 : 0;
-    if (pt) vstore<2>(pt + 2 * i, q);
-    if (normal) vstore<2>(normal + 2 * i, nn);
+    if (pt) vstore<D>(pt + D * i, q);
+    if (normal) vstore<D>(normal + D * i, nn);
     if (pdf) pdf[i] = got ? pd : 0.0f;
 }
 
@@ -496,45 +540,61 @@ __global__ void __launch_bounds__(256) sampleReflectingKernel(SceneView sv, int6
 // ------------------------------------------------------------------------------------
 static inline unsigned wostBlocksFor(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-hipError_t launchWostFirstSphere(const WosParams& P, int width, const float* callerFsr, float* fsr, hipStream_t s) {
+// one launch of a kernel template over (dim, width)
+#define ZG_DW(K, dim, width, ...)                                                   \
+    do {                                                                            \
+        if ((dim) == 2) {                                                           \
+            if ((width) == 4) hipLaunchKernelGGL((K<2, 4>), __VA_ARGS__);           \
+            else hipLaunchKernelGGL((K<2, 8>), __VA_ARGS__);                        \
+        } else {                                                                    \
+            if ((width) == 4) hipLaunchKernelGGL((K<3, 4>), __VA_ARGS__);           \
+            else hipLaunchKernelGGL((K<3, 8>), __VA_ARGS__);                        \
+        }                                                                           \
+    } while (0)
+
+hipError_t launchWostFirstSphere(const WosParams& P, int dim, int width, const float* callerFsr, float* fsr,
+                                 hipStream_t s) {
     if (P.nPoints <= 0) return hipSuccess;
     dim3 g(wostBlocksFor(P.nPoints)), bl(256);
-    if (width == 4) hipLaunchKernelGGL((wostFirstSphereKernel<4>), g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
-    else hipLaunchKernelGGL((wostFirstSphereKernel<8>), g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
+    ZG_DW(wostFirstSphereKernel, dim, width, g, bl, kStackLdsBytes, s, P, callerFsr, fsr);
     return hipGetLastError();
 }
 
-hipError_t launchWostGradFirstSphere(const WosParams& P, int width, float* fsr, hipStream_t s) {
+hipError_t launchWostGradFirstSphere(const WosParams& P, int dim, int width, float* fsr, hipStream_t s) {
     if (P.nPoints <= 0) return hipSuccess;
-    dim3 g((unsigned)((P.nPoints + 255) / 256)), bl(256);
-    if (width == 4) hipLaunchKernelGGL((wostGradFirstSphereKernel<4>), g, bl, kStackLdsBytes, s, P, fsr);
-    else hipLaunchKernelGGL((wostGradFirstSphereKernel<8>), g, bl, kStackLdsBytes, s, P, fsr);
+    dim3 g(wostBlocksFor(P.nPoints)), bl(256);
+    ZG_DW(wostGradFirstSphereKernel, dim, width, g, bl, kStackLdsBytes, s, P, fsr);
     return hipGetLastError();
 }
 
-hipError_t launchWostGrad(const WosParams& P, int width, int gridBlocks, hipStream_t s) {
+hipError_t launchWostGrad(const WosParams& P, int dim, int width, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
     const bool robin = !P.pde.are_robin_conditions_pure_neumann;
-    if (width == 4) {
-        if (robin) hipLaunchKernelGGL((wostKernel<4, false, true, true>), g, bl, kStackLdsBytes, s, P);
-        else hipLaunchKernelGGL((wostKernel<4, false, true, false>), g, bl, kStackLdsBytes, s, P);
+#define ZG_WOSTG(Dd, Wd, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, false, true, R>), g, bl, kStackLdsBytes, s, P)
+    if (dim == 3) {
+        if (width == 4) ZG_WOSTG(3, 4, false); else ZG_WOSTG(3, 8, false);
+    } else if (width == 4) {
+        if (robin) ZG_WOSTG(2, 4, true); else ZG_WOSTG(2, 4, false);
     } else {
-        if (robin) hipLaunchKernelGGL((wostKernel<8, false, true, true>), g, bl, kStackLdsBytes, s, P);
-        else hipLaunchKernelGGL((wostKernel<8, false, true, false>), g, bl, kStackLdsBytes, s, P);
+        if (robin) ZG_WOSTG(2, 8, true); else ZG_WOSTG(2, 8, false);
     }
+#undef ZG_WOSTG
     return hipGetLastError();
 }
 
-hipError_t launchWost(const WosParams& P, int width, bool count, int gridBlocks, hipStream_t s) {
+hipError_t launchWost(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
     const bool robin = !P.pde.are_robin_conditions_pure_neumann;
-#define ZG_WOST(Wd, C, R) hipLaunchKernelGGL((wostKernel<Wd, C, false, R>), g, bl, kStackLdsBytes, s, P)
-    if (width == 4) {
-        if (robin) { if (count) ZG_WOST(4, true, true); else ZG_WOST(4, false, true); }
-        else { if (count) ZG_WOST(4, true, false); else ZG_WOST(4, false, false); }
+#define ZG_WOST(Dd, Wd, C, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, C, false, R>), g, bl, kStackLdsBytes, s, P)
+    if (dim == 3) {
+        if (width == 4) { if (count) ZG_WOST(3, 4, true, false); else ZG_WOST(3, 4, false, false); }
+        else { if (count) ZG_WOST(3, 8, true, false); else ZG_WOST(3, 8, false, false); }
+    } else if (width == 4) {
+        if (robin) { if (count) ZG_WOST(2, 4, true, true); else ZG_WOST(2, 4, false, true); }
+        else { if (count) ZG_WOST(2, 4, true, false); else ZG_WOST(2, 4, false, false); }
     } else {
-        if (robin) { if (count) ZG_WOST(8, true, true); else ZG_WOST(8, false, true); }
-        else { if (count) ZG_WOST(8, true, false); else ZG_WOST(8, false, false); }
+        if (robin) { if (count) ZG_WOST(2, 8, true, true); else ZG_WOST(2, 8, false, true); }
+        else { if (count) ZG_WOST(2, 8, true, false); else ZG_WOST(2, 8, false, false); }
     }
 #undef ZG_WOST
     return hipGetLastError();
@@ -544,8 +604,7 @@ hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, co
                             float prec, const uint8_t* flip, float* radius, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 g(wostBlocksFor(n)), bl(256);
-    if (sv.width == 4) hipLaunchKernelGGL((starRadiusKernel<4>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, flip, radius);
-    else hipLaunchKernelGGL((starRadiusKernel<8>), g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, flip, radius);
+    ZG_DW(starRadiusKernel, sv.dim, sv.width, g, bl, kStackLdsBytes, s, sv, n, pts, minR, maxR, prec, flip, radius);
     return hipGetLastError();
 }
 
@@ -553,8 +612,7 @@ hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, co
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 g(wostBlocksFor(n)), bl(256);
-    if (sv.width == 4) hipLaunchKernelGGL((lineOfSightKernel<4>), g, bl, kStackLdsBytes, s, sv, n, p1, p2, visible);
-    else hipLaunchKernelGGL((lineOfSightKernel<8>), g, bl, kStackLdsBytes, s, sv, n, p1, p2, visible);
+    ZG_DW(lineOfSightKernel, sv.dim, sv.width, g, bl, kStackLdsBytes, s, sv, n, p1, p2, visible);
     return hipGetLastError();
 }
 
@@ -562,9 +620,10 @@ hipError_t launchSampleReflecting(const SceneView& sv, int64_t n, const float* x
                                   uint8_t* ok, float* pt, float* normal, float* pdf, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 g(wostBlocksFor(n)), bl(256);
-    if (sv.width == 4) hipLaunchKernelGGL((sampleReflectingKernel<4>), g, bl, 0, s, sv, n, x, radius, u, ok, pt, normal, pdf);
-    else hipLaunchKernelGGL((sampleReflectingKernel<8>), g, bl, 0, s, sv, n, x, radius, u, ok, pt, normal, pdf);
+    ZG_DW(sampleReflectingKernel, sv.dim, sv.width, g, bl, 0, s, sv, n, x, radius, u, ok, pt, normal, pdf);
     return hipGetLastError();
 }
+
+#undef ZG_DW
 
 }  // namespace zg

@@ -274,12 +274,13 @@ class Scene:
         return out
 
     def sample_reflecting(self, x, radius, u):
+        """u: [n, dim] uniform numbers (primitive choice, then the point on it)."""
         torch = _torch()
         n = x.shape[0]
         dev = x.device
         ok = torch.empty(n, dtype=torch.uint8, device=dev)
-        pt = torch.empty((n, 2), dtype=torch.float32, device=dev)
-        nr = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float32, device=dev)
+        nr = torch.empty((n, self.dim), dtype=torch.float32, device=dev)
         pdf = torch.empty(n, dtype=torch.float32, device=dev)
         A.check(A.lib().zg_sample_reflecting(self._h, n, _ptr(x), _ptr(radius), _ptr(u), _ptr(ok), _ptr(pt), _ptr(nr),
                                              _ptr(pdf), _stream_ptr()))
@@ -308,6 +309,14 @@ class DevicePoints:
     @property
     def n(self) -> int:
         return int(self.pt.shape[0])
+
+    def check_dim(self, dim: int):
+        """The C-ABI takes plain pointers: points, normals and derivative directions must hold
+        `dim` floats per point (the scene's dimension)."""
+        for name in ("pt", "normal", "direction_for_derivative"):
+            a = getattr(self, name)
+            if a is not None and (a.dim() != 2 or a.shape[0] != self.n or a.shape[1] != dim):
+                raise ValueError(f"{name} has shape {tuple(a.shape)}; the scene needs [{self.n}, {dim}]")
 
     def to_c(self) -> A.zg_sample_points:
         s = A.zg_sample_points()
@@ -440,6 +449,7 @@ def stratified_samples(dims: int, n_tables: int, n_samples: int, seed: int, stre
 def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
               opts: SolveOptions | None = None, stream=None) -> SolveReport:
     opts = opts or SolveOptions()
+    points.check_dim(scene.dim)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
@@ -496,6 +506,7 @@ class Batch:
 def batch_wos_solve(batch: Batch, ws: A.zg_walk_settings, points: DevicePoints, instance_of_point,
                     stats: DeviceStats, opts: SolveOptions | None = None, stream=None) -> SolveReport:
     opts = opts or SolveOptions()
+    points.check_dim(3)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
@@ -513,6 +524,7 @@ class SolveGraph:
                  stats: DeviceStats, opts: SolveOptions | None = None):
         opts = opts or SolveOptions()
         self._keep = (batch, points, instance_of_point, stats)
+        points.check_dim(3)
         sp = points.to_c()
         st = stats.to_c()
         oc = opts.to_c()
@@ -544,6 +556,7 @@ class SolveGraph:
 def wost_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
                opts: SolveOptions | None = None, stream=None) -> SolveReport:
     opts = opts or SolveOptions()
+    points.check_dim(scene.dim)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()

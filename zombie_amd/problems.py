@@ -10,6 +10,8 @@ benchmark configurations:
   mixed2d    2D Laplace, 4096-segment star curve + 32 interior slits, Neumann on the slits
              and on the arc t in [0, pi], u = x^3 - 3 x y^2
   screened   screened Poisson instances u = exp(a.x), |a|^2 = sigma, on geodesic spheres
+  lprism3d   3D Laplace on an L-shaped prism with Neumann re-entrant walls, top cap and an
+             interior double-sided plate, u = e^x cos y (walk on stars in 3D)
 """
 from __future__ import annotations
 
@@ -303,6 +305,73 @@ def mixed2d_robin(kappa=2.0, n_segments=1024, n_slits=8, seed=0x5EED, n_walks=25
     p.meta["kappa"] = kappa
     p.name = "mixed2d_robin"
     return p
+
+
+def lprism3d(n_sub=4, n_z=4, plate=True, n_walks=256, eps=1e-3, max_len=16384) -> Problem:
+    """L-shaped prism ([-1, 1]^2 minus the quadrant (0, 1]^2) x [-1, 1] with outward normals,
+    u = e^x cos y (harmonic in 3D).  Neumann data on the two re-entrant walls (x = 0 and y = 0
+    inside the notch) and on the top cap z = 1, Dirichlet elsewhere; with `plate` a
+    double-sided Neumann square z = -0.3, x, y in [-0.8, -0.3] inside the domain (an open
+    surface: its rim edges are silhouettes from everywhere).  The re-entrant edge along
+    (0, 0, z) is the concave silhouette candidate of the closed part.  Polygon edges are split
+    into n_sub pieces and the height into n_z layers.  meta['inside'] tests interior points."""
+    poly = np.array([(-1, -1), (1, -1), (1, 0), (0, 0), (0, 1), (-1, 1)], np.float64)
+    ring, edge_of = [], []
+    for e in range(6):
+        a, b = poly[e], poly[(e + 1) % 6]
+        for k in range(n_sub):
+            ring.append(a + (b - a) * (k / n_sub))
+            edge_of.append(e)
+    ring = np.array(ring)
+    nb = len(ring)
+    zs = np.linspace(-1.0, 1.0, n_z + 1)
+    V = np.concatenate([np.column_stack([ring, np.full(nb, z)]) for z in zs])
+    tris, refl = [], []
+    notch = {2, 3}                     # polygon edges (1,0)->(0,0) and (0,0)->(0,1)
+    for k in range(n_z):
+        for i in range(nb):
+            j = (i + 1) % nb
+            a0, b0, a1, b1 = k * nb + i, k * nb + j, (k + 1) * nb + i, (k + 1) * nb + j
+            r = edge_of[i] in notch
+            tris += [(a0, b0, b1), (a0, b1, a1)]
+            refl += [r, r]
+    c = 3 * n_sub                      # ring index of the re-entrant vertex (0, 0)
+    for i in range(nb):
+        j = (i + 1) % nb
+        if edge_of[i] in notch:        # segments on the lines through c: degenerate fan triangles
+            continue
+        top, bot = n_z * nb, 0
+        tris.append((top + c, top + i, top + j)); refl.append(True)
+        tris.append((bot + c, bot + j, bot + i)); refl.append(False)
+    if plate:
+        m = 2 * n_sub
+        g = np.linspace(-0.8, -0.3, m + 1)
+        base = len(V)
+        PV = np.array([(x, y, -0.3) for y in g for x in g])
+        V = np.concatenate([V, PV])
+        for yi in range(m):
+            for xi in range(m):
+                v00 = base + yi * (m + 1) + xi
+                v10, v01, v11 = v00 + 1, v00 + m + 1, v00 + m + 2
+                tris += [(v00, v10, v11), (v00, v11, v01)]
+                refl += [True, True]
+    V = V.astype(np.float32)
+    I = np.array(tris, np.int32)
+    refl = np.array(refl, np.uint8)
+    lo = np.full(3, -1.0 - 1e-3, np.float32)
+    hi = np.full(3, 1.0 + 1e-3, np.float32)
+    u = E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_VALUE, (1.0, 1.0))
+    h = E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_NORMAL_DERIVATIVE, (1.0, 1.0))
+    pde = E.make_pde(source=E.zero(), dirichlet=u, robin=h, is_source_constant=True)
+    ws = E.walk_settings_host(eps, eps, max_len, plate)
+    prob = Problem("lprism3d", 3, V, I, refl, lo, hi, pde, ws, n_walks, exact_fn(exp_cos), solve_double_sided=plate)
+
+    def inside(x, margin=0.0):
+        x = np.asarray(x)
+        box = np.all(np.abs(x) < 1.0 - margin, axis=1)
+        return box & ~((x[:, 0] > -margin) & (x[:, 1] > -margin))
+    prob.meta["inside"] = inside
+    return prob
 
 
 def sphere_sdf_grid(dim=3, res=64, radius=1.0, margin=0.2):
