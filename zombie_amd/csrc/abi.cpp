@@ -761,11 +761,17 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
             return fail(ZG_ERR_INVALID_ARGUMENT, "gradient points need the gradient members of zg_statistics");
         words += gradLayout(S).words;
     }
+    // the single-scene walk kernels read the PDE from a device copy after the workspace
+    const size_t pdeWords = (sizeof(zg_pde) + 7) / 8;
     uint64_t* wsp = nullptr;
-    ZG_HIP(hipMallocAsync((void**)&wsp, words * 8, strm));
+    ZG_HIP(hipMallocAsync((void**)&wsp, (words + pdeWords) * 8, strm));
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipError_t e = hipSuccess;
-    if (report) {
+    if (!S.batch) {
+        S.P.pdeDev = reinterpret_cast<const zg_pde*>(wsp + words);
+        e = hipMemcpyAsync(wsp + words, &S.P.pde, sizeof(zg_pde), hipMemcpyHostToDevice, strm);
+    }
+    if (report && e == hipSuccess) {
         e = hipEventCreate(&ev0);
         if (e == hipSuccess) e = hipEventCreate(&ev1);
     }

@@ -178,7 +178,9 @@ wosKernel(WosParams P) {
     const InstanceDesc* inst = nullptr;
     WalkView view{P.scene.abs.nodes, P.scene.abs.prims, P.scene.abs.nNodes};
     float sigma = P.pde.absorption_coeff;
-    auto pde = [&]() -> const zg_pde& { return BATCH ? inst->pde : P.pde; };
+    // the PDE is read from device memory where it is used: a kernel-argument copy keeps its
+    // ~100 descriptor words live in scalar registers across the loop and spills them
+    auto pde = [&]() -> const zg_pde& { return BATCH ? inst->pde : *P.pdeDev; };
 
     // lane state: 0 idle, 1 step pending (dist holds the radius at x), 2 distance query in flight,
     // 3 the claim's next walk is to be started

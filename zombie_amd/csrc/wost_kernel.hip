@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, fl
     fsr[i] = 0.99f * fminf(P.dist0[i], dR);
 }
 
-// 4 waves per SIMD (106 VGPRs); forcing 5 or 6 waves (spills) measured no faster.
+// 2D: 5 waves per SIMD (93 VGPRs, no VGPR spill) with the PDE read from its device copy (the
+// kernel-argument copy held 106 VGPRs, 4 waves, and ran 13% slower on the 2D mixed problem).
 // GRAD = true: estimateSolutionAndGradient (walk_on_stars.h:664-848) for the points asking for
 // it -- the explicit first step of the walk-on-spheres gradient walks inside the first ball (no
 // boundary in it), then walk-on-stars steps from the boundary point.
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     Stack st; st.init(spill);
     VisitCount vc{0, 0};
     const zg_walk_settings& ws = P.ws;
-    const zg_pde& pde = P.pde;
+    const zg_pde& pde = *P.pdeDev;   // device copy, read where used
     const SceneView& sc = P.scene;
     const float epsA = ws.epsilon_shell_for_absorbing_boundary;
     const float prec = ws.silhouette_precision;
