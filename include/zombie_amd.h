@@ -381,6 +381,16 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
 zg_status zg_stratified_samples(int32_t dims, int64_t n_tables, int32_t n_samples, uint64_t seed,
                                 uint32_t stream_offset, float* out, void* stream);
 
+/* getSpatiallySortedPointIndices (utils/nearest_neighbor_finder.h:112-128): the index order
+ * of a kd-tree over points_host[n*dim] with leaves of at most 10 points, built as nanoflann's
+ * KDTreeSingleIndexAdaptor builds it (widest box dimension, midpoint clamped to the points'
+ * range, ties kept together; nanoflann itself is not part of this library).  Leaves are
+ * contiguous and in depth-first order, so consecutive points are spatial neighbours: sorting
+ * a solve's points this way keeps a wave's walks close together.  Host arrays; runs on the
+ * host (no device needed).  indices_host[n] receives a permutation of 0..n-1. */
+zg_status zg_spatially_sorted_indices(int32_t dim, const float* points_host, int64_t n,
+                                      int32_t* indices_host);
+
 /* ------------------------------------------------------------------------------------ */
 /* Boundary value caching (variance_reduction/boundary_value_caching.h): splatting cached  */
 /* boundary and domain samples into evaluation points with the free-space kernels.         */

@@ -241,6 +241,7 @@ SIGNATURES = {
                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(zg_bvc_statistics),
                                  _P]),
     "zg_stratified_samples": (C.c_int32, [C.c_int32, C.c_int64, C.c_int32, C.c_uint64, C.c_uint32, _P, _P]),
+    "zg_spatially_sorted_indices": (C.c_int32, [C.c_int32, _P, C.c_int64, _P]),
     "zg_evaluate_function": (C.c_int32, [C.c_int32, C.POINTER(zg_function), C.c_int64, _P, _P, _P, C.c_float, _P,
                                          _P]),
     "zg_probe_read_bandwidth": (C.c_int32, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_float), _P]),

@@ -30,7 +30,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["bvh_build.cpp", "abi.cpp", "wos_kernel.hip", "wost_kernel.hip", "mlp_kernels.hip", "probe_kernel.hip",
-           "bvc_kernel.hip", "sdf_kernel.hip"]
+           "bvc_kernel.hip", "sdf_kernel.hip", "spatial_sort.cpp"]
 
 
 def _hipcc() -> str:

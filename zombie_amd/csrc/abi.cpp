@@ -839,6 +839,16 @@ zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* S, c
     return ok();
 }
 
+zg_status zg_spatially_sorted_indices(int32_t dim, const float* pts, int64_t n, int32_t* out) {
+    if (dim != 2 && dim != 3) return fail(ZG_ERR_INVALID_ARGUMENT, "dim must be 2 or 3, got %d", dim);
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && (!pts || !out)))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "invalid point array (n = %lld)", (long long)n);
+    for (int64_t k = 0; k < n * dim; k++)
+        if (!std::isfinite(pts[k])) return fail(ZG_ERR_INVALID_ARGUMENT, "coordinate %lld is not finite", (long long)k);
+    spatiallySortedIndices(dim, pts, n, out);
+    return ok();
+}
+
 zg_status zg_stratified_samples(int32_t dims, int64_t nTables, int32_t nSamples, uint64_t seed, uint32_t streamOffset,
                                 float* out, void* stream) {
     if (dims < 1 || dims > 2) return fail(ZG_ERR_INVALID_ARGUMENT, "dims must be 1 or 2");

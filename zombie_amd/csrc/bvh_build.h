@@ -26,4 +26,7 @@ void buildSilhouettes2D(int width, const float* V, int64_t nV, const int32_t* I,
 void buildSilhouettes3D(int width, const float* V, int64_t nV, const int32_t* I, const std::vector<int>& sel,
                         bool solveDoubleSided, float precision, BuiltBoundary& out);
 
+// kd-tree leaf order of n points (spatial_sort.cpp)
+void spatiallySortedIndices(int dim, const float* pts, int64_t n, int32_t* out);
+
 }  // namespace zg

@@ -436,6 +436,17 @@ class SdfScene(Scene):
         self.info = info
 
 
+def spatially_sorted_point_indices(points) -> np.ndarray:
+    """getSpatiallySortedPointIndices (utils/nearest_neighbor_finder.h:112-128): the kd-tree
+    leaf order of host points [n, dim] (int32 permutation); runs on the host."""
+    pts = np.ascontiguousarray(points, np.float32)
+    if pts.ndim != 2 or pts.shape[1] not in (2, 3):
+        raise ValueError(f"points must be [n, 2] or [n, 3], got {pts.shape}")
+    out = np.empty(len(pts), np.int32)
+    A.check(A.lib().zg_spatially_sorted_indices(pts.shape[1], pts.ctypes.data, len(pts), out.ctypes.data))
+    return out
+
+
 def stratified_samples(dims: int, n_tables: int, n_samples: int, seed: int, stream_offset: int = 0,
                        device="cuda", stream=None):
     """The per-point Latin hypercube tables of gradient solves: [n_tables, n_samples, dims]."""
