@@ -179,13 +179,14 @@ zg_status zg_sdf_scene_create(int32_t dim, const float* values_host, const int32
                               const float* grid_min_host, const float* grid_max_host,
                               const float* domain_min_host, const float* domain_max_host, zg_scene** out);
 
-/* Robin coefficient bounds of the reflecting primitives (FcpwRobinBoundaryHandler<2>::
- * buildAccelerationStructure / updateCoefficientValues, fcpw_geometric_queries.h:726-831):
- * min_coeff_host[p] <= max_coeff_host[p] for every primitive p of the scene (entries of
- * absorbing primitives are ignored); both NULL clears them (Neumann conditions).  With bounds
- * set, star radii (zg_star_radius, zg_wost_solve) also respect the Robin bound of every segment
- * with max coefficient > 0 (reflectance_boundary_bvh/geometry.h:86-140, robin_bounds.h:12-35).
- * The pointwise coefficient of the walk throughput is pde.robin_coeff.  2D scenes only. */
+/* Robin coefficient bounds of the reflecting primitives (FcpwRobinBoundaryHandler<2>, <3>::
+ * buildAccelerationStructure / updateCoefficientValues, fcpw_geometric_queries.h:726-1000):
+ * min_coeff_host[p] <= max_coeff_host[p] for every primitive p of the scene (arrays of
+ * n_prims entries; entries of absorbing primitives are ignored); both NULL clears them
+ * (Neumann conditions).  With bounds set, star radii (zg_star_radius, zg_wost_solve) also
+ * respect the Robin bound of every segment or triangle with max coefficient > 0
+ * (reflectance_boundary_bvh/geometry.h:86-236, robin_bounds.h:12-80).  The pointwise
+ * coefficient of the walk throughput is pde.robin_coeff.  Mesh scenes. */
 zg_status zg_scene_set_robin_coefficients(zg_scene* scene, const float* min_coeff_host,
                                           const float* max_coeff_host);
 zg_status zg_scene_get_info(const zg_scene* scene, zg_scene_info* info_host);
@@ -357,11 +358,12 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        zg_solve_report* report_host, void* stream);
 
 /* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
- * walk :345-490); mesh scenes, harmonic Green's function (absorption 0).  2D: Neumann or Robin
- * reflecting conditions (h = pde.robin; with are_robin_conditions_pure_neumann = 0 the
- * throughput takes the Robin reflectance with kappa = pde.robin_coeff, and
- * zg_scene_set_robin_coefficients gives the per-segment bounds of the star radius).  3D:
- * Neumann conditions (silhouette edges bound the star radius; Robin data is rejected).  points_host->normal, type and
+ * walk :345-490); mesh scenes (2D segments, 3D triangles), harmonic Green's function
+ * (absorption 0), Neumann or Robin reflecting conditions (h = pde.robin; with
+ * are_robin_conditions_pure_neumann = 0 the throughput takes the Robin reflectance with
+ * kappa = pde.robin_coeff, and zg_scene_set_robin_coefficients gives the per-primitive bounds
+ * of the star radius).  Silhouette vertices (2D) or edges (3D) bound the star radius.
+ * points_host->normal, type and
  * estimate_boundary_normal_aligned describe points on the reflecting boundary;
  * first_sphere_radius (nullable) records the star radius of each point's first step on the
  * first call and is reused while the point has estimates.  Gradient points run

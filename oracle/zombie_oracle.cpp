@@ -1151,16 +1151,50 @@ inline void robinSegmentBound(const Mesh<2>& m, int p, const V<2>& x, float& r2)
     float rad = (h / c) * std::exp(c * c / (h * kmax));
     r2 = std::min(r2, rad * rad);
 }
-// minimum over the reflecting segments within sqrt(r2) of x (BVH pruned by box distance)
-inline float robinRadius2(const Mesh<2>& m, const V<2>& x, float r2, bool brute = false) {
+// Robin part of the star radius of a triangle (ReflectanceTriangle::computeSquaredStarRadius with
+// RobinTriangleBound, reflectance_boundary_bvh/geometry.h:170-236, robin_bounds.h:52-80): h the
+// distance to the triangle's plane, cosUpper / cosLower the view-angle cosines of its closest
+// point and farthest vertex; when sqrt(h kappa_max) >= cosLower, c = sqrt(h kappa_max) / sqrt(3)
+// clamped to [cosLower, cosUpper] gives the radius h^2 kappa_max / (c (h kappa_max - c^2)).
+inline void robinTriangleBound(const Mesh<3>& m, int p, const V<3>& x, float& r2) {
+    float kmax = m.robinMax.empty() ? 0.0f : m.robinMax[p];
+    if (!(kmax > FLT_EPSILON)) return;
+    V<3> pa = m.vtx(p, 0), pb = m.vtx(p, 1), pc = m.vtx(p, 2), cp;
+    float cd = closestPointTriangle(pa, pb, pc, x, cp);
+    float cd2 = cd * cd;
+    if (cd2 > r2) return;
+    if (cd < FLT_EPSILON) { r2 = std::min(r2, cd); return; }
+    V<3> n0 = triangleNormal(pa, pb, pc);
+    float h = std::fabs(dot(n0, pa - x));
+    if (!(h > 0.0f)) { r2 = std::min(r2, cd2); return; }
+    float da = norm(x - pa), db = norm(x - pb), dc = norm(x - pc);
+    V<3> fp = pa; float fd = da;
+    if (db > fd) { fp = pb; fd = db; }
+    if (dc > fd) { fp = pc; fd = dc; }
+    float cosUpper = std::fabs(dot((x - cp) * (1.0f / cd), n0));
+    float cosLower = std::fabs(dot((x - fp) * (1.0f / fd), n0));
+    float maxCos = std::sqrt(h * kmax);
+    if (maxCos < cosLower) return;
+    float c = std::min(std::max(maxCos / std::sqrt(3.0f), cosLower), cosUpper);
+    float rad = h * h * kmax / (c * (h * kmax - c * c));
+    r2 = std::min(r2, rad * rad);
+}
+template <int D>
+inline void robinPrimBound(const Mesh<D>& m, int p, const V<D>& x, float& r2) {
+    if constexpr (D == 2) robinSegmentBound(m, p, x, r2);
+    else robinTriangleBound(m, p, x, r2);
+}
+// minimum over the reflecting primitives within sqrt(r2) of x (BVH pruned by box distance)
+template <int D>
+float robinRadius2(const Mesh<D>& m, const V<D>& x, float r2, bool brute = false) {
     if (m.nPrims() == 0 || m.robinMax.empty()) return r2;
-    if (brute) { for (int p = 0; p < m.nPrims(); p++) robinSegmentBound(m, p, x, r2); return r2; }
+    if (brute) { for (int p = 0; p < m.nPrims(); p++) robinPrimBound<D>(m, p, x, r2); return r2; }
     const auto& nodes = m.bvh.nodes;
     int stack[128]; int sp = 0; stack[sp++] = 0;
     while (sp > 0) {
         const auto& nd = nodes[stack[--sp]];
         if (nd.box.dist2(x) > r2) continue;
-        if (nd.left < 0) { for (int i = nd.start; i < nd.start + nd.count; i++) robinSegmentBound(m, m.bvh.order[i], x, r2); continue; }
+        if (nd.left < 0) { for (int i = nd.start; i < nd.start + nd.count; i++) robinPrimBound<D>(m, m.bvh.order[i], x, r2); continue; }
         stack[sp++] = nd.right; stack[sp++] = nd.left;
     }
     return r2;
@@ -1679,13 +1713,19 @@ float wostStarRadius(const Scene<D>& sc, const SolveCtx& cx, const V<D>& x, floa
             // Robin boundaries also bound the radius (populateStarRadiusQueryForRobinBoundary,
             // fcpw_geometric_queries.h:1272-1289)
             if (minR <= distA && sc.refl.hasRobin()) {
-                float r2 = robinRadius2(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX);
+                float r2 = robinRadius2<2>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX);
                 r = std::max(std::min(r, std::sqrt(r2)), minR);
             }
         }
     } else {
-        if (sc.hasRefl())
-            r = starRadius3Bvh(sc.refl, x, ws.epsilon_shell_for_reflecting_boundary, distA, ws.silhouette_precision, flip);
+        const float minR = ws.epsilon_shell_for_reflecting_boundary;
+        if (sc.hasRefl()) {
+            r = starRadius3Bvh(sc.refl, x, minR, distA, ws.silhouette_precision, flip);
+            if (minR <= distA && sc.refl.hasRobin()) {
+                float r2 = robinRadius2<3>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX);
+                r = std::max(std::min(r, std::sqrt(r2)), minR);
+            }
+        }
     }
     if (sc.hasRefl() && ws.epsilon_shell_for_reflecting_boundary <= distA)
         r = std::max(0.99f * r, ws.epsilon_shell_for_reflecting_boundary);
@@ -2129,8 +2169,6 @@ static int solveImpl(bool wost, Scene<D>& sc, const zg_pde* pde, const zg_walk_s
                      double* gM2 = nullptr, double* gFirst = nullptr, double* gDeriv = nullptr,
                      const float* distR = nullptr) {
     if (pde->absorption_coeff > 0.0f && D == 2) return ZG_ERR_UNSUPPORTED;
-    // the Robin star-radius bound is restated for segments only
-    if (wost && D == 3 && !pde->are_robin_conditions_pure_neumann) return ZG_ERR_UNSUPPORTED;
     bool anyGrad = false;
     if (quantity) for (int64_t i = 0; i < n; i++) anyGrad = anyGrad || quantity[i] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT;
     if (anyGrad && (!gN || !gMean || !gM2 || !gFirst || !gDeriv)) return ZG_ERR_INVALID_ARGUMENT;
@@ -2342,21 +2380,26 @@ void orc_free_space_kernels(int dim, const float* x, const float* y, const float
 
 // per-primitive Robin coefficient bounds over the caller's primitives (reflecting ones are read);
 // NULL clears them (pure Neumann)
-void orc_scene_set_robin(void* hv, const float* minC, const float* maxC) {
-    SceneHandle* h = (SceneHandle*)hv;
-    if (h->dim != 2) return;
-    Mesh<2>& m = h->s2->refl;
+extern "C++" {
+template <int D>
+static void setRobin(Mesh<D>& m, const float* minC, const float* maxC) {
     m.robinMin.clear(); m.robinMax.clear();
     if (!minC || !maxC) return;
     for (int p = 0; p < m.nPrims(); p++) { m.robinMin.push_back(minC[m.origId[p]]); m.robinMax.push_back(maxC[m.origId[p]]); }
+}
+}
+void orc_scene_set_robin(void* hv, const float* minC, const float* maxC) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (h->dim == 2) setRobin<2>(h->s2->refl, minC, maxC);
+    else setRobin<3>(h->s3->refl, minC, maxC);
 }
 // Robin radius bound sqrt(min(maxR^2, Robin bounds)) at n points (brute: every segment)
 void orc_robin_radius(void* hv, int64_t n, const float* pts, const float* maxR, int brute, float* out) {
     SceneHandle* h = (SceneHandle*)hv;
     for (int64_t i = 0; i < n; i++) {
-        if (h->dim != 2) { out[i] = maxR[i]; continue; }
         float r2 = maxR[i] < FLT_MAX ? maxR[i] * maxR[i] : FLT_MAX;
-        out[i] = std::sqrt(robinRadius2(h->s2->refl, load<2>(pts + 2 * i), r2, brute != 0));
+        out[i] = h->dim == 2 ? std::sqrt(robinRadius2<2>(h->s2->refl, load<2>(pts + 2 * i), r2, brute != 0))
+                             : std::sqrt(robinRadius2<3>(h->s3->refl, load<3>(pts + 3 * i), r2, brute != 0));
     }
 }
 // out: [absNodes, absDepth, reflNodes, reflDepth, nSilhouetteCandidates]

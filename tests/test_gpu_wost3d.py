@@ -1,8 +1,9 @@
-"""Walk on stars on the GPU in 3D (Neumann reflecting triangle meshes): the silhouette-edge
-star radius, line of sight and reflecting-boundary sampler against the oracle and closed
-forms, and the estimator (solution and gradient) against the oracle (4 standard errors) and
-the manufactured solution u = e^x cos y on an L-shaped prism with Neumann re-entrant walls,
-top cap and an interior double-sided plate."""
+"""Walk on stars on the GPU in 3D (Neumann and Robin reflecting triangle meshes): the
+silhouette-edge star radius (with the triangles' Robin bound), line of sight and
+reflecting-boundary sampler against the oracle and closed forms, and the estimator (solution
+and gradient) against the oracle (4 standard errors) and the manufactured solution
+u = e^x cos y on an L-shaped prism with reflecting re-entrant walls, top cap and an interior
+double-sided plate."""
 import math
 
 import numpy as np
@@ -193,6 +194,56 @@ def test_wost3d_gradient(gpu, oracle_lib):
     assert int((np.abs(z) > 4.0).sum()) <= 3 and abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05
 
 
+@pytest.mark.parametrize("width", [4, 8])
+def test_robin_star_radius_matches_oracle(gpu, oracle_lib, width):
+    """with per-triangle Robin bounds the star radius is min(silhouette radius, Robin bound)"""
+    torch = gpu
+    O = oracle_lib
+    p = PB.lprism3d_robin(kappa=3.0, plate=True)
+    osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    osc.set_robin(*p.meta["robin"])
+    sc = _scene(p, width)
+    sc.set_robin_coefficients(*p.meta["robin"])
+    pts = _interior(p, 600, 19)
+    d = osc.closest_point(pts, boundary=0)[0]
+    eps = p.settings.epsilon_shell_for_reflecting_boundary
+    mn = np.full(len(pts), eps, np.float32)
+    g = sc.star_radius(_t(torch, pts), _t(torch, mn), _t(torch, d)).cpu().numpy()
+    o = np.maximum(np.minimum(osc.star_radius(pts, mn, d), osc.robin_radius(pts, d)), eps)
+    assert np.isclose(g, o, rtol=3e-4, atol=1e-6).mean() > 0.99
+    assert np.mean(osc.robin_radius(pts, d) < osc.star_radius(pts, mn, d) - 1e-6) > 0.02   # the bound binds
+    with pytest.raises(ValueError):
+        sc.set_robin_coefficients(np.zeros(3, np.float32), np.ones(3, np.float32))
+
+
+@pytest.mark.parametrize("kappa", [0.5, 10.0])
+def test_robin_wost3d_parity_with_oracle_and_closed_form(gpu, oracle_lib, kappa):
+    torch = gpu
+    O = oracle_lib
+    p = PB.lprism3d_robin(kappa=kappa, plate=True)
+    osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    osc.set_robin(*p.meta["robin"])
+    sc = _scene(p, 8)
+    sc.set_robin_coefficients(*p.meta["robin"])
+    pts = _interior(p, 400, 23)
+    d = osc.closest_point(pts, boundary=0)[0]
+    n = len(pts)
+    st = E.DeviceStats.zeros(n)
+    dp = E.DevicePoints(pt=_t(torch, pts), dist_to_absorbing=_t(torch, d),
+                        n_walks=torch.full((n,), 256, dtype=torch.int32, device="cuda"), n_walks_max=256)
+    rep = E.wost_solve(sc, p.pde, p.settings, dp, st, E.SolveOptions(seed=41))
+    g = _host(st)
+    ost = O.Stats(n)
+    O.solve(osc, p.pde, p.settings, pts, d, 256, ost, seed=42, wost=True)
+    assert rep.completion_codes[3] <= 1e-4 * rep.total_walks
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, f"robin wost 3d gpu vs oracle (kappa {kappa})")
+    pooled = (g["mean"] - p.exact(pts)).sum() / np.sqrt((g["var"] / g["n"]).sum())
+    assert abs(pooled) < 4.0, pooled
+    ml_g, ml_o = g["len"].sum() / g["n"].sum(), ost.walk_length.sum() / ost.n.sum()
+    assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)
+
+
 def test_wost3d_rejections(gpu):
     torch = gpu
     p = PB.lprism3d(plate=False)
@@ -201,13 +252,10 @@ def test_wost3d_rejections(gpu):
     d = sc.closest_point(x, want_point=False)[0].contiguous()
     dp = E.DevicePoints(pt=x, dist_to_absorbing=d, n_walks=torch.full((4,), 8, dtype=torch.int32, device="cuda"),
                         n_walks_max=8)
-    robin = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True,
-                       robin_coeff=E.constant(1.0))
-    with pytest.raises(A.ZombieError):
-        E.wost_solve(sc, robin, p.settings, dp, E.DeviceStats.zeros(4))
     screened = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True,
                           absorption=1.0)
     with pytest.raises(A.ZombieError):
         E.wost_solve(sc, screened, p.settings, dp, E.DeviceStats.zeros(4))
-    with pytest.raises(A.ZombieError):
-        sc.set_robin_coefficients(np.zeros(len(p.indices), np.float32), np.ones(len(p.indices), np.float32))
+    with pytest.raises(ValueError):       # 2D points for a 3D scene
+        E.wost_solve(sc, p.pde, p.settings, E.DevicePoints(pt=x[:, :2].contiguous(), dist_to_absorbing=d,
+                                                           n_walks=dp.n_walks, n_walks_max=8), E.DeviceStats.zeros(4))

@@ -6,7 +6,9 @@ mathematics fixes: the candidate set of a known polyhedron, the star-shapedness 
 exists for (no ray from x crosses the reflecting surface twice inside it, which the plain
 absorbing distance violates on the same inputs), an open rim's closed-form distance, and the
 BVH forms against brute force; the sampler by the surface-area identity E[1{|y - x| < R} / pdf]
-= area inside the ball; the estimator by a manufactured harmonic solution."""
+= area inside the ball; the triangles' Robin bound by the non-negative reflectance it exists for
+(and its tightness); the estimator by a manufactured harmonic solution with Neumann and with
+Robin data."""
 import math
 
 import numpy as np
@@ -181,14 +183,68 @@ def test_walk_on_stars_3d_matches_exact(oracle_lib, plate):
     assert abs(np.sum(st0.mean - u) / np.sqrt(np.sum(se0 ** 2))) > 10.0
 
 
-def test_robin_walk_on_stars_3d_is_unsupported(oracle_lib):
-    """the Robin star-radius bound is stated for segments only"""
-    O = oracle_lib
-    p = PB.lprism3d(plate=False)
+def _robin_scene(O, kappa, plate=True):
+    p = PB.lprism3d_robin(kappa=kappa, plate=plate)
     sc = _scene(O, p)
-    x = _interior(p, 4, 6)
+    sc.set_robin(*p.meta["robin"])
+    return p, sc
+
+
+@pytest.mark.parametrize("kappa", [0.5, 3.0, 20.0])
+def test_robin_triangle_bound_keeps_reflectance_nonnegative(oracle_lib, kappa):
+    """RobinTriangleBound (robin_bounds.h:52-80): inside the bound R every point y of a Robin
+    triangle at distance r < R has |cos(n, y - x)| >= kappa r (1 - r / R), i.e. the 3D
+    reflectance 1 - kappa G / P (G ~ 1/r - 1/R, P ~ cos / r^2) of a step from x to y is >= 0 --
+    the property the bound exists for -- and the bound is tight: the minimising triangle
+    touches the constraint.  BVH pruning equals the scan of every triangle."""
+    O = oracle_lib
+    p, sc = _robin_scene(O, kappa)
+    V = p.positions.astype(np.float64)
+    T = p.indices[p.is_reflecting.astype(bool)]
+    x = _interior(p, 80, 5)
+    R = sc.robin_radius(x, 10.0).astype(np.float64)
+    np.testing.assert_array_equal(R, sc.robin_radius(x, 10.0, brute=True))
+    g = np.linspace(0, 1, 9)
+    bary = np.array([(a, b) for a in g for b in g if a + b <= 1 + 1e-12])
+    A_, B_, C_ = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    Y = A_[:, None] + bary[None, :, :1] * (B_ - A_)[:, None] + bary[None, :, 1:] * (C_ - A_)[:, None]
+    N = np.cross(B_ - A_, C_ - A_)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    slack = []
+    for xi, Ri in zip(x.astype(np.float64), R):
+        d = Y - xi
+        r = np.linalg.norm(d, axis=2)
+        m = (r < Ri) & (r > 1e-9)
+        if not m.any():
+            continue
+        cos = np.abs(np.einsum("tkd,td->tk", d, N)) / np.maximum(r, 1e-30)
+        need = kappa * r * (1 - r / Ri)
+        sl = (cos - need)[m]
+        assert sl.min() >= -1e-4, (xi, Ri, sl.min())
+        slack.append(sl.min())
+    assert len(slack) >= 40 and np.median(slack) < 0.02, np.median(slack)
+    sc.set_robin(None, None)
+    np.testing.assert_array_equal(sc.robin_radius(x, 10.0), np.full(len(x), 10.0, np.float32))
+
+
+@pytest.mark.parametrize("kappa", [0.5, 10.0])
+def test_robin_walk_on_stars_3d_matches_exact(oracle_lib, kappa):
+    """Robin data du/dn + kappa u = h on the reflecting triangles (reflectance throughput,
+    epsilon radius on the Robin boundary, triangle bound of the star radius): 4-SE agreement with
+    u = e^x cos y; the same data treated as Neumann is detected."""
+    O = oracle_lib
+    p, sc = _robin_scene(O, kappa, plate=False)
+    x = _interior(p, 200, 3)
     d = sc.closest_point(x, boundary=0)[0]
-    pde = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True,
-                     robin_coeff=E.constant(1.0))
-    with pytest.raises(RuntimeError):
-        O.solve(sc, pde, p.settings, x, d, 4, O.Stats(len(x)), wost=True)
+    st = O.Stats(len(x))
+    O.solve(sc, p.pde, p.settings, x, d, 256, st, wost=True)
+    u = p.exact(x)
+    se = np.sqrt(st.variance() / st.n)
+    pooled = np.sum(st.mean - u) / np.sqrt(np.sum(se ** 2))
+    z = (st.mean - u) / se
+    assert abs(pooled) < 4.0 and 0.8 < z.std() < 1.3, (pooled, z.std())
+    neumann = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True)
+    st0 = O.Stats(len(x))
+    O.solve(sc, neumann, p.settings, x, d, 256, st0, wost=True)
+    se0 = np.sqrt(st0.variance() / st0.n)
+    assert abs(np.sum(st0.mean - u) / np.sqrt(np.sum(se0 ** 2))) > 10.0

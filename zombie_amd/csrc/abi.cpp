@@ -278,7 +278,6 @@ zg_status zg_scene_create(int32_t dim, const float* V, int64_t nV, const int32_t
 
 zg_status zg_scene_set_robin_coefficients(zg_scene* s, const float* minCoeff, const float* maxCoeff) {
     if (!s) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL scene");
-    if (s->dim != 2) return fail(ZG_ERR_UNSUPPORTED, "Robin conditions run in 2D walk on stars");
     if ((minCoeff == nullptr) != (maxCoeff == nullptr))
         return fail(ZG_ERR_INVALID_ARGUMENT, "give both coefficient arrays or neither");
     const std::vector<int32_t>& ids = s->refl.host.primIds;
@@ -505,8 +504,6 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
         if (!s) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on single scenes");
         if (dim == 3 && pde->absorption_coeff > 0.0f)
             return fail(ZG_ERR_UNSUPPORTED, "3D walk on stars runs the harmonic kernel (absorption 0)");
-        if (dim == 3 && !pde->are_robin_conditions_pure_neumann)
-            return fail(ZG_ERR_UNSUPPORTED, "Robin conditions run in 2D walk on stars");
     }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))

@@ -1072,13 +1072,44 @@ ZG_DEV void robinSegmentBound(const Prim<2>& pr, float kmax, const Vec<2>& x, fl
     r2 = fminf(r2, rad * rad);
 }
 
-template <int W>
-ZG_DEV float robinRadius2(const BoundaryView& b, const Vec<2>& x, float r2, Stack& st) {
+// Robin bound of a triangle (ReflectanceTriangle::computeSquaredStarRadius with
+// RobinTriangleBound, reflectance_boundary_bvh/geometry.h:170-236, robin_bounds.h:52-80): h the
+// distance to the triangle's plane, cosUpper / cosLower the view-angle cosines of its closest
+// point and farthest vertex; when sqrt(h kappa_max) >= cosLower, c = sqrt(h kappa_max / 3)
+// clamped to [cosLower, cosUpper] gives the radius h^2 kappa_max / (c (h kappa_max - c^2)),
+// inside which the 3D reflectance of every point of the triangle seen from x stays >= 0.
+ZG_DEV void robinTriangleBound(const Prim<3>& pr, float kmax, const Vec<3>& x, float& r2) {
+    if (!(kmax > FLT_EPSILON)) return;
+    Vec<3> cp;
+    float cd2 = pr.closest2(x, cp);
+    if (cd2 > r2) return;
+    float cd = __builtin_sqrtf(cd2);
+    if (cd < FLT_EPSILON) { r2 = fminf(r2, cd); return; }
+    Vec<3> n0 = pr.normal();
+    float h = fabsf(dot(n0, pr.a - x));
+    if (!(h > 0.0f)) { r2 = fminf(r2, cd2); return; }
+    float da = __builtin_sqrtf(norm2(x - pr.a)), db = __builtin_sqrtf(norm2(x - pr.b)), dc = __builtin_sqrtf(norm2(x - pr.c));
+    Vec<3> fp = pr.a;
+    float fd = da;
+    if (db > fd) { fp = pr.b; fd = db; }
+    if (dc > fd) { fp = pr.c; fd = dc; }
+    float cosUpper = fabsf(dot(x - cp, n0)) / cd;
+    float cosLower = fabsf(dot(x - fp, n0)) / fd;
+    float maxCos = __builtin_sqrtf(h * kmax);
+    if (maxCos < cosLower) return;
+    float c = fminf(fmaxf(maxCos * 0.57735027f, cosLower), cosUpper);
+    float rad = h * h * kmax / (c * (h * kmax - c * c));
+    r2 = fminf(r2, rad * rad);
+}
+
+// minimum of the Robin bounds of the reflecting primitives within sqrt(r2) of x
+template <int D, int W>
+ZG_DEV float robinRadius2(const BoundaryView& b, const Vec<D>& x, float r2, Stack& st) {
     if (b.nNodes == 0 || !b.robinMax) return r2;
     st.sp = 0; st.ovf = 0;
     int node = 0;
     for (;;) {
-        WideNode<2, W> nd;
+        WideNode<D, W> nd;
         nd.load(b.nodes, node);
 #pragma unroll
         for (int k = 0; k < W; k++) {
@@ -1087,8 +1118,9 @@ ZG_DEV float robinRadius2(const BoundaryView& b, const Vec<2>& x, float r2, Stac
             if (isLeaf(c)) {
                 int first = leafFirstD(c), cnt = leafCountD(c);
                 for (int j = first; j < first + cnt; j++) {
-                    Prim<2> pr; pr.load(b.prims, j);
-                    robinSegmentBound(pr, b.robinMax[j], x, r2);
+                    Prim<D> pr; pr.load(b.prims, j);
+                    if constexpr (D == 2) robinSegmentBound(pr, b.robinMax[j], x, r2);
+                    else robinTriangleBound(pr, b.robinMax[j], x, r2);
                 }
             } else {
                 st.push(c, nd.dist2(k, x));

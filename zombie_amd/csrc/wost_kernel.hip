@@ -1,4 +1,4 @@
-// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D Neumann or Robin, 3D Neumann reflecting
+// wost_kernel.hip -- persistent Walk-on-Stars kernel (2D and 3D, Neumann or Robin reflecting
 // boundaries) for gfx950, its per-point first-sphere prologue, and the batched star-radius,
 // line-of-sight and reflecting-boundary sampling queries.
 //
@@ -16,7 +16,7 @@
 // (:189-250), computeSourceContribution (:252-275) and computeWalkStepThroughput (:277-307)
 // for the harmonic Green's function: reflectance 1 under Neumann conditions, 1 - kappa G / P
 // under Robin conditions (the star radius then also respects the Robin bound of every segment
-// and shrinks to the reflecting epsilon on the Robin boundary).
+// or triangle and shrinks to the reflecting epsilon on the Robin boundary).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -36,12 +36,12 @@ ZG_DEV float wostStarRadius(const SceneView& sc, const zg_walk_settings& ws, con
         const float minR = ws.epsilon_shell_for_reflecting_boundary;
         if constexpr (D == 2) {
             r = starRadius2<W>(sc.refl, x, minR, distA, ws.silhouette_precision, flip, st);
-            // Robin segments bound the radius too (fcpw_geometric_queries.h:1272-1289)
-            if (ROBIN && sc.refl.robinMax && minR <= distA)
-                r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), minR);
         } else {
             r = starRadius3<W>(sc.refl, x, minR, distA, ws.silhouette_precision, flip, st);
         }
+        // Robin primitives bound the radius too (fcpw_geometric_queries.h:1272-1289)
+        if (ROBIN && sc.refl.robinMax && minR <= distA)
+            r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), minR);
         if (minR <= distA) r = fmaxf(0.99f * r, minR);
     }
     return r;
@@ -138,10 +138,8 @@ __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, fl
 // it -- the explicit first step of the walk-on-spheres gradient walks inside the first ball (no
 // boundary in it), then walk-on-stars steps from the boundary point.
 // ROBIN = true compiles the Robin reflectance and star-radius bound in (non-Neumann PDEs).
-// ROBIN runs in 2D only (the Robin star-radius bound is stated for segments).
 template <int D, int W, bool COUNT, bool GRAD = false, bool ROBIN = false>
 __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
-    static_assert(!ROBIN || D == 2, "Robin conditions run in 2D");
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
     VisitCount vc{0, 0};
@@ -442,8 +440,10 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             float kappa = evalField<D>(pde.robin_coeff, x, nrm, fl, 0.0f);
             float refl = 1.0f;
             if (fabsf(kappa) > 0.0f) {
-                float Pk = dot(nr, prevDir) / prevDist;
-                float G = __logf(R / prevDist);
+                // harmonic ball kernels without their common constant: 2D P = cos / r,
+                // G = log(R / r); 3D P = cos / r^2, G = 1 / r - 1 / R
+                float Pk = D == 2 ? dot(nr, prevDir) / prevDist : dot(nr, prevDir) / (prevDist * prevDist);
+                float G = D == 2 ? __logf(R / prevDist) : 1.0f / prevDist - 1.0f / R;
                 refl = 1.0f - kappa * G / Pk;
             }
             T *= fminf(fmaxf(refl, 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
@@ -495,14 +495,13 @@ __global__ void __launch_bounds__(256) starRadiusKernel(SceneView sv, int64_t n,
     float r;
     if (sv.refl.nPrims == 0) {
         r = maxR[i];
-    } else if constexpr (D == 2) {
-        r = starRadius2<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
+    } else {
+        if constexpr (D == 2) r = starRadius2<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
+        else r = starRadius3<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
         if (sv.refl.robinMax && minR[i] <= maxR[i]) {
             float m = maxR[i];
-            r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<W>(sv.refl, x, m < FLT_MAX ? m * m : FLT_MAX, st))), minR[i]);
+            r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sv.refl, x, m < FLT_MAX ? m * m : FLT_MAX, st))), minR[i]);
         }
-    } else {
-        r = starRadius3<W>(sv.refl, x, minR[i], maxR[i], prec, fl, st);
     }
     radius[i] = r;
 }
@@ -573,7 +572,8 @@ hipError_t launchWostGrad(const WosParams& P, int dim, int width, int gridBlocks
     const bool robin = !P.pde.are_robin_conditions_pure_neumann;
 #define ZG_WOSTG(Dd, Wd, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, false, true, R>), g, bl, kStackLdsBytes, s, P)
     if (dim == 3) {
-        if (width == 4) ZG_WOSTG(3, 4, false); else ZG_WOSTG(3, 8, false);
+        if (width == 4) { if (robin) ZG_WOSTG(3, 4, true); else ZG_WOSTG(3, 4, false); }
+        else { if (robin) ZG_WOSTG(3, 8, true); else ZG_WOSTG(3, 8, false); }
     } else if (width == 4) {
         if (robin) ZG_WOSTG(2, 4, true); else ZG_WOSTG(2, 4, false);
     } else {
@@ -588,8 +588,13 @@ hipError_t launchWost(const WosParams& P, int dim, int width, bool count, int gr
     const bool robin = !P.pde.are_robin_conditions_pure_neumann;
 #define ZG_WOST(Dd, Wd, C, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, C, false, R>), g, bl, kStackLdsBytes, s, P)
     if (dim == 3) {
-        if (width == 4) { if (count) ZG_WOST(3, 4, true, false); else ZG_WOST(3, 4, false, false); }
-        else { if (count) ZG_WOST(3, 8, true, false); else ZG_WOST(3, 8, false, false); }
+        if (width == 4) {
+            if (robin) { if (count) ZG_WOST(3, 4, true, true); else ZG_WOST(3, 4, false, true); }
+            else { if (count) ZG_WOST(3, 4, true, false); else ZG_WOST(3, 4, false, false); }
+        } else {
+            if (robin) { if (count) ZG_WOST(3, 8, true, true); else ZG_WOST(3, 8, false, true); }
+            else { if (count) ZG_WOST(3, 8, true, false); else ZG_WOST(3, 8, false, false); }
+        }
     } else if (width == 4) {
         if (robin) { if (count) ZG_WOST(2, 4, true, true); else ZG_WOST(2, 4, false, true); }
         else { if (count) ZG_WOST(2, 4, true, false); else ZG_WOST(2, 4, false, false); }

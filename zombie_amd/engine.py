@@ -257,9 +257,12 @@ class Scene:
             return
         lo = np.ascontiguousarray(min_coeff, np.float32)
         hi = np.ascontiguousarray(max_coeff, np.float32)
+        n = self.info.n_absorbing_prims + self.info.n_reflecting_prims
+        if lo.shape != (n,) or hi.shape != (n,):
+            raise ValueError(f"coefficient bounds need one entry per primitive ({n}), got {lo.shape} and {hi.shape}")
         A.check(A.lib().zg_scene_set_robin_coefficients(self.handle, lo.ctypes.data, hi.ctypes.data))
 
-    # walk-on-stars queries on the reflecting boundary (2D)
+    # walk-on-stars queries on the reflecting boundary
     def star_radius(self, pts, min_r, max_r, precision: float = 1e-3, flip=None):
         torch = _torch()
         out = torch.empty(pts.shape[0], dtype=torch.float32, device=pts.device)

@@ -374,6 +374,21 @@ def lprism3d(n_sub=4, n_z=4, plate=True, n_walks=256, eps=1e-3, max_len=16384) -
     return prob
 
 
+def lprism3d_robin(kappa=2.0, plate=False, n_walks=256, eps=1e-3, max_len=16384) -> Problem:
+    """lprism3d with Robin conditions du/dn + kappa u = h on every reflecting triangle,
+    u = e^x cos y.  meta['robin'] holds the per-primitive coefficient bounds (kappa on
+    reflecting primitives)."""
+    p = lprism3d(plate=plate, n_walks=n_walks, eps=eps, max_len=max_len)
+    h = E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_ROBIN, (1.0, 1.0, 0.0, 0.0, 0.0, kappa))
+    p.pde = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=h, is_source_constant=True,
+                       robin_coeff=E.constant(kappa))
+    k = np.where(p.is_reflecting.astype(bool), np.float32(kappa), np.float32(0.0)).astype(np.float32)
+    p.meta["robin"] = (k, k)
+    p.meta["kappa"] = kappa
+    p.name = "lprism3d_robin"
+    return p
+
+
 def sphere_sdf_grid(dim=3, res=64, radius=1.0, margin=0.2):
     """Signed distance |x| - radius of a ball sampled at the cell centres of a res^dim grid over
     [-(radius + margin), radius + margin]^dim (DenseGrid layout, last axis fastest)."""
