@@ -358,8 +358,9 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        zg_solve_report* report_host, void* stream);
 
 /* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
- * walk :345-490); mesh scenes (2D segments, 3D triangles), harmonic Green's function
- * (absorption 0), Neumann or Robin reflecting conditions (h = pde.robin; with
+ * walk :345-490); mesh scenes (2D segments, 3D triangles), harmonic Green's function or, in
+ * 3D with absorption > 0, the Yukawa ball (dampening on steps that miss the reflecting
+ * boundary, Yukawa reflectance on those that hit it); Neumann or Robin reflecting conditions (h = pde.robin; with
  * are_robin_conditions_pure_neumann = 0 the throughput takes the Robin reflectance with
  * kappa = pde.robin_coeff, and zg_scene_set_robin_coefficients gives the per-primitive bounds
  * of the star radius).  Silhouette vertices (2D) or edges (3D) bound the star radius.

@@ -1,4 +1,5 @@
-"""Walk on stars on the GPU in 3D (Neumann and Robin reflecting triangle meshes): the
+"""Walk on stars on the GPU in 3D (Neumann and Robin reflecting triangle meshes, harmonic and
+screened): the
 silhouette-edge star radius (with the triangles' Robin bound), line of sight and
 reflecting-boundary sampler against the oracle and closed forms, and the estimator (solution
 and gradient) against the oracle (4 standard errors) and the manufactured solution
@@ -244,6 +245,38 @@ def test_robin_wost3d_parity_with_oracle_and_closed_form(gpu, oracle_lib, kappa)
     assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)
 
 
+@pytest.mark.parametrize("case", [(4.0, None, True), (9.0, 2.0, False)], ids=["neumann_plate", "robin"])
+def test_screened_wost3d_parity_with_oracle_and_closed_form(gpu, oracle_lib, case):
+    """the Yukawa ball in 3D walk on stars: u = exp(a . x), -Lap u + sigma u = 0"""
+    torch = gpu
+    O = oracle_lib
+    sigma, kappa, plate = case
+    p = PB.lprism3d_screened(sigma=sigma, kappa=kappa, plate=plate)
+    osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max,
+                  solve_double_sided=p.solve_double_sided)
+    sc = _scene(p, 8)
+    if kappa is not None:
+        osc.set_robin(*p.meta["robin"])
+        sc.set_robin_coefficients(*p.meta["robin"])
+    pts = _interior(p, 400, 29)
+    d = osc.closest_point(pts, boundary=0)[0]
+    n = len(pts)
+    st = E.DeviceStats.zeros(n)
+    dp = E.DevicePoints(pt=_t(torch, pts), dist_to_absorbing=_t(torch, d),
+                        n_walks=torch.full((n,), 512, dtype=torch.int32, device="cuda"), n_walks_max=512)
+    rep = E.wost_solve(sc, p.pde, p.settings, dp, st, E.SolveOptions(seed=51))
+    g = _host(st)
+    ost = O.Stats(n)
+    O.solve(osc, p.pde, p.settings, pts, d, 512, ost, seed=52, wost=True)
+    assert rep.completion_codes[3] <= 1e-4 * rep.total_walks
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, f"screened wost 3d gpu vs oracle {case}")
+    pooled = (g["mean"] - p.exact(pts)).sum() / np.sqrt((g["var"] / g["n"]).sum())
+    assert abs(pooled) < 4.0, pooled
+    ml_g, ml_o = g["len"].sum() / g["n"].sum(), ost.walk_length.sum() / ost.n.sum()
+    assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)
+
+
 def test_wost3d_rejections(gpu):
     torch = gpu
     p = PB.lprism3d(plate=False)
@@ -252,10 +285,9 @@ def test_wost3d_rejections(gpu):
     d = sc.closest_point(x, want_point=False)[0].contiguous()
     dp = E.DevicePoints(pt=x, dist_to_absorbing=d, n_walks=torch.full((4,), 8, dtype=torch.int32, device="cuda"),
                         n_walks_max=8)
-    screened = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True,
-                          absorption=1.0)
-    with pytest.raises(A.ZombieError):
-        E.wost_solve(sc, screened, p.settings, dp, E.DeviceStats.zeros(4))
+    with pytest.raises(A.ZombieError):      # splitting runs in walk on spheres only
+        E.wost_solve(sc, p.pde, E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5), dp,
+                     E.DeviceStats.zeros(4))
     with pytest.raises(ValueError):       # 2D points for a 3D scene
         E.wost_solve(sc, p.pde, p.settings, E.DevicePoints(pt=x[:, :2].contiguous(), dist_to_absorbing=d,
                                                            n_walks=dp.n_walks, n_walks_max=8), E.DeviceStats.zeros(4))

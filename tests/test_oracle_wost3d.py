@@ -248,3 +248,30 @@ def test_robin_walk_on_stars_3d_matches_exact(oracle_lib, kappa):
     O.solve(sc, neumann, p.settings, x, d, 256, st0, wost=True)
     se0 = np.sqrt(st0.variance() / st0.n)
     assert abs(np.sum(st0.mean - u) / np.sqrt(np.sum(se0 ** 2))) > 10.0
+
+
+@pytest.mark.parametrize("case", [(4.0, None, True), (9.0, 2.0, False)], ids=["neumann_plate", "robin"])
+def test_screened_walk_on_stars_3d_matches_exact(oracle_lib, case):
+    """-Lap u + sigma u = 0, u = exp(a . x): the Yukawa ball's dampening on every step that does
+    not hit the reflecting boundary and the Yukawa reflectance Q - kappa G / P on those that do
+    (distributions.h:965-985, walk_on_stars.h:277-307); the same data walked with the harmonic
+    ball (absorption 0) is detected."""
+    O = oracle_lib
+    sigma, kappa, plate = case
+    p = PB.lprism3d_screened(sigma=sigma, kappa=kappa, plate=plate)
+    sc = _scene(O, p)
+    if kappa is not None:
+        sc.set_robin(*p.meta["robin"])
+    x = _interior(p, 200, 3)
+    d = sc.closest_point(x, boundary=0)[0]
+    st = O.Stats(len(x))
+    O.solve(sc, p.pde, p.settings, x, d, 512, st, wost=True)
+    u = p.exact(x)
+    se = np.sqrt(st.variance() / st.n)
+    assert abs(np.sum(st.mean - u) / np.sqrt(np.sum(se ** 2))) < 4.0
+    harmonic = E.make_pde(source=E.zero(), dirichlet=p.pde.dirichlet, robin=p.pde.robin, is_source_constant=True,
+                          robin_coeff=None if kappa is None else E.constant(kappa))
+    st0 = O.Stats(len(x))
+    O.solve(sc, harmonic, p.settings, x, d, 256, st0, wost=True)
+    se0 = np.sqrt(st0.variance() / st0.n)
+    assert abs(np.sum(st0.mean - u) / np.sqrt(np.sum(se0 ** 2))) > 10.0

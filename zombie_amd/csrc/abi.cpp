@@ -502,8 +502,6 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
             return fail(ZG_ERR_UNSUPPORTED, "splitting_threshold %g < 1: walk splitting runs in walk on spheres only",
                         ws->splitting_threshold);
         if (!s) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on single scenes");
-        if (dim == 3 && pde->absorption_coeff > 0.0f)
-            return fail(ZG_ERR_UNSUPPORTED, "3D walk on stars runs the harmonic kernel (absorption 0)");
     }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))

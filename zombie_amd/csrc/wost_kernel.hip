@@ -53,8 +53,8 @@ ZG_DEV bool hasNonZeroRobinCoeff(const zg_pde& pde, const Vec<D>& x) {
     if (pde.are_robin_conditions_pure_neumann) return false;
     Vec<D> n = vzero<D>();
     n[0] = 1.0f;
-    return fabsf(evalField<D>(pde.robin_coeff, x, n, true, 0.0f)) > 0.0f ||
-           fabsf(evalField<D>(pde.robin_coeff, x, n, false, 0.0f)) > 0.0f;
+    return fabsf(evalField<D>(pde.robin_coeff, x, n, true, pde.absorption_coeff)) > 0.0f ||
+           fabsf(evalField<D>(pde.robin_coeff, x, n, false, pde.absorption_coeff)) > 0.0f;
 }
 
 // first reflecting-boundary sample of a step: a point, its normal and density (DIM uniform
@@ -150,6 +150,8 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     const float prec = ws.silhouette_precision;
     const bool hasRefl = sc.refl.nPrims > 0;
     const bool robin = ROBIN && !pde.are_robin_conditions_pure_neumann;
+    const float sigma = pde.absorption_coeff;
+    const bool screened = D == 3 && sigma > 0.0f;   // Yukawa ball of the 3D screened equation
     const int lane = __lane_id();
     const uint32_t streamOffset = P.streamOffsetDev ? *P.streamOffsetDev : P.streamOffset;
 
@@ -165,7 +167,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     int len = 0;
     int absPrim = -1;   // closest absorbing primitive of the last distance query (seeds the next)
     bool onRefl = false, flip = false, firstStep = true;
-    Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = 0.0f; ball.R = 0.0f;
+    Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = __builtin_sqrtf(fmaxf(sigma, 0.0f)); ball.R = 0.0f;
     Pcg32 rng; rng.state = 0; rng.inc = 1;
     // gradient walks: boundary / source gradient directions, first-step source term, control variates
     Vec<D> gB = vzero<D>(), gS = vzero<D>();
@@ -178,20 +180,21 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         const Vec<D> c0 = vload<D>(P.pt + D * (size_t)p);
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
+        ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
         ball.update(P.fsr[p]);
         const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + (size_t)(D - 1) * 2 * (size_t)pair;
         fsc = 0.0f; gS = vzero<D>();
         if (!ws.ignore_source_contribution) {
             float gNorm = ball.norm();
             if (pde.is_source_constant) {
-                fsc = gNorm * evalField<D>(pde.source, c0, vzero<D>(), false, 0.0f);
+                fsc = gNorm * evalField<D>(pde.source, c0, vzero<D>(), false, sigma);
             } else {
                 float pdf;
                 Vec<D> sd = sphereFromU<D>(tab);
                 float r = ball.sampleRadiusPdf(rng, pdf);
                 Vec<D> ys = c0 + sd * r;
                 if (mirror) ys = c0 - (ys - c0);
-                fsc = gNorm * evalField<D>(pde.source, ys, vzero<D>(), false, 0.0f);
+                fsc = gNorm * evalField<D>(pde.source, ys, vzero<D>(), false, sigma);
                 gS = (ys - c0) * (ball.gradientNorm(r) / (pdf * gNorm));
             }
             Ssrc += fsc;
@@ -255,6 +258,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         firstStep = true;
         absPrim = -1;
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
+        ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
         rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
         atomicAdd(&blk[5], 1ull);
         active = true;
@@ -278,7 +282,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                     xp = cp; nn = n;
                     al = ws.solve_double_sided && sd > 0.0f;
                 }
-                term = evalField<D>(pde.dirichlet, xp, nn, al, 0.0f);
+                term = evalField<D>(pde.dirichlet, xp, nn, al, sigma);
             }
             float total = T * term + Sref + Ssrc;
             double c = (double)total;
@@ -414,7 +418,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                     Vec<D> p2 = offsetPoint<D>(bpt, bnS * -1.0f);
                     if (lineOfSightD<D, W>(sc.refl, p1, p2, st)) {
                         float G = ball.evaluate(fmaxf(1e-4f, distS));
-                        float h = evalField<D>(pde.robin, bpt, bn, ws.solve_double_sided && al, 0.0f);
+                        float h = evalField<D>(pde.robin, bpt, bn, ws.solve_double_sided && al, sigma);
                         Sref += T * alpha * G * h / bpdf;
                     }
                 }
@@ -424,7 +428,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         // source contribution along the walk direction (computeSourceContribution :252-275)
         if (!ws.ignore_source_contribution) {
             float r = ball.sampleRadius(rng);
-            if (r <= idist) Ssrc += T * ball.norm() * evalField<D>(pde.source, x + dir * r, vzero<D>(), false, 0.0f);
+            if (r <= idist) Ssrc += T * ball.norm() * evalField<D>(pde.source, x + dir * r, vzero<D>(), false, sigma);
         }
 
         // move; throughput (computeWalkStepThroughput :277-307): 1 for Neumann conditions and
@@ -434,19 +438,28 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         x = ipt;
         nrm = inrm;
         onRefl = hit;
-        if constexpr (ROBIN) if (robin && onRefl && prevDist > FLT_EPSILON) {
-            bool fl = ws.solve_double_sided && dot(prevDir, nrm) < 0.0f;
-            Vec<D> nr = fl ? nrm * -1.0f : nrm;
-            float kappa = evalField<D>(pde.robin_coeff, x, nrm, fl, 0.0f);
-            float refl = 1.0f;
-            if (fabsf(kappa) > 0.0f) {
-                // harmonic ball kernels without their common constant: 2D P = cos / r,
-                // G = log(R / r); 3D P = cos / r^2, G = 1 / r - 1 / R
-                float Pk = D == 2 ? dot(nr, prevDir) / prevDist : dot(nr, prevDir) / (prevDist * prevDist);
-                float G = D == 2 ? __logf(R / prevDist) : 1.0f / prevDist - 1.0f / R;
-                refl = 1.0f - kappa * G / Pk;
+        const bool yukawa = D == 3 && ball.yukawa;
+        if (onRefl && prevDist > FLT_EPSILON) {
+            if (robin || yukawa) {
+                // reflectance Q - kappa G / P (distributions.h:535-543, 643-652, 971-985): Q the
+                // Yukawa dampening (1 for the harmonic ball), G and P the ball's kernels without
+                // their common constant (2D log(R / r), cos / r; 3D 1 / r - 1 / R or its
+                // Yukawa form, cos / r^2)
+                float refl = yukawa ? ball.dampening(prevDist) : 1.0f;
+                if (robin) {
+                    bool fl = ws.solve_double_sided && dot(prevDir, nrm) < 0.0f;
+                    Vec<D> nr = fl ? nrm * -1.0f : nrm;
+                    float kappa = evalField<D>(pde.robin_coeff, x, nrm, fl, sigma);
+                    if (fabsf(kappa) > 0.0f) {
+                        float Pk = D == 2 ? dot(nr, prevDir) / prevDist : dot(nr, prevDir) / (prevDist * prevDist);
+                        float G = (D == 2 ? 2.0f * kPi : 4.0f * kPi) * ball.evaluate(prevDist);
+                        refl -= kappa * G / Pk;
+                    }
+                }
+                T *= fminf(fmaxf(refl, 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
             }
-            T *= fminf(fmaxf(refl, 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
+        } else if (yukawa) {
+            T *= ball.dampening(fmaxf(1e-4f, prevDist));   // directionSampledPoissonKernel
         }
         if (!onRefl && outsideDomain<D>(sc.dmin, sc.dmax, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
         if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :309-343
@@ -456,6 +469,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         }
         len++;
         if (len >= ws.max_walk_length) { finishWalk(ZG_EXCEEDED_MAX_WALK_LENGTH); continue; }
+        if (screened && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
         // distance to the absorbing boundary for the next step
         float b2 = FLT_MAX;
         Vec<D> cp = x;

@@ -389,6 +389,28 @@ def lprism3d_robin(kappa=2.0, plate=False, n_walks=256, eps=1e-3, max_len=16384)
     return p
 
 
+def lprism3d_screened(sigma=4.0, kappa=None, plate=False, n_walks=256, eps=1e-3, max_len=16384) -> Problem:
+    """lprism3d for the screened equation -Lap u + sigma u = 0 with u = exp(a . x),
+    a = sqrt(sigma) (0.48, 0.6, 0.64): Neumann data du/dn on the reflecting triangles, or with
+    kappa Robin data du/dn + kappa u (meta['robin'] then holds the coefficient bounds)."""
+    p = lprism3d(plate=plate, n_walks=n_walks, eps=eps, max_len=max_len)
+    a = tuple(float(v) for v in math.sqrt(sigma) * np.array([0.48, 0.6, 0.64]))
+    u = E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a))
+    if kappa is None:
+        h = E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_NORMAL_DERIVATIVE, (1.0, *a))
+        p.pde = E.make_pde(source=E.zero(), dirichlet=u, robin=h, absorption=sigma, is_source_constant=True)
+    else:
+        h = E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_ROBIN, (1.0, *a, 0.0, kappa))
+        p.pde = E.make_pde(source=E.zero(), dirichlet=u, robin=h, absorption=sigma, is_source_constant=True,
+                           robin_coeff=E.constant(kappa))
+        k = np.where(p.is_reflecting.astype(bool), np.float32(kappa), np.float32(0.0)).astype(np.float32)
+        p.meta["robin"] = (k, k)
+    p.exact = exact_fn(exp_linear, a=np.array(a))
+    p.meta["a"] = a
+    p.name = "lprism3d_screened"
+    return p
+
+
 def sphere_sdf_grid(dim=3, res=64, radius=1.0, margin=0.2):
     """Signed distance |x| - radius of a ball sampled at the cell centres of a res^dim grid over
     [-(radius + margin), radius + margin]^dim (DenseGrid layout, last axis fastest)."""
