@@ -211,7 +211,9 @@ zg_status zg_inside_domain(const zg_scene* scene, int64_t n, const float* pts,
 
 /* intersectReflectingBoundary / intersectAbsorbingBoundary first hit
  * (fcpw_geometric_queries.h:1097-1115,1178-1196): ray origin + t dir, 0 <= t <= t_max.
- * hit[i] = 1 when a primitive is hit; t, point, normal of the first hit. */
+ * hit[i] = 1 when a primitive is hit; t, point, normal of the first hit (t = t_max and a zero
+ * point on a miss).  On SDF-grid scenes (boundary 0) the ray is sphere-traced against the zero
+ * level set (SdfGrid::intersectZeroLevelSet, sdf_grid_geometric_queries.h:147-205). */
 zg_status zg_intersect_ray(const zg_scene* scene, int32_t boundary, int64_t n,
                            const float* origins, const float* dirs, const float* t_max,
                            uint8_t* hit, float* t, float* point, float* normal, void* stream);

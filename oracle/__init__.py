@@ -67,6 +67,7 @@ def lib():
                                     C.c_int64, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, P, P, P, P, P]
         L.orc_sdf_solve.restype = C.c_int
         L.orc_sdf_query.argtypes = [C.c_int, P, P, P, P, C.c_int64, P, P, P, P]
+        L.orc_sdf_ray.argtypes = [C.c_int, P, P, P, P, C.c_int64, P, P, P, P, P, P, P]
         L.orc_bvc_splat.argtypes = [C.c_int, C.POINTER(A.zg_bvc_samples), C.POINTER(A.zg_bvc_eval_points), C.c_float,
                                     C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P, P, C.c_int]
         L.orc_free_space_kernels.argtypes = [C.c_int, P, P, P, C.c_float, C.c_float, P]
@@ -148,6 +149,20 @@ class SdfGrid:
                             _p(pr), _p(nr))
         return d, pr, nr
 
+
+    def ray(self, o, d, tmax):
+        """intersectZeroLevelSet: hit, t, point, normal per ray"""
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = len(o)
+        tm = np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float32), n), np.float32)
+        hit = np.zeros(n, np.uint8)
+        t = np.zeros(n, np.float32)
+        pt = np.zeros((n, self.dim), np.float32)
+        nr = np.zeros((n, self.dim), np.float32)
+        lib().orc_sdf_ray(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), n, _p(o), _p(d), _p(tm),
+                          _p(hit), _p(t), _p(pt), _p(nr))
+        return hit.astype(bool), t, pt, nr
     def solve(self, pde, ws, pts, dist, n_walks, stats: "Stats", seed=0x5EED, stream_offset=0, rng_mode=RNG_PER_WALK,
               threads=None, types=None):
         pts = np.ascontiguousarray(pts, np.float32)

@@ -21,6 +21,7 @@
 //     (2D vertices; 3D edges with the dihedral angle of reflectance_boundary_bvh setup,
 //     fcpw_geometric_queries.h:955-967, and FCPW's edge test restated from the WoSt paper)
 //   branchTraversalWeight                                 fcpw_geometric_queries.h:1391-1398
+//   SdfGrid value / gradient / projection / sphere tracing  utils/sdf_grid_geometric_queries.h:87-205
 //   SampleStatistics (FP64 here; float in the reference) point_estimation/common.h:200-395
 //   WalkOnSpheres::estimateSolution / walk / weight window / terminal
 //                                                        point_estimation/walk_on_spheres.h:170-414
@@ -1965,6 +1966,37 @@ struct SdfGeom {
         for (int a = 0; a < D; a++) if (x[a] < dmin[a] || x[a] > dmax[a]) return true;
         return false;
     }
+    // intersectZeroLevelSet (sdf_grid_geometric_queries.h:147-205): clip the ray to the grid
+    // box, sphere-trace with steps |sdf| (at most 128, hit when |sdf| < 1e-5); on a sign change
+    // between two samples, one secant step and one gradient step give the hit point
+    bool intersect(const V<D>& o, const V<D>& dir, float tMax, V<D>& pt, V<D>& n, float& tHit) const {
+        float tEnter = 0.0f, tExit = tMax;
+        for (int a = 0; a < D; a++) {
+            float inv = 1.0f / dir[a];
+            float t0 = (f.grid_min[a] - o[a]) * inv, t1 = (f.grid_max[a] - o[a]) * inv;
+            tEnter = std::fmax(tEnter, std::fmin(t0, t1));
+            tExit = std::fmin(tExit, std::fmax(t0, t1));
+        }
+        if (tEnter > tExit) return false;
+        float tPrev = tEnter, sPrev = 0.0f, t = tEnter;
+        for (int i = 0; i < 128 && t <= tExit; i++) {
+            V<D> p = o + dir * t;
+            float s = value(p);
+            if (std::fabs(s) < 1e-5f) { pt = p; n = normal(p); tHit = t; return true; }
+            if (sPrev * s < 0.0f) {
+                float th = tPrev + (sPrev / (sPrev - s)) * (t - tPrev);
+                V<D> ph = o + dir * th;
+                float sh = value(ph);
+                V<D> g = gradient(ph);
+                ph = ph - g * (sh / (norm(g) + 1e-6f));
+                pt = ph; n = normal(ph); tHit = dot(ph - o, dir);
+                return true;
+            }
+            tPrev = t; sPrev = s;
+            t += std::fabs(s);
+        }
+        return false;
+    }
 };
 
 // estimateSolution + walk (walk_on_spheres.h:228-414) over the SDF geometry
@@ -2328,6 +2360,24 @@ void orc_sdf_query(int dim, const float* grid, const int32_t* shape, const float
         } else {
             SdfGeom<3> g; g.f = f; V<3> x = load<3>(pts + 3 * i), nn; dist[i] = std::fabs(g.value(x));
             g.project(x, nn); store<3>(proj + 3 * i, x); store<3>(nrm + 3 * i, nn);
+        }
+    }
+}
+
+// sphere tracing of n rays against the zero level set: hit, t, point, normal per ray
+void orc_sdf_ray(int dim, const float* grid, const int32_t* shape, const float* gmin, const float* gmax, int64_t n,
+                 const float* o, const float* d, const float* tmax, uint8_t* hit, float* t, float* pt, float* nrm) {
+    zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
+    for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
+    for (int64_t i = 0; i < n; i++) {
+        if (dim == 2) {
+            SdfGeom<2> g; g.f = f; V<2> p = V<2>::zero(), nn = V<2>::zero(); float th = 0.0f;
+            hit[i] = g.intersect(load<2>(o + 2 * i), load<2>(d + 2 * i), tmax[i], p, nn, th);
+            t[i] = th; store<2>(pt + 2 * i, p); store<2>(nrm + 2 * i, nn);
+        } else {
+            SdfGeom<3> g; g.f = f; V<3> p = V<3>::zero(), nn = V<3>::zero(); float th = 0.0f;
+            hit[i] = g.intersect(load<3>(o + 3 * i), load<3>(d + 3 * i), tmax[i], p, nn, th);
+            t[i] = th; store<3>(pt + 3 * i, p); store<3>(nrm + 3 * i, nn);
         }
     }
 }

@@ -1,5 +1,6 @@
 """Walk on spheres over SDF-grid Dirichlet geometry on the GPU
-(utils/sdf_grid_geometric_queries.h): the distance / projection queries against the oracle, and
+(utils/sdf_grid_geometric_queries.h): the distance / projection / sphere-tracing queries against
+the oracle, and
 the estimator against the oracle (4-SE) and against closed forms in 2D and 3D."""
 import numpy as np
 import pytest
@@ -97,6 +98,30 @@ def test_sdf_screened_with_roulette(gpu, oracle_lib):
     assert abs(pooled) < 4.0, pooled
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sdf_rays_match_oracle(gpu, oracle_lib, dim):
+    """zg_intersect_ray on an SDF scene: sphere tracing equal to the oracle's on the same rays"""
+    torch = gpu
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(dim, 96 if dim == 2 else 48)
+    sc = E.SdfScene(vals, lo, hi)
+    og = O.SdfGrid(vals, lo, hi)
+    rng = np.random.default_rng(20 + dim)
+    o = rng.uniform(-1.15, 1.15, (4000, dim)).astype(np.float32)
+    d = rng.normal(size=(4000, dim))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    tm = rng.uniform(0.1, 3.0, 4000).astype(np.float32)
+    hit, t, pt, nr = sc.intersect_ray(_t(torch, o), _t(torch, d), _t(torch, tm))
+    oh, ot, opt, onr = og.ray(o, d, tm)
+    hit = hit.cpu().numpy().astype(bool)
+    assert (hit != oh).sum() <= 4 and oh.sum() > 1000 and (~oh).sum() > 500
+    both = hit & oh
+    np.testing.assert_allclose(t.cpu().numpy()[both], ot[both], atol=1e-4)
+    np.testing.assert_allclose(pt.cpu().numpy()[both], opt[both], atol=1e-4)
+    np.testing.assert_allclose(nr.cpu().numpy()[both], onr[both], atol=1e-3)
+    assert np.all(t.cpu().numpy()[~hit] == tm[~hit])
+
+
 def test_sdf_scene_rejections(gpu):
     torch = gpu
     vals, lo, hi = PB.sphere_sdf_grid(2, 32)
@@ -112,5 +137,7 @@ def test_sdf_scene_rejections(gpu):
         E.wos_solve(sc, pde, E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5), dp, E.DeviceStats.zeros(4))
     with pytest.raises(A.ZombieError):      # no reflecting boundary
         sc.closest_point(x, boundary=1)
+    with pytest.raises(A.ZombieError):
+        sc.intersect_ray(x, x, torch.ones(4, device="cuda"), boundary=1)
     with pytest.raises(A.ZombieError):
         E.SdfScene(np.zeros((1, 4), np.float32), lo, hi)

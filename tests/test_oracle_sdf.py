@@ -1,6 +1,6 @@
-"""The oracle's SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h:87-190, 247-272)
-against the exact sphere: distances, projections and normals of an interpolated signed-distance
-grid, and walk on spheres over it against a harmonic closed form."""
+"""The oracle's SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h:87-205, 247-272)
+against the exact sphere: distances, projections, normals and sphere-traced ray hits of an
+interpolated signed-distance grid, and walk on spheres over it against a harmonic closed form."""
 import numpy as np
 import pytest
 
@@ -46,3 +46,31 @@ def test_sdf_walk_on_spheres_matches_exact(oracle_lib, dim):
     assert rep["codes"][0] == 256 * len(x)
     # the walk length of walk on spheres in a ball grows like log(1 / eps)
     assert 5 < rep["total_steps"] / st.n.sum() < 60
+
+
+@pytest.mark.parametrize("dim,res,tol", [(2, 96, 5e-4), (3, 48, 3e-3)])
+def test_sphere_tracing_matches_the_exact_sphere(oracle_lib, dim, res, tol):
+    """intersectZeroLevelSet (sdf_grid_geometric_queries.h:147-205) on the sampled unit sphere:
+    rays from inside hit at t = -o.d + sqrt((o.d)^2 - |o|^2 + 1) with the outward normal; rays
+    from outside aimed inward hit at distance 0.15; rays aimed away and rays stopped at half the
+    hit distance miss.  Errors stay within the grid's interpolation error."""
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(dim, res)
+    g = O.SdfGrid(vals, lo, hi)
+    rng = np.random.default_rng(1)
+    o = rng.normal(size=(1500, dim))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * rng.random((1500, 1)) * 0.9
+    d = rng.normal(size=(1500, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    hit, t, pt, nr = g.ray(o.astype(np.float32), d.astype(np.float32), 10.0)
+    od = (o * d).sum(1)
+    t_exact = -od + np.sqrt(od ** 2 - (o ** 2).sum(1) + 1)
+    assert hit.all()
+    assert np.abs(t - t_exact).max() < tol
+    assert np.abs(np.linalg.norm(pt, axis=1) - 1).max() < tol
+    assert np.abs((nr * pt).sum(1) - 1).max() < tol
+    outside = (d * 1.15).astype(np.float32)
+    hit2, t2, _, nr2 = g.ray(outside, (-d).astype(np.float32), 10.0)
+    assert hit2.all() and np.abs(t2 - 0.15).max() < tol and np.all((nr2 * d).sum(1) > 0.99)
+    assert not g.ray(outside, d.astype(np.float32), 10.0)[0].any()
+    assert not g.ray(o.astype(np.float32), d.astype(np.float32), (0.5 * t_exact).astype(np.float32))[0].any()

@@ -379,7 +379,11 @@ zg_status zg_intersect_ray(const zg_scene* s, int32_t boundary, int64_t n, const
                            const float* tmax, uint8_t* hit, float* t, float* point, float* normal, void* stream) {
     if (!s || n < 0 || (n > 0 && (!o || !d || !tmax || !hit))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
     if (boundary != 0 && boundary != 1) return fail(ZG_ERR_INVALID_ARGUMENT, "boundary must be 0 or 1");
-    if (s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "ray queries are not available on SDF-grid scenes");
+    if (s->sdfGrid) {
+        if (boundary != 0) return fail(ZG_ERR_UNSUPPORTED, "SDF-grid scenes have no reflecting boundary");
+        ZG_HIP(launchSdfRay(s->sdf, s->dim, n, o, d, tmax, hit, t, point, normal, (hipStream_t)stream));
+        return ok();
+    }
     ZG_HIP(launchRay(s->view, boundary, n, o, d, tmax, hit, t, point, normal, (hipStream_t)stream));
     return ok();
 }

@@ -412,6 +412,40 @@ ZG_DEV void sdfProject(const zg_function& f, Vec<D>& x, Vec<D>& n) {
     }
     n = sdfNormal<D>(f, x);
 }
+// intersectZeroLevelSet (sdf_grid_geometric_queries.h:147-205): the ray clipped to the grid box,
+// sphere-traced with steps |sdf| (at most 128; hit when |sdf| < 1e-5); a sign change between two
+// samples is resolved by one secant step and one gradient step.
+template <int D>
+ZG_DEV bool sdfIntersect(const zg_function& f, const Vec<D>& o, const Vec<D>& dir, float tMax, Vec<D>& pt, Vec<D>& n,
+                         float& tHit) {
+    float tEnter = 0.0f, tExit = tMax;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const float inv = 1.0f / dir[a];
+        const float t0 = (f.grid_min[a] - o[a]) * inv, t1 = (f.grid_max[a] - o[a]) * inv;
+        tEnter = fmaxf(tEnter, fminf(t0, t1));
+        tExit = fminf(tExit, fmaxf(t0, t1));
+    }
+    if (tEnter > tExit) return false;
+    float tPrev = tEnter, sPrev = 0.0f, t = tEnter;
+    for (int i = 0; i < 128 && t <= tExit; i++) {
+        const Vec<D> p = o + dir * t;
+        const float s = gridValue<D>(f, p);
+        if (fabsf(s) < 1e-5f) { pt = p; n = sdfNormal<D>(f, p); tHit = t; return true; }
+        if (sPrev * s < 0.0f) {
+            const float th = tPrev + (sPrev / (sPrev - s)) * (t - tPrev);
+            Vec<D> ph = o + dir * th;
+            const float sh = gridValue<D>(f, ph);
+            const Vec<D> g = sdfGradient<D>(f, ph);
+            ph = ph - g * (sh / (__builtin_sqrtf(norm2(g)) + 1e-6f));
+            pt = ph; n = sdfNormal<D>(f, ph); tHit = dot(ph - o, dir);
+            return true;
+        }
+        tPrev = t; sPrev = s;
+        t += fabsf(s);
+    }
+    return false;
+}
 
 // ------------------------------------------------------------------------------------
 // primitives (Ericson, "Real-Time Collision Detection" 5.1.2 / 5.1.5; the same closed forms

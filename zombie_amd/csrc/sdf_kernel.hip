@@ -121,6 +121,23 @@ __global__ void __launch_bounds__(256) sdfQueryKernel(zg_function sdf, int64_t n
     }
 }
 
+// first crossing of the zero level set along each ray (zg_intersect_ray on an SDF scene)
+template <int D>
+__global__ void __launch_bounds__(256) sdfRayKernel(zg_function sdf, int64_t n, const float* __restrict__ o,
+                                                    const float* __restrict__ d, const float* __restrict__ tmax,
+                                                    uint8_t* __restrict__ hit, float* __restrict__ t,
+                                                    float* __restrict__ point, float* __restrict__ normal) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Vec<D> p = vzero<D>(), nn = vzero<D>();
+    float th = 0.0f;
+    const bool h = sdfIntersect<D>(sdf, vload<D>(o + (size_t)i * D), vload<D>(d + (size_t)i * D), tmax[i], p, nn, th);
+    hit[i] = h ? 1 : 0;
+    if (t) t[i] = h ? th : tmax[i];
+    if (point) vstore<D>(point + (size_t)i * D, p);
+    if (normal) vstore<D>(normal + (size_t)i * D, nn);
+}
+
 template <int D>
 __global__ void __launch_bounds__(256) sdfInsideKernel(zg_function sdf, int64_t n, const float* __restrict__ pts,
                                                        uint8_t* __restrict__ inside) {
@@ -142,6 +159,15 @@ hipError_t launchSdfQuery(const zg_function& sdf, int dim, int64_t n, const floa
     dim3 g((unsigned)((n + 255) / 256)), bl(256);
     if (dim == 2) hipLaunchKernelGGL(sdfQueryKernel<2>, g, bl, 0, s, sdf, n, pts, dist, closest, normal, prim);
     else hipLaunchKernelGGL(sdfQueryKernel<3>, g, bl, 0, s, sdf, n, pts, dist, closest, normal, prim);
+    return hipGetLastError();
+}
+
+hipError_t launchSdfRay(const zg_function& sdf, int dim, int64_t n, const float* o, const float* d, const float* tmax,
+                        uint8_t* hit, float* t, float* point, float* normal, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 g((unsigned)((n + 255) / 256)), bl(256);
+    if (dim == 2) hipLaunchKernelGGL(sdfRayKernel<2>, g, bl, 0, s, sdf, n, o, d, tmax, hit, t, point, normal);
+    else hipLaunchKernelGGL(sdfRayKernel<3>, g, bl, 0, s, sdf, n, o, d, tmax, hit, t, point, normal);
     return hipGetLastError();
 }
 

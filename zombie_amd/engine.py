@@ -417,7 +417,8 @@ class DeviceStats:
 class SdfScene(Scene):
     """SDF-grid Dirichlet geometry (utils/sdf_grid_geometric_queries.h): signed distances at the
     cell centres of a grid over [lo, hi] (values[shape], last axis fastest), trilinear.  Used like
-    a Scene by closest_point (|sdf| + projection), inside_domain and wos_solve."""
+    a Scene by closest_point (|sdf| + projection), inside_domain, intersect_ray (sphere tracing of
+    the zero level set) and wos_solve."""
 
     def __init__(self, values, lo, hi, domain_min=None, domain_max=None):
         vals = np.ascontiguousarray(values, dtype=np.float32)
