@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=16000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0x5EED)
+    ap.add_argument("--point-order", default="kd", choices=["kd", "random"],
+                    help="sphere3d: solve the points in kd-tree order (getSpatiallySortedPointIndices) or as drawn")
     return ap.parse_args()
 
 
@@ -117,6 +119,9 @@ def main():
     log(f"scene: {scene.info.n_absorbing_prims} tris, {scene.info.n_absorbing_nodes} nodes, "
         f"depth {scene.info.absorbing_depth}, {scene.info.device_bytes/1e6:.2f} MB")
     P = interior_points(torch, scene, prob, args.points, args.seed, dev)
+    if args.point_order == "kd":
+        # the same points in kd-tree order: neighbouring lanes walk from nearby points
+        P = P[torch.from_numpy(E.spatially_sorted_point_indices(P.cpu().numpy())).to(dev).long()].contiguous()
     dist, _, _, _ = scene.closest_point(P, want_point=False)
     n = P.shape[0]
     dp = E.DevicePoints(pt=P, dist_to_absorbing=dist.contiguous(),
@@ -149,9 +154,11 @@ def main():
                               distance_tolerance=tol if tolerance is None else tolerance)
 
     # ---------------- instrumented sample ----------------
-    sub = 65536
-    dps = E.DevicePoints(pt=P[:sub].contiguous(), dist_to_absorbing=dp.dist_to_absorbing[:sub].contiguous(),
-                         n_walks=dp.n_walks[:sub].contiguous(), n_walks_max=args.walks)
+    # a strided sample over the whole point set (in kd order the first points share one region)
+    stride = max(1, n // 65536)
+    sub = len(range(0, n, stride))
+    dps = E.DevicePoints(pt=P[::stride].contiguous(), dist_to_absorbing=dp.dist_to_absorbing[::stride].contiguous(),
+                         n_walks=dp.n_walks[::stride].contiguous(), n_walks_max=args.walks)
     rc = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, True))
     rx = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, True, 0.0))
     len_exact = rx.total_steps / max(rx.total_walks, 1)
@@ -289,7 +296,8 @@ def main():
                    "bvh_width": args.bvh_width, "leaf_size": leaf, "walks_per_lane": args.walks_per_lane,
                    "regroup_interval": args.regroup, "step_interval": args.step_interval,
                    "distance_tolerance": tol, "points": n,
-                   "walks_per_point_per_step": args.walks, "autotuned": tune is not None},
+                   "walks_per_point_per_step": args.walks, "autotuned": tune is not None,
+                   "point_order": args.point_order},
         "value_definition": "completed walks/s x mean walk length with exact radii (reference walk-steps/s)",
         "walks_per_s": walks_per_s,
         "raw_walk_steps_per_s": raw_steps_per_s,
@@ -312,8 +320,9 @@ def main():
                              "from the committed SQ counter profile)"},
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(prob, P[: args.cpu_points].cpu().numpy(),
-                                              dist[: args.cpu_points].cpu().numpy(), args.walks)
+        cs = max(1, n // args.cpu_points)   # strided: spread over the domain whatever the point order
+        result["cpu_baseline"] = cpu_baseline(prob, P[::cs][: args.cpu_points].cpu().numpy(),
+                                              dist[::cs][: args.cpu_points].cpu().numpy(), args.walks)
         result["speedup_vs_cpu_baseline"] = steps_per_s / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)
 
