@@ -1,6 +1,6 @@
 """Walk on spheres over SDF-grid Dirichlet geometry on the GPU
 (utils/sdf_grid_geometric_queries.h): the distance / projection / sphere-tracing queries against
-the oracle, and
+the oracle, gradient estimation against manufactured gradients, and
 the estimator against the oracle (4-SE) and against closed forms in 2D and 3D."""
 import numpy as np
 import pytest
@@ -120,6 +120,58 @@ def test_sdf_rays_match_oracle(gpu, oracle_lib, dim):
     np.testing.assert_allclose(pt.cpu().numpy()[both], opt[both], atol=1e-4)
     np.testing.assert_allclose(nr.cpu().numpy()[both], onr[both], atol=1e-3)
     assert np.all(t.cpu().numpy()[~hit] == tm[~hit])
+
+
+def _grad_points(torch, sc, x, n_walks):
+    n = len(x)
+    d = sc.closest_point(_t(torch, x), want_point=False)[0].contiguous()
+    return E.DevicePoints(pt=_t(torch, x), dist_to_absorbing=d,
+                          n_walks=torch.full((n,), n_walks, dtype=torch.int32, device="cuda"), n_walks_max=n_walks,
+                          estimation_quantity=torch.full((n,), A.ZG_ESTIMATE_SOLUTION_AND_GRADIENT, dtype=torch.uint8,
+                                                         device="cuda"))
+
+
+@pytest.mark.parametrize("case", ["harmonic2d", "harmonic3d", "poisson3d", "screened3d"])
+def test_sdf_gradient_matches_analytic(gpu, case):
+    """estimateSolutionAndGradient over SDF-grid geometry (walk_on_spheres.h:416-593 with the
+    SdfGrid queries): gradients and solutions against the manufactured u on the unit disk / ball
+    (harmonic e^x cos y, Poisson sin 2x sin 2y sin 2z with its source, screened exp(a . x))"""
+    torch = gpu
+    dim = 2 if case == "harmonic2d" else 3
+    vals, lo, hi = PB.sphere_sdf_grid(dim, 96 if dim == 2 else 48)
+    sc = E.SdfScene(vals, lo, hi)
+    x = _ball(np.random.default_rng(31), 400, dim, 0.7)
+    xd = x.astype(np.float64)
+    if case.startswith("harmonic"):
+        pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_VALUE, (1.0, 1.0)),
+                         is_source_constant=True)
+        e = np.exp(xd[:, 0])
+        u = e * np.cos(xd[:, 1])
+        g = np.zeros_like(xd); g[:, 0] = e * np.cos(xd[:, 1]); g[:, 1] = -e * np.sin(xd[:, 1])
+    elif case == "poisson3d":
+        k = 2.0
+        pde = E.make_pde(source=E.function(A.ZG_FN_SIN_PRODUCT, A.ZG_EVAL_SCREENED_SOURCE, (1.0, k)),
+                         dirichlet=E.function(A.ZG_FN_SIN_PRODUCT, A.ZG_EVAL_VALUE, (1.0, k)))
+        s, c = np.sin(k * xd), np.cos(k * xd)
+        u = s.prod(1)
+        g = k * np.stack([c[:, 0] * s[:, 1] * s[:, 2], s[:, 0] * c[:, 1] * s[:, 2], s[:, 0] * s[:, 1] * c[:, 2]], 1)
+    else:
+        sigma = 4.0
+        a = np.sqrt(sigma) * np.array([0.48, 0.6, 0.64])
+        pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a)),
+                         absorption=sigma, is_source_constant=True)
+        u = np.exp(xd @ a)
+        g = u[:, None] * a[None, :]
+    ws = E.walk_settings_host(1e-3, 1e-3, 1024)
+    st = E.DeviceStats.zeros(len(x), dim=dim)
+    rep = E.wos_solve(sc, pde, ws, _grad_points(torch, sc, x, 256), st, E.SolveOptions(seed=32))
+    ng = st.n_grad.cpu().numpy()
+    assert (ng == 256).all() and rep.completion_codes[0] > 0
+    gm, gv = st.grad_mean.cpu().numpy(), st.grad_variance().cpu().numpy()
+    pooled = np.sum(gm - g, axis=0) / np.sqrt(np.sum(gv / ng[:, None], axis=0))
+    assert np.all(np.abs(pooled) < 4.0), (case, pooled)
+    mean, var, cnt = st.mean.cpu().numpy(), st.variance().cpu().numpy(), st.n.cpu().numpy()
+    assert abs(np.sum(mean - u) / np.sqrt(np.sum(var / cnt))) < 4.0
 
 
 def test_sdf_scene_rejections(gpu):

@@ -728,7 +728,8 @@ hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipS
         P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
         if (e == hipSuccess)
             e = S.wost ? launchWostGrad(P, S.dim, S.width, S.blocks, strm)
-                       : launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
+                : S.sdf ? launchWosSdfGrad(P, D, S.screened, S.blocks, strm)
+                        : launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
         if (e == hipSuccess) e = launchMergeStats(M, D, strm);
         u0 = u1;
     }
@@ -753,7 +754,6 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
     }
     size_t words = S.words;
     if (nGrad > 0) {
-        if (S.sdf) return fail(ZG_ERR_UNSUPPORTED, "gradient estimation is not available on SDF-grid scenes");
         const zg_statistics& st = S.stats;
         if (!st.n_gradient_estimates || !st.gradient_mean || !st.gradient_m2 || !st.total_first_source_contribution ||
             !st.total_derivative_contribution)

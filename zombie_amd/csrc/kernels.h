@@ -152,6 +152,7 @@ hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const fl
 hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s);
 hipError_t launchSdfQuery(const zg_function& sdf, int dim, int64_t n, const float* pts, float* dist, float* closest,
                           float* normal, int32_t* prim, hipStream_t s);
+hipError_t launchWosSdfGrad(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s);
 hipError_t launchSdfRay(const zg_function& sdf, int dim, int64_t n, const float* o, const float* d, const float* tmax,
                         uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
 hipError_t launchSdfInside(const zg_function& sdf, int dim, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);

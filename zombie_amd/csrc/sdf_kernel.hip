@@ -18,7 +18,13 @@
 
 namespace zg {
 
-template <int D, bool SCREENED>
+// GRAD = true: estimateSolutionAndGradient (walk_on_spheres.h:416-593) for the points asking for
+// it, as the mesh walk kernel runs it: the stage's walks [gradW0, gradW1) of antithetic pairs,
+// an explicit first step inside the ball of radius 0.99 |sdf| (Latin hypercube directions from
+// the point's table, source sample and boundary point, mirrored for the pair's second walk),
+// then SDF walk-on-spheres steps from the boundary point; gradient control variates from the
+// statistics; FP64 gradient, first-source and derivative sums.
+template <int D, bool SCREENED, bool GRAD = false>
 __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
     const zg_walk_settings& ws = P.ws;
     const zg_pde& pde = *P.pdeDev;   // device copy, read where used
@@ -39,18 +45,78 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
         const int p = (int)(c / (unsigned long long)P.chunksPerPoint);
         const int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
         int nw = P.nWalks[p];
-        const uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
-        if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
-        else if (P.dist0[p] <= eps) nw = min(nw, 1);
-        const int wEnd = min((k + 1) * P.wpl, nw);
-        for (int w = k * P.wpl; w < wEnd; w++) {
+        int w0 = k * P.wpl;
+        if (GRAD) {
+            // walks of the stage: [gradW0, min(gradW1, halves * pairs))
+            const int pairs = P.gradHalves == 2 ? max(1, nw / 2) : nw;
+            nw = P.quantity[p] == ZG_ESTIMATE_SOLUTION_AND_GRADIENT ? min(pairs * P.gradHalves, P.gradW1) : 0;
+            w0 = P.gradW0 + k * P.wpl;
+        } else {
+            const uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+            if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
+            else if (P.dist0[p] <= eps) nw = min(nw, 1);
+        }
+        const int wEnd = min(w0 + P.wpl, nw);
+        for (int w = w0; w < wEnd; w++) {
             Vec<D> x = vload<D>(P.pt + (size_t)p * D);
             float dist = P.dist0[p];
             float T = 1.0f, S = 0.0f;
             int len = 0;
             int code = ZG_REACHED_ABSORBING_BOUNDARY;
             ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
-            rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+            // gradient walks: first-step source term, source / boundary gradient directions, control variates
+            float fsc = 0.0f, cvB = 0.0f, cvS = 0.0f;
+            Vec<D> gB = vzero<D>(), gS = vzero<D>();
+            if (GRAD) {
+                const int pair = P.gradHalves == 2 ? (w >> 1) : w;
+                const bool mirror = P.gradHalves == 2 && (w & 1);
+                const Vec<D> c0 = x;
+                rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
+                ball.update(0.99f * dist);
+                const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + (size_t)(D - 1) * 2 * (size_t)pair;
+                if (!ws.ignore_source_contribution) {
+                    const float gNorm = ball.norm();
+                    if (pde.is_source_constant) {
+                        fsc = gNorm * evalField<D>(pde.source, c0, vzero<D>(), false, sigma);
+                    } else {
+                        float pdf;
+                        const Vec<D> sd = sphereFromU<D>(tab);
+                        const float r = ball.sampleRadiusPdf(rng, pdf);
+                        Vec<D> ys = c0 + sd * r;
+                        if (mirror) ys = c0 - (ys - c0);
+                        fsc = gNorm * evalField<D>(pde.source, ys, vzero<D>(), false, sigma);
+                        gS = (ys - c0) * (ball.gradientNorm(r) / (pdf * gNorm));
+                    }
+                    S += fsc;
+                }
+                Vec<D> bd;
+                float bpdf;
+                if (ws.use_cosine_sampling_for_derivatives) {
+                    bd = hemisphereCosineFromU<D>(tab + (D - 1));
+                    if (rng.nextFloat() < 0.5f) bd[D - 1] = -bd[D - 1];
+                    bpdf = 0.5f * pdfHemisphereCosine<D>(fabsf(bd[D - 1]));
+                    Vec<D> nd = vzero<D>();
+                    nd[0] = 1.0f;
+                    if (P.dirDeriv) nd = vload<D>(P.dirDeriv + (size_t)p * D);
+                    bd = fromFrame(nd, bd);
+                } else {
+                    bd = sphereFromU<D>(tab + (D - 1));
+                    bpdf = D == 2 ? 1.0f / (2.0f * kPi) : 1.0f / (4.0f * kPi);
+                }
+                Vec<D> yb = c0 + bd * ball.R;
+                if (mirror) yb = c0 - (yb - c0);
+                x = yb;
+                T *= ball.poissonKernel() / bpdf;
+                gB = (yb - c0) * (ball.poissonKernelGradientScale() / (bpdf * T));
+                if (ws.use_gradient_control_variates) {
+                    const int64_t N = P.cvN[p];
+                    cvB = (float)P.cvMean[p];
+                    cvS = (float)(P.cvFirst[p] / (double)(N > 1 ? N : 1));
+                }
+                dist = fabsf(gridValue<D>(sdf, x));
+            } else {
+                rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+            }
             atomicAdd(&blk[5], 1ull);
             while (dist > eps) {
                 const Vec<D> cen = x;
@@ -91,10 +157,31 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
                 sdfProject<D>(sdf, x, n);
                 g = evalField<D>(pde.dirichlet, x, n, ws.solve_double_sided && sd > 0.0f, sigma);
             }
-            const double v = (double)(T * g + S);
+            const float total = T * g + S;
+            const double v = (double)total;
             atomicAdd(&P.accSum[p], v);
             atomicAdd(&P.accSq[p], v * v);
             atomicAdd(&P.accN[p], 1ull);
+            if (GRAD) {
+                // gradient estimate and directional derivative (walk_on_spheres.h:566-586)
+                const float bc = total - fsc;
+                Vec<D> nd = vzero<D>();
+                nd[0] = 1.0f;
+                if (P.dirDeriv) nd = vload<D>(P.dirDeriv + (size_t)p * D);
+                float dd = 0.0f;
+#pragma unroll
+                for (int i = 0; i < D; i++) {
+                    const float bi = (bc - cvB) * gB[i];
+                    const float si = (fsc - cvS) * gS[i];
+                    dd += bi * nd[i];
+                    dd += si * nd[i];
+                    const double e = (double)(bi + si);
+                    atomicAdd(&P.accG[(size_t)p * D + i], e);
+                    atomicAdd(&P.accG2[(size_t)p * D + i], e * e);
+                }
+                atomicAdd(&P.accFirst[p], (double)fsc);
+                atomicAdd(&P.accDeriv[p], (double)dd);
+            }
             atomicAdd(&P.accLen[p], (unsigned long long)len);
             atomicAdd(&blk[4], (unsigned long long)len);
         }
@@ -150,6 +237,14 @@ hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, int gridBloc
     if (dim == 2) hipLaunchKernelGGL((wosSdfKernel<2, false>), g, bl, 0, s, P);
     else if (screened) hipLaunchKernelGGL((wosSdfKernel<3, true>), g, bl, 0, s, P);
     else hipLaunchKernelGGL((wosSdfKernel<3, false>), g, bl, 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launchWosSdfGrad(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s) {
+    dim3 g(gridBlocks), bl(256);
+    if (dim == 2) hipLaunchKernelGGL((wosSdfKernel<2, false, true>), g, bl, 0, s, P);
+    else if (screened) hipLaunchKernelGGL((wosSdfKernel<3, true, true>), g, bl, 0, s, P);
+    else hipLaunchKernelGGL((wosSdfKernel<3, false, true>), g, bl, 0, s, P);
     return hipGetLastError();
 }
 
