@@ -760,7 +760,7 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
             return fail(ZG_ERR_INVALID_ARGUMENT, "gradient points need the gradient members of zg_statistics");
         words += gradLayout(S).words;
     }
-    // the single-scene walk kernels read the PDE from a device copy after the workspace
+    // the walk-on-stars kernel reads the PDE from a device copy after the workspace
     const size_t pdeWords = (sizeof(zg_pde) + 7) / 8;
     uint64_t* wsp = nullptr;
     ZG_HIP(hipMallocAsync((void**)&wsp, (words + pdeWords) * 8, strm));

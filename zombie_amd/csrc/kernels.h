@@ -59,7 +59,7 @@ struct WosParams {
     const InstanceDesc* instances;    // batched solves: instance table
     const int32_t* instanceOf;        // batched solves: instance of each point
     const uint32_t* streamOffsetDev;  // when set, replaces streamOffset (graph replays advance it)
-    const zg_pde* pdeDev;             // single-scene walks: device copy of pde, read where used
+    const zg_pde* pdeDev;             // walk on stars: device copy of pde, read where used
     const float* fsr;                 // walk on stars: first sphere radius per point
     const int64_t* prevN;             // walk on stars: caller's estimate counts (first call when 0)
     unsigned long long* accSplits;    // walk splitting: split parts per point

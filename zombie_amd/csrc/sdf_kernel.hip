@@ -27,7 +27,7 @@ namespace zg {
 template <int D, bool SCREENED, bool GRAD = false>
 __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
     const zg_walk_settings& ws = P.ws;
-    const zg_pde& pde = *P.pdeDev;   // device copy, read where used
+    const zg_pde& pde = P.pde;
     const zg_function& sdf = P.sdf;
     const float eps = ws.epsilon_shell_for_absorbing_boundary;
     const float sigma = pde.absorption_coeff;
