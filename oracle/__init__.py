@@ -68,6 +68,10 @@ def lib():
         L.orc_sdf_solve.restype = C.c_int
         L.orc_sdf_query.argtypes = [C.c_int, P, P, P, P, C.c_int64, P, P, P, P]
         L.orc_sdf_ray.argtypes = [C.c_int, P, P, P, P, C.c_int64, P, P, P, P, P, P, P]
+        L.orc_sdf_solve_grad.argtypes = [C.c_int, P, P, P, P, P, P, C.POINTER(A.zg_pde), C.POINTER(A.zg_walk_settings),
+                                         C.c_int64, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                         P, P, P, P, P, P, P, P, P]
+        L.orc_sdf_solve_grad.restype = C.c_int
         L.orc_bvc_splat.argtypes = [C.c_int, C.POINTER(A.zg_bvc_samples), C.POINTER(A.zg_bvc_eval_points), C.c_float,
                                     C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P, P, C.c_int]
         L.orc_free_space_kernels.argtypes = [C.c_int, P, P, P, C.c_float, C.c_float, P]
@@ -163,6 +167,23 @@ class SdfGrid:
         lib().orc_sdf_ray(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), n, _p(o), _p(d), _p(tm),
                           _p(hit), _p(t), _p(pt), _p(nr))
         return hit.astype(bool), t, pt, nr
+    def solve_grad(self, pde, ws, pts, dist, n_walks, stats: "GradStats", seed=0x5EED, stream_offset=0,
+                   rng_mode=RNG_PER_WALK, cv_mode=None, direction=None, threads=None):
+        """estimateSolutionAndGradient at every point, walks continued over the SDF geometry
+        (cv_mode: CV_STAGED, the GPU's control-variate stages, by default)"""
+        cv_mode = CV_STAGED if cv_mode is None else cv_mode
+        pts = np.ascontiguousarray(pts, np.float32)
+        n = len(pts)
+        dist = np.ascontiguousarray(dist, np.float32)
+        nw = np.ascontiguousarray(np.broadcast_to(np.asarray(n_walks, np.int32), n), np.int32)
+        d = None if direction is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(direction, np.float32), (n, self.dim)), np.float32)
+        lib().orc_sdf_solve_grad(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), _p(self.dmin),
+                                 _p(self.dmax), C.byref(pde), C.byref(ws), n, _p(pts), _p(dist), _p(nw), _p(d), seed,
+                                 stream_offset, rng_mode, cv_mode, threads or default_threads(), _p(stats.n),
+                                 _p(stats.mean), _p(stats.m2), _p(stats.walk_length), _p(stats.n_grad),
+                                 _p(stats.grad_mean), _p(stats.grad_m2), _p(stats.first_source), _p(stats.derivative))
+
     def solve(self, pde, ws, pts, dist, n_walks, stats: "Stats", seed=0x5EED, stream_offset=0, rng_mode=RNG_PER_WALK,
               threads=None, types=None):
         pts = np.ascontiguousarray(pts, np.float32)

@@ -22,6 +22,7 @@
 //     fcpw_geometric_queries.h:955-967, and FCPW's edge test restated from the WoSt paper)
 //   branchTraversalWeight                                 fcpw_geometric_queries.h:1391-1398
 //   SdfGrid value / gradient / projection / sphere tracing  utils/sdf_grid_geometric_queries.h:87-205
+//     (walks and gradient walks over it: walk_on_spheres.h:228-593 with the SdfGrid queries)
 //   SampleStatistics (FP64 here; float in the reference) point_estimation/common.h:200-395
 //   WalkOnSpheres::estimateSolution / walk / weight window / terminal
 //                                                        point_estimation/walk_on_spheres.h:170-414
@@ -1544,6 +1545,15 @@ struct GradCv {
     }
 };
 
+// SDF-grid geometry (defined with its walk below): the gradient estimator continues its walks
+// over it when given one
+template <int D> struct SdfGeom;
+template <int D>
+Code sdfWalk(const SdfGeom<D>& g, const SolveCtx& cx, float dist, GreensBall<D>& gf, Pcg32& rng, WalkState<D>& st,
+             std::deque<WalkState<D>>& q);
+template <int D>
+float sdfTerminal(const SdfGeom<D>& g, const SolveCtx& cx, Code code, const WalkState<D>& st);
+
 // estimateSolutionAndGradient (walk_on_spheres.h:416-593; walk on stars :664-848, whose first
 // ball has radius 0.99 min(d_absorbing, d_reflecting) and whose walk goes on as a walk-on-stars
 // walk from the boundary point).  rng_mode 0 consumes one generator
@@ -1555,7 +1565,8 @@ struct GradCv {
 template <int D>
 void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t p, const V<D>& pt, float distA,
                      int nWalks, const V<D>& dirDeriv, Stats& out, GradStats<D>& g, int64_t codes[4],
-                     QueryCount* qc, float* firstSphereRadius, bool wost = false, float distR = FLT_MAX) {
+                     QueryCount* qc, float* firstSphereRadius, bool wost = false, float distR = FLT_MAX,
+                     const SdfGeom<D>* sdf = nullptr) {
     const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
     int nHalves = 1;
     if (ws.use_gradient_antithetic_variates) { nWalks = std::max(1, nWalks / 2); nHalves = 2; }
@@ -1631,12 +1642,15 @@ void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t 
                     if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
                 }
                 float dist; V<D> cp; int prim;
-                closestPoint(sc.abs, s2.currentPt, dist, cp, prim, qc);
-                Code code = wost ? wostWalk<D>(sc, cx, dist, 0.0f, false, gf, rng, s2, q, qc)
-                                 : wosWalk<D>(sc, cx, dist, gf, rng, s2, q, qc);
+                if (sdf) dist = std::fabs(sdf->value(s2.currentPt));
+                else closestPoint(sc.abs, s2.currentPt, dist, cp, prim, qc);
+                Code code = sdf ? sdfWalk<D>(*sdf, cx, dist, gf, rng, s2, q)
+                            : wost ? wostWalk<D>(sc, cx, dist, 0.0f, false, gf, rng, s2, q, qc)
+                                   : wosWalk<D>(sc, cx, dist, gf, rng, s2, q, qc);
                 codes[code]++;
                 if (code != Escaped) {
-                    float term = wost ? wostTerminal<D>(sc, cx, code, s2, qc) : wosTerminal<D>(sc, cx, code, s2, qc);
+                    float term = sdf ? sdfTerminal<D>(*sdf, cx, code, s2)
+                                 : wost ? wostTerminal<D>(sc, cx, code, s2, qc) : wosTerminal<D>(sc, cx, code, s2, qc);
                     total += s2.throughput * term + s2.totalReflectingBoundaryContribution + s2.totalSourceContribution;
                     out.len += s2.walkLength;
                     success = true;
@@ -1999,16 +2013,49 @@ struct SdfGeom {
     }
 };
 
-// estimateSolution + walk (walk_on_spheres.h:228-414) over the SDF geometry
+// walk (walk_on_spheres.h:228-291) over the SDF geometry from a point at distance dist
+template <int D>
+Code sdfWalk(const SdfGeom<D>& g, const SolveCtx& cx, float dist, GreensBall<D>& gf, Pcg32& rng, WalkState<D>& st,
+             std::deque<WalkState<D>>& q) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    while (dist > ws.epsilon_shell_for_absorbing_boundary) {
+        gf.updateBall(st.currentPt, dist);
+        if (!ws.ignore_source_contribution) {
+            float sc_;
+            if (pde.is_source_constant) sc_ = gf.norm() * evalField<D>(pde.source, st.currentPt, V<D>::zero(), false, pde.absorption_coeff);
+            else { float r, pdf; V<D> y = gf.sampleVolume(rng, r, pdf); sc_ = gf.norm() * evalField<D>(pde.source, y, V<D>::zero(), false, pde.absorption_coeff); }
+            st.totalSourceContribution += st.throughput * sc_;
+        }
+        V<D> dir = sampleUnitSphere<D>(rng);
+        st.currentPt = st.currentPt + dist * dir;
+        st.throughput *= gf.directionSampledPoissonKernel(st.currentPt);
+        if (g.outside(st.currentPt)) return Escaped;
+        if (applyWeightWindow(ws, rng, st, q)) return RR;
+        st.walkLength++;
+        if (st.walkLength >= ws.max_walk_length) return MaxLen;
+        if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == st.walkLength) gf.makeYukawa(pde.absorption_coeff);
+        dist = std::fabs(g.value(st.currentPt));
+    }
+    return Reached;
+}
+
+// terminal contribution: the Dirichlet value at the steepest-descent projection onto the zero
+// level set, on the side given by the sign of the SDF (getTerminalContribution :293-318)
+template <int D>
+float sdfTerminal(const SdfGeom<D>& g, const SolveCtx& cx, Code code, const WalkState<D>& st) {
+    const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
+    if (!((code == Reached || code == MaxLen) && !ws.ignore_absorbing_boundary_contribution)) return 0.0f;
+    V<D> x = st.currentPt;
+    float sd = g.value(x);
+    V<D> n; g.project(x, n);
+    return evalField<D>(pde.dirichlet, x, n, ws.solve_double_sided && sd > 0.0f, pde.absorption_coeff);
+}
+
+// estimateSolution (walk_on_spheres.h:320-414) over the SDF geometry
 template <int D>
 void wosEstimateSdf(const SdfGeom<D>& g, const SolveCtx& cx, int64_t p, const V<D>& pt, uint8_t type, float distA,
                     int nWalks, Stats& out, int64_t codes[4], int64_t prevN) {
     const zg_walk_settings& ws = *cx.ws; const zg_pde& pde = *cx.pde;
-    auto terminal = [&](V<D> x) {
-        float sd = g.value(x);
-        V<D> n; g.project(x, n);
-        return evalField<D>(pde.dirichlet, x, n, ws.solve_double_sided && sd > 0.0f, pde.absorption_coeff);
-    };
     if (type == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) {
         if (prevN == 0) out.add(ws.ignore_absorbing_boundary_contribution ? 0.0 :
                                 (double)evalField<D>(pde.dirichlet, pt, V<D>::zero(), false, pde.absorption_coeff));
@@ -2023,31 +2070,11 @@ void wosEstimateSdf(const SdfGeom<D>& g, const SolveCtx& cx, int64_t p, const V<
         GreensBall<D> gf;
         if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
         WalkState<D> st; st.currentPt = pt;
-        float dist = distA;
-        Code code = Reached;
         std::deque<WalkState<D>> q;
-        while (dist > ws.epsilon_shell_for_absorbing_boundary) {
-            gf.updateBall(st.currentPt, dist);
-            if (!ws.ignore_source_contribution) {
-                float sc_;
-                if (pde.is_source_constant) sc_ = gf.norm() * evalField<D>(pde.source, st.currentPt, V<D>::zero(), false, pde.absorption_coeff);
-                else { float r, pdf; V<D> y = gf.sampleVolume(rng, r, pdf); sc_ = gf.norm() * evalField<D>(pde.source, y, V<D>::zero(), false, pde.absorption_coeff); }
-                st.totalSourceContribution += st.throughput * sc_;
-            }
-            V<D> dir = sampleUnitSphere<D>(rng);
-            st.currentPt = st.currentPt + dist * dir;
-            st.throughput *= gf.directionSampledPoissonKernel(st.currentPt);
-            if (g.outside(st.currentPt)) { code = Escaped; break; }
-            if (applyWeightWindow(ws, rng, st, q)) { code = RR; break; }
-            st.walkLength++;
-            if (st.walkLength >= ws.max_walk_length) { code = MaxLen; break; }
-            if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == st.walkLength) gf.makeYukawa(pde.absorption_coeff);
-            dist = std::fabs(g.value(st.currentPt));
-        }
+        Code code = sdfWalk<D>(g, cx, distA, gf, rng, st, q);
         codes[code]++;
         if (code == Escaped) continue;
-        float term = 0.0f;
-        if ((code == Reached || code == MaxLen) && !ws.ignore_absorbing_boundary_contribution) term = terminal(st.currentPt);
+        float term = sdfTerminal<D>(g, cx, code, st);
         out.add((double)(st.throughput * term + st.totalSourceContribution));
         out.len += st.walkLength;
     }
@@ -2346,6 +2373,50 @@ int orc_sdf_solve(int dim, const float* grid, const int32_t* shape, const float*
     };
     if (dim == 2) run(std::integral_constant<int, 2>{}); else run(std::integral_constant<int, 3>{});
     if (report) { for (int k = 0; k < 5; k++) { report[k] = 0; for (auto& a : per) report[k] += a[k]; } }
+    return 0;
+}
+// gradient solves over the SDF geometry: every point runs estimateSolutionAndGradient
+// (walk_on_spheres.h:416-593) with its walks continued over the SDF (cvMode as orc_solve_grad)
+int orc_sdf_solve_grad(int dim, const float* grid, const int32_t* shape, const float* gmin, const float* gmax,
+                       const float* dmin, const float* dmax, const zg_pde* pde, const zg_walk_settings* ws, int64_t n,
+                       const float* pts, const float* distA, const int32_t* nWalks, const float* dirDeriv,
+                       uint64_t seed, uint32_t streamOffset, int rngMode, int cvMode, int nThreads, int64_t* sN,
+                       double* sMean, double* sM2, int64_t* sLen, int64_t* gN, double* gMean, double* gM2,
+                       double* gFirst, double* gDeriv) {
+    zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
+    for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
+    SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
+    int nt = std::max(1, nThreads);
+    auto run = [&](auto dimTag) {
+        constexpr int D = decltype(dimTag)::value;
+        SdfGeom<D> g; g.f = f; g.dmin = load<D>(dmin); g.dmax = load<D>(dmax);
+        Scene<D> none;   // the walks query the SDF, not a mesh
+        parallelFor(n, nt, [&](int64_t i, int) {
+            V<D> dd = V<D>::zero(); dd[0] = 1.0f;
+            if (dirDeriv) dd = load<D>(dirDeriv + i * D);
+            GradCv cv{cvMode, sN[i], sMean[i], gFirst[i]};
+            Stats st; GradStats<D> gs; int64_t codes[4] = {0, 0, 0, 0}; QueryCount qc;
+            wosEstimateGrad<D>(none, cx, cv, i, load<D>(pts + i * D), distA[i], nWalks[i], dd, st, gs, codes, &qc,
+                               nullptr, false, FLT_MAX, &g);
+            mergeInto(sN[i], sMean[i], sM2[i], st);
+            if (gs.n > 0) {
+                int64_t N1 = gN[i], N2 = gs.n, N = N1 + N2;
+                for (int k = 0; k < D; k++) {
+                    double& m = gMean[i * D + k]; double& m2 = gM2[i * D + k];
+                    if (N1 == 0) { m = gs.mean[k]; m2 = gs.m2[k]; }
+                    else {
+                        double dlt = gs.mean[k] - m;
+                        m += dlt * ((double)N2 / (double)N);
+                        m2 += gs.m2[k] + dlt * dlt * ((double)N1 * (double)N2 / (double)N);
+                    }
+                }
+                gN[i] = N;
+            }
+            gFirst[i] += gs.firstSource; gDeriv[i] += gs.derivative;
+            sLen[i] += st.len;
+        });
+    };
+    if (dim == 2) run(std::integral_constant<int, 2>{}); else run(std::integral_constant<int, 3>{});
     return 0;
 }
 // SDF queries: |sdf|, projected point and normal (projectToAbsorbingBoundary for an SdfGrid)

@@ -132,7 +132,7 @@ def _grad_points(torch, sc, x, n_walks):
 
 
 @pytest.mark.parametrize("case", ["harmonic2d", "harmonic3d", "poisson3d", "screened3d"])
-def test_sdf_gradient_matches_analytic(gpu, case):
+def test_sdf_gradient_matches_analytic_and_oracle(gpu, oracle_lib, case):
     """estimateSolutionAndGradient over SDF-grid geometry (walk_on_spheres.h:416-593 with the
     SdfGrid queries): gradients and solutions against the manufactured u on the unit disk / ball
     (harmonic e^x cos y, Poisson sin 2x sin 2y sin 2z with its source, screened exp(a . x))"""
@@ -172,6 +172,17 @@ def test_sdf_gradient_matches_analytic(gpu, case):
     assert np.all(np.abs(pooled) < 4.0), (case, pooled)
     mean, var, cnt = st.mean.cpu().numpy(), st.variance().cpu().numpy(), st.n.cpu().numpy()
     assert abs(np.sum(mean - u) / np.sqrt(np.sum(var / cnt))) < 4.0
+    # against the oracle's gradient walks over the same SDF (4 SE per point and component)
+    og = oracle_lib.SdfGrid(vals, lo, hi)
+    ost = oracle_lib.GradStats(len(x), dim)
+    og.solve_grad(pde, ws, x, sc.closest_point(_t(torch, x), want_point=False)[0].cpu().numpy(), 256, ost, seed=33)
+    zs = []
+    for k in range(dim):
+        se2 = gv[:, k] / ng + ost.grad_variance()[:, k] / ost.n_grad
+        m = se2 > 0
+        zs.append((gm[m, k] - ost.grad_mean[m, k]) / np.sqrt(se2[m]))
+    z = np.concatenate(zs)
+    assert int((np.abs(z) > 4.0).sum()) <= 3 and abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05, (case, z.mean())
 
 
 def test_sdf_scene_rejections(gpu):

@@ -74,3 +74,34 @@ def test_sphere_tracing_matches_the_exact_sphere(oracle_lib, dim, res, tol):
     assert hit2.all() and np.abs(t2 - 0.15).max() < tol and np.all((nr2 * d).sum(1) > 0.99)
     assert not g.ray(outside, d.astype(np.float32), 10.0)[0].any()
     assert not g.ray(o.astype(np.float32), d.astype(np.float32), (0.5 * t_exact).astype(np.float32))[0].any()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sdf_gradient_walks_match_analytic(oracle_lib, dim):
+    """estimateSolutionAndGradient continued over the SDF geometry (walk_on_spheres.h:416-593 with
+    the SdfGrid queries): gradients and solutions of u = e^x cos y on the sampled unit disk / ball
+    within 4 pooled standard errors; dropping the first step's boundary term is detected."""
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(dim, 96 if dim == 2 else 48)
+    g = O.SdfGrid(vals, lo, hi)
+    rng = np.random.default_rng(31)
+    x = rng.normal(size=(300, dim))
+    x = (x / np.linalg.norm(x, axis=1, keepdims=True) * rng.random((300, 1)) ** (1 / dim) * 0.7).astype(np.float32)
+    d, _, _ = g.query(x)
+    pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_COS, A.ZG_EVAL_VALUE, (1.0, 1.0)),
+                     is_source_constant=True)
+    ws = E.walk_settings_host(1e-3, 1e-3, 1024)
+    st = O.GradStats(len(x), dim)
+    g.solve_grad(pde, ws, x, d, 256, st, seed=3)
+    assert (st.n_grad == 256).all() and (st.n == 256).all()
+    xd = x.astype(np.float64)
+    e = np.exp(xd[:, 0])
+    gr = np.zeros_like(xd)
+    gr[:, 0] = e * np.cos(xd[:, 1])
+    gr[:, 1] = -e * np.sin(xd[:, 1])
+    pooled = np.sum(st.grad_mean - gr, 0) / np.sqrt(np.sum(st.grad_variance() / st.n_grad[:, None], 0))
+    assert np.all(np.abs(pooled) < 4.0), pooled
+    u = e * np.cos(xd[:, 1])
+    assert abs(np.sum(st.mean - u) / np.sqrt(np.sum(st.variance() / st.n))) < 4.0
+    # the gradient is not the trivial one: its x component is far from zero in every point's units
+    assert np.abs(np.sum(st.grad_mean[:, 0]) / np.sqrt(np.sum(st.grad_variance()[:, 0] / st.n_grad))) > 10.0
