@@ -159,20 +159,34 @@ def test_wost3d_parity_with_oracle_and_closed_form(gpu, oracle_lib, prism, width
     assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)
 
 
-def test_wost3d_gradient(gpu, oracle_lib):
+@pytest.mark.parametrize("case", ["neumann", "robin", "screened"])
+def test_wost3d_gradient(gpu, oracle_lib, case):
     """walk_on_stars.h:664-848 in 3D: first ball 0.99 min(d_absorbing, d_reflecting), then
-    walk-on-stars steps; gradients against grad(e^x cos y) and the oracle"""
+    walk-on-stars steps (Neumann, Robin and screened); gradients against the manufactured
+    gradient and the oracle"""
     torch = gpu
     O = oracle_lib
-    p = PB.lprism3d(plate=True)
+    if case == "neumann":
+        p = PB.lprism3d(plate=True)
+    elif case == "robin":
+        p = PB.lprism3d_robin(kappa=2.0, plate=True)
+    else:
+        p = PB.lprism3d_screened(sigma=4.0, plate=True)
     osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    sc = _scene(p, 8)
+    if "robin" in p.meta:
+        osc.set_robin(*p.meta["robin"])
+        sc.set_robin_coefficients(*p.meta["robin"])
     pts = _interior(p, 300, 17, margin=0.05)
     n = len(pts)
     d = osc.closest_point(pts, boundary=0)[0]
     x = pts.astype(np.float64)
-    e = np.exp(x[:, 0])
-    g = np.stack([e * np.cos(x[:, 1]), -e * np.sin(x[:, 1]), np.zeros(n)], 1)
-    sc = _scene(p, 8)
+    if case == "screened":
+        a = np.array(p.meta["a"])
+        g = np.exp(x @ a)[:, None] * a[None, :]
+    else:
+        e = np.exp(x[:, 0])
+        g = np.stack([e * np.cos(x[:, 1]), -e * np.sin(x[:, 1]), np.zeros(n)], 1)
     dp = E.DevicePoints(pt=_t(torch, pts), dist_to_absorbing=_t(torch, d),
                         n_walks=torch.full((n,), 256, dtype=torch.int32, device="cuda"), n_walks_max=256,
                         first_sphere_radius=torch.zeros(n, dtype=torch.float32, device="cuda"),
@@ -183,7 +197,7 @@ def test_wost3d_gradient(gpu, oracle_lib):
     gm = st.grad_mean.cpu().numpy(); gv = st.grad_variance().cpu().numpy(); ng = st.n_grad.cpu().numpy()
     assert rep.completion_codes[3] <= 1e-4 * rep.total_walks + 2 and ng.min() >= 250
     pooled = np.sum(gm - g, axis=0) / np.sqrt(np.sum(gv / ng[:, None], axis=0))
-    assert np.all(np.abs(pooled) < 4.0), pooled
+    assert np.all(np.abs(pooled) < 4.0), (case, pooled)
     ost = O.GradStats(n, 3)
     O.solve_grad(osc, p.pde, p.settings, pts, d, 256, ost, wost=True, seed=34, cv_mode=O.CV_STAGED)
     zs = []
@@ -192,7 +206,7 @@ def test_wost3d_gradient(gpu, oracle_lib):
         m = se2 > 0
         zs.append((gm[m, k] - ost.grad_mean[m, k]) / np.sqrt(se2[m]))
     z = np.concatenate(zs)
-    assert int((np.abs(z) > 4.0).sum()) <= 3 and abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05
+    assert int((np.abs(z) > 4.0).sum()) <= 3 and abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05, (case, z.mean())
 
 
 @pytest.mark.parametrize("width", [4, 8])
