@@ -178,11 +178,13 @@ class SdfGrid:
         nw = np.ascontiguousarray(np.broadcast_to(np.asarray(n_walks, np.int32), n), np.int32)
         d = None if direction is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(direction, np.float32), (n, self.dim)), np.float32)
-        lib().orc_sdf_solve_grad(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), _p(self.dmin),
+        status = lib().orc_sdf_solve_grad(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), _p(self.dmin),
                                  _p(self.dmax), C.byref(pde), C.byref(ws), n, _p(pts), _p(dist), _p(nw), _p(d), seed,
                                  stream_offset, rng_mode, cv_mode, threads or default_threads(), _p(stats.n),
                                  _p(stats.mean), _p(stats.m2), _p(stats.walk_length), _p(stats.n_grad),
                                  _p(stats.grad_mean), _p(stats.grad_m2), _p(stats.first_source), _p(stats.derivative))
+        if status != 0:
+            raise RuntimeError(f"oracle SDF gradient solve failed with status {status}")
 
     def solve(self, pde, ws, pts, dist, n_walks, stats: "Stats", seed=0x5EED, stream_offset=0, rng_mode=RNG_PER_WALK,
               threads=None, types=None):
@@ -192,10 +194,12 @@ class SdfGrid:
         nw = np.ascontiguousarray(np.broadcast_to(np.asarray(n_walks, np.int32), n), np.int32)
         types = None if types is None else np.ascontiguousarray(types, np.uint8)
         rep = np.zeros(5, np.int64)
-        lib().orc_sdf_solve(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), _p(self.dmin),
+        status = lib().orc_sdf_solve(self.dim, _p(self.values), _p(self.shape), _p(self.lo), _p(self.hi), _p(self.dmin),
                             _p(self.dmax), C.byref(pde), C.byref(ws), n, _p(pts), _p(types), _p(dist), _p(nw), seed,
                             stream_offset, rng_mode, threads or default_threads(), _p(stats.n), _p(stats.mean),
                             _p(stats.m2), _p(stats.walk_length), _p(rep))
+        if status != 0:
+            raise RuntimeError(f"oracle SDF solve failed with status {status}")
         return dict(codes=rep[:4].copy(), total_steps=int(rep[4]))
 
 

@@ -2353,6 +2353,8 @@ int orc_sdf_solve(int dim, const float* grid, const int32_t* shape, const float*
                   const float* pts, const uint8_t* types, const float* distA, const int32_t* nWalks, uint64_t seed,
                   uint32_t streamOffset, int rngMode, int nThreads, int64_t* sN, double* sMean, double* sM2,
                   int64_t* sLen, int64_t* report) {
+    // the SDF walks keep no queue of split parts (walk splitting runs on mesh scenes)
+    if (ws->splitting_threshold < 1.0f) return ZG_ERR_UNSUPPORTED;
     zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
     for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
     SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
@@ -2383,6 +2385,7 @@ int orc_sdf_solve_grad(int dim, const float* grid, const int32_t* shape, const f
                        uint64_t seed, uint32_t streamOffset, int rngMode, int cvMode, int nThreads, int64_t* sN,
                        double* sMean, double* sM2, int64_t* sLen, int64_t* gN, double* gMean, double* gM2,
                        double* gFirst, double* gDeriv) {
+    if (ws->splitting_threshold < 1.0f) return ZG_ERR_UNSUPPORTED;   // as orc_sdf_solve
     zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
     for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
     SolveCtx cx{pde, ws, seed, streamOffset, rngMode};

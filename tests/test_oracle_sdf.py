@@ -105,3 +105,19 @@ def test_sdf_gradient_walks_match_analytic(oracle_lib, dim):
     assert abs(np.sum(st.mean - u) / np.sqrt(np.sum(st.variance() / st.n))) < 4.0
     # the gradient is not the trivial one: its x component is far from zero in every point's units
     assert np.abs(np.sum(st.grad_mean[:, 0]) / np.sqrt(np.sum(st.grad_variance()[:, 0] / st.n_grad))) > 10.0
+
+
+def test_sdf_splitting_is_rejected(oracle_lib):
+    """the SDF walks keep no queue of split parts: a splitting threshold below 1 is refused (as on
+    the GPU) rather than dropping the split parts"""
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(2, 32)
+    g = O.SdfGrid(vals, lo, hi)
+    x = np.zeros((2, 2), np.float32)
+    d, _, _ = g.query(x)
+    pde = E.make_pde(source=E.zero(), dirichlet=E.constant(1.0))
+    ws = E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5)
+    with pytest.raises(RuntimeError):
+        g.solve(pde, ws, x, d, 4, O.Stats(2))
+    with pytest.raises(RuntimeError):
+        g.solve_grad(pde, ws, x, d, 4, O.GradStats(2, 2))
