@@ -70,7 +70,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0x5EED)
     ap.add_argument("--point-order", default="kd", choices=["kd", "random"],
-                    help="sphere3d: solve the points in kd-tree order (getSpatiallySortedPointIndices) or as drawn")
+                    help="sphere3d, mixed2d: solve the points in kd-tree order (getSpatiallySortedPointIndices) "
+                         "or as drawn")
     return ap.parse_args()
 
 
@@ -478,6 +479,8 @@ def bench_mixed2d(args, torch, dev):
     n_target = args.points if args.points != 1_000_000 else 200_000
     cand = PB.sample_box(prob.domain_min, prob.domain_max, 2 * n_target, args.seed)
     pts = cand[PB.points_in_polygon(cand, V, I)][:n_target]
+    if args.point_order == "kd":
+        pts = pts[E.spatially_sorted_point_indices(pts)]
     P = torch.from_numpy(pts).to(dev)
     n = P.shape[0]
     dist, _, _, _ = scene.closest_point(P, boundary=0, want_point=False)
@@ -517,8 +520,9 @@ def bench_mixed2d(args, torch, dev):
     steps = sum(r.total_steps for r in reps)
     walks_done = sum(r.total_walks for r in reps)
     kms = sum(r.kernel_ms for r in reps) / len(reps)
-    sub = 16384
-    dps = E.DevicePoints(pt=P[:sub].contiguous(), dist_to_absorbing=dist[:sub].contiguous(),
+    stride = max(1, n // 16384)   # a strided sample spread over the domain whatever the point order
+    sub = len(range(0, n, stride))
+    dps = E.DevicePoints(pt=P[::stride].contiguous(), dist_to_absorbing=dist[::stride].contiguous(),
                          n_walks=torch.full((sub,), 64, dtype=torch.int32, device=dev), n_walks_max=64)
     rc = E.wost_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(99, True))
     # exact-distance walk length on the sample: the value is reported in the reference's unit
@@ -531,7 +535,7 @@ def bench_mixed2d(args, torch, dev):
         "workload": f"2D WoSt, star curve 4,096 segments + 32 Neumann slits, {n} points, {walks} walks/pt/pass, "
                     "eps 1e-3", "bvh_width": args.bvh_width, "leaf_size": args.leaf_size,
         "walks_per_lane": args.walks_per_lane, "regroup_interval": args.regroup,
-        "distance_tolerance": args.tolerance, "points": n}, value, el, args,
+        "distance_tolerance": args.tolerance, "points": n, "point_order": args.point_order}, value, el, args,
         {"value_definition": "completed walks/s x mean walk length with exact distances (reference walk-steps/s)",
          "walks_per_s": walks_done / el, "raw_walk_steps_per_s": steps / el, "mean_walk_length_exact": len_exact,
          "mean_walk_length": steps / max(walks_done, 1), "rmse_vs_analytic": rmse,
@@ -541,12 +545,13 @@ def bench_mixed2d(args, torch, dev):
                                "ray, sampling, visibility) are not counted")})
     if not args.no_cpu_baseline:
         k, kw = 2048, 64
+        ks = max(1, n // k)   # strided sample
         threads = min(os.cpu_count() or 1, 16)
         osc = O.Scene(prob.positions, prob.indices, prob.is_reflecting, prob.domain_min, prob.domain_max,
                       solve_double_sided=True)
         res["cpu_baseline"] = _oracle_baseline(
             f"{k} points x {kw} walks (binary-BVH star radius, sampling, visibility and distance queries)",
-            lambda: O.solve(osc, prob.pde, prob.settings, pts[:k], dist[:k].cpu().numpy(), kw, O.Stats(k), seed=7,
+            lambda: O.solve(osc, prob.pde, prob.settings, pts[::ks][:k], dist[::ks][:k].cpu().numpy(), kw, O.Stats(k), seed=7,
                             wost=True, rng_mode=O.RNG_PER_POINT, threads=threads), threads)
         res["speedup_vs_cpu_baseline"] = res["value"] / res["cpu_baseline"]["value"]
     return res
