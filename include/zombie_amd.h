@@ -173,7 +173,7 @@ zg_status zg_scene_destroy(zg_scene* scene);
  * bi/trilinear interpolation clamped to the edge.  zg_closest_point returns |sdf|, the
  * steepest-descent projection onto the zero level set and its normal (prim -1);
  * zg_inside_domain tests sdf < 0; zg_intersect_ray sphere-traces the zero level set;
- * zg_wos_solve walks with radii |sdf|, gradient points included (no walk splitting or
+ * zg_wos_solve walks with radii |sdf|, gradient points and walk splitting included (no
  * control-variate gradients).  Walks leaving [domain_min, domain_max]
  * (NULL: the grid box) escape. */
 zg_status zg_sdf_scene_create(int32_t dim, const float* values_host, const int32_t* shape_host,

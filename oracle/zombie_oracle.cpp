@@ -2067,16 +2067,26 @@ void wosEstimateSdf(const SdfGeom<D>& g, const SolveCtx& cx, int64_t p, const V<
     for (int w = 0; w < nWalks; w++) {
         Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
         Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
-        GreensBall<D> gf;
-        if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
-        WalkState<D> st; st.currentPt = pt;
+        // the walk and its split parts (applyWeightWindow queues them); the estimate is their sum
         std::deque<WalkState<D>> q;
-        Code code = sdfWalk<D>(g, cx, distA, gf, rng, st, q);
-        codes[code]++;
-        if (code == Escaped) continue;
-        float term = sdfTerminal<D>(g, cx, code, st);
-        out.add((double)(st.throughput * term + st.totalSourceContribution));
-        out.len += st.walkLength;
+        WalkState<D> st0; st0.currentPt = pt;
+        q.push_back(st0);
+        int splits = -1; float total = 0.0f; bool success = false;
+        while (!q.empty()) {
+            WalkState<D> st = q.front(); q.pop_front();
+            splits++;
+            GreensBall<D> gf;
+            if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
+            float dist = splits > 0 ? std::fabs(g.value(st.currentPt)) : distA;
+            Code code = sdfWalk<D>(g, cx, dist, gf, rng, st, q);
+            codes[code]++;
+            if (code == Escaped) continue;
+            float term = sdfTerminal<D>(g, cx, code, st);
+            total += st.throughput * term + st.totalSourceContribution;
+            out.len += st.walkLength;
+            success = true;
+        }
+        if (success) { out.add((double)total); out.splits += splits; }
     }
 }
 
@@ -2353,8 +2363,6 @@ int orc_sdf_solve(int dim, const float* grid, const int32_t* shape, const float*
                   const float* pts, const uint8_t* types, const float* distA, const int32_t* nWalks, uint64_t seed,
                   uint32_t streamOffset, int rngMode, int nThreads, int64_t* sN, double* sMean, double* sM2,
                   int64_t* sLen, int64_t* report) {
-    // the SDF walks keep no queue of split parts (walk splitting runs on mesh scenes)
-    if (ws->splitting_threshold < 1.0f) return ZG_ERR_UNSUPPORTED;
     zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
     for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
     SolveCtx cx{pde, ws, seed, streamOffset, rngMode};
@@ -2385,7 +2393,6 @@ int orc_sdf_solve_grad(int dim, const float* grid, const int32_t* shape, const f
                        uint64_t seed, uint32_t streamOffset, int rngMode, int cvMode, int nThreads, int64_t* sN,
                        double* sMean, double* sM2, int64_t* sLen, int64_t* gN, double* gMean, double* gM2,
                        double* gFirst, double* gDeriv) {
-    if (ws->splitting_threshold < 1.0f) return ZG_ERR_UNSUPPORTED;   // as orc_sdf_solve
     zg_function f{}; f.kind = ZG_FN_DENSE_GRID; f.grid = grid; f.interpolate = 1;
     for (int a = 0; a < 3; a++) { f.shape[a] = a < dim ? shape[a] : 1; f.grid_min[a] = a < dim ? gmin[a] : 0; f.grid_max[a] = a < dim ? gmax[a] : 1; }
     SolveCtx cx{pde, ws, seed, streamOffset, rngMode};

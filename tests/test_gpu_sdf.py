@@ -185,6 +185,37 @@ def test_sdf_gradient_matches_analytic_and_oracle(gpu, oracle_lib, case):
     assert int((np.abs(z) > 4.0).sum()) <= 3 and abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05, (case, z.mean())
 
 
+def test_sdf_walk_splitting_parity(gpu, oracle_lib):
+    """walk splitting over SDF walks (screened, threshold 0.4): parts are queued per lane and
+    summed; GPU estimates agree with the oracle's (4 SE) and with exp(a . x), and the split count
+    reaches the caller's statistics"""
+    torch = gpu
+    O = oracle_lib
+    vals, lo, hi = PB.sphere_sdf_grid(3, 48)
+    sc = E.SdfScene(vals, lo, hi)
+    og = O.SdfGrid(vals, lo, hi)
+    sigma = 6.0
+    a = np.sqrt(sigma) * np.array([0.0, 0.6, 0.8])
+    pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a)),
+                     absorption=sigma, is_source_constant=True)
+    ws = E.walk_settings_host(1e-3, 1e-3, 1024, splitting_threshold=0.4, russian_roulette_threshold=0.05)
+    x = _ball(np.random.default_rng(8), 800, 3, 0.8)
+    d = sc.closest_point(_t(torch, x), want_point=False)[0]
+    n = len(x)
+    dp = E.DevicePoints(pt=_t(torch, x), dist_to_absorbing=d.contiguous(),
+                        n_walks=torch.full((n,), 256, dtype=torch.int32, device="cuda"), n_walks_max=256)
+    st = E.DeviceStats.zeros(n)
+    E.wos_solve(sc, pde, ws, dp, st, E.SolveOptions(seed=12))
+    g = dict(n=st.n.cpu().numpy(), mean=st.mean.cpu().numpy(), var=st.variance().cpu().numpy())
+    assert (g["n"] == 256).all() and st.splits.sum().item() > 0.2 * 256 * n   # walks did split
+    ost = O.Stats(n)
+    og.solve(pde, ws, x, d.cpu().numpy(), 256, ost, seed=13)
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, "sdf splitting gpu vs oracle")
+    u = np.exp(x.astype(np.float64) @ a)
+    assert abs(np.sum(g["mean"] - u) / np.sqrt(np.sum(g["var"] / g["n"]))) < 4.0
+
+
 def test_sdf_scene_rejections(gpu):
     torch = gpu
     vals, lo, hi = PB.sphere_sdf_grid(2, 32)
@@ -196,8 +227,6 @@ def test_sdf_scene_rejections(gpu):
                         n_walks_max=8)
     with pytest.raises(A.ZombieError):      # walk on stars needs a mesh
         E.wost_solve(sc, pde, E.walk_settings_host(1e-3, 1e-3, 64), dp, E.DeviceStats.zeros(4))
-    with pytest.raises(A.ZombieError):      # no splitting on SDF scenes
-        E.wos_solve(sc, pde, E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5), dp, E.DeviceStats.zeros(4))
     with pytest.raises(A.ZombieError):      # no reflecting boundary
         sc.closest_point(x, boundary=1)
     with pytest.raises(A.ZombieError):

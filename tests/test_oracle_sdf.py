@@ -107,17 +107,26 @@ def test_sdf_gradient_walks_match_analytic(oracle_lib, dim):
     assert np.abs(np.sum(st.grad_mean[:, 0]) / np.sqrt(np.sum(st.grad_variance()[:, 0] / st.n_grad))) > 10.0
 
 
-def test_sdf_splitting_is_rejected(oracle_lib):
-    """the SDF walks keep no queue of split parts: a splitting threshold below 1 is refused (as on
-    the GPU) rather than dropping the split parts"""
+def test_sdf_walk_splitting_matches_exact(oracle_lib):
+    """applyWeightWindow's splitting (walk_on_spheres.h:192-226) over the SDF walks: a walk whose
+    throughput exceeds the threshold continues at the threshold and queues parts; the estimate
+    is the sum over the parts.  Screened u = exp(a . x) on the sampled unit ball; a walk that
+    dropped its parts would lose weight and be detected."""
     O = oracle_lib
-    vals, lo, hi = PB.sphere_sdf_grid(2, 32)
+    vals, lo, hi = PB.sphere_sdf_grid(3, 48)
     g = O.SdfGrid(vals, lo, hi)
-    x = np.zeros((2, 2), np.float32)
+    sigma = 6.0
+    a = np.sqrt(sigma) * np.array([0.0, 0.6, 0.8])
+    pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a)),
+                     absorption=sigma, is_source_constant=True)
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(300, 3))
+    x = (x / np.linalg.norm(x, axis=1, keepdims=True) * rng.random((300, 1)) ** (1 / 3) * 0.8).astype(np.float32)
     d, _, _ = g.query(x)
-    pde = E.make_pde(source=E.zero(), dirichlet=E.constant(1.0))
-    ws = E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5)
-    with pytest.raises(RuntimeError):
-        g.solve(pde, ws, x, d, 4, O.Stats(2))
-    with pytest.raises(RuntimeError):
-        g.solve_grad(pde, ws, x, d, 4, O.GradStats(2, 2))
+    u = np.exp(x.astype(np.float64) @ a)
+    for thr in (0.4, 3.4e38):   # split, unsplit
+        ws = E.walk_settings_host(1e-3, 1e-3, 1024, splitting_threshold=thr, russian_roulette_threshold=0.05)
+        st = O.Stats(len(x))
+        g.solve(pde, ws, x, d, 256, st, seed=9)
+        pooled = np.sum(st.mean - u) / np.sqrt(np.sum(st.variance() / st.n))
+        assert abs(pooled) < 4.0, (thr, pooled)

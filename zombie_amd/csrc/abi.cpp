@@ -549,7 +549,6 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     else { P.instances = b->table; P.instanceOf = instanceOf; P.nInstances = b->n; }
     if (s && s->sdfGrid) {
         if (wost) return fail(ZG_ERR_UNSUPPORTED, "walk on stars needs a mesh scene");
-        if (split) return fail(ZG_ERR_UNSUPPORTED, "walk splitting is not available on SDF-grid scenes");
         if (pts->cv_gradient || pts->cv_gradient_grid)
             return fail(ZG_ERR_UNSUPPORTED, "control-variate gradients are not available on SDF-grid scenes");
         P.sdf = s->sdf;
@@ -642,7 +641,7 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     if (e == hipSuccess && S.nWalksMax > 0)
         e = S.wost ? launchWost(P, S.dim, S.width, S.count, S.blocks, strm)
             : S.batch ? launchWosBatch(P, S.width, S.count, S.screened, S.split, S.blocks, strm)
-            : S.sdf ? launchWosSdf(P, S.dim, S.screened, S.blocks, strm)
+            : S.sdf ? launchWosSdf(P, S.dim, S.screened, S.split, S.blocks, strm)
                     : launchWos(P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
     if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
     if (e == hipSuccess) e = launchMergeStats(S.M, S.dim, strm);
@@ -728,7 +727,7 @@ hipError_t enqueueGradient(SolveSetup& S, uint64_t* wsp, bool fillCounters, hipS
         P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
         if (e == hipSuccess)
             e = S.wost ? launchWostGrad(P, S.dim, S.width, S.blocks, strm)
-                : S.sdf ? launchWosSdfGrad(P, D, S.screened, S.blocks, strm)
+                : S.sdf ? launchWosSdfGrad(P, D, S.screened, S.split, S.blocks, strm)
                         : launchWosGrad(P, D, S.width, S.screened, S.split, S.blocks, strm);
         if (e == hipSuccess) e = launchMergeStats(M, D, strm);
         u0 = u1;

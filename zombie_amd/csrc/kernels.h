@@ -149,10 +149,10 @@ hipError_t launchStratified(int dims, float* out, int64_t nTables, int64_t strid
 hipError_t launchEvalFunction(const zg_function& f, int dim, int64_t n, const float* pts, const float* normals,
                               const uint8_t* aligned, float absorption, float* out, hipStream_t s);
 // SDF-grid scenes: walk on spheres and the distance / projection query
-hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s);
+hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, bool split, int gridBlocks, hipStream_t s);
 hipError_t launchSdfQuery(const zg_function& sdf, int dim, int64_t n, const float* pts, float* dist, float* closest,
                           float* normal, int32_t* prim, hipStream_t s);
-hipError_t launchWosSdfGrad(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s);
+hipError_t launchWosSdfGrad(const WosParams& P, int dim, bool screened, bool split, int gridBlocks, hipStream_t s);
 hipError_t launchSdfRay(const zg_function& sdf, int dim, int64_t n, const float* o, const float* d, const float* tmax,
                         uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
 hipError_t launchSdfInside(const zg_function& sdf, int dim, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);

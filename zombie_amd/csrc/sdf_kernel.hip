@@ -8,7 +8,9 @@
 // end.  A walk that enters the epsilon shell (or stops at the maximum length) projects to the
 // zero level set by steepest descent and takes the Dirichlet value there, on the side given by
 // the sign of the SDF.  Statistics as in the walk-on-spheres kernel: FP64 per-point atomics,
-// merged into the caller's statistics by mergeStatsKernel.
+// merged into the caller's statistics by mergeStatsKernel.  SPLIT = true adds walk splitting
+// with the walk-on-spheres kernel's per-lane queue of parts (a walk's estimate is the sum over
+// its parts).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -24,7 +26,7 @@ namespace zg {
 // the point's table, source sample and boundary point, mirrored for the pair's second walk),
 // then SDF walk-on-spheres steps from the boundary point; gradient control variates from the
 // statistics; FP64 gradient, first-source and derivative sums.
-template <int D, bool SCREENED, bool GRAD = false>
+template <int D, bool SCREENED, bool GRAD = false, bool SPLIT = false>
 __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
     const zg_walk_settings& ws = P.ws;
     const zg_pde& pde = P.pde;
@@ -62,7 +64,6 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
             float dist = P.dist0[p];
             float T = 1.0f, S = 0.0f;
             int len = 0;
-            int code = ZG_REACHED_ABSORBING_BOUNDARY;
             ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
             // gradient walks: first-step source term, source / boundary gradient directions, control variates
             float fsc = 0.0f, cvB = 0.0f, cvS = 0.0f;
@@ -118,46 +119,82 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
                 rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
             }
             atomicAdd(&blk[5], 1ull);
-            while (dist > eps) {
-                const Vec<D> cen = x;
-                ball.update(dist);
-                if (!ws.ignore_source_contribution) {
-                    float f;
-                    if (pde.is_source_constant) {
-                        f = evalField<D>(pde.source, cen, vzero<D>(), false, sigma);
-                    } else {
-                        Vec<D> sdir = sampleUnitSphere<D>(rng);
-                        float r = ball.sampleRadius(rng);
-                        f = evalField<D>(pde.source, cen + sdir * r, vzero<D>(), false, sigma);
+            // splitting: parts of the walk still to run (position, throughput) and the sum over its
+            // finished parts; a part starts with no source sum and length 0
+            float4 splitQ[SPLIT ? kSplitQueue : 1];
+            int nq = 0, nSplits = 0;
+            float partSum = 0.0f;
+            bool partOk = false;
+            for (;;) {
+                int code = ZG_REACHED_ABSORBING_BOUNDARY;
+                while (dist > eps) {
+                    const Vec<D> cen = x;
+                    ball.update(dist);
+                    if (!ws.ignore_source_contribution) {
+                        float f;
+                        if (pde.is_source_constant) {
+                            f = evalField<D>(pde.source, cen, vzero<D>(), false, sigma);
+                        } else {
+                            Vec<D> sdir = sampleUnitSphere<D>(rng);
+                            float r = ball.sampleRadius(rng);
+                            f = evalField<D>(pde.source, cen + sdir * r, vzero<D>(), false, sigma);
+                        }
+                        S += T * ball.norm() * f;
                     }
-                    S += T * ball.norm() * f;
+                    Vec<D> dir = sampleUnitSphere<D>(rng);
+                    x = cen + dir * dist;
+                    if (SCREENED && ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - cen))));
+                    if (outsideDomain<D>(P.scene.dmin, P.scene.dmax, x)) { code = ZG_ESCAPED_DOMAIN; break; }
+                    if (SPLIT && T > ws.splitting_threshold) {   // applyWeightWindow :198-213
+                        const float thr = ws.splitting_threshold;
+                        float left = T - thr;
+                        int copies = 0;
+                        for (float l = left; l > thr && copies <= kSplitQueue; l -= thr) copies++;
+                        if (nq + copies + 1 <= kSplitQueue) {   // else unsplit (just as unbiased)
+                            T = thr;
+                            const float4 e = make_float4(x[0], x[1], D == 3 ? x[D - 1] : 0.0f, thr);
+                            while (left > thr) { left -= thr; splitQ[nq++] = e; }
+                            if (rng.nextFloat() < left / thr) splitQ[nq++] = e;
+                        }
+                    } else if (T < ws.russian_roulette_threshold) {   // applyWeightWindow :214-223
+                        const float survival = T / ws.russian_roulette_threshold;
+                        if (survival < rng.nextFloat()) { T = 0.0f; code = ZG_TERMINATED_WITH_RUSSIAN_ROULETTE; break; }
+                        T = ws.russian_roulette_threshold;
+                    }
+                    len++;
+                    if (len >= ws.max_walk_length) { code = ZG_EXCEEDED_MAX_WALK_LENGTH; break; }
+                    if (SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
+                    dist = fabsf(gridValue<D>(sdf, x));
                 }
-                Vec<D> dir = sampleUnitSphere<D>(rng);
-                x = cen + dir * dist;
-                if (SCREENED && ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - cen))));
-                if (outsideDomain<D>(P.scene.dmin, P.scene.dmax, x)) { code = ZG_ESCAPED_DOMAIN; break; }
-                if (T < ws.russian_roulette_threshold) {   // applyWeightWindow :214-223
-                    const float survival = T / ws.russian_roulette_threshold;
-                    if (survival < rng.nextFloat()) { T = 0.0f; code = ZG_TERMINATED_WITH_RUSSIAN_ROULETTE; break; }
-                    T = ws.russian_roulette_threshold;
+                atomicAdd(&blk[code], 1ull);
+                if (code != ZG_ESCAPED_DOMAIN) {
+                    float g = 0.0f;
+                    if ((code == ZG_REACHED_ABSORBING_BOUNDARY || code == ZG_EXCEEDED_MAX_WALK_LENGTH) &&
+                        !ws.ignore_absorbing_boundary_contribution) {
+                        // projectToAbsorbingBoundary for an SdfGrid (sdf_grid_geometric_queries.h:254-259)
+                        const float sd = gridValue<D>(sdf, x);
+                        Vec<D> n;
+                        sdfProject<D>(sdf, x, n);
+                        g = evalField<D>(pde.dirichlet, x, n, ws.solve_double_sided && sd > 0.0f, sigma);
+                    }
+                    partSum += T * g + S;
+                    partOk = true;
+                    atomicAdd(&P.accLen[p], (unsigned long long)len);
+                    atomicAdd(&blk[4], (unsigned long long)len);
                 }
-                len++;
-                if (len >= ws.max_walk_length) { code = ZG_EXCEEDED_MAX_WALK_LENGTH; break; }
-                if (SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
+                if (!SPLIT || nq == 0) break;
+                nq--;   // the next part of the same walk
+                const float4 e = splitQ[SPLIT ? nq : 0];
+                x[0] = e.x; x[1] = e.y;
+                if (D == 3) x[D - 1] = e.z;
+                T = e.w; S = 0.0f; len = 0;
+                ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
                 dist = fabsf(gridValue<D>(sdf, x));
+                nSplits++;
             }
-            atomicAdd(&blk[code], 1ull);
-            if (code == ZG_ESCAPED_DOMAIN) continue;
-            float g = 0.0f;
-            if ((code == ZG_REACHED_ABSORBING_BOUNDARY || code == ZG_EXCEEDED_MAX_WALK_LENGTH) &&
-                !ws.ignore_absorbing_boundary_contribution) {
-                // projectToAbsorbingBoundary for an SdfGrid (sdf_grid_geometric_queries.h:254-259)
-                const float sd = gridValue<D>(sdf, x);
-                Vec<D> n;
-                sdfProject<D>(sdf, x, n);
-                g = evalField<D>(pde.dirichlet, x, n, ws.solve_double_sided && sd > 0.0f, sigma);
-            }
-            const float total = T * g + S;
+            if (!partOk) continue;
+            if (SPLIT && nSplits) atomicAdd(&P.accSplits[p], (unsigned long long)nSplits);
+            const float total = partSum;
             const double v = (double)total;
             atomicAdd(&P.accSum[p], v);
             atomicAdd(&P.accSq[p], v * v);
@@ -182,8 +219,6 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
                 atomicAdd(&P.accFirst[p], (double)fsc);
                 atomicAdd(&P.accDeriv[p], (double)dd);
             }
-            atomicAdd(&P.accLen[p], (unsigned long long)len);
-            atomicAdd(&blk[4], (unsigned long long)len);
         }
     }
     __syncthreads();
@@ -232,20 +267,23 @@ __global__ void __launch_bounds__(256) sdfInsideKernel(zg_function sdf, int64_t 
     if (i < n) inside[i] = gridValue<D>(sdf, vload<D>(pts + (size_t)i * D)) < 0.0f ? 1 : 0;
 }
 
-hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s) {
+template <bool GRAD>
+static hipError_t launchSdfWalks(const WosParams& P, int dim, bool screened, bool split, int gridBlocks, hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
-    if (dim == 2) hipLaunchKernelGGL((wosSdfKernel<2, false>), g, bl, 0, s, P);
-    else if (screened) hipLaunchKernelGGL((wosSdfKernel<3, true>), g, bl, 0, s, P);
-    else hipLaunchKernelGGL((wosSdfKernel<3, false>), g, bl, 0, s, P);
+#define ZG_SDF(Dd, S, SP) hipLaunchKernelGGL((wosSdfKernel<Dd, S, GRAD, SP>), g, bl, 0, s, P)
+    if (dim == 2) { if (split) ZG_SDF(2, false, true); else ZG_SDF(2, false, false); }
+    else if (screened) { if (split) ZG_SDF(3, true, true); else ZG_SDF(3, true, false); }
+    else { if (split) ZG_SDF(3, false, true); else ZG_SDF(3, false, false); }
+#undef ZG_SDF
     return hipGetLastError();
 }
 
-hipError_t launchWosSdfGrad(const WosParams& P, int dim, bool screened, int gridBlocks, hipStream_t s) {
-    dim3 g(gridBlocks), bl(256);
-    if (dim == 2) hipLaunchKernelGGL((wosSdfKernel<2, false, true>), g, bl, 0, s, P);
-    else if (screened) hipLaunchKernelGGL((wosSdfKernel<3, true, true>), g, bl, 0, s, P);
-    else hipLaunchKernelGGL((wosSdfKernel<3, false, true>), g, bl, 0, s, P);
-    return hipGetLastError();
+hipError_t launchWosSdf(const WosParams& P, int dim, bool screened, bool split, int gridBlocks, hipStream_t s) {
+    return launchSdfWalks<false>(P, dim, screened, split, gridBlocks, s);
+}
+
+hipError_t launchWosSdfGrad(const WosParams& P, int dim, bool screened, bool split, int gridBlocks, hipStream_t s) {
+    return launchSdfWalks<true>(P, dim, screened, split, gridBlocks, s);
 }
 
 hipError_t launchSdfQuery(const zg_function& sdf, int dim, int64_t n, const float* pts, float* dist, float* closest,
