@@ -243,6 +243,7 @@ zg_status zg_scene_create(int32_t dim, const float* V, int64_t nV, const int32_t
             s->refl.silConePrecision = prec;   // the cones' margin covers queries up to this precision
         } else if (dim == 3 && !selR.empty()) {
             buildSilhouettes3D(o.bvh_width, V, nV, I, selR, dbl != 0, prec, s->refl.host);
+            s->refl.silConePrecision = prec;
         }
     } catch (const std::exception& e) {
         delete s;

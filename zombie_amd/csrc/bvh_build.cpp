@@ -370,6 +370,76 @@ void buildSilhouettes2D(int width, const float* V, int64_t nV, const int32_t* I,
     else buildSilhouetteCones<8>(out, precision);
 }
 
+// Normal cones of the 3D silhouette-edge BVH: for every child subtree a cone around the
+// normalised sum of its candidates' triangle normals whose half-angle is the largest angle to any
+// of them plus the margin asin(precision) + 1e-4 (a conservative, not minimal, cone).  The
+// pruning rule of the 2D cones then holds on the sphere of directions: seen from x under view
+// directions whose angle to every normal stays away from 90 degrees by more than the margin, both
+// triangles of every edge face x, or both face away, by more than the precision -- no silhouette.
+// Subtrees with an open edge (always a silhouette) get a full cone.  Stored per child as two
+// float4: (axis.xyz, sin(half + margin)), (cos(half + margin), 0, 0, 0); (0, 0, 0, 1), (-1, ...)
+// never prunes.
+template <int W>
+static void buildSilhouetteCones3D(BuiltBoundary& out, float precision) {
+    using L = NodeLayout<3, W>;
+    const int nn = out.nSilNodes;
+    out.silCones.assign((size_t)nn * W * 8, 0.0f);
+    if (nn == 0) return;
+    const double margin = std::asin(std::min(1.0, (double)std::max(precision, 0.0f))) + 1e-4;
+    std::vector<int> rlo(nn, INT32_MAX), rhi(nn, -1);
+    auto childRange = [&](int c, int& lo, int& hi) {
+        if (c < 0) { lo = leafFirst(c); hi = lo + leafCount(c); } else { lo = rlo[c]; hi = rhi[c]; }
+    };
+    for (int i = nn - 1; i >= 0; i--) {
+        const int32_t* ch = reinterpret_cast<const int32_t*>(&out.silNodes[(size_t)i * L::kFloats + L::kChildOffset]);
+        const float* lo0 = &out.silNodes[(size_t)i * L::kFloats];
+        for (int k = 0; k < W; k++) {
+            if (lo0[k] >= 1e29f) continue;
+            int lo, hi;
+            childRange(ch[k], lo, hi);
+            if (hi <= lo) continue;
+            rlo[i] = std::min(rlo[i], lo); rhi[i] = std::max(rhi[i], hi);
+        }
+    }
+    for (int i = 0; i < nn; i++) {
+        const int32_t* ch = reinterpret_cast<const int32_t*>(&out.silNodes[(size_t)i * L::kFloats + L::kChildOffset]);
+        const float* lo0 = &out.silNodes[(size_t)i * L::kFloats];
+        for (int k = 0; k < W; k++) {
+            float* cone = &out.silCones[((size_t)i * W + k) * 8];
+            cone[0] = cone[1] = cone[2] = 0.0f; cone[3] = 1.0f; cone[4] = -1.0f;   // never prunes
+            if (lo0[k] >= 1e29f) continue;
+            int lo, hi;
+            childRange(ch[k], lo, hi);
+            if (hi <= lo) continue;
+            bool full = false;
+            double ax[3] = {0.0, 0.0, 0.0};
+            for (int c = lo; c < hi && !full; c++) {
+                const float* cd = &out.sil[(size_t)c * kSil3Floats];
+                if (cd[3] != 0.0f) { full = true; break; }
+                for (int a = 0; a < 3; a++) ax[a] += (double)cd[8 + a] + (double)cd[12 + a];
+            }
+            if (full) continue;
+            double l = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+            if (!(l > 1e-6)) continue;
+            for (int a = 0; a < 3; a++) ax[a] /= l;
+            double theta = 0.0;
+            for (int c = lo; c < hi; c++) {
+                const float* cd = &out.sil[(size_t)c * kSil3Floats];
+                for (int o : {8, 12}) {
+                    double dt = ax[0] * cd[o] + ax[1] * cd[o + 1] + ax[2] * cd[o + 2];
+                    double nl = std::sqrt((double)cd[o] * cd[o] + (double)cd[o + 1] * cd[o + 1] + (double)cd[o + 2] * cd[o + 2]);
+                    theta = std::max(theta, std::acos(std::min(1.0, std::max(-1.0, dt / std::max(nl, 1e-30)))));
+                }
+            }
+            double half = theta + margin + 1e-6;
+            if (half >= 0.5 * M_PI) continue;
+            cone[0] = (float)ax[0]; cone[1] = (float)ax[1]; cone[2] = (float)ax[2];
+            cone[3] = std::nextafter((float)std::sin(half), 2.0f);
+            cone[4] = std::nextafter((float)std::cos(half), -2.0f);
+        }
+    }
+}
+
 // 3D silhouette candidates of a reflecting triangle mesh: every edge with its (first two)
 // adjacent triangles; an edge with one triangle (an open edge) is always a silhouette.  With
 // one-sided problems an edge whose dihedral angle atan2(e . (n1 x n0), n0 . n1) -- e the edge
@@ -438,7 +508,8 @@ void buildSilhouettes3D(int width, const float* V, int64_t nV, const int32_t* I,
     out.sil.resize(cand.size());
     for (int i = 0; i < n; i++)
         std::memcpy(&out.sil[(size_t)i * kSil3Floats], &cand[(size_t)order[i] * kSil3Floats], kSil3Floats * 4);
-    out.silCones.clear();
+    if (width == 4) buildSilhouetteCones3D<4>(out, precision);
+    else buildSilhouetteCones3D<8>(out, precision);
 }
 
 }  // namespace zg

@@ -13,7 +13,7 @@ struct BuiltBoundary {
     std::vector<int32_t> primIds;  // caller primitive index per packed primitive
     std::vector<float> sil;        // 2D silhouette candidates, leaf order of silNodes
     std::vector<float> silNodes;   // wide BVH over the candidates
-    std::vector<float> silCones;   // nSilNodes * W float4: normal cone of each child subtree
+    std::vector<float> silCones;   // normal cone of each child subtree: nSilNodes * W float4 (2D), 2 float4 (3D)
     int nNodes = 0, nPrims = 0, depth = 0, nSil = 0, nSilNodes = 0;
 };
 

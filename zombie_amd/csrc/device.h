@@ -1322,10 +1322,31 @@ ZG_DEV bool isSilhouette3(const float4& c0, const float4& c2, const float4& c3, 
     return d0 * d1 < 0.0f;
 }
 
+// whether a 3D silhouette-BVH child may hold a silhouette seen from x: the rule of
+// coneMaySeeSilhouette on the sphere of directions (cone c0 = (axis.xyz, sin), c1.x = cos of
+// the widened half-angle; a negative cos never prunes)
+ZG_DEV bool coneMaySeeSilhouette3(const float4& c0, const float4& c1, const float* lo, const float* hi, const Vec<3>& x) {
+    if (c1.x < 0.0f) return true;
+    float ux = x[0] - 0.5f * (lo[0] + hi[0]), uy = x[1] - 0.5f * (lo[1] + hi[1]), uz = x[2] - 0.5f * (lo[2] + hi[2]);
+    float ex = 0.5f * (hi[0] - lo[0]), ey = 0.5f * (hi[1] - lo[1]), ez = 0.5f * (hi[2] - lo[2]);
+    float d2 = ux * ux + uy * uy + uz * uz;
+    float r2 = ex * ex + ey * ey + ez * ez;
+    if (d2 <= r2 * 1.0001f) return true;
+    float invd = 1.0f / __builtin_sqrtf(d2);
+    float sphi = __builtin_sqrtf(r2) * invd * 1.0001f;
+    float cphi = __builtin_sqrtf(fmaxf(0.0f, 1.0f - sphi * sphi));
+    float sinSum = c0.w * cphi + c1.x * sphi;
+    float cosSum = c1.x * cphi - c0.w * sphi;
+    if (cosSum <= 0.0f) return true;
+    float calpha = (c0.x * ux + c0.y * uy + c0.z * uz) * invd;
+    return fabsf(calpha) <= sinSum * 1.0001f + 1e-6f;
+}
+
 // computeStarRadiusForReflectingBoundary in 3D: distance to the closest point of the nearest
 // silhouette edge within maxRadius, clamped below by minRadius (maxRadius clamped below by
-// minRadius when there is none).  Nearest-first traversal of the candidate-edge BVH, stopped
-// once a silhouette closer than minRadius is found.
+// minRadius when there is none).  Nearest-first traversal of the candidate-edge BVH, pruned by
+// box distance and by the children's normal cones, stopped once a silhouette closer than
+// minRadius is found.
 template <int W>
 ZG_DEV float starRadius3(const BoundaryView& b, const Vec<3>& x, float minRadius, float maxRadius, float precision,
                          bool flip, Stack& st) {
@@ -1333,15 +1354,27 @@ ZG_DEV float starRadius3(const BoundaryView& b, const Vec<3>& x, float minRadius
     if (b.nSilNodes == 0) return fmaxf(maxRadius, minRadius);
     float best2 = maxRadius < FLT_MAX ? maxRadius * maxRadius : FLT_MAX;
     const float min2 = minRadius * minRadius;
+    const bool cones = b.silCones != nullptr && precision <= b.silConePrecision;
     bool found = false;
     st.sp = 0; st.ovf = 0;
     int node = 0;
     for (;;) {
         WideNode<3, W> nd;
         nd.load(b.silNodes, node);
+        const float4* cn = reinterpret_cast<const float4*>(b.silCones) + (size_t)node * W * 2;
+        bool keep[W];
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            if (!isLeaf(nd.ch[k]) || nd.dist2(k, x) > best2) continue;
+            keep[k] = nd.dist2(k, x) <= best2;
+            if (cones && keep[k]) {
+                const float lo[3] = {nd.lo[0][k], nd.lo[1][k], nd.lo[2][k]};
+                const float hi[3] = {nd.hi[0][k], nd.hi[1][k], nd.hi[2][k]};
+                keep[k] = coneMaySeeSilhouette3(cn[2 * k], cn[2 * k + 1], lo, hi, x);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (!keep[k] || !isLeaf(nd.ch[k])) continue;
             int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
             for (int i = first; i < first + cnt; i++) {
                 const float4* c = reinterpret_cast<const float4*>(b.sil) + 4 * i;
@@ -1358,12 +1391,12 @@ ZG_DEV float starRadius3(const BoundaryView& b, const Vec<3>& x, float minRadius
 #pragma unroll
         for (int k = 0; k < W; k++) {
             float dk = nd.dist2(k, x);
-            if (!isLeaf(nd.ch[k]) && dk <= best2 && (next < 0 || dk < ndist)) { next = nd.ch[k]; ndist = dk; nk = k; }
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && (next < 0 || dk < ndist)) { next = nd.ch[k]; ndist = dk; nk = k; }
         }
 #pragma unroll
         for (int k = 0; k < W; k++) {
             float dk = nd.dist2(k, x);
-            if (!isLeaf(nd.ch[k]) && dk <= best2 && k != nk) st.push(nd.ch[k], dk);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && k != nk) st.push(nd.ch[k], dk);
         }
         if (next < 0) {
             int n; float dn;

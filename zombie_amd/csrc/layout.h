@@ -50,7 +50,8 @@ struct BoundaryView {
     const int32_t* primIds;// 2D: caller index per packed primitive
     const float* sil;      // reflecting: silhouette candidates (kSilFloats each in 2D, kSil3Floats in 3D)
     const float* silNodes; // reflecting: wide BVH over the silhouette candidates
-    const float* silCones; // per silhouette-BVH child: normal cone (axis.xy, sin, cos of half-angle + margin)
+    const float* silCones; // per silhouette-BVH child: normal cone (2D: axis.xy, sin, cos of half-angle + margin;
+                           // 3D: axis.xyz, sin; cos, 0, 0, 0)
     float silConePrecision;// the silhouette precision the cones' margin covers
     const float* robinMax; // 2D reflecting: largest Robin coefficient per packed primitive (null: 0)
     int32_t nNodes;
