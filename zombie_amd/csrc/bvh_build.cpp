@@ -377,8 +377,12 @@ void buildSilhouettes2D(int width, const float* V, int64_t nV, const int32_t* I,
 // directions whose angle to every normal stays away from 90 degrees by more than the margin, both
 // triangles of every edge face x, or both face away, by more than the precision -- no silhouette.
 // Subtrees with an open edge (always a silhouette) get a full cone.  Stored per child as two
-// float4: (axis.xyz, sin(half + margin)), (cos(half + margin), 0, 0, 0); (0, 0, 0, 1), (-1, ...)
-// never prunes.
+// float4: (axis.xyz, sin(half + margin)), (cos(half + margin), plane offset, flat, tolerance);
+// (0, 0, 0, 1), (-1, ...) never prunes.  A child whose triangles are coplanar (every normal
+// within 1e-5 rad of the axis, every edge end point within 1e-6 (1 + |p|) of the plane
+// axis . y = offset) is flagged flat: from x at distance h of that plane every view direction
+// v to a point at distance r has |n . v| = h / r up to the tolerance, which lets the query skip
+// the child when all its points lie beyond the precision band or all inside it.
 template <int W>
 static void buildSilhouetteCones3D(BuiltBoundary& out, float precision) {
     using L = NodeLayout<3, W>;
@@ -436,6 +440,25 @@ static void buildSilhouetteCones3D(BuiltBoundary& out, float precision) {
             cone[0] = (float)ax[0]; cone[1] = (float)ax[1]; cone[2] = (float)ax[2];
             cone[3] = std::nextafter((float)std::sin(half), 2.0f);
             cone[4] = std::nextafter((float)std::cos(half), -2.0f);
+            // coplanar subtree: plane through the first end point, every end point within tolerance
+            if (theta < 1e-5) {
+                const float* c0 = &out.sil[(size_t)lo * kSil3Floats];
+                double off = ax[0] * c0[0] + ax[1] * c0[1] + ax[2] * c0[2];
+                double dev = 0.0, mag = 0.0;
+                for (int c = lo; c < hi; c++) {
+                    const float* cd = &out.sil[(size_t)c * kSil3Floats];
+                    for (int o : {0, 4}) {
+                        dev = std::max(dev, std::fabs(ax[0] * cd[o] + ax[1] * cd[o + 1] + ax[2] * cd[o + 2] - off));
+                        mag = std::max(mag, std::fabs((double)cd[o]) + std::fabs((double)cd[o + 1]) + std::fabs((double)cd[o + 2]));
+                    }
+                }
+                if (dev <= 1e-6 * (1.0 + mag)) {
+                    cone[5] = (float)off;
+                    cone[6] = 1.0f;
+                    // plane-distance error (end points, float rounding of x . n) + normal spread
+                    cone[7] = (float)(2.0 * dev + 4e-7 * (1.0 + mag) + 1e-7);
+                }
+            }
         }
     }
 }

@@ -1324,9 +1324,27 @@ ZG_DEV bool isSilhouette3(const float4& c0, const float4& c2, const float4& c3, 
 
 // whether a 3D silhouette-BVH child may hold a silhouette seen from x: the rule of
 // coneMaySeeSilhouette on the sphere of directions (cone c0 = (axis.xyz, sin), c1.x = cos of
-// the widened half-angle; a negative cos never prunes)
-ZG_DEV bool coneMaySeeSilhouette3(const float4& c0, const float4& c1, const float* lo, const float* hi, const Vec<3>& x) {
+// the widened half-angle; a negative cos never prunes).  A coplanar child (c1.z = 1, plane
+// axis . y = c1.y within tolerance c1.w, normals within 1e-5 rad of the axis) is also skipped
+// when every point y of its box sees x with |n . v| = h / |x - y| beyond precision + 1e-4 (both
+// triangles of each edge face x the same way) or within precision - 1e-4 (the tie rule then
+// reports no silhouette), h the distance of x to the plane.
+ZG_DEV bool coneMaySeeSilhouette3(const float4& c0, const float4& c1, const float* lo, const float* hi, const Vec<3>& x,
+                                  float precision) {
     if (c1.x < 0.0f) return true;
+    if (c1.z != 0.0f) {
+        const float h = fabsf(c0.x * x[0] + c0.y * x[1] + c0.z * x[2] - c1.y);
+        float n2 = 0.0f, f2 = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float dn = fmaxf(fmaxf(lo[a] - x[a], x[a] - hi[a]), 0.0f);
+            const float df = fmaxf(fabsf(x[a] - lo[a]), fabsf(x[a] - hi[a]));
+            n2 += dn * dn; f2 += df * df;
+        }
+        const float nearD = __builtin_sqrtf(n2) * 0.9999f, farD = __builtin_sqrtf(f2) * 1.0001f;
+        if (h - c1.w > (precision + 1e-4f) * farD) return false;
+        if (nearD > 0.0f && h + c1.w < (precision - 1e-4f) * nearD) return false;
+    }
     float ux = x[0] - 0.5f * (lo[0] + hi[0]), uy = x[1] - 0.5f * (lo[1] + hi[1]), uz = x[2] - 0.5f * (lo[2] + hi[2]);
     float ex = 0.5f * (hi[0] - lo[0]), ey = 0.5f * (hi[1] - lo[1]), ez = 0.5f * (hi[2] - lo[2]);
     float d2 = ux * ux + uy * uy + uz * uz;
@@ -1369,7 +1387,7 @@ ZG_DEV float starRadius3(const BoundaryView& b, const Vec<3>& x, float minRadius
             if (cones && keep[k]) {
                 const float lo[3] = {nd.lo[0][k], nd.lo[1][k], nd.lo[2][k]};
                 const float hi[3] = {nd.hi[0][k], nd.hi[1][k], nd.hi[2][k]};
-                keep[k] = coneMaySeeSilhouette3(cn[2 * k], cn[2 * k + 1], lo, hi, x);
+                keep[k] = coneMaySeeSilhouette3(cn[2 * k], cn[2 * k + 1], lo, hi, x, precision);
             }
         }
 #pragma unroll
