@@ -69,3 +69,20 @@ def test_ctypes_layouts_match_c(tmp_path):
         assert int(got[s]) == C.sizeof(cls), s
         for f, _ in cls._fields_:
             assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, f"{s}.{f}"
+
+
+def test_points_dimension_check():
+    """the solve wrappers hand plain pointers to the C-ABI: points, normals and derivative
+    directions of the wrong width are refused on the host before any call"""
+    import torch
+    from zombie_amd import engine as E
+    n = 5
+    ok = E.DevicePoints(pt=torch.zeros(n, 3), dist_to_absorbing=torch.ones(n), n_walks=torch.ones(n, dtype=torch.int32),
+                        n_walks_max=1, normal=torch.zeros(n, 3))
+    ok.check_dim(3)
+    with pytest.raises(ValueError):
+        ok.check_dim(2)
+    bad = E.DevicePoints(pt=torch.zeros(n, 3), dist_to_absorbing=torch.ones(n), n_walks=torch.ones(n, dtype=torch.int32),
+                         n_walks_max=1, direction_for_derivative=torch.zeros(n, 2))
+    with pytest.raises(ValueError):
+        bad.check_dim(3)
