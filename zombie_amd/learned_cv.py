@@ -1,7 +1,7 @@
 """Learned control variate: a Fourier-feature MLP c(x) trained on the GPU from the solver's own
 walk estimates; its input gradient gives the zero-mean first-step control variate
 -grad c(x0).(x1 - x0) of every walk (x1 uniform on the first sphere, so the estimator stays
-unbiased whatever c is).  Plain walk on spheres is the fallback and the judge of its error.
+unbiased whatever c is).  Plain walk on spheres is the fallback and the yardstick its error is measured against.
 
 Network (defaults): phi(x) = [sin(B x), cos(B x)] with B a 32 x d matrix of N(0, 4^2) entries
 (64 features), 4 hidden layers of 128 SiLU units, 1 linear output; bf16 GEMM operands on MFMA
