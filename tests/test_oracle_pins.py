@@ -4,19 +4,25 @@ PCG32: O'Neill's published pcg32-demo output (seed 42, stream 54).  Samplers and
 functions: distributions and integrals computed here independently (quadrature, KS tests),
 not retyped formulas.
 """
+import json
 import math
+import os
 
 import numpy as np
 import pytest
 from scipy import integrate, stats
 
-# pcg32_srandom_r(&rng, 42u, 54u); first outputs of pcg32_random_r (pcg-c-basic pcg32-demo)
-PCG32_42_54 = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def test_pcg32_known_answers(oracle_lib):
-    out = oracle_lib.pcg32_u32(42, 54, 6)
-    assert [int(v) for v in out] == PCG32_42_54
+    # pcg32_srandom_r(&rng, 42u, 54u); first outputs of pcg32_random_r (pcg-c-basic pcg32-demo)
+    with open(os.path.join(GOLDEN, "pcg32_demo_42_54.json")) as f:
+        kat = json.load(f)
+    want = [int(v, 16) for v in kat["u32"]]
+    assert want[0] == 0xA15C02B7 and len(want) == 6
+    out = oracle_lib.pcg32_u32(kat["seed"], kat["stream"], len(want))
+    assert [int(v) for v in out] == want
 
 
 def test_pcg32_float_is_mantissa_construction(oracle_lib):
