@@ -23,9 +23,16 @@
  *    with Chan's parallel merge (point_estimation/common.h:268-310): k calls of W walks
  *    estimate the same quantity as one call of k*W walks.
  *  - Random numbers: PCG32 (O'Neill, PCG-XSH-RR 64/32).  Walk w of point p in a call uses
- *    the generator pcg32(initstate = seed, initseq = (p << 32) | (stream_offset + w)),
- *    so results are reproducible and independent of the launch shape.  The reference seeds
- *    one generator per point from the clock (core/sampling.h:36-41); parity is statistical.
+ *    PCG32 stream p (initseq = p) started at initstate = mix(seed, p, stream_offset + w),
+ *    where mix is a chain of three SplitMix64 finalisers:
+ *      h = f(seed ^ 0x9e3779b97f4a7c15); h = f(h ^ p * 0xd6e8feb86659fd93);
+ *      initstate = f(h ^ (stream_offset + w) * 0xa0761d6478bd642f)   (64-bit wrap-around,
+ *    the walk index taken modulo 2^32), f(z) = SplitMix64's output function.  The walks of a
+ *    point therefore start at distinct, unrelated positions of the point's stream, results
+ *    are reproducible and independent of the launch shape, and k calls with advancing
+ *    stream_offset equal one long call.  The reference seeds one generator per point from
+ *    the clock and a stream counter (core/sampling.h:36-41) and draws its walks from it in
+ *    turn; parity is statistical.
  */
 #ifndef ZOMBIE_AMD_H
 #define ZOMBIE_AMD_H
@@ -381,7 +388,8 @@ zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
 
 /* The per-point Latin hypercube tables of a gradient solve (generateStratifiedSamples<DIM-1>,
  * core/sampling.h:435-458): table t holds n_samples rows of `dims` floats and is drawn from
- * pcg32(~seed, (t << 32) | stream_offset).  A solve with seed s and stream offset o uses the
+ * the generator of walk stream_offset of point t under the seed ~seed (see Random numbers
+ * above).  A solve with seed s and stream offset o uses the
  * table of point p with 2 * n_pairs rows (n_pairs = n_walks / 2 with antithetic variates, else
  * n_walks).  out: device array of n_tables * n_samples * dims floats. */
 zg_status zg_stratified_samples(int32_t dims, int64_t n_tables, int32_t n_samples, uint64_t seed,

@@ -19,6 +19,7 @@ _lib = None
 
 RNG_PER_POINT = 0   # the reference's scheme: one generator per point, consumed across walks
 RNG_PER_WALK = 1    # the GPU's scheme: one generator per walk
+RNG_LOCKSTEP = 2    # counter-example only: per-walk increments on one initstate (correlated walks)
 
 
 def lib():
@@ -30,6 +31,11 @@ def lib():
         P = C.c_void_p
         L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, P]
         L.orc_pcg32_f32.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, P]
+        L.orc_splitmix64_mix.argtypes = [C.c_uint64]
+        L.orc_splitmix64_mix.restype = C.c_uint64
+        L.orc_walk_init_state.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_walk_init_state.restype = C.c_uint64
+        L.orc_walk_floats.argtypes = [C.c_uint64, P, P, C.c_int64, C.c_int32, P]
         L.orc_pcg32_default_u32.argtypes = [C.c_int64, P]
         L.orc_sample_unit_sphere.argtypes = [C.c_int, C.c_uint64, C.c_int64, P]
         L.orc_sample_ball_radius.argtypes = [C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int64, P, P]
@@ -96,6 +102,24 @@ def pcg32_u32(initstate, initseq, n):
 def pcg32_f32(initstate, initseq, n):
     out = np.empty(n, np.float32)
     lib().orc_pcg32_f32(initstate, initseq, n, _p(out))
+    return out
+
+
+def splitmix64_mix(z):
+    return int(lib().orc_splitmix64_mix(z & (2 ** 64 - 1)))
+
+
+def walk_init_state(seed, p, w):
+    """initstate of the generator of walk w (stream offset included) of point p; its stream is p"""
+    return int(lib().orc_walk_init_state(seed & (2 ** 64 - 1), p, w & (2 ** 32 - 1)))
+
+
+def walk_floats(seed, p, w, n_draws):
+    """[len(p), n_draws] first nextFloat() values of the generators of walks (p[i], w[i])"""
+    p = np.ascontiguousarray(p, np.uint32)
+    w = np.ascontiguousarray(w, np.uint32)
+    out = np.empty((len(p), n_draws), np.float32)
+    lib().orc_walk_floats(seed & (2 ** 64 - 1), _p(p), _p(w), len(p), n_draws, _p(out))
     return out
 
 

@@ -102,6 +102,23 @@ struct Pcg32 {
     }
 };
 
+// SplitMix64's output finaliser (Steele, Lea & Flood, OOPSLA 2014; constants of Vigna's
+// splitmix64.c), a bijection of 64-bit words.
+inline uint64_t splitmix64Mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+// Generator of walk w (counted from the call's stream offset) of point p: PCG32 stream p from
+// the state mix(seed, p, w).  Three bijective rounds: for a fixed seed and point, distinct
+// walks start at distinct, unrelated positions of the stream's cycle.
+inline uint64_t walkInitState(uint64_t seed, uint32_t p, uint32_t w) {
+    uint64_t h = splitmix64Mix(seed ^ 0x9e3779b97f4a7c15ULL);
+    h = splitmix64Mix(h ^ ((uint64_t)p * 0xd6e8feb86659fd93ULL));
+    return splitmix64Mix(h ^ ((uint64_t)w * 0xa0761d6478bd642fULL));
+}
+inline Pcg32 walkRng(uint64_t seed, uint32_t p, uint32_t w) { return Pcg32(walkInitState(seed, p, w), (uint64_t)p); }
+
 // ------------------------------------------------------------------------------------
 // small fixed-size vectors
 // ------------------------------------------------------------------------------------
@@ -1432,9 +1449,13 @@ float wosTerminal(const Scene<D>& sc, const SolveCtx& cx, Code code, WalkState<D
     return 0.0f;
 }
 
+// rng_mode 1: walk w's own generator; rng_mode 0: the point's one generator (that of its walk 0).
+// rng_mode 2 is a deliberately flawed per-walk scheme kept as a counter-example for the
+// independence tests: one shared initstate on increments 2((p << 32) | w) + 1, under which the
+// states of a point's walks lie on an arithmetic progression in w and their draws correlate.
 inline Pcg32 makeRng(const SolveCtx& cx, int64_t p, int w) {
-    uint64_t seq = ((uint64_t)p << 32) | (uint64_t)(uint32_t)(cx.streamOffset + (cx.rngMode == 1 ? (uint32_t)w : 0u));
-    return Pcg32(cx.seed, seq);
+    if (cx.rngMode == 2) return Pcg32(cx.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(cx.streamOffset + (uint32_t)w));
+    return walkRng(cx.seed, (uint32_t)p, cx.streamOffset + (cx.rngMode == 1 ? (uint32_t)w : 0u));
 }
 
 // estimateSolution (walk_on_spheres.h:320-414)
@@ -1460,8 +1481,8 @@ void wosEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>& 
     Pcg32 pointRng = makeRng(cx, p, 0);
     std::deque<WalkState<D>> q;
     for (int w = 0; w < nWalks; w++) {
-        Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
-        Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
+        Pcg32 walkRng = cx.rngMode != 0 ? makeRng(cx, p, w) : Pcg32();
+        Pcg32& rng = cx.rngMode != 0 ? walkRng : pointRng;
         WalkState<D> st0; st0.currentPt = pt; st0.currentNormal = V<D>::zero(); st0.prevDirection = V<D>::zero();
         q.push_back(st0);
         int splits = -1; float total = 0.0f; bool success = false;
@@ -1559,8 +1580,8 @@ float sdfTerminal(const SdfGeom<D>& g, const SolveCtx& cx, Code code, const Walk
 // walk from the boundary point).  rng_mode 0 consumes one generator
 // per point in the reference's order (stratified table, then every walk; the antithetic half
 // replays the generator saved after the first step).  rng_mode 1 is the GPU's scheme: the table
-// comes from pcg32(~seed, (p << 32) | stream_offset) and both halves of pair w start from
-// pcg32(seed, (p << 32) | (stream_offset + w)), the second half redrawing the first step before
+// comes from walkRng(~seed, p, stream_offset) and both halves of pair w start from
+// walkRng(seed, p, stream_offset + w), the second half redrawing the first step before
 // mirroring it (the same numbers, so the same walk continuation as the reference's replay).
 template <int D>
 void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t p, const V<D>& pt, float distA,
@@ -1574,7 +1595,7 @@ void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t 
     if (firstSphereRadius) *firstSphereRadius = R0;
     Pcg32 pointRng = makeRng(cx, p, 0);
     std::vector<float> strat;
-    if (cx.rngMode == 1) { Pcg32 t(~cx.seed, ((uint64_t)p << 32) | (uint64_t)cx.streamOffset); stratifiedSamples(D - 1, 2 * nWalks, t, strat); }
+    if (cx.rngMode != 0) { Pcg32 t = walkRng(~cx.seed, (uint32_t)p, cx.streamOffset); stratifiedSamples(D - 1, 2 * nWalks, t, strat); }
     else stratifiedSamples(D - 1, 2 * nWalks, pointRng, strat);
     std::deque<WalkState<D>> q;
     for (int w = 0; w < nWalks; w++) {
@@ -1584,10 +1605,10 @@ void wosEstimateGrad(const Scene<D>& sc, const SolveCtx& cx, GradCv cv, int64_t 
         float cvB = 0.0f, cvS = 0.0f;
         if (ws.use_gradient_control_variates) { cv.atPair(w, out, g.firstSource); cvB = cv.b; cvS = cv.s; }
         for (int half = 0; half < nHalves; half++) {
-            Pcg32 pairRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
-            Pcg32& rng = cx.rngMode == 1 ? pairRng : pointRng;
+            Pcg32 pairRng = cx.rngMode != 0 ? makeRng(cx, p, w) : Pcg32();
+            Pcg32& rng = cx.rngMode != 0 ? pairRng : pointRng;
             // the first step: GPU scheme redraws it in the second half, then mirrors
-            bool draw = half == 0 || cx.rngMode == 1;
+            bool draw = half == 0 || cx.rngMode != 0;
             WalkState<D> st; st.currentPt = pt; st.currentNormal = V<D>::zero(); st.prevDirection = V<D>::zero();
             GreensBall<D> gf;
             if (pde.absorption_coeff > 0.0f && ws.steps_before_applying_tikhonov == 0) gf.makeYukawa(pde.absorption_coeff);
@@ -1869,8 +1890,8 @@ void wostEstimate(const Scene<D>& sc, const SolveCtx& cx, int64_t p, const V<D>&
     Pcg32 pointRng = makeRng(cx, p, 0);
     std::deque<WalkState<D>> q;
     for (int w = 0; w < nWalks; w++) {
-        Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
-        Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
+        Pcg32 walkRng = cx.rngMode != 0 ? makeRng(cx, p, w) : Pcg32();
+        Pcg32& rng = cx.rngMode != 0 ? walkRng : pointRng;
         WalkState<D> st0; st0.currentPt = pt; st0.currentNormal = curN; st0.prevDirection = prevDir;
         st0.prevDistance = prevDist; st0.onReflectingBoundary = type == ZG_SAMPLE_ON_REFLECTING_BOUNDARY;
         q.push_back(st0);
@@ -2065,8 +2086,8 @@ void wosEstimateSdf(const SdfGeom<D>& g, const SolveCtx& cx, int64_t p, const V<
     }
     Pcg32 pointRng = makeRng(cx, p, 0);
     for (int w = 0; w < nWalks; w++) {
-        Pcg32 walkRng = cx.rngMode == 1 ? makeRng(cx, p, w) : Pcg32();
-        Pcg32& rng = cx.rngMode == 1 ? walkRng : pointRng;
+        Pcg32 walkRng = cx.rngMode != 0 ? makeRng(cx, p, w) : Pcg32();
+        Pcg32& rng = cx.rngMode != 0 ? walkRng : pointRng;
         // the walk and its split parts (applyWeightWindow queues them); the estimate is their sum
         std::deque<WalkState<D>> q;
         WalkState<D> st0; st0.currentPt = pt;
@@ -2301,6 +2322,15 @@ extern "C" {
 void orc_pcg32_u32(uint64_t initstate, uint64_t initseq, int64_t n, uint32_t* out) {
     Pcg32 r(initstate, initseq);
     for (int64_t i = 0; i < n; i++) out[i] = r.nextUInt();
+}
+uint64_t orc_splitmix64_mix(uint64_t z) { return splitmix64Mix(z); }
+uint64_t orc_walk_init_state(uint64_t seed, uint32_t p, uint32_t w) { return walkInitState(seed, p, w); }
+// the first n_draws nextFloat() values of the generators of walks (p[i], w[i]), row i of out
+void orc_walk_floats(uint64_t seed, const uint32_t* p, const uint32_t* w, int64_t n, int32_t nDraws, float* out) {
+    for (int64_t i = 0; i < n; i++) {
+        Pcg32 r = walkRng(seed, p[i], w[i]);
+        for (int32_t k = 0; k < nDraws; k++) out[i * nDraws + k] = r.nextFloat();
+    }
 }
 void orc_pcg32_f32(uint64_t initstate, uint64_t initseq, int64_t n, float* out) {
     Pcg32 r(initstate, initseq);
@@ -2649,7 +2679,8 @@ int orc_line_of_sight_bvh(void* hv, const float* p1, const float* p2) {
 }
 
 // Solves.  rng_mode 0: one generator per point consumed across its walks (the reference's
-// scheme, seeded deterministically); 1: one generator per walk (the GPU's scheme).
+// scheme, seeded deterministically: the generator of the point's walk 0); 1: one generator per
+// walk, walkRng(seed, p, stream_offset + w) (the GPU's scheme).
 // Stats arrays are accumulated with Chan's merge.  report: [codes0..3, totalSteps, nodeVisits, primVisits]
 
 int orc_solve(int wost, void* hv, const zg_pde* pde, const zg_walk_settings* ws, int64_t n, const float* pts,
@@ -2677,11 +2708,11 @@ int orc_solve_grad(int wost, void* hv, const zg_pde* pde, const zg_walk_settings
 }
 
 // the per-point Latin hypercube tables of a gradient solve: table t (2 * n_pairs rows of
-// `dims` coordinates) is drawn from pcg32(~seed, (t << 32) | stream_offset)
+// `dims` coordinates) is drawn from the generator of walk stream_offset of point t under ~seed
 void orc_stratified_samples(int dims, int64_t nTables, int nSamples, uint64_t seed, uint32_t streamOffset, float* out) {
     std::vector<float> s;
     for (int64_t t = 0; t < nTables; t++) {
-        Pcg32 r(~seed, ((uint64_t)t << 32) | (uint64_t)streamOffset);
+        Pcg32 r = walkRng(~seed, (uint32_t)t, streamOffset);
         stratifiedSamples(dims, nSamples, r, s);
         std::memcpy(out + (size_t)t * dims * nSamples, s.data(), s.size() * sizeof(float));
     }

@@ -86,3 +86,73 @@ def test_points_dimension_check():
                          n_walks_max=1, direction_for_derivative=torch.zeros(n, 2))
     with pytest.raises(ValueError):
         bad.check_dim(3)
+
+
+def _points(n, dim=3, **kw):
+    import torch
+    from zombie_amd import engine as E
+    a = dict(pt=torch.zeros(n, dim), dist_to_absorbing=torch.ones(n), n_walks=torch.ones(n, dtype=torch.int32),
+             n_walks_max=1)
+    a.update(kw)
+    return E.DevicePoints(**a)
+
+
+@pytest.mark.parametrize("field,value,err", [
+    ("n_walks", "int64", TypeError),             # torch.full's default integer type: wrong walk counts
+    ("pt", "float64", TypeError),                # 8-byte coordinates misread as floats
+    ("pt", "strided", ValueError),               # a sliced view: the kernel reads the wrong values
+    ("dist_to_absorbing", "short", ValueError),  # shorter than n: reads past its end
+    ("n_walks", "short", ValueError),
+    ("type", "int32", TypeError),
+    ("first_sphere_radius", "short", ValueError),
+    ("n_walks", "too_many", ValueError),         # a point asks for more walks than n_walks_max
+])
+def test_points_validation_refuses_mismatched_arrays(field, value, err):
+    """Every array the solve wrappers pass to the C-ABI is checked on the host: element type,
+    length, contiguity (these checks come before the device check, so they run on the CPU)."""
+    import torch
+    n = 6
+    bad = {
+        "int64": torch.ones(n, dtype=torch.int64),
+        "float64": torch.zeros(n, 3, dtype=torch.float64),
+        "strided": torch.zeros(n, 6)[:, ::2],
+        "short": torch.ones(n - 1, dtype=torch.int32 if field == "n_walks" else torch.float32),
+        "int32": torch.zeros(n, dtype=torch.int32),
+        "too_many": torch.full((n,), 2, dtype=torch.int32),
+    }[value]
+    p = _points(n, **{field: bad})
+    with pytest.raises(err):
+        p.validate(3)
+
+
+def test_points_validation_requires_a_gpu_device():
+    p = _points(4)
+    with pytest.raises(ValueError, match="GPU"):
+        p.validate(3)
+
+
+@pytest.mark.parametrize("field,kind", [("n", "float"), ("mean", "short"), ("m2", "float32"),
+                                        ("walk_length", "int32"), ("grad_mean", "shape")])
+def test_stats_validation_refuses_mismatched_arrays(field, kind):
+    import torch
+    from zombie_amd import engine as E
+    n = 5
+    st = E.DeviceStats.zeros(n, device="cpu", dim=3)
+    st.validate(n, 3, torch.device("cpu"), need_gradient=True)
+    cur = getattr(st, field)
+    bad = {"float": cur.to(torch.float64), "short": cur[:-1].clone(), "float32": cur.to(torch.float32),
+           "int32": cur.to(torch.int32), "shape": torch.zeros(n, 2, dtype=torch.float64)}[kind]
+    setattr(st, field, bad)
+    with pytest.raises((TypeError, ValueError)):
+        st.validate(n, 3, torch.device("cpu"), need_gradient=True)
+
+
+def test_stats_validation_needs_gradient_members_for_gradient_points():
+    import torch
+    from zombie_amd import engine as E
+    st = E.DeviceStats.zeros(4, device="cpu")
+    st.validate(4, 3, torch.device("cpu"))
+    with pytest.raises(ValueError, match="gradient"):
+        st.validate(4, 3, torch.device("cpu"), need_gradient=True)
+    with pytest.raises(ValueError, match="on"):
+        E.DeviceStats.zeros(4, device="cpu").validate(4, 3, torch.device("meta"))

@@ -63,7 +63,7 @@ def test_stratified_table_known_answer_two_rows(oracle_lib):
     O = oracle_lib
     seed, off = 0xBEEF, 3
     for t in range(6):
-        init, seq = (~seed) & (2 ** 64 - 1), (t << 32) | off
+        init, seq = O.walk_init_state(~seed, t, off), t
         raw = O.pcg32_u32(init, seq, 3)
         f = ((raw[:2] >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1.0)
         rows = [np.float32(np.float32(0.0) + f[0]) * np.float32(0.5), np.float32(np.float32(1.0) + f[1]) * np.float32(0.5)]

@@ -46,10 +46,11 @@ zg_status ok() { g_lastError.clear(); return ZG_OK; }
     } while (0)
 
 int cuCount(int device) {
+    constexpr int kMaxDevices = 64;
     static std::mutex m;
-    static std::vector<int> cache;
+    static int cache[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return 256;
     std::lock_guard<std::mutex> g(m);
-    if ((int)cache.size() <= device) cache.resize(device + 1, 0);
     if (cache[device] == 0) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) cache[device] = prop.multiProcessorCount;

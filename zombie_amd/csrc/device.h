@@ -68,6 +68,30 @@ struct Pcg32 {
     }
 };
 
+// SplitMix64's output finaliser (Steele, Lea & Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014; the constants of Vigna's splitmix64.c): a bijection of 64-bit words
+// with full avalanche.
+ZG_DEV uint64_t splitmix64Mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// The generator of walk w (counted from the call's stream offset) of point p: PCG32 stream p,
+// started at the state mix(seed, p, w).  Each mixing round is a bijection, so for a fixed seed
+// and point the walks start at distinct, unrelated positions of the stream's 2^64-long cycle
+// and never overlap in practice (a walk takes at most a few thousand draws).  Walks that
+// shared one initstate on increments 2(p << 32 | w) + 1 would instead move in lockstep: their
+// states at any draw index lie on an arithmetic progression in w, and their outputs correlate.
+ZG_DEV uint64_t walkInitState(uint64_t seed, uint32_t p, uint32_t w) {
+    uint64_t h = splitmix64Mix(seed ^ 0x9e3779b97f4a7c15ULL);
+    h = splitmix64Mix(h ^ ((uint64_t)p * 0xd6e8feb86659fd93ULL));
+    return splitmix64Mix(h ^ ((uint64_t)w * 0xa0761d6478bd642fULL));
+}
+ZG_DEV void seedWalk(Pcg32& rng, uint64_t seed, uint32_t p, uint32_t w) {
+    rng.seed(walkInitState(seed, p, w), (uint64_t)p);
+}
+
 // ------------------------------------------------------------------------------------
 // SphereSampler<DIM>::sampleUnitSphereUniform (core/sampling.h:104-111,164-174).
 // v_sin_f32 / v_cos_f32 take their argument in revolutions: sin(2*pi*u) = v_sin(u).

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
                 const int pair = P.gradHalves == 2 ? (w >> 1) : w;
                 const bool mirror = P.gradHalves == 2 && (w & 1);
                 const Vec<D> c0 = x;
-                rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
+                seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)pair);
                 ball.update(0.99f * dist);
                 const float* tab = P.strat + (size_t)p * (size_t)P.stratStride + (size_t)(D - 1) * 2 * (size_t)pair;
                 if (!ws.ignore_source_contribution) {
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) wosSdfKernel(WosParams P) {
                 }
                 dist = fabsf(gridValue<D>(sdf, x));
             } else {
-                rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+                seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)w);
             }
             atomicAdd(&blk[5], 1ull);
             // splitting: parts of the walk still to run (position, throughput) and the sum over its

@@ -159,7 +159,7 @@ ZG_DEV Vec<D> cvField(const WosParams& P, const Vec<D>& x) {
 // for it: walk w of a point is half (w & 1) of antithetic pair w >> 1 (or pair w without
 // antithetic variates).  The first step is explicit: a ball of radius 0.99 * d(x0), a source
 // sample and a boundary point from the point's stratified table (directions) and the pair's
-// generator pcg32(seed, (p << 32) | (stream_offset + pair)) (source radius, cosine-sampling
+// generator seedWalk(seed, p, stream_offset + pair) (source radius, cosine-sampling
 // sign), mirrored through x0 for the second half, which therefore continues with exactly the
 // random numbers of the first (the reference replays its saved generator).  The walk then goes
 // on from the boundary point as a plain walk; its estimate feeds the solution statistics and the
@@ -225,7 +225,7 @@ wosKernel(WosParams P) {
         const int pair = P.gradHalves == 2 ? (w >> 1) : w;
         const bool mirror = P.gradHalves == 2 && (w & 1);
         const Vec<D> c0 = vload<D>(P.pt + (size_t)p * D);
-        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
+        seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)pair);
         T = 1.0f; S = 0.0f; len = 0;
         ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
         ball.update(0.99f * P.dist0[p]);
@@ -318,7 +318,7 @@ wosKernel(WosParams P) {
         T = 1.0f; S = 0.0f; len = 0;
         ball.yukawa = SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == 0;
         ycValid = false;
-        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+        seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)w);
         if (SPLIT) { nq = 0; nSplits = 0; partSum = 0.0f; partOk = false; }
         atomicAdd(&blk[5], 1ull);
         if (dist > eps) { mode = 1; pendingCode = -1; }
@@ -592,7 +592,7 @@ __global__ void __launch_bounds__(256) evalFunctionKernel(zg_function f, int64_t
 }
 
 // Latin hypercube tables (generateStratifiedSamples<DIMS>, core/sampling.h:435-458), one lane per
-// table: table t is drawn from pcg32(~seed, (t << 32) | streamOffset).  With nWalks given, table
+// table: table t is drawn from the generator seedWalk(~seed, t, streamOffset).  With nWalks given, table
 // p holds 2 * pairs(p) rows (at most nSamples, the rows a table has room for) for the gradient
 // points only; otherwise every table has nSamples rows.
 template <int DIMS>
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(256) stratifiedKernel(float* __restrict__ out,
     if (ns <= 0) return;
     float* s = out + (size_t)t * (size_t)stride;
     Pcg32 rng;
-    rng.seed(~seed, ((uint64_t)t << 32) | (uint64_t)streamOffset);
+    seedWalk(rng, ~seed, (uint32_t)t, streamOffset);
     const float inv = 1.0f / (float)ns;
     const float oneMinusEps = 1.0f - FLT_EPSILON;
     for (int i = 0; i < ns; i++)

@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         const int pair = P.gradHalves == 2 ? (w >> 1) : w;
         const bool mirror = P.gradHalves == 2 && (w & 1);
         const Vec<D> c0 = vload<D>(P.pt + D * (size_t)p);
-        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)pair));
+        seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)pair);
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
         ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
         ball.update(P.fsr[p]);
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         absPrim = -1;
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
         ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
-        rng.seed(P.seed, ((uint64_t)p << 32) | (uint64_t)(uint32_t)(streamOffset + (uint32_t)w));
+        seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)w);
         atomicAdd(&blk[5], 1ull);
         active = true;
     };

@@ -1,8 +1,7 @@
 """Array-level host API over the C-ABI (device arrays are torch tensors on the GPU).
 
 PyTorch is used only for device memory and streams; every computation runs in
-libzombie_amd.so.  This layer is what the reference-shaped API in zombie_amd.api and the
-benchmark build on.
+libzombie_amd.so.  The benchmark and the reference-shaped layers build on this one.
 """
 from __future__ import annotations
 
@@ -21,6 +20,24 @@ def _torch():
 
 def _ptr(t) -> int | None:
     return None if t is None else t.data_ptr()
+
+
+def check_array(name: str, t, dtype, shape: tuple, device=None):
+    """The C-ABI takes raw device pointers and trusts their extents: refuse, on the host, an
+    array of the wrong element type, shape or layout, or one on another device, before its
+    pointer reaches a kernel (which would read or write past its end or misread its bytes)."""
+    if t is None:
+        return
+    if not hasattr(t, "data_ptr"):
+        raise TypeError(f"{name} must be a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} has dtype {t.dtype}; the C-ABI needs {dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}; the C-ABI needs {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} is not contiguous")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, the solve's arrays are on {device}")
 
 
 def _stream_ptr(stream=None) -> int | None:
@@ -321,6 +338,36 @@ class DevicePoints:
             if a is not None and (a.dim() != 2 or a.shape[0] != self.n or a.shape[1] != dim):
                 raise ValueError(f"{name} has shape {tuple(a.shape)}; the scene needs [{self.n}, {dim}]")
 
+    def validate(self, dim: int, check_walk_bound: bool = True):
+        """Full host-side check of every array a solve hands to the C-ABI (element type, shape,
+        contiguity, one GPU device), required arrays present, and n_walks[] <= n_walks_max."""
+        torch = _torch()
+        self.check_dim(dim)
+        n = self.n
+        dev = self.pt.device
+        if self.dist_to_absorbing is None or self.n_walks is None:
+            raise ValueError("dist_to_absorbing and n_walks are required")
+        f32, u8 = torch.float32, torch.uint8
+        for name, dtype, shape in (("pt", f32, (n, dim)), ("normal", f32, (n, dim)), ("type", u8, (n,)),
+                                   ("dist_to_absorbing", f32, (n,)), ("dist_to_reflecting", f32, (n,)),
+                                   ("aligned", u8, (n,)), ("first_sphere_radius", f32, (n,)),
+                                   ("n_walks", torch.int32, (n,)), ("cv_gradient", f32, (n, dim)),
+                                   ("estimation_quantity", u8, (n,)), ("direction_for_derivative", f32, (n, dim))):
+            check_array(name, getattr(self, name), dtype, shape, dev)
+        if self.cv_gradient_grid is not None:
+            g = self.cv_gradient_grid.data
+            if g is None or g.dim() != 4 or g.shape[3] != 4:
+                raise ValueError("cv_gradient_grid.data must be [res_z, res_y, res_x, 4]")
+            check_array("cv_gradient_grid.data", g, f32, tuple(g.shape), dev)
+        if int(self.n_walks_max) < 0:
+            raise ValueError("n_walks_max must be >= 0")
+        if check_walk_bound and n > 0:
+            most = int(self.n_walks.max())
+            if most > int(self.n_walks_max):
+                raise ValueError(f"n_walks holds {most} walks for a point, above n_walks_max = {self.n_walks_max}")
+        if dev.type != "cuda":
+            raise ValueError(f"points must be on a GPU device, pt is on {dev}")
+
     def to_c(self) -> A.zg_sample_points:
         s = A.zg_sample_points()
         s.n = self.n
@@ -385,6 +432,24 @@ class DeviceStats:
             st.n_grad, st.grad_mean, st.grad_m2 = i64(count), z64(count, dim), z64(count, dim)
             st.first_source, st.derivative = z64(count), z64(count)
         return st
+
+    def validate(self, count: int, dim: int, device, need_gradient: bool = False):
+        """Element types, lengths, contiguity and device of every member the solve writes;
+        need_gradient: the gradient members must be present (gradient points)."""
+        torch = _torch()
+        i64, f64 = torch.int64, torch.float64
+        for name in ("n", "mean", "m2", "walk_length", "splits"):
+            if getattr(self, name) is None:
+                raise ValueError(f"statistics member {name} is required")
+        grad = ("n_grad", "grad_mean", "grad_m2", "first_source", "derivative")
+        if need_gradient and any(getattr(self, g) is None for g in grad):
+            raise ValueError("gradient points need the gradient statistics (DeviceStats.zeros(..., dim=d))")
+        for name, dtype, shape in (("n", i64, (count,)), ("mean", f64, (count,)), ("m2", f64, (count,)),
+                                   ("walk_length", i64, (count,)), ("splits", i64, (count,)),
+                                   ("n_grad", i64, (count,)), ("grad_mean", f64, (count, dim)),
+                                   ("grad_m2", f64, (count, dim)), ("first_source", f64, (count,)),
+                                   ("derivative", f64, (count,))):
+            check_array(f"stats.{name}", getattr(self, name), dtype, shape, device)
 
     def to_c(self) -> A.zg_statistics:
         s = A.zg_statistics()
@@ -461,10 +526,20 @@ def stratified_samples(dims: int, n_tables: int, n_samples: int, seed: int, stre
     return out
 
 
+def _validate_solve(dim: int, points: DevicePoints, stats: DeviceStats, instance_of_point=None):
+    points.validate(dim)
+    need_grad = False
+    if points.estimation_quantity is not None and points.n > 0:
+        need_grad = bool((points.estimation_quantity == A.ZG_ESTIMATE_SOLUTION_AND_GRADIENT).any())
+    stats.validate(points.n, dim, points.pt.device, need_grad)
+    if instance_of_point is not None:
+        check_array("instance_of_point", instance_of_point, _torch().int32, (points.n,), points.pt.device)
+
+
 def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
               opts: SolveOptions | None = None, stream=None) -> SolveReport:
     opts = opts or SolveOptions()
-    points.check_dim(scene.dim)
+    _validate_solve(scene.dim, points, stats)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
@@ -521,7 +596,9 @@ class Batch:
 def batch_wos_solve(batch: Batch, ws: A.zg_walk_settings, points: DevicePoints, instance_of_point,
                     stats: DeviceStats, opts: SolveOptions | None = None, stream=None) -> SolveReport:
     opts = opts or SolveOptions()
-    points.check_dim(3)
+    if instance_of_point is None:
+        raise ValueError("instance_of_point is required")
+    _validate_solve(3, points, stats, instance_of_point)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
@@ -539,7 +616,9 @@ class SolveGraph:
                  stats: DeviceStats, opts: SolveOptions | None = None):
         opts = opts or SolveOptions()
         self._keep = (batch, points, instance_of_point, stats)
-        points.check_dim(3)
+        if instance_of_point is None:
+            raise ValueError("instance_of_point is required")
+        _validate_solve(3, points, stats, instance_of_point)
         sp = points.to_c()
         st = stats.to_c()
         oc = opts.to_c()
@@ -571,7 +650,7 @@ class SolveGraph:
 def wost_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
                opts: SolveOptions | None = None, stream=None) -> SolveReport:
     opts = opts or SolveOptions()
-    points.check_dim(scene.dim)
+    _validate_solve(scene.dim, points, stats)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
