@@ -465,6 +465,75 @@ def screened_batch_spec(n_instances: int = 256, seed: int = 0x5EED, tri_range=(2
                  direction=dirs[i], seed=int(seed + 1 + i)) for i in range(n_instances)]
 
 
+def engine_wost(fixture, device="cuda") -> Problem:
+    """The reference's engine demo problem (demo/model_problems/engine/wost.json), set up as its
+    demo does (demo/model_problem.h:129-328, demo/demo.cpp:94-146, demo/grid.h:23-40) from the
+    data in `fixture` (an .npz written by tests/golden/make_engine_fixture.py):
+
+    - the OBJ polyline normalised (centroid at the origin, largest vertex distance 1,
+      fcpw_geometric_queries.h:255-273) with every segment's orientation flipped (:284-289);
+      the domain is its square bounding box (:291-317);
+    - a segment is reflecting (Neumann) when the reflecting-boundary image is > 0 at its
+      midpoint (partitionBoundaryMesh, :459-509); the image lookup is Image::get(uv) with
+      uv = (x - bMin) / maxLength: the cell containing uv, rows flipped (demo/image.h:103-110);
+    - Dirichlet data: the absorbing-value image under the same lookup, as a nearest-cell dense
+      grid over [bMin, bMin + maxLength]^2; pure Neumann reflecting conditions (robinCoeff 0);
+      source and reflecting data ignored (ignoreSourceContribution, ignoreReflecting...
+      Contribution in wost.json);
+    - walk settings of wost.json: 128 walks, epsilon 1e-3 on both boundaries, silhouette
+      precision 1e-3, Russian roulette 0.99, splitting 1.5, max length 1024.
+
+    meta: 'grid' [res*res, 2] points (index i*res + j at x index i, y index j), 'solution'
+    [res, res] the reference's result indexed [x index, y index], 'mask_distance' (pixels
+    closer than it to the boundary are written as 0), 'dirichlet_grid' (keeps the device
+    values alive), 'reference_walks' (128)."""
+    z = np.load(fixture, allow_pickle=False)
+    cfg = dict(zip([str(n) for n in z["setting_names"]], z["settings"].tolist()))
+    V = z["positions"].astype(np.float64)
+    V = V - V.mean(0)
+    V = (V / np.linalg.norm(V, axis=1).max()).astype(np.float32)
+    I = z["segments"][:, ::-1].astype(np.int32).copy()          # flipOrientation
+    lo, hi = square_bounding_box(V)
+    max_len = np.float32((hi - lo).max())
+
+    def lookup_index(img_shape, x):
+        h, w = img_shape
+        uv = (x.astype(np.float32) - lo) / max_len
+        i = np.clip((uv[:, 1] * np.float32(h)).astype(np.int64), 0, h - 1)
+        j = np.clip((uv[:, 0] * np.float32(w)).astype(np.int64), 0, w - 1)
+        return h - i - 1, j
+
+    mask_shape = tuple(int(v) for v in z["reflecting_mask_shape"])
+    mask = np.unpackbits(z["reflecting_mask_bits"])[: mask_shape[0] * mask_shape[1]].reshape(mask_shape).astype(bool)
+    mid = 0.5 * (V[I[:, 0]] + V[I[:, 1]])
+    r, c = lookup_index(mask_shape, mid)
+    refl = mask[r, c].astype(np.uint8)
+
+    g = z["absorbing_value"]
+    # grid[x cell, y cell] = image buffer[h - 1 - y cell, x cell]
+    grid_vals = np.ascontiguousarray(g[::-1, :].T)
+    dirichlet = E.GridFunction(grid_vals, lo, lo + max_len, interpolate=False, device=device)
+    pde = E.make_pde(source=E.zero(), dirichlet=dirichlet.fn, robin=E.zero(), is_source_constant=False)
+    ws = E.walk_settings_host(cfg["epsilonShellForAbsorbingBoundary"], cfg["epsilonShellForReflectingBoundary"],
+                              int(cfg["maxWalkLength"]), bool(cfg["solveDoubleSided"]),
+                              russian_roulette_threshold=cfg["russianRouletteThreshold"],
+                              splitting_threshold=cfg["splittingThreshold"],
+                              ignore_absorbing_boundary_contribution=int(cfg["ignoreAbsorbingBoundaryContribution"]),
+                              ignore_reflecting_boundary_contribution=int(cfg["ignoreReflectingBoundaryContribution"]),
+                              ignore_source_contribution=int(cfg["ignoreSourceContribution"]))
+    res = int(cfg["gridRes"])
+    ext = hi - lo
+    ii, jj = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    grid = np.stack([(ii / np.float32(res)).astype(np.float32) * ext[0] + lo[0],
+                     (jj / np.float32(res)).astype(np.float32) * ext[1] + lo[1]], -1).reshape(-1, 2).astype(np.float32)
+    sol = z["solution"]   # file rows: row h-1-j holds y index j, column i holds x index i
+    prob = Problem("engine_wost", 2, V, I, refl, lo, hi, pde, ws, int(cfg["nWalks"]), None)
+    prob.meta.update(grid=grid, solution=np.ascontiguousarray(sol[::-1, :].T), dirichlet_grid=dirichlet,
+                     mask_distance=float(cfg["boundaryDistanceMask"]), reference_walks=int(cfg["nWalks"]),
+                     watertight=bool(cfg["domainIsWatertight"]))
+    return prob
+
+
 def sample_box(lo, hi, n, seed):
     rng = np.random.default_rng(seed)
     return rng.uniform(lo, hi, size=(n, len(lo))).astype(np.float32)
