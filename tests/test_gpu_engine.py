@@ -88,3 +88,69 @@ def test_engine_wost_matches_reference_solution(gpu, engine):
     zr = np.array(zr)
     assert np.abs(zr).max() < 4.5, f"region z max {np.abs(zr).max():.2f}"
     assert abs(zr.mean()) < 0.3 and 0.7 < zr.std() < 1.3, f"region z mean {zr.mean():.3f} std {zr.std():.3f}"
+
+
+def _write_pfm_gray(path, img):
+    """img [h, w] in the demo reader's storage order (file rows), one channel."""
+    h, w = img.shape
+    with open(path, "wb") as f:
+        f.write(f"Pf\n{w} {h}\n-1\n".encode())
+        f.write(np.ascontiguousarray(img, "<f4").tobytes())
+
+
+def test_engine_json_problem_through_the_reference_python_surface(gpu, tmp_path):
+    """The engine model problem as files in the reference's format (OBJ, PFM images, wost.json),
+    run by zombie_amd.demo through the reference-shaped layer zombie_amd.zombie (the flow of
+    demo/demo.py), compared with the reference's own solution."""
+    import json
+    from zombie_amd import demo
+
+    z = np.load(FIXTURE, allow_pickle=False)
+    cfg = dict(zip([str(n) for n in z["setting_names"]], z["settings"].tolist()))
+    data = tmp_path / "data"
+    data.mkdir()
+    with open(data / "geometry.obj", "w") as f:
+        for p in z["positions"]:
+            f.write(f"v {float(p[0])!r} {float(p[1])!r} 0\n")
+        for a, b in z["segments"]:
+            f.write(f"l {a + 1} {b + 1}\n")
+    _write_pfm_gray(data / "absorbing_boundary_value.pfm", z["absorbing_value"])
+    shape = tuple(int(v) for v in z["reflecting_mask_shape"])
+    mask = np.unpackbits(z["reflecting_mask_bits"])[: shape[0] * shape[1]].reshape(shape).astype(np.float32)
+    _write_pfm_gray(data / "is_reflecting_boundary.pfm", mask)
+    _write_pfm_gray(data / "reflecting_boundary_value.pfm", np.zeros((8, 8), np.float32))
+    _write_pfm_gray(data / "source_value.pfm", np.zeros((8, 8), np.float32))
+    problem = {
+        "solverType": "wost",
+        "solver": {"nWalks": int(cfg["nWalks"]), "maxWalkLength": int(cfg["maxWalkLength"]),
+                   "epsilonShellForAbsorbingBoundary": cfg["epsilonShellForAbsorbingBoundary"],
+                   "epsilonShellForReflectingBoundary": cfg["epsilonShellForReflectingBoundary"],
+                   "russianRouletteThreshold": cfg["russianRouletteThreshold"],
+                   "splittingThreshold": cfg["splittingThreshold"],
+                   "ignoreAbsorbingBoundaryContribution": bool(cfg["ignoreAbsorbingBoundaryContribution"]),
+                   "ignoreReflectingBoundaryContribution": bool(cfg["ignoreReflectingBoundaryContribution"]),
+                   "ignoreSourceContribution": bool(cfg["ignoreSourceContribution"])},
+        "modelProblem": {"geometry": "data/geometry.obj", "isReflectingBoundary": "data/is_reflecting_boundary.pfm",
+                         "absorbingBoundaryValue": "data/absorbing_boundary_value.pfm",
+                         "reflectingBoundaryValue": "data/reflecting_boundary_value.pfm",
+                         "sourceValue": "data/source_value.pfm", "robinCoeff": cfg["robinCoeff"],
+                         "absorptionCoeff": cfg["absorptionCoeff"], "solveDoubleSided": False,
+                         "solveExterior": False, "domainIsWatertight": True, "useSdfForAbsorbingBoundary": False},
+        "output": {"solutionFile": "solutions/wost.pfm", "gridRes": int(cfg["gridRes"]),
+                   "boundaryDistanceMask": cfg["boundaryDistanceMask"]},
+    }
+    (tmp_path / "wost.json").write_text(json.dumps(problem))
+    print("engine demo driver: 1024 walks per pixel", flush=True)
+    r = demo.run(str(tmp_path / "wost.json"), root=str(tmp_path), n_walks=1024, seed=77)
+    out = demo.read_image(tmp_path / "solutions" / "wost.pfm")        # file rows, as the demo stores it
+    ref_file = np.ascontiguousarray(z["solution"])
+    got, var, ref = r["values"].reshape(-1), r["variance"].reshape(-1), PB.engine_wost(FIXTURE, device=None).meta["solution"].reshape(-1)
+    # the written PFM is laid out as the reference's wost.pfm
+    assert out.shape == ref_file.shape
+    assert np.array_equal(out != 0, ref_file != 0) or ((out != 0) != (ref_file != 0)).sum() <= 256
+    both = (got != 0) & (ref != 0)
+    assert both.sum() > 0.95 * (ref != 0).sum()
+    se2 = var[both] / r["walks"] + var[both] / 128.0
+    ok = se2 > 0
+    zz = (got[both][ok] - ref[both][ok]) / np.sqrt(se2[ok])
+    check_four_se(zz, np.ones(len(zz), bool), "engine demo driver vs reference wost.pfm")
