@@ -156,3 +156,12 @@ def test_stats_validation_needs_gradient_members_for_gradient_points():
         st.validate(4, 3, torch.device("cpu"), need_gradient=True)
     with pytest.raises(ValueError, match="on"):
         E.DeviceStats.zeros(4, device="cpu").validate(4, 3, torch.device("meta"))
+
+
+def test_cpp_adapter_compiles_and_links(tmp_path):
+    """include/zombie_amd.hpp (the reference-signature C++ adapter) builds with plain g++ -Wall
+    -Wextra against the C-ABI: tests/adapter_check.cpp, the caller the GPU test runs, compiles
+    and links against libzombie_amd.so (no GPU needed to link)."""
+    from zombie_amd import _build
+    out = _build.build_adapter_check(out=tmp_path / "adapter_check")
+    assert out.exists()

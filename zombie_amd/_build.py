@@ -30,7 +30,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["bvh_build.cpp", "abi.cpp", "wos_kernel.hip", "wost_kernel.hip", "mlp_kernels.hip", "probe_kernel.hip",
-           "bvc_kernel.hip", "sdf_kernel.hip", "spatial_sort.cpp"]
+           "bvc_kernel.hip", "sdf_kernel.hip", "spatial_sort.cpp", "noalloc.hip"]
 
 
 def _hipcc() -> str:
@@ -83,6 +83,26 @@ def build_oracle(verbose: bool = False) -> Path:
     return lib
 
 
+ADAPTER_CHECK = BUILD_DIR / "adapter_check"
+
+
+def build_adapter_check(verbose: bool = False, out: Path | None = None) -> Path:
+    """Compiles tests/adapter_check.cpp, a C++ caller of the header-only adapter
+    include/zombie_amd.hpp, against libzombie_amd.so (plain g++, no HIP headers)."""
+    out = Path(out) if out is not None else ADAPTER_CHECK
+    out.parent.mkdir(parents=True, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", str(REPO / "include"),
+           str(REPO / "tests" / "adapter_check.cpp"), "-L", str(PKG_DIR), "-lzombie_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + str(PKG_DIR), "-Wl,-rpath,/opt/rocm/lib", "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"adapter check build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {out}", file=sys.stderr)
+    return out
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
     build_oracle(verbose=True)
+    build_adapter_check(verbose=True)
