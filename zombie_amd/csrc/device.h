@@ -623,21 +623,27 @@ constexpr int kStackLdsBytes = kLdsStack * 256 * 8;
 
 extern __shared__ uint2 zgLdsStack[];
 
+// The spill array is addressed through a private (address space 5) pointer: its loads and
+// stores are scratch instructions that stay in the cache hierarchy, and the LDS load of pop()
+// cannot be merged with the spill load into one flat load.  (A volatile generic pointer, the
+// earlier form, compiled to system-scope write-through flat stores, each followed by a wait.)
+typedef __attribute__((address_space(5))) unsigned long long PrivU64;
+
 struct Stack {
     int col;          // threadIdx.x
     int stride;       // blockDim.x
     int sp;           // entries in LDS
     int ovf;          // entries in the spill array
     bool overflow;
-    unsigned long long* spill;  // caller's private array of kSpillStack entries
+    PrivU64* spill;   // caller's private array of kSpillStack entries
     ZG_DEV void init(unsigned long long* spillArray) {
-        col = threadIdx.x; stride = blockDim.x; sp = 0; ovf = 0; overflow = false; spill = spillArray;
+        col = threadIdx.x; stride = blockDim.x; sp = 0; ovf = 0; overflow = false; spill = (PrivU64*)spillArray;
     }
     ZG_DEV void push(int node, float d2) {
         uint2 e = make_uint2((unsigned)node, __float_as_uint(d2));
         if (sp < kLdsStack) { zgLdsStack[sp * stride + col] = e; sp++; }
         else if (ovf < kSpillStack) {
-            reinterpret_cast<volatile unsigned long long*>(spill)[ovf] = ((unsigned long long)e.y << 32) | e.x;
+            spill[ovf] = ((unsigned long long)e.y << 32) | e.x;
             ovf++;
         }
         else overflow = true;
@@ -645,10 +651,8 @@ struct Stack {
     ZG_DEV bool pop(int& node, float& d2) {
         uint2 e;
         if (ovf > 0) {
-            // volatile keeps this private-memory load apart from the LDS load below (merged,
-            // the two would become one slow flat load on the hot path)
             ovf--;
-            unsigned long long v = reinterpret_cast<volatile unsigned long long*>(spill)[ovf];
+            unsigned long long v = spill[ovf];
             e.x = (unsigned)v;
             e.y = (unsigned)(v >> 32);
         } else if (sp > 0) {
