@@ -367,6 +367,22 @@ zg_status zg_wos_solve(const zg_scene* scene, const zg_pde* pde_host,
                        zg_statistics* stats, const zg_solve_options* opts_host,
                        zg_solve_report* report_host, void* stream);
 
+/* No counterpart in the reference: zg_wos_solve for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as zg_wos_solve, but the per-point workspace (32 B per point,
+ * 40 B with walk splitting, plus 128 B of counters) lives in a fixed 32 MiB device array that
+ * the library holds on each GPU it runs on until the process ends; the call makes no
+ * allocation.  Limits: solution estimates only (estimation_quantity must be NULL), at most
+ * 1,048,572 points per call (838,857 with splitting_threshold < 1), otherwise
+ * ZG_ERR_INVALID_ARGUMENT / ZG_ERR_UNSUPPORTED.  The call synchronises `stream` (the counters
+ * come back to fill report_host, whose kernel_ms is 0: no timing events are created).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (other host threads,
+ * other streams) race and must be serialised by the caller. */
+zg_status zg_wos_solve_noalloc(const zg_scene* scene, const zg_pde* pde_host,
+                               const zg_walk_settings* settings_host, const zg_sample_points* points_host,
+                               zg_statistics* stats, const zg_solve_options* opts_host,
+                               zg_solve_report* report_host, void* stream);
+
 /* WalkOnStars<float,DIM>::solve (walk_on_stars.h:150-187 -> estimateSolution :518-662,
  * walk :345-490); mesh scenes (2D segments, 3D triangles), harmonic Green's function or, in
  * 3D with absorption > 0, the Yukawa ball (dampening on steps that miss the reflecting

@@ -249,6 +249,9 @@ SIGNATURES = {
     "zg_sample_reflecting": (C.c_int32, [_P, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_wos_solve": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points),
                                  C.POINTER(zg_statistics), C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),
+    "zg_wos_solve_noalloc": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings),
+                                         C.POINTER(zg_sample_points), C.POINTER(zg_statistics),
+                                         C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),
     "zg_wost_solve": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points),
                                   C.POINTER(zg_statistics), C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),
     "zg_batch_create": (C.c_int32, [C.c_int32, _P, C.POINTER(zg_pde), C.POINTER(C.c_void_p)]),

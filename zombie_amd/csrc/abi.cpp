@@ -819,6 +819,38 @@ zg_status zg_wos_solve(const zg_scene* s, const zg_pde* pde, const zg_walk_setti
     return runSolve(S, report, (hipStream_t)stream);
 }
 
+zg_status zg_wos_solve_noalloc(const zg_scene* s, const zg_pde* pde, const zg_walk_settings* ws,
+                               const zg_sample_points* pts, zg_statistics* stats, const zg_solve_options* opts,
+                               zg_solve_report* report, void* stream) {
+    if (!s) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL scene");
+    SolveSetup S;
+    zg_status st = prepareSolve(s, nullptr, pde, ws, pts, nullptr, stats, opts, S);
+    if (st != ZG_OK) return st;
+    if (S.P.quantity)
+        return fail(ZG_ERR_UNSUPPORTED, "zg_wos_solve_noalloc estimates solutions only (estimation_quantity must be NULL)");
+    const size_t cap = kNoallocScratchBytes / 8;
+    if (S.words > cap)
+        return fail(ZG_ERR_INVALID_ARGUMENT,
+                    "%lld points need %zu bytes of workspace; zg_wos_solve_noalloc holds %zu (at most %lld points%s)",
+                    (long long)S.n, S.words * 8, kNoallocScratchBytes,
+                    (long long)((cap - kCntSlots) / (S.split ? 5 : 4)), S.split ? " with walk splitting" : "");
+    if (S.n == 0) { if (report) std::memset(report, 0, sizeof(*report)); return ok(); }
+    unsigned long long* wsp = nullptr;
+    hipStream_t strm = (hipStream_t)stream;
+    hipError_t e = noallocScratch(&wsp);
+    if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "workspace lookup failed: %s", hipGetErrorString(e));
+    // the walk-on-spheres kernels read the PDE from their arguments (no device copy needed)
+    e = enqueueSolve(S, reinterpret_cast<uint64_t*>(wsp), strm, nullptr, nullptr);
+    unsigned long long cnt[kCntSlots] = {0};
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt, S.P.counters, sizeof(cnt), hipMemcpyDeviceToHost, strm);
+    if (e == hipSuccess) e = hipStreamSynchronize(strm);
+    if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "walk-on-spheres launch failed: %s", hipGetErrorString(e));
+    st = checkErrorFlags(cnt[kCntError], S.depth, S.width);
+    if (st != ZG_OK) return st;
+    if (report) fillReport(cnt, 0.0f, report);
+    return ok();
+}
+
 zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* S, const zg_bvc_eval_points* E,
                        float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR,
                        zg_bvc_statistics* out, void* stream) {

@@ -173,4 +173,10 @@ hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, co
 hipError_t launchSampleReflecting(const SceneView& sv, int64_t n, const float* x, const float* radius, const float* u,
                                   uint8_t* ok, float* pt, float* normal, float* pdf, hipStream_t s);
 
+
+// zg_wos_solve_noalloc: fixed device workspace (noalloc.hip), one copy per GPU for the life of
+// the process.  noallocScratch() returns the current device's copy.
+constexpr size_t kNoallocScratchBytes = size_t(32) << 20;
+hipError_t noallocScratch(unsigned long long** out);
+
 }  // namespace zg

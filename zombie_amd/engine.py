@@ -549,6 +549,22 @@ def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: Devic
     return SolveReport.from_c(rep)
 
 
+def wos_solve_noalloc(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints,
+                      stats: DeviceStats, opts: SolveOptions | None = None, stream=None) -> SolveReport:
+    """wos_solve through zg_wos_solve_noalloc: the workspace is the library's fixed per-GPU device
+    array (no allocation by the call); solution estimates only, at most 1,048,572 points per call;
+    calls that overlap on one GPU race.  The report's kernel_ms is 0."""
+    opts = opts or SolveOptions()
+    _validate_solve(scene.dim, points, stats)
+    sp = points.to_c()
+    st = stats.to_c()
+    oc = opts.to_c()
+    rep = A.zg_solve_report()
+    A.check(A.lib().zg_wos_solve_noalloc(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st),
+                                         C.byref(oc), C.byref(rep), _stream_ptr(stream)))
+    return SolveReport.from_c(rep)
+
+
 def probe_read_bandwidth(nbytes: int, reps: int, device="cuda", stream=None) -> float:
     """Read bandwidth in GB/s of a float4 streaming read of an nbytes buffer, repeated reps times
     (an L2-sized buffer measures the cache ceiling, a large one the HBM ceiling)."""
