@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,7 +28,9 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "s to RMSE 1e-3 vs analytic, 1M pts: plain vs learned-CV WoS; walk-steps/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 TARGET_RMSE = 1e-3
+T_START = time.time()
 
 
 def parse():
@@ -43,6 +46,11 @@ def parse():
     ap.add_argument("--walks", type=int, default=None, help="walks per point per pass (workload default)")
     ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
     ap.add_argument("--chunk-walks", type=int, default=128, help="mixed2d: walks per point per solve call")
+    ap.add_argument("--wost-step-interval", type=int, default=4,
+                    help="mixed2d: iterations (one BVH item per query in flight) between the step logic runs")
+    ap.add_argument("--wost-run-to-completion", action="store_true",
+                    help="mixed2d: run each walk-on-stars query to completion inside its step (the "
+                         "non-interleaved kernel)")
     ap.add_argument("--bvh-width", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=2)
     ap.add_argument("--walks-per-lane", type=int, default=1)
@@ -60,10 +68,13 @@ def parse():
     ap.add_argument("--cv-steps", type=int, default=5000)
     ap.add_argument("--cv-batch", type=int, default=65536)
     ap.add_argument("--cv-grid", type=int, default=96)
-    ap.add_argument("--to-rmse", type=float, default=0.0,
+    ap.add_argument("--to-rmse", type=float, default=1e-3,
                     help="after the timed passes, run learned-CV passes until the true RMSE vs the analytic "
                          "solution reaches this value, and report the measured time (0: skip)")
     ap.add_argument("--to-rmse-max-passes", type=int, default=400)
+    ap.add_argument("--budget-s", type=float, default=450.0,
+                    help="soft wall-time budget of the whole run: the plain run of equal wall time stops "
+                         "early (and says so) when it would exceed it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for the PMC byte counters)")
     ap.add_argument("--cpu-points", type=int, default=16000)
@@ -160,14 +171,24 @@ def main():
     sub = len(range(0, n, stride))
     dps = E.DevicePoints(pt=P[::stride].contiguous(), dist_to_absorbing=dp.dist_to_absorbing[::stride].contiguous(),
                          n_walks=dp.n_walks[::stride].contiguous(), n_walks_max=args.walks)
-    rc = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, True))
-    rx = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, True, 0.0))
+    # the algorithmic work: exact radii (the reference's walk) and, at every query point of those
+    # walks, the nodes and triangles an exact nearest-first traversal of this BVH reads
+    # (count_visits 2); the kernel's own visits at the tuned tolerance are reported beside them
+    rx = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, 2, 0.0))
+    rc = E.wos_solve(scene, prob.pde, prob.settings, dps, E.DeviceStats.zeros(sub), opts(10_000, 1))
     len_exact = rx.total_steps / max(rx.total_walks, 1)
-    nodes_per_step = rc.node_visits / max(rc.total_steps, 1)
-    prims_per_step = rc.prim_visits / max(rc.total_steps, 1)
+    nodes_per_step = rx.node_visits / max(rx.total_steps, 1)
+    prims_per_step = rx.prim_visits / max(rx.total_steps, 1)
     bytes_per_step = nodes_per_step * scene.info.node_bytes + prims_per_step * scene.info.prim_bytes + 48.0
-    log(f"visits/step: nodes {nodes_per_step:.2f} prims {prims_per_step:.2f} -> {bytes_per_step:.0f} B/step; "
-        f"walk length exact {len_exact:.3f}, tuned {rc.total_steps / max(rc.total_walks, 1):.3f}")
+    own_nodes_per_step = rc.node_visits / max(rc.total_steps, 1)
+    own_prims_per_step = rc.prim_visits / max(rc.total_steps, 1)
+    visit_source = ("exact walks (tolerance 0) on a strided 65,536-point sample; at each of their query points "
+                    "an exact nearest-first traversal from the root (count_visits 2, pinned against the oracle's "
+                    "packed-BVH traversal by tests/test_gpu_roofline.py)")
+    log(f"algorithmic visits/exact step: nodes {nodes_per_step:.2f} prims {prims_per_step:.2f} -> "
+        f"{bytes_per_step:.0f} B/step; kernel's own visits/step at tolerance {tol}: nodes {own_nodes_per_step:.2f} "
+        f"prims {own_prims_per_step:.2f}; walk length exact {len_exact:.3f}, tuned "
+        f"{rc.total_steps / max(rc.total_walks, 1):.3f}")
 
     if args.calibrate_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
@@ -178,24 +199,54 @@ def main():
         del src, dst
 
     # ---------------- timed passes ----------------
-    def timed(points, label, seed):
-        stats = E.DeviceStats.zeros(n)
+    ex_dev = torch.from_numpy(exact).to(dev)
+
+    class Progressive:
+        """Progressive solves of all points (fresh walk streams per pass, statistics merged)."""
+
+        def __init__(self, points, seed, label):
+            self.points, self.seed, self.label = points, seed, label
+            self.stats = E.DeviceStats.zeros(n)
+            self.passes, self.solve_s, self.walks, self.steps = 0, 0.0, 0, 0
+            self.kernel_ms = []
+            self.curve = []
+
+        def run(self, count):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = []
+            for _ in range(count):
+                reps.append(E.wos_solve(scene, prob.pde, prob.settings, self.points, self.stats,
+                                        opts(self.passes, seed=self.seed)))
+                self.passes += 1
+                log(f"{self.label} pass {self.passes}: {reps[-1].total_steps} steps, {reps[-1].total_walks} walks, "
+                    f"kernel {reps[-1].kernel_ms:.1f} ms")
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            self.solve_s += el
+            self.walks += sum(r.total_walks for r in reps)
+            self.steps += sum(r.total_steps for r in reps)
+            self.kernel_ms += [r.kernel_ms for r in reps]
+            return reps, el
+
+        def rmse(self):
+            r = float(torch.sqrt(torch.mean((self.stats.mean - ex_dev) ** 2)).item())
+            self.curve.append((self.passes * args.walks, round(self.solve_s, 2), r))
+            return r
+
+        def host(self):
+            return dict(mean=self.stats.mean.cpu().numpy(), var=self.stats.variance().cpu().numpy(),
+                        n=self.stats.n.cpu().numpy())
+
+    def warm(points, seed):
         for w in range(args.warmup):
             E.wos_solve(scene, prob.pde, prob.settings, points, E.DeviceStats.zeros(n), opts(1000 + w, seed=seed))
         torch.cuda.synchronize()
-        reports = []
-        t0 = time.perf_counter()
-        for s in range(args.steps):
-            reports.append(E.wos_solve(scene, prob.pde, prob.settings, points, stats, opts(s, seed=seed)))
-            log(f"{label} pass {s}: {reports[-1].total_steps} steps, {reports[-1].total_walks} walks, "
-                f"kernel {reports[-1].kernel_ms:.1f} ms")
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        return dict(reports=reports, elapsed=elapsed, mean=stats.mean.cpu().numpy(),
-                    var=stats.variance().cpu().numpy(), n=stats.n.cpu().numpy())
 
-    plain = timed(dp, "plain", args.seed)
-    reports, elapsed = plain["reports"], plain["elapsed"]
+    warm(dp, args.seed)
+    plain_run = Progressive(dp, args.seed, "plain")
+    reports, elapsed = plain_run.run(args.steps)
+    plain = plain_run.host()
     total_steps = sum(r.total_steps for r in reports)
     total_walks = sum(r.total_walks for r in reports)
     kernel_ms = sum(r.kernel_ms for r in reports) / len(reports)
@@ -204,18 +255,24 @@ def main():
     # reported in the reference's unit: walks completed per second times the mean walk length
     # that exact radii give (equal to the raw rate when the tolerance is 0)
     steps_per_s = walks_per_s * len_exact if tol > 0 else raw_steps_per_s
-    rmse = float(np.sqrt(np.mean((plain["mean"] - exact) ** 2)))
-    se_rmse = float(np.sqrt(np.mean(plain["var"] / np.maximum(plain["n"], 1))))
+    rmse = plain_run.rmse()
+    se2 = float(np.mean(plain["var"] / np.maximum(plain["n"], 1)))
+    se_rmse = math.sqrt(se2)
     var_plain = float(np.mean(plain["var"]))
     s_to_rmse_plain = var_plain / TARGET_RMSE ** 2 * n / walks_per_s
 
-    achieved_gbs = bytes_per_step * (total_steps / len(reports)) / (kernel_ms * 1e-3) / 1e9
+    # algorithmic bytes of one launch: its walks x the exact walk length x bytes per exact step
+    exact_steps_per_pass = total_walks / len(reports) * len_exact
+    achieved_gbs = bytes_per_step * exact_steps_per_pass / (kernel_ms * 1e-3) / 1e9
+    # what the kernel itself reads: its own visits per (relaxed, raw) step
+    own_bytes_per_step = own_nodes_per_step * scene.info.node_bytes + own_prims_per_step * scene.info.prim_bytes + 48.0
+    own_read_gbs = own_bytes_per_step * (total_steps / len(reports)) / (kernel_ms * 1e-3) / 1e9
     # measured read ceilings: a 16 MB buffer stays in the XCDs' L2 caches, a 4 GiB one streams HBM
     E.probe_read_bandwidth(16 << 20, 20)
     l2_gbs = E.probe_read_bandwidth(16 << 20, 400)
     hbm_gbs = E.probe_read_bandwidth(4 << 30, 4)
     log(f"read probe: L2-resident {l2_gbs:.0f} GB/s, HBM {hbm_gbs:.0f} GB/s")
-    # HBM/MALL traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # traffic past L2 per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # the same kernel configuration, scaled to this launch's step count
     traffic = None
     issue = {}
@@ -227,7 +284,7 @@ def main():
         if (c["bvh_width"], c["leaf_size"], c["distance_tolerance"]) == (args.bvh_width, leaf, tol):
             traffic = pj["traffic_bytes"] / pj["raw_steps"] * (total_steps / len(reports))
             issue = {"valu_busy": pj.get("valu_busy"), "valu_lane_utilisation": pj.get("valu_lane_utilisation"),
-                     "issue_source": pj["source"]}
+                     "write_bytes_per_launch": pj.get("write_bytes"), "issue_source": pj["source"]}
 
     # ---------------- learned control variate ----------------
     learned = None
@@ -243,41 +300,87 @@ def main():
         net = LC.MLP(LC.MLPConfig(in_dim=3, seed=args.seed), max_batch=args.cv_batch)
         losses = LC.train(net, X, Y, steps=args.cv_steps, batch=args.cv_batch, lr=1e-3, seed=args.seed,
                           log_every=max(1, args.cv_steps // 5))
+        torch.cuda.synchronize()
+        t_fit = time.perf_counter() - t0
         _, g0 = net.evaluate(P, value=False)
         grid = LC.bake_gradient_grid(net, prob.domain_min, prob.domain_max, args.cv_grid)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
-        log(f"learned cv: data {t_data:.1f}s, train+bake {t_train:.1f}s, losses {losses}")
+        train_flops = args.cv_steps * args.cv_batch * net.flops_per_sample()
+        log(f"learned cv: data {t_data:.1f}s, train {t_fit:.1f}s ({train_flops / t_fit / 1e12:.1f} TFLOP/s), "
+            f"train+bake {t_train:.1f}s, losses {losses}")
         dcv = E.DevicePoints(pt=P, dist_to_absorbing=dp.dist_to_absorbing, n_walks=dp.n_walks,
                              n_walks_max=args.walks, cv_gradient=g0.contiguous(), cv_gradient_grid=grid)
-        cv = timed(dcv, "cv", args.seed + 2)   # independent streams from the plain passes
-        cv_walks = sum(r.total_walks for r in cv["reports"])
-        cv_wps = cv_walks / cv["elapsed"]
-        var_cv = float(np.mean(cv["var"]))
+        cv_run = Progressive(dcv, args.seed + 2, "cv")   # independent streams from the plain passes (same kernel: warm)
+        cv_reports, cv_el = cv_run.run(args.steps)
+        cvh = cv_run.host()
+        cv_wps = sum(r.total_walks for r in cv_reports) / cv_el
+        var_cv = float(np.mean(cvh["var"]))
         s_to_rmse_cv = var_cv / TARGET_RMSE ** 2 * n / cv_wps + t_data + t_train
         # 4-SE agreement of the two estimators on a held-out 10k-point set
         h = slice(n - 10000, n)
-        se = np.sqrt(plain["var"][h] / np.maximum(plain["n"][h], 1) + cv["var"][h] / np.maximum(cv["n"][h], 1))
+        se = np.sqrt(plain["var"][h] / np.maximum(plain["n"][h], 1) + cvh["var"][h] / np.maximum(cvh["n"][h], 1))
         ok = se > 0
-        z = (plain["mean"][h][ok] - cv["mean"][h][ok]) / se[ok]
+        z = (plain["mean"][h][ok] - cvh["mean"][h][ok]) / se[ok]
+        rmse_cv_k = cv_run.rmse()
         learned = {
-            "train_data_s": t_data, "train_s": t_train, "final_loss": losses[-1][1] if losses else None,
+            "train_data_s": t_data, "train_s": t_train, "fit_s": t_fit, "final_loss": losses[-1][1] if losses else None,
+            "training_mfma": {"flops": train_flops, "flops_per_sample": net.flops_per_sample(),
+                              "achieved_tflops": train_flops / t_fit / 1e12,
+                              "peak_tflops": BF16_PEAK_TFLOPS, "frac": train_flops / t_fit / 1e12 / BF16_PEAK_TFLOPS,
+                              "note": "forward + backward + Adam of the 5k training steps (bf16 MFMA, fp32 "
+                                      "accumulate) over their wall time"},
             "per_walk_variance_plain": var_plain, "per_walk_variance_cv": var_cv,
             "variance_reduction": var_plain / var_cv, "walks_per_s_cv": cv_wps,
-            "rmse_vs_analytic_cv": float(np.sqrt(np.mean((cv["mean"] - exact) ** 2))),
+            "rmse_vs_analytic_cv": rmse_cv_k,
             "projected_s_to_rmse_1e-3_cv": s_to_rmse_cv,
-            "speedup_vs_plain_at_equal_rmse": s_to_rmse_plain / s_to_rmse_cv,
             "heldout_points": int(ok.sum()), "heldout_beyond_4se": int((np.abs(z) > 4).sum()),
             "heldout_z_mean": float(z.mean()), "heldout_z_std": float(z.std()),
-            "measured_s_to_rmse": None,
             "variate": "first step: exact grad c(x0) per point; later steps: grad c baked on a "
                        f"{args.cv_grid}^3 grid (trilinear); MLP 3 -> 64 Fourier features -> 4x128 SiLU -> 1, "
                        "bf16 MFMA, Adam lr 1e-3",
         }
 
-    if learned is not None and args.to_rmse > 0:
-        learned["measured_s_to_rmse"] = run_to_rmse(torch, args, E, scene, prob, dcv, opts, exact, n,
-                                                    learned["train_data_s"] + learned["train_s"])
+        # ---------------- the headline: time to RMSE 1e-3, measured ----------------
+        if args.to_rmse > 0:
+            rmse_cv = rmse_cv_k
+            while rmse_cv > args.to_rmse and cv_run.passes < args.to_rmse_max_passes:
+                cv_run.run(1)
+                rmse_cv = cv_run.rmse()
+            cv_total = cv_run.solve_s + t_data + t_train
+            learned["measured_to_rmse"] = {
+                "target_rmse": args.to_rmse, "reached": rmse_cv <= args.to_rmse, "rmse": rmse_cv,
+                "passes": cv_run.passes, "walks_per_point": cv_run.passes * args.walks,
+                "solve_s": cv_run.solve_s, "train_s": t_data + t_train, "total_s": cv_total,
+                "curve": cv_run.curve[:: max(1, len(cv_run.curve) // 12)] + cv_run.curve[-1:]}
+            log(f"learned cv reached rmse {rmse_cv:.3e} after {cv_run.passes} passes, {cv_total:.1f} s")
+            # plain walk on spheres for the same wall time (bounded by the bench's time budget),
+            # continuing the plain passes above; its measured RMSE, and the time plain needs for
+            # the target along its measured 1/t curve
+            left = args.budget_s - (time.time() - T_START) - 20.0   # keep room for the CPU leg
+            per_pass = plain_run.solve_s / max(plain_run.passes, 1)
+            target_s = min(cv_total, plain_run.solve_s + max(0.0, left))
+            extra = max(0, int(math.floor((target_s - plain_run.solve_s) / per_pass)))
+            for _ in range(extra):
+                plain_run.run(1)
+                if plain_run.passes % 8 == 0:
+                    plain_run.rmse()
+            rmse_plain = plain_run.rmse()
+            t_plain = plain_run.solve_s * (rmse_plain / args.to_rmse) ** 2
+            learned["plain_same_time"] = {
+                "solve_s": plain_run.solve_s, "passes": plain_run.passes,
+                "walks_per_point": plain_run.passes * args.walks, "rmse": rmse_plain,
+                "equal_wall_time": plain_run.solve_s >= 0.98 * cv_total,
+                "s_to_rmse_1e-3_on_measured_curve": t_plain,
+                "curve": plain_run.curve,
+                "note": "plain passes continue the timed ones; RMSE^2 falls as 1/t (no bias: "
+                        "rmse_sq_over_se_sq ~ 1), so the time to the target is the measured time times "
+                        "(rmse / target)^2"}
+            learned["speedup_vs_plain_at_equal_rmse"] = t_plain / cv_total
+            log(f"plain: {plain_run.solve_s:.1f} s -> rmse {rmse_plain:.3e}; plain to {args.to_rmse:g}: "
+                f"{t_plain:.1f} s; learned cv {cv_total:.1f} s; speed-up {t_plain / cv_total:.2f}")
+        else:
+            learned["speedup_vs_plain_at_equal_rmse"] = s_to_rmse_plain / s_to_rmse_cv
 
     result = {
         "metric": METRIC,
@@ -305,20 +408,28 @@ def main():
         "mean_walk_length_exact": len_exact,
         "rmse_vs_analytic": rmse,
         "estimated_se_rmse": se_rmse,
+        "rmse_sq_over_se_sq": rmse ** 2 / se2,
         "walks_per_point_done": int(plain["n"].mean()),
         "mean_walk_length": total_steps / max(total_walks, 1),
         "projected_s_to_rmse_1e-3_plain": s_to_rmse_plain,
-        "projected_s_to_rmse_1e-3_learned_cv": learned["projected_s_to_rmse_1e-3_cv"] if learned else None,
         "learned_cv": learned,
-        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+        "roofline": {"bound": "l2", "achieved": achieved_gbs, "peak": l2_gbs, "unit": "GB/s",
+                     "frac": achieved_gbs / l2_gbs, "traffic": traffic,
                      "bytes_per_step": bytes_per_step, "nodes_per_step": nodes_per_step,
-                     "prims_per_step": prims_per_step, "kernel_ms": kernel_ms, **issue,
-                     "l2_read_measured": l2_gbs, "hbm_read_measured": hbm_gbs, "frac_of_l2_read": achieved_gbs / l2_gbs,
-                     "note": "algorithmic bytes = BVH node + triangle bytes touched per step; the scene is "
-                             "cache-resident (L2/MALL), so HBM is not the binding level: the kernel is bound by "
-                             "VALU instruction issue under lane divergence (valu_busy, valu_lane_utilisation "
-                             "from the committed SQ counter profile)"},
+                     "prims_per_step": prims_per_step, "visit_count": visit_source,
+                     "exact_steps_per_launch": exact_steps_per_pass,
+                     "own_nodes_per_step": own_nodes_per_step, "own_prims_per_step": own_prims_per_step,
+                     "own_read_gbs": own_read_gbs, "own_read_frac_of_l2": own_read_gbs / l2_gbs,
+                     "kernel_ms": kernel_ms, **issue,
+                     "hbm_peak": HBM_PEAK_GBS, "hbm_read_measured": hbm_gbs,
+                     "traffic_past_l2_gbs": None if traffic is None else traffic / (kernel_ms * 1e-3) / 1e9,
+                     "hbm_frac": None if traffic is None else traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "note": "algorithmic bytes = BVH node + triangle bytes an exact nearest-first traversal "
+                             "from the root reads per step (counted at the walks' own positions) + 48 B of walk "
+                             "state; the 7 MB scene lives in L2 / MALL, so the binding memory level is L2 and "
+                             "peak is the measured L2 read ceiling; traffic past L2 is a fraction of HBM peak "
+                             "(hbm_frac); the kernel itself is bound by VALU issue under lane divergence "
+                             "(valu_busy, valu_lane_utilisation from the committed SQ counter profile)"},
     }
     if not args.no_cpu_baseline:
         cs = max(1, n // args.cpu_points)   # strided: spread over the domain whatever the point order
@@ -326,32 +437,6 @@ def main():
                                               dist[::cs][: args.cpu_points].cpu().numpy(), args.walks)
         result["speedup_vs_cpu_baseline"] = steps_per_s / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)
-
-
-def run_to_rmse(torch, args, E, scene, prob, points, opts, exact, n, t_train):
-    """Progressive learned-CV passes (fresh streams) until the RMSE of the running estimates
-    against the analytic solution reaches args.to_rmse; wall time of the passes plus the
-    training time."""
-    ex = torch.from_numpy(exact).to(points.pt.device)
-    stats = E.DeviceStats.zeros(n)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rmse, k = float("inf"), 0
-    history = []
-    while k < args.to_rmse_max_passes:
-        E.wos_solve(scene, prob.pde, prob.settings, points, stats, opts(k, seed=args.seed + 5))
-        k += 1
-        rmse = float(torch.sqrt(torch.mean((stats.mean - ex) ** 2)).item())
-        el = time.perf_counter() - t0
-        log(f"to-rmse pass {k}: {k * args.walks} walks/pt, {el:.1f} s, rmse {rmse:.3e}")
-        if k % 8 == 0 or rmse <= args.to_rmse:
-            history.append((k * args.walks, round(el, 2), rmse))
-        if rmse <= args.to_rmse:
-            break
-    el = time.perf_counter() - t0
-    return {"target_rmse": args.to_rmse, "reached": rmse <= args.to_rmse, "rmse": rmse, "passes": k,
-            "walks_per_point": k * args.walks, "solve_s": el, "train_s": t_train, "total_s": el + t_train,
-            "history": history}
 
 
 def cpu_baseline(prob, pts, dist, walks):
@@ -491,8 +576,9 @@ def bench_mixed2d(args, torch, dev):
 
     def opts(s, count=False, tol=None):
         return E.SolveOptions(seed=args.seed, stream_offset=s * walks, walks_per_lane=args.walks_per_lane,
-                              regroup_interval=args.regroup, count_visits=count,
-                              distance_tolerance=args.tolerance if tol is None else tol)
+                              regroup_interval=args.regroup, step_interval=args.wost_step_interval,
+                              count_visits=count, distance_tolerance=args.tolerance if tol is None else tol,
+                              wost_run_to_completion=args.wost_run_to_completion)
 
     # a pass of `walks` walks per point runs as progressive calls of `chunk` walks (fresh stream
     # offsets; equal to one call of `walks` walks), so a long pass reports its progress
@@ -535,6 +621,8 @@ def bench_mixed2d(args, torch, dev):
         "workload": f"2D WoSt, star curve 4,096 segments + 32 Neumann slits, {n} points, {walks} walks/pt/pass, "
                     "eps 1e-3", "bvh_width": args.bvh_width, "leaf_size": args.leaf_size,
         "walks_per_lane": args.walks_per_lane, "regroup_interval": args.regroup,
+        "step_interval": args.wost_step_interval,
+        "kernel": "run-to-completion" if args.wost_run_to_completion else "interleaved",
         "distance_tolerance": args.tolerance, "points": n, "point_order": args.point_order}, value, el, args,
         {"value_definition": "completed walks/s x mean walk length with exact distances (reference walk-steps/s)",
          "walks_per_s": walks_done / el, "raw_walk_steps_per_s": steps / el, "mean_walk_length_exact": len_exact,

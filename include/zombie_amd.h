@@ -208,6 +208,15 @@ zg_status zg_scene_get_info(const zg_scene* scene, zg_scene_info* info_host);
  * closest primitive (segment: (s.y,-s.x)/|s|; triangle: (b-a)x(c-a)/|.|, geometry_helpers.h
  * :15-21,77-81) and the index of that primitive in the caller's mesh.  Outputs nullable
  * except dist.  dist = FLT_MAX when the boundary is empty. */
+/* Copies the packed wide BVH of one boundary part (0 absorbing, 1 reflecting) to the host:
+ * nodes_host gets n_*_nodes * node_bytes / 4 floats, prims_host n_*_prims * prim_bytes / 4
+ * floats (zg_scene_info), in the device layout (lo[DIM][W], hi[DIM][W], child[W] as int32
+ * bits, padded to 64-byte multiples; primitives in leaf order).  Either pointer may be NULL.
+ * No counterpart in the reference (FCPW's BVH is internal): it lets a test or a profiler
+ * traverse exactly what the GPU traverses. */
+zg_status zg_scene_copy_bvh(const zg_scene* scene, int32_t boundary, float* nodes_host, int64_t node_floats,
+                            float* prims_host, int64_t prim_floats);
+
 zg_status zg_closest_point(const zg_scene* scene, int32_t boundary, int64_t n,
                            const float* pts, float* dist, float* closest, float* normal,
                            int32_t* prim, void* stream);
@@ -329,13 +338,21 @@ typedef struct zg_solve_options {
     int32_t regroup_interval; /* iterations between wavefront ballots that refill idle lanes */
     int32_t step_interval;    /* iterations (one BVH node visit each) between walk steps   */
     int32_t waves_per_cu;     /* persistent grid size = CUs * waves_per_cu (0 = auto)    */
-    int32_t count_visits;     /* 1: instrumented kernel counting BVH node/primitive visits */
+    int32_t count_visits;     /* 1: instrumented kernel counting the BVH node/primitive visits
+                                 of its own queries; 2: instead, at every query point, those of
+                                 an exact nearest-first traversal from the root (the query's
+                                 algorithmic work; walk on spheres only)                   */
     float distance_tolerance; /* tau in [0, 0.5): 0 = exact closest-point radii.  tau > 0 lets
                                  the distance query stop once every unexplored box lies within
                                  (1 - tau) of the best distance and walk with the certified
                                  lower bound instead (a ball inside the domain keeps walk on
                                  spheres unbiased; walks get slightly longer).  Queries within
                                  2 epsilon of the boundary and terminal projections stay exact. */
+    int32_t wost_run_to_completion; /* walk on stars: 0 (default) advances every lane's star-radius,
+                                 ray, visibility and distance queries one BVH item per loop
+                                 iteration, interleaved across the lanes of a wave; 1 runs each
+                                 query of a step to completion inside the step (same walks and
+                                 estimates; the kernel used with walk splitting and gradients) */
 } zg_solve_options;
 
 void zg_solve_options_default(zg_solve_options* opts);
@@ -353,7 +370,11 @@ typedef struct zg_solve_report {
 
 /* WalkOnSpheres<float,DIM>::solve (walk_on_spheres.h:131-168 -> estimateSolution :320-414,
  * walk :228-291, getTerminalContribution :293-318) for every point in one persistent launch.
- * report_host nullable; when given, the call synchronises `stream` and fills it.
+ * report_host nullable; when given, the call synchronises `stream` and fills it.  Without a
+ * report (and without gradient points) the call returns as soon as its work is enqueued on
+ * `stream`: the workspace is freed in stream order, the statistics are final once the stream
+ * has reached that point, and device-side errors (a traversal-stack overflow, an instance index
+ * outside the batch) are reported by the next zg_synchronize().
  * Points with estimation_quantity ZG_ESTIMATE_SKIP are left alone; points asking for
  * ZG_ESTIMATE_SOLUTION_AND_GRADIENT run estimateSolutionAndGradient (:416-593): first sphere of
  * radius 0.99 * dist_to_absorbing_boundary, Latin hypercube directions, antithetic pairs,
@@ -396,7 +417,12 @@ zg_status zg_wos_solve_noalloc(const zg_scene* scene, const zg_pde* pde_host,
  * first call and is reused while the point has estimates.  Gradient points run
  * estimateSolutionAndGradient (:664-848) as in zg_wos_solve, with a first ball of radius
  * 0.99 min(dist_to_absorbing_boundary, dist_to_reflecting_boundary) (the reflecting distance is
- * queried when dist_to_reflecting_boundary is NULL). */
+ * queried when dist_to_reflecting_boundary is NULL).  Walks split (applyWeightWindow,
+ * walk_on_stars.h:309-343: parts continue from the walk's state with throughput =
+ * splitting_threshold, the estimate is the sum of the parts, splits are counted in
+ * total_splits) when splitting_threshold < 1, or with Robin coefficients that may be negative
+ * (are_robin_coeffs_nonnegative = 0, reflectance clamped to [0, 2]) and a finite threshold;
+ * splitting with gradient points is ZG_ERR_UNSUPPORTED. */
 zg_status zg_wost_solve(const zg_scene* scene, const zg_pde* pde_host,
                         const zg_walk_settings* settings_host, const zg_sample_points* points_host,
                         zg_statistics* stats, const zg_solve_options* opts_host,
@@ -463,13 +489,15 @@ typedef struct zg_bvc_statistics {
 } zg_bvc_statistics;
 
 /* BoundaryValueCaching::splat over every sample for every evaluation point
- * (boundary_value_caching.h:370-402,683-810): free-space harmonic Green's function G, Poisson
- * kernel P and their gradients at r = max(radius_clamp, |y - x|), optional kernel regularization
+ * (boundary_value_caching.h:370-402,683-810): free-space Green's function G, Poisson kernel P
+ * and their gradients at r = max(radius_clamp, |y - x|) -- harmonic, or with absorption > 0 the
+ * 3D Yukawa kernels (YukawaGreensFnFreeSpace<3>, distributions.h:271-321; grad P is the
+ * derivative of P, where the reference's formula at :307-317 differs) -- optional kernel regularization
  * (distributions.h:12-55), the Robin / Neumann / Dirichlet forms of the boundary estimate,
  * alpha = 2 for evaluation points on the boundary (gradient 0 there); evaluation points closer
  * than the cutoffs to the boundaries are skipped; non-finite kernels skip the sample.  One
- * thread per evaluation point, samples staged through LDS.  absorption must be 0 (the harmonic
- * kernels; ZG_ERR_UNSUPPORTED otherwise). */
+ * thread per evaluation point, samples staged through LDS.  absorption >= 0; > 0 in 2D is
+ * ZG_ERR_UNSUPPORTED (screened Poisson runs in 3D). */
 zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* samples_host,
                        const zg_bvc_eval_points* eval_host, float radius_clamp, float kernel_regularization,
                        float robin_coeff_cutoff_for_normal_derivative, float cutoff_dist_to_absorbing_boundary,
@@ -564,6 +592,8 @@ zg_status zg_free(void* ptr);
 zg_status zg_memcpy_htod(void* dst, const void* src_host, size_t bytes);
 zg_status zg_memcpy_dtoh(void* dst_host, const void* src, size_t bytes);
 zg_status zg_memset(void* dst, int32_t value, size_t bytes);
+/* Waits for the device; reports (and clears) the device-side errors of solves that returned
+ * without synchronising. */
 zg_status zg_synchronize(void);
 
 /* Text of the last error on the calling thread (empty when none); returns its length. */

@@ -78,9 +78,10 @@ def lib():
                                          C.c_int64, P, P, P, P, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                          P, P, P, P, P, P, P, P, P]
         L.orc_sdf_solve_grad.restype = C.c_int
-        L.orc_bvc_splat.argtypes = [C.c_int, C.POINTER(A.zg_bvc_samples), C.POINTER(A.zg_bvc_eval_points), C.c_float,
-                                    C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P, P, C.c_int]
-        L.orc_free_space_kernels.argtypes = [C.c_int, P, P, P, C.c_float, C.c_float, P]
+        L.orc_bvc_splat.argtypes = [C.c_int, C.c_float, C.POINTER(A.zg_bvc_samples), C.POINTER(A.zg_bvc_eval_points),
+                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P, P, C.c_int]
+        L.orc_free_space_kernels.argtypes = [C.c_int, P, P, P, C.c_float, C.c_float, C.c_float, P]
+        L.orc_yukawa_ball.argtypes = [C.c_float, C.c_float, C.c_float, P]
         L.orc_stratified_samples.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_uint64, C.c_uint32, P]
         L.orc_pcg32_bounded.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, P]
         L.orc_greens_ball_gradient.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, P]
@@ -227,14 +228,24 @@ class SdfGrid:
         return dict(codes=rep[:4].copy(), total_steps=int(rep[4]))
 
 
-def free_space_kernels(dim, x, y, n, r, kreg=0.0):
-    """[G, P, dG, dP, G_reg, P_reg] of the free-space harmonic kernels (pole x, sample y, normal n)."""
+def free_space_kernels(dim, x, y, n, r, kreg=0.0, absorption=0.0):
+    """[G, P, dG, dP, G_reg, P_reg] of the free-space kernels (pole x, sample y, normal n): harmonic,
+    or in 3D with absorption > 0 the Yukawa kernels."""
     out = np.zeros(2 + 2 * dim + 2, np.float32)
     f = lambda a: np.ascontiguousarray(a, np.float32)
     xx, yy, nn = f(x), f(y), f(n)
-    lib().orc_free_space_kernels(dim, _p(xx), _p(yy), _p(nn), r, kreg, _p(out))
+    lib().orc_free_space_kernels(dim, _p(xx), _p(yy), _p(nn), r, kreg, absorption, _p(out))
     return dict(G=float(out[0]), P=float(out[1]), dG=out[2:2 + dim].copy(), dP=out[2 + dim:2 + 2 * dim].copy(),
                 G_reg=float(out[2 + 2 * dim]), P_reg=float(out[3 + 2 * dim]))
+
+
+def yukawa_ball(R, lam, r):
+    """The oracle's 3D Yukawa ball terms: norm, dampening(r), gradientNorm(r), Poisson-kernel
+    gradient scale, G(r), P (centred)."""
+    out = np.zeros(6, np.float32)
+    lib().orc_yukawa_ball(R, lam, r, _p(out))
+    return dict(norm=float(out[0]), dampening=float(out[1]), gradient_norm=float(out[2]),
+                pk_gradient_scale=float(out[3]), G=float(out[4]), P=float(out[5]))
 
 
 class BvcStats:
@@ -254,7 +265,7 @@ class BvcStats:
 
 
 def bvc_splat(dim, samples: dict, evals: dict, stats: BvcStats, radius_clamp=1e-3, kernel_regularization=0.0,
-              robin_cutoff=np.inf, cutoff_abs=0.0, cutoff_refl=0.0, threads=None):
+              robin_cutoff=np.inf, cutoff_abs=0.0, cutoff_refl=0.0, threads=None, absorption=0.0):
     """samples: pt, normal, type, normal_aligned, pdf, solution, normal_derivative, contribution, robin_coeff
     (numpy; the last two and normal_aligned optional); evals: pt, type, dist_abs, dist_refl."""
     keep = []
@@ -277,9 +288,11 @@ def bvc_splat(dim, samples: dict, evals: dict, stats: BvcStats, radius_clamp=1e-
     Ev.pt = arr(evals["pt"], np.float32); Ev.type = arr(evals.get("type"), np.uint8)
     Ev.dist_to_absorbing_boundary = arr(evals["dist_abs"], np.float32)
     Ev.dist_to_reflecting_boundary = arr(evals.get("dist_refl"), np.float32)
-    lib().orc_bvc_splat(dim, C.byref(S), C.byref(Ev), radius_clamp, kernel_regularization,
-                        float(min(robin_cutoff, 3.0e38)), cutoff_abs, cutoff_refl, _p(stats.n), _p(stats.mean),
-                        _p(stats.m2), _p(stats.grad_mean), _p(stats.grad_m2), threads or default_threads())
+    st = lib().orc_bvc_splat(dim, absorption, C.byref(S), C.byref(Ev), radius_clamp, kernel_regularization,
+                             float(min(robin_cutoff, 3.0e38)), cutoff_abs, cutoff_refl, _p(stats.n), _p(stats.mean),
+                             _p(stats.m2), _p(stats.grad_mean), _p(stats.grad_m2), threads or default_threads())
+    if st != 0:
+        raise RuntimeError(f"oracle splat refused its arguments (status {st})")
     return stats
 
 

@@ -243,6 +243,19 @@ inline void stratifiedSamples(int dims, int nSamples, Pcg32& rng, std::vector<fl
 // ------------------------------------------------------------------------------------
 // Green's functions of the ball (core/distributions.h:323-665, 835-1005)
 // ------------------------------------------------------------------------------------
+// sinh x and I(x) = cosh x - sinh x / x from e = e^{-x}, with Taylor series below the points
+// where the exponential forms lose their digits to cancellation in float (the reference's
+// updateBall, distributions.h:843-852, uses the exponential forms throughout: at x = 0.004
+// cosh x - sinh x / x keeps no correct digit, which turns its gradient terms into noise)
+inline float yukawaSinh(float x, float e) {
+    if (x < 0.1f) { float x2 = x * x; return x * (1.0f + x2 / 6.0f * (1.0f + x2 / 20.0f * (1.0f + x2 / 42.0f))); }
+    return (1.0f - e * e) / (2.0f * e);
+}
+inline float yukawaI(float x, float e) {
+    if (x < 0.5f) { float x2 = x * x; return x2 * (1.0f / 3.0f + x2 * (1.0f / 30.0f + x2 * (1.0f / 840.0f + x2 / 45360.0f))); }
+    return (1.0f + e * e) / (2.0f * e) - yukawaSinh(x, e) / x;
+}
+
 template <int D>
 struct GreensBall {
     bool yukawa = false;
@@ -261,11 +274,9 @@ struct GreensBall {
         if (yukawa) {
             muR = R * sqrtLambda;
             expmuR = std::exp(-muR);
-            float exp2muR = expmuR * expmuR;
-            float coshmuR = (1.0f + exp2muR) / (2.0f * expmuR);
-            sinhmuR = (1.0f - exp2muR) / (2.0f * expmuR);
+            sinhmuR = yukawaSinh(muR, expmuR);
             K32muR = expmuR * (1.0f + 1.0f / muR);
-            I32muR = coshmuR - sinhmuR / muR;
+            I32muR = yukawaI(muR, expmuR);
         }
     }
 
@@ -276,7 +287,7 @@ struct GreensBall {
         }
         float mur = r * sqrtLambda;
         float expmur = std::exp(-mur);
-        float sinhmur = (1.0f - expmur * expmur) / (2.0f * expmur);
+        float sinhmur = yukawaSinh(mur, expmur);
         return (expmur - expmuR * sinhmur / sinhmuR) / (4.0f * PI_F * r);
     }
 
@@ -287,6 +298,10 @@ struct GreensBall {
 
     float norm() const {
         if (!yukawa) return D == 2 ? R * R / 4.0f : R * R / 6.0f;
+        if (muR < 0.5f) {   // 1 - x / sinh x = x^2/6 - 7 x^4/360 + 31 x^6/15120 - 127 x^8/604800
+            float x2 = muR * muR;
+            return R * R * (1.0f / 6.0f - x2 * (7.0f / 360.0f - x2 * (31.0f / 15120.0f - x2 * (127.0f / 604800.0f))));
+        }
         float lambda = sqrtLambda * sqrtLambda;
         return (1.0f - 4.0f * PI_F * poissonKernel()) / lambda;
     }
@@ -294,11 +309,8 @@ struct GreensBall {
     float poissonKernelDampeningFactor(float r) const {
         float mur = r * sqrtLambda;
         float expmur = std::exp(-mur);
-        float exp2mur = expmur * expmur;
-        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
-        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
         float K32mur = expmur * (1.0f + 1.0f / mur);
-        float I32mur = coshmur - sinhmur / mur;
+        float I32mur = yukawaI(mur, expmur);
         float Q = K32mur + I32mur * expmuR / sinhmuR;
         return mur * Q;
     }
@@ -325,7 +337,7 @@ struct GreensBall {
             float P = dot(n, dir) / (r * r);
             float mur = r * sqrtLambda;
             float expmur = std::exp(-mur);
-            float sinhmur = (1.0f - expmur * expmur) / (2.0f * expmur);
+            float sinhmur = yukawaSinh(mur, expmur);
             float G = (expmur - expmuR * sinhmur / sinhmuR) / r;
             return Q - kappa * G / P;
         }
@@ -341,11 +353,8 @@ struct GreensBall {
         }
         float mur = r * sqrtLambda;
         float expmur = std::exp(-mur);
-        float exp2mur = expmur * expmur;
-        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
-        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
         float K32mur = expmur * (1.0f + 1.0f / mur);
-        float I32mur = coshmur - sinhmur / mur;
+        float I32mur = yukawaI(mur, expmur);
         float Qr = sqrtLambda * (K32mur - I32mur * K32muR / I32muR);
         return Qr / (4.0f * PI_F * r * r);
     }
@@ -2120,8 +2129,15 @@ template <int D>
 struct FreeSpaceKernels {
     float G, P;
     V<D> dG, dP;
-    // pole x, sample y with normal n, r = clamped distance
-    void eval(const V<D>& x, const V<D>& y, const V<D>& n, float r) {
+    // pole x, sample y with normal n, r = clamped distance; mu = sqrt(absorption) > 0 selects the
+    // 3D Yukawa kernels (YukawaGreensFnFreeSpace<3>, distributions.h:271-321):
+    //   G = e^{-mu r} / (4 pi r),  grad_x G = -(x - y) e^{-mu r} (mu r + 1) / (4 pi r^3),
+    //   P = n.(x - y) e^{-mu r} (mu r + 1) / (4 pi r^3),
+    //   grad_x P = [n e^{-mu r} (mu r + 1) - (x - y) (n.(x - y) / r^2) e^{-mu r} (mu^2 r^2 + 3 mu r + 3)]
+    //              / (4 pi r^3),
+    // the last one the derivative of P; the reference's :307-317 writes (2 mu r + 3) for the
+    // factor (mu^2 r^2 + 3 mu r + 3), which is not the derivative of its own P (both agree at mu = 0)
+    void eval(const V<D>& x, const V<D>& y, const V<D>& n, float r, float mu = 0.0f) {
         V<D> xy = x - y;
         float r2 = r * r;
         if (D == 2) {
@@ -2131,10 +2147,13 @@ struct FreeSpaceKernels {
             dP = (n - xy * (2.0f * (dot(n, xy) / r2))) * (1.0f / (2.0f * PI_F * r2));
         } else {
             float r3 = r2 * r;
-            G = 1.0f / (4.0f * PI_F * r);
-            dG = xy * (-1.0f / (4.0f * PI_F * r3));
-            P = dot(n, xy) / (4.0f * PI_F * r3);
-            dP = (n - xy * (3.0f * (dot(n, xy) / r2))) * (1.0f / (4.0f * PI_F * r3));
+            float e = std::exp(-mu * r), mr = mu * r;
+            float q1 = e * (mr + 1.0f);
+            G = e / (4.0f * PI_F * r);
+            dG = xy * (-q1 / (4.0f * PI_F * r3));
+            P = dot(n, xy) * q1 / (4.0f * PI_F * r3);
+            float q2 = e * (mr * mr + 3.0f * mr + 3.0f);
+            dP = (n * q1 - xy * ((dot(n, xy) / r2) * q2)) * (1.0f / (4.0f * PI_F * r3));
         }
     }
 };
@@ -2155,7 +2174,7 @@ struct BvcAcc {   // Welford per category
 
 template <int D>
 void bvcSplatPoint(const zg_bvc_samples& S, const V<D>& x, bool evalOnBoundary, float radiusClamp, float kreg,
-                   float robinCutoff, BvcAcc acc[5]) {
+                   float robinCutoff, float mu, BvcAcc acc[5]) {
     const float alpha = evalOnBoundary ? 2.0f : 1.0f;
     const float alphaG = evalOnBoundary ? 0.0f : alpha;
     for (int64_t j = 0; j < S.n; j++) {
@@ -2167,7 +2186,7 @@ void bvcSplatPoint(const zg_bvc_samples& S, const V<D>& x, bool evalOnBoundary, 
         FreeSpaceKernels<D> k;
         V<D> n = V<D>::zero();
         if (ty != ZG_SAMPLE_IN_DOMAIN) n = load<D>(S.normal + j * D) * (aligned ? -1.0f : 1.0f);
-        k.eval(x, y, n, r);
+        k.eval(x, y, n, r, mu);
         double est, grad[3];
         if (ty == ZG_SAMPLE_IN_DOMAIN) {
             float dGn = norm(k.dG);
@@ -2495,9 +2514,11 @@ void orc_sdf_ray(int dim, const float* grid, const int32_t* shape, const float* 
 
 // BoundaryValueCaching::splat of a sample cache into evaluation points (host arrays); stats are
 // Chan-merged like zg_bvc_statistics
-int orc_bvc_splat(int dim, const zg_bvc_samples* S, const zg_bvc_eval_points* E, float radiusClamp, float kreg,
-                  float robinCutoff, float cutoffA, float cutoffR, int64_t* sn, double* smean, double* sm2,
-                  double* sgm, double* sg2, int nThreads) {
+int orc_bvc_splat(int dim, float absorption, const zg_bvc_samples* S, const zg_bvc_eval_points* E,
+                  float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR, int64_t* sn,
+                  double* smean, double* sm2, double* sgm, double* sg2, int nThreads) {
+    if (absorption < 0.0f || (absorption > 0.0f && dim != 3)) return 2;   // screened kernels in 3D only
+    const float mu = std::sqrt(absorption);
     parallelFor(E->n, std::max(1, nThreads), [&](int64_t i, int) {
         float dA = E->dist_to_absorbing_boundary[i];
         float dR = E->dist_to_reflecting_boundary ? E->dist_to_reflecting_boundary[i] : FLT_MAX;
@@ -2505,8 +2526,8 @@ int orc_bvc_splat(int dim, const zg_bvc_samples* S, const zg_bvc_eval_points* E,
         uint8_t ty = E->type ? E->type[i] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
         bool onB = ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY || ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY;
         BvcAcc acc[5];
-        if (dim == 2) bvcSplatPoint<2>(*S, load<2>(E->pt + 2 * i), onB, radiusClamp, kreg, robinCutoff, acc);
-        else bvcSplatPoint<3>(*S, load<3>(E->pt + 3 * i), onB, radiusClamp, kreg, robinCutoff, acc);
+        if (dim == 2) bvcSplatPoint<2>(*S, load<2>(E->pt + 2 * i), onB, radiusClamp, kreg, robinCutoff, 0.0f, acc);
+        else bvcSplatPoint<3>(*S, load<3>(E->pt + 3 * i), onB, radiusClamp, kreg, robinCutoff, mu, acc);
         for (int c = 0; c < 5; c++) {
             const BvcAcc& a = acc[c];
             if (a.n == 0) continue;
@@ -2525,15 +2546,32 @@ int orc_bvc_splat(int dim, const zg_bvc_samples* S, const zg_bvc_eval_points* E,
     });
     return 0;
 }
+// the 3D Yukawa ball of radius R at distance r from its centre, for pins: [norm, dampening(r),
+// gradientNorm(r), poissonKernelGradient scale, evaluate(r), poissonKernel]
+void orc_yukawa_ball(float R, float lambda, float r, float* out) {
+    GreensBall<3> g;
+    g.makeYukawa(lambda);
+    g.updateBall(V<3>::zero(), R);
+    V<3> e = V<3>::zero();
+    e[0] = 1.0f;
+    out[0] = g.norm();
+    out[1] = g.poissonKernelDampeningFactor(r);
+    out[2] = g.gradientNorm(r);
+    out[3] = g.poissonKernelGradient(e)[0];
+    out[4] = g.evaluate(r);
+    out[5] = g.poissonKernel();
+}
+
 // free-space kernels at (x, y, n, r) for pins: out = [G, P, dG..., dP...], regularized G and P in
 // out[2 + 2 dim], out[3 + 2 dim] (kreg > 0)
-void orc_free_space_kernels(int dim, const float* x, const float* y, const float* n, float r, float kreg, float* out) {
+void orc_free_space_kernels(int dim, const float* x, const float* y, const float* n, float r, float kreg,
+                            float absorption, float* out) {
     if (dim == 2) {
         FreeSpaceKernels<2> k; k.eval(load<2>(x), load<2>(y), load<2>(n), r);
         out[0] = k.G; out[1] = k.P; store<2>(out + 2, k.dG); store<2>(out + 4, k.dP);
         out[6] = kreg > 0 ? k.G * regularizationG<2>(r / kreg) : k.G; out[7] = kreg > 0 ? k.P * regularizationP<2>(r / kreg) : k.P;
     } else {
-        FreeSpaceKernels<3> k; k.eval(load<3>(x), load<3>(y), load<3>(n), r);
+        FreeSpaceKernels<3> k; k.eval(load<3>(x), load<3>(y), load<3>(n), r, std::sqrt(std::max(absorption, 0.0f)));
         out[0] = k.G; out[1] = k.P; store<3>(out + 2, k.dG); store<3>(out + 5, k.dP);
         out[8] = kreg > 0 ? k.G * regularizationG<3>(r / kreg) : k.G; out[9] = kreg > 0 ? k.P * regularizationP<3>(r / kreg) : k.P;
     }

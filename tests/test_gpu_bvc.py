@@ -74,8 +74,13 @@ def test_splat_argument_errors(gpu):
                      pdf=torch.ones(4, device="cuda"), solution=torch.zeros(4, device="cuda"),
                      normal_derivative=torch.zeros(4, device="cuda"))
     ev = E.BvcEvalPoints(pt=z, dist_to_absorbing=torch.ones(4, device="cuda"))
+    s2 = E.BvcSamples(pt=z[:, :2].contiguous(), normal=z[:, :2].contiguous(), type=s.type, pdf=s.pdf,
+                      solution=s.solution, normal_derivative=s.normal_derivative)
+    ev2 = E.BvcEvalPoints(pt=z[:, :2].contiguous(), dist_to_absorbing=ev.dist_to_absorbing)
+    with pytest.raises(A.ZombieError):            # screened kernels are 3D only
+        E.bvc_splat(2, s2, ev2, E.BvcStats.zeros(4, 2), absorption=1.0)
     with pytest.raises(A.ZombieError):
-        E.bvc_splat(3, s, ev, E.BvcStats.zeros(4, 3), absorption=1.0)
+        E.bvc_splat(3, s, ev, E.BvcStats.zeros(4, 3), absorption=-1.0)
     with pytest.raises(A.ZombieError):
         E.bvc_splat(4, s, ev, E.BvcStats.zeros(4, 3))
 
@@ -201,3 +206,82 @@ def test_bvc_solver_3d_mixed_neumann(gpu, oracle_lib):
     print(f"bvc 3d mixed: rmse {rmse:.4g}, mean error {err.mean():.3g}, median se {np.median(se):.3g}")
     assert rmse < 4 * np.sqrt(np.mean(se ** 2)) + 0.01
     assert abs(err.mean()) < 3 * np.median(se) + 0.005
+
+
+@pytest.mark.parametrize("kreg", [0.0, 0.05])
+def test_screened_splat_matches_oracle(gpu, oracle_lib, kreg):
+    """3D Yukawa kernels (absorption > 0): the GPU splat equals the oracle's, whose kernels are
+    pinned by finite differences and the screened Green's representation."""
+    torch = gpu
+    O = oracle_lib
+    dim, lam = 3, 3.0
+    rng = np.random.default_rng(77 + int(kreg * 100))
+    S = _random_cache(rng, dim, 3000)
+    m = 1200
+    x = rng.normal(size=(m, dim)) * 0.5
+    dA = rng.uniform(0.0, 1.0, m)
+    ost = O.BvcStats(m, dim)
+    O.bvc_splat(dim, S, dict(pt=x, dist_abs=dA), ost, radius_clamp=1e-3, kernel_regularization=kreg,
+                robin_cutoff=1.0, cutoff_abs=0.05, absorption=lam)
+    samples = E.BvcSamples(pt=_t(torch, S["pt"]), normal=_t(torch, S["normal"]), type=_t(torch, S["type"], np.uint8),
+                           pdf=_t(torch, S["pdf"]), solution=_t(torch, S["solution"]),
+                           normal_derivative=_t(torch, S["normal_derivative"]), contribution=_t(torch, S["contribution"]),
+                           robin_coeff=_t(torch, S["robin_coeff"]), normal_aligned=_t(torch, S["normal_aligned"], np.uint8))
+    ev = E.BvcEvalPoints(pt=_t(torch, x), dist_to_absorbing=_t(torch, dA))
+    st = E.BvcStats.zeros(m, dim)
+    E.bvc_splat(dim, samples, ev, st, radius_clamp=1e-3, kernel_regularization=kreg, robin_coeff_cutoff=1.0,
+                cutoff_dist_to_absorbing=0.05, absorption=lam)
+    assert np.array_equal(st.n.cpu().numpy(), ost.n)
+    scale = np.abs(ost.mean).max() + 1
+    np.testing.assert_allclose(st.mean.cpu().numpy(), ost.mean, rtol=2e-4, atol=2e-5 * scale)
+    gscale = np.abs(ost.grad_mean).max() + 1
+    np.testing.assert_allclose(st.grad_mean.cpu().numpy(), ost.grad_mean, rtol=2e-4, atol=2e-5 * gscale)
+    # the screened kernels differ from the harmonic ones
+    st0 = E.BvcStats.zeros(m, dim)
+    E.bvc_splat(dim, samples, ev, st0, radius_clamp=1e-3, kernel_regularization=kreg, robin_coeff_cutoff=1.0,
+                cutoff_dist_to_absorbing=0.05)
+    assert np.abs(st0.mean.cpu().numpy() - ost.mean).max() > 1e-3 * scale
+
+
+def test_bvc_solver_3d_screened_dirichlet(gpu):
+    """Screened Poisson -Lap u + lambda u = 0 on the unit icosphere with u = exp(a.x), |a|^2 =
+    lambda: cached boundary estimates (GPU walk on spheres with the Yukawa ball) splatted with the
+    Yukawa free-space kernels (boundary_value_caching.h:386-387,422-423) reproduce u."""
+    torch = gpu
+    lam = 4.0
+    a = np.array([0.8, -1.1, 1.2]); a = a / np.linalg.norm(a) * np.sqrt(lam)
+    V, F = PB.icosphere(4)
+    V = V.astype(np.float32)
+    sc = E.Scene(V, F.astype(np.int32), None, V.min(0) - 1e-3, V.max(0) + 1e-3)
+    pde = E.make_pde(source=E.zero(), dirichlet=E.function(A.ZG_FN_EXP_LINEAR, A.ZG_EVAL_VALUE, (1.0, *a)),
+                     absorption=lam, is_source_constant=True)
+    ws = E.walk_settings_host(1e-4, 1e-4, 1024)
+    solver = BV.BoundaryValueCachingSolver(sc, BV.UniformBoundarySampler(V, F), None, None, seed=5)
+    solver.generate_samples(40000, 0, 0, 2e-3, 0.0, False)
+    print("bvc screened: boundary estimates", flush=True)
+    solver.compute_sample_estimates(pde, ws, 64, 64)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(2000, 3)); x = x / np.linalg.norm(x, axis=1, keepdims=True) * rng.random((2000, 1)) ** (1 / 3) * 0.7
+    xt = torch.from_numpy(x.astype(np.float32)).cuda()
+    d = sc.closest_point(xt, want_point=False)[0]
+    ev = E.BvcEvalPoints(pt=xt, dist_to_absorbing=d.contiguous())
+    st = E.BvcStats.zeros(len(x), 3)
+    solver.splat(ev, st, radius_clamp=1e-3)
+    u = st.solution().cpu().numpy()
+    exact = np.exp(x.astype(np.float64) @ a)
+    err = u - exact
+    rmse = float(np.sqrt(np.mean(err ** 2)))
+    se = _splat_se(st)
+    spread = float(np.sqrt(np.mean((err - err.mean()) ** 2)))
+    print(f"bvc 3d screened: rmse {rmse:.4g}, mean error {err.mean():.3g}, spread {spread:.3g}, "
+          f"median se {np.median(se):.3g}", flush=True)
+    # the sample set is shared by all points: a common offset within the standard error, a small spread
+    assert abs(err.mean()) < 3 * np.median(se) + 0.005
+    assert spread < 0.04
+    # the harmonic kernels would not reproduce the screened solution
+    solver.absorption = 0.0
+    st0 = E.BvcStats.zeros(len(x), 3)
+    solver.splat(ev, st0, radius_clamp=1e-3)
+    rmse0 = float(np.sqrt(np.mean((st0.solution().cpu().numpy() - exact) ** 2)))
+    print(f"bvc 3d screened with harmonic kernels: rmse {rmse0:.4g}", flush=True)
+    assert rmse0 > 2 * rmse

@@ -37,6 +37,26 @@ def test_mlp_loss_and_parameter_gradients_match_oracle(gpu, net):
     assert _rel(g, g64) < 0.05, _rel(g, g64)    # bf16 operands vs the exact model
 
 
+def test_ragged_shapes_match_oracle(gpu):
+    """Widths that are not multiples of the 64-column lane chunks or of the 32-deep MFMA k-tile
+    (hidden 200, 24 Fourier features), a 2D input and a batch that is not a multiple of a tile."""
+    torch = gpu
+    net = LC.MLP(LC.MLPConfig(in_dim=2, n_frequencies=12, hidden=200, n_hidden_layers=2, frequency_sigma=3.0, seed=11),
+                 max_batch=2048)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, size=(999, 2)).astype(np.float32)
+    y = np.sin(3 * x).prod(1).astype(np.float32)
+    loss, g = net.gradients(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
+    theta, B = net.get_parameters(), net.frequencies()
+    lb, gb = M.gradients(theta, B, x, y, 200, 2, bf16=True)
+    assert abs(loss - lb) < 1e-3 * abs(lb) + 1e-6
+    assert _rel(g, gb) < 0.02, _rel(g, gb)
+    v, gx = net.evaluate(torch.from_numpy(x).cuda())
+    cb, gxb = M.input_gradient(theta, B, x, 200, 2, bf16=True)
+    assert _rel(v.cpu().numpy(), cb) < 0.01
+    assert _rel(gx.cpu().numpy(), gxb) < 0.02
+
+
 def test_mlp_value_and_input_gradient_match_oracle(gpu, net):
     torch = gpu
     rng = np.random.default_rng(2)

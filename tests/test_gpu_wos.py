@@ -308,3 +308,26 @@ def test_walks_of_a_point_are_independent(gpu, c1):
     m = np.array(means).mean(0)
     pooled = (m - u).sum() / np.sqrt((np.array(se2).mean(0) / 16).sum())
     assert abs(pooled) < 4.0, pooled
+
+
+def test_async_solves_overlap_and_match_synchronous(gpu, ball3):
+    """Without a report a solve returns once enqueued (no host wait): two solves on two streams,
+    then zg_synchronize, give the statistics of the same solves run synchronously."""
+    torch = gpu
+    p, pts, d, _ = ball3
+    pts, d = pts[:600], d[:600]
+    sc = E.Scene(p.positions, p.indices, None, p.domain_min, p.domain_max)
+    dp = _device_points(torch, pts, d, 64)
+    ref = [E.DeviceStats.zeros(len(pts)) for _ in range(2)]
+    for k in range(2):
+        E.wos_solve(sc, p.pde, p.settings, dp, ref[k], E.SolveOptions(seed=70 + k))
+    out = [E.DeviceStats.zeros(len(pts)) for _ in range(2)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert E.wos_solve(sc, p.pde, p.settings, dp, out[k], E.SolveOptions(seed=70 + k), stream=streams[k],
+                           report=False) is None
+    E.synchronize()
+    for k in range(2):
+        assert torch.equal(out[k].n, ref[k].n) and torch.equal(out[k].walk_length, ref[k].walk_length)
+        np.testing.assert_allclose(out[k].mean.cpu().numpy(), ref[k].mean.cpu().numpy(), rtol=1e-12, atol=1e-12)

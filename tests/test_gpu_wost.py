@@ -268,9 +268,12 @@ def test_wost_argument_errors(gpu, mixed):
     screened = E.make_pde(dirichlet=p.pde.dirichlet, robin=p.pde.robin, absorption=1.0)
     with pytest.raises(A.ZombieError):   # screened Poisson is 3D only
         E.wost_solve(sc, screened, p.settings, dp, E.DeviceStats.zeros(8))
-    with pytest.raises(A.ZombieError):   # splitting runs in walk on spheres only
-        E.wost_solve(sc, p.pde, E.walk_settings_host(1e-3, 1e-3, 1024, True, splitting_threshold=0.5), dp,
-                     E.DeviceStats.zeros(8))
+    # splitting with gradient points runs in walk on spheres only
+    dq = _device_points(torch, pts[:8], d[:8], 4)
+    dq.estimation_quantity = torch.full((8,), A.ZG_ESTIMATE_SOLUTION_AND_GRADIENT, dtype=torch.uint8, device="cuda")
+    with pytest.raises(A.ZombieError, match="splitting"):
+        E.wost_solve(sc, p.pde, E.walk_settings_host(1e-3, 1e-3, 1024, True, splitting_threshold=0.5), dq,
+                     E.DeviceStats.zeros(8, dim=2))
     s3 = PB.sphere3d(level=2)
     sc3 = E.Scene(s3.positions, s3.indices, None, s3.domain_min, s3.domain_max)
     with pytest.raises(ValueError):      # 2D points for a 3D scene: caught before the C-ABI call
@@ -346,3 +349,73 @@ def test_robin_wost_parity_with_oracle_and_closed_form(gpu, oracle_lib, robin):
     ml_g, ml_o = g["len"].sum() / g["n"].sum(), ost.walk_length.sum() / ost.n.sum()
     assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)
     assert rep.completion_codes[3] <= 1e-4 * rep.total_walks + 2
+
+
+@pytest.mark.parametrize("thr", [0.5, 0.3])
+def test_wost_splitting_matches_oracle_and_closed_form(gpu, oracle_lib, mixed, thr):
+    """applyWeightWindow's splitting in walk on stars (walk_on_stars.h:309-343): with a threshold
+    below 1 every walk splits after its first step into parts of throughput thr (one extra part
+    at 0.5; two or three at 0.3), each continuing from the walk's state; the estimate is the sum
+    of its parts.  GPU vs the oracle (which splits with the reference's queue) and vs u."""
+    torch = gpu
+    O = oracle_lib
+    p, osc, pts, d = mixed
+    pts, d = pts[:300], d[:300]
+    ws = A.zg_walk_settings.from_buffer_copy(p.settings)
+    ws.splitting_threshold = thr
+    sc = _scene(p, 4)
+    st = E.DeviceStats.zeros(len(pts))
+    print(f"wost splitting: threshold {thr}", flush=True)
+    rep = E.wost_solve(sc, p.pde, ws, _device_points(torch, pts, d, 128), st, E.SolveOptions(seed=61))
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    orep = O.solve(osc, p.pde, ws, pts, d, 128, ost, seed=62, wost=True, rng_mode=O.RNG_PER_POINT)
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, f"wost splitting {thr} gpu vs oracle")
+    exact = p.exact(pts)
+    pooled = (g["mean"] - exact).sum() / np.sqrt((g["var"] / g["n"]).sum())
+    assert abs(pooled) < 4.0, pooled
+    splits = st.splits.cpu().numpy()
+    walks = g["n"].sum()
+    if thr == 0.5:
+        assert abs(splits.sum() - walks) <= rep.completion_codes[3] + 1e-3 * walks
+    else:
+        # 1 - 0.3 = 0.7 = 2 x 0.3 + 0.1: two copies, a third with probability 1/3
+        assert abs(splits.sum() / walks - (2 + 1 / 3)) < 0.03
+    assert abs(splits.sum() / max(ost.splits.sum(), 1) - 1.0) < 0.03
+    assert rep.total_walks == 128 * len(pts)
+
+
+def _same_walks(torch, sc, p, pts, d, seed, n_walks=128, tol=0.2):
+    """One solve with each walk-on-stars kernel on the same walk streams."""
+    out = []
+    for rtc in (False, True):
+        st = E.DeviceStats.zeros(len(pts))
+        rep = E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, n_walks), st,
+                           E.SolveOptions(seed=seed, distance_tolerance=tol, wost_run_to_completion=rtc))
+        out.append((rep, _host(st)))
+    return out
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("case", ["neumann", "robin"])
+def test_interleaved_kernel_walks_equal_run_to_completion(gpu, mixed, robin, width, case):
+    """The interleaved kernel visits the BVH items of a step's queries in another order, which
+    leaves every query's answer unchanged: walk for walk the same steps, completion codes and
+    estimates as the run-to-completion kernel (FP64 sums agree up to their accumulation order)."""
+    torch = gpu
+    p, osc, pts, d = mixed if case == "neumann" else robin
+    pts, d = pts[:300], d[:300]
+    sc = _scene(p, width)
+    if case == "robin":
+        sc.set_robin_coefficients(*p.meta["robin"])
+    (ri, gi), (rr, gr) = _same_walks(torch, sc, p, pts, d, seed=61)
+    print(f"interleaved {case} w{width}: steps {ri.total_steps} vs {rr.total_steps}, codes {ri.completion_codes} "
+          f"vs {rr.completion_codes}, kernel {ri.kernel_ms:.1f} vs {rr.kernel_ms:.1f} ms", flush=True)
+    assert ri.total_walks == rr.total_walks == 128 * len(pts)
+    assert ri.completion_codes == rr.completion_codes
+    assert ri.total_steps == rr.total_steps
+    np.testing.assert_array_equal(gi["n"], gr["n"])
+    np.testing.assert_array_equal(gi["len"], gr["len"])
+    np.testing.assert_allclose(gi["mean"], gr["mean"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(gi["m2"], gr["m2"], rtol=1e-7, atol=1e-9)

@@ -299,9 +299,52 @@ def test_wost3d_rejections(gpu):
     d = sc.closest_point(x, want_point=False)[0].contiguous()
     dp = E.DevicePoints(pt=x, dist_to_absorbing=d, n_walks=torch.full((4,), 8, dtype=torch.int32, device="cuda"),
                         n_walks_max=8)
-    with pytest.raises(A.ZombieError):      # splitting runs in walk on spheres only
-        E.wost_solve(sc, p.pde, E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5), dp,
-                     E.DeviceStats.zeros(4))
+    # splitting with gradient points runs in walk on spheres only
+    dq = E.DevicePoints(pt=x, dist_to_absorbing=d, n_walks=dp.n_walks, n_walks_max=8,
+                        estimation_quantity=torch.full((4,), A.ZG_ESTIMATE_SOLUTION_AND_GRADIENT, dtype=torch.uint8,
+                                                       device="cuda"))
+    with pytest.raises(A.ZombieError, match="splitting"):
+        E.wost_solve(sc, p.pde, E.walk_settings_host(1e-3, 1e-3, 64, splitting_threshold=0.5), dq,
+                     E.DeviceStats.zeros(4, dim=3))
     with pytest.raises(ValueError):       # 2D points for a 3D scene
         E.wost_solve(sc, p.pde, p.settings, E.DevicePoints(pt=x[:, :2].contiguous(), dist_to_absorbing=d,
                                                            n_walks=dp.n_walks, n_walks_max=8), E.DeviceStats.zeros(4))
+
+
+@pytest.mark.parametrize("case", ["closed", "plate", "robin", "screened"])
+def test_interleaved_kernel_walks_equal_run_to_completion(gpu, oracle_lib, case):
+    """3D walk on stars: the interleaved kernel runs the same walks as the run-to-completion
+    kernel (same steps, completion codes and estimates; FP64 sums up to their accumulation order)."""
+    torch = gpu
+    O = oracle_lib
+    if case in ("closed", "plate"):
+        p = PB.lprism3d(plate=case == "plate")
+    elif case == "robin":
+        p = PB.lprism3d_robin(kappa=2.0, plate=True)
+    else:
+        p = PB.lprism3d_screened(sigma=4.0, plate=True)
+    osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max,
+                  solve_double_sided=p.solve_double_sided)
+    sc = _scene(p, 4)
+    if "robin" in p.meta:
+        sc.set_robin_coefficients(*p.meta["robin"])
+    pts = _interior(p, 200, 29)
+    d = osc.closest_point(pts, boundary=0)[0]
+    n = len(pts)
+    res = []
+    for rtc in (False, True):
+        st = E.DeviceStats.zeros(n)
+        dp = E.DevicePoints(pt=_t(torch, pts), dist_to_absorbing=_t(torch, d),
+                            n_walks=torch.full((n,), 128, dtype=torch.int32, device="cuda"), n_walks_max=128)
+        rep = E.wost_solve(sc, p.pde, p.settings, dp, st,
+                           E.SolveOptions(seed=71, distance_tolerance=0.2, wost_run_to_completion=rtc))
+        res.append((rep, _host(st)))
+    (ri, gi), (rr, gr) = res
+    print(f"interleaved 3d {case}: steps {ri.total_steps} vs {rr.total_steps}, codes {ri.completion_codes} vs "
+          f"{rr.completion_codes}, kernel {ri.kernel_ms:.1f} vs {rr.kernel_ms:.1f} ms", flush=True)
+    assert ri.total_walks == rr.total_walks == 128 * n
+    assert ri.completion_codes == rr.completion_codes
+    assert ri.total_steps == rr.total_steps
+    np.testing.assert_array_equal(gi["n"], gr["n"])
+    np.testing.assert_array_equal(gi["len"], gr["len"])
+    np.testing.assert_allclose(gi["mean"], gr["mean"], rtol=1e-9, atol=1e-9)

@@ -135,3 +135,66 @@ def test_splat_source_term_and_cutoffs(oracle_lib):
     # se of the source estimate is the statistic's own
     se = np.sqrt(st.m2[:3, 4] / (st.n[:3, 4] - 1) / st.n[:3, 4])
     assert np.all(np.abs(got[:3] - exact[:3]) < 4 * se + 2e-3)
+
+
+def test_free_space_kernels_yukawa(oracle_lib):
+    """3D Yukawa kernels (YukawaGreensFnFreeSpace<3>, distributions.h:271-321): G = e^{-mu r}/(4 pi r)
+    solves -Lap G + lambda G = 0 away from the pole; P = n . grad_y G; the gradients in the pole
+    match central differences -- grad P is the derivative of P (the reference's own formula at
+    :307-317 is not, and fails this check for mu > 0); absorption 0 gives the harmonic kernels."""
+    O = oracle_lib
+    lam = 6.0
+    mu = math.sqrt(lam)
+    rng = np.random.default_rng(11)
+    Gf = lambda z, y: math.exp(-mu * np.linalg.norm(z - y)) / (4 * math.pi * np.linalg.norm(z - y))
+    for _ in range(20):
+        x = rng.normal(size=3) * 0.5
+        y = rng.normal(size=3) * 0.5
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        r = float(np.linalg.norm(x - y))
+        k = O.free_space_kernels(3, x, y, n, r, absorption=lam)
+        assert abs(k["G"] - Gf(x, y)) < 1e-5 * (1 + abs(Gf(x, y)))
+        P = float(np.dot(n, _fd_grad(lambda z: Gf(x, z), y)))
+        assert abs(k["P"] - P) < 2e-3 * (1 + abs(P)), (k["P"], P)
+        np.testing.assert_allclose(k["dG"], _fd_grad(lambda z: Gf(z, y), x), rtol=3e-3, atol=1e-4)
+        Px = lambda z: float(O.free_space_kernels(3, z, y, n, float(np.linalg.norm(z - y)), absorption=lam)["P"])
+        np.testing.assert_allclose(k["dP"], _fd_grad(Px, x, 1e-3), rtol=5e-3, atol=2e-3)
+        # -Lap G + lambda G = 0 (second differences, h = 1e-3, away from the pole)
+        if r > 0.3:
+            h = 1e-3
+            lap = sum((Gf(x + h * e, y) - 2 * Gf(x, y) + Gf(x - h * e, y)) / h ** 2 for e in np.eye(3))
+            assert abs(-lap + lam * Gf(x, y)) < 1e-3 * lam * Gf(x, y), (lap, lam * Gf(x, y))
+    h0 = O.free_space_kernels(3, np.zeros(3), np.ones(3) * 0.2, np.eye(3)[0], 0.35, absorption=0.0)
+    h1 = O.free_space_kernels(3, np.zeros(3), np.ones(3) * 0.2, np.eye(3)[0], 0.35)
+    assert all(np.allclose(h0[k], h1[k]) for k in h0)
+
+
+def test_screened_splat_reproduces_green_representation(oracle_lib):
+    """u = exp(a.x) with |a|^2 = lambda solves -Lap u + lambda u = 0: on the unit ball
+    u(x) = oint (G du/dn - u dG/dn_y) with the Yukawa kernels, and grad u(x) the same with the
+    kernels' gradients; a splat of exact boundary data reproduces both."""
+    O = oracle_lib
+    lam = 4.0
+    a = np.array([1.2, -1.0, 1.2]); a = a / np.linalg.norm(a) * math.sqrt(lam)
+    y, pdf = _sphere_samples(3, 60000)
+    u = np.exp(y @ a)
+    dn = u * (y @ a)
+    samples = dict(pt=y, normal=y, type=np.full(len(y), A.ZG_SAMPLE_ON_ABSORBING_BOUNDARY, np.uint8),
+                   pdf=np.full(len(y), pdf), solution=u, normal_derivative=dn)
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=(150, 3)); x = x / np.linalg.norm(x, axis=1, keepdims=True) * rng.random((150, 1)) * 0.7
+    evals = dict(pt=x, dist_abs=1 - np.linalg.norm(x, axis=1))
+    st = O.BvcStats(len(x), 3)
+    O.bvc_splat(3, samples, evals, st, radius_clamp=1e-4, absorption=lam)
+    exact = np.exp(x @ a)
+    np.testing.assert_allclose(st.solution(), exact, rtol=2e-3, atol=2e-3)
+    np.testing.assert_allclose(st.gradient(), exact[:, None] * a[None, :], rtol=5e-3, atol=6e-3)
+    # the harmonic kernels (absorption 0) do not reproduce the screened solution
+    st0 = O.BvcStats(len(x), 3)
+    O.bvc_splat(3, samples, evals, st0, radius_clamp=1e-4)
+    assert np.abs(st0.solution() - exact).max() > 0.05
+    # screened kernels are 3D only
+    with pytest.raises(RuntimeError):
+        O.bvc_splat(2, dict(pt=np.zeros((1, 2)), normal=np.ones((1, 2)), type=np.ones(1, np.uint8),
+                            pdf=np.ones(1), solution=np.ones(1), normal_derivative=np.ones(1)),
+                    dict(pt=np.zeros((1, 2)) + 0.1, dist_abs=np.ones(1)), O.BvcStats(1, 2), absorption=1.0)

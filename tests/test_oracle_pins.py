@@ -120,3 +120,39 @@ def test_offset_point_follows_rtg_ch6(oracle_lib):
     p0 = np.array([0.01, -0.02, 0.0], np.float32)
     q0 = oracle_lib.offset_point(p0, n)
     np.testing.assert_allclose(q0, p0 + n / 65536.0, rtol=0, atol=1e-9)
+
+
+def test_yukawa_ball_terms_at_small_and_large_radii(oracle_lib):
+    """The 3D Yukawa ball (distributions.h:835-1005) against float64 closed forms:
+    G(r) = sinh(mu (R - r)) / (4 pi r sinh(mu R)), norm = (1 - mu R / sinh mu R) / lambda,
+    dampening mu r (K(mu r) + I(mu r) e^{-mu R} / sinh mu R), gradient norm
+    mu (K(mu r) - I(mu r) K(mu R) / I(mu R)) / (4 pi r^2) and Poisson-kernel gradient scale
+    lambda / (4 pi I(mu R)), with K(x) = e^{-x} (1 + 1/x), I(x) = cosh x - sinh x / x -- down to
+    mu R = 4e-4, where the exponential forms of these terms lose every digit in float."""
+    import math
+    O = oracle_lib
+
+    def I(x):
+        if x < 1e-2:
+            x2 = x * x
+            return x2 * (1 / 3 + x2 * (1 / 30 + x2 / 840))
+        return math.cosh(x) - math.sinh(x) / x
+
+    def K(x):
+        return math.exp(-x) * (1 + 1 / x)
+
+    for lam in (0.5, 4.0, 10.0):
+        mu = math.sqrt(lam)
+        for R in (1e-4, 2e-3, 0.05, 0.3, 1.5):
+            x = mu * R
+            for frac in (0.1, 0.5, 0.9):
+                r = frac * R
+                t = O.yukawa_ball(R, lam, r)
+                want = dict(norm=(1 - x / math.sinh(x)) / lam if x > 1e-2 else R * R * (1 / 6 - 7 * x * x / 360),
+                            dampening=mu * r * (K(mu * r) + I(mu * r) * math.exp(-x) / math.sinh(x)),
+                            gradient_norm=mu * (K(mu * r) - I(mu * r) * K(x) / I(x)) / (4 * math.pi * r * r),
+                            pk_gradient_scale=lam / I(x) / (4 * math.pi),
+                            G=math.sinh(mu * (R - r)) / (4 * math.pi * r * math.sinh(x)),
+                            P=x / (4 * math.pi * math.sinh(x)))
+                for k, v in want.items():
+                    assert math.isfinite(t[k]) and abs(t[k] - v) <= 2e-4 * abs(v) + 1e-30, (lam, R, r, k, t[k], v)

@@ -194,6 +194,7 @@ class zg_solve_options(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("count_visits", C.c_int32),
         ("distance_tolerance", C.c_float),
+        ("wost_run_to_completion", C.c_int32),
     ]
 
 
@@ -249,6 +250,7 @@ SIGNATURES = {
     "zg_sample_reflecting": (C.c_int32, [_P, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "zg_wos_solve": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings), C.POINTER(zg_sample_points),
                                  C.POINTER(zg_statistics), C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),
+    "zg_scene_copy_bvh": (C.c_int32, [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64]),
     "zg_wos_solve_noalloc": (C.c_int32, [_P, C.POINTER(zg_pde), C.POINTER(zg_walk_settings),
                                          C.POINTER(zg_sample_points), C.POINTER(zg_statistics),
                                          C.POINTER(zg_solve_options), C.POINTER(zg_solve_report), _P]),

@@ -171,6 +171,7 @@ class BoundaryValueCachingSolver:
         self.caches: list[SampleCache] = []
         self.seed = seed
         self.calls = 0
+        self.absorption = 0.0   # the PDE's, set by compute_sample_estimates: Yukawa kernels when > 0 (3D)
 
     # -------- generateSamples (:846-874) --------
     def generate_samples(self, absorbing_cache_size: int, reflecting_cache_size: int, domain_cache_size: int,
@@ -195,6 +196,8 @@ class BoundaryValueCachingSolver:
                                  n_walks_gradient: int, robin_coeff_cutoff: float = float("inf")):
         torch = _torch()
         dev = "cuda"
+        # the splat uses the free-space kernels of the PDE (boundary_value_caching.h:386-387,422-423)
+        self.absorption = float(pde.absorption_coeff)
         for c in self.caches:
             n = len(c)
             if n == 0:
@@ -256,7 +259,7 @@ class BoundaryValueCachingSolver:
             if len(c) == 0:
                 continue
             E.bvc_splat(self.dim, c.device(), evals, stats, radius_clamp, kernel_regularization, robin_coeff_cutoff,
-                        cutoff_dist_to_absorbing, cutoff_dist_to_reflecting)
+                        cutoff_dist_to_absorbing, cutoff_dist_to_reflecting, absorption=self.absorption)
 
     # -------- estimateSolutionNearBoundary (:931-945, :476-512) --------
     def estimate_solution_near_boundary(self, pde: A.zg_pde, ws: A.zg_walk_settings, evals: E.BvcEvalPoints,

@@ -213,7 +213,16 @@ zg_status zg_free(void* p) { ZG_HIP(hipFree(p)); return ok(); }
 zg_status zg_memcpy_htod(void* d, const void* s, size_t b) { ZG_HIP(hipMemcpy(d, s, b, hipMemcpyHostToDevice)); return ok(); }
 zg_status zg_memcpy_dtoh(void* d, const void* s, size_t b) { ZG_HIP(hipMemcpy(d, s, b, hipMemcpyDeviceToHost)); return ok(); }
 zg_status zg_memset(void* d, int32_t v, size_t b) { ZG_HIP(hipMemset(d, v, b)); return ok(); }
-zg_status zg_synchronize(void) { ZG_HIP(hipDeviceSynchronize()); return ok(); }
+zg_status zg_synchronize(void) {
+    ZG_HIP(hipDeviceSynchronize());
+    // error flags latched by solves that returned without synchronising (no report requested)
+    unsigned int flags = 0;
+    ZG_HIP(deviceErrorsTakeAndClear(&flags));
+    if (flags & 2u)
+        return fail(ZG_ERR_INVALID_ARGUMENT, "an asynchronous batched solve met an instance_of_point outside [0, n_instances)");
+    if (flags & 1u) return fail(ZG_ERR_DEVICE, "an asynchronous solve overflowed its BVH traversal stack");
+    return ok();
+}
 
 zg_status zg_scene_create(int32_t dim, const float* V, int64_t nV, const int32_t* I, int64_t nP,
                           const uint8_t* refl, const float* dmin, const float* dmax, int32_t dbl, float prec,
@@ -357,6 +366,22 @@ zg_status zg_scene_get_info(const zg_scene* s, zg_scene_info* info) {
     return ok();
 }
 
+zg_status zg_scene_copy_bvh(const zg_scene* s, int32_t boundary, float* nodes_host, int64_t node_floats,
+                            float* prims_host, int64_t prim_floats) {
+    if (!s) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL scene");
+    if (boundary != 0 && boundary != 1) return fail(ZG_ERR_INVALID_ARGUMENT, "boundary must be 0 or 1");
+    if (s->sdfGrid) return fail(ZG_ERR_UNSUPPORTED, "SDF-grid scenes have no BVH");
+    const BoundaryView& b = boundary == 0 ? s->view.abs : s->view.refl;
+    const int64_t nf = (int64_t)b.nNodes * (s->info.node_bytes / 4);
+    const int64_t pf = (int64_t)b.nPrims * (s->info.prim_bytes / 4);
+    if ((nodes_host && node_floats < nf) || (prims_host && prim_floats < pf))
+        return fail(ZG_ERR_INVALID_ARGUMENT, "buffers hold %lld / %lld floats, the BVH needs %lld / %lld",
+                    (long long)node_floats, (long long)prim_floats, (long long)nf, (long long)pf);
+    if (nodes_host && nf > 0) ZG_HIP(hipMemcpy(nodes_host, b.nodes, (size_t)nf * 4, hipMemcpyDeviceToHost));
+    if (prims_host && pf > 0) ZG_HIP(hipMemcpy(prims_host, b.prims, (size_t)pf * 4, hipMemcpyDeviceToHost));
+    return ok();
+}
+
 zg_status zg_closest_point(const zg_scene* s, int32_t boundary, int64_t n, const float* pts, float* dist,
                            float* closest, float* normal, int32_t* prim, void* stream) {
     if (!s || n < 0 || (n > 0 && (!pts || !dist))) return fail(ZG_ERR_INVALID_ARGUMENT, "invalid arguments");
@@ -472,7 +497,7 @@ struct SolveSetup {
     WosParams P;
     MergeParams M;
     int dim = 3, width = 8, blocks = 1, depth = 0;
-    bool batch = false, count = false, wost = false, screened = false, split = false;
+    bool batch = false, count = false, wost = false, screened = false, split = false, interleave = true;
     int64_t n = 0;
     int nWalksMax = 0;
     size_t words = 0;
@@ -500,14 +525,16 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     int dim = s ? s->dim : b->dim;
     int sceneDevice = s ? s->device : b->device;
     if (s) { st = checkPde(pde, dim); if (st != ZG_OK) return st; }
-    // walk throughputs start at 1 and never grow for these PDEs (harmonic and Yukawa kernels,
-    // pure Neumann reflectance), so walks split only below a threshold of 1
+    // walk throughputs start at 1 and never grow under the harmonic and Yukawa kernels and pure
+    // Neumann reflectance, so those walks split only below a threshold of 1; walk on stars with
+    // Robin coefficients that may be negative clamps the reflectance to [0, 2]
+    // (walk_on_stars.h:298-303), so its throughput can grow past any finite threshold
     bool split = ws->splitting_threshold < 1.0f;
     if (wost) {
-        if (split)
-            return fail(ZG_ERR_UNSUPPORTED, "splitting_threshold %g < 1: walk splitting runs in walk on spheres only",
-                        ws->splitting_threshold);
         if (!s) return fail(ZG_ERR_UNSUPPORTED, "walk on stars runs on single scenes");
+        if (pde && !pde->are_robin_conditions_pure_neumann && !pde->are_robin_coeffs_nonnegative &&
+            ws->splitting_threshold < FLT_MAX)
+            split = true;
     }
     if (pts->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative point count");
     if (pts->n > 0 && (!pts->pt || !pts->dist_to_absorbing_boundary || !pts->n_walks))
@@ -523,8 +550,12 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     if (o.walks_per_lane < 1 || o.walks_per_lane > 64) return fail(ZG_ERR_INVALID_ARGUMENT, "walks_per_lane must be 1..64");
     if (o.regroup_interval < 1 || o.regroup_interval > 1024) return fail(ZG_ERR_INVALID_ARGUMENT, "regroup_interval must be 1..1024");
     if (o.step_interval < 1 || o.step_interval > 1024) return fail(ZG_ERR_INVALID_ARGUMENT, "step_interval must be 1..1024");
+    if (o.count_visits < 0 || o.count_visits > 2) return fail(ZG_ERR_INVALID_ARGUMENT, "count_visits must be 0, 1 or 2");
+    if (o.count_visits == 2 && wost) return fail(ZG_ERR_UNSUPPORTED, "count_visits 2 runs in walk on spheres only");
     if (!(o.distance_tolerance >= 0.0f && o.distance_tolerance < 0.5f))
         return fail(ZG_ERR_INVALID_ARGUMENT, "distance_tolerance must be in [0, 0.5)");
+    if (o.wost_run_to_completion != 0 && o.wost_run_to_completion != 1)
+        return fail(ZG_ERR_INVALID_ARGUMENT, "wost_run_to_completion must be 0 or 1");
     if ((uint64_t)o.stream_offset + (uint64_t)pts->n_walks_max > 0xffffffffull)
         return fail(ZG_ERR_INVALID_ARGUMENT, "stream_offset + n_walks_max must fit in 32 bits");
     int dev = 0;
@@ -565,6 +596,7 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     P.chunksPerPoint = std::max(1, (pts->n_walks_max + o.walks_per_lane - 1) / o.walks_per_lane);
     P.nClaims = (unsigned long long)n * (unsigned long long)P.chunksPerPoint;
     P.seed = o.seed; P.streamOffset = o.stream_offset;
+    P.countMode = o.count_visits;
     P.relax2 = (1.0f - o.distance_tolerance) * (1.0f - o.distance_tolerance);
     float e2 = 2.0f * ws->epsilon_shell_for_absorbing_boundary;
     P.guard2 = e2 * e2 / P.relax2;
@@ -596,6 +628,7 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     if (P.quantity) S.words += (size_t)(n + 1) / 2;   // masked walk counts, last in the workspace
     S.wost = wost;
     S.split = split;
+    S.interleave = o.wost_run_to_completion == 0;
     M.sSplits = stats->total_splits;
     if (wost) {
         S.callerFsr = pts->first_sphere_radius;
@@ -615,7 +648,7 @@ void bindWorkspace(SolveSetup& S, uint64_t* wsp) {
     S.P.counters = reinterpret_cast<unsigned long long*>(wsp + 4 * n);
     S.P.claimCounter = S.P.counters + kCntClaims;
     S.M.accSum = S.P.accSum; S.M.accSq = S.P.accSq; S.M.accN = S.P.accN; S.M.accLen = S.P.accLen;
-    if (S.split) {   // after the walk-on-stars radii (never both: splitting runs in walk on spheres only)
+    if (S.split) {   // after the walk-on-stars radii
         S.P.accSplits = reinterpret_cast<unsigned long long*>(wsp + 4 * n + kCntSlots + (S.wost ? (n + 1) / 2 : 0));
         S.M.accSplits = S.P.accSplits;
     }
@@ -641,7 +674,7 @@ hipError_t enqueueSolve(SolveSetup& S, uint64_t* wsp, hipStream_t strm, hipEvent
     }
     if (e == hipSuccess && S.wost) e = launchWostFirstSphere(P, S.dim, S.width, S.callerFsr, const_cast<float*>(P.fsr), strm);
     if (e == hipSuccess && S.nWalksMax > 0)
-        e = S.wost ? launchWost(P, S.dim, S.width, S.count, S.blocks, strm)
+        e = S.wost ? launchWost(P, S.dim, S.width, S.count, S.split, S.interleave, S.blocks, strm)
             : S.batch ? launchWosBatch(P, S.width, S.count, S.screened, S.split, S.blocks, strm)
             : S.sdf ? launchWosSdf(P, S.dim, S.screened, S.split, S.blocks, strm)
                     : launchWos(P, S.dim, S.width, S.count, S.screened, S.split, S.blocks, strm);
@@ -759,6 +792,8 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
         if (!st.n_gradient_estimates || !st.gradient_mean || !st.gradient_m2 || !st.total_first_source_contribution ||
             !st.total_derivative_contribution)
             return fail(ZG_ERR_INVALID_ARGUMENT, "gradient points need the gradient members of zg_statistics");
+        if (S.wost && S.split)
+            return fail(ZG_ERR_UNSUPPORTED, "walk splitting with gradient points runs in walk on spheres only");
         words += gradLayout(S).words;
     }
     // the walk-on-stars kernel reads the PDE from a device copy after the workspace
@@ -789,6 +824,15 @@ zg_status runSolve(SolveSetup& S, zg_solve_report* report, hipStream_t strm) {
         e = launchFillZero(reinterpret_cast<uint64_t*>(S.P.counters), kCntSlots, strm);
         if (e == hipSuccess && ev0) e = hipEventRecord(ev0, strm);
         if (e == hipSuccess && ev1) e = hipEventRecord(ev1, strm);
+    }
+    if (!report && nGrad == 0) {
+        // asynchronous: nothing waits for the device; the workspace is freed in stream order and
+        // the solve's error flags are latched for zg_synchronize
+        if (e == hipSuccess) e = launchLatchErrors(S.P.counters, strm);
+        hipError_t ef = hipFreeAsync(wsp, strm);
+        if (e == hipSuccess) e = ef;
+        if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "walk-on-spheres launch failed: %s", hipGetErrorString(e));
+        return ok();
     }
     unsigned long long cnt[kCntSlots] = {0};
     if (e == hipSuccess) e = hipMemcpyAsync(cnt, S.P.counters, sizeof(cnt), hipMemcpyDeviceToHost, strm);
@@ -856,7 +900,9 @@ zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* S, c
                        zg_bvc_statistics* out, void* stream) {
     if (!S || !E || !out) return fail(ZG_ERR_INVALID_ARGUMENT, "NULL argument");
     if (dim != 2 && dim != 3) return fail(ZG_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
-    if (absorption != 0.0f) return fail(ZG_ERR_UNSUPPORTED, "the splat uses the harmonic free-space kernels (absorption 0)");
+    if (!(absorption >= 0.0f)) return fail(ZG_ERR_INVALID_ARGUMENT, "absorption must be >= 0");
+    if (absorption > 0.0f && dim == 2)
+        return fail(ZG_ERR_UNSUPPORTED, "screened boundary value caching (absorption > 0) is supported in 3D only");
     if (S->n < 0 || E->n < 0) return fail(ZG_ERR_INVALID_ARGUMENT, "negative counts");
     if (E->n >= (1ll << 31)) return fail(ZG_ERR_INVALID_ARGUMENT, "at most 2^31-1 evaluation points");
     if (S->n > 0 && (!S->pt || !S->normal || !S->type || !S->pdf || !S->solution || !S->normal_derivative))
@@ -866,7 +912,8 @@ zg_status zg_bvc_splat(int32_t dim, float absorption, const zg_bvc_samples* S, c
     if (E->n > 0 && (!out->n || !out->solution_mean || !out->solution_m2 || !out->gradient_mean || !out->gradient_m2))
         return fail(ZG_ERR_INVALID_ARGUMENT, "statistics arrays are required");
     if (!(radiusClamp >= 0.0f) || !(kreg >= 0.0f)) return fail(ZG_ERR_INVALID_ARGUMENT, "radius clamp and kernel regularization must be >= 0");
-    hipError_t e = launchBvcSplat(dim, *S, *E, *out, radiusClamp, kreg, robinCutoff, cutoffA, cutoffR, (hipStream_t)stream);
+    hipError_t e = launchBvcSplat(dim, absorption, *S, *E, *out, radiusClamp, kreg, robinCutoff, cutoffA, cutoffR,
+                                  (hipStream_t)stream);
     if (e != hipSuccess) return fail(ZG_ERR_DEVICE, "zg_bvc_splat: %s", hipGetErrorString(e));
     return ok();
 }

@@ -1,7 +1,7 @@
 // bvc_kernel.hip -- boundary value caching splat for gfx950
 // (variance_reduction/boundary_value_caching.h:370-402 splat, :683-810 splatBoundaryData /
 // splatSourceData, core/distributions.h:12-55 kernel regularization, :84-177 free-space harmonic
-// Green's function and Poisson kernel with their gradients).
+// and :271-321 3D Yukawa Green's functions and Poisson kernels with their gradients).
 //
 // Every evaluation point x receives one estimate per cached sample y: the boundary term of
 // Green's representation formula (Dirichlet, Neumann or Robin form) or the source term, each
@@ -27,6 +27,7 @@ struct BvcParams {
     zg_bvc_eval_points E;
     zg_bvc_statistics out;
     float radiusClamp, kreg, robinCutoff, cutoffA, cutoffR;
+    float mu;   // sqrt(absorption): 3D Yukawa kernels when > 0
 };
 
 template <int D>
@@ -91,21 +92,29 @@ __global__ void __launch_bounds__(256) bvcSplatKernel(BvcParams B) {
             for (int a = 0; a < D; a++) { y[a] = sPt[a][t]; n[a] = sN[a][t]; xy[a] = x[a] - y[a]; }
             const float r = fmaxf(B.radiusClamp, sqrtf(norm2(xy)));
             const float r2 = r * r;
-            // free-space harmonic kernels (distributions.h:84-177), gradients in the pole x
-            float G, P, kG, kP;
+            // free-space kernels, gradients in the pole x: harmonic (distributions.h:84-177), or in
+            // 3D with absorption the Yukawa ones (:271-321) with q1 = e^{-mu r} (mu r + 1):
+            // G = e^{-mu r} / (4 pi r), grad G = -(x - y) q1 / (4 pi r^3), P = n.(x - y) q1 / (4 pi r^3),
+            // grad P = [n q1 - (x - y) (n.(x - y) / r^2) e^{-mu r} (mu^2 r^2 + 3 mu r + 3)] / (4 pi r^3)
+            // (the derivative of P; mu = 0 gives the harmonic kernels)
+            float G, P, kG, kP, kPn;
             if (D == 2) {
                 G = -logf(r) / (2.0f * kPi);
                 kG = -1.0f / (2.0f * kPi * r2);
                 P = dot(n, xy) / (2.0f * kPi * r2);
                 kP = 2.0f * (dot(n, xy) / r2);
+                kPn = 1.0f / (2.0f * kPi * r2);
             } else {
                 const float r3 = r2 * r;
-                G = 1.0f / (4.0f * kPi * r);
-                kG = -1.0f / (4.0f * kPi * r3);
-                P = dot(n, xy) / (4.0f * kPi * r3);
-                kP = 3.0f * (dot(n, xy) / r2);
+                const float mr = B.mu * r;
+                const float e = B.mu > 0.0f ? expf(-mr) : 1.0f;
+                const float q1 = e * (mr + 1.0f);
+                G = e / (4.0f * kPi * r);
+                kG = -q1 / (4.0f * kPi * r3);
+                P = dot(n, xy) * q1 / (4.0f * kPi * r3);
+                kP = (dot(n, xy) / r2) * ((mr * mr + 3.0f * mr + 3.0f) / (mr + 1.0f));
+                kPn = q1 / (4.0f * kPi * r3);
             }
-            const float kPn = D == 2 ? 1.0f / (2.0f * kPi * r2) : 1.0f / (4.0f * kPi * r * r2);
             Vec<D> dG, dP;
 #pragma unroll
             for (int a = 0; a < D; a++) { dG[a] = xy[a] * kG; dP[a] = (n[a] - xy[a] * kP) * kPn; }
@@ -189,10 +198,11 @@ __global__ void __launch_bounds__(256) bvcSplatKernel(BvcParams B) {
     }
 }
 
-hipError_t launchBvcSplat(int dim, const zg_bvc_samples& S, const zg_bvc_eval_points& E, const zg_bvc_statistics& out,
-                          float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR, hipStream_t s) {
+hipError_t launchBvcSplat(int dim, float absorption, const zg_bvc_samples& S, const zg_bvc_eval_points& E,
+                          const zg_bvc_statistics& out, float radiusClamp, float kreg, float robinCutoff, float cutoffA,
+                          float cutoffR, hipStream_t s) {
     if (E.n <= 0) return hipSuccess;
-    BvcParams B{S, E, out, radiusClamp, kreg, robinCutoff, cutoffA, cutoffR};
+    BvcParams B{S, E, out, radiusClamp, kreg, robinCutoff, cutoffA, cutoffR, sqrtf(fmaxf(absorption, 0.0f))};
     dim3 g((unsigned)((E.n + 255) / 256)), bl(256);
     if (dim == 2) hipLaunchKernelGGL(bvcSplatKernel<2>, g, bl, 0, s, B);
     else hipLaunchKernelGGL(bvcSplatKernel<3>, g, bl, 0, s, B);

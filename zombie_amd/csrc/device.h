@@ -168,6 +168,23 @@ ZG_DEV Vec<3> fromFrame(const Vec<3>& n, const Vec<3>& d) {
 // Green's function of the ball (core/distributions.h:453-665 harmonic, 835-1005 Yukawa 3D)
 // ------------------------------------------------------------------------------------
 
+// sinh x and I(x) = cosh x - sinh x / x (x^{3/2} I_{3/2} up to a constant) from e = e^{-x}:
+// below the thresholds the exponential forms lose most digits to cancellation in float (at
+// x = 0.004, 1 - e^2 keeps 4 correct digits and cosh x - sinh x / x none), so the Taylor series
+// take over there (sinh: x (1 + x^2/6 (1 + x^2/20 (1 + x^2/42))); I: x^2/3 + x^4/30 + x^6/840 +
+// x^8/45360, relative truncation error below 1e-8 at x = 0.5)
+ZG_DEV float yukawaSinh(float x, float e) {
+    if (x < 0.1f) { float x2 = x * x; return x * (1.0f + x2 * (1.0f / 6.0f) * (1.0f + x2 * (1.0f / 20.0f) * (1.0f + x2 * (1.0f / 42.0f)))); }
+    return (1.0f - e * e) / (2.0f * e);
+}
+ZG_DEV float yukawaI(float x, float e) {
+    if (x < 0.5f) {
+        float x2 = x * x;
+        return x2 * (1.0f / 3.0f + x2 * (1.0f / 30.0f + x2 * (1.0f / 840.0f + x2 * (1.0f / 45360.0f))));
+    }
+    return (1.0f + e * e) / (2.0f * e) - yukawaSinh(x, e) / x;
+}
+
 template <int D>
 struct Ball {
     bool yukawa;
@@ -180,7 +197,7 @@ struct Ball {
         if (yukawa) {
             muR = R * sqrtLambda;
             expmuR = __expf(-muR);
-            sinhmuR = (1.0f - expmuR * expmuR) / (2.0f * expmuR);
+            sinhmuR = yukawaSinh(muR, expmuR);
         }
     }
     ZG_DEV float evaluate(float r) const {
@@ -190,23 +207,26 @@ struct Ball {
         }
         float mur = r * sqrtLambda;
         float expmur = __expf(-mur);
-        float sinhmur = (1.0f - expmur * expmur) / (2.0f * expmur);
+        float sinhmur = yukawaSinh(mur, expmur);
         return (expmur - expmuR * sinhmur / sinhmuR) / (4.0f * kPi * r);
     }
     ZG_DEV float norm() const {
         if (!yukawa) return D == 2 ? R * R * 0.25f : R * R * (1.0f / 6.0f);
+        // (1 - 4 pi P)/lambda, P = muR/(4 pi sinh muR); for small muR the series of
+        // 1 - x / sinh x = x^2/6 - 7 x^4/360 + 31 x^6/15120 - 127 x^8/604800 (Bernoulli numbers)
+        if (muR < 0.5f) {
+            float x2 = muR * muR;
+            return R * R * (1.0f / 6.0f - x2 * (7.0f / 360.0f - x2 * (31.0f / 15120.0f - x2 * (127.0f / 604800.0f))));
+        }
         float lambda = sqrtLambda * sqrtLambda;
-        return (1.0f - muR / sinhmuR) / lambda;  // (1 - 4 pi P)/lambda, P = muR/(4 pi sinh muR)
+        return (1.0f - muR / sinhmuR) / lambda;
     }
     // poissonKernelDampeningFactor(r) = mur (K32(mur) + I32(mur) e^{-muR}/sinh muR)
     ZG_DEV float dampening(float r) const {
         float mur = r * sqrtLambda;
         float expmur = __expf(-mur);
-        float exp2mur = expmur * expmur;
-        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
-        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
         float K32 = expmur * (1.0f + 1.0f / mur);
-        float I32 = coshmur - sinhmur / mur;
+        float I32 = yukawaI(mur, expmur);
         return mur * (K32 + I32 * expmuR / sinhmuR);
     }
     ZG_DEV float pdfSphere(float r) const { return D == 2 ? 1.0f / (2.0f * kPi * r) : 1.0f / (4.0f * kPi * r * r); }
@@ -220,19 +240,13 @@ struct Ball {
         }
         float mur = r * sqrtLambda;
         float expmur = __expf(-mur);
-        float exp2mur = expmur * expmur;
-        float coshmur = (1.0f + exp2mur) / (2.0f * expmur);
-        float sinhmur = (1.0f - exp2mur) / (2.0f * expmur);
         float K32mur = expmur * (1.0f + 1.0f / mur);
-        float I32mur = coshmur - sinhmur / mur;
+        float I32mur = yukawaI(mur, expmur);
         float Qr = sqrtLambda * (K32mur - I32mur * K32R() / I32R());
         return Qr / (4.0f * kPi * r * r);
     }
     ZG_DEV float K32R() const { return expmuR * (1.0f + 1.0f / muR); }
-    ZG_DEV float I32R() const {
-        float coshmuR = (1.0f + expmuR * expmuR) / (2.0f * expmuR);
-        return coshmuR - sinhmuR / muR;
-    }
+    ZG_DEV float I32R() const { return yukawaI(muR, expmuR); }
     // Poisson kernel of the centred ball per unit solid angle, and its gradient in the centre
     // at y on the sphere (distributions.h:512-515,545-550, 628-631,654-659, 934-936,987-994)
     ZG_DEV float poissonKernel() const {

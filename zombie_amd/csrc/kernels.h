@@ -50,6 +50,8 @@ struct WosParams {
     float relax2;        // (1 - distance_tolerance)^2
     float guard2;        // relaxation only when best^2 > guard2 (keeps the shell test exact)
     int32_t nInstances;  // batched solves: size of the instance table
+    int32_t countMode;   // COUNT kernels: 1 = visits of the kernel's own queries, 2 = visits of an exact
+                         // nearest-first traversal from the root at every query point
     double* accSum;
     double* accSq;
     unsigned long long* accN;
@@ -129,6 +131,9 @@ struct MergeParams {
 };
 
 hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s);
+// solves without a report: their error flags are latched on the device and reported by zg_synchronize
+hipError_t launchLatchErrors(const unsigned long long* counters, hipStream_t s);
+hipError_t deviceErrorsTakeAndClear(unsigned int* flags);
 hipError_t launchMaskWalks(const int32_t* nWalks, const uint8_t* quantity, int64_t n, int32_t* out, hipStream_t s);
 hipError_t launchClosestPoint(const SceneView& sv, int boundary, int64_t n, const float* pts, float* dist,
                               float* closest, float* normal, int32_t* prim, hipStream_t s);
@@ -157,13 +162,15 @@ hipError_t launchSdfRay(const zg_function& sdf, int dim, int64_t n, const float*
                         uint8_t* hit, float* t, float* point, float* normal, hipStream_t s);
 hipError_t launchSdfInside(const zg_function& sdf, int dim, int64_t n, const float* pts, uint8_t* inside, hipStream_t s);
 // boundary value caching splat
-hipError_t launchBvcSplat(int dim, const zg_bvc_samples& S, const zg_bvc_eval_points& E, const zg_bvc_statistics& out,
-                          float radiusClamp, float kreg, float robinCutoff, float cutoffA, float cutoffR, hipStream_t s);
+hipError_t launchBvcSplat(int dim, float absorption, const zg_bvc_samples& S, const zg_bvc_eval_points& E,
+                          const zg_bvc_statistics& out, float radiusClamp, float kreg, float robinCutoff, float cutoffA,
+                          float cutoffR, hipStream_t s);
 // read-bandwidth probe
 hipError_t launchReadProbe(const void* buf, int64_t bytes, int reps, int blocks, float* sink, hipStream_t s);
 // walk on stars (2D)
 hipError_t launchWostFirstSphere(const WosParams& P, int dim, int width, const float* callerFsr, float* fsr, hipStream_t s);
-hipError_t launchWost(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s);
+hipError_t launchWost(const WosParams& P, int dim, int width, bool count, bool split, bool interleave, int gridBlocks,
+                      hipStream_t s);
 hipError_t launchWostGradFirstSphere(const WosParams& P, int dim, int width, float* fsr, hipStream_t s);
 hipError_t launchWostGrad(const WosParams& P, int dim, int width, int gridBlocks, hipStream_t s);
 hipError_t launchStarRadius(const SceneView& sv, int64_t n, const float* pts, const float* minR, const float* maxR,
@@ -172,7 +179,6 @@ hipError_t launchLineOfSight(const SceneView& sv, int64_t n, const float* p1, co
                              hipStream_t s);
 hipError_t launchSampleReflecting(const SceneView& sv, int64_t n, const float* x, const float* radius, const float* u,
                                   uint8_t* ok, float* pt, float* normal, float* pdf, hipStream_t s);
-
 
 // zg_wos_solve_noalloc: fixed device workspace (noalloc.hip), one copy per GPU for the life of
 // the process.  noallocScratch() returns the current device's copy.

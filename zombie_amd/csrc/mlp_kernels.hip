@@ -8,8 +8,8 @@
 // the control variate used by the walk kernel (first step, zero mean).
 //
 // Dense layers use one tiled kernel: C[M x N] = A[M x K] * B[K x N] with arbitrary element
-// strides for A and B, a 64 x 64 block tile (4 waves, each 32 x 32 = 2 x 2 MFMA 16x16x16
-// tiles) staged through LDS, and a fused epilogue:
+// strides for A and B, a 64 x 64 block tile (4 waves, each 32 x 32 = 2 x 2 MFMA 16x16x32
+// tiles) staged through LDS 32 deep in k, and a fused epilogue:
 //   FWD       : z = acc + bias[n];  Z = z (fp32);  H = bf16(silu(z))
 //   BWD_DATA  : dZprev = bf16(acc * silu'(Zprev))   (gradient w.r.t. the previous pre-activation)
 //   BWD_INPUT : dPhi = acc (fp32)                    (gradient w.r.t. the Fourier features)
@@ -49,18 +49,49 @@ struct GemmArgs {
     float* Z;                              // FWD: out fp32 [M x N];  BWD_DATA: in, previous pre-activation
     uint16_t* H;                           // FWD: out bf16 [M x N];  BWD_DATA: out dZprev bf16 [M x N]
     float* F;                              // BWD_INPUT: out fp32 [M x N];  BWD_WEIGHT: atomic fp32 [M x N]
+    float* dbias;                          // BWD_DATA (nullable): atomic fp32 [N], column sums of H
     int ldc;                               // row stride of the outputs
 };
 
-constexpr int kBM = 64, kBN = 64, kBK = 16;
+constexpr int kBM = 64, kBN = 64, kBK = 32;
+constexpr int kLdsK = kBK + 8;   // padded LDS row (bf16): 80 B, a multiple of 16 B for 128-bit reads
 
-// MFMA v_mfma_f32_16x16x16_bf16 operand layout (lane l of 64, element j of 4):
-//   A[row = l % 16][k = 4 (l / 16) + j],  B[k = 4 (l / 16) + j][col = l % 16],
-//   D[row = 4 (l / 16) + j][col = l % 16]
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// 8 elements of a strided bf16 operand starting at element `base`, `step` elements apart: one
+// 128-bit load when they are contiguous, aligned and inside the valid range, else element by element
+__device__ __forceinline__ uint4 load8(const uint16_t* __restrict__ p, int64_t base, int64_t step, int valid) {
+    if (step == 1 && valid >= 8 && ((reinterpret_cast<uintptr_t>(p + base) & 15) == 0))
+        return *reinterpret_cast<const uint4*>(p + base);
+    uint16_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = j < valid ? p[base + (int64_t)j * step] : (uint16_t)0;
+    return make_uint4((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16),
+                      (uint32_t)v[4] | ((uint32_t)v[5] << 16), (uint32_t)v[6] | ((uint32_t)v[7] << 16));
+}
+
+// stores 8 loaded elements into an LDS tile kept [row][k]: along k when the load ran along k,
+// else down 8 rows at one k
+__device__ __forceinline__ void store8(uint16_t (*T)[kLdsK], int row, int k, bool alongK, uint4 v) {
+    if (alongK) {
+        *reinterpret_cast<uint4*>(&T[row][k]) = v;
+    } else {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) T[row + j][k] = (uint16_t)(w[j >> 1] >> (16 * (j & 1)));
+    }
+}
+
+// MFMA v_mfma_f32_16x16x32_bf16 (gfx950) operand layout, lane l of 64, element j of 8:
+//   A[row = l % 16][k = 8 (l / 16) + j],  B[k = 8 (l / 16) + j][col = l % 16],
+//   D[row = 4 (l / 16) + j][col = l % 16] (j of 4)
+// Both operands sit in LDS with k contiguous (A as [m][k], B transposed as [n][k]), so each
+// lane fetches its 8 elements with one 128-bit LDS read.  Global operands are read 8 elements at a
+// time along whichever of their two dimensions is contiguous (128-bit loads when aligned).
 template <int EPI>
 __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
-    __shared__ uint16_t As[kBM][kBK + 4];
-    __shared__ uint16_t Bs[kBK][kBN + 4];
+    __shared__ __attribute__((aligned(16))) uint16_t As[kBM][kLdsK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN][kLdsK];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
@@ -72,6 +103,9 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
         kBegin = blockIdx.z * chunk;
         kEnd = min(g.K, kBegin + chunk);
     }
+    // how each operand tile is read: along k (A row-major / B column-major) or along m / n
+    const bool aAlongK = g.sAk == 1;
+    const bool bAlongK = g.sBk == 1;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x4 acc[2][2];
 #pragma unroll
@@ -80,34 +114,55 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
     for (int k0 = kBegin; k0 < kEnd; k0 += kBK) {
-        // stage A (64 x 16) and B (16 x 64): 1024 elements each, 4 per thread
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            int idx = tid + e * 256;
-            int r = idx / kBK, c = idx % kBK;
-            int gm = m0 + r, gk = k0 + c;
-            As[r][c] = (gm < g.M && gk < kEnd) ? g.A[(int64_t)gm * g.sAm + (int64_t)gk * g.sAk] : (uint16_t)0;
-            int rb = idx / kBN, cb = idx % kBN;
-            int gk2 = k0 + rb, gn = n0 + cb;
-            Bs[rb][cb] = (gk2 < kEnd && gn < g.N) ? g.B[(int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn] : (uint16_t)0;
+        // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
+        {
+            int r, kk, valid;
+            int64_t base, step;
+            if (aAlongK) {
+                r = tid >> 2; kk = (tid & 3) * 8;
+                int gm = m0 + r, gk = k0 + kk;
+                valid = gm < g.M ? max(0, min(8, kEnd - gk)) : 0;
+                base = (int64_t)gm * g.sAm + (int64_t)gk; step = 1;
+            } else {
+                kk = tid >> 3; r = (tid & 7) * 8;
+                int gm = m0 + r, gk = k0 + kk;
+                valid = gk < kEnd ? max(0, min(8, g.M - gm)) : 0;
+                base = (int64_t)gm * g.sAm + (int64_t)gk * g.sAk; step = g.sAm;
+            }
+            store8(As, r, kk, aAlongK, load8(g.A, base, step, valid));
+        }
+        {
+            int c, kk, valid;
+            int64_t base, step;
+            if (bAlongK) {
+                c = tid >> 2; kk = (tid & 3) * 8;
+                int gn = n0 + c, gk = k0 + kk;
+                valid = gn < g.N ? max(0, min(8, kEnd - gk)) : 0;
+                base = (int64_t)gk + (int64_t)gn * g.sBn; step = 1;
+            } else {
+                kk = tid >> 3; c = (tid & 7) * 8;
+                int gn = n0 + c, gk = k0 + kk;
+                valid = gk < kEnd ? max(0, min(8, g.N - gn)) : 0;
+                base = (int64_t)gk * g.sBk + (int64_t)gn * g.sBn; step = g.sBn;
+            }
+            store8(Bs, c, kk, bAlongK, load8(g.B, base, step, valid));
         }
         __syncthreads();
-        bf16x4 af[2], bfr[2];
+        bf16x8 af[2], bfr[2];
+        const int kk = 8 * (lane >> 4);
 #pragma unroll
         for (int i = 0; i < 2; i++) {
-            int r = wm + 16 * i + (lane & 15);
-            int kk = 4 * (lane >> 4);
-            af[i] = bf16x4{(short)As[r][kk], (short)As[r][kk + 1], (short)As[r][kk + 2], (short)As[r][kk + 3]};
-            int c = wn + 16 * i + (lane & 15);
-            bfr[i] = bf16x4{(short)Bs[kk][c], (short)Bs[kk + 1][c], (short)Bs[kk + 2][c], (short)Bs[kk + 3][c]};
+            af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&As[wm + 16 * i + (lane & 15)][kk]));
+            bfr[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[wn + 16 * i + (lane & 15)][kk]));
         }
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
     // epilogue
+    float csum[2] = {0.0f, 0.0f};   // BWD_DATA: this lane's share of the tile's column sums
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -124,13 +179,29 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
                     g.Z[o] = z;
                     g.H[o] = toBf16(silu(z));
                 } else if (EPI == kBwdData) {
-                    g.H[o] = toBf16(v * siluGrad(g.Z[o]));
+                    const uint16_t h = toBf16(v * siluGrad(g.Z[o]));
+                    g.H[o] = h;
+                    csum[j] += fromBf16(h);   // the bias gradient sums the stored (bf16) values
                 } else if (EPI == kBwdInput) {
                     g.F[o] = v;
                 } else {
                     atomicAdd(&g.F[o], v);
                 }
             }
+    if (EPI == kBwdData && g.dbias) {
+        // column sums of the block's 64 x 64 tile: over the lane's rows (above), the four row
+        // groups of a wave (lanes l, l ^ 16, l ^ 32, l ^ 48), then the two waves that share the
+        // columns; one atomic per column and block
+        __shared__ float colSum[2][kBN];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            csum[j] += __shfl_xor(csum[j], 16);
+            csum[j] += __shfl_xor(csum[j], 32);
+            if (lane < 16) colSum[wave >> 1][wn + 16 * j + lane] = csum[j];
+        }
+        __syncthreads();
+        if (tid < kBN && n0 + tid < g.N) atomicAdd(&g.dbias[n0 + tid], colSum[0][tid] + colSum[1][tid]);
+    }
 }
 
 // phi = [sin(B x), cos(B x)] in bf16, [n x 2F]
@@ -147,67 +218,109 @@ __global__ void fourierKernel(const float* __restrict__ x, int n, int d, const f
     phi[(int64_t)row * 2 * F + F + f] = toBf16(c);
 }
 
-// output layer c = H . w + b (fp32 weights); loss gradient dc = (c - y) / n; dZlast =
-// bf16(dc * w * silu'(Zlast)); dw, db accumulated with block reductions + atomics
+// Output layer and loss, one wave per row: lane l of a wave owns the columns k = l + 64 j
+// (j < NCH, hidden <= 64 NCH), so a row is read as contiguous 128-byte pieces, and each lane
+// accumulates the gradient sums of its own columns over the rows its wave visits; the four waves
+// of a block are then combined in LDS and added to the results with one atomic per column and
+// block.  A wave keeps kRowsInFlight rows' loads in flight.
+constexpr int kRowBlocks = 512;   // grid cap of the row-parallel kernels
+constexpr int kRowsInFlight = 4;
+
+// adds the per-lane column partials of the block's four waves to dst[0, H)
+template <int NCH>
+__device__ __forceinline__ void blockColumnAtomics(const float (&part)[NCH], int H, float* __restrict__ dst) {
+    __shared__ float sc[4][NCH * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) sc[wave][lane + 64 * j] = part[j];
+    __syncthreads();
+    for (int k = threadIdx.x; k < H; k += blockDim.x) atomicAdd(&dst[k], sc[0][k] + sc[1][k] + sc[2][k] + sc[3][k]);
+    __syncthreads();
+}
+
+// c = H . w + b (fp32 weights); mode 0: values only; mode 1 (training, needs y): loss
+// sum 0.5 (c - y)^2 / n, dc = (c - y) / n, dZlast = bf16(dc * w * silu'(Zlast)), and the
+// gradients dw = sum_i dc_i H_i, db = sum_i dc_i, dbl = sum_i dZlast_i (the last hidden layer's
+// bias gradient, from the stored bf16 values); mode 2 (input gradient): dZlast with dc = 1.
+// Launch: 256 threads, any grid.
+template <int NCH>
 __global__ void __launch_bounds__(256) outputKernel(const uint16_t* __restrict__ Hl, const float* __restrict__ Zl, int n,
                                                     int H, const float* __restrict__ w, const float* __restrict__ b,
                                                     const float* __restrict__ y, float invN, float* __restrict__ out,
                                                     uint16_t* __restrict__ dZl, float* __restrict__ dw,
-                                                    float* __restrict__ db, float* __restrict__ loss, int mode) {
-    // mode 0: value only; 1: training (needs y); 2: input gradient (dc = 1)
-    __shared__ float sdw[256];
-    __shared__ float sred[256];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float c = 0.0f, dc = 0.0f;
-    bool valid = i < n;
-    if (valid) {
-        float acc = b[0];
-        for (int k = 0; k < H; k++) acc += fromBf16(Hl[(int64_t)i * H + k]) * w[k];
-        c = acc;
-        if (out) out[i] = c;
-        if (mode == 1) dc = (c - y[i]) * invN;
-        else if (mode == 2) dc = 1.0f;
-        if (mode != 0)
-            for (int k = 0; k < H; k++)
-                dZl[(int64_t)i * H + k] = toBf16(dc * w[k] * siluGrad(Zl[(int64_t)i * H + k]));
+                                                    float* __restrict__ db, float* __restrict__ dbl,
+                                                    float* __restrict__ loss, int mode) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float wk[NCH], dwp[NCH], dbp[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const int k = lane + 64 * j;
+        wk[j] = k < H ? w[k] : 0.0f;
+        dwp[j] = 0.0f;
+        dbp[j] = 0.0f;
+    }
+    const float b0 = b[0];
+    float lossp = 0.0f, dcp = 0.0f;
+    const int stride = gridDim.x * 4;
+    for (int r0 = blockIdx.x * 4 + wave; r0 < n; r0 += kRowsInFlight * stride) {
+        float h[kRowsInFlight][NCH], z[kRowsInFlight][NCH], acc[kRowsInFlight], yv[kRowsInFlight];
+#pragma unroll
+        for (int u = 0; u < kRowsInFlight; u++) {
+            const int i = r0 + u * stride;
+            const bool ok = i < n;
+            const uint16_t* hr = Hl + (int64_t)i * H;
+            const float* zr = Zl + (int64_t)i * H;
+            acc[u] = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NCH; j++) {
+                const int k = lane + 64 * j;
+                h[u][j] = ok && k < H ? fromBf16(hr[k]) : 0.0f;
+                z[u][j] = ok && k < H && mode != 0 ? zr[k] : 0.0f;
+                acc[u] += h[u][j] * wk[j];
+            }
+            yv[u] = ok && mode == 1 ? y[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsInFlight; u++) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[u] += __shfl_xor(acc[u], off);
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsInFlight; u++) {
+            const int i = r0 + u * stride;
+            if (i >= n) break;
+            const float c = acc[u] + b0;
+            if (out && lane == 0) out[i] = c;
+            if (mode == 0) continue;
+            float dc = 1.0f;
+            if (mode == 1) {
+                const float e = c - yv[u];
+                dc = e * invN;
+                lossp += 0.5f * e * e * invN;
+                dcp += dc;
+            }
+            uint16_t* dr = dZl + (int64_t)i * H;
+#pragma unroll
+            for (int j = 0; j < NCH; j++) {
+                const int k = lane + 64 * j;
+                if (k < H) {
+                    const uint16_t dz = toBf16(dc * wk[j] * siluGrad(z[u][j]));
+                    dr[k] = dz;
+                    dbp[j] += fromBf16(dz);
+                }
+                dwp[j] += dc * h[u][j];
+            }
+        }
     }
     if (mode != 1) return;
-    // loss and bias gradient
-    sred[threadIdx.x] = valid ? 0.5f * (c - y[i]) * (c - y[i]) * invN : 0.0f;
-    sdw[threadIdx.x] = dc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) { sred[threadIdx.x] += sred[threadIdx.x + s]; sdw[threadIdx.x] += sdw[threadIdx.x + s]; }
-        __syncthreads();
+    __shared__ float sl[4][2];
+    if (lane == 0) { sl[wave][0] = lossp; sl[wave][1] = dcp; }
+    blockColumnAtomics<NCH>(dwp, H, dw);   // synchronizes the block before sl is read
+    if (threadIdx.x == 0) {
+        if (loss) atomicAdd(loss, sl[0][0] + sl[1][0] + sl[2][0] + sl[3][0]);
+        atomicAdd(db, sl[0][1] + sl[1][1] + sl[2][1] + sl[3][1]);
     }
-    if (threadIdx.x == 0) { if (loss) atomicAdd(loss, sred[0]); atomicAdd(db, sdw[0]); }
-    // dw[k] = sum_i dc_i H[i][k]
-    for (int k = 0; k < H; k++) {
-        __syncthreads();
-        sred[threadIdx.x] = valid ? dc * fromBf16(Hl[(int64_t)i * H + k]) : 0.0f;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) sred[threadIdx.x] += sred[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicAdd(&dw[k], sred[0]);
-    }
-}
-
-// db[n] = sum_rows dZ[row][n] (bf16 dZ), split over blocks; columns are strided over the
-// block when N exceeds it
-__global__ void __launch_bounds__(256) biasGradKernel(const uint16_t* __restrict__ dZ, int rows, int N,
-                                                      float* __restrict__ db) {
-    int lanesPerCol = max(1, (int)blockDim.x / N);
-    int sub = threadIdx.x / N;
-    if (sub >= lanesPerCol) return;
-    for (int col = threadIdx.x % N; col < N; col += (int)blockDim.x) {
-        float acc = 0.0f;
-        for (int r = blockIdx.x * lanesPerCol + sub; r < rows; r += gridDim.x * lanesPerCol)
-            acc += fromBf16(dZ[(int64_t)r * N + col]);
-        atomicAdd(&db[col], acc);
-        if (N <= (int)blockDim.x) break;
-    }
+    if (dbl) blockColumnAtomics<NCH>(dbp, H, dbl);
 }
 
 // grad_x c = sum_f dPhi_sin[f] cos(B_f x) B_f - dPhi_cos[f] sin(B_f x) B_f
@@ -312,8 +425,16 @@ hipError_t mlpBackward(MlpDevice& m, const float* y, int n, int mode, float* los
     const float* bOut = m.theta + m.bOffset[L];
     float* dwOut = m.grad + m.wOffset[L];
     float* dbOut = m.grad + m.bOffset[L];
-    hipLaunchKernelGGL(outputKernel, dim3(blocks1d(n)), dim3(256), 0, s, m.act[L - 1], m.pre[L - 1], n, Hd, wOut, bOut,
-                       y, 1.0f / (float)n, out, m.dz[0], dwOut, dbOut, loss, mode);
+    const unsigned rowBlocks = (unsigned)std::max(1, std::min(kRowBlocks, (n + 15) / 16));
+    float* dbLast = mode == 1 ? m.grad + m.bOffset[L - 1] : nullptr;
+    switch ((Hd + 63) / 64) {
+#define ZG_OUT(C) case C: hipLaunchKernelGGL(outputKernel<C>, dim3(rowBlocks), dim3(256), 0, s, m.act[L - 1], m.pre[L - 1], \
+                                             n, Hd, wOut, bOut, y, 1.0f / (float)n, out, m.dz[0], dwOut, dbOut, dbLast, \
+                                             loss, mode); break
+        ZG_OUT(1); ZG_OUT(2); ZG_OUT(3); ZG_OUT(4); ZG_OUT(5); ZG_OUT(6); ZG_OUT(7); ZG_OUT(8);
+#undef ZG_OUT
+        default: return hipErrorInvalidValue;   // hidden is validated to 1..512
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int cur = 0;
@@ -330,10 +451,6 @@ hipError_t mlpBackward(MlpDevice& m, const float* y, int n, int mode, float* los
             int split = std::max(1, std::min(256, n / 512));
             e = gemm(kBwdWeight, g, split, s);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(biasGradKernel, dim3(std::max(1, std::min(512, n / 64))), dim3(256), 0, s, m.dz[cur], n,
-                               Hd, m.grad + m.bOffset[l]);
-            e = hipGetLastError();
-            if (e != hipSuccess) return e;
         }
         // dH_{l-1} = dZ_l . W_l ([n x Hd] . [Hd x K]); through SiLU' for hidden layers
         GemmArgs g{};
@@ -343,6 +460,7 @@ hipError_t mlpBackward(MlpDevice& m, const float* y, int n, int mode, float* los
         g.ldc = K;
         if (l > 0) {
             g.Z = m.pre[l - 1]; g.H = m.dz[cur ^ 1];
+            g.dbias = mode == 1 ? m.grad + m.bOffset[l - 1] : nullptr;   // bias gradient of layer l - 1
             e = gemm(kBwdData, g, 1, s);
             cur ^= 1;
         } else if (mode == 2) {

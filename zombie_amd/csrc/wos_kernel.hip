@@ -211,7 +211,16 @@ wosKernel(WosParams P) {
     __syncthreads();
     uint32_t queries = 0;
 
+    VisitCount cvc{0, 0};   // countMode 2: visits of the exact traversal from the root
     auto beginQuery = [&](float bound2, float relax2, int code) {
+        if (COUNT && P.countMode == 2) {
+            // the algorithmic work of this query: an exact nearest-first traversal of the packed
+            // BVH from the root, counted, then discarded (the walk uses its own query below)
+            ClosestQuery<D> cq;
+            cq.begin(FLT_MAX, 1.0f, 0.0f);
+            st.sp = 0; st.ovf = 0;
+            while (!cq.template visit<W, true>(view, x, st, cvc)) {}
+        }
         q.begin(bound2, relax2, P.guard2);
         q.cp = yc;
         st.sp = 0; st.ovf = 0;
@@ -500,8 +509,8 @@ wosKernel(WosParams P) {
     __syncthreads();
     if (threadIdx.x < 6 && blk[threadIdx.x]) atomicAdd(&P.counters[threadIdx.x], blk[threadIdx.x]);
     if (COUNT) {
-        atomicAdd(&P.counters[6], (unsigned long long)vc.nodes);
-        atomicAdd(&P.counters[7], (unsigned long long)vc.prims);
+        atomicAdd(&P.counters[6], (unsigned long long)(P.countMode == 2 ? cvc.nodes : vc.nodes));
+        atomicAdd(&P.counters[7], (unsigned long long)(P.countMode == 2 ? cvc.prims : vc.prims));
         atomicAdd(&P.counters[8], (unsigned long long)queries);
     }
     if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 1u);
@@ -641,10 +650,34 @@ __global__ void fillZeroKernel(uint64_t* __restrict__ p, int64_t n) {
     for (; i < n; i += stride) p[i] = 0;
 }
 
+// error flags of solves that return without synchronising (no report requested): the solve's
+// counters are folded into this per-device word, read and cleared by deviceErrorsTakeAndClear
+// (zg_synchronize)
+__device__ unsigned int zgDeviceErrors;
+
+__global__ void latchErrorsKernel(const unsigned long long* __restrict__ counters) {
+    unsigned int f = (unsigned int)counters[kCntError];
+    if (f) atomicOr(&zgDeviceErrors, f);
+}
+
 // ------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------
 static inline unsigned blocksFor(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launchLatchErrors(const unsigned long long* counters, hipStream_t s) {
+    hipLaunchKernelGGL(latchErrorsKernel, dim3(1), dim3(1), 0, s, counters);
+    return hipGetLastError();
+}
+
+hipError_t deviceErrorsTakeAndClear(unsigned int* flags) {
+    unsigned int zero = 0;
+    hipError_t e = hipMemcpyFromSymbol(flags, HIP_SYMBOL(zgDeviceErrors), sizeof(unsigned int), 0,
+                                       hipMemcpyDeviceToHost);
+    if (e == hipSuccess && *flags) e = hipMemcpyToSymbol(HIP_SYMBOL(zgDeviceErrors), &zero, sizeof(unsigned int), 0,
+                                                          hipMemcpyHostToDevice);
+    return e;
+}
 
 hipError_t launchFillZero(uint64_t* p, int64_t nWords, hipStream_t s) {
     if (nWords <= 0) return hipSuccess;

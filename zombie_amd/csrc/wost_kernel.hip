@@ -138,7 +138,14 @@ __global__ void __launch_bounds__(256) wostGradFirstSphereKernel(WosParams P, fl
 // it -- the explicit first step of the walk-on-spheres gradient walks inside the first ball (no
 // boundary in it), then walk-on-stars steps from the boundary point.
 // ROBIN = true compiles the Robin reflectance and star-radius bound in (non-Neumann PDEs).
-template <int D, int W, bool COUNT, bool GRAD = false, bool ROBIN = false>
+// SPLIT = true adds walk splitting (applyWeightWindow, walk_on_stars.h:309-343): a walk whose
+// throughput exceeds splitting_threshold continues at the threshold and queues copies of its
+// state (position, normal, previous direction and distance, on-boundary flag; throughput = the
+// threshold, length 0, no contributions yet) in a per-lane private queue; each part restarts
+// with an exact absorbing distance and a fresh Green's function (estimateSolution :600-650), and
+// the walk's estimate is the sum over its parts.  When the queue cannot take the copies the walk
+// is not split (an unsplit walk is just as unbiased).
+template <int D, int W, bool COUNT, bool GRAD = false, bool ROBIN = false, bool SPLIT = false>
 __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
@@ -172,6 +179,13 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
     // gradient walks: boundary / source gradient directions, first-step source term, control variates
     Vec<D> gB = vzero<D>(), gS = vzero<D>();
     float fsc = 0.0f, cvB = 0.0f, cvS = 0.0f;
+    // splitting: parts of the current walk still to run (x, normal, previous direction, previous
+    // distance with the on-boundary flag in its sign bit), the running sum of its parts
+    struct Part { float v[3 * D + 1]; };
+    Part splitQ[SPLIT ? kSplitQueue : 1];
+    int nq = 0, nSplits = 0;
+    float partSum = 0.0f;
+    bool partOk = false;
 
     // first step of a gradient walk (walk_on_stars.h:705-791)
     auto startGradWalk = [&]() {
@@ -260,6 +274,7 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
         ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
         seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)w);
+        if (SPLIT) { nq = 0; nSplits = 0; partSum = 0.0f; partOk = false; }
         atomicAdd(&blk[5], 1ull);
         active = true;
     };
@@ -285,10 +300,15 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 term = evalField<D>(pde.dirichlet, xp, nn, al, sigma);
             }
             float total = T * term + Sref + Ssrc;
+            if (SPLIT) {
+                partSum += total;
+                partOk = true;
+            } else {
             double c = (double)total;
             atomicAdd(&P.accSum[p], c);
             atomicAdd(&P.accSq[p], c * c);
             atomicAdd(&P.accN[p], 1ull);
+            }
             if (GRAD) {
                 // gradient estimate and directional derivative (walk_on_stars.h:818-838)
                 float bc = total - fsc;
@@ -311,6 +331,36 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             }
             atomicAdd(&P.accLen[p], (unsigned long long)len);
             atomicAdd(&blk[4], (unsigned long long)len);
+        }
+        if (SPLIT) {
+            if (nq > 0) {
+                // next part of the same walk: its state, an exact absorbing distance, a star
+                // radius computed at its first step
+                nq--;
+                const Part& e = splitQ[nq];
+#pragma unroll
+                for (int i = 0; i < D; i++) { x[i] = e.v[i]; nrm[i] = e.v[D + i]; prevDir[i] = e.v[2 * D + i]; }
+                prevDist = fabsf(e.v[3 * D]);
+                onRefl = __float_as_uint(e.v[3 * D]) >> 31;
+                T = ws.splitting_threshold; Sref = 0.0f; Ssrc = 0.0f; len = 0;
+                ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
+                flip = false; firstStep = true; fsr0 = 0.0f; absPrim = -1;
+                float b2 = FLT_MAX;
+                Vec<D> cp = x;
+                int prim = -1;
+                bool gotA = closestPoint<D, W, COUNT>(sc.abs, x, b2, cp, prim, st, vc);
+                distA = gotA ? __builtin_sqrtf(b2) : FLT_MAX;
+                absPrim = gotA ? prim : -1;
+                nSplits++;
+                return;
+            }
+            if (partOk) {
+                double c = (double)partSum;
+                atomicAdd(&P.accSum[p], c);
+                atomicAdd(&P.accSq[p], c * c);
+                atomicAdd(&P.accN[p], 1ull);
+                if (nSplits) atomicAdd(&P.accSplits[p], (unsigned long long)nSplits);
+            }
         }
         w++;
         if (w < wEnd) startWalk();
@@ -462,7 +512,21 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
             T *= ball.dampening(fmaxf(1e-4f, prevDist));   // directionSampledPoissonKernel
         }
         if (!onRefl && outsideDomain<D>(sc.dmin, sc.dmax, x)) { finishWalk(ZG_ESCAPED_DOMAIN); continue; }
-        if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :309-343
+        if (SPLIT && T > ws.splitting_threshold) {  // applyWeightWindow :313-328
+            const float thr = ws.splitting_threshold;
+            float left = T - thr;
+            int copies = 0;
+            for (float l = left; l > thr && copies <= kSplitQueue; l -= thr) copies++;
+            if (nq + copies + 1 <= kSplitQueue) {
+                T = thr;
+                Part e;
+#pragma unroll
+                for (int i = 0; i < D; i++) { e.v[i] = x[i]; e.v[D + i] = nrm[i]; e.v[2 * D + i] = prevDir[i]; }
+                e.v[3 * D] = __uint_as_float(__float_as_uint(prevDist) | (onRefl ? 0x80000000u : 0u));
+                while (left > thr) { left -= thr; splitQ[nq++] = e; }
+                if (rng.nextFloat() < left / thr) splitQ[nq++] = e;
+            }
+        } else if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :329-338
             float survival = T / ws.russian_roulette_threshold;
             if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
             T = ws.russian_roulette_threshold;
@@ -489,6 +553,537 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         atomicAdd(&P.counters[kCntNodes], (unsigned long long)vc.nodes);
         atomicAdd(&P.counters[kCntPrims], (unsigned long long)vc.prims);
     }
+    if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 1u);
+}
+
+// ------------------------------------------------------------------------------------
+// interleaved walk-on-stars kernel
+// ------------------------------------------------------------------------------------
+
+// Resumable queries of one walk-on-stars step, and the points of the step where a lane waits
+// for the step logic (the kernel below).
+enum StarsQueryKind : int {
+    kQDist = 0,    // closest absorbing point for the next step (certified relaxed radius)
+    kQProj = 1,    // closest absorbing point of a walk that ended there (exact)
+    kQStar = 2,    // closest silhouette within the absorbing distance
+    kQRay = 3,     // first reflecting-boundary hit within the star radius
+    kQLos = 4,     // visibility of the reflecting-boundary sample
+    kQStep = 5,    // (no query) the next step starts: the absorbing distance is known
+    kQRadius = 6,  // (no query) the star radius is final
+};
+
+// Squared distances from x to the W children of silhouette-BVH node idx; children beyond
+// bound2, or whose normal cone rules out a silhouette seen from x (coneMaySeeSilhouette /
+// coneMaySeeSilhouette3), get FLT_MAX.
+template <int D, int W>
+ZG_DEV void starChildKeys(const BoundaryView& b, int idx, const Vec<D>& x, float bound2, float precision, bool cones,
+                          float (&key)[W], int (&ch)[W]) {
+    WideNode<D, W> nd;
+    nd.load(b.silNodes, idx);
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        float d2 = nd.dist2(k, x);
+        key[k] = d2 <= bound2 ? d2 : FLT_MAX;
+        ch[k] = nd.ch[k];
+    }
+    if (!cones) return;
+    if constexpr (D == 2) {
+        const float4* cn = reinterpret_cast<const float4*>(b.silCones) + (size_t)idx * W;
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            if (key[k] < FLT_MAX && !coneMaySeeSilhouette<W>(cn[k], nd.lo[0][k], nd.lo[1][k], nd.hi[0][k], nd.hi[1][k], x))
+                key[k] = FLT_MAX;
+    } else {
+        const float4* cn = reinterpret_cast<const float4*>(b.silCones) + (size_t)idx * W * 2;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (!(key[k] < FLT_MAX)) continue;
+            const float lo[3] = {nd.lo[0][k], nd.lo[1][k], nd.lo[2][k]};
+            const float hi[3] = {nd.hi[0][k], nd.hi[1][k], nd.hi[2][k]};
+            if (!coneMaySeeSilhouette3(cn[2 * k], cn[2 * k + 1], lo, hi, x, precision)) key[k] = FLT_MAX;
+        }
+    }
+}
+
+// Entry parameters t0 of the ray o + t dir (inv = 1 / dir) into the W child boxes of node idx
+// for t in [0, tMax]; FLT_MAX for the children it misses.
+template <int D, int W>
+ZG_DEV void slabChildKeys(const float* nodes, int idx, const Vec<D>& o, const Vec<D>& inv, float tMax, float (&key)[W],
+                          int (&ch)[W]) {
+    WideNode<D, W> nd;
+    nd.load(nodes, idx);
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        float t0 = 0.0f, t1 = tMax;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            float ta = (nd.lo[a][k] - o[a]) * inv[a], tb = (nd.hi[a][k] - o[a]) * inv[a];
+            t0 = fmaxf(t0, fminf(ta, tb));
+            t1 = fminf(t1, fmaxf(ta, tb));
+        }
+        key[k] = t0 <= t1 ? t0 : FLT_MAX;
+        ch[k] = nd.ch[k];
+    }
+}
+
+// Walk on stars with the queries of every step interleaved across the lanes of a wave.
+//
+// The run-to-completion kernel above makes a wave wait for its slowest lane in each of the
+// five traversals of a step.  Here every traversal is resumable: one loop iteration advances
+// each lane's query in flight by one work item -- an inner BVH node (its W child keys sorted,
+// the nearest held, the others pushed) or one primitive of a leaf -- whatever kind the query is
+// (closest absorbing point, closest silhouette, first ray hit, visibility).  Lanes whose query
+// has finished run the step logic between queries every `stepInterval` iterations (or at once
+// when no lane of the wave has a query in flight), so the walk arithmetic is shared by the
+// lanes that reach it together.  The walk, its random numbers and its estimates are those of
+// wostKernel<D, W, false, false, ROBIN> (WalkOnStars::walk, walk_on_stars.h:345-490, with
+// estimateSolution's rules :518-662); only the order in which the BVH items are visited
+// differs, which leaves every query's answer unchanged.  The boundary sample's stochastic
+// descent (one node per level, no stack) and the Robin radius bound run inside the step logic.
+template <int D, int W, bool ROBIN>
+__global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
+    unsigned long long spill[kSpillStack];
+    Stack st; st.init(spill);
+    const zg_walk_settings& ws = P.ws;
+    const zg_pde& pde = *P.pdeDev;
+    const SceneView& sc = P.scene;
+    const float epsA = ws.epsilon_shell_for_absorbing_boundary;
+    const float epsR = ws.epsilon_shell_for_reflecting_boundary;
+    const float prec = ws.silhouette_precision;
+    const bool hasRefl = sc.refl.nPrims > 0;
+    const bool cones = sc.refl.silCones != nullptr && prec <= sc.refl.silConePrecision;
+    const bool robin = ROBIN && !pde.are_robin_conditions_pure_neumann;
+    const float sigma = pde.absorption_coeff;
+    const bool screened = D == 3 && sigma > 0.0f;
+    const int lane = __lane_id();
+    const uint32_t streamOffset = P.streamOffsetDev ? *P.streamOffsetDev : P.streamOffset;
+
+    __shared__ unsigned long long blk[6];
+    if (threadIdx.x < 6) blk[threadIdx.x] = 0;
+    __syncthreads();
+
+    // lane stage: no walk, the claim's next walk to start, a query in flight, a result (or a
+    // step point) waiting for the step logic
+    enum { kIdle = 0, kStart = 1, kQuery = 2, kReady = 3 };
+    int stage = kIdle;
+    int qk = kQStep;
+    int p = 0, w = 0, wEnd = 0;
+
+    // walk state (WalkState, common.h:93-130)
+    Vec<D> x = vzero<D>(), nrm = vzero<D>(), prevDir = vzero<D>();
+    float prevDist = FLT_MAX, T = 1.0f, Sref = 0.0f, Ssrc = 0.0f, distA = 0.0f, fsr0 = 0.0f;
+    int len = 0;
+    int absPrim = -1;
+    bool onRefl = false, flip = false, firstStep = true;
+    Ball<D> ball; ball.yukawa = false; ball.sqrtLambda = __builtin_sqrtf(fmaxf(sigma, 0.0f)); ball.R = 0.0f;
+    Pcg32 rng; rng.state = 0; rng.inc = 1;
+    // state of the current step carried across its queries
+    float R = 0.0f, pendS = 0.0f;
+    Vec<D> dir = vzero<D>(), o = vzero<D>();
+
+    // the query in flight: current item, pruning bound (closest / silhouette: squared distance;
+    // ray / visibility: ray parameter), closest-point bookkeeping (see ClosestQuery)
+    int qItem = 0;
+    float qKey = 0.0f, qBound = FLT_MAX, qBest2 = FLT_MAX, qLower2 = FLT_MAX, qRelax2 = 1.0f, qGuard2 = 0.0f;
+    bool qHas = false, qFound = false, qRetried = false;
+    int qPrim = -1;
+    Vec<D> qCp = vzero<D>(), qd = vzero<D>(), qInv = vzero<D>();
+
+    auto beginClosest = [&](int kind, float relax2, float guard2) {
+        qk = kind; stage = kQuery;
+        qBest2 = FLT_MAX; qRelax2 = relax2; qGuard2 = guard2;
+        qBound = qBest2 > qGuard2 ? qBest2 * qRelax2 : qBest2; qLower2 = FLT_MAX;
+        qItem = 0; qKey = 0.0f; qHas = true; qPrim = -1; qFound = false; qRetried = false;
+        qCp = x;
+        st.sp = 0; st.ovf = 0;
+        if (absPrim >= 0) {
+            // the previous closest primitive seeds the bound (as closestPoint's seedPrim)
+            Prim<D> pr; pr.load(sc.abs.prims, absPrim);
+            Vec<D> qq;
+            float d2 = pr.closest2(x, qq);
+            if (d2 < qBest2) {
+                qBest2 = d2; qCp = qq; qPrim = absPrim; qFound = true;
+                qBound = qBest2 > qGuard2 ? qBest2 * qRelax2 : qBest2;
+            }
+        }
+    };
+    auto beginRayLike = [&](int kind, const Vec<D>& d, float tMax) {
+        qk = kind; stage = kQuery;
+        qd = d;
+#pragma unroll
+        for (int a = 0; a < D; a++) qInv[a] = 1.0f / d[a];
+        qBound = tMax;
+        qItem = 0; qKey = 0.0f; qHas = true; qPrim = -1; qFound = false;
+        st.sp = 0; st.ovf = 0;
+    };
+
+    // one work item of the query in flight; true once the query is complete
+    auto advance = [&]() -> bool {
+        const bool closest = qk <= kQProj;
+        if (!qHas) {
+            int n; float dn;
+            if (st.pop(n, dn)) {
+                if (dn > qBound) {
+                    if (closest && dn <= qBest2) qLower2 = fminf(qLower2, dn);
+                    return false;
+                }
+                qItem = n; qKey = dn; qHas = true;
+            } else {
+                if (!closest || qFound || qRetried) return true;
+                // nothing within the seeded bound: search again without it
+                qRetried = true;
+                qBest2 = FLT_MAX; qBound = FLT_MAX; qLower2 = FLT_MAX;
+                qItem = 0; qKey = 0.0f; qHas = true;
+            }
+        }
+        if (!isLeaf(qItem)) {
+            float key[W];
+            int ch[W];
+            if (closest) {
+                if (sc.abs.nNodes == 0) return true;
+                childDistances<D, W>(sc.abs.nodes, qItem, x, key, ch);
+            } else if (qk == kQStar) {
+                starChildKeys<D, W>(sc.refl, qItem, x, qBound, prec, cones, key, ch);
+            } else {
+                if (sc.refl.nNodes == 0) return true;
+                slabChildKeys<D, W>(sc.refl.nodes, qItem, o, qInv, qBound, key, ch);
+            }
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                if (key[k] > qBound) {
+                    if (closest && key[k] <= qBest2) qLower2 = fminf(qLower2, key[k]);
+                    key[k] = FLT_MAX;
+                }
+            }
+            ClosestQuery<D>::template sortAscending<W>(key, ch);
+            qHas = key[0] < FLT_MAX;
+            qItem = ch[0]; qKey = key[0];
+#pragma unroll
+            for (int k = W - 1; k >= 1; k--)
+                if (key[k] < FLT_MAX) st.push(ch[k], key[k]);
+            if (!qHas) return false;
+        }
+        if (isLeaf(qItem)) {
+            // one primitive of the leaf
+            const int first = leafFirstD(qItem), cnt = leafCountD(qItem);
+            bool done = false;
+            if (closest) {
+                Prim<D> pr; pr.load(sc.abs.prims, first);
+                Vec<D> qq;
+                float dd = pr.closest2(x, qq);
+                if (dd < qBest2) {
+                    qBest2 = dd; qCp = qq; qPrim = first; qFound = true;
+                    qBound = qBest2 > qGuard2 ? qBest2 * qRelax2 : qBest2;
+                }
+            } else if (qk == kQStar) {
+                float dd;
+                bool sil;
+                if constexpr (D == 2) {
+                    const float4* c = reinterpret_cast<const float4*>(sc.refl.sil) + 2 * first;
+                    float4 c0 = c[0], c1 = c[1];
+                    float dx = x[0] - c0.x, dy = x[1] - c0.y;
+                    dd = dx * dx + dy * dy;
+                    sil = dd < qBound && isSilhouette2(c0, c1, x, prec, flip);
+                } else {
+                    const float4* c = reinterpret_cast<const float4*>(sc.refl.sil) + 4 * first;
+                    float4 c0 = c[0], c1 = c[1];
+                    Vec<3> pa, pb, cp;
+                    pa[0] = c0.x; pa[1] = c0.y; pa[2] = c0.z;
+                    pb[0] = c1.x; pb[1] = c1.y; pb[2] = c1.z;
+                    dd = closestOnSegment3(pa, pb, x, cp);
+                    sil = dd < qBound && isSilhouette3(c0, c[2], c[3], x, cp, prec, flip);
+                }
+                if (sil) { qBound = dd; qFound = true; }
+                // a silhouette closer than the reflecting epsilon fixes the radius at epsilon
+                done = qFound && qBound <= epsR * epsR;
+            } else {
+                Prim<D> pr; pr.load(sc.refl.prims, first);
+                float t;
+                if (qk == kQRay) {
+                    if (pr.ray(o, qd, qBound, t) && (!qFound || t < qBound)) { qBound = t; qPrim = first; qFound = true; }
+                } else if (pr.ray(o, qd, qBound, t) && t > 0.0f && t < qBound) {
+                    qFound = true;   // visibility blocked
+                    done = true;
+                }
+            }
+            if (done) return true;
+            qHas = cnt > 1 && qKey <= qBound;
+            if (closest && cnt > 1 && !qHas && qKey <= qBest2) qLower2 = fminf(qLower2, qKey);
+            qItem = ~(((first + 1) << 4) | (cnt - 2));
+        }
+        return false;
+    };
+
+    auto startWalk = [&]() {
+        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+        Vec<D> n = P.normal ? vload<D>(P.normal + D * (size_t)p) : vzero<D>();
+        bool al = P.aligned ? P.aligned[p] != 0 : false;
+        x = vload<D>(P.pt + D * (size_t)p);
+        nrm = n; prevDir = n; prevDist = FLT_MAX; flip = false;
+        if (ws.solve_double_sided && ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY && al) {
+            nrm = n * -1.0f; prevDir = n * -1.0f; flip = true;
+        }
+        onRefl = ty == ZG_SAMPLE_ON_REFLECTING_BOUNDARY;
+        distA = P.dist0[p];
+        fsr0 = P.fsr[p];
+        firstStep = true;
+        absPrim = -1;
+        T = 1.0f; Sref = 0.0f; Ssrc = 0.0f; len = 0;
+        ball.yukawa = screened && ws.steps_before_applying_tikhonov == 0;
+        seedWalk(rng, P.seed, (uint32_t)p, streamOffset + (uint32_t)w);
+        atomicAdd(&blk[5], 1ull);
+        stage = kReady; qk = kQStep;
+    };
+
+    auto accumulate = [&](int code, float term) {
+        atomicAdd(&blk[code], 1ull);
+        if (code != ZG_ESCAPED_DOMAIN) {
+            float total = T * term + Sref + Ssrc;
+            double c = (double)total;
+            atomicAdd(&P.accSum[p], c);
+            atomicAdd(&P.accSq[p], c * c);
+            atomicAdd(&P.accN[p], 1ull);
+            atomicAdd(&P.accLen[p], (unsigned long long)len);
+            atomicAdd(&blk[4], (unsigned long long)len);
+        }
+        w++;
+        stage = w < wEnd ? kStart : kIdle;
+    };
+    auto finishWalk = [&](int code) {
+        if (code == ZG_REACHED_ABSORBING_BOUNDARY && !ws.ignore_absorbing_boundary_contribution)
+            beginClosest(kQProj, 1.0f, 0.0f);   // projection to the absorbing boundary (exact)
+        else
+            accumulate(code, 0.0f);
+    };
+
+    bool workLeft = true;
+    int iter = 0;
+    for (;;) {
+        // -------- regroup: refill idle lanes with new claims --------
+        if (iter % P.regroup == 0 || !__any(stage != kIdle)) {
+            for (int round = 0; round < 4; round++) {
+                uint64_t idle = __ballot(stage == kIdle);
+                if (!workLeft || idle == 0) break;
+                uint32_t nIdle = (uint32_t)__popcll(idle);
+                int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(P.claimCounter, (unsigned long long)nIdle);
+                base = __shfl(base, leader);
+                if (base + nIdle >= P.nClaims) workLeft = false;
+                if (stage == kIdle) {
+                    unsigned long long c = base + mbcnt64(idle);
+                    if (c < P.nClaims) {
+                        p = (int)(c / (unsigned long long)P.chunksPerPoint);
+                        int k = (int)(c - (unsigned long long)p * P.chunksPerPoint);
+                        int nw = P.nWalks[p];
+                        uint8_t ty = P.type ? P.type[p] : (uint8_t)ZG_SAMPLE_IN_DOMAIN;
+                        if (ty == ZG_SAMPLE_ON_ABSORBING_BOUNDARY) nw = 0;
+                        else if (P.dist0[p] <= epsA) nw = min(nw, 1);
+                        w = k * P.wpl;
+                        wEnd = min(w + P.wpl, nw);
+                        if (w < wEnd) stage = kStart;
+                    }
+                }
+            }
+            if (!workLeft && !__any(stage != kIdle)) break;
+        }
+        iter++;
+
+        // -------- one BVH work item of every query in flight --------
+        if (stage == kQuery && advance()) stage = kReady;
+
+        // -------- step logic --------
+        if (iter % P.stepInterval != 0 && __any(stage == kQuery)) continue;
+
+        if (stage == kReady && qk == kQProj) {
+            // terminal projection done (estimateSolution :627-645)
+            float term = 0.0f;
+            Vec<D> xp = x, nn = nrm;
+            bool al = false;
+            if (qFound) {
+                Prim<D> pr; pr.load(sc.abs.prims, qPrim);
+                Vec<D> n = pr.normal();
+                float sd = dot(x - qCp, n);
+                xp = qCp; nn = n;
+                al = ws.solve_double_sided && sd > 0.0f;
+            }
+            term = evalField<D>(pde.dirichlet, xp, nn, al, sigma);
+            accumulate(ZG_REACHED_ABSORBING_BOUNDARY, term);
+        }
+        if (stage == kStart) startWalk();
+        if (stage == kReady && qk == kQDist) {
+            // with distance_tolerance > 0 the radius is a certified lower bound of the distance
+            distA = qFound ? __builtin_sqrtf(fminf(qBest2, qLower2)) : FLT_MAX;
+            absPrim = qFound ? qPrim : -1;
+            firstStep = false;
+            qk = kQStep;
+        }
+        if (stage == kReady && qk == kQStep) {
+            // -------- a walk-on-stars step begins (walk_on_stars.h:366-392) --------
+            if (!(distA > epsA)) {
+                finishWalk(ZG_REACHED_ABSORBING_BOUNDARY);
+            } else if (firstStep && fsr0 > 0.0f) {
+                R = fsr0; qk = kQRadius;
+            } else {
+                flip = false;
+                if (ws.solve_double_sided && onRefl && prevDist > 0.0f && dot(prevDir, nrm) < 0.0f) {
+                    nrm = nrm * -1.0f;
+                    flip = true;
+                }
+                if (ws.steps_before_using_maximal_spheres <= len) {
+                    R = distA; qk = kQRadius;
+                } else if (robin && onRefl && hasNonZeroRobinCoeff<D>(pde, x)) {
+                    R = epsR; qk = kQRadius;
+                } else if (!hasRefl) {
+                    R = distA; qk = kQRadius;
+                } else if (epsR > distA || sc.refl.nSilNodes == 0) {
+                    // wostStarRadius without a traversal
+                    float r = epsR > distA ? distA : fmaxf(distA, epsR);
+                    if (ROBIN && sc.refl.robinMax && epsR <= distA)
+                        r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), epsR);
+                    if (epsR <= distA) r = fmaxf(0.99f * r, epsR);
+                    R = r; qk = kQRadius;
+                } else {
+                    // closest silhouette within [epsR, distA] (starRadius2 / starRadius3)
+                    qk = kQStar; stage = kQuery;
+                    qBound = distA < FLT_MAX ? distA * distA : FLT_MAX;
+                    qItem = 0; qKey = 0.0f; qHas = true; qFound = false;
+                    st.sp = 0; st.ovf = 0;
+                }
+            }
+        }
+        if (stage == kReady && qk == kQStar) {
+            float r = qFound ? fmaxf(__builtin_sqrtf(qBound), epsR) : fmaxf(distA, epsR);
+            // Robin primitives bound the radius too (fcpw_geometric_queries.h:1272-1289)
+            if (ROBIN && sc.refl.robinMax)
+                r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), epsR);
+            R = fmaxf(0.99f * r, epsR);
+            qk = kQRadius;
+        }
+        if (stage == kReady && qk == kQRadius) {
+            ball.update(R);
+            dir = sampleUnitSphere<D>(rng);
+            if (onRefl && dot(nrm, dir) > 0.0f) dir = dir * -1.0f;
+            o = onRefl ? offsetPoint<D>(x, nrm * -1.0f) : x;
+            // first reflecting-boundary hit within the star (intersectReflectingBoundary)
+            if (hasRefl) beginRayLike(kQRay, dir, R);
+            else { qFound = false; qBound = R; qk = kQRay; }
+        }
+        if (stage == kReady && qk == kQRay) {
+            // the step's end point: the hit, else the star's boundary
+            Vec<D> ipt, inrm = vzero<D>();
+            float idist;
+            const bool hit = qFound;
+            if (hit) {
+                ipt = hitPointD<D>(sc.refl.prims, qPrim, o, dir, qBound);
+                Prim<D> pr; pr.load(sc.refl.prims, qPrim);
+                inrm = pr.normal();
+                idist = qBound;
+            } else {
+                ipt = o + dir * R;
+                idist = R;
+            }
+            // reflecting boundary contribution (computeReflectingBoundaryContribution :189-250):
+            // the sample and its weight now, its visibility by the next query
+            pendS = 0.0f;
+            Vec<D> p2 = o;
+            bool needLos = false;
+            if (hasRefl && !ws.ignore_reflecting_boundary_contribution) {
+                Vec<D> bpt, bn;
+                float bpdf = 0.0f;
+                if (sampleReflectingD<D, W>(sc.refl, x, R, rng, bpt, bn, bpdf)) {
+                    Vec<D> toS = bpt - x;
+                    float distS = __builtin_sqrtf(norm2(toS));
+                    float alpha = onRefl ? 2.0f : 1.0f;
+                    bool al = false;
+                    if (ws.solve_double_sided) {
+                        toS = toS * (1.0f / distS);
+                        if (flip) {
+                            al = true;
+                        } else if (dot(toS, bn) < -prec) {
+                            bool flipS = alpha > 1.0f ? dot(toS, nrm) < -prec : true;
+                            if (flipS) al = true;
+                        }
+                    }
+                    Vec<D> bnS = bn * (al ? -1.0f : 1.0f);
+                    if (bpdf > 0.0f && distS < R) {
+                        p2 = offsetPoint<D>(bpt, bnS * -1.0f);
+                        float G = ball.evaluate(fmaxf(1e-4f, distS));
+                        float h = evalField<D>(pde.robin, bpt, bn, ws.solve_double_sided && al, sigma);
+                        pendS = T * alpha * G * h / bpdf;
+                        needLos = true;
+                    }
+                }
+            }
+            // the step's end point and normal wait in the walk state until the move
+            prevDist = idist;
+            prevDir = dir;
+            onRefl = hit;
+            nrm = inrm;
+            // x stays the step's start until the source sample below; the end point is kept in qCp
+            qCp = ipt;
+            if (needLos) {
+                Vec<D> d = p2 - o;
+                float L = __builtin_sqrtf(norm2(d));
+                if (L <= 0.0f || sc.refl.nNodes == 0) { qFound = false; qk = kQLos; }
+                else beginRayLike(kQLos, d * (1.0f / L), L);
+            } else {
+                pendS = 0.0f; qFound = true; qk = kQLos;
+            }
+        }
+        if (stage == kReady && qk == kQLos) {
+            if (!qFound) Sref += pendS;   // visible
+            // source contribution along the walk direction (computeSourceContribution :252-275)
+            if (!ws.ignore_source_contribution) {
+                float r = ball.sampleRadius(rng);
+                if (r <= prevDist) Ssrc += T * ball.norm() * evalField<D>(pde.source, x + dir * r, vzero<D>(), false, sigma);
+            }
+            // move; throughput (computeWalkStepThroughput :277-307)
+            x = qCp;
+            const bool yukawa = D == 3 && ball.yukawa;
+            if (onRefl && prevDist > FLT_EPSILON) {
+                if (robin || yukawa) {
+                    float refl = yukawa ? ball.dampening(prevDist) : 1.0f;
+                    if (robin) {
+                        bool fl = ws.solve_double_sided && dot(prevDir, nrm) < 0.0f;
+                        Vec<D> nr = fl ? nrm * -1.0f : nrm;
+                        float kappa = evalField<D>(pde.robin_coeff, x, nrm, fl, sigma);
+                        if (fabsf(kappa) > 0.0f) {
+                            float Pk = D == 2 ? dot(nr, prevDir) / prevDist : dot(nr, prevDir) / (prevDist * prevDist);
+                            float G = (D == 2 ? 2.0f * kPi : 4.0f * kPi) * ball.evaluate(prevDist);
+                            refl -= kappa * G / Pk;
+                        }
+                    }
+                    T *= fminf(fmaxf(refl, 0.0f), pde.are_robin_coeffs_nonnegative ? 1.0f : 2.0f);
+                }
+            } else if (yukawa) {
+                T *= ball.dampening(fmaxf(1e-4f, prevDist));   // directionSampledPoissonKernel
+            }
+            if (!onRefl && outsideDomain<D>(sc.dmin, sc.dmax, x)) {
+                accumulate(ZG_ESCAPED_DOMAIN, 0.0f);
+            } else {
+                bool alive = true;
+                if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :329-338
+                    float survival = T / ws.russian_roulette_threshold;
+                    if (survival < rng.nextFloat()) { T = 0.0f; accumulate(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE, 0.0f); alive = false; }
+                    else T = ws.russian_roulette_threshold;
+                }
+                if (alive) {
+                    len++;
+                    if (len >= ws.max_walk_length) {
+                        accumulate(ZG_EXCEEDED_MAX_WALK_LENGTH, 0.0f);
+                    } else {
+                        if (screened && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
+                        // distance to the absorbing boundary for the next step
+                        beginClosest(kQDist, P.relax2, P.guard2);
+                    }
+                }
+            }
+        }
+    }
+
+    __syncthreads();
+    if (threadIdx.x < 6 && blk[threadIdx.x]) atomicAdd(&P.counters[threadIdx.x], blk[threadIdx.x]);
     if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 1u);
 }
 
@@ -597,9 +1192,30 @@ hipError_t launchWostGrad(const WosParams& P, int dim, int width, int gridBlocks
     return hipGetLastError();
 }
 
-hipError_t launchWost(const WosParams& P, int dim, int width, bool count, int gridBlocks, hipStream_t s) {
+hipError_t launchWost(const WosParams& P, int dim, int width, bool count, bool split, bool interleave, int gridBlocks,
+                      hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
     const bool robin = !P.pde.are_robin_conditions_pure_neumann;
+    if (interleave && !count && !split) {
+#define ZG_WOSTI(Dd, Wd, R) hipLaunchKernelGGL((wostInterleavedKernel<Dd, Wd, R>), g, bl, kStackLdsBytes, s, P)
+        if (dim == 3) {
+            if (width == 4) { if (robin) ZG_WOSTI(3, 4, true); else ZG_WOSTI(3, 4, false); }
+            else { if (robin) ZG_WOSTI(3, 8, true); else ZG_WOSTI(3, 8, false); }
+        } else if (width == 4) { if (robin) ZG_WOSTI(2, 4, true); else ZG_WOSTI(2, 4, false); }
+        else { if (robin) ZG_WOSTI(2, 8, true); else ZG_WOSTI(2, 8, false); }
+#undef ZG_WOSTI
+        return hipGetLastError();
+    }
+    if (split) {   // no visit counting with splitting
+#define ZG_WOSTS(Dd, Wd, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, false, false, R, true>), g, bl, kStackLdsBytes, s, P)
+        if (dim == 3) {
+            if (width == 4) { if (robin) ZG_WOSTS(3, 4, true); else ZG_WOSTS(3, 4, false); }
+            else { if (robin) ZG_WOSTS(3, 8, true); else ZG_WOSTS(3, 8, false); }
+        } else if (width == 4) { if (robin) ZG_WOSTS(2, 4, true); else ZG_WOSTS(2, 4, false); }
+        else { if (robin) ZG_WOSTS(2, 8, true); else ZG_WOSTS(2, 8, false); }
+#undef ZG_WOSTS
+        return hipGetLastError();
+    }
 #define ZG_WOST(Dd, Wd, C, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, C, false, R>), g, bl, kStackLdsBytes, s, P)
     if (dim == 3) {
         if (width == 4) {

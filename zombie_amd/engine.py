@@ -160,8 +160,9 @@ class SolveOptions:
     regroup_interval: int = 16
     step_interval: int = 4
     waves_per_cu: int = 0
-    count_visits: bool = False
+    count_visits: int = 0   # 1: count the kernel's own BVH visits; 2: those of an exact traversal from the root
     distance_tolerance: float = 0.0
+    wost_run_to_completion: bool = False   # walk on stars: run each query of a step to completion
 
     def to_c(self) -> A.zg_solve_options:
         o = A.zg_solve_options()
@@ -171,8 +172,9 @@ class SolveOptions:
         o.regroup_interval = self.regroup_interval
         o.step_interval = self.step_interval
         o.waves_per_cu = self.waves_per_cu
-        o.count_visits = 1 if self.count_visits else 0
+        o.count_visits = int(self.count_visits)
         o.distance_tolerance = self.distance_tolerance
+        o.wost_run_to_completion = int(bool(self.wost_run_to_completion))
         return o
 
 
@@ -254,6 +256,18 @@ class Scene:
         out = torch.empty(pts.shape[0], dtype=torch.uint8, device=pts.device)
         A.check(A.lib().zg_inside_domain(self._h, pts.shape[0], _ptr(pts), _ptr(out), _stream_ptr()))
         return out
+
+    def copy_bvh(self, boundary: int = 0):
+        """The packed wide BVH of one boundary part on the host: (nodes, primitives) as float32
+        arrays in the device layout (zg_scene_copy_bvh)."""
+        info = self.info
+        nn = info.n_absorbing_nodes if boundary == 0 else info.n_reflecting_nodes
+        npr = info.n_absorbing_prims if boundary == 0 else info.n_reflecting_prims
+        nodes = np.empty(max(1, nn * info.node_bytes // 4), np.float32)
+        prims = np.empty(max(1, npr * info.prim_bytes // 4), np.float32)
+        A.check(A.lib().zg_scene_copy_bvh(self._h, boundary, nodes.ctypes.data, nodes.size, prims.ctypes.data,
+                                          prims.size))
+        return nodes[: nn * info.node_bytes // 4], prims[: npr * info.prim_bytes // 4]
 
     def intersect_ray(self, origins, dirs, t_max, boundary: int = 0):
         torch = _torch()
@@ -537,16 +551,23 @@ def _validate_solve(dim: int, points: DevicePoints, stats: DeviceStats, instance
 
 
 def wos_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
-              opts: SolveOptions | None = None, stream=None) -> SolveReport:
+              opts: SolveOptions | None = None, stream=None, report: bool = True) -> SolveReport | None:
+    """zg_wos_solve.  report=False returns as soon as the work is enqueued on `stream` (no host
+    wait; device-side errors surface at synchronize()) and returns None."""
     opts = opts or SolveOptions()
     _validate_solve(scene.dim, points, stats)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
-    rep = A.zg_solve_report()
+    rep = A.zg_solve_report() if report else None
     A.check(A.lib().zg_wos_solve(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st), C.byref(oc),
-                                 C.byref(rep), _stream_ptr(stream)))
-    return SolveReport.from_c(rep)
+                                 C.byref(rep) if report else None, _stream_ptr(stream)))
+    return SolveReport.from_c(rep) if report else None
+
+
+def synchronize() -> None:
+    """zg_synchronize: waits for the device and raises the errors of solves run without a report."""
+    A.check(A.lib().zg_synchronize())
 
 
 def wos_solve_noalloc(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints,
@@ -664,16 +685,17 @@ class SolveGraph:
 
 
 def wost_solve(scene: Scene, pde: A.zg_pde, ws: A.zg_walk_settings, points: DevicePoints, stats: DeviceStats,
-               opts: SolveOptions | None = None, stream=None) -> SolveReport:
+               opts: SolveOptions | None = None, stream=None, report: bool = True) -> SolveReport | None:
+    """zg_wost_solve; report=False as in wos_solve."""
     opts = opts or SolveOptions()
     _validate_solve(scene.dim, points, stats)
     sp = points.to_c()
     st = stats.to_c()
     oc = opts.to_c()
-    rep = A.zg_solve_report()
+    rep = A.zg_solve_report() if report else None
     A.check(A.lib().zg_wost_solve(scene.handle, C.byref(pde), C.byref(ws), C.byref(sp), C.byref(st), C.byref(oc),
-                                  C.byref(rep), _stream_ptr(stream)))
-    return SolveReport.from_c(rep)
+                                  C.byref(rep) if report else None, _stream_ptr(stream)))
+    return SolveReport.from_c(rep) if report else None
 
 
 # ------------------------------------------------------------------------------------

@@ -50,6 +50,14 @@ class MLP:
         A.check(A.lib().zg_mlp_parameter_count(self._h, C.byref(n)))
         self.n_params = n.value
 
+    def flops_per_sample(self) -> int:
+        """Dense-layer FLOPs of one training sample: forward 2 * MACs (features -> hidden,
+        hidden -> hidden, hidden -> 1), backward twice that (input and weight gradients)."""
+        c = self.config
+        feat = 2 * c.n_frequencies
+        macs = feat * c.hidden + (c.n_hidden_layers - 1) * c.hidden * c.hidden + c.hidden
+        return 3 * 2 * macs
+
     def close(self):
         if getattr(self, "_h", None):
             A.lib().zg_mlp_destroy(self._h)
