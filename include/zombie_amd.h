@@ -348,11 +348,14 @@ typedef struct zg_solve_options {
                                  lower bound instead (a ball inside the domain keeps walk on
                                  spheres unbiased; walks get slightly longer).  Queries within
                                  2 epsilon of the boundary and terminal projections stay exact. */
-    int32_t wost_run_to_completion; /* walk on stars: 0 (default) advances every lane's star-radius,
-                                 ray, visibility and distance queries one BVH item per loop
-                                 iteration, interleaved across the lanes of a wave; 1 runs each
-                                 query of a step to completion inside the step (same walks and
-                                 estimates; the kernel used with walk splitting and gradients) */
+    int32_t wost_interleave_queries; /* walk on stars: 0 (default) runs each query of a step (star
+                                 radius, ray, visibility, distance) to completion inside the step;
+                                 1 advances every lane's query one BVH item per loop iteration,
+                                 interleaved across the lanes of a wave, with the step logic every
+                                 step_interval iterations (same walks and estimates, slower on
+                                 the measured scenes; not with splitting or gradient points).
+                                 count_visits 1 in walk on stars uses the interleaved kernel's
+                                 counts, which cover the reflecting and silhouette BVHs too. */
 } zg_solve_options;
 
 void zg_solve_options_default(zg_solve_options* opts);
@@ -361,11 +364,17 @@ typedef struct zg_solve_report {
     int64_t completion_codes[4]; /* walks per WalkCompletionCode                         */
     int64_t total_steps;         /* sum of walkLength over completed walks               */
     int64_t total_walks;         /* walks started                                        */
-    int64_t node_visits;         /* count_visits only: BVH nodes visited (all queries)   */
-    int64_t prim_visits;         /* count_visits only: primitives tested                 */
+    int64_t node_visits;         /* count_visits only: BVH nodes visited (all queries; walk on
+                                    stars: the absorbing BVH's)                           */
+    int64_t prim_visits;         /* count_visits only: primitives tested (as node_visits) */
     int64_t queries;             /* count_visits only: closest-point / traversal queries */
     float kernel_ms;             /* walk-kernel time, HIP events on `stream`             */
     float reserved;
+    /* walk on stars with count_visits 1 (interleaved kernel): nodes and primitives read on the
+       reflecting boundary's BVH (ray, visibility, boundary sample, Robin bound) and on the
+       silhouette BVH (star radius; primitives = silhouette candidates) */
+    int64_t reflecting_node_visits, reflecting_prim_visits;
+    int64_t silhouette_node_visits, silhouette_prim_visits;
 } zg_solve_report;
 
 /* WalkOnSpheres<float,DIM>::solve (walk_on_spheres.h:131-168 -> estimateSolution :320-414,

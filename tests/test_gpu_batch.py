@@ -189,3 +189,49 @@ def test_batch_autotuner_covers_its_grid(gpu, batch_setup):
         {(w, l, g) for w in (4, 8) for l in (1, 2, 4) for g in (8, 16, 32)}
     assert res.walks_per_s == max(r["walks_per_s"] for r in res.table)
     assert res.bvh_width in (4, 8) and res.walks_per_lane in (1, 2, 4) and res.regroup_interval in (8, 16, 32)
+
+
+def test_uneven_instances_reproduce_single_scene_solves_and_closed_form(gpu, oracle_lib):
+    """Instances of very different sizes in one launch, as in the benchmarked batch (geodesic
+    spheres of 80 to 20,480 triangles, 40 to 640 points each, sigma from 0 to 9): walk for walk
+    equal to single-scene solves, each instance's pooled error within 4 SE of exp(a . x), and
+    the largest instance within the 4-SE rule of the oracle."""
+    torch = gpu
+    O = oracle_lib
+    sizes = [(80, 40, 0.0), (320, 80, 9.0), (1280, 160, 1.0), (5120, 320, 4.0), (20480, 640, 0.25)]
+    rng = np.random.default_rng(17)
+    probs, scenes, pts, dist, inst = [], [], [], [], []
+    for i, (nt, npts, sigma) in enumerate(sizes):
+        pr = PB.screened_instance(nt, sigma, rng.normal(size=3), seed=300 + i)
+        sc = E.Scene(pr.positions, pr.indices, None, pr.domain_min, pr.domain_max, bvh_width=4, leaf_size=2)
+        p, d = _instance_points(torch, pr, sc, npts, 400 + i)
+        probs.append(pr); scenes.append(sc); pts.append(p); dist.append(d)
+        inst.append(np.full(len(p), i, np.int32))
+    tris = [len(pr.indices) for pr in probs]
+    assert min(tris) <= 100 and max(tris) >= 20000, tris
+    batch = E.Batch(scenes, [pr.pde for pr in probs])
+    P, D, I = np.concatenate(pts), np.concatenate(dist), np.concatenate(inst)
+    n = len(P)
+    opts = E.SolveOptions(seed=41, walks_per_lane=1, regroup_interval=16, distance_tolerance=0.1)
+    st = E.DeviceStats.zeros(n)
+    rep = E.batch_wos_solve(batch, probs[0].settings, _points(torch, P, D, 256), torch.from_numpy(I).cuda(), st, opts)
+    g = _host(st)
+    assert rep.total_walks == 256 * n
+    for i, pr in enumerate(probs):
+        sel = I == i
+        nw = np.where(sel, 256, 0).astype(np.int32)
+        di = np.where(sel, D, 1.0).astype(np.float32)
+        si = E.DeviceStats.zeros(n)
+        E.wos_solve(scenes[i], pr.pde, pr.settings, _points(torch, P, di, nw), si, opts)
+        _same(g, _host(si), sel, rtol=2e-6)
+        err = g["mean"][sel] - PB.exp_linear(P[sel], pr.meta["a"])
+        pooled = err.sum() / np.sqrt((g["var"][sel] / g["n"][sel]).sum())
+        assert abs(pooled) < 4.0, f"instance {i} ({tris[i]} triangles, sigma {sizes[i][2]}): pooled z {pooled:.2f}"
+    big = len(sizes) - 1
+    sel = I == big
+    pr = probs[big]
+    osc = O.Scene(pr.positions, pr.indices, None, pr.domain_min, pr.domain_max)
+    ost = O.Stats(int(sel.sum()))
+    O.solve(osc, pr.pde, pr.settings, P[sel], D[sel], 256, ost, seed=43, rng_mode=O.RNG_PER_POINT)
+    z, ok = z_scores(g["mean"][sel], g["var"][sel], g["n"][sel], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, "uneven batch: largest instance vs oracle")

@@ -389,10 +389,11 @@ def test_wost_splitting_matches_oracle_and_closed_form(gpu, oracle_lib, mixed, t
 def _same_walks(torch, sc, p, pts, d, seed, n_walks=128, tol=0.2):
     """One solve with each walk-on-stars kernel on the same walk streams."""
     out = []
-    for rtc in (False, True):
+    for inter in (True, False):
         st = E.DeviceStats.zeros(len(pts))
         rep = E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, n_walks), st,
-                           E.SolveOptions(seed=seed, distance_tolerance=tol, wost_run_to_completion=rtc))
+                           E.SolveOptions(seed=seed, distance_tolerance=tol, step_interval=8,
+                                          wost_interleave_queries=inter))
         out.append((rep, _host(st)))
     return out
 
@@ -419,3 +420,43 @@ def test_interleaved_kernel_walks_equal_run_to_completion(gpu, mixed, robin, wid
     np.testing.assert_array_equal(gi["len"], gr["len"])
     np.testing.assert_allclose(gi["mean"], gr["mean"], rtol=1e-9, atol=1e-9)
     np.testing.assert_allclose(gi["m2"], gr["m2"], rtol=1e-7, atol=1e-9)
+    # the visit-counting variant runs the same walks and counts every BVH the step reads
+    st = E.DeviceStats.zeros(len(pts))
+    rc = E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, 128), st,
+                      E.SolveOptions(seed=61, distance_tolerance=0.2, count_visits=1))
+    assert rc.total_steps == ri.total_steps and rc.completion_codes == ri.completion_codes
+    assert rc.node_visits > rc.total_steps and rc.prim_visits > 0
+    assert rc.reflecting_node_visits > rc.total_steps and rc.reflecting_prim_visits > 0
+    assert rc.silhouette_node_visits > 0 and rc.silhouette_prim_visits > 0
+
+
+def test_wost_benchmarked_configuration(gpu, oracle_lib):
+    """The benchmarked walk-on-stars problem at its full size (4,096 segments, 32 double-sided
+    Neumann slits) with the bench's launch shape and distance tolerance 0.2, against the
+    oracle's exact-radius walks (4-SE rule) and against u = x^3 - 3xy^2."""
+    torch = gpu
+    O = oracle_lib
+    p = PB.mixed2d()
+    assert len(p.indices) == 4096 + 32
+    osc = O.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, solve_double_sided=True)
+    V, I = p.meta["outer"]
+    cand = PB.sample_box(p.domain_min, p.domain_max, 1500, 23)
+    pts = cand[PB.points_in_polygon(cand, V, I)][:300]
+    d, *_ = osc.closest_point(pts, boundary=0)
+    sc = E.Scene(p.positions, p.indices, p.is_reflecting, p.domain_min, p.domain_max, bvh_width=4, leaf_size=2,
+                 solve_double_sided=True)
+    st = E.DeviceStats.zeros(len(pts))
+    rep = E.wost_solve(sc, p.pde, p.settings, _device_points(torch, pts, d, 256), st,
+                       E.SolveOptions(seed=25, walks_per_lane=1, regroup_interval=16, distance_tolerance=0.2))
+    g = _host(st)
+    ost = O.Stats(len(pts))
+    orep = O.solve(osc, p.pde, p.settings, pts, d, 256, ost, seed=26, wost=True, rng_mode=O.RNG_PER_POINT)
+    walks = 256 * len(pts)
+    assert rep.total_walks == walks
+    assert rep.completion_codes[3] <= 1e-3 * walks and orep["codes"][3] <= 1e-3 * walks
+    z, ok = z_scores(g["mean"], g["var"], g["n"], ost.mean, ost.variance(), ost.n)
+    check_four_se(z, ok, "wost benchmarked configuration vs oracle")
+    pooled = (g["mean"] - p.exact(pts)).sum() / np.sqrt((g["var"] / g["n"]).sum())
+    assert abs(pooled) < 4.0, pooled
+    ml_g, ml_o = g["len"].sum() / g["n"].sum(), ost.walk_length.sum() / ost.n.sum()
+    assert abs(ml_g / ml_o - 1) < 0.05, (ml_g, ml_o)

@@ -332,12 +332,13 @@ def test_interleaved_kernel_walks_equal_run_to_completion(gpu, oracle_lib, case)
     d = osc.closest_point(pts, boundary=0)[0]
     n = len(pts)
     res = []
-    for rtc in (False, True):
+    for inter in (True, False):
         st = E.DeviceStats.zeros(n)
         dp = E.DevicePoints(pt=_t(torch, pts), dist_to_absorbing=_t(torch, d),
                             n_walks=torch.full((n,), 128, dtype=torch.int32, device="cuda"), n_walks_max=128)
         rep = E.wost_solve(sc, p.pde, p.settings, dp, st,
-                           E.SolveOptions(seed=71, distance_tolerance=0.2, wost_run_to_completion=rtc))
+                           E.SolveOptions(seed=71, distance_tolerance=0.2, step_interval=8,
+                                          wost_interleave_queries=inter))
         res.append((rep, _host(st)))
     (ri, gi), (rr, gr) = res
     print(f"interleaved 3d {case}: steps {ri.total_steps} vs {rr.total_steps}, codes {ri.completion_codes} vs "

@@ -194,7 +194,7 @@ class zg_solve_options(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("count_visits", C.c_int32),
         ("distance_tolerance", C.c_float),
-        ("wost_run_to_completion", C.c_int32),
+        ("wost_interleave_queries", C.c_int32),
     ]
 
 
@@ -208,6 +208,10 @@ class zg_solve_report(C.Structure):
         ("queries", C.c_int64),
         ("kernel_ms", C.c_float),
         ("reserved", C.c_float),
+        ("reflecting_node_visits", C.c_int64),
+        ("reflecting_prim_visits", C.c_int64),
+        ("silhouette_node_visits", C.c_int64),
+        ("silhouette_prim_visits", C.c_int64),
     ]
 
 

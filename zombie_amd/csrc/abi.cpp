@@ -497,7 +497,7 @@ struct SolveSetup {
     WosParams P;
     MergeParams M;
     int dim = 3, width = 8, blocks = 1, depth = 0;
-    bool batch = false, count = false, wost = false, screened = false, split = false, interleave = true;
+    bool batch = false, count = false, wost = false, screened = false, split = false, interleave = false;
     int64_t n = 0;
     int nWalksMax = 0;
     size_t words = 0;
@@ -554,8 +554,8 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     if (o.count_visits == 2 && wost) return fail(ZG_ERR_UNSUPPORTED, "count_visits 2 runs in walk on spheres only");
     if (!(o.distance_tolerance >= 0.0f && o.distance_tolerance < 0.5f))
         return fail(ZG_ERR_INVALID_ARGUMENT, "distance_tolerance must be in [0, 0.5)");
-    if (o.wost_run_to_completion != 0 && o.wost_run_to_completion != 1)
-        return fail(ZG_ERR_INVALID_ARGUMENT, "wost_run_to_completion must be 0 or 1");
+    if (o.wost_interleave_queries != 0 && o.wost_interleave_queries != 1)
+        return fail(ZG_ERR_INVALID_ARGUMENT, "wost_interleave_queries must be 0 or 1");
     if ((uint64_t)o.stream_offset + (uint64_t)pts->n_walks_max > 0xffffffffull)
         return fail(ZG_ERR_INVALID_ARGUMENT, "stream_offset + n_walks_max must fit in 32 bits");
     int dev = 0;
@@ -628,7 +628,7 @@ zg_status prepareSolve(const zg_scene* s, const zg_batch* b, const zg_pde* pde, 
     if (P.quantity) S.words += (size_t)(n + 1) / 2;   // masked walk counts, last in the workspace
     S.wost = wost;
     S.split = split;
-    S.interleave = o.wost_run_to_completion == 0;
+    S.interleave = o.wost_interleave_queries != 0;
     M.sSplits = stats->total_splits;
     if (wost) {
         S.callerFsr = pts->first_sphere_radius;
@@ -699,6 +699,10 @@ void fillReport(const unsigned long long* cnt, float ms, zg_solve_report* report
     report->queries = (int64_t)cnt[kCntQueries];
     report->kernel_ms = ms;
     report->reserved = 0.0f;
+    report->reflecting_node_visits = (int64_t)cnt[kCntReflNodes];
+    report->reflecting_prim_visits = (int64_t)cnt[kCntReflPrims];
+    report->silhouette_node_visits = (int64_t)cnt[kCntSilNodes];
+    report->silhouette_prim_visits = (int64_t)cnt[kCntSilPrims];
 }
 
 // gradient layout after the solve workspace: sums and squares of the gradient estimates

@@ -1180,19 +1180,21 @@ ZG_DEV void robinTriangleBound(const Prim<3>& pr, float kmax, const Vec<3>& x, f
 
 // minimum of the Robin bounds of the reflecting primitives within sqrt(r2) of x
 template <int D, int W>
-ZG_DEV float robinRadius2(const BoundaryView& b, const Vec<D>& x, float r2, Stack& st) {
+ZG_DEV float robinRadius2(const BoundaryView& b, const Vec<D>& x, float r2, Stack& st, VisitCount* vc = nullptr) {
     if (b.nNodes == 0 || !b.robinMax) return r2;
     st.sp = 0; st.ovf = 0;
     int node = 0;
     for (;;) {
         WideNode<D, W> nd;
         nd.load(b.nodes, node);
+        if (vc) vc->nodes++;
 #pragma unroll
         for (int k = 0; k < W; k++) {
             if (nd.dist2(k, x) > r2) continue;
             int c = nd.ch[k];
             if (isLeaf(c)) {
                 int first = leafFirstD(c), cnt = leafCountD(c);
+                if (vc) vc->prims += cnt;
                 for (int j = first; j < first + cnt; j++) {
                     Prim<D> pr; pr.load(b.prims, j);
                     if constexpr (D == 2) robinSegmentBound(pr, b.robinMax[j], x, r2);
@@ -1269,7 +1271,7 @@ ZG_DEV float branchWeight(float r2) {
 // alpha G h / pdf is unbiased.
 template <int W>
 ZG_DEV bool sampleReflecting2(const BoundaryView& b, const Vec<2>& x, float radius, float u0, float u1,
-                              Vec<2>& pt, Vec<2>& nrm, float& pdf) {
+                              Vec<2>& pt, Vec<2>& nrm, float& pdf, VisitCount* vc = nullptr) {
     if (b.nNodes == 0) return false;
     const float R2 = radius * radius;
     float prob = 1.0f;
@@ -1278,6 +1280,7 @@ ZG_DEV bool sampleReflecting2(const BoundaryView& b, const Vec<2>& x, float radi
     for (int depth = 0; depth < 64; depth++) {
         WideNode<2, W> nd;
         nd.load(b.nodes, node);
+        if (vc) vc->nodes++;
         float w[W];
         float total = 0.0f;
 #pragma unroll
@@ -1304,6 +1307,7 @@ ZG_DEV bool sampleReflecting2(const BoundaryView& b, const Vec<2>& x, float radi
         int c = nd.ch[sel];
         if (!isLeaf(c)) { node = c; continue; }
         int first = leafFirstD(c), cnt = leafCountD(c);
+        if (vc) vc->prims += cnt;
         float lw[16];
         float ltotal = 0.0f;
         for (int j = 0; j < cnt; j++) {
@@ -1520,7 +1524,7 @@ ZG_DEV bool lineOfSight3(const BoundaryView& b, const Vec<3>& p1, const Vec<3>& 
 // point on the triangle from (u1, u2) by the square-root warp.  pdf is the exact density.
 template <int W>
 ZG_DEV bool sampleReflecting3(const BoundaryView& b, const Vec<3>& x, float radius, float u0, float u1, float u2,
-                              Vec<3>& pt, Vec<3>& nrm, float& pdf) {
+                              Vec<3>& pt, Vec<3>& nrm, float& pdf, VisitCount* vc = nullptr) {
     if (b.nNodes == 0) return false;
     const float R2 = radius * radius;
     float prob = 1.0f;
@@ -1529,6 +1533,7 @@ ZG_DEV bool sampleReflecting3(const BoundaryView& b, const Vec<3>& x, float radi
     for (int depth = 0; depth < 64; depth++) {
         WideNode<3, W> nd;
         nd.load(b.nodes, node);
+        if (vc) vc->nodes++;
         float w[W];
         float total = 0.0f;
 #pragma unroll
@@ -1555,6 +1560,7 @@ ZG_DEV bool sampleReflecting3(const BoundaryView& b, const Vec<3>& x, float radi
         int c = nd.ch[sel];
         if (!isLeaf(c)) { node = c; continue; }
         int first = leafFirstD(c), cnt = leafCountD(c);
+        if (vc) vc->prims += cnt;
         float lw[16];
         float ltotal = 0.0f;
         for (int j = 0; j < cnt; j++) {

@@ -12,7 +12,11 @@ namespace zg {
 enum CounterSlot {
     kCntReached = 0, kCntRR = 1, kCntMaxLen = 2, kCntEscaped = 3,
     kCntSteps = 4, kCntWalks = 5, kCntNodes = 6, kCntPrims = 7, kCntQueries = 8,
-    kCntClaims = 9, kCntError = 10, kCntSlots = 16
+    kCntClaims = 9, kCntError = 10,
+    // walk on stars, visit counting: reflecting-boundary BVH (ray, visibility, boundary sample,
+    // Robin bound) and silhouette BVH (star radius) nodes and primitives
+    kCntReflNodes = 11, kCntReflPrims = 12, kCntSilNodes = 13, kCntSilPrims = 14,
+    kCntSlots = 16
 };
 
 // one instance of a batched solve: scene part walked (absorbing boundary), bounding domain, PDE

@@ -61,16 +61,16 @@ ZG_DEV bool hasNonZeroRobinCoeff(const zg_pde& pde, const Vec<D>& x) {
 // numbers, as sampleReflectingBoundary draws them), then the visibility test
 template <int D, int W>
 ZG_DEV bool sampleReflectingD(const BoundaryView& b, const Vec<D>& x, float radius, Pcg32& rng, Vec<D>& pt, Vec<D>& nrm,
-                              float& pdf) {
+                              float& pdf, VisitCount* vc = nullptr) {
     if constexpr (D == 2) {
         float u0 = rng.nextFloat();
         float u1 = rng.nextFloat();
-        return sampleReflecting2<W>(b, x, radius, u0, u1, pt, nrm, pdf);
+        return sampleReflecting2<W>(b, x, radius, u0, u1, pt, nrm, pdf, vc);
     } else {
         float u0 = rng.nextFloat();
         float u1 = rng.nextFloat();
         float u2 = rng.nextFloat();
-        return sampleReflecting3<W>(b, x, radius, u0, u1, u2, pt, nrm, pdf);
+        return sampleReflecting3<W>(b, x, radius, u0, u1, u2, pt, nrm, pdf, vc);
     }
 }
 template <int D, int W>
@@ -640,10 +640,23 @@ ZG_DEV void slabChildKeys(const float* nodes, int idx, const Vec<D>& o, const Ve
 // estimateSolution's rules :518-662); only the order in which the BVH items are visited
 // differs, which leaves every query's answer unchanged.  The boundary sample's stochastic
 // descent (one node per level, no stack) and the Robin radius bound run inside the step logic.
-template <int D, int W, bool ROBIN>
+//
+// Opt-in (wost_interleave_queries): on the measured scenes it is slower than the
+// run-to-completion kernel.  On the 2D mixed problem (4,096 segments, 32 slits; 50,000 points x
+// 128 walks) it takes 4.6 s at a step interval of 8 against 2.7 s: the lanes of a wave sit in
+// up to five query kinds, so each iteration runs the node and leaf code of every kind present
+// (VALU lane utilisation 9% against 14%, 517 against 262 VALU instructions per step), which
+// costs more than the wait for the slowest lane it removes.  Advancing only the most common
+// kind per iteration was slower still (5.1 s).  Its visit-counting variant is the one that
+// counts the reflecting and silhouette BVH reads of a step.
+// COUNT = true counts the BVH nodes and primitives every query reads, per BVH: absorbing
+// (distance and projection), reflecting (ray, visibility, boundary sample, Robin bound) and
+// silhouette (star radius).
+template <int D, int W, bool ROBIN, bool COUNT = false>
 __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
     unsigned long long spill[kSpillStack];
     Stack st; st.init(spill);
+    VisitCount vcAbs{0, 0}, vcRefl{0, 0}, vcSil{0, 0};
     const zg_walk_settings& ws = P.ws;
     const zg_pde& pde = *P.pdeDev;
     const SceneView& sc = P.scene;
@@ -698,6 +711,7 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
         st.sp = 0; st.ovf = 0;
         if (absPrim >= 0) {
             // the previous closest primitive seeds the bound (as closestPoint's seedPrim)
+            if (COUNT) vcAbs.prims++;
             Prim<D> pr; pr.load(sc.abs.prims, absPrim);
             Vec<D> qq;
             float d2 = pr.closest2(x, qq);
@@ -742,11 +756,14 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
             if (closest) {
                 if (sc.abs.nNodes == 0) return true;
                 childDistances<D, W>(sc.abs.nodes, qItem, x, key, ch);
+                if (COUNT) vcAbs.nodes++;
             } else if (qk == kQStar) {
                 starChildKeys<D, W>(sc.refl, qItem, x, qBound, prec, cones, key, ch);
+                if (COUNT) vcSil.nodes++;
             } else {
                 if (sc.refl.nNodes == 0) return true;
                 slabChildKeys<D, W>(sc.refl.nodes, qItem, o, qInv, qBound, key, ch);
+                if (COUNT) vcRefl.nodes++;
             }
 #pragma unroll
             for (int k = 0; k < W; k++) {
@@ -767,6 +784,11 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
             // one primitive of the leaf
             const int first = leafFirstD(qItem), cnt = leafCountD(qItem);
             bool done = false;
+            if (COUNT) {
+                if (closest) vcAbs.prims++;
+                else if (qk == kQStar) vcSil.prims++;
+                else vcRefl.prims++;
+            }
             if (closest) {
                 Prim<D> pr; pr.load(sc.abs.prims, first);
                 Vec<D> qq;
@@ -940,7 +962,7 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
                     // wostStarRadius without a traversal
                     float r = epsR > distA ? distA : fmaxf(distA, epsR);
                     if (ROBIN && sc.refl.robinMax && epsR <= distA)
-                        r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), epsR);
+                        r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st, COUNT ? &vcRefl : nullptr))), epsR);
                     if (epsR <= distA) r = fmaxf(0.99f * r, epsR);
                     R = r; qk = kQRadius;
                 } else {
@@ -956,7 +978,7 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
             float r = qFound ? fmaxf(__builtin_sqrtf(qBound), epsR) : fmaxf(distA, epsR);
             // Robin primitives bound the radius too (fcpw_geometric_queries.h:1272-1289)
             if (ROBIN && sc.refl.robinMax)
-                r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st))), epsR);
+                r = fmaxf(fminf(r, __builtin_sqrtf(robinRadius2<D, W>(sc.refl, x, distA < FLT_MAX ? distA * distA : FLT_MAX, st, COUNT ? &vcRefl : nullptr))), epsR);
             R = fmaxf(0.99f * r, epsR);
             qk = kQRadius;
         }
@@ -991,7 +1013,7 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
             if (hasRefl && !ws.ignore_reflecting_boundary_contribution) {
                 Vec<D> bpt, bn;
                 float bpdf = 0.0f;
-                if (sampleReflectingD<D, W>(sc.refl, x, R, rng, bpt, bn, bpdf)) {
+                if (sampleReflectingD<D, W>(sc.refl, x, R, rng, bpt, bn, bpdf, COUNT ? &vcRefl : nullptr)) {
                     Vec<D> toS = bpt - x;
                     float distS = __builtin_sqrtf(norm2(toS));
                     float alpha = onRefl ? 2.0f : 1.0f;
@@ -1084,6 +1106,14 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
 
     __syncthreads();
     if (threadIdx.x < 6 && blk[threadIdx.x]) atomicAdd(&P.counters[threadIdx.x], blk[threadIdx.x]);
+    if (COUNT) {
+        atomicAdd(&P.counters[kCntNodes], (unsigned long long)vcAbs.nodes);
+        atomicAdd(&P.counters[kCntPrims], (unsigned long long)vcAbs.prims);
+        atomicAdd(&P.counters[kCntReflNodes], (unsigned long long)vcRefl.nodes);
+        atomicAdd(&P.counters[kCntReflPrims], (unsigned long long)vcRefl.prims);
+        atomicAdd(&P.counters[kCntSilNodes], (unsigned long long)vcSil.nodes);
+        atomicAdd(&P.counters[kCntSilPrims], (unsigned long long)vcSil.prims);
+    }
     if (st.overflow) atomicOr(reinterpret_cast<unsigned int*>(&P.counters[kCntError]), 1u);
 }
 
@@ -1196,8 +1226,14 @@ hipError_t launchWost(const WosParams& P, int dim, int width, bool count, bool s
                       hipStream_t s) {
     dim3 g(gridBlocks), bl(256);
     const bool robin = !P.pde.are_robin_conditions_pure_neumann;
-    if (interleave && !count && !split) {
-#define ZG_WOSTI(Dd, Wd, R) hipLaunchKernelGGL((wostInterleavedKernel<Dd, Wd, R>), g, bl, kStackLdsBytes, s, P)
+    // visit counting runs in the interleaved kernel, which counts every BVH a step reads (the
+    // walks are those of the run-to-completion kernel)
+    if ((interleave || count) && !split) {
+#define ZG_WOSTI(Dd, Wd, R)                                                                                   \
+    do {                                                                                                      \
+        if (count) hipLaunchKernelGGL((wostInterleavedKernel<Dd, Wd, R, true>), g, bl, kStackLdsBytes, s, P); \
+        else hipLaunchKernelGGL((wostInterleavedKernel<Dd, Wd, R, false>), g, bl, kStackLdsBytes, s, P);      \
+    } while (0)
         if (dim == 3) {
             if (width == 4) { if (robin) ZG_WOSTI(3, 4, true); else ZG_WOSTI(3, 4, false); }
             else { if (robin) ZG_WOSTI(3, 8, true); else ZG_WOSTI(3, 8, false); }
@@ -1216,21 +1252,14 @@ hipError_t launchWost(const WosParams& P, int dim, int width, bool count, bool s
 #undef ZG_WOSTS
         return hipGetLastError();
     }
-#define ZG_WOST(Dd, Wd, C, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, C, false, R>), g, bl, kStackLdsBytes, s, P)
+#define ZG_WOST(Dd, Wd, R) hipLaunchKernelGGL((wostKernel<Dd, Wd, false, false, R>), g, bl, kStackLdsBytes, s, P)
     if (dim == 3) {
-        if (width == 4) {
-            if (robin) { if (count) ZG_WOST(3, 4, true, true); else ZG_WOST(3, 4, false, true); }
-            else { if (count) ZG_WOST(3, 4, true, false); else ZG_WOST(3, 4, false, false); }
-        } else {
-            if (robin) { if (count) ZG_WOST(3, 8, true, true); else ZG_WOST(3, 8, false, true); }
-            else { if (count) ZG_WOST(3, 8, true, false); else ZG_WOST(3, 8, false, false); }
-        }
+        if (width == 4) { if (robin) ZG_WOST(3, 4, true); else ZG_WOST(3, 4, false); }
+        else { if (robin) ZG_WOST(3, 8, true); else ZG_WOST(3, 8, false); }
     } else if (width == 4) {
-        if (robin) { if (count) ZG_WOST(2, 4, true, true); else ZG_WOST(2, 4, false, true); }
-        else { if (count) ZG_WOST(2, 4, true, false); else ZG_WOST(2, 4, false, false); }
+        if (robin) ZG_WOST(2, 4, true); else ZG_WOST(2, 4, false);
     } else {
-        if (robin) { if (count) ZG_WOST(2, 8, true, true); else ZG_WOST(2, 8, false, true); }
-        else { if (count) ZG_WOST(2, 8, true, false); else ZG_WOST(2, 8, false, false); }
+        if (robin) ZG_WOST(2, 8, true); else ZG_WOST(2, 8, false);
     }
 #undef ZG_WOST
     return hipGetLastError();

@@ -162,7 +162,7 @@ class SolveOptions:
     waves_per_cu: int = 0
     count_visits: int = 0   # 1: count the kernel's own BVH visits; 2: those of an exact traversal from the root
     distance_tolerance: float = 0.0
-    wost_run_to_completion: bool = False   # walk on stars: run each query of a step to completion
+    wost_interleave_queries: bool = False  # walk on stars: interleave the lanes' queries item by item
 
     def to_c(self) -> A.zg_solve_options:
         o = A.zg_solve_options()
@@ -174,7 +174,7 @@ class SolveOptions:
         o.waves_per_cu = self.waves_per_cu
         o.count_visits = int(self.count_visits)
         o.distance_tolerance = self.distance_tolerance
-        o.wost_run_to_completion = int(bool(self.wost_run_to_completion))
+        o.wost_interleave_queries = int(bool(self.wost_interleave_queries))
         return o
 
 
@@ -187,11 +187,17 @@ class SolveReport:
     prim_visits: int
     queries: int
     kernel_ms: float
+    # walk on stars, count_visits 1: reflecting-boundary and silhouette BVH reads
+    reflecting_node_visits: int = 0
+    reflecting_prim_visits: int = 0
+    silhouette_node_visits: int = 0
+    silhouette_prim_visits: int = 0
 
     @staticmethod
     def from_c(r: A.zg_solve_report) -> "SolveReport":
         return SolveReport(tuple(r.completion_codes), r.total_steps, r.total_walks, r.node_visits,
-                           r.prim_visits, r.queries, r.kernel_ms)
+                           r.prim_visits, r.queries, r.kernel_ms, r.reflecting_node_visits,
+                           r.reflecting_prim_visits, r.silhouette_node_visits, r.silhouette_prim_visits)
 
 
 # ------------------------------------------------------------------------------------

@@ -182,3 +182,34 @@ def gradient_grid_from_function(fn, lo, hi, res: int, dim: int) -> E.GridField:
     g = np.zeros(shape + (4,), np.float32)
     g.reshape(-1, 4)[:, :dim] = fn(pts)
     return E.GridField(data=torch.from_numpy(g).cuda(), lo=tuple(lo[:dim]), hi=tuple(hi[:dim]))
+
+
+def time_training(steps: int = 1000, batch: int = 65536, n: int = 2_000_000, seed: int = 1) -> dict:
+    """Times `steps` Adam steps of the default network at `batch` on n synthetic pairs
+    (x uniform in [-1, 1]^3, y = sin 2x sin 2y sin 2z) after a 20-step warm-up; returns the time
+    per step and the achieved FLOP rate (6 x multiply-adds per sample and step)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = (torch.rand(n, 3, device="cuda", generator=g) * 2 - 1).contiguous()
+    y = torch.sin(2 * x).prod(1).contiguous()
+    net = MLP(MLPConfig(in_dim=3, seed=seed), max_batch=batch)
+    train(net, x, y, steps=20, batch=batch, seed=seed)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    losses = train(net, x, y, steps=steps, batch=batch, seed=seed + 1, log_every=max(1, steps // 2))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    flops = steps * batch * net.flops_per_sample()
+    return {"steps": steps, "batch": batch, "ms_per_step": el / steps * 1e3, "tflops": flops / el / 1e12,
+            "losses": losses}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="time the learned control variate's training on the GPU")
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--batch", type=int, default=65536)
+    a = ap.parse_args()
+    print(json.dumps(time_training(a.steps, a.batch)))
