@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--cv-train-walks", type=int, default=32)
     ap.add_argument("--cv-steps", type=int, default=5000)
     ap.add_argument("--cv-batch", type=int, default=65536)
-    ap.add_argument("--cv-grid", type=int, default=96)
+    ap.add_argument("--cv-grid", type=int, default=64)
     ap.add_argument("--to-rmse", type=float, default=1e-3,
                     help="after the timed passes, run learned-CV passes until the true RMSE vs the analytic "
                          "solution reaches this value, and report the measured time (0: skip)")

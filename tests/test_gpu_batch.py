@@ -225,6 +225,10 @@ def test_uneven_instances_reproduce_single_scene_solves_and_closed_form(gpu, ora
         E.wos_solve(scenes[i], pr.pde, pr.settings, _points(torch, P, di, nw), si, opts)
         _same(g, _host(si), sel, rtol=2e-6)
         err = g["mean"][sel] - PB.exp_linear(P[sel], pr.meta["a"])
+        if sizes[i][2] == 0.0:
+            # harmonic with the constant boundary value 1: every walk returns exactly 1
+            assert np.abs(err).max() < 1e-6 and np.abs(g["var"][sel]).max() < 1e-9
+            continue
         pooled = err.sum() / np.sqrt((g["var"][sel] / g["n"][sel]).sum())
         assert abs(pooled) < 4.0, f"instance {i} ({tris[i]} triangles, sigma {sizes[i][2]}): pooled z {pooled:.2f}"
     big = len(sizes) - 1

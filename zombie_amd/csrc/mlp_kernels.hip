@@ -49,7 +49,7 @@ struct GemmArgs {
     float* Z;                              // FWD: out fp32 [M x N];  BWD_DATA: in, previous pre-activation
     uint16_t* H;                           // FWD: out bf16 [M x N];  BWD_DATA: out dZprev bf16 [M x N]
     float* F;                              // BWD_INPUT: out fp32 [M x N];  BWD_WEIGHT: atomic fp32 [M x N]
-    float* dbias;                          // BWD_DATA (nullable): atomic fp32 [N], column sums of H
+    float* dbias;                          // BWD_WEIGHT (nullable): atomic fp32 [M], row sums of A
     int ldc;                               // row stride of the outputs
 };
 
@@ -112,6 +112,10 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // BWD_WEIGHT with dbias: the blocks of the first column tile also sum the rows of their A
+    // tiles over k (thread t: row t / 4, k quarter t % 4 of each LDS tile)
+    const bool rowSums = EPI == kBwdWeight && g.dbias && blockIdx.x == 0;
+    float rsum = 0.0f;
 
     for (int k0 = kBegin; k0 < kEnd; k0 += kBK) {
         // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
@@ -148,6 +152,12 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
             store8(Bs, c, kk, bAlongK, load8(g.B, base, step, valid));
         }
         __syncthreads();
+        if (rowSums) {
+            const uint4 v = *reinterpret_cast<const uint4*>(&As[tid >> 2][8 * (tid & 3)]);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) rsum += fromBf16((uint16_t)(w[j] & 0xffffu)) + fromBf16((uint16_t)(w[j] >> 16));
+        }
         bf16x8 af[2], bfr[2];
         const int kk = 8 * (lane >> 4);
 #pragma unroll
@@ -161,8 +171,12 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
             for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
+    if (rowSums) {
+        rsum += __shfl_xor(rsum, 1);
+        rsum += __shfl_xor(rsum, 2);
+        if ((tid & 3) == 0 && m0 + (tid >> 2) < g.M) atomicAdd(&g.dbias[m0 + (tid >> 2)], rsum);
+    }
     // epilogue
-    float csum[2] = {0.0f, 0.0f};   // BWD_DATA: this lane's share of the tile's column sums
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -179,29 +193,13 @@ __global__ void __launch_bounds__(256) gemmKernel(GemmArgs g) {
                     g.Z[o] = z;
                     g.H[o] = toBf16(silu(z));
                 } else if (EPI == kBwdData) {
-                    const uint16_t h = toBf16(v * siluGrad(g.Z[o]));
-                    g.H[o] = h;
-                    csum[j] += fromBf16(h);   // the bias gradient sums the stored (bf16) values
+                    g.H[o] = toBf16(v * siluGrad(g.Z[o]));
                 } else if (EPI == kBwdInput) {
                     g.F[o] = v;
                 } else {
                     atomicAdd(&g.F[o], v);
                 }
             }
-    if (EPI == kBwdData && g.dbias) {
-        // column sums of the block's 64 x 64 tile: over the lane's rows (above), the four row
-        // groups of a wave (lanes l, l ^ 16, l ^ 32, l ^ 48), then the two waves that share the
-        // columns; one atomic per column and block
-        __shared__ float colSum[2][kBN];
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            csum[j] += __shfl_xor(csum[j], 16);
-            csum[j] += __shfl_xor(csum[j], 32);
-            if (lane < 16) colSum[wave >> 1][wn + 16 * j + lane] = csum[j];
-        }
-        __syncthreads();
-        if (tid < kBN && n0 + tid < g.N) atomicAdd(&g.dbias[n0 + tid], colSum[0][tid] + colSum[1][tid]);
-    }
 }
 
 // phi = [sin(B x), cos(B x)] in bf16, [n x 2F]
@@ -240,24 +238,22 @@ __device__ __forceinline__ void blockColumnAtomics(const float (&part)[NCH], int
 
 // c = H . w + b (fp32 weights); mode 0: values only; mode 1 (training, needs y): loss
 // sum 0.5 (c - y)^2 / n, dc = (c - y) / n, dZlast = bf16(dc * w * silu'(Zlast)), and the
-// gradients dw = sum_i dc_i H_i, db = sum_i dc_i, dbl = sum_i dZlast_i (the last hidden layer's
-// bias gradient, from the stored bf16 values); mode 2 (input gradient): dZlast with dc = 1.
+// gradients dw = sum_i dc_i H_i, db = sum_i dc_i; mode 2 (input gradient): dZlast with dc = 1.
 // Launch: 256 threads, any grid.
 template <int NCH>
 __global__ void __launch_bounds__(256) outputKernel(const uint16_t* __restrict__ Hl, const float* __restrict__ Zl, int n,
                                                     int H, const float* __restrict__ w, const float* __restrict__ b,
                                                     const float* __restrict__ y, float invN, float* __restrict__ out,
                                                     uint16_t* __restrict__ dZl, float* __restrict__ dw,
-                                                    float* __restrict__ db, float* __restrict__ dbl,
+                                                    float* __restrict__ db,
                                                     float* __restrict__ loss, int mode) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    float wk[NCH], dwp[NCH], dbp[NCH];
+    float wk[NCH], dwp[NCH];
 #pragma unroll
     for (int j = 0; j < NCH; j++) {
         const int k = lane + 64 * j;
         wk[j] = k < H ? w[k] : 0.0f;
         dwp[j] = 0.0f;
-        dbp[j] = 0.0f;
     }
     const float b0 = b[0];
     float lossp = 0.0f, dcp = 0.0f;
@@ -303,11 +299,7 @@ __global__ void __launch_bounds__(256) outputKernel(const uint16_t* __restrict__
 #pragma unroll
             for (int j = 0; j < NCH; j++) {
                 const int k = lane + 64 * j;
-                if (k < H) {
-                    const uint16_t dz = toBf16(dc * wk[j] * siluGrad(z[u][j]));
-                    dr[k] = dz;
-                    dbp[j] += fromBf16(dz);
-                }
+                if (k < H) dr[k] = toBf16(dc * wk[j] * siluGrad(z[u][j]));
                 dwp[j] += dc * h[u][j];
             }
         }
@@ -320,7 +312,6 @@ __global__ void __launch_bounds__(256) outputKernel(const uint16_t* __restrict__
         if (loss) atomicAdd(loss, sl[0][0] + sl[1][0] + sl[2][0] + sl[3][0]);
         atomicAdd(db, sl[0][1] + sl[1][1] + sl[2][1] + sl[3][1]);
     }
-    if (dbl) blockColumnAtomics<NCH>(dbp, H, dbl);
 }
 
 // grad_x c = sum_f dPhi_sin[f] cos(B_f x) B_f - dPhi_cos[f] sin(B_f x) B_f
@@ -426,10 +417,9 @@ hipError_t mlpBackward(MlpDevice& m, const float* y, int n, int mode, float* los
     float* dwOut = m.grad + m.wOffset[L];
     float* dbOut = m.grad + m.bOffset[L];
     const unsigned rowBlocks = (unsigned)std::max(1, std::min(kRowBlocks, (n + 15) / 16));
-    float* dbLast = mode == 1 ? m.grad + m.bOffset[L - 1] : nullptr;
     switch ((Hd + 63) / 64) {
 #define ZG_OUT(C) case C: hipLaunchKernelGGL(outputKernel<C>, dim3(rowBlocks), dim3(256), 0, s, m.act[L - 1], m.pre[L - 1], \
-                                             n, Hd, wOut, bOut, y, 1.0f / (float)n, out, m.dz[0], dwOut, dbOut, dbLast, \
+                                             n, Hd, wOut, bOut, y, 1.0f / (float)n, out, m.dz[0], dwOut, dbOut, \
                                              loss, mode); break
         ZG_OUT(1); ZG_OUT(2); ZG_OUT(3); ZG_OUT(4); ZG_OUT(5); ZG_OUT(6); ZG_OUT(7); ZG_OUT(8);
 #undef ZG_OUT
@@ -448,6 +438,7 @@ hipError_t mlpBackward(MlpDevice& m, const float* y, int n, int mode, float* los
             g.A = m.dz[cur]; g.sAm = 1; g.sAk = Hd;
             g.B = prevAct; g.sBk = K; g.sBn = 1;
             g.F = m.grad + m.wOffset[l]; g.ldc = K;
+            g.dbias = m.grad + m.bOffset[l];   // bias gradient of layer l: the row sums of dZ_l^T
             int split = std::max(1, std::min(256, n / 512));
             e = gemm(kBwdWeight, g, split, s);
             if (e != hipSuccess) return e;
@@ -460,7 +451,6 @@ hipError_t mlpBackward(MlpDevice& m, const float* y, int n, int mode, float* los
         g.ldc = K;
         if (l > 0) {
             g.Z = m.pre[l - 1]; g.H = m.dz[cur ^ 1];
-            g.dbias = mode == 1 ? m.grad + m.bOffset[l - 1] : nullptr;   // bias gradient of layer l - 1
             e = gemm(kBwdData, g, 1, s);
             cur ^= 1;
         } else if (mode == 2) {
