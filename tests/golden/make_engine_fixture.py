@@ -21,7 +21,10 @@ demo reads it, into one compressed fixture:
 - solution [256, 256] float32: wost.pfm's first channel (all three are equal), in file row
   order, as Image::writePFM wrote it (demo/image.h:265-290: file row h-1-j holds grid column
   j's y index);
-- the wost.json settings the test needs.
+- the wost.json settings the test needs;
+- bvc_solution [256, 256] float32: solutions/bvc.pfm's first channel, the same problem solved
+  by the reference's BoundaryValueCachingSolver (demo/demo.cpp:199-320 with bvc.json), and the
+  bvc.json solver settings (bvc_settings / bvc_setting_names).
 
 Run from the repository root:  python tests/golden/make_engine_fixture.py
 """
@@ -86,6 +89,15 @@ def main() -> int:
     if not (np.array_equal(sol[..., 0], sol[..., 1]) and np.array_equal(sol[..., 0], sol[..., 2])):
         raise SystemExit("wost.pfm channels differ")
 
+    bvc_cfg = json.loads((ENGINE / "bvc.json").read_text())
+    if bvc_cfg["modelProblem"] != cfg["modelProblem"] or bvc_cfg["output"]["gridRes"] != cfg["output"]["gridRes"]:
+        raise SystemExit("bvc.json and wost.json describe different problems")
+    bvc = read_pfm(ENGINE / "solutions" / "bvc.pfm")
+    if not (np.array_equal(bvc[..., 0], bvc[..., 1]) and np.array_equal(bvc[..., 0], bvc[..., 2])):
+        raise SystemExit("bvc.pfm channels differ")
+    bvc_names = np.array(sorted(k for k, v in bvc_cfg["solver"].items() if isinstance(v, (int, float))))
+    bvc_settings = np.array([float(bvc_cfg["solver"][k]) for k in bvc_names], np.float64)
+
     s, m, o = cfg["solver"], cfg["modelProblem"], cfg["output"]
     settings = np.array([s["nWalks"], s["maxWalkLength"], s["epsilonShellForAbsorbingBoundary"],
                          s["epsilonShellForReflectingBoundary"], s["russianRouletteThreshold"],
@@ -101,7 +113,8 @@ def main() -> int:
     np.savez_compressed(OUT, positions=positions, segments=segments, absorbing_value=gray(absorbing_pfm),
                         reflecting_mask_bits=np.packbits(mask.reshape(-1)),
                         reflecting_mask_shape=np.array(mask.shape, np.int32),
-                        solution=sol[..., 0].copy(), settings=settings, setting_names=names)
+                        solution=sol[..., 0].copy(), settings=settings, setting_names=names,
+                        bvc_solution=bvc[..., 0].copy(), bvc_settings=bvc_settings, bvc_setting_names=bvc_names)
     print(f"wrote {OUT} ({OUT.stat().st_size} bytes): {len(positions)} vertices, {len(segments)} segments, "
           f"values {absorbing_pfm.shape[:2]}, mask {mask.shape}, solution {sol.shape[:2]}")
     return 0

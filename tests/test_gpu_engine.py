@@ -98,15 +98,8 @@ def _write_pfm_gray(path, img):
         f.write(np.ascontiguousarray(img, "<f4").tobytes())
 
 
-def test_engine_json_problem_through_the_reference_python_surface(gpu, tmp_path):
-    """The engine model problem as files in the reference's format (OBJ, PFM images, wost.json),
-    run by zombie_amd.demo through the reference-shaped layer zombie_amd.zombie (the flow of
-    demo/demo.py), compared with the reference's own solution."""
-    import json
-    from zombie_amd import demo
-
-    z = np.load(FIXTURE, allow_pickle=False)
-    cfg = dict(zip([str(n) for n in z["setting_names"]], z["settings"].tolist()))
+def _write_engine_files(z, tmp_path):
+    """The engine problem's data as the reference's demo reads them (OBJ, PFM images)."""
     data = tmp_path / "data"
     data.mkdir()
     with open(data / "geometry.obj", "w") as f:
@@ -120,6 +113,27 @@ def test_engine_json_problem_through_the_reference_python_surface(gpu, tmp_path)
     _write_pfm_gray(data / "is_reflecting_boundary.pfm", mask)
     _write_pfm_gray(data / "reflecting_boundary_value.pfm", np.zeros((8, 8), np.float32))
     _write_pfm_gray(data / "source_value.pfm", np.zeros((8, 8), np.float32))
+
+
+def _model_problem(cfg):
+    return {"geometry": "data/geometry.obj", "isReflectingBoundary": "data/is_reflecting_boundary.pfm",
+            "absorbingBoundaryValue": "data/absorbing_boundary_value.pfm",
+            "reflectingBoundaryValue": "data/reflecting_boundary_value.pfm",
+            "sourceValue": "data/source_value.pfm", "robinCoeff": cfg["robinCoeff"],
+            "absorptionCoeff": cfg["absorptionCoeff"], "solveDoubleSided": False,
+            "solveExterior": False, "domainIsWatertight": True, "useSdfForAbsorbingBoundary": False}
+
+
+def test_engine_json_problem_through_the_reference_python_surface(gpu, tmp_path):
+    """The engine model problem as files in the reference's format (OBJ, PFM images, wost.json),
+    run by zombie_amd.demo through the reference-shaped layer zombie_amd.zombie (the flow of
+    demo/demo.py), compared with the reference's own solution."""
+    import json
+    from zombie_amd import demo
+
+    z = np.load(FIXTURE, allow_pickle=False)
+    cfg = dict(zip([str(n) for n in z["setting_names"]], z["settings"].tolist()))
+    _write_engine_files(z, tmp_path)
     problem = {
         "solverType": "wost",
         "solver": {"nWalks": int(cfg["nWalks"]), "maxWalkLength": int(cfg["maxWalkLength"]),
@@ -130,12 +144,7 @@ def test_engine_json_problem_through_the_reference_python_surface(gpu, tmp_path)
                    "ignoreAbsorbingBoundaryContribution": bool(cfg["ignoreAbsorbingBoundaryContribution"]),
                    "ignoreReflectingBoundaryContribution": bool(cfg["ignoreReflectingBoundaryContribution"]),
                    "ignoreSourceContribution": bool(cfg["ignoreSourceContribution"])},
-        "modelProblem": {"geometry": "data/geometry.obj", "isReflectingBoundary": "data/is_reflecting_boundary.pfm",
-                         "absorbingBoundaryValue": "data/absorbing_boundary_value.pfm",
-                         "reflectingBoundaryValue": "data/reflecting_boundary_value.pfm",
-                         "sourceValue": "data/source_value.pfm", "robinCoeff": cfg["robinCoeff"],
-                         "absorptionCoeff": cfg["absorptionCoeff"], "solveDoubleSided": False,
-                         "solveExterior": False, "domainIsWatertight": True, "useSdfForAbsorbingBoundary": False},
+        "modelProblem": _model_problem(cfg),
         "output": {"solutionFile": "solutions/wost.pfm", "gridRes": int(cfg["gridRes"]),
                    "boundaryDistanceMask": cfg["boundaryDistanceMask"]},
     }
@@ -154,3 +163,48 @@ def test_engine_json_problem_through_the_reference_python_surface(gpu, tmp_path)
     ok = se2 > 0
     zz = (got[both][ok] - ref[both][ok]) / np.sqrt(se2[ok])
     check_four_se(zz, np.ones(len(zz), bool), "engine demo driver vs reference wost.pfm")
+
+
+def test_engine_bvc_through_the_demo_driver_matches_reference_bvc(gpu, tmp_path):
+    """The engine problem with the reference's boundary value caching settings (bvc.json: caches
+    of 3,072 absorbing and 3,072 reflecting samples, 96 walks per solution and 960 per gradient
+    estimate) run by the demo driver's "bvc" path, against solutions/bvc.pfm, which the
+    reference's BoundaryValueCachingSolver wrote for the same problem.
+
+    A BVC solution is one realisation of a spatially correlated estimator (every pixel sums the
+    same random caches), so pixel z-scores do not apply.  Three runs with independent seeds
+    measure the estimator's own run-to-run spread: if the reference's solution is another draw
+    of the same estimator, its mean squared difference to our runs equals the mean squared
+    difference between our runs (ratio 1; a bias or a different estimator raises it), and the
+    difference of its pixel mean to ours stays within the spread of our runs' pixel means."""
+    import json
+    from zombie_amd import demo
+
+    z = np.load(FIXTURE, allow_pickle=False)
+    cfg = dict(zip([str(n) for n in z["setting_names"]], z["settings"].tolist()))
+    bcfg = dict(zip([str(n) for n in z["bvc_setting_names"]], z["bvc_settings"].tolist()))
+    _write_engine_files(z, tmp_path)
+    solver = {k: (bool(v) if k.startswith("ignore") else int(v) if float(v).is_integer() and "Threshold" not in k
+                  and "epsilon" not in k and "radius" not in k else v) for k, v in bcfg.items()}
+    problem = {"solverType": "bvc", "solver": solver, "modelProblem": _model_problem(cfg),
+               "output": {"solutionFile": "solutions/bvc.pfm", "gridRes": int(cfg["gridRes"]),
+                          "boundaryDistanceMask": cfg["boundaryDistanceMask"]}}
+    (tmp_path / "bvc.json").write_text(json.dumps(problem))
+    runs = []
+    for seed in (101, 202, 303):
+        print(f"engine bvc: run with seed {seed}", flush=True)
+        r = demo.run(str(tmp_path / "bvc.json"), root=str(tmp_path), seed=seed)
+        runs.append(r["values"].reshape(-1).astype(np.float64))
+    ref = np.ascontiguousarray(z["bvc_solution"][::-1, :].T).reshape(-1).astype(np.float64)
+    both = (runs[0] != 0) & (ref != 0)
+    assert both.sum() > 0.95 * (ref != 0).sum()
+    ours = np.stack([r[both] for r in runs])
+    R = ref[both]
+    d_ours = np.mean([np.mean((ours[i] - ours[j]) ** 2) for i in range(3) for j in range(i + 1, 3)])
+    d_ref = np.mean([np.mean((R - ours[i]) ** 2) for i in range(3)])
+    means = ours.mean(1)
+    spread = means.std(ddof=1)
+    print(f"engine bvc: rms between our runs {np.sqrt(d_ours):.4f}, rms to the reference {np.sqrt(d_ref):.4f}, "
+          f"ratio {d_ref / d_ours:.3f}; pixel means ours {means} reference {R.mean():.4f}", flush=True)
+    assert d_ref / d_ours < 1.5, (d_ref, d_ours)
+    assert abs(R.mean() - means.mean()) < 4.0 * spread * np.sqrt(1 + 1 / 3) + 2e-3, (R.mean(), means)

@@ -295,3 +295,68 @@ class BoundaryValueCachingSolver:
             t[idx] = 0
         stats.n[idx, cat] = 1
         stats.mean[idx, cat] = st.mean
+
+
+def signed_volume(positions, indices) -> float:
+    """Signed area (2D polyline) or volume (3D triangle mesh) enclosed by a boundary mesh: the
+    sum of the origin-based simplex volumes of its primitives (computeSignedVolume,
+    fcpw_geometric_queries.h:430-456)."""
+    V = np.asarray(positions, np.float64)
+    P = V[np.asarray(indices, np.int64)]
+    if V.shape[1] == 2:
+        return float(0.5 * np.sum(P[:, 0, 0] * P[:, 1, 1] - P[:, 0, 1] * P[:, 1, 0]))
+    return float(np.sum(np.einsum("ij,ij->i", np.cross(P[:, 0], P[:, 1]), P[:, 2])) / 6.0)
+
+
+# demo/demo.cpp:211-248: the settings runBoundaryValueCaching reads, with its defaults
+# (normalOffsetForAbsorbingBoundary defaults to 5 x the absorbing epsilon)
+DEMO_DEFAULTS = {
+    "nWalksForCachedSolutionEstimates": 128, "nWalksForCachedGradientEstimates": 640,
+    "absorbingBoundaryCacheSize": 1024, "reflectingBoundaryCacheSize": 1024, "domainCacheSize": 1024,
+    "robinCoeffCutoffForNormalDerivative": float("inf"), "normalOffsetForReflectingBoundary": 0.0,
+    "radiusClampForKernels": 0.0, "regularizationForKernels": 0.0,
+}
+
+
+def solve_like_demo(scene: E.Scene, pde: A.zg_pde, ws: A.zg_walk_settings, absorbing_mesh, reflecting_mesh,
+                    pt, dist_to_absorbing, dist_to_reflecting, solver_cfg: dict, domain_volume: float | None = None,
+                    seed: int = 0x5EED):
+    """runBoundaryValueCaching (demo/demo.cpp:199-320) on the GPU for interior points `pt`
+    (device [n, dim]): boundary samplers on the absorbing and reflecting meshes ((positions,
+    indices) pairs, None when empty), a domain sampler when the source counts, generateSamples,
+    computeSampleEstimates, splat with the normal offsets as cutoffs, estimateSolutionNearBoundary
+    for the points within them.  Returns the per-point solution (device, FP64)."""
+    cfg = {**DEMO_DEFAULTS, **solver_cfg}
+    eps_a = float(cfg.get("epsilonShellForAbsorbingBoundary", ws.epsilon_shell_for_absorbing_boundary))
+    off_a = float(cfg.get("normalOffsetForAbsorbingBoundary", 5.0 * eps_a))
+    off_r = float(cfg["normalOffsetForReflectingBoundary"])
+    robin_cut = float(cfg["robinCoeffCutoffForNormalDerivative"])
+    n_sol, n_grad = int(cfg["nWalksForCachedSolutionEstimates"]), int(cfg["nWalksForCachedGradientEstimates"])
+    dom_size = 0 if ws.ignore_source_contribution else int(cfg["domainCacheSize"])
+    double_sided = bool(ws.solve_double_sided)
+    samplers = []
+    for mesh in (absorbing_mesh, reflecting_mesh):
+        ok = mesh is not None and len(mesh[1]) > 0
+        samplers.append(UniformBoundarySampler(*mesh) if ok else None)
+    dom = None
+    if dom_size > 0:
+        # the solve region: the bounding domain when double sided, else the domain itself
+        # (demo/demo.cpp:250-262)
+        lo, hi = np.asarray(scene.domain_min, np.float64), np.asarray(scene.domain_max, np.float64)
+        if double_sided:
+            vol = float(np.prod(hi - lo))
+        elif domain_volume is not None:
+            vol = float(domain_volume)
+        else:
+            vol = abs(sum(signed_volume(*m) for m in (absorbing_mesh, reflecting_mesh) if m is not None and len(m[1])))
+        dom = UniformDomainSampler(scene, lo, hi, vol)
+    solver = BoundaryValueCachingSolver(scene, samplers[0], samplers[1], dom, seed=seed)
+    solver.generate_samples(int(cfg["absorbingBoundaryCacheSize"]), int(cfg["reflectingBoundaryCacheSize"]),
+                            dom_size, off_a, off_r, double_sided)
+    solver.compute_sample_estimates(pde, ws, n_sol, n_grad, robin_cut)
+    ev = E.BvcEvalPoints(pt=pt, dist_to_absorbing=dist_to_absorbing, dist_to_reflecting=dist_to_reflecting)
+    st = E.BvcStats.zeros(int(pt.shape[0]), scene.dim)
+    solver.splat(ev, st, float(cfg["radiusClampForKernels"]), float(cfg["regularizationForKernels"]), robin_cut,
+                 off_a, off_r)
+    solver.estimate_solution_near_boundary(pde, ws, ev, st, off_a, off_r, n_sol)
+    return st.solution()

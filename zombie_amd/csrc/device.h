@@ -627,13 +627,19 @@ ZG_DEV bool isLeaf(int c) { return c < 0; }
 ZG_DEV int leafFirstD(int c) { return (~c) >> 4; }
 ZG_DEV int leafCountD(int c) { return ((~c) & 15) + 1; }
 
-// Traversal stack: the top kLdsStack entries of a lane live in LDS (dynamic shared memory
-// laid out [depth][blockDim], so lanes at the same depth hit consecutive banks); deeper
-// entries spill to a per-lane private array (scratch) owned by the caller, touched only by
-// rare deep traversals.  Kernels that traverse launch with kStackLdsBytes of dynamic LDS.
-constexpr int kLdsStack = 12;
+// Traversal stack: the top kLdsStack entries of a lane live in LDS (dynamic shared memory), as
+// two arrays laid out [depth][blockDim] so that lanes at the same depth hit consecutive banks:
+// the 32-bit node codes, then the upper 16 bits of the keys.  A non-negative float cut to its
+// upper 16 bits rounds toward zero, so a popped key is a lower bound of the pushed one (relative
+// error below 2^-8): pruning on pop stays conservative (at worst a node is visited and its
+// children pruned), and a radius certified from pruned keys can only get smaller.  Six bytes per
+// entry give 16 entries in the 24 KB per 256-thread block that 6 blocks per CU allow (eight-byte
+// entries gave 12, and 10.6% of the pushes of the 3D walk went to the spill array).  Deeper
+// entries spill, with their full keys, to a per-lane private array owned by the caller.
+// Kernels that traverse launch with kStackLdsBytes of dynamic LDS and at most 256 threads.
+constexpr int kLdsStack = 16;
 constexpr int kSpillStack = 96;
-constexpr int kStackLdsBytes = kLdsStack * 256 * 8;
+constexpr int kStackLdsBytes = kLdsStack * 256 * 6;
 
 extern __shared__ uint2 zgLdsStack[];
 
@@ -653,29 +659,37 @@ struct Stack {
     ZG_DEV void init(unsigned long long* spillArray) {
         col = threadIdx.x; stride = blockDim.x; sp = 0; ovf = 0; overflow = false; spill = (PrivU64*)spillArray;
     }
+    ZG_DEV int* codes() const { return reinterpret_cast<int*>(zgLdsStack); }
+    ZG_DEV uint16_t* keys() const {
+        return reinterpret_cast<uint16_t*>(reinterpret_cast<int*>(zgLdsStack) + kLdsStack * stride);
+    }
+    // d2 >= 0 (a squared distance or a ray parameter)
     ZG_DEV void push(int node, float d2) {
-        uint2 e = make_uint2((unsigned)node, __float_as_uint(d2));
-        if (sp < kLdsStack) { zgLdsStack[sp * stride + col] = e; sp++; }
-        else if (ovf < kSpillStack) {
-            spill[ovf] = ((unsigned long long)e.y << 32) | e.x;
+        const unsigned k = __float_as_uint(d2);
+        if (sp < kLdsStack) {
+            codes()[sp * stride + col] = node;
+            keys()[sp * stride + col] = (uint16_t)(k >> 16);
+            sp++;
+        } else if (ovf < kSpillStack) {
+            spill[ovf] = ((unsigned long long)k << 32) | (unsigned)node;
             ovf++;
+        } else {
+            overflow = true;
         }
-        else overflow = true;
     }
     ZG_DEV bool pop(int& node, float& d2) {
-        uint2 e;
         if (ovf > 0) {
             ovf--;
-            unsigned long long v = spill[ovf];
-            e.x = (unsigned)v;
-            e.y = (unsigned)(v >> 32);
+            const unsigned long long v = spill[ovf];
+            node = (int)(unsigned)v;
+            d2 = __uint_as_float((unsigned)(v >> 32));
         } else if (sp > 0) {
             sp--;
-            e = zgLdsStack[sp * stride + col];
+            node = codes()[sp * stride + col];
+            d2 = __uint_as_float((unsigned)keys()[sp * stride + col] << 16);
         } else {
             return false;
         }
-        node = (int)e.x; d2 = __uint_as_float(e.y);
         return true;
     }
 };

@@ -1,9 +1,10 @@
 """Model-problem driver: the reference's demo (demo/demo.cpp, demo/demo.py) for its walk-on-
-spheres and walk-on-stars solvers, run on the GPU through zombie_amd.zombie.
+spheres, walk-on-stars and boundary value caching solvers, run on the GPU through
+zombie_amd.zombie (and zombie_amd.bvc).
 
     python -m zombie_amd.demo path/to/model_problem.json [--root DIR] [--walks N] [--seed S]
 
-reads a model problem in the reference's JSON format (solverType "wos" or "wost"; paths
+reads a model problem in the reference's JSON format (solverType "wos", "wost" or "bvc"; paths
 relative to --root, default the current directory), sets it up as demo/model_problem.h does
 (OBJ polyline normalised and flipped, square bounding box, PDE data from images as nearest-cell
 dense grids, boundary partitioned by the reflecting-boundary image at segment midpoints), solves
@@ -13,8 +14,9 @@ in Image::writePFM's layout (demo/grid.h:42-95, demo/image.h:265-290).
 Images are read as the C++ demo reads them: PFM samples in file row order, three channels
 reduced to 0.299 R + 0.587 G + 0.114 B (demo/image.h:150-225); a missing PFM is replaced by the
 PNG of the same name converted by the rule that made the reference's PFMs (rows flipped, / 255,
-demo/model_problems/image2pfm.py).  Exterior problems (Kelvin transform), SDF absorbing
-boundaries, boundary value caching and reverse walk on stars are outside this driver.
+demo/model_problems/image2pfm.py).  "bvc" follows runBoundaryValueCaching (demo/demo.cpp:
+199-320; zombie_amd.bvc.solve_like_demo).  Exterior problems (Kelvin transform), SDF absorbing
+boundaries and reverse walk on stars are outside this driver.
 """
 from __future__ import annotations
 
@@ -71,8 +73,8 @@ def run(config_path: str, root: str = ".", n_walks: int | None = None, seed: int
     cfg = json.loads(Path(config_path).read_text())
     root = Path(root)
     solver_type = cfg["solverType"]
-    if solver_type not in ("wos", "wost"):
-        raise SystemExit(f"solverType {solver_type!r}: this driver runs 'wos' and 'wost'")
+    if solver_type not in ("wos", "wost", "bvc"):
+        raise SystemExit(f"solverType {solver_type!r}: this driver runs 'wos', 'wost' and 'bvc'")
     S, M, O = cfg["solver"], cfg["modelProblem"], cfg["output"]
     if M.get("solveExterior", False) or M.get("useSdfForAbsorbingBoundary", False):
         raise SystemExit("exterior problems and SDF absorbing boundaries are outside this driver")
@@ -139,34 +141,46 @@ def run(config_path: str, root: str = ".", n_walks: int | None = None, seed: int
     U.compute_dist_to_boundary(queries, grid, False, dA, dR, dim=dim)
     dA, dR = np.asarray(dA, np.float32), np.asarray(dR, np.float32)
     idx = np.nonzero(valid)[0]
-    pts = Solvers.SamplePointList([Solvers.SamplePoint(grid[k], np.zeros(dim, np.float32), Solvers.SampleType.InDomain,
-                                                       Solvers.EstimationQuantity.Solution, 1.0, float(dA[k]),
-                                                       float(dR[k]), dim=dim) for k in idx], dim=dim)
+    pts = [] if solver_type == "bvc" else Solvers.SamplePointList(
+        [Solvers.SamplePoint(grid[k], np.zeros(dim, np.float32), Solvers.SampleType.InDomain,
+                             Solvers.EstimationQuantity.Solution, 1.0, float(dA[k]), float(dR[k]), dim=dim)
+         for k in idx], dim=dim)
     walks = int(n_walks if n_walks is not None else S.get("nWalks", 128))
     max_len = int(S.get("maxWalkLength", 1024))
     eps_a = float(S.get("epsilonShellForAbsorbingBoundary", 1e-3))
     settings = Solvers.WalkSettings(
-        eps_a, float(S.get("epsilonShellForReflectingBoundary", 1e-3)) if solver_type == "wost" else 0.0,
-        float(S.get("silhouettePrecision", 1e-3)) if solver_type == "wost" else 0.0,
+        eps_a, float(S.get("epsilonShellForReflectingBoundary", 1e-3)) if solver_type != "wos" else 0.0,
+        float(S.get("silhouettePrecision", 1e-3)) if solver_type != "wos" else 0.0,
         float(S.get("russianRouletteThreshold", 0.0)), float(S.get("splittingThreshold", np.inf)), max_len,
         int(S.get("stepsBeforeApplyingTikhonov", 0)),
-        int(S.get("stepsBeforeUsingMaximalSpheres", max_len)) if solver_type == "wost" else 0,
+        int(S.get("stepsBeforeUsingMaximalSpheres", max_len)) if solver_type != "wos" else 0,
         solve_double_sided, not S.get("disableGradientControlVariates", False),
         not S.get("disableGradientAntitheticVariates", False),
         bool(S.get("useCosineSamplingForDirectionalDerivatives", False)),
         bool(S.get("ignoreAbsorbingBoundaryContribution", False)),
-        bool(S.get("ignoreReflectingBoundaryContribution", False)) if solver_type == "wost" else True,
+        bool(S.get("ignoreReflectingBoundaryContribution", False)) if solver_type != "wos" else True,
         bool(S.get("ignoreSourceContribution", False)), False)
-    stats = Solvers.create_sample_statistics_list(len(pts), dim=dim)
-    Solver = Solvers.WalkOnStars if solver_type == "wost" else Solvers.WalkOnSpheres
-    Solver(queries, dim=dim, seed=seed).solve(pde, settings, zombie.IntList([walks] * len(pts)), pts, stats)
+    if solver_type == "bvc":
+        import torch
+        from . import bvc as BV
+        scene = queries.scene(solve_double_sided, settings.silhouette_precision)
+        meshes = [(np.asarray(p_, np.float32).reshape(-1, dim), np.asarray(i_, np.int32).reshape(-1, dim))
+                  for p_, i_ in ((ap, ai), (rp, ri))]
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        sol = BV.solve_like_demo(scene, pde.channel(0), settings.to_c(), meshes[0], meshes[1], dev(grid[idx]),
+                                 dev(dA[idx]), dev(dR[idx]), S, seed=seed).cpu().numpy().astype(np.float32)
+        var = np.zeros_like(sol)
+    else:
+        stats = Solvers.create_sample_statistics_list(len(pts), dim=dim)
+        Solver = Solvers.WalkOnStars if solver_type == "wost" else Solvers.WalkOnSpheres
+        Solver(queries, dim=dim, seed=seed).solve(pde, settings, zombie.IntList([walks] * len(pts)), pts, stats)
+        sol = np.array([s.get_estimated_solution() for s in stats], np.float32)
+        var = np.array([s.get_estimated_solution_variance() for s in stats], np.float32)
 
     # grid values (grid.h:55-80): masked pixels are 0
     mask = float(O.get("boundaryDistanceMask", 0.0))
     values = np.zeros(res * res, np.float32)
     variance = np.zeros(res * res, np.float32)
-    sol = np.array([s.get_estimated_solution() for s in stats], np.float32)
-    var = np.array([s.get_estimated_solution_variance() for s in stats], np.float32)
     keep = np.minimum(dA[idx], dR[idx]) >= mask
     values[idx[keep]] = sol[keep]
     variance[idx[keep]] = var[keep]
