@@ -287,7 +287,49 @@ wosKernel(WosParams P) {
     };
 
     // FP64 per-point accumulation of one walk's estimate (the sum over its parts when split)
+    // Walks that end in the step section (where the lanes of a wave take their steps together)
+    // defer their statistics: the lanes that ended a walk of the same point combine them with a
+    // wave reduction and one lane adds them to the point's sums.  A device-scope atomic costs
+    // far more memory traffic than its 8 bytes, and the lanes of a wave mostly walk one point
+    // (claims are consecutive walks).  Elsewhere (terminal projections, gradient walks) the
+    // atomics are issued at once.  The deferred values live only until the end of the step
+    // section, so they add no register pressure to the traversal.
+    bool deferStats = false;
+    bool dHas = false;
+    float dVal = 0.0f;
+    unsigned long long dLen = 0;
+    auto flushDeferred = [&]() {   // all lanes of the wave active
+        uint64_t todo = __ballot(dHas || dLen > 0);
+        while (todo) {   // one round per distinct point among the deferring lanes
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int pl = __shfl(p, leader);
+            const bool mine = (dHas || dLen > 0) && p == pl;
+            todo &= ~__ballot(mine);
+            double sm = mine && dHas ? (double)dVal : 0.0;
+            double sq = sm * sm;
+            unsigned long long nn = mine && dHas ? 1ull : 0ull;
+            unsigned long long ll = mine ? dLen : 0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                sm += __shfl_xor(sm, off);
+                sq += __shfl_xor(sq, off);
+                nn += __shfl_xor(nn, off);
+                ll += __shfl_xor(ll, off);
+            }
+            if (lane == leader) {
+                if (nn) {
+                    atomicAdd(&P.accSum[pl], sm);
+                    atomicAdd(&P.accSq[pl], sq);
+                    atomicAdd(&P.accN[pl], nn);
+                }
+                if (ll) atomicAdd(&P.accLen[pl], ll);
+            }
+        }
+        dHas = false;
+        dLen = 0;
+    };
     auto accumulate = [&](float total) {
+        if (!GRAD && deferStats) { dHas = true; dVal = total; return; }
         double c = (double)total;
         atomicAdd(&P.accSum[p], c);
         atomicAdd(&P.accSq[p], c * c);
@@ -355,7 +397,8 @@ wosKernel(WosParams P) {
             } else {
                 accumulate(T * g + S);
             }
-            atomicAdd(&P.accLen[p], (unsigned long long)len);
+            if (!GRAD && deferStats) dLen += (unsigned long long)len;
+            else atomicAdd(&P.accLen[p], (unsigned long long)len);
             atomicAdd(&blk[4], (unsigned long long)len);
         }
         if (SPLIT) {
@@ -379,6 +422,75 @@ wosKernel(WosParams P) {
         }
         w++;
         mode = w < wEnd ? 3 : 0;   // 3: the claim's next walk starts at the top of the loop
+    };
+
+    // one walk step of a lane whose distance is known (mode 1)
+    auto stepWalk = [&]() {
+        if (dist <= eps) { finishWalk(ZG_REACHED_ABSORBING_BOUNDARY); return; }
+        float R = dist;
+        Vec<D> c = x;
+        ball.update(R);
+        if (!ws.ignore_source_contribution) {
+            float f;
+            if (pde().is_source_constant) {
+                f = evalField<D>(pde().source, x, vzero<D>(), false, sigma);
+            } else {
+                Vec<D> sdir = sampleUnitSphere<D>(rng);
+                float r = ball.sampleRadius(rng);
+                f = evalField<D>(pde().source, c + sdir * r, vzero<D>(), false, sigma);
+            }
+            S += T * ball.norm() * f;
+        }
+        Vec<D> dir = sampleUnitSphere<D>(rng);
+        x = c + dir * R;
+        if (len == 0 && P.cvGrad) {
+            // first-step control variate: E[dir] = 0, so -grad c(x0).(x1 - x0) has zero mean
+            S -= dot(vload<D>(P.cvGrad + (size_t)p * D), dir) * R;
+        } else if (P.cvGrid) {
+            // later steps: -T G(x_k).(x_{k+1} - x_k) with G a fixed field, zero mean as well
+            S -= T * dot(cvField<D>(P, c), dir) * R;
+        }
+        if (SCREENED && ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
+        if (outsideDomain<D>(BATCH ? inst->dmin : P.scene.dmin, BATCH ? inst->dmax : P.scene.dmax, x)) {
+            finishWalk(ZG_ESCAPED_DOMAIN);
+            return;
+        }
+        if (SPLIT && T > ws.splitting_threshold) {  // applyWeightWindow :198-213
+            const float thr = ws.splitting_threshold;
+            float left = T - thr;
+            int copies = 0;
+            for (float l = left; l > thr && copies <= kSplitQueue; l -= thr) copies++;
+            if (nq + copies + 1 <= kSplitQueue) {
+                T = thr;
+                float4 e = make_float4(x[0], x[1], D == 3 ? x[D - 1] : 0.0f, thr);
+                while (left > thr) { left -= thr; splitQ[nq++] = e; }
+                if (rng.nextFloat() < left / thr) splitQ[nq++] = e;
+            }
+        } else if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
+            float survival = T / ws.russian_roulette_threshold;
+            if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); return; }
+            T = ws.russian_roulette_threshold;
+        }
+        len++;
+        if (len >= ws.max_walk_length) {
+            if (ws.ignore_absorbing_boundary_contribution) finishWalk(ZG_EXCEEDED_MAX_WALK_LENGTH);
+            else beginQuery(FLT_MAX, 1.0f, ZG_EXCEEDED_MAX_WALK_LENGTH);  // terminal projection, exact
+            return;
+        }
+        if (SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
+        // next radius: the new distance is at most |x - y_prev| (y_prev lies on the boundary)
+        float bound2;
+        if (ycValid) bound2 = norm2(x - yc) * 1.0001f + 1e-30f;
+        else { float r2 = 2.0f * R; bound2 = r2 * r2 * 1.0001f + 1e-30f; }
+        beginQuery(bound2, P.relax2, -1);
+        if (ycValid && ycPrim >= 0) {
+            // the previous closest triangle is usually close to the new closest one: its exact
+            // distance seeds the bound, so the traversal from the root prunes early
+            Prim<D> pr; pr.load(view.prims, ycPrim);
+            Vec<D> qq;
+            q.seed(qq, ycPrim, pr.closest2(x, qq));
+            if (COUNT) vc.prims++;
+        }
     };
 
     bool workLeft = true;
@@ -437,71 +549,13 @@ wosKernel(WosParams P) {
 
         // -------- walk steps (walk_on_spheres.h:237-287) --------
         if (iter % P.stepInterval != 0 && __any(mode == 2)) continue;
-        if (mode != 1) continue;
-        if (dist <= eps) { finishWalk(ZG_REACHED_ABSORBING_BOUNDARY); continue; }
-        float R = dist;
-        Vec<D> c = x;
-        ball.update(R);
-        if (!ws.ignore_source_contribution) {
-            float f;
-            if (pde().is_source_constant) {
-                f = evalField<D>(pde().source, x, vzero<D>(), false, sigma);
-            } else {
-                Vec<D> sdir = sampleUnitSphere<D>(rng);
-                float r = ball.sampleRadius(rng);
-                f = evalField<D>(pde().source, c + sdir * r, vzero<D>(), false, sigma);
-            }
-            S += T * ball.norm() * f;
-        }
-        Vec<D> dir = sampleUnitSphere<D>(rng);
-        x = c + dir * R;
-        if (len == 0 && P.cvGrad) {
-            // first-step control variate: E[dir] = 0, so -grad c(x0).(x1 - x0) has zero mean
-            S -= dot(vload<D>(P.cvGrad + (size_t)p * D), dir) * R;
-        } else if (P.cvGrid) {
-            // later steps: -T G(x_k).(x_{k+1} - x_k) with G a fixed field, zero mean as well
-            S -= T * dot(cvField<D>(P, c), dir) * R;
-        }
-        if (SCREENED && ball.yukawa) T *= ball.dampening(fmaxf(1e-4f, __builtin_sqrtf(norm2(x - c))));
-        if (outsideDomain<D>(BATCH ? inst->dmin : P.scene.dmin, BATCH ? inst->dmax : P.scene.dmax, x)) {
-            finishWalk(ZG_ESCAPED_DOMAIN);
-            continue;
-        }
-        if (SPLIT && T > ws.splitting_threshold) {  // applyWeightWindow :198-213
-            const float thr = ws.splitting_threshold;
-            float left = T - thr;
-            int copies = 0;
-            for (float l = left; l > thr && copies <= kSplitQueue; l -= thr) copies++;
-            if (nq + copies + 1 <= kSplitQueue) {
-                T = thr;
-                float4 e = make_float4(x[0], x[1], D == 3 ? x[D - 1] : 0.0f, thr);
-                while (left > thr) { left -= thr; splitQ[nq++] = e; }
-                if (rng.nextFloat() < left / thr) splitQ[nq++] = e;
-            }
-        } else if (T < ws.russian_roulette_threshold) {  // applyWeightWindow :214-223
-            float survival = T / ws.russian_roulette_threshold;
-            if (survival < rng.nextFloat()) { T = 0.0f; finishWalk(ZG_TERMINATED_WITH_RUSSIAN_ROULETTE); continue; }
-            T = ws.russian_roulette_threshold;
-        }
-        len++;
-        if (len >= ws.max_walk_length) {
-            if (ws.ignore_absorbing_boundary_contribution) finishWalk(ZG_EXCEEDED_MAX_WALK_LENGTH);
-            else beginQuery(FLT_MAX, 1.0f, ZG_EXCEEDED_MAX_WALK_LENGTH);  // terminal projection, exact
-            continue;
-        }
-        if (SCREENED && sigma > 0.0f && ws.steps_before_applying_tikhonov == len) ball.yukawa = true;
-        // next radius: the new distance is at most |x - y_prev| (y_prev lies on the boundary)
-        float bound2;
-        if (ycValid) bound2 = norm2(x - yc) * 1.0001f + 1e-30f;
-        else { float r2 = 2.0f * R; bound2 = r2 * r2 * 1.0001f + 1e-30f; }
-        beginQuery(bound2, P.relax2, -1);
-        if (ycValid && ycPrim >= 0) {
-            // the previous closest triangle is usually close to the new closest one: its exact
-            // distance seeds the bound, so the traversal from the root prunes early
-            Prim<D> pr; pr.load(view.prims, ycPrim);
-            Vec<D> qq;
-            q.seed(qq, ycPrim, pr.closest2(x, qq));
-            if (COUNT) vc.prims++;
+        if (GRAD) {
+            if (mode == 1) stepWalk();
+        } else {
+            deferStats = true;
+            if (mode == 1) stepWalk();
+            deferStats = false;
+            flushDeferred();
         }
     }
 
