@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--bvh-width", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=2)
     ap.add_argument("--walks-per-lane", type=int, default=1)
-    ap.add_argument("--regroup", type=int, default=16)
+    ap.add_argument("--regroup", type=int, default=32)
     ap.add_argument("--step-interval", type=int, default=16)
     ap.add_argument("--tolerance", type=float, default=0.2, help="distance tolerance when not autotuning")
     ap.add_argument("--waves-per-cu", type=int, default=0)
