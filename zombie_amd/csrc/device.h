@@ -663,8 +663,7 @@ struct Stack {
     ZG_DEV uint16_t* keys() const {
         return reinterpret_cast<uint16_t*>(reinterpret_cast<int*>(zgLdsStack) + kLdsStack * stride);
     }
-    // d2: a squared distance or a ray parameter (>= 0), or a negated one whose sign carries a
-    // flag (rayAndLineOfSight); the LDS copy keeps the sign and rounds the magnitude down
+    // d2 >= 0 (a squared distance or a ray parameter)
     ZG_DEV void push(int node, float d2) {
         const unsigned k = __float_as_uint(d2);
         if (sp < kLdsStack) {
@@ -939,111 +938,6 @@ ZG_DEV bool intersectRay(const BoundaryView& b, const Vec<D>& o, const Vec<D>& d
             int n; float tnn;
             bool got = false;
             while (st.pop(n, tnn)) { if (tnn <= tHit) { got = true; break; } }
-            if (!got) break;
-            next = n;
-        }
-        node = next;
-    }
-    return found;
-}
-
-// The two reflecting-boundary queries of a walk-on-stars step that start at the same point o,
-// in one traversal: the first hit of the ray o + t dir (0 <= t <= tMax; the result, order and
-// tie-breaking of intersectRay) and, when `los` is set, whether the open segment o -> p2 is free
-// of primitives (hasLineOfSight).  A child box is entered when either query needs it; a box the
-// ray misses holds no box the ray hits (child boxes lie inside their parent's), so the nodes
-// entered for the visibility test alone leave the ray's state untouched, and the ray's nodes are
-// visited in intersectRay's order.  Stack keys: the ray's entry parameter (FLT_MAX when the ray
-// misses the box), negated when the visibility test needs the box; the visibility test stops
-// at its first blocking primitive.  Returns whether the ray hit; `visible` is the visibility.
-template <int D, int W>
-ZG_DEV bool rayAndLineOfSight(const BoundaryView& b, const Vec<D>& o, const Vec<D>& dir, float tMax, float& tHit,
-                              int& prim, bool los, const Vec<D>& p2, bool& visible, Stack& st) {
-    bool found = false;
-    tHit = tMax;
-    visible = true;
-    if (b.nNodes == 0) return false;
-    Vec<D> inv, ld = vzero<D>(), linv = vzero<D>();
-#pragma unroll
-    for (int a = 0; a < D; a++) inv[a] = 1.0f / dir[a];
-    float L = 0.0f;
-    if (los) {
-        Vec<D> d = p2 - o;
-        L = __builtin_sqrtf(norm2(d));
-        if (L <= 0.0f) {
-            los = false;
-        } else {
-            ld = d * (1.0f / L);
-#pragma unroll
-            for (int a = 0; a < D; a++) linv[a] = 1.0f / ld[a];
-        }
-    }
-    st.sp = 0; st.ovf = 0;
-    int node = 0;
-    for (;;) {
-        WideNode<D, W> nd;
-        nd.load(b.nodes, node);
-        float tn[W];
-        bool hitk[W], losk[W];
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            float t0 = 0.0f, t1 = tHit;
-            float u0 = 0.0f, u1 = L;
-#pragma unroll
-            for (int a = 0; a < D; a++) {
-                float ta = (nd.lo[a][k] - o[a]) * inv[a], tb = (nd.hi[a][k] - o[a]) * inv[a];
-                t0 = fmaxf(t0, fminf(ta, tb));
-                t1 = fminf(t1, fmaxf(ta, tb));
-                if (los) {
-                    float ua = (nd.lo[a][k] - o[a]) * linv[a], ub = (nd.hi[a][k] - o[a]) * linv[a];
-                    u0 = fmaxf(u0, fminf(ua, ub));
-                    u1 = fminf(u1, fmaxf(ua, ub));
-                }
-            }
-            hitk[k] = t0 <= t1;
-            tn[k] = t0;
-            losk[k] = los && u0 <= u1;
-        }
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            if (!isLeaf(nd.ch[k])) continue;
-            const int first = leafFirstD(nd.ch[k]), cnt = leafCountD(nd.ch[k]);
-            if (hitk[k] && tn[k] <= tHit) {
-                for (int i = first; i < first + cnt; i++) {
-                    Prim<D> pr; pr.load(b.prims, i);
-                    float t;
-                    if (pr.ray(o, dir, tHit, t) && (!found || t < tHit)) { tHit = t; prim = i; found = true; }
-                }
-            }
-            if (los && losk[k]) {
-                for (int i = first; i < first + cnt; i++) {
-                    Prim<D> pr; pr.load(b.prims, i);
-                    float t;
-                    if (pr.ray(o, ld, L, t) && t > 0.0f && t < L) { visible = false; los = false; break; }
-                }
-            }
-        }
-        int next = -1; float nextT = 0.0f; int nk = -1;
-#pragma unroll
-        for (int k = 0; k < W; k++)
-            if (hitk[k] && !isLeaf(nd.ch[k]) && tn[k] <= tHit && (next < 0 || tn[k] < nextT)) { next = nd.ch[k]; nextT = tn[k]; nk = k; }
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            if (isLeaf(nd.ch[k]) || k == nk) continue;
-            const bool r = hitk[k] && tn[k] <= tHit;
-            const bool l = los && losk[k];
-            if (r || l) {
-                const float key = r ? tn[k] : FLT_MAX;
-                st.push(nd.ch[k], l ? -key : key);
-            }
-        }
-        if (next < 0) {
-            int n; float key;
-            bool got = false;
-            while (st.pop(n, key)) {
-                const bool lflag = __float_as_uint(key) >> 31;
-                if (fabsf(key) <= tHit || (los && lflag)) { got = true; break; }
-            }
             if (!got) break;
             next = n;
         }

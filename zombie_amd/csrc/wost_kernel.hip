@@ -8,9 +8,8 @@
 // statistics.  One iteration advances every active lane by one full walk-on-stars step; the
 // queries of a step run to completion inside it:
 //   star radius     closest silhouette vertex (2D) / edge (3D) of the reflecting boundary
-//   boundary sample stochastic descent of the reflecting BVH
-//   ray + sight     first reflecting-boundary hit within the star radius and the boundary
-//                   sample's line of sight, one traversal (both start at the walk's point)
+//   ray             first reflecting-boundary hit within the star radius
+//   boundary sample stochastic descent of the reflecting BVH + line-of-sight test
 //   distance        closest point of the absorbing boundary for the next step
 // Per walk the lane restates WalkOnStars::walk (point_estimation/walk_on_stars.h:345-490)
 // with estimateSolution's per-point rules (:518-662), computeReflectingBoundaryContribution
@@ -428,13 +427,24 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
         Vec<D> dir = sampleUnitSphere<D>(rng);
         if (onRefl && dot(nrm, dir) > 0.0f) dir = dir * -1.0f;
 
-        // reflecting boundary contribution (computeReflectingBoundaryContribution :189-250): the
-        // boundary sample and its weight first (its random numbers follow the direction's, as in
-        // the reference), its visibility from the traversal below
+        // first reflecting-boundary hit within the star (intersectReflectingBoundary)
         Vec<D> o = onRefl ? offsetPoint<D>(x, nrm * -1.0f) : x;
-        bool needLos = false;
-        float pendS = 0.0f;
-        Vec<D> p2 = o;
+        float tHit = R;
+        int hitPrim = -1;
+        bool hit = hasRefl && intersectRay<D, W>(sc.refl, o, dir, R, tHit, hitPrim, st);
+        Vec<D> ipt, inrm = vzero<D>();
+        float idist;
+        if (hit) {
+            ipt = hitPointD<D>(sc.refl.prims, hitPrim, o, dir, tHit);
+            Prim<D> pr; pr.load(sc.refl.prims, hitPrim);
+            inrm = pr.normal();
+            idist = tHit;
+        } else {
+            ipt = o + dir * R;
+            idist = R;
+        }
+
+        // reflecting boundary contribution (computeReflectingBoundaryContribution :189-250)
         if (hasRefl && !ws.ignore_reflecting_boundary_contribution) {
             Vec<D> bpt, bn;
             float bpdf = 0.0f;
@@ -454,32 +464,15 @@ __global__ void __launch_bounds__(256) wostKernel(WosParams P) {
                 }
                 Vec<D> bnS = bn * (al ? -1.0f : 1.0f);
                 if (bpdf > 0.0f && distS < R) {
-                    p2 = offsetPoint<D>(bpt, bnS * -1.0f);
-                    float G = ball.evaluate(fmaxf(1e-4f, distS));
-                    float h = evalField<D>(pde.robin, bpt, bn, ws.solve_double_sided && al, sigma);
-                    pendS = T * alpha * G * h / bpdf;
-                    needLos = true;
+                    Vec<D> p1 = onRefl ? offsetPoint<D>(x, nrm * -1.0f) : x;
+                    Vec<D> p2 = offsetPoint<D>(bpt, bnS * -1.0f);
+                    if (lineOfSightD<D, W>(sc.refl, p1, p2, st)) {
+                        float G = ball.evaluate(fmaxf(1e-4f, distS));
+                        float h = evalField<D>(pde.robin, bpt, bn, ws.solve_double_sided && al, sigma);
+                        Sref += T * alpha * G * h / bpdf;
+                    }
                 }
             }
-        }
-
-        // first reflecting-boundary hit within the star (intersectReflectingBoundary) and the
-        // sample's line of sight from the same origin, in one traversal of the reflecting BVH
-        float tHit = R;
-        int hitPrim = -1;
-        bool visible = true;
-        bool hit = hasRefl && rayAndLineOfSight<D, W>(sc.refl, o, dir, R, tHit, hitPrim, needLos, p2, visible, st);
-        if (needLos && visible) Sref += pendS;
-        Vec<D> ipt, inrm = vzero<D>();
-        float idist;
-        if (hit) {
-            ipt = hitPointD<D>(sc.refl.prims, hitPrim, o, dir, tHit);
-            Prim<D> pr; pr.load(sc.refl.prims, hitPrim);
-            inrm = pr.normal();
-            idist = tHit;
-        } else {
-            ipt = o + dir * R;
-            idist = R;
         }
 
         // source contribution along the walk direction (computeSourceContribution :252-275)
