@@ -72,7 +72,7 @@ def parse():
                     help="after the timed passes, run learned-CV passes until the true RMSE vs the analytic "
                          "solution reaches this value, and report the measured time (0: skip)")
     ap.add_argument("--to-rmse-max-passes", type=int, default=400)
-    ap.add_argument("--budget-s", type=float, default=450.0,
+    ap.add_argument("--budget-s", type=float, default=420.0,
                     help="soft wall-time budget of the whole run: the plain run of equal wall time stops "
                          "early (and says so) when it would exceed it")
     ap.add_argument("--calibrate-copy", action="store_true",
