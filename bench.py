@@ -81,8 +81,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0x5EED)
     ap.add_argument("--point-order", default="kd", choices=["kd", "random"],
-                    help="sphere3d, mixed2d: solve the points in kd-tree order (getSpatiallySortedPointIndices) "
-                         "or as drawn")
+                    help="solve the points in kd-tree order (getSpatiallySortedPointIndices; batch: within "
+                         "each instance) or as drawn; star2d solves its grid points in grid order")
     return ap.parse_args()
 
 
@@ -494,6 +494,27 @@ def _roofline(rep_count, node_bytes, prim_bytes, kernel_ms_per_pass, steps_per_p
             "kernel_ms": kernel_ms_per_pass, "hbm_peak": HBM_PEAK_GBS, "note": note}
 
 
+def _roofline_batch(rep_exact, rep_own, info, kernel_ms_per_pass, exact_steps_per_pass, l2_gbs):
+    """Batched workload: the algorithmic bytes per exact step (an exact nearest-first traversal
+    from the root of the instance's BVH at every query point of exact walks, count_visits 2) x
+    the exact-step equivalent of the timed replays (completed walks x exact walk length) / their
+    graph time.  The instances' points are solved instance by instance, so the working set at a
+    time is a few instances' BVHs (0.2-17 MB each, by triangle count): the bytes come from L2 and the MALL,
+    and peak is the measured L2 read ceiling, as for the headline workload."""
+    nodes = rep_exact.node_visits / max(rep_exact.total_steps, 1)
+    prims = rep_exact.prim_visits / max(rep_exact.total_steps, 1)
+    bps = nodes * info.node_bytes + prims * info.prim_bytes + 48.0
+    ach = bps * exact_steps_per_pass / (kernel_ms_per_pass * 1e-3) / 1e9
+    s = max(rep_own.total_steps, 1)
+    return {"bound": "l2", "achieved": ach, "peak": l2_gbs, "unit": "GB/s", "frac": ach / l2_gbs,
+            "traffic": None, "bytes_per_step": bps, "nodes_per_step": nodes, "prims_per_step": prims,
+            "own_nodes_per_raw_step": rep_own.node_visits / s, "own_prims_per_raw_step": rep_own.prim_visits / s,
+            "kernel_ms": kernel_ms_per_pass, "hbm_peak": HBM_PEAK_GBS,
+            "note": "exact nearest-first traversal reads per exact step (count_visits 2 on exact walks over a "
+                    "strided sample) x exact-step equivalent of the timed replays / their graph time (HIP events "
+                    "on the launch stream); the 256 scenes (0.5 GB) are in HBM, solved instance by instance"}
+
+
 def _roofline_wost(rep_count, info, kernel_ms_per_pass, steps_per_pass, l2_gbs):
     """Walk-on-stars bytes per step: every BVH node and primitive the step's queries read (the
     kernel's own visits, counted by the visit-counting variant of the interleaved kernel on a
@@ -711,6 +732,9 @@ def bench_batch(args, torch, dev):
             seed += 1
             got.append(c[sc.inside_domain(c).bool()])
         p = torch.cat(got)[:need].contiguous()
+        if args.point_order == "kd":
+            # each instance's points in kd-tree order: neighbouring lanes walk from nearby points
+            p = p[torch.from_numpy(E.spatially_sorted_point_indices(p.cpu().numpy())).to(dev).long()].contiguous()
         d, _, _, _ = sc.closest_point(p, want_point=False)
         pts.append(p); dists.append(d); inst.append(torch.full((need,), i, dtype=torch.int32, device=dev))
         exact.append(PB.exp_linear(p.cpu().numpy(), pr.meta["a"]))
@@ -752,24 +776,35 @@ def bench_batch(args, torch, dev):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     steps = walks_done = 0
+    replay_ms = 0.0
     for s in range(args.steps):
+        # events on the stream the graph is launched on (torch's current stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         graph.launch()
+        e1.record()
         r = graph.report()          # synchronises: counters of this replay
         steps += r.total_steps; walks_done += r.total_walks
+        e1.synchronize()
+        replay_ms += e0.elapsed_time(e1)
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
     graph.close()
-    rc = E.batch_wos_solve(batch, probs[0].settings, dp, IOP, E.DeviceStats.zeros(n), E.SolveOptions(
-        seed=args.seed + 3, walks_per_lane=wpl, regroup_interval=rg, step_interval=args.step_interval,
-        distance_tolerance=args.tolerance, count_visits=True))
-    # walk length with exact radii on a strided sample: the value is reported in the reference's
-    # unit (completed walks/s x exact-radius mean walk length), as for the headline workload
+    # on a strided sample: exact walks (tolerance 0, the reference's walk) whose query points
+    # each also run a counted exact nearest-first traversal from the root of their instance's BVH
+    # (count_visits 2: the algorithmic reads per step, and the exact walk length: the value is
+    # reported in the reference's unit, completed walks/s x exact-radius mean walk length); and
+    # the kernel's own visits at the timed tolerance (count_visits 1), reported beside them
     idx = torch.arange(0, n, max(1, n // 262144), device=dev)[:262144]
     sx = E.DevicePoints(pt=P[idx].contiguous(), dist_to_absorbing=D[idx].contiguous(),
                         n_walks=torch.full((idx.numel(),), 32, dtype=torch.int32, device=dev), n_walks_max=32)
+    sop = dict(walks_per_lane=wpl, regroup_interval=rg, step_interval=args.step_interval)
     rx = E.batch_wos_solve(batch, probs[0].settings, sx, IOP[idx].contiguous(), E.DeviceStats.zeros(idx.numel()),
-                           E.SolveOptions(seed=args.seed + 4, walks_per_lane=wpl, regroup_interval=rg,
-                                          step_interval=args.step_interval, distance_tolerance=0.0))
+                           E.SolveOptions(seed=args.seed + 4, distance_tolerance=0.0, count_visits=2, **sop))
+    rc = E.batch_wos_solve(batch, probs[0].settings, sx, IOP[idx].contiguous(), E.DeviceStats.zeros(idx.numel()),
+                           E.SolveOptions(seed=args.seed + 3, distance_tolerance=args.tolerance, count_visits=1, **sop))
+    E.probe_read_bandwidth(16 << 20, 20)
+    l2_gbs = E.probe_read_bandwidth(16 << 20, 400)
     len_exact = rx.total_steps / max(rx.total_walks, 1)
     mean = stats.mean.cpu().numpy()
     var = stats.variance().cpu().numpy()
@@ -780,15 +815,14 @@ def bench_batch(args, torch, dev):
         "workload": f"{len(probs)} screened-Poisson instances, {n_tris} triangles, {n} points, {walks} walks/pt/pass, "
                     "eps 1e-4", "bvh_width": width, "leaf_size": args.leaf_size, "walks_per_lane": wpl,
         "regroup_interval": rg, "step_interval": args.step_interval, "distance_tolerance": args.tolerance,
-        "points": n, "instances": len(probs), "autotuned": tune is not None},
+        "points": n, "instances": len(probs), "autotuned": tune is not None, "point_order": args.point_order},
         walks_done / el * len_exact if args.tolerance > 0 else steps / el, el, args,
         {"value_definition": "completed walks/s x mean walk length with exact radii (reference walk-steps/s)",
          "walks_per_s": walks_done / el, "raw_walk_steps_per_s": steps / el,
          "mean_walk_length": steps / max(walks_done, 1), "mean_walk_length_exact": len_exact,
          "rmse_vs_analytic": float(np.sqrt(np.mean(err ** 2))), "pooled_z_vs_analytic": pooled,
-         "roofline": _roofline(rc, scenes[0].info.node_bytes, scenes[0].info.prim_bytes, rc.kernel_ms,
-                               rc.total_steps, "closest-point bytes the kernel's own queries read per step "
-                               "(instrumented pass); the 256 scenes (11.2M triangles) are HBM-resident")})
+         "roofline": _roofline_batch(rx, rc, batches[width][0][0].info, replay_ms / args.steps,
+                                     walks_done / args.steps * len_exact, l2_gbs)})
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         i0 = int(np.argmin([abs(sp["n_tris"] - 43000) for sp in spec]))   # an instance of typical size
