@@ -775,9 +775,7 @@ __global__ void __launch_bounds__(256) wostInterleavedKernel(WosParams P) {
             ClosestQuery<D>::template sortAscending<W>(key, ch);
             qHas = key[0] < FLT_MAX;
             qItem = ch[0]; qKey = key[0];
-#pragma unroll
-            for (int k = W - 1; k >= 1; k--)
-                if (key[k] < FLT_MAX) st.push(ch[k], key[k]);
+            st.pushChildren<W>(key, ch);
             if (!qHas) return false;
         }
         if (isLeaf(qItem)) {
