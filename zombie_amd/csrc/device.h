@@ -678,9 +678,10 @@ struct Stack {
         }
     }
     // pushes the children k = W-1 .. 1 of a sorted child list whose keys are < FLT_MAX (a
-    // prefix: invalid children sort last), the nearest on top.  With room for W-1 entries in
-    // LDS every slot is written without a branch: the valid children fill sp .. sp+nv-1 and the
-    // invalid ones land above the new top, where they are never read.
+    // prefix: invalid children sort last), the nearest on top, as that many push() calls would.
+    // With room for W-1 entries in LDS every slot is written without a branch: the valid
+    // children fill sp .. sp+nv-1 and the invalid ones land above the new top, where they are
+    // never read.
     template <int W>
     ZG_DEV void pushChildren(const float (&d2)[W], const int (&ch)[W]) {
         int nv = 0;
@@ -698,31 +699,6 @@ struct Stack {
 #pragma unroll
             for (int k = W - 1; k >= 1; k--)
                 if (d2[k] < FLT_MAX) push(ch[k], d2[k]);
-        }
-    }
-    // pushes node[k] for every k with take[k], in k order (the last on top), as that many push()
-    // calls would; with room for W entries in LDS every slot is written without a branch (the
-    // skipped entries land above the new top, where they are never read)
-    template <int W>
-    ZG_DEV void pushMasked(const bool (&take)[W], const int (&node)[W], const float (&key)[W]) {
-        if (ovf == 0 && sp + W <= kLdsStack) {
-            int nv = 0;
-#pragma unroll
-            for (int k = 0; k < W; k++) nv += take[k] ? 1 : 0;
-            int below = 0, above = nv;
-#pragma unroll
-            for (int k = 0; k < W; k++) {
-                const int slot = sp + (take[k] ? below : above);
-                below += take[k] ? 1 : 0;
-                above += take[k] ? 0 : 1;
-                codes()[slot * stride + col] = node[k];
-                keys()[slot * stride + col] = (uint16_t)(__float_as_uint(key[k]) >> 16);
-            }
-            sp += nv;
-        } else {
-#pragma unroll
-            for (int k = 0; k < W; k++)
-                if (take[k]) push(node[k], key[k]);
         }
     }
     ZG_DEV bool pop(int& node, float& d2) {
@@ -977,10 +953,9 @@ ZG_DEV bool intersectRay(const BoundaryView& b, const Vec<D>& o, const Vec<D>& d
 #pragma unroll
         for (int k = 0; k < W; k++)
             if (hitk[k] && !isLeaf(nd.ch[k]) && tn[k] <= tHit && (next < 0 || tn[k] < nextT)) { next = nd.ch[k]; nextT = tn[k]; nk = k; }
-        bool take[W];
 #pragma unroll
-        for (int k = 0; k < W; k++) take[k] = hitk[k] && !isLeaf(nd.ch[k]) && tn[k] <= tHit && k != nk;
-        st.pushMasked<W>(take, nd.ch, tn);
+        for (int k = 0; k < W; k++)
+            if (hitk[k] && !isLeaf(nd.ch[k]) && tn[k] <= tHit && k != nk) st.push(nd.ch[k], tn[k]);
         if (next < 0) {
             int n; float tnn;
             bool got = false;
@@ -1160,16 +1135,16 @@ ZG_DEV float starRadius2(const BoundaryView& b, const Vec<2>& x, float minRadius
         }
         if (found && best2 <= min2) break;
         int next = -1; float ndist = 0.0f; int nk = -1;
-        float dk[W];
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            dk[k] = nd.dist2(k, x);
-            if (keep[k] && !isLeaf(nd.ch[k]) && dk[k] <= best2 && (next < 0 || dk[k] < ndist)) { next = nd.ch[k]; ndist = dk[k]; nk = k; }
+            float dk = nd.dist2(k, x);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && (next < 0 || dk < ndist)) { next = nd.ch[k]; ndist = dk; nk = k; }
         }
-        bool take[W];
 #pragma unroll
-        for (int k = 0; k < W; k++) take[k] = keep[k] && !isLeaf(nd.ch[k]) && dk[k] <= best2 && k != nk;
-        st.pushMasked<W>(take, nd.ch, dk);
+        for (int k = 0; k < W; k++) {
+            float dk = nd.dist2(k, x);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && k != nk) st.push(nd.ch[k], dk);
+        }
         if (next < 0) {
             int n; float dn;
             bool got = false;
@@ -1513,16 +1488,16 @@ ZG_DEV float starRadius3(const BoundaryView& b, const Vec<3>& x, float minRadius
         }
         if (found && best2 <= min2) break;
         int next = -1; float ndist = 0.0f; int nk = -1;
-        float dk[W];
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            dk[k] = nd.dist2(k, x);
-            if (keep[k] && !isLeaf(nd.ch[k]) && dk[k] <= best2 && (next < 0 || dk[k] < ndist)) { next = nd.ch[k]; ndist = dk[k]; nk = k; }
+            float dk = nd.dist2(k, x);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && (next < 0 || dk < ndist)) { next = nd.ch[k]; ndist = dk; nk = k; }
         }
-        bool take[W];
 #pragma unroll
-        for (int k = 0; k < W; k++) take[k] = keep[k] && !isLeaf(nd.ch[k]) && dk[k] <= best2 && k != nk;
-        st.pushMasked<W>(take, nd.ch, dk);
+        for (int k = 0; k < W; k++) {
+            float dk = nd.dist2(k, x);
+            if (keep[k] && !isLeaf(nd.ch[k]) && dk <= best2 && k != nk) st.push(nd.ch[k], dk);
+        }
         if (next < 0) {
             int n; float dn;
             bool got = false;
