@@ -857,10 +857,14 @@ struct ClosestQuery {
             int ch[W];
             childDistances<D, W>(b.nodes, item, x, d2, ch);
             if (COUNT) vc.nodes++;
+            float low = lower2;
 #pragma unroll
             for (int k = 0; k < W; k++) {
-                if (d2[k] > prune2) { prunedBox(d2[k]); d2[k] = FLT_MAX; }
+                const bool pr = d2[k] > prune2;
+                low = fminf(low, pr && d2[k] <= best2 ? d2[k] : FLT_MAX);
+                d2[k] = pr ? FLT_MAX : d2[k];
             }
+            lower2 = low;
             sortAscending<W>(d2, ch);
             has = d2[0] < FLT_MAX;
             item = ch[0]; itemD2 = d2[0];
@@ -875,10 +879,13 @@ struct ClosestQuery {
             Vec<D> qq;
             float dd = pr.closest2(x, qq);
             if (COUNT) vc.prims++;
-            if (dd < best2) {
-                best2 = dd; cp = qq; prim = first; found = true;
-                prune2 = best2 > guard2 ? best2 * relax2 : best2;
-            }
+            const bool better = dd < best2;
+#pragma unroll
+            for (int a = 0; a < D; a++) cp[a] = better ? qq[a] : cp[a];
+            prim = better ? first : prim;
+            found = found || better;
+            best2 = better ? dd : best2;
+            prune2 = better ? (best2 > guard2 ? best2 * relax2 : best2) : prune2;
             has = cnt > 1 && itemD2 <= prune2;
             if (cnt > 1 && !has) prunedBox(itemD2);
             item = ~(((first + 1) << 4) | (cnt - 2));
